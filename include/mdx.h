@@ -1,0 +1,233 @@
+/*
+ * mdx.h -- C ABI of the MI355X force engine for MDynaMix.
+ *
+ * The engine replaces the force path of the reference program:
+ *   CHCNB(LRDF)     neighbour lists                     forces.f:1813-2205
+ *   ALLFORCE        all forces (SHAKE / leap-frog step)  forces.f:77-100
+ *   FFORCES(LAVER)  fast forces (r-RESPA inner step)     forces.f:36-51
+ *   SLFORCES        slow forces (r-RESPA outer step)     forces.f:55-73
+ * together with what they call: LOCALF (813), FORCES (1014), FURIR (1217),
+ * INIFUR (1356), ETERMS (1482), GETBND (113), GETANG (216), GETTRS (323)
+ * and the reductions of CM_ADDLA/CM_ADDLF/CM_ADDF (scalar.f:28-116).
+ *
+ * The reference routines take no arguments and share state through COMMON
+ * blocks (common_inc.f:133-249).  Here the same state crosses the boundary
+ * as plain arrays: the topology and parameters once (mdx_create), the
+ * coordinates every step (mdx_set_positions), and forces plus every
+ * energy/virial scalar the integrators and averages read come back from
+ * mdx_compute.  All calls return an mdx_status; nothing aborts the process
+ * (the reference calls FINAL/stop instead, forces.f:1414-1419,2188-2194).
+ *
+ * Units are the caller's: coulomb_factor and the Lennard-Jones tables fix
+ * the energy unit, coordinates fix the length unit, and forces come back in
+ * energy/length.  The Fortran shim passes the reference's internal units
+ * (COULF, A6LJ...), the Python host side passes kJ/mol, Angstrom, e.
+ *
+ * Indices are 0-based except in the exported reference-layout lists, which
+ * keep the reference's 1-based signed form (negative first index marks a
+ * 1-4 pair, forces.f:1956).
+ *
+ * Threading: one host thread per handle; a handle owns one HIP stream.
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mdx_status {
+    MDX_OK = 0,
+    MDX_ERR_ARG = 1,            /* invalid argument or unsupported option      */
+    MDX_ERR_LIST_OVERFLOW = 2,  /* pair-list capacity exceeded (forces.f:2188)  */
+    MDX_ERR_NKV = 3,            /* too many k-vectors (forces.f:1414)           */
+    MDX_ERR_HIP = 4,            /* HIP runtime error                            */
+    MDX_ERR_NOMEM = 5,          /* device allocation failed                     */
+    MDX_ERR_STATE = 6           /* call order (e.g. compute before a list)      */
+} mdx_status;
+
+/* Neighbour-list semantics. */
+enum {
+    /* CHCNB semantics: lists at R_cut with no skin; every listed pair is
+     * evaluated until the next rebuild with no cutoff test (forces.f:820,
+     * 1024); pairs keep the short/long class they had when listed. */
+    MDX_LIST_REFERENCE = 0,
+    /* Verlet list at R_cut + skin; pairs beyond R_cut at evaluation time
+     * are skipped.  The short/long class is still fixed at build time. */
+    MDX_LIST_VERLET = 1
+};
+
+/* Force groups of mdx_compute. */
+enum {
+    MDX_FORCES_ALL = 0,   /* ALLFORCE: fast + slow, F = G + H                    */
+    MDX_FORCES_FAST = 1,  /* FFORCES: short-list pairs + bonded terms -> H       */
+    MDX_FORCES_SLOW = 2   /* SLFORCES: reciprocal + exclusions + long pairs -> G */
+};
+
+/* Optional work in mdx_compute. */
+enum {
+    MDX_WANT_DETAIL = 1   /* per type-pair / per type energy breakdown
+                             (POTE1, POTL1, PES141, PSR141, SPEE)             */
+};
+
+#define MDX_MAX_MOLTYPES 16
+#define MDX_MAX_TYPEPAIRS (MDX_MAX_MOLTYPES * (MDX_MAX_MOLTYPES + 1) / 2 + 1)
+
+/*
+ * Topology and force-field tables (COMMON /ADDREFS/, /SITTES/, /SITSITM/,
+ * /LISTNB/ INBB, /BONBON/, /ANGANG/, /TORTOR/, /SOMMOL/ C14*).
+ * Atoms are ordered molecule type by molecule type, molecule by molecule,
+ * as in the reference (setup.f:633-694).  All arrays are caller-owned and
+ * only read during mdx_create.
+ */
+typedef struct mdx_topology {
+    int32_t natoms;
+    int32_t nmols;
+    int32_t nmoltypes;            /* NTYPES (<= MDX_MAX_MOLTYPES)            */
+    int32_t nljtypes;             /* NNBT                                    */
+    int32_t ntypepairs;           /* MOLINT                                  */
+
+    /* per atom */
+    const int32_t *atom_moltype;  /* ITYPE - 1                               */
+    const int32_t *atom_mol;      /* NNUM - 1                                */
+    const int32_t *atom_ljtype;   /* INBT(NSITE) - 1                         */
+    const double  *atom_charge;   /* Q: charge of intermolecular terms       */
+    const double  *atom_charge0;  /* CHARGE(NSITE): unscaled site charge     */
+    const double  *atom_mass;     /* MASS(NSITE), for molecular COMs         */
+
+    /* per molecule type */
+    const int32_t *type_ee;       /* IEE: 1 normal, 0 lambda-coupled type    */
+    const int32_t *type_move;     /* LMOVE                                   */
+    const double  *type_c14el;    /* C14EL                                   */
+    const double  *type_c14lj;    /* C14LJ                                   */
+    const int32_t *type_pair;     /* MDX(ITYP,JTYP) - 1, nmoltypes^2         */
+
+    /* Lennard-Jones tables, nljtypes^2 row-major: E = (A6 + B12 r^-6) r^-6 */
+    const double *a6, *b12;       /* A6LJ,  B12LJ   (intermolecular)         */
+    const double *a6_0, *b12_0;   /* A6LJ0, B12LJ0  (intramolecular)         */
+    const double *a6_14, *b12_14; /* A6_14, B12_14  (1-4 pairs)              */
+
+    /* bound-atom lists NNBB/INBB (setup.f:753-888) in CSR form:
+     * neighbours of atom i are excl_atom[excl_start[i] .. excl_start[i+1]),
+     * excl_is14 marks the 1-4 entries (negative INBB entries).            */
+    const int32_t *excl_start;
+    const int32_t *excl_atom;
+    const int8_t  *excl_is14;
+
+    /* bonds (GETBND): E = K (r-r0)^2, or Morse when morse != 0 and r-r0<=r0 */
+    int32_t nbonds;
+    const int32_t *bond_i, *bond_j, *bond_param;
+    int32_t nbondparams;
+    const int32_t *bp_morse;      /* ID == 1                                 */
+    const double  *bp_r0, *bp_k, *bp_d, *bp_rho;
+
+    /* angles (GETANG): E = K (theta-theta0)^2, theta0 in radians */
+    int32_t nangles;
+    const int32_t *angle_i, *angle_j, *angle_k, *angle_param;
+    int32_t nangleparams;
+    const double  *ap_theta0, *ap_k;
+
+    /* dihedrals (GETTRS) */
+    int32_t ntors;
+    const int32_t *tors_i, *tors_j, *tors_k, *tors_l, *tors_param;
+    int32_t ntorsparams;
+    const int32_t *tp_kind;       /* ITORS: 0 AMBER, 1 MM3, 5 cos^n, -1 improper */
+    const int32_t *tp_mult;       /* NMUL                                    */
+    const double  *tp_k;          /* FT                                      */
+    const double  *tp_delta;      /* RT, radians                             */
+    const double  *tp_k1, *tp_k2, *tp_k3, *tp_k4, *tp_k5;   /* FT1..FT5      */
+} mdx_topology;
+
+/* Run parameters (COMMON /SIZES/, /ENERGY/ COULF, /MDEE/ RAEX). */
+typedef struct mdx_params {
+    double box[3];                /* BOXL, BOYL, BOZL (rectangular cell)     */
+    double rcut;                  /* RCUTT                                   */
+    double rshort;                /* SHORT                                   */
+    double skin;                  /* Verlet skin (MDX_LIST_VERLET only)      */
+    double alpha;                 /* ALPHAD (Ewald); 0 disables Ewald        */
+    double fexp;                  /* FEXP: k-space cut |k|^2 <= 4 alpha^2 FEXP */
+    double coulomb_factor;        /* COULF                                   */
+    int32_t list_mode;            /* MDX_LIST_REFERENCE / MDX_LIST_VERLET    */
+    int32_t ee_enabled;           /* LMDEE                                   */
+    double ee_raex;               /* RAEX: soft-core shift of EE pairs       */
+    double ee_self_scale;         /* EC(ME): charge scale of EE types in the
+                                     Ewald self term (forces.f:1434-1435)   */
+} mdx_params;
+
+/* Energies and virials of one force call (COMMON /ENERGY/, /VIRRR/,
+ * /SOMMOL/ as left by ZEROFS + the force routines + CM_ADD*). */
+typedef struct mdx_energies {
+    double pe1, pe2;              /* LJ energy, long / short list            */
+    double pels1, pels2;          /* electrostatic energy (slow incl. k-space) / short */
+    double spe;                   /* Ewald self + exclusion correction       */
+    double pint;                  /* bonded energy                           */
+    double vir1, vir2, virb;      /* LJ virials (long/short), bond virial    */
+    double virx, viry, virz;      /* slow virial projections                 */
+    double virfx, virfy, virfz;   /* fast virial projections                 */
+    double wirs, wirss;           /* molecular virial corrections            */
+    int64_t npairs_short;         /* pairs evaluated in this call            */
+    int64_t npairs_long;
+    /* MDX_WANT_DETAIL only */
+    double pote1[MDX_MAX_TYPEPAIRS];
+    double potl1[MDX_MAX_TYPEPAIRS];
+    double pes141[MDX_MAX_MOLTYPES];
+    double psr141[MDX_MAX_MOLTYPES];
+    double spee[MDX_MAX_MOLTYPES];
+} mdx_energies;
+
+typedef struct mdx_engine mdx_engine;
+
+/* Library and device. */
+const char *mdx_version(void);
+mdx_status mdx_device_info(int32_t *n_cu, int32_t *clock_mhz, char *name, int32_t name_len);
+
+/* Handle life cycle.  mdx_create copies topology and parameters to the
+ * device, enumerates the k-vectors (INIFUR) and computes the self terms. */
+mdx_status mdx_create(const mdx_topology *topo, const mdx_params *params, mdx_engine **out);
+mdx_status mdx_destroy(mdx_engine *h);
+const char *mdx_last_error(const mdx_engine *h);
+
+/* Coordinates: FP64 SoA, caller-owned host arrays of natoms (SINHR/GETCOM
+ * output).  The device-pointer variant takes arrays already in HBM.       */
+mdx_status mdx_set_positions(mdx_engine *h, const double *x, const double *y, const double *z);
+mdx_status mdx_set_positions_device(mdx_engine *h, const double *dx, const double *dy, const double *dz);
+/* Box change (NPT); re-enumerates k-vectors like FURIR when an edge changed
+ * by more than 5 % since the last enumeration (forces.f:1250-1265). */
+mdx_status mdx_set_box(mdx_engine *h, const double box[3]);
+/* Molecular centres of mass used by the molecular virial (X, Y, Z of
+ * COMMON /COMCOM/).  NULL restores the default: COMs from atom masses. */
+mdx_status mdx_set_molecule_com(mdx_engine *h, const double *x, const double *y, const double *z);
+
+/* CHCNB: rebuild the neighbour lists from the current coordinates. */
+mdx_status mdx_build_list(mdx_engine *h);
+/* Pair counts of the last build (MBSH, MBLN). */
+mdx_status mdx_list_counts(mdx_engine *h, int64_t *n_short, int64_t *n_long);
+/* Export the lists in the reference layout and order (NBS1/NBS2, NBL1/NBL2:
+ * 1-based, the larger index of a normal pair first, 1-4 pairs as
+ * (-outer, inner) in the short list).  Capacities in pairs. */
+mdx_status mdx_get_list(mdx_engine *h, int32_t *nbs1, int32_t *nbs2, int64_t cap_short,
+                        int32_t *nbl1, int32_t *nbl2, int64_t cap_long);
+
+/* One force call.  group is MDX_FORCES_*.  Output arrays are caller-owned
+ * host FP64 arrays of natoms; any may be NULL.  f receives the forces of
+ * the group (ALL: G + H), g and h the slow and fast parts when non-NULL. */
+mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want,
+                       double *fx, double *fy, double *fz,
+                       double *gx, double *gy, double *gz,
+                       double *hx, double *hy, double *hz,
+                       mdx_energies *e);
+/* Same call with device output arrays of natoms (forces stay in HBM). */
+mdx_status mdx_compute_device(mdx_engine *h, int32_t group, int32_t want,
+                              double *dfx, double *dfy, double *dfz, mdx_energies *e);
+
+/* Timing of the last mdx_compute / mdx_build_list per stage, in ms
+ * (HIP events on the handle's stream): list, pairs, recip_sk, recip_force,
+ * bonded+exclusions, reduce. */
+mdx_status mdx_stage_times(mdx_engine *h, double *ms, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDX_H */

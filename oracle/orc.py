@@ -1,0 +1,116 @@
+"""ctypes wrapper of the FP64 oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, never by the engine.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Dict, Tuple
+
+import numpy as np
+
+from mdynamix_amd.abi import MdxEnergies, MDX_FORCES_ALL, as_ptr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+SRC = os.path.join(HERE, "mdx_oracle.c")
+
+
+def build() -> str:
+    """Compile the oracle: plain IEEE double, no FMA contraction."""
+    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", SRC, "-o", LIB, "-lm"]
+    subprocess.run(cmd, check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+            build()
+        L = C.CDLL(LIB)
+        pd, pi = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        pi64 = C.POINTER(C.c_int64)
+        L.orc_chcnb.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, pi, pi, C.c_int64, pi64, pi, pi, C.c_int64, pi64]
+        L.orc_chcnb.restype = C.c_int
+        L.orc_inifur.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, C.c_int64, pd, pd]
+        L.orc_inifur.restype = C.c_int64
+        L.orc_compute.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, pd, pd, pd, pi, pi, C.c_int64, pi, pi,
+                                  C.c_int64, pd, pd, pd, C.c_int64, pd, C.c_double, C.c_int32,
+                                  pd, pd, pd, pd, pd, pd, pd, pd, pd, C.c_void_p]
+        L.orc_compute.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _xyz(pos):
+    pos = np.asarray(pos, dtype=np.float64)
+    return [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+
+
+def chcnb(topo_struct, params_struct, pos, cap_short=None, cap_long=None):
+    """Reference-order lists (1-based signed) as four int32 arrays."""
+    n = pos.shape[0]
+    cap_s = cap_short or max(1024, 64 * n)
+    cap_l = cap_long or max(4096, 1200 * n)
+    x, y, z = _xyz(pos)
+    s1 = np.zeros(cap_s, np.int32); s2 = np.zeros(cap_s, np.int32)
+    l1 = np.zeros(cap_l, np.int32); l2 = np.zeros(cap_l, np.int32)
+    ms, ml = C.c_int64(0), C.c_int64(0)
+    st = lib().orc_chcnb(C.byref(topo_struct), C.byref(params_struct), as_ptr(x), as_ptr(y), as_ptr(z),
+                         as_ptr(s1, C.c_int32), as_ptr(s2, C.c_int32), cap_s, C.byref(ms),
+                         as_ptr(l1, C.c_int32), as_ptr(l2, C.c_int32), cap_l, C.byref(ml))
+    if st != 0:
+        raise RuntimeError("orc_chcnb overflow: %d %d" % (ms.value, ml.value))
+    return s1[:ms.value].copy(), s2[:ms.value].copy(), l1[:ml.value].copy(), l2[:ml.value].copy()
+
+
+def inifur(topo_struct, params_struct, nmoltypes, cap=200000):
+    rk = [np.zeros(cap) for _ in range(3)]
+    spee = np.zeros(max(1, nmoltypes))
+    spei = C.c_double(0)
+    nkv = lib().orc_inifur(C.byref(topo_struct), C.byref(params_struct), as_ptr(rk[0]), as_ptr(rk[1]),
+                           as_ptr(rk[2]), cap, as_ptr(spee), C.byref(spei))
+    if nkv < 0:
+        raise RuntimeError("too many k-vectors")
+    return rk[0][:nkv].copy(), rk[1][:nkv].copy(), rk[2][:nkv].copy(), spee, spei.value
+
+
+def molecule_com(topo, pos):
+    """Mass-weighted molecular centres (GETCOM, service.f:463-500)."""
+    m = topo.atom_mass
+    nm = topo.nmols
+    w = np.zeros(nm)
+    np.add.at(w, topo.atom_mol, m)
+    com = np.zeros((nm, 3))
+    for k in range(3):
+        np.add.at(com[:, k], topo.atom_mol, m * pos[:, k])
+    return com / w[:, None]
+
+
+def compute(topo, topo_struct, params_struct, pos, lists, group=MDX_FORCES_ALL, com=None, kvec=None):
+    """One force call on given lists; returns (f, g, h, energies dict)."""
+    n = pos.shape[0]
+    x, y, z = _xyz(pos)
+    if com is None:
+        com = molecule_com(topo, pos)
+    cx, cy, cz = _xyz(com)
+    s1, s2, l1, l2 = [np.ascontiguousarray(a, dtype=np.int32) for a in lists]
+    if kvec is None:
+        kvec = inifur(topo_struct, params_struct, topo.nmoltypes)
+    rkx, rky, rkz, spee, spei = kvec
+    out = [np.zeros(n) for _ in range(9)]
+    e = MdxEnergies()
+    pi = C.c_int32
+    lib().orc_compute(C.byref(topo_struct), C.byref(params_struct), as_ptr(x), as_ptr(y), as_ptr(z),
+                      as_ptr(cx), as_ptr(cy), as_ptr(cz), as_ptr(s1, pi), as_ptr(s2, pi), len(s1),
+                      as_ptr(l1, pi), as_ptr(l2, pi), len(l1), as_ptr(rkx), as_ptr(rky), as_ptr(rkz),
+                      len(rkx), as_ptr(spee), spei, group, *[as_ptr(a) for a in out], C.byref(e))
+    f = np.stack(out[0:3], 1); g = np.stack(out[3:6], 1); h = np.stack(out[6:9], 1)
+    return f, g, h, e.as_dict(topo.nmoltypes, topo.ntypepairs)

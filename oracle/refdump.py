@@ -1,0 +1,52 @@
+"""Reader for the binary dump written by the reference harness
+(oracle/ref/mdx_dump.f) and driver that runs the reference program on a
+generated system.  Test infrastructure: used only by tests/ and by the
+fixture generator under tests/golden/.
+
+Record layout: 8-byte name, int32 kind (1 = int32, 2 = float64),
+int32 count, count values.  Names that occur twice (per-call arrays of the
+second force evaluation) are returned as lists.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+from typing import Dict, List
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_EXE = os.path.join(HERE, "_ref", "mdx_ref")
+
+
+def read_dump(path: str) -> Dict[str, List[np.ndarray]]:
+    raw = open(path, "rb").read()
+    out: Dict[str, List[np.ndarray]] = {}
+    off = 0
+    while off < len(raw):
+        name = raw[off:off + 8].decode().strip()
+        kind, n = np.frombuffer(raw, dtype="<i4", count=2, offset=off + 8)
+        off += 16
+        dt = np.dtype("<i4") if kind == 1 else np.dtype("<f8")
+        arr = np.frombuffer(raw, dtype=dt, count=int(n), offset=off).copy()
+        off += int(n) * dt.itemsize
+        out.setdefault(name, []).append(arr)
+    return out
+
+
+def run_reference(system, workdir: str | None = None, timeout: float = 600.0) -> Dict[str, List[np.ndarray]]:
+    """Write the system's input files, run the reference harness in a
+    scratch directory and return its dump."""
+    if not os.path.exists(REF_EXE):
+        raise FileNotFoundError("reference harness not built: run make -C oracle/ref")
+    own = workdir is None
+    d = tempfile.mkdtemp(prefix="mdxref_") if own else workdir
+    system.write_reference_inputs(d)
+    res = subprocess.run([REF_EXE, "md.in"], cwd=d, capture_output=True, text=True, timeout=timeout)
+    dump = os.path.join(d, "mdx_dump.bin")
+    if not os.path.exists(dump):
+        raise RuntimeError("reference run failed:\n" + res.stdout[-4000:] + res.stderr[-2000:])
+    out = read_dump(dump)
+    out["_stdout"] = [np.array(res.stdout)]
+    return out
