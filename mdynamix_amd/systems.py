@@ -210,7 +210,7 @@ def spce_water() -> MolType:
     sigma_OO=3.166 A, eps_OO=0.650 kJ/mol.  The H-H distance is a third
     harmonic bond so SHAKE keeps the triangle rigid."""
     g = _water_geometry(1.0, 109.47)
-    hh = math.dist(g[1], g[2])
+    hh = round(math.dist(g[1], g[2]), 8)     # as written to the .mmol file
     sites = [Site("OW", g[0], 15.9994, -0.8476, 3.166, 0.650),
              Site("HW1", g[1], 1.008, 0.4238, 0.0, 0.0),
              Site("HW2", g[2], 1.008, 0.4238, 0.0, 0.0)]
@@ -273,7 +273,7 @@ def ua_chain(n_sites: int = 46, seed: int = 7, n_charged: int = 4) -> MolType:
     sites = [Site("C%d" % (i + 1), tuple(geo[i]), 14.027, float(q[i]), 3.9, 0.5) for i in range(n_sites)]
     bonds = [(0, i + 1, i + 2, 1.53, 2500.0, 0.0, 0.0) for i in range(n_sites - 1)]
     angles = [(i + 1, i + 2, i + 3, 112.0, 500.0) for i in range(n_sites - 2)]
-    coef = rng.uniform(-5.0, 5.0, size=(3,))
+    coef = np.round(rng.uniform(-5.0, 5.0, size=(3,)), 6)   # as written to the .mmol file
     tors1 = [(i + 1, i + 2, i + 3, i + 4, float(coef[0]), float(coef[1]), float(coef[2]))
              for i in range(n_sites - 3)]
     return MolType("UACH", sites, bonds, angles, [], tors1, c14el=0.5, c14lj=0.5)

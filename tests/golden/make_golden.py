@@ -1,0 +1,77 @@
+"""Generate the golden fixtures of the force path from the reference program.
+
+Runs the reference MDynaMix (built by `make -C oracle/ref` from the sources
+under /root/reference into oracle/_ref/mdx_ref) on seeded systems and
+stores, per system, the inputs exactly as the reference held them
+(parameters in its internal units, positions after its start-up) and its
+outputs (neighbour lists or their digest, forces F/G/H, every energy and
+virial scalar, per-type breakdowns).  Fixtures are data only.
+
+    python tests/golden/make_golden.py        # rewrites tests/golden/*.npz
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from mdynamix_amd import systems  # noqa: E402
+from oracle import refdump  # noqa: E402
+
+# name -> (builder, lambda, keep full lists)
+CASES = {
+    "water_tiny": (lambda: systems.water_nacl(20, 1, 12.0, 1, name="wtiny", rcut=5.5, rshort=3.0,
+                                              ewald_a=2.8, ewald_b=9.0), 1.0, True),
+    "water_508": (systems.cfg_small, 1.0, False),
+    "chains_tiny": (lambda: systems.chain_mixture(4, 60, 17.0, 5, name="chtiny", chain_sites=12,
+                                                  rcut=8.0, rshort=4.0, ewald_a=2.8, ewald_b=9.0,
+                                                  shake=False, nfreq=10), 1.0, True),
+    "ee_tiny": (lambda: systems.ee_job(400, 10, 0.35, 9), 0.35, True),
+}
+
+STATIC = ["SIZES", "SCALARS", "ITYPE", "NNUM", "NSITE", "INBTA", "CHARGEA", "Q", "MASSDA", "MASSA",
+          "IEE", "LMOVE", "C14EL", "C14LJ", "NSPEC", "NSITS", "SPEEI", "MDX", "A6LJ", "B12LJ",
+          "A6LJ0", "B12LJ0", "A6_14", "B12_14", "NNBB", "INBB", "RKX", "RKY", "RKZ", "IBI", "JBJ",
+          "IBUK", "ID", "RB", "FB", "DB", "RO", "IAI", "JAJ", "KAK", "IAUK", "RA", "FA", "ITI", "JTJ",
+          "KTK", "LTL", "ITUK", "ITORS", "NMUL", "FT", "RT", "FT1", "FT2", "FT3", "FT4", "FT5",
+          "LISTLEN"]
+PERCALL = ["SX", "SY", "SZ", "XCOM", "YCOM", "ZCOM", "FX", "FY", "FZ", "GX", "GY", "GZ", "HX", "HY",
+           "HZ", "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE"]
+
+
+def list_digest(a: np.ndarray, b: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, "<i4").tobytes() +
+                          np.ascontiguousarray(b, "<i4").tobytes()).hexdigest()
+
+
+def main() -> None:
+    for name, (builder, lam, full_lists) in CASES.items():
+        s = builder()
+        d = refdump.run_reference(s)
+        out = {"lam": np.array(lam)}
+        for k in STATIC:
+            out[k] = d[k][0]
+        ncall = len(d["SX"])
+        out["NCALL"] = np.array(ncall)
+        for c in range(ncall):
+            for k in PERCALL:
+                out["%s_%d" % (k, c)] = d[k][c]
+        if full_lists:
+            for k in ("NBS1", "NBS2", "NBL1", "NBL2"):
+                out[k] = d[k][0]
+        out["DIGEST_SHORT"] = np.array(list_digest(d["NBS1"][0], d["NBS2"][0]))
+        out["DIGEST_LONG"] = np.array(list_digest(d["NBL1"][0], d["NBL2"][0]))
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **out)
+        print("%-12s atoms=%d calls=%d lists=%s -> %s (%d KB)" % (
+            name, s.natoms, ncall, tuple(d["LISTLEN"][0]), path, os.path.getsize(path) // 1024))
+
+
+if __name__ == "__main__":
+    main()
