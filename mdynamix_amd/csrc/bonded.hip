@@ -1,0 +1,375 @@
+// Bonded terms, Ewald exclusion corrections, force assembly and reductions.
+//
+// k_eterms   ETERMS  forces.f:1482-1676 (one bound/intramolecular pair per thread)
+// k_bonds    GETBND  forces.f:113-204
+// k_angles   GETANG  forces.f:216-309
+// k_tors     GETTRS  forces.f:323-513
+// All in FP64 with the reference's expressions; forces are scattered in
+// 64-bit fixed point (order independent), energies reduced per block in a
+// fixed tree order.
+// k_combine  F = G + H (CM_ADDLA, scalar.f:82-116) and the molecular virial
+//            sums WIRS/WIRSS (scalar.f:72-76, forces.f:991-995)
+// k_com      molecular centres of mass (GETCOM, service.f:471-500)
+// k_ereduce  final FP64 sum of all per-block partials
+#include "mdx_internal.h"
+
+namespace mdx {
+
+template <int NQ>
+__device__ inline void block_reduce_store(double (&q)[NQ], const int (&slot)[NQ], double *out) {
+    __shared__ double red[256];
+    for (int i = 0; i < NQ; ++i) {
+        red[threadIdx.x] = q[i];
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[slot[i]] = red[0];
+        __syncthreads();
+    }
+}
+
+__device__ inline void zero_slots(double *o) {
+    if (threadIdx.x == 0)
+        for (int k = 0; k < Q_N; ++k) o[k] = 0.0;
+    __syncthreads();
+}
+
+struct ScatterArgs {
+    const double *x, *y, *z;
+    Box box;
+    double fscale;
+    unsigned long long *fx, *fy, *fz;
+    double *epart;
+};
+
+// pairs: (i, j) with i > j; tp: molecule type of i
+__global__ __launch_bounds__(256) void k_eterms(int np, const int2 *__restrict__ pairs, const int *__restrict__ atom_moltype,
+                                                const double *__restrict__ q, double coulf, double alpha,
+                                                ScatterArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0, 0, 0, 0};
+    if (k < np) {
+        const int isp = pairs[k].x, jsp = pairs[k].y;
+        const double qij = coulf * q[isp] * q[jsp];
+        double dx = a.x[isp] - a.x[jsp], dy = a.y[isp] - a.y[jsp], dz = a.z[isp] - a.z[jsp];
+        min_image(dx, dy, dz, a.box);
+        double r2 = dx * dx + dy * dy;
+        r2 = r2 + dz * dz;
+        const double r2i = 1.0 / r2;
+        const double r1 = sqrt(r2);
+        const double r1i = r1 * r2i;
+        const double alphar = alpha * r1;
+        const double ttt = 1.0 / (1.0 + kB1 * alphar);
+        const double exp2a = exp(-(alphar * alphar));
+        const double erfc_ = ((((kA5 * ttt + kA4) * ttt + kA3) * ttt + kA2) * ttt + kA1) * ttt * exp2a;
+        v[0] = qij * (erfc_ - 1.0) * r1i;
+        const double fes = qij * ((kTOTPI * alphar * exp2a + erfc_) - 1.0) * r1i * r2i;
+        fx_add(a.fx + isp, fes * dx, a.fscale); fx_add(a.fy + isp, fes * dy, a.fscale);
+        fx_add(a.fz + isp, fes * dz, a.fscale);
+        fx_add(a.fx + jsp, -fes * dx, a.fscale); fx_add(a.fy + jsp, -fes * dy, a.fscale);
+        fx_add(a.fz + jsp, -fes * dz, a.fscale);
+        v[1] = fes * (dx * dx); v[2] = fes * (dy * dy); v[3] = fes * (dz * dz);
+    }
+    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    zero_slots(o);
+    const int slots[4] = {Q_SPE, Q_VIRX, Q_VIRY, Q_VIRZ};
+    block_reduce_store<4>(v, slots, o);
+}
+
+struct BondArgs {
+    int n;
+    const int *bi, *bj, *bpar;
+    const int *morse;
+    const double *r0, *k, *d, *rho;
+};
+
+__global__ __launch_bounds__(256) void k_bonds(BondArgs t, ScatterArgs a) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[5] = {0, 0, 0, 0, 0};
+    if (n < t.n) {
+        const int i = t.bi[n], j = t.bj[n], m = t.bpar[n];
+        const double req = t.r0[m];
+        double bx = a.x[j] - a.x[i], by = a.y[j] - a.y[i], bz = a.z[j] - a.z[i];
+        min_image(bx, by, bz, a.box);
+        double bsq = bx * bx + by * by;
+        bsq = bsq + bz * bz;
+        const double bd = sqrt(bsq);
+        const double r1i = 1.0 / bd;
+        const double bbb = bd - req;
+        double ebdd, forb;
+        if (!t.morse[m] || bbb > req) {
+            const double dbnd = bbb * t.k[m];
+            ebdd = dbnd * bbb;
+            forb = -2.0 * dbnd * r1i;
+        } else {
+            const double exm = exp(-t.rho[m] * bbb);
+            const double exmp = 1.0 - exm;
+            ebdd = t.d[m] * (exmp * exmp);
+            forb = -2.0 * t.d[m] * t.rho[m] * exm * exmp * r1i;
+        }
+        v[0] = ebdd;
+        v[1] = forb * bsq;
+        v[2] = forb * (bx * bx); v[3] = forb * (by * by); v[4] = forb * (bz * bz);
+        fx_add(a.fx + i, -bx * forb, a.fscale); fx_add(a.fy + i, -by * forb, a.fscale);
+        fx_add(a.fz + i, -bz * forb, a.fscale);
+        fx_add(a.fx + j, bx * forb, a.fscale); fx_add(a.fy + j, by * forb, a.fscale);
+        fx_add(a.fz + j, bz * forb, a.fscale);
+    }
+    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    zero_slots(o);
+    const int slots[5] = {Q_PINT, Q_VIRB, Q_VIRFX, Q_VIRFY, Q_VIRFZ};
+    block_reduce_store<5>(v, slots, o);
+}
+
+struct AngleArgs {
+    int n;
+    const int *ai, *aj, *ak, *apar;
+    const double *theta0, *k;
+};
+
+__global__ __launch_bounds__(256) void k_angles(AngleArgs t, ScatterArgs a) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0, 0, 0, 0};
+    if (n < t.n) {
+        const int i = t.ai[n], j = t.aj[n], k = t.ak[n], m = t.apar[n];
+        const double fcn = t.k[m], aeq = t.theta0[m];
+        double ax = a.x[i] - a.x[j], ay = a.y[i] - a.y[j], az = a.z[i] - a.z[j];
+        min_image(ax, ay, az, a.box);
+        double asq = ax * ax + ay * ay; asq = asq + az * az;
+        const double asqi = 1.0 / asq;
+        double bx = a.x[k] - a.x[j], by = a.y[k] - a.y[j], bz = a.z[k] - a.z[j];
+        min_image(bx, by, bz, a.box);
+        double bsq = bx * bx + by * by; bsq = bsq + bz * bz;
+        const double bsqi = 1.0 / bsq;
+        const double ab = sqrt(asq * bsq);
+        const double abi = 1.0 / ab;
+        double cosa = ax * bx + ay * by; cosa = cosa + az * bz;
+        cosa = cosa * abi;
+        cosa = fmin(cosa, 1.0);
+        cosa = fmax(cosa, -1.0);
+        const double alf = acos(cosa);
+        const double sinai = 1.0 / sin(alf);
+        const double dalf = alf - aeq;
+        const double abc = dalf * fcn;
+        v[0] = abc * dalf;
+        const double sab = -2.0 * abc * sinai;
+        const double cab = sab * cosa;
+        const double fab = sab * abi;
+        const double faa = cab * asqi, fbb = cab * bsqi;
+        const double fax = fab * bx - faa * ax, fay = fab * by - faa * ay, faz = fab * bz - faa * az;
+        const double fbx = fab * ax - fbb * bx, fby = fab * ay - fbb * by, fbz = fab * az - fbb * bz;
+        fx_add(a.fx + i, -fax, a.fscale); fx_add(a.fy + i, -fay, a.fscale); fx_add(a.fz + i, -faz, a.fscale);
+        fx_add(a.fx + j, fax + fbx, a.fscale); fx_add(a.fy + j, fay + fby, a.fscale);
+        fx_add(a.fz + j, faz + fbz, a.fscale);
+        fx_add(a.fx + k, -fbx, a.fscale); fx_add(a.fy + k, -fby, a.fscale); fx_add(a.fz + k, -fbz, a.fscale);
+        v[1] = -ax * fax - bx * fbx;
+        v[2] = -ay * fay - by * fby;
+        v[3] = -az * faz - bz * fbz;
+    }
+    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    zero_slots(o);
+    const int slots[4] = {Q_PINT, Q_VIRFX, Q_VIRFY, Q_VIRFZ};
+    block_reduce_store<4>(v, slots, o);
+}
+
+struct TorsArgs {
+    int n;
+    const int *ti, *tj, *tk, *tl, *tpar;
+    const int *kind, *mult;
+    const double *k, *delta, *k1, *k2, *k3, *k4, *k5;
+};
+
+__global__ __launch_bounds__(256) void k_tors(TorsArgs t, ScatterArgs a) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0, 0, 0, 0};
+    if (n < t.n) {
+        const int i = t.ti[n], j = t.tj[n], k = t.tk[n], l = t.tl[n], m = t.tpar[n];
+        double ax = a.x[j] - a.x[i], ay = a.y[j] - a.y[i], az = a.z[j] - a.z[i];
+        double bx = a.x[k] - a.x[j], by = a.y[k] - a.y[j], bz = a.z[k] - a.z[j];
+        double cx = a.x[l] - a.x[k], cy = a.y[l] - a.y[k], cz = a.z[l] - a.z[k];
+        min_image(ax, ay, az, a.box); min_image(bx, by, bz, a.box); min_image(cx, cy, cz, a.box);
+        double ab = ax * bx + ay * by; ab = ab + az * bz;
+        double bc = bx * cx + by * cy; bc = bc + bz * cz;
+        double ac = ax * cx + ay * cy; ac = ac + az * cz;
+        double at = ax * ax + ay * ay; at = at + az * az;
+        double bt = bx * bx + by * by; bt = bt + bz * bz;
+        double ct = cx * cx + cy * cy; ct = ct + cz * cz;
+        double axb = (at * bt) - (ab * ab);
+        double bxc = (bt * ct) - (bc * bc);
+        const double fnum = (ab * bc) - (ac * bt);
+        double den = axb * bxc;
+        if (den > 1.0e-10) {
+            den = sqrt(den);
+            double co = fnum / den;
+            co = fmin(co, 1.0);
+            co = fmax(co, -1.0);
+            double signum = ax * (by * cz - cy * bz) + ay * (bz * cx - cz * bx);
+            signum = signum + az * (bx * cy - cx * by);
+            const double arg = fsign(acos(co), signum);
+            const double sin1f = sin(arg);
+            double si = sin1f;
+            if (fabs(si) < 1.0e-10) si = fsign(1.0e-10, si);
+            double pott = 0.0, deri = 0.0;
+            const int kd = t.kind[m];
+            if (kd == 0) {
+                const double earg = t.mult[m] * arg - t.delta[m];
+                pott = t.k[m] * (1.0 + cos(earg));
+                deri = -t.k[m] * t.mult[m] * sin(earg);
+            } else if (kd == 1) {
+                const double c1 = cos(arg);
+                const double c2 = 2.0 * (c1 * c1) - 1.0;
+                const double c3 = c1 * (2.0 * c2 - 1.0);
+                const double s2 = 2.0 * sin1f * c1;
+                const double s3 = sin1f * c2 + s2 * c1;
+                pott = t.k1[m] * (1.0 + c1) + t.k2[m] * (1.0 - c2) + t.k3[m] * (1.0 + c3);
+                deri = -t.k1[m] * sin1f + 2.0 * t.k2[m] * s2 - 3.0 * t.k3[m] * s3;
+            } else if (kd == 5) {
+                const double earg = arg - t.delta[m];
+                const double c1 = cos(earg), c2 = c1 * c1, c3 = c1 * c2, c4 = c1 * c3, c5 = c1 * c4;
+                pott = t.k[m] + t.k1[m] * c1 + t.k2[m] * c2 + t.k3[m] * c3 + t.k4[m] * c4 + t.k5[m] * c5;
+                deri = -sin(earg) * (t.k1[m] + 2.0 * t.k2[m] * c1 + 3.0 * t.k3[m] * c2 + 4.0 * t.k4[m] * c3 +
+                                     5.0 * t.k5[m] * c4);
+            } else if (kd == -1) {
+                const double earg = arg - t.delta[m];
+                pott = t.k[m] * (earg * earg);
+                deri = 2.0 * t.k[m] * earg;
+            }
+            v[0] = pott;
+            const double de1 = deri / den / si;
+            axb = axb / den * co;
+            bxc = bxc / den * co;
+            double f[3][4];
+            const double A[3] = {ax, ay, az}, B[3] = {bx, by, bz}, Cc[3] = {cx, cy, cz};
+            for (int d = 0; d < 3; ++d) {
+                double dnum = Cc[d] * bt - B[d] * bc;
+                double dden = (ab * B[d] - A[d] * bt) * bxc;
+                f[d][0] = (dnum - dden) * de1;
+                dnum = ((B[d] - A[d]) * bc - ab * Cc[d]) + (2.0 * ac * B[d] - Cc[d] * bt);
+                dden = axb * (bc * Cc[d] - B[d] * ct) + (A[d] * bt - at * B[d] - ab * (B[d] - A[d])) * bxc;
+                f[d][1] = (dnum - dden) * de1;
+                dnum = ab * B[d] - A[d] * bt;
+                dden = axb * (bt * Cc[d] - bc * B[d]);
+                f[d][3] = (dnum - dden) * de1;
+                f[d][2] = -(f[d][0] + f[d][1] + f[d][3]);
+            }
+            unsigned long long *F[3] = {a.fx, a.fy, a.fz};
+            const int at4[4] = {i, j, k, l};
+            for (int d = 0; d < 3; ++d)
+                for (int q = 0; q < 4; ++q) fx_add(F[d] + at4[q], f[d][q], a.fscale);
+            for (int d = 0; d < 3; ++d) v[1 + d] = -(A[d] + B[d]) * f[d][0] - B[d] * f[d][1] + Cc[d] * f[d][3];
+        }
+    }
+    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    zero_slots(o);
+    const int slots[4] = {Q_PINT, Q_VIRFX, Q_VIRFY, Q_VIRFZ};
+    block_reduce_store<4>(v, slots, o);
+}
+
+// Molecular centres of mass: atoms of molecule m are [mstart[m], mstart[m+1]).
+__global__ void k_com(int nmol, const int *__restrict__ mstart, const double *__restrict__ mass,
+                      const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+                      double *__restrict__ cx, double *__restrict__ cy, double *__restrict__ cz) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    double sx = 0, sy = 0, sz = 0, sm = 0;
+    for (int i = mstart[m]; i < mstart[m + 1]; ++i) {
+        sx = sx + mass[i] * x[i];
+        sy = sy + mass[i] * y[i];
+        sz = sz + mass[i] * z[i];
+        sm = sm + mass[i];
+    }
+    cx[m] = sx / sm; cy[m] = sy / sm; cz[m] = sz / sm;
+}
+
+struct CombineArgs {
+    int n;
+    int group;                     // 0 ALL, 1 FAST, 2 SLOW
+    const double *fsx, *fsy, *fsz; // pair short
+    const double *flx, *fly, *flz; // pair long
+    const double *recip;           // [nslice][3][n]
+    int nslice;
+    const unsigned long long *gx, *gy, *gz;    // fixed: slow scatter (ETERMS, long specials)
+    const unsigned long long *hlx, *hly, *hlz; // fixed: LOCALF specials
+    const unsigned long long *hbx, *hby, *hbz; // fixed: bonded
+    double inv_scale;
+    const double *x, *y, *z;
+    const int *atom_mol;
+    const double *cx, *cy, *cz;
+    double *ox, *oy, *oz;          // F of the group
+    double *Gx, *Gy, *Gz;          // slow (optional)
+    double *Hx, *Hy, *Hz;          // fast (optional)
+    double *epart;
+};
+
+__device__ inline double fixd(const unsigned long long *p, int i, double inv) {
+    return (double)(long long)p[i] * inv;
+}
+
+__global__ __launch_bounds__(256) void k_combine(CombineArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0, 0};
+    if (i < a.n) {
+        double G[3] = {0, 0, 0}, HL[3] = {0, 0, 0}, H[3] = {0, 0, 0};
+        if (a.group != 1) {
+            G[0] = a.flx[i] + fixd(a.gx, i, a.inv_scale);
+            G[1] = a.fly[i] + fixd(a.gy, i, a.inv_scale);
+            G[2] = a.flz[i] + fixd(a.gz, i, a.inv_scale);
+            for (int s = 0; s < a.nslice; ++s)
+                for (int d = 0; d < 3; ++d) G[d] += a.recip[((size_t)s * 3 + d) * a.n + i];
+        }
+        if (a.group != 2) {
+            HL[0] = a.fsx[i] + fixd(a.hlx, i, a.inv_scale);
+            HL[1] = a.fsy[i] + fixd(a.hly, i, a.inv_scale);
+            HL[2] = a.fsz[i] + fixd(a.hlz, i, a.inv_scale);
+            H[0] = HL[0] + fixd(a.hbx, i, a.inv_scale);
+            H[1] = HL[1] + fixd(a.hby, i, a.inv_scale);
+            H[2] = HL[2] + fixd(a.hbz, i, a.inv_scale);
+        }
+        const int m = a.atom_mol[i];
+        const double wx = a.x[i] - a.cx[m], wy = a.y[i] - a.cy[m], wz = a.z[i] - a.cz[m];
+        v[0] = -G[0] * wx - G[1] * wy - G[2] * wz;
+        v[1] = -HL[0] * wx - HL[1] * wy - HL[2] * wz;
+        if (a.ox) {
+            a.ox[i] = G[0] + H[0]; a.oy[i] = G[1] + H[1]; a.oz[i] = G[2] + H[2];
+        }
+        if (a.Gx) { a.Gx[i] = G[0]; a.Gy[i] = G[1]; a.Gz[i] = G[2]; }
+        if (a.Hx) { a.Hx[i] = H[0]; a.Hy[i] = H[1]; a.Hz[i] = H[2]; }
+    }
+    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    zero_slots(o);
+    const int slots[2] = {Q_WIRS, Q_WIRSS};
+    block_reduce_store<2>(v, slots, o);
+}
+
+// Sum rows [0, nrows) of epart; block q sums slot q with a fixed
+// strided-then-tree order (256 threads).
+__global__ __launch_bounds__(256) void k_ereduce(int nrows, const double *__restrict__ epart,
+                                                 double *__restrict__ out) {
+    __shared__ double red[256];
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < nrows; r += 256) s += epart[(size_t)r * Q_N + q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
+}
+
+// Zero fixed-point accumulators (kernel, not a memset node, so the call
+// sequence can be captured in a graph).
+__global__ void k_zero_u64(size_t n, unsigned long long *__restrict__ p) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_fill_i32(int n, int *__restrict__ p, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace mdx

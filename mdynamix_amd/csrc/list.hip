@@ -1,0 +1,230 @@
+// Neighbour-list construction (replaces CHCNB, forces.f:1813-2205).
+//
+// The reference scans all N^2/2 pairs.  Here atoms are binned into x-y
+// columns, sorted by z inside each column and cut into clusters of 8; an
+// i-cluster lists every j-cluster whose bounding box comes within the list
+// radius (full list: both (i,j) and (j,i) appear).  For each listed cluster
+// pair a wavefront tests all 64 atom pairs with the reference's arithmetic
+// (FP64 differences of the caller's coordinates, one-shift minimum image,
+// rr2 = (dx^2 + dy^2) + dz^2 without FMA contraction, inclusive <= against
+// R_cut^2 and R_short^2), so the set of listed pairs and their short/long
+// class are those CHCNB would produce.  Same-molecule pairs closer than 8 A
+// are checked against the bound-atom list (forces.f:1943-1962): 1-2/1-3
+// pairs are dropped, 1-4 pairs and intramolecular pairs of lambda-coupled
+// molecules go to the special list (classified by k_special_classify).
+#include "mdx_internal.h"
+
+namespace mdx {
+
+// Wrap into [0, L) and bin into x-y columns; key = column << 32 | z rank.
+__global__ void k_bin(int n, const double *__restrict__ x, const double *__restrict__ y,
+                      const double *__restrict__ z, Box b, int ncx, int ncy,
+                      unsigned long long *__restrict__ keys, int *__restrict__ vals,
+                      int *__restrict__ colcount) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double w[3] = {x[i], y[i], z[i]};
+    for (int k = 0; k < 3; ++k) {
+        w[k] -= b.l[k] * floor(w[k] / b.l[k]);
+        if (!(w[k] < b.l[k]) || w[k] < 0.0) w[k] = 0.0;
+    }
+    int cx = (int)(w[0] / b.l[0] * ncx), cy = (int)(w[1] / b.l[1] * ncy);
+    cx = min(max(cx, 0), ncx - 1);
+    cy = min(max(cy, 0), ncy - 1);
+    const int col = cy * ncx + cx;
+    double fz = w[2] / b.l[2] * 4294967295.0;
+    unsigned int zq = (unsigned int)fmin(fmax(fz, 0.0), 4294967295.0);
+    keys[i] = ((unsigned long long)col << 32) | zq;
+    vals[i] = i;
+    atomicAdd(&colcount[col], 1);
+}
+
+// Clusters per column: ceil(count / 8).
+__global__ void k_col_clusters(int ncol, const int *__restrict__ colcount, int *__restrict__ colclu_n) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncol) colclu_n[c] = (colcount[c] + kClu - 1) / kClu;
+}
+
+// Sorted atom p -> cluster slot.
+__global__ void k_fill(int n, const unsigned long long *__restrict__ skeys, const int *__restrict__ svals,
+                       const int *__restrict__ colstart, const int *__restrict__ colclu,
+                       int *__restrict__ clu_atom, int *__restrict__ atom_slot, int *__restrict__ clu_col) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int col = (int)(skeys[p] >> 32);
+    const int r = p - colstart[col];
+    const int c = colclu[col] + r / kClu;
+    const int slot = c * kClu + (r % kClu);
+    const int a = svals[p];
+    clu_atom[slot] = a;
+    atom_slot[a] = slot;
+    if (r % kClu == 0) clu_col[c] = col;
+}
+
+// Cluster anchor (bounding-box centre of the wrapped coordinates, FP64)
+// and half extents.
+__global__ void k_cluster_geom(int nclu, const int *__restrict__ clu_atom, const double *__restrict__ x,
+                               const double *__restrict__ y, const double *__restrict__ z, Box b,
+                               double *__restrict__ anchor, float *__restrict__ ext) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nclu) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int s = 0; s < kClu; ++s) {
+        const int a = clu_atom[c * kClu + s];
+        if (a < 0) continue;
+        double w[3] = {x[a], y[a], z[a]};
+        for (int k = 0; k < 3; ++k) {
+            w[k] -= b.l[k] * floor(w[k] / b.l[k]);
+            lo[k] = fmin(lo[k], w[k]);
+            hi[k] = fmax(hi[k], w[k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        anchor[c * 3 + k] = 0.5 * (lo[k] + hi[k]);
+        ext[c * 3 + k] = (float)(0.5 * (hi[k] - lo[k])) * 1.0001f + 1e-4f;
+    }
+}
+
+// Candidate j-clusters of each i-cluster: bounding boxes within r_list.
+// One thread per i-cluster; neighbour columns within reach are visited once
+// each (periodic wrap).
+__global__ void k_candidates(int nclu, const double *__restrict__ anchor, const float *__restrict__ ext,
+                             const int *__restrict__ clu_col, const int *__restrict__ colclu,
+                             const int *__restrict__ colclu_n, int ncx, int ncy, int rx, int ry, Box b,
+                             double rlist, int cap, int *__restrict__ cand, int *__restrict__ ncand,
+                             int *__restrict__ overflow) {
+    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nclu) return;
+    const int col = clu_col[ci];
+    const int cx0 = col % ncx, cy0 = col / ncx;
+    const double ai[3] = {anchor[ci * 3], anchor[ci * 3 + 1], anchor[ci * 3 + 2]};
+    const float ei[3] = {ext[ci * 3], ext[ci * 3 + 1], ext[ci * 3 + 2]};
+    const int nx = min(2 * rx + 1, ncx), ny = min(2 * ry + 1, ncy);
+    const int sx = (2 * rx + 1 >= ncx) ? 0 : cx0 - rx + ncx;
+    const int sy = (2 * ry + 1 >= ncy) ? 0 : cy0 - ry + ncy;
+    const double r2 = rlist * rlist;
+    int cnt = 0;
+    for (int oy = 0; oy < ny; ++oy) {
+        const int cy = (sy + oy) % ncy;
+        for (int ox = 0; ox < nx; ++ox) {
+            const int cx = (sx + ox) % ncx;
+            const int c2 = cy * ncx + cx;
+            const int beg = colclu[c2], end = beg + colclu_n[c2];
+            for (int cj = beg; cj < end; ++cj) {
+                double d2 = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    double d = ai[k] - anchor[cj * 3 + k];
+                    d -= b.l[k] * rint(d / b.l[k]);
+                    double g = fabs(d) - (double)(ei[k] + ext[cj * 3 + k]);
+                    if (g > 0.0) d2 += g * g;
+                }
+                if (d2 <= r2) {
+                    if (cnt < cap) cand[(size_t)ci * cap + cnt] = cj;
+                    ++cnt;
+                }
+            }
+        }
+    }
+    ncand[ci] = min(cnt, cap);
+    if (cnt > cap) atomicMax(overflow, cnt);
+}
+
+// Is j bound to i (returns 1 for 1-2/1-3, 2 for 1-4, 0 otherwise)?
+__device__ inline int bound_kind(int i, int j, const int *__restrict__ excl_start,
+                                 const int *__restrict__ excl_atom, const signed char *__restrict__ excl_is14) {
+    for (int k = excl_start[i]; k < excl_start[i + 1]; ++k)
+        if (excl_atom[k] == j) return excl_is14[k] ? 2 : 1;
+    return 0;
+}
+
+// Pair masks for every candidate of an i-cluster; one wavefront per
+// i-cluster, lane = i_slot * 8 + j_slot.
+__global__ __launch_bounds__(64) void k_masks(
+    int nclu, const int *__restrict__ cand, const int *__restrict__ ncand, int cap,
+    const int *__restrict__ clu_atom, const double *__restrict__ anchor, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const int *__restrict__ atom_mol,
+    const int *__restrict__ atom_moltype, const int *__restrict__ type_ee, const int *__restrict__ excl_start,
+    const int *__restrict__ excl_atom, const signed char *__restrict__ excl_is14, Box b, double rl2,
+    double short2, PairEntry *__restrict__ list, int *__restrict__ nlist, int list_cap,
+    int *__restrict__ overflow) {
+#pragma clang fp contract(off)
+    const int ci = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int il = lane >> 3, jl = lane & 7;
+    const int ai = clu_atom[ci * kClu + il];
+    double xi = 0, yi = 0, zi = 0;
+    int moli = -1, eei = 1;
+    if (ai >= 0) {
+        xi = x[ai]; yi = y[ai]; zi = z[ai];
+        moli = atom_mol[ai];
+        eei = type_ee[atom_moltype[ai]];
+    }
+    const double aci[3] = {anchor[ci * 3], anchor[ci * 3 + 1], anchor[ci * 3 + 2]};
+    const int nc = ncand[ci];
+    int cnt = 0;
+    for (int k = 0; k < nc; ++k) {
+        const int cj = cand[(size_t)ci * cap + k];
+        const int aj = clu_atom[cj * kClu + jl];
+        bool listed = false, shrt = false;
+        if (ai >= 0 && aj >= 0 && ai != aj) {
+            double dx = xi - x[aj], dy = yi - y[aj], dz = zi - z[aj];
+            min_image(dx, dy, dz, b);
+            double rr2 = dx * dx + dy * dy;
+            rr2 = rr2 + dz * dz;
+            listed = rr2 <= rl2;
+            shrt = rr2 <= short2;
+            if (listed && moli == atom_mol[aj]) {
+                if (rr2 <= 64.0 && bound_kind(ai, aj, excl_start, excl_atom, excl_is14) != 0) listed = false;
+                else if (eei == 0) listed = false;
+            }
+        }
+        const unsigned long long m = __ballot(listed);
+        const unsigned long long ms = __ballot(listed && shrt);
+        if (m != 0ull) {
+            if (lane == 0 && cnt < list_cap) {
+                PairEntry e;
+                e.cj = cj;
+                double s[3];
+                for (int q = 0; q < 3; ++q) {
+                    double d = aci[q] - anchor[cj * 3 + q];
+                    s[q] = d - b.l[q] * rint(d / b.l[q]);
+                }
+                e.sx = (float)s[0]; e.sy = (float)s[1]; e.sz = (float)s[2];
+                e.mask = m;
+                e.mshort = ms;
+                list[(size_t)ci * list_cap + cnt] = e;
+            }
+            ++cnt;
+        }
+    }
+    if (lane == 0) {
+        nlist[ci] = min(cnt, list_cap);
+        if (cnt > list_cap) atomicMax(overflow, cnt);
+    }
+}
+
+// Special pairs (1-4 pairs, intramolecular pairs of lambda-coupled
+// molecules) re-classified from the current coordinates with the same
+// arithmetic and rules as CHCNB.  Candidates are (i > j) pairs fixed by the
+// topology; kind bit 0: bound 1-2/1-3, bit 1: bound 1-4, bit 2: EE molecule.
+__global__ void k_special_classify(int ns, const int2 *__restrict__ pairs, const signed char *__restrict__ kind,
+                                   const double *__restrict__ x, const double *__restrict__ y,
+                                   const double *__restrict__ z, Box b, double rl2, double short2,
+                                   signed char *__restrict__ cls) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int i = pairs[k].x, j = pairs[k].y;
+    double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+    min_image(dx, dy, dz, b);
+    double rr2 = dx * dx + dy * dy;
+    rr2 = rr2 + dz * dz;
+    const int kd = kind[k];
+    signed char c = SP_NONE;
+    if (rr2 <= 64.0 && (kd & 1)) c = SP_NONE;
+    else if (rr2 <= 64.0 && (kd & 2)) c = SP_SHORT14;
+    else if (kd & 4) c = rr2 <= short2 ? SP_SHORT : (rr2 <= rl2 ? SP_LONG : SP_NONE);
+    cls[k] = c;
+}
+
+}  // namespace mdx

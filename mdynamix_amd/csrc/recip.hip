@@ -1,0 +1,183 @@
+// Ewald reciprocal sum (replaces FURIR, forces.f:1217-1339).
+//
+// The half-space k set of INIFUR (forces.f:1356-1427) is regrouped into
+// rows of fixed (n_x, n_y) whose n_z run over a contiguous range; rows are
+// sorted by length so the 64 rows of one wavefront have similar lengths.
+//
+// k_sk: lane = row, atoms of one chunk streamed with uniform (scalar) loads.
+//   For each atom the phase 2 pi (n_x f_x + n_y f_y + n_z0 f_z) is formed
+//   in FP64 fractional coordinates and reduced to one turn before an FP32
+//   sincospi; the remaining terms of the row follow by the complex
+//   recurrence P <- P exp(2 pi i f_z).  Per-chunk FP32 sums are written as
+//   partials and added in FP64 by k_sk_reduce, which also forms the energy
+//   QPE, the virial projections and A_k = 2 AKC S(k) for the force pass.
+// k_recip_force: lane = atom, rows streamed uniformly; the force on an atom
+//   is q sum_k k Im(exp(i k.r) conj(A_k)) (forces.f:1320-1326), again with
+//   one reduced-phase sincospi per row and a recurrence along n_z.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+struct RecipRow {           // 16 bytes
+    int nx, ny, nz0, len;
+};
+
+__global__ void k_recip_prep(int n, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, const double *__restrict__ q, Box b,
+                             double4 *__restrict__ fq, float2 *__restrict__ stepz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double fx = x[i] / b.l[0], fy = y[i] / b.l[1], fz = z[i] / b.l[2];
+    fq[i] = make_double4(fx, fy, fz, q[i]);
+    double u = fz - rint(fz);
+    float s, c;
+    sincospif((float)(2.0 * u), &s, &c);
+    stepz[i] = make_float2(c, s);
+}
+
+__device__ inline void reduced_phase(const RecipRow &r, double4 a, float &c, float &s) {
+    double u = (double)r.nx * a.x + (double)r.ny * a.y;
+    u = u + (double)r.nz0 * a.z;
+    u = u - rint(u);
+    sincospif((float)(2.0 * u), &s, &c);
+}
+
+template <int MAXLEN>
+__global__ __launch_bounds__(64) void k_sk(int nrows, const RecipRow *__restrict__ rows,
+                                           const int *__restrict__ rowoff, int natoms, int chunk,
+                                           const double4 *__restrict__ fq, const float2 *__restrict__ stepz,
+                                           int nkv, float2 *__restrict__ part) {
+    const int r = blockIdx.x * 64 + threadIdx.x;
+    const int a0 = blockIdx.y * chunk;
+    const int a1 = min(a0 + chunk, natoms);
+    RecipRow row = {0, 0, 0, 0};
+    if (r < nrows) row = rows[r];
+    float are[MAXLEN], aim[MAXLEN];
+#pragma unroll
+    for (int j = 0; j < MAXLEN; ++j) { are[j] = 0.f; aim[j] = 0.f; }
+    for (int i = a0; i < a1; ++i) {
+        const double4 a = fq[i];
+        const float2 st = stepz[i];
+        float c, s;
+        reduced_phase(row, a, c, s);
+        const float qf = (float)a.w;
+        float pr = qf * c, pim = qf * s;
+#pragma unroll
+        for (int j = 0; j < MAXLEN; ++j) {
+            if (j < row.len) { are[j] += pr; aim[j] += pim; }
+            const float tr = pr * st.x - pim * st.y;
+            pim = pr * st.y + pim * st.x;
+            pr = tr;
+        }
+    }
+    if (r < nrows) {
+        const int off = rowoff[r];
+        float2 *o = part + (size_t)blockIdx.y * nkv + off;
+#pragma unroll
+        for (int j = 0; j < MAXLEN; ++j)
+            if (j < row.len) o[j] = make_float2(are[j], aim[j]);
+    }
+}
+
+template __global__ void k_sk<16>(int, const RecipRow *, const int *, int, int, const double4 *, const float2 *, int, float2 *);
+template __global__ void k_sk<32>(int, const RecipRow *, const int *, int, int, const double4 *, const float2 *, int, float2 *);
+template __global__ void k_sk<64>(int, const RecipRow *, const int *, int, int, const double4 *, const float2 *, int, float2 *);
+template __global__ void k_sk<96>(int, const RecipRow *, const int *, int, int, const double4 *, const float2 *, int, float2 *);
+
+// Sum chunk partials in FP64 (fixed order), energy and virial terms of each
+// k (forces.f:1317-1331) and A_k = 2 AKC S(k).  kidx holds (nx, ny, nz) of
+// every k in row order.
+__global__ __launch_bounds__(256) void k_sk_reduce(int nkv, int nchunk, const float2 *__restrict__ part,
+                                                   const int4 *__restrict__ kidx, Box b, double alpha,
+                                                   double coulf, float2 *__restrict__ ak,
+                                                   double2 *__restrict__ sk, double *__restrict__ epart) {
+    __shared__ double red[4][256];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+    if (k < nkv) {
+        double sr = 0.0, si = 0.0;
+        for (int c = 0; c < nchunk; ++c) {
+            const float2 p = part[(size_t)c * nkv + k];
+            sr += (double)p.x;
+            si += (double)p.y;
+        }
+        const int4 id = kidx[k];
+        const double twopi = 2.0 * kPI;
+        const double rxv = twopi * id.x / b.l[0], ryv = twopi * id.y / b.l[1], rzv = twopi * id.z / b.l[2];
+        double rk2 = rxv * rxv + ryv * ryv;
+        rk2 = rk2 + rzv * rzv;
+        const double alp2 = alpha * alpha;
+        const double vol = b.l[0] * b.l[1] * b.l[2];
+        const double cfur = 4.0 * kPI * coulf / vol;
+        const double akc = cfur * exp(-0.25 * rk2 / alp2) / rk2;
+        const double s2 = sr * sr + si * si;
+        e = akc * s2;
+        const double akv = 0.5 * akc * (4.0 * alp2 + rk2) / (alp2 * rk2);
+        vx = e - rxv * rxv * akv * s2;
+        vy = e - ryv * ryv * akv * s2;
+        vz = e - rzv * rzv * akv * s2;
+        ak[k] = make_float2((float)(2.0 * akc * sr), (float)(2.0 * akc * si));
+        sk[k] = make_double2(sr, si);
+    }
+    red[0][threadIdx.x] = e; red[1][threadIdx.x] = vx; red[2][threadIdx.x] = vy; red[3][threadIdx.x] = vz;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *o = epart + (size_t)blockIdx.x * Q_N;
+        for (int q = 0; q < Q_N; ++q) o[q] = 0.0;
+        o[Q_PELS1] = red[0][0];
+        o[Q_VIRX] = red[1][0];
+        o[Q_VIRY] = red[2][0];
+        o[Q_VIRZ] = red[3][0];
+    }
+}
+
+// Reciprocal forces; blockIdx.y selects a slice of rows, each slice writes
+// its own partial force array (summed by the combine kernel).
+__global__ __launch_bounds__(64) void k_recip_force(int natoms, int nrows, int rows_per_slice,
+                                                    const RecipRow *__restrict__ rows, const int *__restrict__ rowoff,
+                                                    const double4 *__restrict__ fq, const float2 *__restrict__ stepz,
+                                                    const float2 *__restrict__ ak, Box b,
+                                                    double *__restrict__ fpart) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int r0 = blockIdx.y * rows_per_slice;
+    const int r1 = min(r0 + rows_per_slice, nrows);
+    double4 a = make_double4(0, 0, 0, 0);
+    float2 st = make_float2(1.f, 0.f);
+    if (i < natoms) { a = fq[i]; st = stepz[i]; }
+    const float qf = (float)a.w;
+    const double tx = 2.0 * kPI / b.l[0], ty = 2.0 * kPI / b.l[1], tz = 2.0 * kPI / b.l[2];
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    for (int r = r0; r < r1; ++r) {
+        const RecipRow row = rows[r];
+        const float2 *A = ak + rowoff[r];
+        float c, s;
+        reduced_phase(row, a, c, s);
+        float wr = qf * c, wi = qf * s;
+        float ts = 0.f, tzs = 0.f;
+        for (int j = 0; j < row.len; ++j) {
+            const float2 Ak = A[j];
+            const float t = wi * Ak.x - wr * Ak.y;
+            ts += t;
+            tzs += (float)j * t;
+            const float nr = wr * st.x - wi * st.y;
+            wi = wr * st.y + wi * st.x;
+            wr = nr;
+        }
+        Fx += tx * row.nx * (double)ts;
+        Fy += ty * row.ny * (double)ts;
+        Fz += tz * ((double)row.nz0 * (double)ts + (double)tzs);
+    }
+    if (i < natoms) {
+        double *o = fpart + (size_t)blockIdx.y * 3 * natoms;
+        o[i] = Fx;
+        o[natoms + i] = Fy;
+        o[2 * natoms + i] = Fz;
+    }
+}
+
+}  // namespace mdx

@@ -1,0 +1,195 @@
+"""Python host side of the MI355X force engine (ctypes over libmdx.so).
+
+`ForceEngine` mirrors the reference force-routine interface (forces.f):
+
+    chcnb()        CHCNB: rebuild the neighbour lists
+    allforce()     ALLFORCE: F = G + H with every energy/virial scalar
+    fforces()      FFORCES: fast forces H (short pairs + bonded)
+    slforces()     SLFORCES: slow forces G (k-space, exclusions, long pairs)
+
+Errors of the C ABI raise `MdxError` carrying the status name and the
+engine's message.  There is no CPU fallback: without the HIP library or a
+device the constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+
+from . import abi
+from .abi import MdxEnergies, as_ptr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdx.so")
+
+_lib = None
+
+
+class MdxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__("%s: %s" % (abi.STATUS_NAMES.get(status, str(status)), msg))
+        self.status = status
+
+
+def load_library() -> C.CDLL:
+    """Load the in-tree HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError("libmdx.so not built: run python -m mdynamix_amd.build")
+    L = C.CDLL(LIB_PATH)
+    pd, pv = C.POINTER(C.c_double), C.c_void_p
+    pi32 = C.POINTER(C.c_int32)
+    L.mdx_version.restype = C.c_char_p
+    L.mdx_last_error.restype = C.c_char_p
+    L.mdx_last_error.argtypes = [pv]
+    L.mdx_device_info.argtypes = [pi32, pi32, C.c_char_p, C.c_int32]
+    L.mdx_create.argtypes = [pv, pv, C.POINTER(pv)]
+    L.mdx_destroy.argtypes = [pv]
+    L.mdx_set_positions.argtypes = [pv, pd, pd, pd]
+    L.mdx_set_positions_device.argtypes = [pv, pv, pv, pv]
+    L.mdx_set_box.argtypes = [pv, pd]
+    L.mdx_set_molecule_com.argtypes = [pv, pd, pd, pd]
+    L.mdx_build_list.argtypes = [pv]
+    L.mdx_list_counts.argtypes = [pv, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.mdx_get_list.argtypes = [pv, pi32, pi32, C.c_int64, pi32, pi32, C.c_int64]
+    L.mdx_compute.argtypes = [pv, C.c_int32, C.c_int32] + [pd] * 9 + [pv]
+    L.mdx_compute_device.argtypes = [pv, C.c_int32, C.c_int32, pv, pv, pv, pv]
+    L.mdx_stage_times.argtypes = [pv, pd, C.c_int32]
+    for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
+                 "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_get_list", "mdx_compute",
+                 "mdx_compute_device", "mdx_stage_times", "mdx_device_info"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+STAGES = ("list", "pairs", "recip_sk", "recip_force", "bonded", "reduce")
+
+
+class ForceEngine:
+    """One system on the GPU.  `topo` is a TopologyArrays, `params` a dict
+    of mdx_params fields (see topology.from_system / from_dump)."""
+
+    def __init__(self, topo: abi.TopologyArrays, params: Dict):
+        self.lib = load_library()
+        self.topo = topo
+        self.params = dict(params)
+        self._ts = topo.struct()
+        self._ps = abi.make_params(**params)
+        h = C.c_void_p()
+        st = self.lib.mdx_create(C.byref(self._ts), C.byref(self._ps), C.byref(h))
+        self._h = h
+        if st != 0:
+            msg = self.lib.mdx_last_error(h).decode() if h.value else ""
+            if h.value:
+                self.lib.mdx_destroy(h)
+                self._h = C.c_void_p()
+            raise MdxError(st, msg)
+        self.n = topo.natoms
+
+    def _check(self, st: int) -> None:
+        if st != 0:
+            raise MdxError(st, self.lib.mdx_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            self.lib.mdx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------
+    def set_positions(self, pos: np.ndarray) -> None:
+        pos = np.asarray(pos, dtype=np.float64)
+        self._xyz = [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+        self._check(self.lib.mdx_set_positions(self._h, *[as_ptr(a) for a in self._xyz]))
+
+    def set_positions_device(self, x_ptr: int, y_ptr: int, z_ptr: int) -> None:
+        self._check(self.lib.mdx_set_positions_device(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                                      C.c_void_p(z_ptr)))
+
+    def set_box(self, box) -> None:
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        self._check(self.lib.mdx_set_box(self._h, as_ptr(b)))
+
+    def set_molecule_com(self, com: Optional[np.ndarray]) -> None:
+        if com is None:
+            self._check(self.lib.mdx_set_molecule_com(self._h, None, None, None))
+            return
+        self._com = [np.ascontiguousarray(com[:, k], dtype=np.float64) for k in range(3)]
+        self._check(self.lib.mdx_set_molecule_com(self._h, *[as_ptr(a) for a in self._com]))
+
+    # -- reference-named entry points ------------------------------------
+    def chcnb(self) -> None:
+        self._check(self.lib.mdx_build_list(self._h))
+
+    build_list = chcnb
+
+    def list_counts(self) -> Tuple[int, int]:
+        s, l = C.c_int64(), C.c_int64()
+        self._check(self.lib.mdx_list_counts(self._h, C.byref(s), C.byref(l)))
+        return s.value, l.value
+
+    def get_list(self):
+        """NBS1, NBS2, NBL1, NBL2 in the reference layout and order."""
+        ns, nl = self.list_counts()
+        s1 = np.zeros(max(ns, 1), np.int32); s2 = np.zeros(max(ns, 1), np.int32)
+        l1 = np.zeros(max(nl, 1), np.int32); l2 = np.zeros(max(nl, 1), np.int32)
+        pi = C.c_int32
+        self._check(self.lib.mdx_get_list(self._h, as_ptr(s1, pi), as_ptr(s2, pi), ns, as_ptr(l1, pi),
+                                          as_ptr(l2, pi), nl))
+        return s1[:ns], s2[:ns], l1[:nl], l2[:nl]
+
+    def compute(self, group: int = abi.MDX_FORCES_ALL, want: int = 0, split: bool = True):
+        """(F, G, H, energies); G and H are None unless split."""
+        n = self.n
+        f = np.zeros((3, n))
+        g = np.zeros((3, n)) if split else None
+        h = np.zeros((3, n)) if split else None
+        e = MdxEnergies()
+        ptrs = [as_ptr(f[k]) for k in range(3)]
+        ptrs += [as_ptr(g[k]) for k in range(3)] if split else [None] * 3
+        ptrs += [as_ptr(h[k]) for k in range(3)] if split else [None] * 3
+        self._check(self.lib.mdx_compute(self._h, group, want, *ptrs, C.byref(e)))
+        ed = e.as_dict(self.topo.nmoltypes, self.topo.ntypepairs)
+        return f.T.copy(), (g.T.copy() if split else None), (h.T.copy() if split else None), ed
+
+    def allforce(self, **kw):
+        return self.compute(abi.MDX_FORCES_ALL, **kw)
+
+    def fforces(self, **kw):
+        return self.compute(abi.MDX_FORCES_FAST, **kw)
+
+    def slforces(self, **kw):
+        return self.compute(abi.MDX_FORCES_SLOW, **kw)
+
+    def compute_device(self, fx_ptr: int, fy_ptr: int, fz_ptr: int, group: int = abi.MDX_FORCES_ALL,
+                       energies: bool = False):
+        e = MdxEnergies() if energies else None
+        self._check(self.lib.mdx_compute_device(self._h, group, 0, C.c_void_p(fx_ptr), C.c_void_p(fy_ptr),
+                                                C.c_void_p(fz_ptr), C.byref(e) if e is not None else None))
+        return e.as_dict(self.topo.nmoltypes, self.topo.ntypepairs) if e is not None else None
+
+    def stage_times(self) -> Dict[str, float]:
+        ms = np.zeros(len(STAGES))
+        self._check(self.lib.mdx_stage_times(self._h, as_ptr(ms), len(STAGES)))
+        return dict(zip(STAGES, ms.tolist()))
+
+
+def device_info() -> Dict:
+    L = load_library()
+    cu, clk = C.c_int32(), C.c_int32()
+    name = C.create_string_buffer(256)
+    st = L.mdx_device_info(C.byref(cu), C.byref(clk), name, 256)
+    if st != 0:
+        raise MdxError(st, "no HIP device")
+    return {"compute_units": cu.value, "clock_mhz": clk.value, "name": name.value.decode()}

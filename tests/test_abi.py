@@ -1,0 +1,44 @@
+"""CPU checks of the drop-in boundary: the HIP library loads and exports
+every entry point include/mdx.h declares; the ctypes structs match the
+header's layout."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from mdynamix_amd import abi, engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "mdx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mdx_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = engine.load_library()
+    names = declared_functions()
+    assert "mdx_create" in names and "mdx_compute" in names
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_version_string():
+    assert b"gfx950" in engine.load_library().mdx_version()
+
+
+def test_struct_layout():
+    # mdx_params: 3 box + rcut + rshort + skin + alpha + fexp + coulf (9 doubles), 2 ints, 2 doubles
+    assert C.sizeof(abi.MdxParams) == 9 * 8 + 2 * 4 + 2 * 8
+    # energies: 17 doubles, 2 int64, 2*MAX_TYPEPAIRS + 3*MAX_MOLTYPES doubles
+    assert C.sizeof(abi.MdxEnergies) == 8 * (17 + 2 + 2 * abi.MDX_MAX_TYPEPAIRS + 3 * abi.MDX_MAX_MOLTYPES)
+
+
+def test_create_rejects_bad_input_without_device_work():
+    """Argument validation happens before any device call."""
+    lib = engine.load_library()
+    h = C.c_void_p()
+    assert lib.mdx_create(None, None, C.byref(h)) == abi.MDX_ERR_ARG
