@@ -222,9 +222,10 @@ mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want,
 mdx_status mdx_compute_device(mdx_engine *h, int32_t group, int32_t want,
                               double *dfx, double *dfy, double *dfz, mdx_energies *e);
 
-/* Timing of the last mdx_compute / mdx_build_list per stage, in ms
- * (HIP events on the handle's stream): list, pairs, recip_sk, recip_force,
- * bonded+exclusions, reduce. */
+/* Timing of the last mdx_build_list and mdx_compute per stage, in ms (HIP
+ * events on the handle's stream): list build, prep (pack + COM), pairs,
+ * special pairs, k-space prep, structure factors, S(k) reduction,
+ * k-space forces, exclusions + bonded, force assembly + energy sums. */
 mdx_status mdx_stage_times(mdx_engine *h, double *ms, int32_t n);
 
 #ifdef __cplusplus

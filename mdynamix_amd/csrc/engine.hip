@@ -29,7 +29,9 @@ using namespace mdx;
 
 namespace {
 
-enum Stage { ST_LIST = 0, ST_PAIRS, ST_SK, ST_RFORCE, ST_BONDED, ST_REDUCE, ST_N };
+// Timed stages (HIP events on the handle's stream).
+// Stage k >= 1 lies between events k-1 and k of compute_impl.
+enum Stage { ST_LIST = 0, ST_PREP, ST_PAIRS, ST_SPECIAL, ST_RPREP, ST_SK, ST_SKRED, ST_RFORCE, ST_BONDED, ST_REDUCE, ST_N };
 
 }  // namespace
 
@@ -84,7 +86,7 @@ struct mdx_engine {
     int *d_cand = nullptr, *d_ncand = nullptr, *d_nlist = nullptr, *d_overflow = nullptr;
     PairEntry *d_list = nullptr;
     float4 *d_xq = nullptr;
-    int *d_info = nullptr;
+    int2 *d_aux = nullptr;
     bool have_list = false;
     // reciprocal space
     int nkv = 0, nrows = 0, maxlen = 0, chunk = 512, nchunk = 0, nslice = 1, rows_per_slice = 0;
@@ -95,7 +97,9 @@ struct mdx_engine {
     int *d_rowoff = nullptr;
     int4 *d_kidx = nullptr;
     double4 *d_fq = nullptr;
-    float2 *d_stepz = nullptr, *d_part = nullptr, *d_ak = nullptr;
+    double4 *d_stepz = nullptr;
+    float2 *d_part = nullptr, *d_ak = nullptr;
+    int *d_wavelen = nullptr;
     double2 *d_sk = nullptr;
     double *d_rpart = nullptr;
     std::vector<int4> h_kidx;   // (nx, ny, nz) in INIFUR order
@@ -107,7 +111,7 @@ struct mdx_engine {
     int epart_rows = 0;
     double *d_energy = nullptr;
     // timing
-    hipEvent_t ev[ST_N + 1] = {};
+    hipEvent_t ev[12] = {};
     double stage_ms[ST_N] = {};
     bool timing = true;
 };
@@ -226,6 +230,11 @@ mdx_status setup_kspace(mdx_engine *h) {
     }
     h->nrows = (int)rows.size();
     h->maxlen = maxlen;
+    // longest row of each wavefront of 64 rows (rows are sorted by length)
+    std::vector<int> wl((rows.size() + 63) / 64, 0);
+    for (size_t r = 0; r < rows.size(); ++r) wl[r / 64] = std::max(wl[r / 64], rows[r].len);
+    CHK(dalloc(h, &h->d_wavelen, wl.size()));
+    HIPCHK(h, hipMemcpy(h->d_wavelen, wl.data(), sizeof(int) * wl.size(), hipMemcpyHostToDevice));
     CHK(dalloc(h, &h->d_rows, rows.size()));
     CHK(dalloc(h, &h->d_rowoff, rows.size()));
     CHK(dalloc(h, &h->d_kidx, kidx.size()));
@@ -295,7 +304,7 @@ mdx_status alloc_clusters(mdx_engine *h) {
     CHK(dalloc(h, &h->d_nlist, h->nclu_cap));
     CHK(dalloc(h, &h->d_overflow, 1));
     CHK(dalloc(h, &h->d_xq, (size_t)h->nclu_cap * kClu));
-    CHK(dalloc(h, &h->d_info, (size_t)h->nclu_cap * kClu));
+    CHK(dalloc(h, &h->d_aux, (size_t)h->nclu_cap * kClu));
     // temporary storage of the sort and scans
     size_t b1 = 0, b2 = 0;
     hipcub::DeviceRadixSort::SortPairs(nullptr, b1, h->d_keys, h->d_keys2, h->d_vals, h->d_vals2, h->n, 0, 64);
@@ -414,7 +423,7 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
         h->err = "hipStreamCreate failed";
         return fail(MDX_ERR_HIP);
     }
-    for (int k = 0; k <= ST_N; ++k) hipEventCreate(&h->ev[k]);
+    for (int k = 0; k < 12; ++k) hipEventCreate(&h->ev[k]);
     const int n = h->n;
     // fixed-point scale: resolution ~1e-14 of coulomb_factor, range ~1e4 * coulomb_factor
     h->fscale = std::ldexp(1.0, (int)std::floor(std::log2(std::ldexp(1.0, 60) / (std::fabs(p->coulomb_factor) * 1e4 + 1e-300))));
@@ -548,12 +557,12 @@ mdx_status mdx_destroy(mdx_engine *h) {
     void *bufs[] = {h->d_sp_cls, h->d_x, h->d_y, h->d_z, h->d_cx, h->d_cy, h->d_cz, h->d_keys, h->d_keys2,
                     h->d_vals, h->d_vals2, h->d_colcount, h->d_colstart, h->d_colclu_n, h->d_colclu, h->d_cub,
                     h->d_clu_atom, h->d_atom_slot, h->d_clu_col, h->d_anchor, h->d_ext, h->d_cand, h->d_ncand,
-                    h->d_nlist, h->d_overflow, h->d_list, h->d_xq, h->d_info, h->d_rows, h->d_rowoff, h->d_kidx,
-                    h->d_fq, h->d_stepz, h->d_part, h->d_ak, h->d_sk, h->d_rpart, h->d_fs, h->d_fl, h->d_fix,
+                    h->d_nlist, h->d_overflow, h->d_list, h->d_xq, h->d_aux, h->d_rows, h->d_rowoff, h->d_kidx,
+                    h->d_fq, h->d_stepz, h->d_part, h->d_ak, h->d_sk, h->d_rpart, h->d_wavelen, h->d_fs, h->d_fl, h->d_fix,
                     h->d_out, h->d_epart, h->d_energy};
     for (void *q : bufs)
         if (q) hipFree(q);
-    for (int k = 0; k <= ST_N; ++k)
+    for (int k = 0; k < 12; ++k)
         if (h->ev[k]) hipEventDestroy(h->ev[k]);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -686,7 +695,7 @@ mdx_status mdx_build_list(mdx_engine *h) {
         }
     }
     k_slot_info<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, h->d_ljtype, h->d_moltype, h->d_type_ee,
-                                                   h->d_info);
+                                                   h->d_q, h->d_aux);
     if (h->nspecial > 0)
         k_special_classify<<<cdiv(h->nspecial, 256), 256, 0, st>>>(h->nspecial, h->d_sp_pairs, h->d_sp_kind, h->d_x,
                                                                    h->d_y, h->d_z, h->box, rl2,
@@ -801,7 +810,6 @@ static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, doubl
     hipStream_t st = h->stream;
     const int n = h->n;
     const int nslot = h->nclu * kClu;
-    const double sqc = std::sqrt(h->p.coulomb_factor);
     const bool verlet = h->p.list_mode == MDX_LIST_VERLET;
     const double rc2 = verlet ? h->p.rcut * h->p.rcut : HUGE_VAL;
     // energy partial rows
@@ -815,13 +823,14 @@ static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, doubl
     const int rows_c = cdiv(n, 256);
     const int total_rows = rows_pairs + rows_sp + rows_sk + rows_et + rows_b + rows_a + rows_t + rows_c;
     CHK(ensure_epart(h, total_rows));
-    if (h->timing) hipEventRecord(h->ev[0], st);
+    auto mark = [&](int k) { if (h->timing) hipEventRecord(h->ev[k], st); };
+    mark(0);
     k_zero_u64<<<cdiv((size_t)total_rows * Q_N, 256), 256, 0, st>>>((size_t)total_rows * Q_N,
                                                                       (unsigned long long *)h->d_epart);
     k_zero_u64<<<cdiv((size_t)9 * n, 256), 256, 0, st>>>((size_t)9 * n, h->d_fix);
     k_zero_u64<<<cdiv((size_t)6 * n, 256), 256, 0, st>>>((size_t)3 * n, (unsigned long long *)h->d_fs);
     k_zero_u64<<<cdiv((size_t)6 * n, 256), 256, 0, st>>>((size_t)3 * n, (unsigned long long *)h->d_fl);
-    k_pack<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, h->d_anchor, h->d_x, h->d_y, h->d_z, h->d_q, sqc,
+    k_pack<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, h->d_anchor, h->d_x, h->d_y, h->d_z, h->d_q,
                                               h->box, h->d_xq);
     if (!h->user_com)
         k_com<<<cdiv(h->nmol, 256), 256, 0, st>>>(h->nmol, h->d_mstart, h->d_mass, h->d_x, h->d_y, h->d_z, h->d_cx,
@@ -832,7 +841,10 @@ static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, doubl
     pa.nlist = h->d_nlist;
     pa.list_cap = h->list_cap;
     pa.xq = h->d_xq;
-    pa.info = h->d_info;
+    pa.aux = h->d_aux;
+    pa.q = h->d_q;
+    pa.coulf_f = (float)h->p.coulomb_factor;
+    pa.coulf = h->p.coulomb_factor;
     pa.lj = h->d_lj;
     pa.nlj = h->nlj;
     pa.clu_atom = h->d_clu_atom;
@@ -854,8 +866,10 @@ static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, doubl
     pa.fs_x = h->d_fs; pa.fs_y = h->d_fs + n; pa.fs_z = h->d_fs + 2 * n;
     pa.fl_x = h->d_fl; pa.fl_y = h->d_fl + n; pa.fl_z = h->d_fl + 2 * n;
     pa.epart = h->d_epart;
+    mark(1);
     if (h->p.ee_enabled) k_pairs<true><<<h->nclu, 64, 0, st>>>(pa);
     else k_pairs<false><<<h->nclu, 64, 0, st>>>(pa);
+    mark(2);
     unsigned long long *fixG = h->d_fix, *fixHL = h->d_fix + 3 * (size_t)n, *fixHB = h->d_fix + 6 * (size_t)n;
     int row = rows_pairs;
     if (h->nspecial) {
@@ -883,35 +897,40 @@ static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, doubl
         k_special<<<rows_sp, 256, 0, st>>>(sa);
         row += rows_sp;
     }
-    if (h->timing) hipEventRecord(h->ev[1], st);
+    mark(3);
     if (group != 1) {
         k_recip_prep<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_x, h->d_y, h->d_z, h->d_q, h->box, h->d_fq,
                                                     h->d_stepz);
+        mark(4);
         const dim3 gsk(cdiv(h->nrows, 64), h->nchunk);
         if (h->maxlen <= 16)
-            k_sk<16><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, n, h->chunk, h->d_fq, h->d_stepz, h->nkv, h->d_part);
+            k_sk<16><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, h->d_wavelen, n, h->chunk, h->d_fq, h->d_stepz,
+                                         h->nkv, h->d_part);
         else if (h->maxlen <= 32)
-            k_sk<32><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, n, h->chunk, h->d_fq, h->d_stepz, h->nkv, h->d_part);
+            k_sk<32><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, h->d_wavelen, n, h->chunk, h->d_fq, h->d_stepz,
+                                         h->nkv, h->d_part);
         else if (h->maxlen <= 64)
-            k_sk<64><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, n, h->chunk, h->d_fq, h->d_stepz, h->nkv, h->d_part);
+            k_sk<64><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, h->d_wavelen, n, h->chunk, h->d_fq, h->d_stepz,
+                                         h->nkv, h->d_part);
         else
-            k_sk<96><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, n, h->chunk, h->d_fq, h->d_stepz, h->nkv, h->d_part);
+            k_sk<96><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, h->d_wavelen, n, h->chunk, h->d_fq, h->d_stepz,
+                                         h->nkv, h->d_part);
+        mark(5);
         k_sk_reduce<<<rows_sk, 256, 0, st>>>(h->nkv, h->nchunk, h->d_part, h->d_kidx, h->box, h->p.alpha,
                                              h->p.coulomb_factor, h->d_ak, h->d_sk, h->d_epart + (size_t)row * Q_N);
         row += rows_sk;
-        if (h->timing) hipEventRecord(h->ev[2], st);
+        mark(6);
         const dim3 gf(cdiv(n, 64), h->nslice);
         k_recip_force<<<gf, 64, 0, st>>>(n, h->nrows, h->rows_per_slice, h->d_rows, h->d_rowoff, h->d_fq, h->d_stepz,
                                          h->d_ak, h->box, h->d_rpart);
-        if (h->timing) hipEventRecord(h->ev[3], st);
+        mark(7);
         ScatterArgs sc{h->d_x, h->d_y, h->d_z, h->box, h->fscale, fixG, fixG + n, fixG + 2 * n,
                        h->d_epart + (size_t)row * Q_N};
         if (h->neterm) k_eterms<<<rows_et, 256, 0, st>>>(h->neterm, h->d_et_pairs, h->d_moltype, h->d_q,
                                                          h->p.coulomb_factor, h->p.alpha, sc);
         row += rows_et;
-    } else if (h->timing) {
-        hipEventRecord(h->ev[2], st);
-        hipEventRecord(h->ev[3], st);
+    } else {
+        mark(4); mark(5); mark(6); mark(7);
     }
     if (group != 2) {
         ScatterArgs sc{h->d_x, h->d_y, h->d_z, h->box, h->fscale, fixHB, fixHB + n, fixHB + 2 * n, nullptr};
@@ -931,7 +950,7 @@ static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, doubl
             row += rows_t;
         }
     }
-    if (h->timing) hipEventRecord(h->ev[4], st);
+    mark(8);
     CombineArgs ca;
     ca.n = n;
     ca.group = group;
@@ -953,7 +972,7 @@ static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, doubl
     k_combine<<<rows_c, 256, 0, st>>>(ca);
     row += rows_c;
     k_ereduce<<<Q_N, 256, 0, st>>>(row, h->d_epart, h->d_energy);
-    if (h->timing) hipEventRecord(h->ev[5], st);
+    mark(9);
     HIPCHK(h, hipGetLastError());
     if (e) {
         double q[Q_N];
@@ -973,13 +992,12 @@ static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, doubl
         for (int t = 0; t < h->ntypes && group != 1; ++t) e->spee[t] = h->spee_self[t];
     }
     if (h->timing) {
-        HIPCHK(h, hipEventSynchronize(h->ev[5]));
-        float ms;
-        hipEventElapsedTime(&ms, h->ev[0], h->ev[1]); h->stage_ms[ST_PAIRS] = ms;
-        hipEventElapsedTime(&ms, h->ev[1], h->ev[2]); h->stage_ms[ST_SK] = ms;
-        hipEventElapsedTime(&ms, h->ev[2], h->ev[3]); h->stage_ms[ST_RFORCE] = ms;
-        hipEventElapsedTime(&ms, h->ev[3], h->ev[4]); h->stage_ms[ST_BONDED] = ms;
-        hipEventElapsedTime(&ms, h->ev[4], h->ev[5]); h->stage_ms[ST_REDUCE] = ms;
+        HIPCHK(h, hipEventSynchronize(h->ev[9]));
+        for (int k = ST_PREP; k <= ST_REDUCE; ++k) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, h->ev[k - 1], h->ev[k]);
+            h->stage_ms[k] = ms;
+        }
     }
     return MDX_OK;
 }

@@ -25,8 +25,11 @@ struct PairArgs {
     const PairEntry *list;
     const int *nlist;
     int list_cap;
-    const float4 *xq;            // per slot: x, y, z relative to anchor; q * sqrt(coulf)
-    const int *info;             // per slot: LJ type | ee << 16
+    const float4 *xq;            // per slot: x, y, z relative to anchor; q (FP32)
+    const int2 *aux;             // per slot: LJ type | ee << 16; q - (float)q as float bits
+    const double *q;             // per atom charge (FP64)
+    float coulf_f;               // coulomb factor (FP32, forces)
+    double coulf;                // coulomb factor (FP64, energies)
     const float2 *lj;            // nlj x nlj: (A6, B12)
     int nlj;
     const int *clu_atom;
@@ -53,8 +56,9 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
     __syncthreads();
 
     const float4 pi = a.xq[ci * kClu + il];
-    const int infoi = a.info[ci * kClu + il];
+    const int infoi = a.aux[ci * kClu + il].x;
     const int ti = infoi & 0xffff;
+    const float qic = pi.w * a.coulf_f;
     const float2 *ljrow = s_lj + ti * a.nlj;
     const int jl = lane & 7;
     const float A1 = (float)kA1, A2 = (float)kA2, A3 = (float)kA3, A4 = (float)kA4, A5 = (float)kA5;
@@ -81,7 +85,8 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
             else if (a.group == 2) m &= ~e.mshort;
             if (m == 0ull) continue;
             const float4 pj = a.xq[e.cj * kClu + jl];
-            const int infoj = a.info[e.cj * kClu + jl];
+            const int2 auxj = a.aux[e.cj * kClu + jl];
+            const int infoj = auxj.x;
             const bool on = (m >> lane) & 1ull;
             const bool isshort = (e.mshort >> lane) & 1ull;
             float dx = (pi.x - pj.x) + e.sx;
@@ -105,8 +110,11 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
             const float t = __builtin_amdgcn_rcpf(1.0f + B1 * ar);
             const float g = __builtin_amdgcn_exp2f(nal2 * r2);
             const float erfct = ((((A5 * t + A4) * t + A3) * t + A2) * t + A1) * t * g;
-            const float ee1 = pi.w * pj.w * rei;
-            const float ees = ee1 * erfct;
+            const float ee1 = qic * pj.w * rei;
+            // energy per unit q_i C with the exact q_j = hi + lo (the i side
+            // is applied in FP64 at the end): no systematic charge rounding
+            const float phi = erfct * rei;
+            const float ees = pj.w * phi + __int_as_float(auxj.y) * phi;
             const float2 lj = ljrow[infoj & 0xffff];
             const float r2i = rei * rei;
             const float r6i = r2i * r2i * r2i;
@@ -139,6 +147,9 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
             Fl[q] += __shfl_xor(Fl[q], off);
         }
     const int ai = a.clu_atom[ci * kClu + il];
+    const double qc = ai >= 0 ? a.q[ai] * a.coulf : 0.0;
+    E[0] *= qc;
+    E[1] *= qc;
     if (jl == 0 && ai >= 0) {
         if (a.fs_x) { a.fs_x[ai] = Fs[0]; a.fs_y[ai] = Fs[1]; a.fs_z[ai] = Fs[2]; }
         if (a.fl_x) { a.fl_x[ai] = Fl[0]; a.fl_y[ai] = Fl[1]; a.fl_z[ai] = Fl[2]; }
@@ -170,7 +181,7 @@ template __global__ void k_pairs<true>(PairArgs);
 // Anchor-relative packed coordinates of every cluster slot (each step).
 __global__ void k_pack(int nslot, const int *__restrict__ clu_atom, const double *__restrict__ anchor,
                        const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-                       const double *__restrict__ q, double sqrt_coulf, Box b, float4 *__restrict__ xq) {
+                       const double *__restrict__ q, Box b, float4 *__restrict__ xq) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslot) return;
     const int a = clu_atom[s];
@@ -179,19 +190,25 @@ __global__ void k_pack(int nslot, const int *__restrict__ clu_atom, const double
     if (a >= 0) {
         double d[3] = {x[a] - anchor[c * 3], y[a] - anchor[c * 3 + 1], z[a] - anchor[c * 3 + 2]};
         for (int k = 0; k < 3; ++k) d[k] -= b.l[k] * rint(d[k] / b.l[k]);
-        o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)(q[a] * sqrt_coulf));
+        o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)q[a]);
     }
     xq[s] = o;
 }
 
-// Per-slot LJ type and ee flag (at list build).
+// Per-slot LJ type, ee flag and the low part of the charge (at list build).
 __global__ void k_slot_info(int nslot, const int *__restrict__ clu_atom, const int *__restrict__ atom_ljtype,
                             const int *__restrict__ atom_moltype, const int *__restrict__ type_ee,
-                            int *__restrict__ info) {
+                            const double *__restrict__ q, int2 *__restrict__ aux) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslot) return;
     const int a = clu_atom[s];
-    info[s] = a >= 0 ? (atom_ljtype[a] | (type_ee[atom_moltype[a]] << 16)) : 0;
+    int2 v = make_int2(0, 0);
+    if (a >= 0) {
+        const float hi = (float)q[a];
+        v.x = atom_ljtype[a] | (type_ee[atom_moltype[a]] << 16);
+        v.y = __float_as_int((float)(q[a] - (double)hi));
+    }
+    aux[s] = v;
 }
 
 struct SpecialArgs {

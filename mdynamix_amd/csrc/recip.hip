@@ -6,9 +6,9 @@
 //
 // k_sk: lane = row, atoms of one chunk streamed with uniform (scalar) loads.
 //   For each atom the phase 2 pi (n_x f_x + n_y f_y + n_z0 f_z) is formed
-//   in FP64 fractional coordinates and reduced to one turn before an FP32
-//   sincospi; the remaining terms of the row follow by the complex
-//   recurrence P <- P exp(2 pi i f_z).  Per-chunk FP32 sums are written as
+//   in FP64 fractional coordinates and reduced to one turn before sincospi;
+//   the remaining terms of the row follow by the complex recurrence
+//   P <- P exp(2 pi i f_z), re-anchored in FP64 every 8 terms.  Per-chunk FP32 sums are written as
 //   partials and added in FP64 by k_sk_reduce, which also forms the energy
 //   QPE, the virial projections and A_k = 2 AKC S(k) for the force pass.
 // k_recip_force: lane = atom, rows streamed uniformly; the force on an atom
@@ -24,32 +24,44 @@ struct RecipRow {           // 16 bytes
 
 __global__ void k_recip_prep(int n, const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, const double *__restrict__ q, Box b,
-                             double4 *__restrict__ fq, float2 *__restrict__ stepz) {
+                             double4 *__restrict__ fq, double4 *__restrict__ stepz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double fx = x[i] / b.l[0], fy = y[i] / b.l[1], fz = z[i] / b.l[2];
     fq[i] = make_double4(fx, fy, fz, q[i]);
-    double u = fz - rint(fz);
-    float s, c;
-    sincospif((float)(2.0 * u), &s, &c);
-    stepz[i] = make_float2(c, s);
+    // exp(2 pi i f_z) and its 8th power, in FP64
+    const double u = fz - rint(fz);
+    double s1, c1, s8, c8;
+    sincospi(2.0 * u, &s1, &c1);
+    double u8 = 8.0 * u;
+    u8 -= rint(u8);
+    sincospi(2.0 * u8, &s8, &c8);
+    stepz[i] = make_double4(c1, s1, c8, s8);
 }
 
-__device__ inline void reduced_phase(const RecipRow &r, double4 a, float &c, float &s) {
+// exp(i 2 pi u) at the start of a row, u reduced to one turn in FP64.
+__device__ inline void row_start(const RecipRow &r, double4 a, double &c, double &s) {
     double u = (double)r.nx * a.x + (double)r.ny * a.y;
     u = u + (double)r.nz0 * a.z;
     u = u - rint(u);
-    sincospif((float)(2.0 * u), &s, &c);
+    float sf, cf;
+    sincospif((float)(2.0 * u), &sf, &cf);
+    c = (double)cf;
+    s = (double)sf;
 }
 
+// The row's terms are generated in blocks of 8: an FP64 anchor advanced by
+// exp(16 pi i f_z) and an FP32 recurrence inside the block, so rounding of
+// the step cannot accumulate over more than 7 terms.
 template <int MAXLEN>
 __global__ __launch_bounds__(64) void k_sk(int nrows, const RecipRow *__restrict__ rows,
-                                           const int *__restrict__ rowoff, int natoms, int chunk,
-                                           const double4 *__restrict__ fq, const float2 *__restrict__ stepz,
-                                           int nkv, float2 *__restrict__ part) {
+                                           const int *__restrict__ rowoff, const int *__restrict__ wavelen,
+                                           int natoms, int chunk, const double4 *__restrict__ fq,
+                                           const double4 *__restrict__ stepz, int nkv, float2 *__restrict__ part) {
     const int r = blockIdx.x * 64 + threadIdx.x;
     const int a0 = blockIdx.y * chunk;
     const int a1 = min(a0 + chunk, natoms);
+    const int wlen = wavelen[blockIdx.x];          // longest row of this wavefront
     RecipRow row = {0, 0, 0, 0};
     if (r < nrows) row = rows[r];
     float are[MAXLEN], aim[MAXLEN];
@@ -57,17 +69,26 @@ __global__ __launch_bounds__(64) void k_sk(int nrows, const RecipRow *__restrict
     for (int j = 0; j < MAXLEN; ++j) { are[j] = 0.f; aim[j] = 0.f; }
     for (int i = a0; i < a1; ++i) {
         const double4 a = fq[i];
-        const float2 st = stepz[i];
-        float c, s;
-        reduced_phase(row, a, c, s);
-        const float qf = (float)a.w;
-        float pr = qf * c, pim = qf * s;
+        const double4 st = stepz[i];
+        double s0, c0;
+        row_start(row, a, c0, s0);
+        double Ar = a.w * c0, Ai = a.w * s0;             // anchor, FP64
+        const float sr = (float)st.x, si = (float)st.y;
 #pragma unroll
-        for (int j = 0; j < MAXLEN; ++j) {
-            if (j < row.len) { are[j] += pr; aim[j] += pim; }
-            const float tr = pr * st.x - pim * st.y;
-            pim = pr * st.y + pim * st.x;
-            pr = tr;
+        for (int j0 = 0; j0 < MAXLEN; j0 += 8) {
+            if (j0 >= wlen) break;
+            float pr = (float)Ar, pim = (float)Ai;
+#pragma unroll
+            for (int j = j0; j < j0 + 8; ++j) {
+                are[j] += pr;
+                aim[j] += pim;
+                const float tr = pr * sr - pim * si;
+                pim = pr * si + pim * sr;
+                pr = tr;
+            }
+            const double tr = Ar * st.z - Ai * st.w;
+            Ai = Ar * st.w + Ai * st.z;
+            Ar = tr;
         }
     }
     if (r < nrows) {
@@ -79,10 +100,13 @@ __global__ __launch_bounds__(64) void k_sk(int nrows, const RecipRow *__restrict
     }
 }
 
-template __global__ void k_sk<16>(int, const RecipRow *, const int *, int, int, const double4 *, const float2 *, int, float2 *);
-template __global__ void k_sk<32>(int, const RecipRow *, const int *, int, int, const double4 *, const float2 *, int, float2 *);
-template __global__ void k_sk<64>(int, const RecipRow *, const int *, int, int, const double4 *, const float2 *, int, float2 *);
-template __global__ void k_sk<96>(int, const RecipRow *, const int *, int, int, const double4 *, const float2 *, int, float2 *);
+#define MDX_SK_INST(L) \
+    template __global__ void k_sk<L>(int, const RecipRow *, const int *, const int *, int, int, const double4 *, \
+                                     const double4 *, int, float2 *);
+MDX_SK_INST(16)
+MDX_SK_INST(32)
+MDX_SK_INST(64)
+MDX_SK_INST(96)
 
 // Sum chunk partials in FP64 (fixed order), energy and virial terms of each
 // k (forces.f:1317-1331) and A_k = 2 AKC S(k).  kidx holds (nx, ny, nz) of
@@ -137,36 +161,44 @@ __global__ __launch_bounds__(256) void k_sk_reduce(int nkv, int nchunk, const fl
 }
 
 // Reciprocal forces; blockIdx.y selects a slice of rows, each slice writes
-// its own partial force array (summed by the combine kernel).
+// its own partial force array (summed by the combine kernel).  Same
+// anchored recurrence as k_sk.
 __global__ __launch_bounds__(64) void k_recip_force(int natoms, int nrows, int rows_per_slice,
                                                     const RecipRow *__restrict__ rows, const int *__restrict__ rowoff,
-                                                    const double4 *__restrict__ fq, const float2 *__restrict__ stepz,
+                                                    const double4 *__restrict__ fq, const double4 *__restrict__ stepz,
                                                     const float2 *__restrict__ ak, Box b,
                                                     double *__restrict__ fpart) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int r0 = blockIdx.y * rows_per_slice;
     const int r1 = min(r0 + rows_per_slice, nrows);
     double4 a = make_double4(0, 0, 0, 0);
-    float2 st = make_float2(1.f, 0.f);
+    double4 st = make_double4(1.0, 0.0, 1.0, 0.0);
     if (i < natoms) { a = fq[i]; st = stepz[i]; }
-    const float qf = (float)a.w;
+    const float sr = (float)st.x, si = (float)st.y;
     const double tx = 2.0 * kPI / b.l[0], ty = 2.0 * kPI / b.l[1], tz = 2.0 * kPI / b.l[2];
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
     for (int r = r0; r < r1; ++r) {
         const RecipRow row = rows[r];
         const float2 *A = ak + rowoff[r];
-        float c, s;
-        reduced_phase(row, a, c, s);
-        float wr = qf * c, wi = qf * s;
+        double s0, c0;
+        row_start(row, a, c0, s0);
+        double Wr = a.w * c0, Wi = a.w * s0;
         float ts = 0.f, tzs = 0.f;
-        for (int j = 0; j < row.len; ++j) {
-            const float2 Ak = A[j];
-            const float t = wi * Ak.x - wr * Ak.y;
-            ts += t;
-            tzs += (float)j * t;
-            const float nr = wr * st.x - wi * st.y;
-            wi = wr * st.y + wi * st.x;
-            wr = nr;
+        for (int j0 = 0; j0 < row.len; j0 += 8) {
+            float wr = (float)Wr, wi = (float)Wi;
+            const int j1 = min(j0 + 8, row.len);
+            for (int j = j0; j < j1; ++j) {
+                const float2 Ak = A[j];
+                const float t = wi * Ak.x - wr * Ak.y;
+                ts += t;
+                tzs += (float)j * t;
+                const float nr = wr * sr - wi * si;
+                wi = wr * si + wi * sr;
+                wr = nr;
+            }
+            const double tr = Wr * st.z - Wi * st.w;
+            Wi = Wr * st.w + Wi * st.z;
+            Wr = tr;
         }
         Fx += tx * row.nx * (double)ts;
         Fy += ty * row.ny * (double)ts;

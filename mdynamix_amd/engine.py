@@ -68,7 +68,8 @@ def load_library() -> C.CDLL:
     return L
 
 
-STAGES = ("list", "pairs", "recip_sk", "recip_force", "bonded", "reduce")
+STAGES = ("list", "prep", "pairs", "special", "recip_prep", "recip_sk", "recip_sk_reduce", "recip_force",
+          "bonded", "reduce")
 
 
 class ForceEngine:
@@ -175,8 +176,9 @@ class ForceEngine:
     def compute_device(self, fx_ptr: int, fy_ptr: int, fz_ptr: int, group: int = abi.MDX_FORCES_ALL,
                        energies: bool = False):
         e = MdxEnergies() if energies else None
-        self._check(self.lib.mdx_compute_device(self._h, group, 0, C.c_void_p(fx_ptr), C.c_void_p(fy_ptr),
-                                                C.c_void_p(fz_ptr), C.byref(e) if e is not None else None))
+        ptr = lambda p: C.c_void_p(p) if p else None
+        self._check(self.lib.mdx_compute_device(self._h, group, 0, ptr(fx_ptr), ptr(fy_ptr), ptr(fz_ptr),
+                                                C.byref(e) if e is not None else None))
         return e.as_dict(self.topo.nmoltypes, self.topo.ntypepairs) if e is not None else None
 
     def stage_times(self) -> Dict[str, float]:

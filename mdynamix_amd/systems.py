@@ -152,20 +152,20 @@ class System:
             return self.ewald_a / self.rcut
         return 2.0 * self.ewald_a / cl
 
-    def write_reference_inputs(self, directory: str, verbose: int = 5) -> None:
+    def write_reference_inputs(self, directory: str, verbose: int = 5, nsteps: int = 1) -> None:
         """Write md.in, the .mmol files and the XMOL start file."""
         os.makedirs(directory, exist_ok=True)
         for t in self.types:
             with open(os.path.join(directory, t.name + ".mmol"), "w") as f:
                 f.write(t.to_mmol())
         lines = ["Main_filename %s" % self.name, "Verbose_level %d" % verbose,
-                 "Path_DB .", "Read_restart no", "Dump_restart 0", "Check_only no",
+                 "Path_DB .", "Read_restart no", "Dump_restart 1000000", "Check_only no",
                  "Molecule_types %d" % len(self.types)]
         for t, c in zip(self.types, self.counts):
             lines.append("%s %d" % (t.name, c))
         lines += ["Box %.12f %.12f %.12f" % self.box, "PBC rect",
                   "Nose_thermostat %.3f 30." % self.temperature,
-                  "Time_step %.6f" % self.dt_fs, "Number_steps 1"]
+                  "Time_step %.6f" % self.dt_fs, "Number_steps %d" % nsteps]
         if self.shake:
             lines.append("Constraints 1.d-6")
         else:
@@ -173,7 +173,7 @@ class System:
         lines += ["R_cutoff %.10f" % self.rcut, "R_short %.10f" % self.rshort,
                   "Neighbour_list %d" % self.rebuild_every,
                   "Electrostatics Ewald %.12f %.12f" % (self.ewald_a, self.ewald_b),
-                  "Startup XMOL", "Output 1"]
+                  "Startup XMOL", "Output 1", "COM_check yes 1000000"]
         if self.ee_type is not None:
             # two sub-ensembles: the job's lambda (started in) and lambda=1
             lines += ["MDEE 2 list", "%.12f 0.0" % self.ee_lambda, "1.0 0.0",
@@ -385,7 +385,22 @@ def chain_mixture(n_chain: int, n_water: int, box: float, seed: int, name: str =
     loc = chain.local_coords()
     chain_atoms = []
     for c in cc:
-        chain_atoms.append(loc @ random_rotation(rng).T + c)
+        # orientation with no chain atom closer than 3.2 A to those placed
+        best, best_d = None, -1.0
+        for _ in range(60):
+            cand_atoms = loc @ random_rotation(rng).T + c
+            if not chain_atoms:
+                best = cand_atoms
+                break
+            prev = np.concatenate(chain_atoms)
+            d = cand_atoms[:, None, :] - prev[None, :, :]
+            d -= L * np.round(d / L)
+            dmin = np.sqrt((d ** 2).sum(-1).min())
+            if dmin > best_d:
+                best, best_d = cand_atoms, dmin
+            if dmin > 3.2:
+                break
+        chain_atoms.append(best)
     ca = np.concatenate(chain_atoms) if chain_atoms else np.zeros((0, 3))
     # candidate water sites: dense lattice minus those near chains
     cand = lattice_sites(int(n_water * 1.6) + 64, L, rng, jitter=0.08)
@@ -447,19 +462,23 @@ def ee_job(n_atoms_target: int, n_solute: int, lam: float, seed: int) -> System:
     box = max(box, 2.0 * 10.0 + 1.0)
     sol = lj_solute(n_solute, seed + 11)
     centres = lattice_sites(nmol, (box, box, box), rng)
-    # the solute takes the site nearest the box centre; waters avoid it
+    # the solute takes the site nearest the box centre; water and ion sites
+    # closer than 3.4 A to any solute atom are dropped
     k0 = int(np.argmin((centres ** 2).sum(1)))
     sc = centres[k0]
+    rot = random_rotation(rng)
+    sol_atoms = sol.local_coords() @ rot.T + sc
     others = np.delete(centres, k0, axis=0)
-    d = others - sc
+    d = others[:, None, :] - sol_atoms[None, :, :]
     d -= box * np.round(d / box)
-    far = np.sqrt((d ** 2).sum(1)) > 4.0
+    far = np.sqrt((d ** 2).sum(-1)).min(1) > 3.4
     others = others[far]
     n_w = min(n_w, len(others) - 2 * n_ion)
     others = others[rng.permutation(len(others))[:n_w + 2 * n_ion]]
     centres = np.concatenate([others[:n_w], others[n_w:n_w + n_ion], others[n_w + n_ion:], sc[None]])
     types = [spce_water(), sodium(), chloride(), sol]
     counts = [n_w, n_ion, n_ion, 1]
-    return assemble("ee%d" % seed, types, counts, centres, rng, box=(box, box, box), seed=seed,
-                    rcut=10.0, rshort=5.0, ewald_a=2.8, ewald_b=9.0, shake=True, skin=1.0,
-                    ee_type=3, ee_lambda=lam)
+    sysm = assemble("ee%d" % seed, types[:3], counts[:3], centres[:-1], rng, box=(box, box, box), seed=seed,
+                    rcut=10.0, rshort=5.0, ewald_a=2.8, ewald_b=9.0, shake=True, skin=1.0)
+    return dataclasses.replace(sysm, types=types, counts=counts, pos=np.concatenate([sysm.pos, sol_atoms]),
+                               ee_type=3, ee_lambda=lam)

@@ -10,11 +10,12 @@
  * with -ffp-contract=off, so every product and sum rounds as on an x86-64
  * gfortran build), the following reference routines:
  *
- *   orc_chcnb     CHCNB    forces.f:1813-2205  (rectangular cell, Ewald)
+ *   orc_chcnb     CHCNB    forces.f:1813-2205  (rectangular cell, Ewald);
+ *                 orc_chcnb_task is one rank's stripe of it
  *   orc_localf    LOCALF   forces.f:813-998
  *   orc_forces    FORCES   forces.f:1014-1202
  *   orc_inifur    INIFUR   forces.f:1356-1449
- *   orc_furir     FURIR    forces.f:1217-1339
+ *   orc_furir     FURIR    forces.f:1217-1339 (orc_furir_task: one rank's k stripe)
  *   orc_eterms    ETERMS   forces.f:1482-1676
  *   orc_getbnd    GETBND   forces.f:113-204
  *   orc_getang    GETANG   forces.f:216-309
@@ -101,12 +102,15 @@ static int64_t chcnb_visit(const mdx_topology *t, const mdx_params *p, const dou
     return 0;
 }
 
-int orc_chcnb(const mdx_topology *t, const mdx_params *p, const double *x, const double *y, const double *z,
-              int32_t *nbs1, int32_t *nbs2, int64_t cap_s, int64_t *mbsh,
-              int32_t *nbl1, int32_t *nbl2, int64_t cap_l, int64_t *mbln) {
+/* Task-striped variant: outer atoms I = NUMTASK-TASKID, +NUMTASK, ...
+ * (1-based; forces.f:1825-1830), i.e. the share of one replicated-data
+ * rank.  numtask = 1 is the serial CHCNB. */
+int orc_chcnb_task(const mdx_topology *t, const mdx_params *p, const double *x, const double *y, const double *z,
+                   int taskid, int numtask, int32_t *nbs1, int32_t *nbs2, int64_t cap_s, int64_t *mbsh,
+                   int32_t *nbl1, int32_t *nbl2, int64_t cap_l, int64_t *mbln) {
     int64_t ms = 0, ml = 0;
     const int n = t->natoms;
-    for (int i = 0; i < n; ++i) {
+    for (int i = numtask - taskid - 1; i < n; i += numtask) {
         for (int j = i - 2; j >= 0; j -= 2)
             chcnb_visit(t, p, x, y, z, i, j, 0, nbs1, nbs2, cap_s, &ms, nbl1, nbl2, cap_l, &ml);
         for (int j = i + 1; j < n; j += 2)
@@ -115,6 +119,12 @@ int orc_chcnb(const mdx_topology *t, const mdx_params *p, const double *x, const
     *mbsh = ms;
     *mbln = ml;
     return (ms > cap_s || ml > cap_l) ? MDX_ERR_LIST_OVERFLOW : MDX_OK;
+}
+
+int orc_chcnb(const mdx_topology *t, const mdx_params *p, const double *x, const double *y, const double *z,
+              int32_t *nbs1, int32_t *nbs2, int64_t cap_s, int64_t *mbsh,
+              int32_t *nbl1, int32_t *nbl2, int64_t cap_l, int64_t *mbln) {
+    return orc_chcnb_task(t, p, x, y, z, 0, 1, nbs1, nbs2, cap_s, mbsh, nbl1, nbl2, cap_l, mbln);
 }
 
 /* ------------------------------------------------------------------ */
@@ -350,9 +360,9 @@ int64_t orc_inifur(const mdx_topology *t, const mdx_params *p, double *rkx, doub
  * (forces.f:1266).  Pinned by: reference GX/PELS1 (golden dumps), the
  * rock-salt Madelung constant and alpha-independence of the total Ewald
  * energy (tests).                                                       */
-void orc_furir(const mdx_topology *t, const mdx_params *p, const double *x, const double *y,
-               const double *z, const double *rkx, const double *rky, const double *rkz, int64_t nkv,
-               double *gx, double *gy, double *gz, mdx_energies *e) {
+void orc_furir_task(const mdx_topology *t, const mdx_params *p, const double *x, const double *y,
+                    const double *z, const double *rkx, const double *rky, const double *rkz, int64_t nkv,
+                    int taskid, int numtask, double *gx, double *gy, double *gz, mdx_energies *e) {
     if (nkv == 0 || t->nmols == 1) return;
     const int n = t->natoms;
     double *ts = (double *)malloc(sizeof(double) * n), *tc = (double *)malloc(sizeof(double) * n);
@@ -360,7 +370,7 @@ void orc_furir(const mdx_topology *t, const mdx_params *p, const double *x, cons
     const double vol = p->box[0] * p->box[1] * p->box[2];
     const double cfur = 4.0 * PI_R * p->coulomb_factor / vol;
     double qpe = 0.0;
-    for (int64_t k = 0; k < nkv; ++k) {
+    for (int64_t k = numtask - taskid - 1; k < nkv; k += numtask) {
         const double rxv = rkx[k] / p->box[0], ryv = rky[k] / p->box[1], rzv = rkz[k] / p->box[2];
         double rk2 = rxv * rxv + ryv * ryv;
         rk2 = rk2 + rzv * rzv;
@@ -389,6 +399,12 @@ void orc_furir(const mdx_topology *t, const mdx_params *p, const double *x, cons
     e->virx += qpe; e->viry += qpe; e->virz += qpe;
     e->pels1 += qpe;
     free(ts); free(tc);
+}
+
+void orc_furir(const mdx_topology *t, const mdx_params *p, const double *x, const double *y,
+               const double *z, const double *rkx, const double *rky, const double *rkz, int64_t nkv,
+               double *gx, double *gy, double *gz, mdx_energies *e) {
+    orc_furir_task(t, p, x, y, z, rkx, rky, rkz, nkv, 0, 1, gx, gy, gz, e);
 }
 
 /* One exclusion correction pair of ETERMS (forces.f:1554-1577). */

@@ -39,6 +39,18 @@ def lib():
         pi64 = C.POINTER(C.c_int64)
         L.orc_chcnb.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, pi, pi, C.c_int64, pi64, pi, pi, C.c_int64, pi64]
         L.orc_chcnb.restype = C.c_int
+        L.orc_chcnb_task.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, C.c_int, C.c_int, pi, pi, C.c_int64, pi64,
+                                     pi, pi, C.c_int64, pi64]
+        L.orc_chcnb_task.restype = C.c_int
+        L.orc_furir_task.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, pd, pd, pd, C.c_int64, C.c_int, C.c_int,
+                                     pd, pd, pd, C.c_void_p]
+        L.orc_furir_task.restype = None
+        L.orc_localf.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, pi, pi, C.c_int64, pd, pd, pd, C.c_void_p]
+        L.orc_localf.restype = None
+        L.orc_forces.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, pi, pi, C.c_int64, pd, pd, pd, C.c_void_p]
+        L.orc_forces.restype = None
+        L.orc_eterms.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, pd, C.c_double, pd, pd, pd, C.c_void_p]
+        L.orc_eterms.restype = None
         L.orc_inifur.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, C.c_int64, pd, pd]
         L.orc_inifur.restype = C.c_int64
         L.orc_compute.argtypes = [C.c_void_p, C.c_void_p, pd, pd, pd, pd, pd, pd, pi, pi, C.c_int64, pi, pi,
@@ -54,8 +66,9 @@ def _xyz(pos):
     return [np.ascontiguousarray(pos[:, k]) for k in range(3)]
 
 
-def chcnb(topo_struct, params_struct, pos, cap_short=None, cap_long=None):
-    """Reference-order lists (1-based signed) as four int32 arrays."""
+def chcnb(topo_struct, params_struct, pos, cap_short=None, cap_long=None, taskid=0, numtask=1):
+    """Reference-order lists (1-based signed) as four int32 arrays; with
+    numtask > 1 only the stripe of rank `taskid` (forces.f:1825-1830)."""
     n = pos.shape[0]
     cap_s = cap_short or max(1024, 64 * n)
     cap_l = cap_long or max(4096, 1200 * n)
@@ -63,8 +76,8 @@ def chcnb(topo_struct, params_struct, pos, cap_short=None, cap_long=None):
     s1 = np.zeros(cap_s, np.int32); s2 = np.zeros(cap_s, np.int32)
     l1 = np.zeros(cap_l, np.int32); l2 = np.zeros(cap_l, np.int32)
     ms, ml = C.c_int64(0), C.c_int64(0)
-    st = lib().orc_chcnb(C.byref(topo_struct), C.byref(params_struct), as_ptr(x), as_ptr(y), as_ptr(z),
-                         as_ptr(s1, C.c_int32), as_ptr(s2, C.c_int32), cap_s, C.byref(ms),
+    st = lib().orc_chcnb_task(C.byref(topo_struct), C.byref(params_struct), as_ptr(x), as_ptr(y), as_ptr(z),
+                         taskid, numtask, as_ptr(s1, C.c_int32), as_ptr(s2, C.c_int32), cap_s, C.byref(ms),
                          as_ptr(l1, C.c_int32), as_ptr(l2, C.c_int32), cap_l, C.byref(ml))
     if st != 0:
         raise RuntimeError("orc_chcnb overflow: %d %d" % (ms.value, ml.value))
