@@ -175,6 +175,9 @@ typedef struct mdx_energies {
     double pes141[MDX_MAX_MOLTYPES];
     double psr141[MDX_MAX_MOLTYPES];
     double spee[MDX_MAX_MOLTYPES];
+    /* path-integral batches (mdx_batch_set_ring): spring energy QEK and
+     * spring virial VIRPI of this bead (PIFORCES, pimd.f:321-394) */
+    double qek, virpi;
 } mdx_energies;
 
 typedef struct mdx_engine mdx_engine;
@@ -221,6 +224,39 @@ mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want,
 /* Same call with device output arrays of natoms (forces stay in HBM). */
 mdx_status mdx_compute_device(mdx_engine *h, int32_t group, int32_t want,
                               double *dfx, double *dfy, double *dfz, mdx_energies *e);
+
+/* Forces of the last call (F, G, H) to caller-owned host arrays (any may be
+ * NULL), e.g. after a batched call. */
+mdx_status mdx_get_forces(mdx_engine *h, double *fx, double *fy, double *fz, double *gx, double *gy, double *gz,
+                          double *hx, double *hy, double *hz);
+/* Per-stage HIP-event timing of mdx_compute (on by default; off replays the
+ * call from a captured hipGraph). */
+mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
+/* Device memory held by a handle, in bytes (admission by HBM budget). */
+mdx_status mdx_memory_bytes(const mdx_engine *h, int64_t *bytes);
+
+/*
+ * Batches: K systems (path-integral beads, expanded-ensemble lambda states,
+ * independent jobs) evaluated by one set of fused launches per force call:
+ * every kernel covers all K systems through a block map, and the sequence
+ * is captured once in a hipGraph and replayed.  Systems keep their own
+ * lists (mdx_build_list per system); a system may be in one batch at a time
+ * besides its own mdx_compute.  No counterpart in the reference, which runs
+ * one bead per MPI rank (pimd.f:18, input.f:651-659).
+ */
+typedef struct mdx_batch mdx_batch;
+mdx_status mdx_batch_create(mdx_engine *const *systems, int32_t k, mdx_batch **out);
+/* Path-integral ring over the K systems (beads of one system): harmonic
+ * springs BEADFAC * m * |r_k - r_k+-1|^2 to both neighbours (PIFORCES,
+ * pimd.f:321-394) added to the fast forces, physical forces scaled by 1/K
+ * (pimd.f:61-69). */
+mdx_status mdx_batch_set_ring(mdx_batch *b, double beadfac);
+mdx_status mdx_batch_use_graph(mdx_batch *b, int32_t on);
+/* One force call of group MDX_FORCES_* for every system; e: K entries or
+ * NULL.  Forces stay in HBM; fetch them with mdx_get_forces. */
+mdx_status mdx_batch_compute(mdx_batch *b, int32_t group, int32_t want, mdx_energies *e);
+const char *mdx_batch_last_error(const mdx_batch *b);
+mdx_status mdx_batch_destroy(mdx_batch *b);
 
 /* Timing of the last mdx_build_list and mdx_compute per stage, in ms (HIP
  * events on the handle's stream): list build, prep (pack + COM), pairs,

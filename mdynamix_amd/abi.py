@@ -79,6 +79,7 @@ class MdxEnergies(C.Structure):
         ("pote1", C.c_double * MDX_MAX_TYPEPAIRS), ("potl1", C.c_double * MDX_MAX_TYPEPAIRS),
         ("pes141", C.c_double * MDX_MAX_MOLTYPES), ("psr141", C.c_double * MDX_MAX_MOLTYPES),
         ("spee", C.c_double * MDX_MAX_MOLTYPES),
+        ("qek", C.c_double), ("virpi", C.c_double),
     ]
 
     SCALARS = ("pe1", "pe2", "pels1", "pels2", "spe", "pint", "vir1", "vir2", "virb", "virx",
@@ -86,6 +87,8 @@ class MdxEnergies(C.Structure):
 
     def as_dict(self, nmoltypes: int = MDX_MAX_MOLTYPES, ntypepairs: int = MDX_MAX_TYPEPAIRS) -> Dict:
         d = {k: float(getattr(self, k)) for k in self.SCALARS}
+        d["qek"] = float(self.qek)
+        d["virpi"] = float(self.virpi)
         d["npairs_short"] = int(self.npairs_short)
         d["npairs_long"] = int(self.npairs_long)
         d["pote1"] = np.array(self.pote1[:ntypepairs])

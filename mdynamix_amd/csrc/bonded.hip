@@ -9,7 +9,7 @@
 // fixed tree order.
 // k_combine  F = G + H (CM_ADDLA, scalar.f:82-116) and the molecular virial
 //            sums WIRS/WIRSS (scalar.f:72-76, forces.f:991-995)
-// k_com      molecular centres of mass (GETCOM, service.f:471-500)
+// k_springs  path-integral bead springs (PIFORCES, pimd.f:321-394)
 // k_ereduce  final FP64 sum of all per-block partials
 #include "mdx_internal.h"
 
@@ -36,23 +36,16 @@ __device__ inline void zero_slots(double *o) {
     __syncthreads();
 }
 
-struct ScatterArgs {
-    const double *x, *y, *z;
-    Box box;
-    double fscale;
-    unsigned long long *fx, *fy, *fz;
-    double *epart;
-};
 
-// pairs: (i, j) with i > j; tp: molecule type of i
-__global__ __launch_bounds__(256) void k_eterms(int np, const int2 *__restrict__ pairs, const int *__restrict__ atom_moltype,
-                                                const double *__restrict__ q, double coulf, double alpha,
-                                                ScatterArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// pairs: (i, j) with i > j
+__device__ inline void eterms_body(const EtermArgs &t, int blk) {
+    const ScatterArgs &a = t.s;
+    const int k = blk * blockDim.x + threadIdx.x;
     double v[4] = {0, 0, 0, 0};
-    if (k < np) {
-        const int isp = pairs[k].x, jsp = pairs[k].y;
-        const double qij = coulf * q[isp] * q[jsp];
+    const double alpha = t.alpha;
+    if (k < t.np) {
+        const int isp = t.pairs[k].x, jsp = t.pairs[k].y;
+        const double qij = t.coulf * t.q[isp] * t.q[jsp];
         double dx = a.x[isp] - a.x[jsp], dy = a.y[isp] - a.y[jsp], dz = a.z[isp] - a.z[jsp];
         min_image(dx, dy, dz, a.box);
         double r2 = dx * dx + dy * dy;
@@ -72,21 +65,16 @@ __global__ __launch_bounds__(256) void k_eterms(int np, const int2 *__restrict__
         fx_add(a.fz + jsp, -fes * dz, a.fscale);
         v[1] = fes * (dx * dx); v[2] = fes * (dy * dy); v[3] = fes * (dz * dz);
     }
-    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
     const int slots[4] = {Q_SPE, Q_VIRX, Q_VIRY, Q_VIRZ};
     block_reduce_store<4>(v, slots, o);
 }
 
-struct BondArgs {
-    int n;
-    const int *bi, *bj, *bpar;
-    const int *morse;
-    const double *r0, *k, *d, *rho;
-};
 
-__global__ __launch_bounds__(256) void k_bonds(BondArgs t, ScatterArgs a) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void bonds_body(const BondArgs &t, int blk) {
+    const ScatterArgs &a = t.s;
+    const int n = blk * blockDim.x + threadIdx.x;
     double v[5] = {0, 0, 0, 0, 0};
     if (n < t.n) {
         const int i = t.bi[n], j = t.bj[n], m = t.bpar[n];
@@ -117,20 +105,16 @@ __global__ __launch_bounds__(256) void k_bonds(BondArgs t, ScatterArgs a) {
         fx_add(a.fx + j, bx * forb, a.fscale); fx_add(a.fy + j, by * forb, a.fscale);
         fx_add(a.fz + j, bz * forb, a.fscale);
     }
-    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
     const int slots[5] = {Q_PINT, Q_VIRB, Q_VIRFX, Q_VIRFY, Q_VIRFZ};
     block_reduce_store<5>(v, slots, o);
 }
 
-struct AngleArgs {
-    int n;
-    const int *ai, *aj, *ak, *apar;
-    const double *theta0, *k;
-};
 
-__global__ __launch_bounds__(256) void k_angles(AngleArgs t, ScatterArgs a) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void angles_body(const AngleArgs &t, int blk) {
+    const ScatterArgs &a = t.s;
+    const int n = blk * blockDim.x + threadIdx.x;
     double v[4] = {0, 0, 0, 0};
     if (n < t.n) {
         const int i = t.ai[n], j = t.aj[n], k = t.ak[n], m = t.apar[n];
@@ -168,21 +152,16 @@ __global__ __launch_bounds__(256) void k_angles(AngleArgs t, ScatterArgs a) {
         v[2] = -ay * fay - by * fby;
         v[3] = -az * faz - bz * fbz;
     }
-    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
     const int slots[4] = {Q_PINT, Q_VIRFX, Q_VIRFY, Q_VIRFZ};
     block_reduce_store<4>(v, slots, o);
 }
 
-struct TorsArgs {
-    int n;
-    const int *ti, *tj, *tk, *tl, *tpar;
-    const int *kind, *mult;
-    const double *k, *delta, *k1, *k2, *k3, *k4, *k5;
-};
 
-__global__ __launch_bounds__(256) void k_tors(TorsArgs t, ScatterArgs a) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void tors_body(const TorsArgs &t, int blk) {
+    const ScatterArgs &a = t.s;
+    const int n = blk * blockDim.x + threadIdx.x;
     double v[4] = {0, 0, 0, 0};
     if (n < t.n) {
         const int i = t.ti[n], j = t.tj[n], k = t.tk[n], l = t.tl[n], m = t.tpar[n];
@@ -261,71 +240,46 @@ __global__ __launch_bounds__(256) void k_tors(TorsArgs t, ScatterArgs a) {
             for (int d = 0; d < 3; ++d) v[1 + d] = -(A[d] + B[d]) * f[d][0] - B[d] * f[d][1] + Cc[d] * f[d][3];
         }
     }
-    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
     const int slots[4] = {Q_PINT, Q_VIRFX, Q_VIRFY, Q_VIRFZ};
     block_reduce_store<4>(v, slots, o);
 }
 
-// Molecular centres of mass: atoms of molecule m are [mstart[m], mstart[m+1]).
-__global__ void k_com(int nmol, const int *__restrict__ mstart, const double *__restrict__ mass,
-                      const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-                      double *__restrict__ cx, double *__restrict__ cy, double *__restrict__ cz) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nmol) return;
-    double sx = 0, sy = 0, sz = 0, sm = 0;
-    for (int i = mstart[m]; i < mstart[m + 1]; ++i) {
-        sx = sx + mass[i] * x[i];
-        sy = sy + mass[i] * y[i];
-        sz = sz + mass[i] * z[i];
-        sm = sm + mass[i];
-    }
-    cx[m] = sx / sm; cy[m] = sy / sm; cz[m] = sz / sm;
-}
 
-struct CombineArgs {
-    int n;
-    int group;                     // 0 ALL, 1 FAST, 2 SLOW
-    const double *fsx, *fsy, *fsz; // pair short
-    const double *flx, *fly, *flz; // pair long
-    const double *recip;           // [nslice][3][n]
-    int nslice;
-    const unsigned long long *gx, *gy, *gz;    // fixed: slow scatter (ETERMS, long specials)
-    const unsigned long long *hlx, *hly, *hlz; // fixed: LOCALF specials
-    const unsigned long long *hbx, *hby, *hbz; // fixed: bonded
-    double inv_scale;
-    const double *x, *y, *z;
-    const int *atom_mol;
-    const double *cx, *cy, *cz;
-    double *ox, *oy, *oz;          // F of the group
-    double *Gx, *Gy, *Gz;          // slow (optional)
-    double *Hx, *Hy, *Hz;          // fast (optional)
-    double *epart;
-};
 
 __device__ inline double fixd(const unsigned long long *p, int i, double inv) {
     return (double)(long long)p[i] * inv;
 }
 
-__global__ __launch_bounds__(256) void k_combine(CombineArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void combine_body(const CombineArgs &a, int blk, int group) {
+    const int i = blk * blockDim.x + threadIdx.x;
     double v[2] = {0, 0};
     if (i < a.n) {
         double G[3] = {0, 0, 0}, HL[3] = {0, 0, 0}, H[3] = {0, 0, 0};
-        if (a.group != 1) {
+        if (group != 1) {
             G[0] = a.flx[i] + fixd(a.gx, i, a.inv_scale);
             G[1] = a.fly[i] + fixd(a.gy, i, a.inv_scale);
             G[2] = a.flz[i] + fixd(a.gz, i, a.inv_scale);
-            for (int s = 0; s < a.nslice; ++s)
+            for (int s = 0; s < (group != 1 ? a.nslice : 0); ++s)
                 for (int d = 0; d < 3; ++d) G[d] += a.recip[((size_t)s * 3 + d) * a.n + i];
         }
-        if (a.group != 2) {
+        if (group != 2) {
             HL[0] = a.fsx[i] + fixd(a.hlx, i, a.inv_scale);
             HL[1] = a.fsy[i] + fixd(a.hly, i, a.inv_scale);
             HL[2] = a.fsz[i] + fixd(a.hlz, i, a.inv_scale);
             H[0] = HL[0] + fixd(a.hbx, i, a.inv_scale);
             H[1] = HL[1] + fixd(a.hby, i, a.inv_scale);
             H[2] = HL[2] + fixd(a.hbz, i, a.inv_scale);
+        }
+        if (a.scale != 1.0) {
+            // path-integral beads: physical forces / P (pimd.f:61-69)
+            for (int d = 0; d < 3; ++d) { G[d] *= a.scale; HL[d] *= a.scale; H[d] *= a.scale; }
+        }
+        if (a.spx && group != 2) {
+            H[0] += fixd(a.spx, i, a.inv_scale);
+            H[1] += fixd(a.spy, i, a.inv_scale);
+            H[2] += fixd(a.spz, i, a.inv_scale);
         }
         const int m = a.atom_mol[i];
         const double wx = a.x[i] - a.cx[m], wy = a.y[i] - a.cy[m], wz = a.z[i] - a.cz[m];
@@ -337,7 +291,7 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs a) {
         if (a.Gx) { a.Gx[i] = G[0]; a.Gy[i] = G[1]; a.Gz[i] = G[2]; }
         if (a.Hx) { a.Hx[i] = H[0]; a.Hy[i] = H[1]; a.Hz[i] = H[2]; }
     }
-    double *o = a.epart + (size_t)blockIdx.x * Q_N;
+    double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
     const int slots[2] = {Q_WIRS, Q_WIRSS};
     block_reduce_store<2>(v, slots, o);
@@ -345,26 +299,90 @@ __global__ __launch_bounds__(256) void k_combine(CombineArgs a) {
 
 // Sum rows [0, nrows) of epart; block q sums slot q with a fixed
 // strided-then-tree order (256 threads).
-__global__ __launch_bounds__(256) void k_ereduce(int nrows, const double *__restrict__ epart,
-                                                 double *__restrict__ out) {
-    __shared__ double red[256];
-    const int q = blockIdx.x;
+__device__ inline void ereduce_body(const ReduceArgs &a, int q, double *red) {
     double s = 0.0;
-    for (int r = threadIdx.x; r < nrows; r += 256) s += epart[(size_t)r * Q_N + q];
+    for (int r = threadIdx.x; r < a.nrows; r += 256) s += a.epart[(size_t)r * Q_N + q];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[q] = red[0];
+    if (threadIdx.x == 0) a.out[q] = red[0];
 }
 
-// Zero fixed-point accumulators (kernel, not a memset node, so the call
-// sequence can be captured in a graph).
-__global__ void k_zero_u64(size_t n, unsigned long long *__restrict__ p) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+// Clear accumulators (a kernel, not a memset node, so the call sequence
+// can be captured in a graph).
+__device__ inline void zero_body(const ZeroSeg &z, int blk) {
+    const long long i = (long long)blk * 256 + threadIdx.x;
+    if (i < z.n) z.p[i] = 0ull;
+}
+
+// Path-integral springs to both neighbouring beads (PIFORCES,
+// pimd.f:321-394): E = BEADFAC m |r_k - r_k+-1|^2, force -2 BEADFAC m d.
+// The neighbours' coordinates are the FP64 ones (the reference moves them
+// through REAL*4 buffers, pimd.f:451-486).
+__device__ inline void springs_body(const SpringArgs &a, int blk) {
+    const int i = blk * blockDim.x + threadIdx.x;
+    double v[5] = {0, 0, 0, 0, 0};
+    if (i < a.n) {
+        double f[3] = {0, 0, 0};
+        for (int side = 0; side < 2; ++side) {
+            const double *ox = side == 0 ? a.xp : a.xm;
+            const double *oy = side == 0 ? a.yp : a.ym;
+            const double *oz = side == 0 ? a.zp : a.zm;
+            double bx = a.x[i] - ox[i], by = a.y[i] - oy[i], bz = a.z[i] - oz[i];
+            min_image(bx, by, bz, a.box);
+            double bsq = bx * bx + by * by;
+            bsq = bsq + bz * bz;
+            const double bbb = sqrt(bsq);
+            const double dbnd = bbb * a.beadfac * a.mass[i];
+            const double ebdd = dbnd * bbb;
+            const double forb = bbb > 0.0 ? -2.0 * dbnd / bbb : 0.0;
+            f[0] += bx * forb; f[1] += by * forb; f[2] += bz * forb;
+            v[0] += ebdd;
+            if (side == 0) {
+                // forward link only (pimd.f:336-354); VIRF* carry the
+                // factor NBEADS that the caller's 1/NBEADS scaling removes
+                v[1] += forb * bsq;
+                v[2] += forb * bx * bx * a.nbeads;
+                v[3] += forb * by * by * a.nbeads;
+                v[4] += forb * bz * bz * a.nbeads;
+            }
+        }
+        fx_add(a.fx + i, f[0], a.fscale); fx_add(a.fy + i, f[1], a.fscale); fx_add(a.fz + i, f[2], a.fscale);
+        v[0] *= 0.5;     // each link is seen from both of its beads (pimd.f:387)
+    }
+    double *o = a.epart + (size_t)blk * Q_N;
+    zero_slots(o);
+    const int slots[5] = {Q_QEK, Q_VIRPI, Q_VIRFX, Q_VIRFY, Q_VIRFZ};
+    block_reduce_store<5>(v, slots, o);
+}
+
+#define MDX_BATCHED(NAME, ARGS, BODY, NT)                                                    \
+    __global__ __launch_bounds__(NT) void NAME(const ARGS *__restrict__ args,               \
+                                               const int2 *__restrict__ map) {              \
+        const int2 m = map[blockIdx.x];                                                      \
+        BODY(args[m.x], m.y);                                                                \
+    }
+
+MDX_BATCHED(k_eterms, EtermArgs, eterms_body, 256)
+MDX_BATCHED(k_bonds, BondArgs, bonds_body, 256)
+MDX_BATCHED(k_angles, AngleArgs, angles_body, 256)
+MDX_BATCHED(k_tors, TorsArgs, tors_body, 256)
+MDX_BATCHED(k_springs, SpringArgs, springs_body, 256)
+MDX_BATCHED(k_zero, ZeroSeg, zero_body, 256)
+
+__global__ __launch_bounds__(256) void k_combine(const CombineArgs *__restrict__ args, const int2 *__restrict__ map,
+                                                 int group) {
+    const int2 m = map[blockIdx.x];
+    combine_body(args[m.x], m.y, group);
+}
+
+__global__ __launch_bounds__(256) void k_ereduce(const ReduceArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ double red[256];
+    const int2 m = map[blockIdx.x];
+    ereduce_body(args[m.x], m.y, red);
 }
 
 __global__ void k_fill_i32(int n, int *__restrict__ p, int v) {

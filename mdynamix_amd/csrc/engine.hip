@@ -30,10 +30,12 @@ using namespace mdx;
 namespace {
 
 // Timed stages (HIP events on the handle's stream).
-// Stage k >= 1 lies between events k-1 and k of compute_impl.
+// Stage k >= 1 lies between events k-1 and k of launch_call.
 enum Stage { ST_LIST = 0, ST_PREP, ST_PAIRS, ST_SPECIAL, ST_RPREP, ST_SK, ST_SKRED, ST_RFORCE, ST_BONDED, ST_REDUCE, ST_N };
 
 }  // namespace
+
+struct mdx_batch;
 
 struct mdx_engine {
     std::string err;
@@ -63,10 +65,24 @@ struct mdx_engine {
     int2 *d_et_pairs = nullptr;
     // bonded
     int nb = 0, na = 0, nt = 0;
-    BondArgs bonds{};
-    AngleArgs angles{};
-    TorsArgs tors{};
     std::vector<void *> owned;
+    // argument structs of the force kernels, refreshed whenever a list
+    // rebuild or box change moves a pointer or count (version counts those)
+    PackArgs a_pack{};
+    PairArgs a_pair{};
+    SpecialArgs a_special{};
+    RecipArgs a_recip{};
+    EtermArgs a_et{};
+    BondArgs a_bonds{};
+    AngleArgs a_angles{};
+    TorsArgs a_tors{};
+    long long version = 1;
+    // per-system rows of the energy partials (fixed by capacities)
+    int row_pairs = 0, row_sp = 0, row_sk = 0, row_et = 0, row_b = 0, row_a = 0, row_t = 0, row_spr = 0,
+        row_c = 0, rows_total = 0;
+    unsigned long long *d_fixsp = nullptr;        // [3][n] bead springs
+    std::map<void *, size_t> dev_sizes;           // live device allocations
+    mdx_batch *self = nullptr;                    // batch of one for mdx_compute
     // coordinates
     double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
     bool have_pos = false;
@@ -110,10 +126,60 @@ struct mdx_engine {
     double *d_epart = nullptr;
     int epart_rows = 0;
     double *d_energy = nullptr;
-    // timing
-    hipEvent_t ev[12] = {};
     double stage_ms[ST_N] = {};
+    hipEvent_t ev[2] = {};
     bool timing = true;
+};
+
+// A launch plan over K systems: per kernel an array of argument structs and
+// a block map in HBM; one launch per kernel covers all systems.
+template <typename T>
+struct KPlan {
+    std::vector<T> h_args;
+    std::vector<int2> h_map;
+    T *d_args = nullptr;
+    int2 *d_map = nullptr;
+    size_t cap_args = 0, cap_map = 0;
+    int nblocks() const { return (int)h_map.size(); }
+    void clear() { h_args.clear(); h_map.clear(); }
+    void add(const T &a, int nblk) {
+        const int s = (int)h_args.size();
+        h_args.push_back(a);
+        for (int b = 0; b < nblk; ++b) h_map.push_back(make_int2(s, b));
+    }
+};
+
+struct mdx_batch {
+    std::vector<mdx_engine *> sys;
+    std::vector<long long> seen;     // argument versions last uploaded
+    std::string err;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool ring = false;
+    double beadfac = 0.0;
+    bool use_graph = true;
+    bool planned = false;
+    int maxlen = 16, nlj_max = 1;
+    bool soft = false;
+    KPlan<ZeroSeg> zero;
+    KPlan<PackArgs> pack;
+    KPlan<PairArgs> pairs;
+    KPlan<SpecialArgs> special;
+    KPlan<RecipArgs> rprep, sk, skred, rforce;
+    KPlan<EtermArgs> et;
+    KPlan<BondArgs> bonds;
+    KPlan<AngleArgs> angles;
+    KPlan<TorsArgs> tors;
+    KPlan<SpringArgs> springs;
+    KPlan<CombineArgs> comb;
+    KPlan<ReduceArgs> red;
+    double *d_energy = nullptr;      // [K][Q_N]
+    double *h_energy = nullptr;      // pinned copy
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[12] = {};
+    bool timing = false;
+    double stage_ms[ST_N] = {};
 };
 
 #define HIPCHK(h, call)                                                              \
@@ -136,13 +202,18 @@ namespace {
 
 template <typename T>
 mdx_status dalloc(mdx_engine *h, T **p, size_t count) {
-    if (*p) { hipFree(*p); *p = nullptr; }
+    if (*p) {
+        h->dev_sizes.erase((void *)*p);
+        hipFree(*p);
+        *p = nullptr;
+    }
     if (count == 0) count = 1;
     hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
     if (e != hipSuccess) {
         h->err = "hipMalloc of " + std::to_string(sizeof(T) * count) + " bytes failed";
         return MDX_ERR_NOMEM;
     }
+    h->dev_sizes[(void *)*p] = sizeof(T) * count;
     return MDX_OK;
 }
 
@@ -329,17 +400,143 @@ mdx_status alloc_clusters(mdx_engine *h) {
     return MDX_OK;
 }
 
-mdx_status ensure_epart(mdx_engine *h, int rows) {
-    if (rows > h->epart_rows) {
-        CHK(dalloc(h, &h->d_epart, (size_t)rows * Q_N));
-        h->epart_rows = rows;
-    }
-    return MDX_OK;
-}
 
 double list_radius2(const mdx_engine *h) {
     const double rl = h->p.rcut + (h->p.list_mode == MDX_LIST_VERLET ? h->p.skin : 0.0);
     return rl * rl;
+}
+
+
+// Rows of the energy partials of one system: fixed by capacities so launch
+// shapes do not change at list rebuilds.
+void layout_rows(mdx_engine *h) {
+    int r = 0;
+    h->row_pairs = r; r += h->nclu_cap;
+    h->row_sp = r; r += h->nspecial ? cdiv(h->nspecial, 256) : 0;
+    h->row_sk = r; r += cdiv(std::max(1, h->nkv), 256);
+    h->row_et = r; r += h->neterm ? cdiv(h->neterm, 256) : 0;
+    h->row_b = r; r += h->nb ? cdiv(h->nb, 256) : 0;
+    h->row_a = r; r += h->na ? cdiv(h->na, 256) : 0;
+    h->row_t = r; r += h->nt ? cdiv(h->nt, 256) : 0;
+    h->row_spr = r; r += cdiv(h->n, 256);
+    h->row_c = r; r += cdiv(h->n, 256);
+    h->rows_total = r;
+}
+
+double *erow(mdx_engine *h, int row) { return h->d_epart + (size_t)row * Q_N; }
+
+// Argument structs of every force kernel from the current state.
+void fill_args(mdx_engine *h) {
+    const int n = h->n;
+    const bool verlet = h->p.list_mode == MDX_LIST_VERLET;
+    const double rc2 = verlet ? h->p.rcut * h->p.rcut : HUGE_VAL;
+    unsigned long long *fixG = h->d_fix, *fixHL = h->d_fix + 3 * (size_t)n, *fixHB = h->d_fix + 6 * (size_t)n;
+    PackArgs &pk = h->a_pack;
+    pk.nclu = h->have_list ? h->nclu : 0;
+    pk.clu_atom = h->d_clu_atom;
+    pk.anchor = h->d_anchor;
+    pk.x = h->d_x; pk.y = h->d_y; pk.z = h->d_z; pk.q = h->d_q;
+    pk.box = h->box;
+    pk.xq = h->d_xq;
+    pk.nmol = h->nmol;
+    pk.user_com = h->user_com ? 1 : 0;
+    pk.mstart = h->d_mstart;
+    pk.mass = h->d_mass;
+    pk.cx = h->d_cx; pk.cy = h->d_cy; pk.cz = h->d_cz;
+
+    PairArgs &pa = h->a_pair;
+    pa.nclu = h->have_list ? h->nclu : 0;
+    pa.list = h->d_list;
+    pa.nlist = h->d_nlist;
+    pa.list_cap = h->list_cap;
+    pa.xq = h->d_xq;
+    pa.aux = h->d_aux;
+    pa.q = h->d_q;
+    pa.coulf_f = (float)h->p.coulomb_factor;
+    pa.coulf = h->p.coulomb_factor;
+    pa.lj = h->d_lj;
+    pa.nlj = h->nlj;
+    pa.clu_atom = h->d_clu_atom;
+    pa.alpha = (float)h->p.alpha;
+    pa.rc2 = verlet ? (float)rc2 : INFINITY;
+    pa.raex = (float)h->p.ee_raex;
+    pa.soft = h->p.ee_enabled ? 1 : 0;
+    {
+        // Cluster image shifts are minimum images of the anchors; with
+        // clusters up to ~5 A across, a pair can need another image only
+        // when half the box is within ~10 A of the list radius.
+        const double rl = std::sqrt(list_radius2(h));
+        const double hmin = 0.5 * std::min(h->p.box[0], std::min(h->p.box[1], h->p.box[2]));
+        pa.wrap = hmin < rl + 10.0;
+        pa.lx = (float)h->p.box[0]; pa.ly = (float)h->p.box[1]; pa.lz = (float)h->p.box[2];
+        pa.ilx = (float)(1.0 / h->p.box[0]); pa.ily = (float)(1.0 / h->p.box[1]); pa.ilz = (float)(1.0 / h->p.box[2]);
+    }
+    pa.fs_x = h->d_fs; pa.fs_y = h->d_fs + n; pa.fs_z = h->d_fs + 2 * n;
+    pa.fl_x = h->d_fl; pa.fl_y = h->d_fl + n; pa.fl_z = h->d_fl + 2 * n;
+    pa.epart = erow(h, h->row_pairs);
+
+    SpecialArgs &sa = h->a_special;
+    sa.ns = h->nspecial;
+    sa.pairs = h->d_sp_pairs;
+    sa.cls = h->d_sp_cls;
+    sa.x = h->d_x; sa.y = h->d_y; sa.z = h->d_z;
+    sa.atom_moltype = h->d_moltype;
+    sa.atom_ljtype = h->d_ljtype;
+    sa.q0 = h->d_q0;
+    sa.type_ee = h->d_type_ee;
+    sa.c14el = h->d_c14el; sa.c14lj = h->d_c14lj;
+    sa.a6_0 = h->d_a6_0; sa.b12_0 = h->d_b12_0; sa.a6_14 = h->d_a6_14; sa.b12_14 = h->d_b12_14;
+    sa.nlj = h->nlj;
+    sa.coulf = h->p.coulomb_factor;
+    sa.alpha = h->p.alpha;
+    sa.rc2 = rc2;
+    sa.box = h->box;
+    sa.fscale = h->fscale;
+    sa.fixh_x = fixHL; sa.fixh_y = fixHL + n; sa.fixh_z = fixHL + 2 * n;
+    sa.fixg_x = fixG; sa.fixg_y = fixG + n; sa.fixg_z = fixG + 2 * n;
+    sa.epart = erow(h, h->row_sp);
+
+    RecipArgs &ra = h->a_recip;
+    ra.natoms = n;
+    ra.x = h->d_x; ra.y = h->d_y; ra.z = h->d_z; ra.q = h->d_q;
+    ra.box = h->box;
+    ra.fq = h->d_fq;
+    ra.stepz = h->d_stepz;
+    ra.nrows = h->nrows;
+    ra.nrowgroups = cdiv(std::max(1, h->nrows), 64);
+    ra.chunk = h->chunk;
+    ra.nchunk = h->nchunk;
+    ra.nkv = h->nkv;
+    ra.rows = h->d_rows;
+    ra.rowoff = h->d_rowoff;
+    ra.wavelen = h->d_wavelen;
+    ra.part = h->d_part;
+    ra.kidx = h->d_kidx;
+    ra.alpha = h->p.alpha;
+    ra.coulf = h->p.coulomb_factor;
+    ra.ak = h->d_ak;
+    ra.sk = h->d_sk;
+    ra.nslice = h->nslice;
+    ra.rows_per_slice = h->rows_per_slice;
+    ra.natom_blocks = cdiv(n, 64);
+    ra.rpart = h->d_rpart;
+    ra.epart = erow(h, h->row_sk);
+
+    ScatterArgs slow{h->d_x, h->d_y, h->d_z, h->box, h->fscale, fixG, fixG + n, fixG + 2 * n, erow(h, h->row_et)};
+    h->a_et.np = h->neterm;
+    h->a_et.pairs = h->d_et_pairs;
+    h->a_et.q = h->d_q;
+    h->a_et.coulf = h->p.coulomb_factor;
+    h->a_et.alpha = h->p.alpha;
+    h->a_et.s = slow;
+    ScatterArgs fast{h->d_x, h->d_y, h->d_z, h->box, h->fscale, fixHB, fixHB + n, fixHB + 2 * n, nullptr};
+    h->a_bonds.s = fast;
+    h->a_bonds.s.epart = erow(h, h->row_b);
+    h->a_angles.s = fast;
+    h->a_angles.s.epart = erow(h, h->row_a);
+    h->a_tors.s = fast;
+    h->a_tors.s.epart = erow(h, h->row_t);
+    h->version++;
 }
 
 }  // namespace
@@ -423,7 +620,7 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
         h->err = "hipStreamCreate failed";
         return fail(MDX_ERR_HIP);
     }
-    for (int k = 0; k < 12; ++k) hipEventCreate(&h->ev[k]);
+    for (int k = 0; k < 2; ++k) hipEventCreate(&h->ev[k]);
     const int n = h->n;
     // fixed-point scale: resolution ~1e-14 of coulomb_factor, range ~1e4 * coulomb_factor
     h->fscale = std::ldexp(1.0, (int)std::floor(std::log2(std::ldexp(1.0, 60) / (std::fabs(p->coulomb_factor) * 1e4 + 1e-300))));
@@ -507,37 +704,37 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
     h->nt = t->ntors;
     int *ip;
     double *dp;
-    h->bonds.n = h->nb;
-    if ((s = upload(h, &ip, t->bond_i, h->nb))) return fail(s); h->bonds.bi = ip;
-    if ((s = upload(h, &ip, t->bond_j, h->nb))) return fail(s); h->bonds.bj = ip;
-    if ((s = upload(h, &ip, t->bond_param, h->nb))) return fail(s); h->bonds.bpar = ip;
-    if ((s = upload(h, &ip, t->bp_morse, t->nbondparams))) return fail(s); h->bonds.morse = ip;
-    if ((s = upload(h, &dp, t->bp_r0, t->nbondparams))) return fail(s); h->bonds.r0 = dp;
-    if ((s = upload(h, &dp, t->bp_k, t->nbondparams))) return fail(s); h->bonds.k = dp;
-    if ((s = upload(h, &dp, t->bp_d, t->nbondparams))) return fail(s); h->bonds.d = dp;
-    if ((s = upload(h, &dp, t->bp_rho, t->nbondparams))) return fail(s); h->bonds.rho = dp;
-    h->angles.n = h->na;
-    if ((s = upload(h, &ip, t->angle_i, h->na))) return fail(s); h->angles.ai = ip;
-    if ((s = upload(h, &ip, t->angle_j, h->na))) return fail(s); h->angles.aj = ip;
-    if ((s = upload(h, &ip, t->angle_k, h->na))) return fail(s); h->angles.ak = ip;
-    if ((s = upload(h, &ip, t->angle_param, h->na))) return fail(s); h->angles.apar = ip;
-    if ((s = upload(h, &dp, t->ap_theta0, t->nangleparams))) return fail(s); h->angles.theta0 = dp;
-    if ((s = upload(h, &dp, t->ap_k, t->nangleparams))) return fail(s); h->angles.k = dp;
-    h->tors.n = h->nt;
-    if ((s = upload(h, &ip, t->tors_i, h->nt))) return fail(s); h->tors.ti = ip;
-    if ((s = upload(h, &ip, t->tors_j, h->nt))) return fail(s); h->tors.tj = ip;
-    if ((s = upload(h, &ip, t->tors_k, h->nt))) return fail(s); h->tors.tk = ip;
-    if ((s = upload(h, &ip, t->tors_l, h->nt))) return fail(s); h->tors.tl = ip;
-    if ((s = upload(h, &ip, t->tors_param, h->nt))) return fail(s); h->tors.tpar = ip;
-    if ((s = upload(h, &ip, t->tp_kind, t->ntorsparams))) return fail(s); h->tors.kind = ip;
-    if ((s = upload(h, &ip, t->tp_mult, t->ntorsparams))) return fail(s); h->tors.mult = ip;
-    if ((s = upload(h, &dp, t->tp_k, t->ntorsparams))) return fail(s); h->tors.k = dp;
-    if ((s = upload(h, &dp, t->tp_delta, t->ntorsparams))) return fail(s); h->tors.delta = dp;
-    if ((s = upload(h, &dp, t->tp_k1, t->ntorsparams))) return fail(s); h->tors.k1 = dp;
-    if ((s = upload(h, &dp, t->tp_k2, t->ntorsparams))) return fail(s); h->tors.k2 = dp;
-    if ((s = upload(h, &dp, t->tp_k3, t->ntorsparams))) return fail(s); h->tors.k3 = dp;
-    if ((s = upload(h, &dp, t->tp_k4, t->ntorsparams))) return fail(s); h->tors.k4 = dp;
-    if ((s = upload(h, &dp, t->tp_k5, t->ntorsparams))) return fail(s); h->tors.k5 = dp;
+    h->a_bonds.n = h->nb;
+    if ((s = upload(h, &ip, t->bond_i, h->nb))) return fail(s); h->a_bonds.bi = ip;
+    if ((s = upload(h, &ip, t->bond_j, h->nb))) return fail(s); h->a_bonds.bj = ip;
+    if ((s = upload(h, &ip, t->bond_param, h->nb))) return fail(s); h->a_bonds.bpar = ip;
+    if ((s = upload(h, &ip, t->bp_morse, t->nbondparams))) return fail(s); h->a_bonds.morse = ip;
+    if ((s = upload(h, &dp, t->bp_r0, t->nbondparams))) return fail(s); h->a_bonds.r0 = dp;
+    if ((s = upload(h, &dp, t->bp_k, t->nbondparams))) return fail(s); h->a_bonds.k = dp;
+    if ((s = upload(h, &dp, t->bp_d, t->nbondparams))) return fail(s); h->a_bonds.d = dp;
+    if ((s = upload(h, &dp, t->bp_rho, t->nbondparams))) return fail(s); h->a_bonds.rho = dp;
+    h->a_angles.n = h->na;
+    if ((s = upload(h, &ip, t->angle_i, h->na))) return fail(s); h->a_angles.ai = ip;
+    if ((s = upload(h, &ip, t->angle_j, h->na))) return fail(s); h->a_angles.aj = ip;
+    if ((s = upload(h, &ip, t->angle_k, h->na))) return fail(s); h->a_angles.ak = ip;
+    if ((s = upload(h, &ip, t->angle_param, h->na))) return fail(s); h->a_angles.apar = ip;
+    if ((s = upload(h, &dp, t->ap_theta0, t->nangleparams))) return fail(s); h->a_angles.theta0 = dp;
+    if ((s = upload(h, &dp, t->ap_k, t->nangleparams))) return fail(s); h->a_angles.k = dp;
+    h->a_tors.n = h->nt;
+    if ((s = upload(h, &ip, t->tors_i, h->nt))) return fail(s); h->a_tors.ti = ip;
+    if ((s = upload(h, &ip, t->tors_j, h->nt))) return fail(s); h->a_tors.tj = ip;
+    if ((s = upload(h, &ip, t->tors_k, h->nt))) return fail(s); h->a_tors.tk = ip;
+    if ((s = upload(h, &ip, t->tors_l, h->nt))) return fail(s); h->a_tors.tl = ip;
+    if ((s = upload(h, &ip, t->tors_param, h->nt))) return fail(s); h->a_tors.tpar = ip;
+    if ((s = upload(h, &ip, t->tp_kind, t->ntorsparams))) return fail(s); h->a_tors.kind = ip;
+    if ((s = upload(h, &ip, t->tp_mult, t->ntorsparams))) return fail(s); h->a_tors.mult = ip;
+    if ((s = upload(h, &dp, t->tp_k, t->ntorsparams))) return fail(s); h->a_tors.k = dp;
+    if ((s = upload(h, &dp, t->tp_delta, t->ntorsparams))) return fail(s); h->a_tors.delta = dp;
+    if ((s = upload(h, &dp, t->tp_k1, t->ntorsparams))) return fail(s); h->a_tors.k1 = dp;
+    if ((s = upload(h, &dp, t->tp_k2, t->ntorsparams))) return fail(s); h->a_tors.k2 = dp;
+    if ((s = upload(h, &dp, t->tp_k3, t->ntorsparams))) return fail(s); h->a_tors.k3 = dp;
+    if ((s = upload(h, &dp, t->tp_k4, t->ntorsparams))) return fail(s); h->a_tors.k4 = dp;
+    if ((s = upload(h, &dp, t->tp_k5, t->ntorsparams))) return fail(s); h->a_tors.k5 = dp;
     // coordinates, COMs, energies
     if ((s = dalloc(h, &h->d_x, n)) || (s = dalloc(h, &h->d_y, n)) || (s = dalloc(h, &h->d_z, n)) ||
         (s = dalloc(h, &h->d_cx, h->nmol)) || (s = dalloc(h, &h->d_cy, h->nmol)) ||
@@ -546,6 +743,19 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
     self_energy(h, t);
     if ((s = setup_kspace(h))) return fail(s);
     if ((s = alloc_clusters(h))) return fail(s);
+    if ((s = dalloc(h, &h->d_fixsp, (size_t)3 * n))) return fail(s);
+    layout_rows(h);
+    if ((s = dalloc(h, &h->d_epart, (size_t)h->rows_total * Q_N))) return fail(s);
+    fill_args(h);
+    mdx_batch *self = nullptr;
+    mdx_engine *one[1] = {h};
+    if ((s = mdx_batch_create(one, 1, &self))) return fail(s);
+    hipStreamDestroy(self->stream);
+    self->stream = h->stream;
+    self->own_stream = false;
+    self->use_graph = false;
+    self->timing = true;
+    h->self = self;
     *out = h;
     return MDX_OK;
 }
@@ -553,16 +763,9 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
 mdx_status mdx_destroy(mdx_engine *h) {
     if (!h) return MDX_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
-    for (void *q : h->owned) hipFree(q);
-    void *bufs[] = {h->d_sp_cls, h->d_x, h->d_y, h->d_z, h->d_cx, h->d_cy, h->d_cz, h->d_keys, h->d_keys2,
-                    h->d_vals, h->d_vals2, h->d_colcount, h->d_colstart, h->d_colclu_n, h->d_colclu, h->d_cub,
-                    h->d_clu_atom, h->d_atom_slot, h->d_clu_col, h->d_anchor, h->d_ext, h->d_cand, h->d_ncand,
-                    h->d_nlist, h->d_overflow, h->d_list, h->d_xq, h->d_aux, h->d_rows, h->d_rowoff, h->d_kidx,
-                    h->d_fq, h->d_stepz, h->d_part, h->d_ak, h->d_sk, h->d_rpart, h->d_wavelen, h->d_fs, h->d_fl, h->d_fix,
-                    h->d_out, h->d_epart, h->d_energy};
-    for (void *q : bufs)
-        if (q) hipFree(q);
-    for (int k = 0; k < 12; ++k)
+    if (h->self) mdx_batch_destroy(h->self);
+    for (auto &kv : h->dev_sizes) hipFree(kv.first);
+    for (int k = 0; k < 2; ++k)
         if (h->ev[k]) hipEventDestroy(h->ev[k]);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -594,6 +797,7 @@ mdx_status mdx_set_molecule_com(mdx_engine *h, const double *x, const double *y,
     if (!h) return MDX_ERR_ARG;
     if (!x || !y || !z) {
         h->user_com = false;
+        fill_args(h);
         return MDX_OK;
     }
     const size_t b = sizeof(double) * h->nmol;
@@ -602,6 +806,7 @@ mdx_status mdx_set_molecule_com(mdx_engine *h, const double *x, const double *y,
     HIPCHK(h, hipMemcpyAsync(h->d_cz, z, b, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->user_com = true;
+    fill_args(h);
     return MDX_OK;
 }
 
@@ -623,8 +828,13 @@ mdx_status mdx_set_box(mdx_engine *h, const double box[3]) {
         const double r = box[k] / h->kbox[k];
         if (r > (double)1.05f || r < (double)0.95f) redo = true;
     }
-    if (redo) CHK(setup_kspace(h));
+    if (redo) {
+        CHK(setup_kspace(h));
+        layout_rows(h);
+        CHK(dalloc(h, &h->d_epart, (size_t)h->rows_total * Q_N));
+    }
     h->have_list = false;
+    fill_args(h);
     return MDX_OK;
 }
 
@@ -709,6 +919,7 @@ mdx_status mdx_build_list(mdx_engine *h) {
     }
     HIPCHK(h, hipGetLastError());
     h->have_list = true;
+    fill_args(h);
     return MDX_OK;
 }
 
@@ -799,239 +1010,418 @@ mdx_status mdx_get_list(mdx_engine *h, int32_t *nbs1, int32_t *nbs2, int64_t cap
     return MDX_OK;
 }
 
-static mdx_status compute_impl(mdx_engine *h, int32_t group, int32_t want, double *dF, double *dG, double *dH,
-                               mdx_energies *e) {
-    (void)want;
-    if (!h->have_list) {
-        h->err = "mdx_compute before mdx_build_list";
-        return MDX_ERR_STATE;
-    }
-    if (group < 0 || group > 2) return MDX_ERR_ARG;
-    hipStream_t st = h->stream;
-    const int n = h->n;
-    const int nslot = h->nclu * kClu;
-    const bool verlet = h->p.list_mode == MDX_LIST_VERLET;
-    const double rc2 = verlet ? h->p.rcut * h->p.rcut : HUGE_VAL;
-    // energy partial rows
-    const int rows_pairs = h->nclu;
-    const int rows_sp = h->nspecial ? cdiv(h->nspecial, 256) : 0;
-    const int rows_sk = group != 1 ? cdiv(h->nkv, 256) : 0;
-    const int rows_et = (group != 1 && h->neterm) ? cdiv(h->neterm, 256) : 0;
-    const int rows_b = (group != 2 && h->nb) ? cdiv(h->nb, 256) : 0;
-    const int rows_a = (group != 2 && h->na) ? cdiv(h->na, 256) : 0;
-    const int rows_t = (group != 2 && h->nt) ? cdiv(h->nt, 256) : 0;
-    const int rows_c = cdiv(n, 256);
-    const int total_rows = rows_pairs + rows_sp + rows_sk + rows_et + rows_b + rows_a + rows_t + rows_c;
-    CHK(ensure_epart(h, total_rows));
-    auto mark = [&](int k) { if (h->timing) hipEventRecord(h->ev[k], st); };
-    mark(0);
-    k_zero_u64<<<cdiv((size_t)total_rows * Q_N, 256), 256, 0, st>>>((size_t)total_rows * Q_N,
-                                                                      (unsigned long long *)h->d_epart);
-    k_zero_u64<<<cdiv((size_t)9 * n, 256), 256, 0, st>>>((size_t)9 * n, h->d_fix);
-    k_zero_u64<<<cdiv((size_t)6 * n, 256), 256, 0, st>>>((size_t)3 * n, (unsigned long long *)h->d_fs);
-    k_zero_u64<<<cdiv((size_t)6 * n, 256), 256, 0, st>>>((size_t)3 * n, (unsigned long long *)h->d_fl);
-    k_pack<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, h->d_anchor, h->d_x, h->d_y, h->d_z, h->d_q,
-                                              h->box, h->d_xq);
-    if (!h->user_com)
-        k_com<<<cdiv(h->nmol, 256), 256, 0, st>>>(h->nmol, h->d_mstart, h->d_mass, h->d_x, h->d_y, h->d_z, h->d_cx,
-                                                  h->d_cy, h->d_cz);
-    // real-space pairs
-    PairArgs pa;
-    pa.list = h->d_list;
-    pa.nlist = h->d_nlist;
-    pa.list_cap = h->list_cap;
-    pa.xq = h->d_xq;
-    pa.aux = h->d_aux;
-    pa.q = h->d_q;
-    pa.coulf_f = (float)h->p.coulomb_factor;
-    pa.coulf = h->p.coulomb_factor;
-    pa.lj = h->d_lj;
-    pa.nlj = h->nlj;
-    pa.clu_atom = h->d_clu_atom;
-    pa.alpha = (float)h->p.alpha;
-    pa.rc2 = verlet ? (float)rc2 : INFINITY;
-    pa.raex = (float)h->p.ee_raex;
-    {
-        // Cluster image shifts are minimum images of the anchors; with
-        // clusters up to ~5 A across, a pair can need another image only
-        // when half the box is within ~10 A of the list radius.
-        const double rl = std::sqrt(list_radius2(h));
-        const double hmin = 0.5 * std::min(h->p.box[0], std::min(h->p.box[1], h->p.box[2]));
-        pa.wrap = hmin < rl + 10.0;
-        pa.lx = (float)h->p.box[0]; pa.ly = (float)h->p.box[1]; pa.lz = (float)h->p.box[2];
-        pa.ilx = (float)(1.0 / h->p.box[0]); pa.ily = (float)(1.0 / h->p.box[1]); pa.ilz = (float)(1.0 / h->p.box[2]);
-    }
-    pa.group_mask = 0;
-    pa.group = group;
-    pa.fs_x = h->d_fs; pa.fs_y = h->d_fs + n; pa.fs_z = h->d_fs + 2 * n;
-    pa.fl_x = h->d_fl; pa.fl_y = h->d_fl + n; pa.fl_z = h->d_fl + 2 * n;
-    pa.epart = h->d_epart;
-    mark(1);
-    if (h->p.ee_enabled) k_pairs<true><<<h->nclu, 64, 0, st>>>(pa);
-    else k_pairs<false><<<h->nclu, 64, 0, st>>>(pa);
-    mark(2);
-    unsigned long long *fixG = h->d_fix, *fixHL = h->d_fix + 3 * (size_t)n, *fixHB = h->d_fix + 6 * (size_t)n;
-    int row = rows_pairs;
-    if (h->nspecial) {
-        SpecialArgs sa;
-        sa.ns = h->nspecial;
-        sa.pairs = h->d_sp_pairs;
-        sa.cls = h->d_sp_cls;
-        sa.x = h->d_x; sa.y = h->d_y; sa.z = h->d_z;
-        sa.atom_moltype = h->d_moltype;
-        sa.atom_ljtype = h->d_ljtype;
-        sa.q0 = h->d_q0;
-        sa.type_ee = h->d_type_ee;
-        sa.c14el = h->d_c14el; sa.c14lj = h->d_c14lj;
-        sa.a6_0 = h->d_a6_0; sa.b12_0 = h->d_b12_0; sa.a6_14 = h->d_a6_14; sa.b12_14 = h->d_b12_14;
-        sa.nlj = h->nlj;
-        sa.coulf = h->p.coulomb_factor;
-        sa.alpha = h->p.alpha;
-        sa.rc2 = rc2;
-        sa.box = h->box;
-        sa.fscale = h->fscale;
-        sa.fixh_x = fixHL; sa.fixh_y = fixHL + n; sa.fixh_z = fixHL + 2 * n;
-        sa.fixg_x = fixG; sa.fixg_y = fixG + n; sa.fixg_z = fixG + 2 * n;
-        sa.group = group;
-        sa.epart = h->d_epart + (size_t)row * Q_N;
-        k_special<<<rows_sp, 256, 0, st>>>(sa);
-        row += rows_sp;
-    }
-    mark(3);
-    if (group != 1) {
-        k_recip_prep<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_x, h->d_y, h->d_z, h->d_q, h->box, h->d_fq,
-                                                    h->d_stepz);
-        mark(4);
-        const dim3 gsk(cdiv(h->nrows, 64), h->nchunk);
-        if (h->maxlen <= 16)
-            k_sk<16><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, h->d_wavelen, n, h->chunk, h->d_fq, h->d_stepz,
-                                         h->nkv, h->d_part);
-        else if (h->maxlen <= 32)
-            k_sk<32><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, h->d_wavelen, n, h->chunk, h->d_fq, h->d_stepz,
-                                         h->nkv, h->d_part);
-        else if (h->maxlen <= 64)
-            k_sk<64><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, h->d_wavelen, n, h->chunk, h->d_fq, h->d_stepz,
-                                         h->nkv, h->d_part);
-        else
-            k_sk<96><<<gsk, 64, 0, st>>>(h->nrows, h->d_rows, h->d_rowoff, h->d_wavelen, n, h->chunk, h->d_fq, h->d_stepz,
-                                         h->nkv, h->d_part);
-        mark(5);
-        k_sk_reduce<<<rows_sk, 256, 0, st>>>(h->nkv, h->nchunk, h->d_part, h->d_kidx, h->box, h->p.alpha,
-                                             h->p.coulomb_factor, h->d_ak, h->d_sk, h->d_epart + (size_t)row * Q_N);
-        row += rows_sk;
-        mark(6);
-        const dim3 gf(cdiv(n, 64), h->nslice);
-        k_recip_force<<<gf, 64, 0, st>>>(n, h->nrows, h->rows_per_slice, h->d_rows, h->d_rowoff, h->d_fq, h->d_stepz,
-                                         h->d_ak, h->box, h->d_rpart);
-        mark(7);
-        ScatterArgs sc{h->d_x, h->d_y, h->d_z, h->box, h->fscale, fixG, fixG + n, fixG + 2 * n,
-                       h->d_epart + (size_t)row * Q_N};
-        if (h->neterm) k_eterms<<<rows_et, 256, 0, st>>>(h->neterm, h->d_et_pairs, h->d_moltype, h->d_q,
-                                                         h->p.coulomb_factor, h->p.alpha, sc);
-        row += rows_et;
-    } else {
-        mark(4); mark(5); mark(6); mark(7);
-    }
-    if (group != 2) {
-        ScatterArgs sc{h->d_x, h->d_y, h->d_z, h->box, h->fscale, fixHB, fixHB + n, fixHB + 2 * n, nullptr};
-        if (h->nb) {
-            sc.epart = h->d_epart + (size_t)row * Q_N;
-            k_bonds<<<rows_b, 256, 0, st>>>(h->bonds, sc);
-            row += rows_b;
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// Launch plans and batched force calls.
+
+namespace {
+
+template <typename T>
+mdx_status kplan_upload(mdx_batch *b, KPlan<T> &k, bool maps) {
+    if (k.h_args.empty()) return MDX_OK;
+    if (k.h_args.size() > k.cap_args) {
+        if (k.d_args) hipFree(k.d_args);
+        k.d_args = nullptr;
+        if (hipMalloc((void **)&k.d_args, sizeof(T) * k.h_args.size()) != hipSuccess) {
+            b->err = "hipMalloc of plan arguments failed";
+            return MDX_ERR_NOMEM;
         }
-        if (h->na) {
-            sc.epart = h->d_epart + (size_t)row * Q_N;
-            k_angles<<<rows_a, 256, 0, st>>>(h->angles, sc);
-            row += rows_a;
-        }
-        if (h->nt) {
-            sc.epart = h->d_epart + (size_t)row * Q_N;
-            k_tors<<<rows_t, 256, 0, st>>>(h->tors, sc);
-            row += rows_t;
-        }
+        k.cap_args = k.h_args.size();
     }
-    mark(8);
-    CombineArgs ca;
-    ca.n = n;
-    ca.group = group;
-    ca.fsx = h->d_fs; ca.fsy = h->d_fs + n; ca.fsz = h->d_fs + 2 * n;
-    ca.flx = h->d_fl; ca.fly = h->d_fl + n; ca.flz = h->d_fl + 2 * n;
-    ca.recip = h->d_rpart;
-    ca.nslice = group != 1 ? h->nslice : 0;
-    ca.gx = fixG; ca.gy = fixG + n; ca.gz = fixG + 2 * n;
-    ca.hlx = fixHL; ca.hly = fixHL + n; ca.hlz = fixHL + 2 * n;
-    ca.hbx = fixHB; ca.hby = fixHB + n; ca.hbz = fixHB + 2 * n;
-    ca.inv_scale = 1.0 / h->fscale;
-    ca.x = h->d_x; ca.y = h->d_y; ca.z = h->d_z;
-    ca.atom_mol = h->d_mol;
-    ca.cx = h->d_cx; ca.cy = h->d_cy; ca.cz = h->d_cz;
-    ca.ox = dF; ca.oy = dF + n; ca.oz = dF + 2 * n;
-    ca.Gx = dG; ca.Gy = dG ? dG + n : nullptr; ca.Gz = dG ? dG + 2 * n : nullptr;
-    ca.Hx = dH; ca.Hy = dH ? dH + n : nullptr; ca.Hz = dH ? dH + 2 * n : nullptr;
-    ca.epart = h->d_epart + (size_t)row * Q_N;
-    k_combine<<<rows_c, 256, 0, st>>>(ca);
-    row += rows_c;
-    k_ereduce<<<Q_N, 256, 0, st>>>(row, h->d_epart, h->d_energy);
-    mark(9);
-    HIPCHK(h, hipGetLastError());
-    if (e) {
-        double q[Q_N];
-        HIPCHK(h, hipMemcpyAsync(q, h->d_energy, sizeof(q), hipMemcpyDeviceToHost, st));
-        HIPCHK(h, hipStreamSynchronize(st));
-        std::memset(e, 0, sizeof(*e));
-        e->pe1 = q[Q_PE1]; e->pe2 = q[Q_PE2]; e->pels1 = q[Q_PELS1]; e->pels2 = q[Q_PELS2];
-        e->spe = q[Q_SPE] + (group != 1 ? h->spei : 0.0);
-        e->pint = q[Q_PINT];
-        e->vir1 = q[Q_VIR1]; e->vir2 = q[Q_VIR2]; e->virb = q[Q_VIRB];
-        e->virx = q[Q_VIRX]; e->viry = q[Q_VIRY]; e->virz = q[Q_VIRZ];
-        e->virfx = q[Q_VIRFX]; e->virfy = q[Q_VIRFY]; e->virfz = q[Q_VIRFZ];
-        e->wirs = group != 1 ? q[Q_WIRS] : 0.0;
-        e->wirss = group != 2 ? q[Q_WIRSS] : 0.0;
-        e->npairs_short = (int64_t)std::llround(q[Q_NPS]);
-        e->npairs_long = (int64_t)std::llround(q[Q_NPL]);
-        for (int t = 0; t < h->ntypes && group != 1; ++t) e->spee[t] = h->spee_self[t];
+    if (hipMemcpy(k.d_args, k.h_args.data(), sizeof(T) * k.h_args.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        b->err = "plan upload failed";
+        return MDX_ERR_HIP;
     }
-    if (h->timing) {
-        HIPCHK(h, hipEventSynchronize(h->ev[9]));
-        for (int k = ST_PREP; k <= ST_REDUCE; ++k) {
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, h->ev[k - 1], h->ev[k]);
-            h->stage_ms[k] = ms;
+    if (maps) {
+        if (k.h_map.size() > k.cap_map) {
+            if (k.d_map) hipFree(k.d_map);
+            k.d_map = nullptr;
+            if (hipMalloc((void **)&k.d_map, sizeof(int2) * k.h_map.size()) != hipSuccess) {
+                b->err = "hipMalloc of plan block map failed";
+                return MDX_ERR_NOMEM;
+            }
+            k.cap_map = k.h_map.size();
+        }
+        if (!k.h_map.empty() &&
+            hipMemcpy(k.d_map, k.h_map.data(), sizeof(int2) * k.h_map.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            b->err = "plan upload failed";
+            return MDX_ERR_HIP;
         }
     }
     return MDX_OK;
 }
 
-mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want, double *fx, double *fy, double *fz, double *gx,
-                       double *gy, double *gz, double *hx, double *hy, double *hz, mdx_energies *e) {
-    if (!h) return MDX_ERR_ARG;
-    const int n = h->n;
-    double *dF = h->d_out, *dG = h->d_out + 3 * (size_t)n, *dH = h->d_out + 6 * (size_t)n;
-    CHK(compute_impl(h, group, want, dF, (gx || gy || gz) ? dG : nullptr, (hx || hy || hz) ? dH : nullptr, e));
-    hipStream_t st = h->stream;
+template <typename T>
+void kplan_free(KPlan<T> &k) {
+    if (k.d_args) hipFree(k.d_args);
+    if (k.d_map) hipFree(k.d_map);
+    k.d_args = nullptr;
+    k.d_map = nullptr;
+}
+
+void drop_graphs(mdx_batch *b) {
+    for (int g = 0; g < 3; ++g) {
+        if (b->exec[g]) hipGraphExecDestroy(b->exec[g]);
+        if (b->graph[g]) hipGraphDestroy(b->graph[g]);
+        b->exec[g] = nullptr;
+        b->graph[g] = nullptr;
+    }
+}
+
+// Argument arrays of every kernel from the systems' current structs.  With
+// maps = true the block maps are rebuilt too (first use, box changes).
+mdx_status plan_fill(mdx_batch *b, bool maps) {
+    const int K = (int)b->sys.size();
+    b->zero.clear(); b->pack.clear(); b->pairs.clear(); b->special.clear();
+    b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
+    b->et.clear(); b->bonds.clear(); b->angles.clear(); b->tors.clear();
+    b->springs.clear(); b->comb.clear(); b->red.clear();
+    b->maxlen = 16;
+    b->nlj_max = 1;
+    b->soft = false;
+    for (int s = 0; s < K; ++s) {
+        mdx_engine *h = b->sys[s];
+        const int n = h->n;
+        b->maxlen = std::max(b->maxlen, h->maxlen);
+        b->nlj_max = std::max(b->nlj_max, h->nlj);
+        b->soft = b->soft || h->p.ee_enabled;
+        const long long nz1 = (long long)h->rows_total * Q_N;
+        b->zero.add(ZeroSeg{(unsigned long long *)h->d_epart, nz1}, cdiv(nz1, 256));
+        b->zero.add(ZeroSeg{h->d_fix, 9LL * n}, cdiv(9LL * n, 256));
+        b->zero.add(ZeroSeg{h->d_fixsp, 3LL * n}, cdiv(3LL * n, 256));
+        b->pack.add(h->a_pack, cdiv((long long)h->nclu_cap * kClu, 256) + cdiv(h->nmol, 256));
+        b->pairs.add(h->a_pair, h->nclu_cap);
+        if (h->nspecial) b->special.add(h->a_special, cdiv(h->nspecial, 256));
+        b->rprep.add(h->a_recip, cdiv(n, 256));
+        b->sk.add(h->a_recip, h->a_recip.nrowgroups * h->nchunk);
+        b->skred.add(h->a_recip, cdiv(std::max(1, h->nkv), 256));
+        b->rforce.add(h->a_recip, h->a_recip.natom_blocks * h->nslice);
+        if (h->neterm) b->et.add(h->a_et, cdiv(h->neterm, 256));
+        if (h->nb) b->bonds.add(h->a_bonds, cdiv(h->nb, 256));
+        if (h->na) b->angles.add(h->a_angles, cdiv(h->na, 256));
+        if (h->nt) b->tors.add(h->a_tors, cdiv(h->nt, 256));
+        if (b->ring && K > 1) {
+            mdx_engine *hp = b->sys[(s + 1) % K], *hm = b->sys[(s + K - 1) % K];
+            SpringArgs sp;
+            sp.n = n;
+            sp.x = h->d_x; sp.y = h->d_y; sp.z = h->d_z;
+            sp.xp = hp->d_x; sp.yp = hp->d_y; sp.zp = hp->d_z;
+            sp.xm = hm->d_x; sp.ym = hm->d_y; sp.zm = hm->d_z;
+            sp.mass = h->d_mass;
+            sp.beadfac = b->beadfac;
+            sp.nbeads = (double)K;
+            sp.box = h->box;
+            sp.fscale = h->fscale;
+            sp.fx = h->d_fixsp; sp.fy = h->d_fixsp + n; sp.fz = h->d_fixsp + 2 * n;
+            sp.epart = erow(h, h->row_spr);
+            b->springs.add(sp, cdiv(n, 256));
+        }
+        CombineArgs ca;
+        unsigned long long *fixG = h->d_fix, *fixHL = h->d_fix + 3 * (size_t)n, *fixHB = h->d_fix + 6 * (size_t)n;
+        ca.n = n;
+        ca.scale = (b->ring && K > 1) ? 1.0 / K : 1.0;
+        ca.fsx = h->d_fs; ca.fsy = h->d_fs + n; ca.fsz = h->d_fs + 2 * n;
+        ca.flx = h->d_fl; ca.fly = h->d_fl + n; ca.flz = h->d_fl + 2 * n;
+        ca.recip = h->d_rpart;
+        ca.nslice = h->nslice;
+        ca.gx = fixG; ca.gy = fixG + n; ca.gz = fixG + 2 * n;
+        ca.hlx = fixHL; ca.hly = fixHL + n; ca.hlz = fixHL + 2 * n;
+        ca.hbx = fixHB; ca.hby = fixHB + n; ca.hbz = fixHB + 2 * n;
+        const bool spr = b->ring && K > 1;
+        ca.spx = spr ? h->d_fixsp : nullptr;
+        ca.spy = spr ? h->d_fixsp + n : nullptr;
+        ca.spz = spr ? h->d_fixsp + 2 * n : nullptr;
+        ca.inv_scale = 1.0 / h->fscale;
+        ca.x = h->d_x; ca.y = h->d_y; ca.z = h->d_z;
+        ca.atom_mol = h->d_mol;
+        ca.cx = h->d_cx; ca.cy = h->d_cy; ca.cz = h->d_cz;
+        ca.ox = h->d_out; ca.oy = h->d_out + n; ca.oz = h->d_out + 2 * n;
+        ca.Gx = h->d_out + 3 * (size_t)n; ca.Gy = ca.Gx + n; ca.Gz = ca.Gx + 2 * n;
+        ca.Hx = h->d_out + 6 * (size_t)n; ca.Hy = ca.Hx + n; ca.Hz = ca.Hx + 2 * n;
+        ca.epart = erow(h, h->row_c);
+        b->comb.add(ca, cdiv(n, 256));
+        b->red.add(ReduceArgs{h->rows_total, h->d_epart, b->d_energy + (size_t)s * Q_N}, Q_N);
+    }
+    if (b->maxlen > 96) {
+        b->err = "k-vector rows longer than 96 are not supported";
+        return MDX_ERR_NKV;
+    }
+    CHK(kplan_upload(b, b->zero, maps)); CHK(kplan_upload(b, b->pack, maps));
+    CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->special, maps));
+    CHK(kplan_upload(b, b->rprep, maps)); CHK(kplan_upload(b, b->sk, maps));
+    CHK(kplan_upload(b, b->skred, maps)); CHK(kplan_upload(b, b->rforce, maps));
+    CHK(kplan_upload(b, b->et, maps)); CHK(kplan_upload(b, b->bonds, maps));
+    CHK(kplan_upload(b, b->angles, maps)); CHK(kplan_upload(b, b->tors, maps));
+    CHK(kplan_upload(b, b->springs, maps)); CHK(kplan_upload(b, b->comb, maps));
+    CHK(kplan_upload(b, b->red, maps));
+    b->seen.assign(K, 0);
+    for (int s = 0; s < K; ++s) b->seen[s] = b->sys[s]->version;
+    b->planned = true;
+    return MDX_OK;
+}
+
+template <typename T, typename F>
+void launch(mdx_batch *b, KPlan<T> &k, int nthreads, size_t lds, F kernel) {
+    if (k.nblocks() == 0) return;
+    kernel<<<k.nblocks(), nthreads, lds, b->stream>>>(k.d_args, k.d_map);
+}
+
+// The kernel sequence of one force call over all systems of the batch.
+void launch_call(mdx_batch *b, int group, bool timing) {
+    hipStream_t st = b->stream;
+    auto mark = [&](int k) { if (timing) hipEventRecord(b->ev[k], st); };
+    mark(0);
+    launch(b, b->zero, 256, 0, k_zero);
+    launch(b, b->pack, 256, 0, k_pack);
+    mark(1);
+    const size_t lj = sizeof(float2) * b->nlj_max * b->nlj_max;
+    if (b->pairs.nblocks()) {
+        if (b->soft) k_pairs<true><<<b->pairs.nblocks(), 64, lj, st>>>(b->pairs.d_args, b->pairs.d_map, group);
+        else k_pairs<false><<<b->pairs.nblocks(), 64, lj, st>>>(b->pairs.d_args, b->pairs.d_map, group);
+    }
+    mark(2);
+    if (b->special.nblocks())
+        k_special<<<b->special.nblocks(), 256, 0, st>>>(b->special.d_args, b->special.d_map, group);
+    mark(3);
+    if (group != MDX_FORCES_FAST) {
+        launch(b, b->rprep, 256, 0, k_recip_prep);
+        mark(4);
+        if (b->maxlen <= 16) launch(b, b->sk, 64, 0, k_sk<16>);
+        else if (b->maxlen <= 32) launch(b, b->sk, 64, 0, k_sk<32>);
+        else if (b->maxlen <= 64) launch(b, b->sk, 64, 0, k_sk<64>);
+        else launch(b, b->sk, 64, 0, k_sk<96>);
+        mark(5);
+        launch(b, b->skred, 256, 0, k_sk_reduce);
+        mark(6);
+        launch(b, b->rforce, 64, 0, k_recip_force);
+        mark(7);
+        launch(b, b->et, 256, 0, k_eterms);
+    } else {
+        mark(4); mark(5); mark(6); mark(7);
+    }
+    if (group != MDX_FORCES_SLOW) {
+        launch(b, b->bonds, 256, 0, k_bonds);
+        launch(b, b->angles, 256, 0, k_angles);
+        launch(b, b->tors, 256, 0, k_tors);
+        launch(b, b->springs, 256, 0, k_springs);
+    }
+    mark(8);
+    if (b->comb.nblocks()) k_combine<<<b->comb.nblocks(), 256, 0, st>>>(b->comb.d_args, b->comb.d_map, group);
+    launch(b, b->red, 256, 0, k_ereduce);
+    hipMemcpyAsync(b->h_energy, b->d_energy, sizeof(double) * Q_N * b->sys.size(), hipMemcpyDeviceToHost, st);
+    mark(9);
+}
+
+mdx_status batch_compute(mdx_batch *b, int group) {
+    if (group < 0 || group > 2) return MDX_ERR_ARG;
+    for (mdx_engine *h : b->sys)
+        if (!h->have_list) {
+            b->err = "mdx_compute before mdx_build_list";
+            return MDX_ERR_STATE;
+        }
+    // work queued on the systems' own streams (coordinates) comes first
+    for (mdx_engine *h : b->sys)
+        if (h->stream != b->stream) hipStreamSynchronize(h->stream);
+    bool stale = !b->planned;
+    for (size_t s = 0; s < b->sys.size() && !stale; ++s) stale = b->seen[s] != b->sys[s]->version;
+    if (stale) {
+        // launch shapes depend on capacities only; a new box (k set) or a
+        // regrown capacity changes the maps and the captured graphs
+        std::vector<int> before = {b->pairs.nblocks(), b->sk.nblocks(), b->rforce.nblocks(), b->skred.nblocks(),
+                                   b->zero.nblocks(), b->maxlen};
+        CHK(plan_fill(b, true));
+        std::vector<int> after = {b->pairs.nblocks(), b->sk.nblocks(), b->rforce.nblocks(), b->skred.nblocks(),
+                                  b->zero.nblocks(), b->maxlen};
+        if (before != after) drop_graphs(b);
+    }
+    if (b->use_graph && !b->timing) {
+        if (!b->exec[group]) {
+            HIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            launch_call(b, group, false);
+            HIPCHK(b, hipStreamEndCapture(b->stream, &b->graph[group]));
+            HIPCHK(b, hipGraphInstantiate(&b->exec[group], b->graph[group], nullptr, nullptr, 0));
+        }
+        HIPCHK(b, hipGraphLaunch(b->exec[group], b->stream));
+    } else {
+        launch_call(b, group, b->timing);
+    }
+    HIPCHK(b, hipStreamSynchronize(b->stream));
+    HIPCHK(b, hipGetLastError());
+    if (b->timing) {
+        for (int k = ST_PREP; k <= ST_REDUCE; ++k) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, b->ev[k - 1], b->ev[k]);
+            b->stage_ms[k] = ms;
+        }
+    }
+    return MDX_OK;
+}
+
+void fill_energies(const mdx_batch *b, int s, int group, mdx_energies *e) {
+    const mdx_engine *h = b->sys[s];
+    const double *q = b->h_energy + (size_t)s * Q_N;
+    std::memset(e, 0, sizeof(*e));
+    e->pe1 = q[Q_PE1]; e->pe2 = q[Q_PE2]; e->pels1 = q[Q_PELS1]; e->pels2 = q[Q_PELS2];
+    e->spe = q[Q_SPE] + (group != MDX_FORCES_FAST ? h->spei : 0.0);
+    e->pint = q[Q_PINT];
+    e->vir1 = q[Q_VIR1]; e->vir2 = q[Q_VIR2]; e->virb = q[Q_VIRB];
+    e->virx = q[Q_VIRX]; e->viry = q[Q_VIRY]; e->virz = q[Q_VIRZ];
+    e->virfx = q[Q_VIRFX]; e->virfy = q[Q_VIRFY]; e->virfz = q[Q_VIRFZ];
+    e->wirs = group != MDX_FORCES_FAST ? q[Q_WIRS] : 0.0;
+    e->wirss = group != MDX_FORCES_SLOW ? q[Q_WIRSS] : 0.0;
+    e->npairs_short = (int64_t)std::llround(q[Q_NPS]);
+    e->npairs_long = (int64_t)std::llround(q[Q_NPL]);
+    e->qek = q[Q_QEK];
+    e->virpi = q[Q_VIRPI];
+    for (int t = 0; t < h->ntypes && group != MDX_FORCES_FAST; ++t) e->spee[t] = h->spee_self[t];
+}
+
+mdx_status copy_forces(mdx_engine *h, double *fx, double *fy, double *fz, double *gx, double *gy, double *gz,
+                       double *hx, double *hy, double *hz, hipMemcpyKind kind) {
     double *outs[9] = {fx, fy, fz, gx, gy, gz, hx, hy, hz};
     for (int k = 0; k < 9; ++k)
         if (outs[k])
-            HIPCHK(h, hipMemcpyAsync(outs[k], h->d_out + (size_t)k * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipStreamSynchronize(st));
+            HIPCHK(h, hipMemcpyAsync(outs[k], h->d_out + (size_t)k * h->n, sizeof(double) * h->n, kind, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return MDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want, double *fx, double *fy, double *fz, double *gx,
+                       double *gy, double *gz, double *hx, double *hy, double *hz, mdx_energies *e) {
+    (void)want;
+    if (!h) return MDX_ERR_ARG;
+    mdx_batch *b = h->self;
+    b->timing = h->timing;
+    mdx_status s = batch_compute(b, group);
+    if (s != MDX_OK) {
+        h->err = b->err;
+        return s;
+    }
+    for (int k = ST_PREP; k < ST_N; ++k) h->stage_ms[k] = b->stage_ms[k];
+    if (e) fill_energies(b, 0, group, e);
+    return copy_forces(h, fx, fy, fz, gx, gy, gz, hx, hy, hz, hipMemcpyDeviceToHost);
 }
 
 mdx_status mdx_compute_device(mdx_engine *h, int32_t group, int32_t want, double *dfx, double *dfy, double *dfz,
                               mdx_energies *e) {
+    (void)want;
     if (!h) return MDX_ERR_ARG;
-    CHK(compute_impl(h, group, want, h->d_out, nullptr, nullptr, e));
-    const size_t b = sizeof(double) * h->n;
-    if (dfx) HIPCHK(h, hipMemcpyAsync(dfx, h->d_out, b, hipMemcpyDeviceToDevice, h->stream));
-    if (dfy) HIPCHK(h, hipMemcpyAsync(dfy, h->d_out + h->n, b, hipMemcpyDeviceToDevice, h->stream));
-    if (dfz) HIPCHK(h, hipMemcpyAsync(dfz, h->d_out + 2 * (size_t)h->n, b, hipMemcpyDeviceToDevice, h->stream));
-    if (!e) HIPCHK(h, hipStreamSynchronize(h->stream));
+    mdx_batch *b = h->self;
+    b->timing = h->timing;
+    mdx_status s = batch_compute(b, group);
+    if (s != MDX_OK) {
+        h->err = b->err;
+        return s;
+    }
+    for (int k = ST_PREP; k < ST_N; ++k) h->stage_ms[k] = b->stage_ms[k];
+    if (e) fill_energies(b, 0, group, e);
+    return copy_forces(h, dfx, dfy, dfz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       hipMemcpyDeviceToDevice);
+}
+
+mdx_status mdx_get_forces(mdx_engine *h, double *fx, double *fy, double *fz, double *gx, double *gy, double *gz,
+                          double *hx, double *hy, double *hz) {
+    if (!h) return MDX_ERR_ARG;
+    return copy_forces(h, fx, fy, fz, gx, gy, gz, hx, hy, hz, hipMemcpyDeviceToHost);
+}
+
+mdx_status mdx_set_timing(mdx_engine *h, int32_t on) {
+    if (!h) return MDX_ERR_ARG;
+    h->timing = on != 0;
+    h->self->use_graph = !h->timing;
     return MDX_OK;
 }
 
 mdx_status mdx_stage_times(mdx_engine *h, double *ms, int32_t n) {
     if (!h || !ms) return MDX_ERR_ARG;
     for (int k = 0; k < n && k < ST_N; ++k) ms[k] = h->stage_ms[k];
+    return MDX_OK;
+}
+
+mdx_status mdx_memory_bytes(const mdx_engine *h, int64_t *bytes) {
+    if (!h || !bytes) return MDX_ERR_ARG;
+    int64_t t = 0;
+    for (auto &kv : h->dev_sizes) t += (int64_t)kv.second;
+    *bytes = t;
+    return MDX_OK;
+}
+
+mdx_status mdx_batch_create(mdx_engine *const *systems, int32_t k, mdx_batch **out) {
+    if (!systems || k <= 0 || !out) return MDX_ERR_ARG;
+    mdx_batch *b = new mdx_batch();
+    for (int s = 0; s < k; ++s) {
+        if (!systems[s]) {
+            delete b;
+            return MDX_ERR_ARG;
+        }
+        b->sys.push_back(systems[s]);
+    }
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
+        delete b;
+        return MDX_ERR_HIP;
+    }
+    b->own_stream = true;
+    for (int e = 0; e < 12; ++e) hipEventCreate(&b->ev[e]);
+    if (hipMalloc((void **)&b->d_energy, sizeof(double) * Q_N * k) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_energy, sizeof(double) * Q_N * k, hipHostMallocDefault) != hipSuccess) {
+        *out = b;
+        return MDX_ERR_NOMEM;
+    }
+    *out = b;
+    return MDX_OK;
+}
+
+mdx_status mdx_batch_set_ring(mdx_batch *b, double beadfac) {
+    if (!b) return MDX_ERR_ARG;
+    const int n0 = b->sys[0]->n;
+    for (mdx_engine *h : b->sys)
+        if (h->n != n0) {
+            b->err = "path-integral beads must have the same atoms";
+            return MDX_ERR_ARG;
+        }
+    b->ring = true;
+    b->beadfac = beadfac;
+    b->planned = false;
+    drop_graphs(b);
+    return MDX_OK;
+}
+
+mdx_status mdx_batch_use_graph(mdx_batch *b, int32_t on) {
+    if (!b) return MDX_ERR_ARG;
+    b->use_graph = on != 0;
+    return MDX_OK;
+}
+
+mdx_status mdx_batch_compute(mdx_batch *b, int32_t group, int32_t want, mdx_energies *e) {
+    (void)want;
+    if (!b) return MDX_ERR_ARG;
+    CHK(batch_compute(b, group));
+    if (e)
+        for (size_t s = 0; s < b->sys.size(); ++s) fill_energies(b, (int)s, group, e + s);
+    return MDX_OK;
+}
+
+const char *mdx_batch_last_error(const mdx_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+mdx_status mdx_batch_destroy(mdx_batch *b) {
+    if (!b) return MDX_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->pairs); kplan_free(b->special);
+    kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
+    kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->angles); kplan_free(b->tors);
+    kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red);
+    if (b->d_energy) hipFree(b->d_energy);
+    if (b->h_energy) hipHostFree(b->h_energy);
+    for (int e = 0; e < 12; ++e)
+        if (b->ev[e]) hipEventDestroy(b->ev[e]);
+    if (b->own_stream && b->stream) hipStreamDestroy(b->stream);
+    delete b;
     return MDX_OK;
 }
 

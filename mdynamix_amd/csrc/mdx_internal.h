@@ -4,8 +4,8 @@
 //   atoms      FP64 SoA x/y/z as given by the caller (reference coordinates)
 //   clusters   8 atoms each, built at every list rebuild from x-y columns
 //              sorted by z; per cluster an FP64 anchor and per slot a packed
-//              float4 (x-anchor, y-anchor, z-anchor, q) plus an int info word
-//              (Lennard-Jones type | ee flag << 16); empty slots hold q = 0 and
+//              float4 (x-anchor, y-anchor, z-anchor, q) plus an int2 (LJ type
+//              | ee flag << 16, low part of q); empty slots hold q = 0 and
 //              never appear in a pair mask
 //   pair list  full cluster-pair list (each i-cluster owns its j-clusters),
 //              32 bytes per entry: j-cluster, FP32 image shift, 64-bit mask
@@ -15,9 +15,17 @@
 //              intramolecular pairs of lambda-coupled molecules, classified
 //              short/long at every rebuild
 //   forces     pair forces FP64 per atom (plain stores, one owner per atom);
-//              scattered terms (bonded, specials, exclusion corrections)
-//              in 64-bit fixed point with integer atomics, so every result is
-//              independent of execution order
+//              scattered terms (bonded, specials, exclusion corrections,
+//              bead springs) in 64-bit fixed point with integer atomics, so
+//              every result is independent of execution order
+//
+// Every kernel is written as a device body taking its argument struct and a
+// block index; the __global__ entry points read the struct of their system
+// from an array in HBM through a block map (system, local block), so one
+// launch covers every system of a batch (PIMD beads, expanded-ensemble
+// jobs) and a single system is a batch of one.  Counts that change at a
+// list rebuild live in the argument structs, launch shapes use capacities,
+// and a captured hipGraph stays valid across rebuilds.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,7 +44,8 @@ constexpr int kWave = 64;          // wavefront size on gfx950
 // Energy/virial slots of the per-block partial sums.
 enum EQ {
     Q_PE1 = 0, Q_PE2, Q_PELS1, Q_PELS2, Q_SPE, Q_PINT, Q_VIR1, Q_VIR2, Q_VIRB,
-    Q_VIRX, Q_VIRY, Q_VIRZ, Q_VIRFX, Q_VIRFY, Q_VIRFZ, Q_WIRS, Q_WIRSS, Q_NPS, Q_NPL, Q_N
+    Q_VIRX, Q_VIRY, Q_VIRZ, Q_VIRFX, Q_VIRFY, Q_VIRFZ, Q_WIRS, Q_WIRSS, Q_NPS, Q_NPL,
+    Q_QEK, Q_VIRPI, Q_N
 };
 
 struct PairEntry {          // 32 bytes, read with scalar loads
@@ -53,6 +62,10 @@ struct Box {
     double l[3], h[3];      // edges and half edges
 };
 
+struct RecipRow {           // 16 bytes
+    int nx, ny, nz0, len;
+};
+
 __host__ __device__ inline double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
 
 // Single-shift minimum image of the reference pair loops (forces.f:1841-1867).
@@ -67,5 +80,171 @@ __device__ inline void fx_add(unsigned long long *p, double v, double scale) {
     long long q = __double2ll_rn(v * scale);
     atomicAdd(p, (unsigned long long)q);
 }
+
+// ---------------------------------------------------------------------
+// Argument structs of the force-call kernels (one per system).
+
+struct ZeroSeg {                 // memory to clear before a call
+    unsigned long long *p;
+    long long n;                 // 8-byte words
+};
+
+struct PackArgs {
+    int nclu;
+    const int *clu_atom;
+    const double *anchor;
+    const double *x, *y, *z, *q;
+    Box box;
+    float4 *xq;
+    // molecular centres (skipped when user_com)
+    int nmol, user_com;
+    const int *mstart;
+    const double *mass;
+    double *cx, *cy, *cz;
+};
+
+struct PairArgs {
+    int nclu;
+    const PairEntry *list;
+    const int *nlist;
+    int list_cap;
+    const float4 *xq;            // per slot: x, y, z relative to anchor; q (FP32)
+    const int2 *aux;             // per slot: LJ type | ee << 16; q - (float)q as float bits
+    const double *q;             // per atom charge (FP64)
+    float coulf_f;               // coulomb factor (FP32, forces)
+    double coulf;                // coulomb factor (FP64, energies)
+    const float2 *lj;            // nlj x nlj: (A6, B12)
+    int nlj;
+    const int *clu_atom;
+    float alpha;
+    float rc2;                   // evaluation cutoff^2 (Verlet) or +inf (reference)
+    float raex;
+    int soft;                    // lambda-coupled types present
+    int wrap;                    // small box: re-apply the minimum image per pair
+    float lx, ly, lz, ilx, ily, ilz;
+    double *fs_x, *fs_y, *fs_z;  // short-class pair forces (per atom)
+    double *fl_x, *fl_y, *fl_z;  // long-class pair forces (per atom)
+    double *epart;               // [nclu_cap][Q_N]
+};
+
+struct SpecialArgs {
+    int ns;
+    const int2 *pairs;
+    const signed char *cls;
+    const double *x, *y, *z;
+    const int *atom_moltype, *atom_ljtype;
+    const double *q0;            // CHARGE(NSITE)
+    const int *type_ee;
+    const double *c14el, *c14lj;
+    const double *a6_0, *b12_0, *a6_14, *b12_14;
+    int nlj;
+    double coulf, alpha, rc2;    // rc2: evaluation cutoff (Verlet) or +inf
+    Box box;
+    double fscale;
+    unsigned long long *fixh_x, *fixh_y, *fixh_z;   // short / 1-4 -> H (LOCALF)
+    unsigned long long *fixg_x, *fixg_y, *fixg_z;   // long -> G (FORCES)
+    double *epart;
+};
+
+struct RecipArgs {
+    int natoms;
+    const double *x, *y, *z, *q;
+    Box box;
+    double4 *fq;                 // fractional coordinates and charge
+    double4 *stepz;              // exp(2 pi i f_z), exp(16 pi i f_z)
+    int nrows, nrowgroups, chunk, nchunk, nkv;
+    const RecipRow *rows;
+    const int *rowoff;
+    const int *wavelen;
+    float2 *part;                // [nchunk][nkv]
+    const int4 *kidx;
+    double alpha, coulf;
+    float2 *ak;                  // 2 AKC S(k)
+    double2 *sk;
+    int nslice, rows_per_slice, natom_blocks;
+    double *rpart;               // [nslice][3][natoms]
+    double *epart;               // rows of k_sk_reduce
+};
+
+struct ScatterArgs {
+    const double *x, *y, *z;
+    Box box;
+    double fscale;
+    unsigned long long *fx, *fy, *fz;
+    double *epart;
+};
+
+struct EtermArgs {
+    int np;
+    const int2 *pairs;
+    const double *q;
+    double coulf, alpha;
+    ScatterArgs s;
+};
+
+struct BondArgs {
+    int n;
+    const int *bi, *bj, *bpar;
+    const int *morse;
+    const double *r0, *k, *d, *rho;
+    ScatterArgs s;
+};
+
+struct AngleArgs {
+    int n;
+    const int *ai, *aj, *ak, *apar;
+    const double *theta0, *k;
+    ScatterArgs s;
+};
+
+struct TorsArgs {
+    int n;
+    const int *ti, *tj, *tk, *tl, *tpar;
+    const int *kind, *mult;
+    const double *k, *delta, *k1, *k2, *k3, *k4, *k5;
+    ScatterArgs s;
+};
+
+// Path-integral ring: harmonic springs to the neighbouring beads
+// (PIFORCES, pimd.f:321-394), into the fast fixed-point forces.
+struct SpringArgs {
+    int n;
+    const double *x, *y, *z;               // this bead
+    const double *xp, *yp, *zp;            // bead + 1
+    const double *xm, *ym, *zm;            // bead - 1
+    const double *mass;
+    double beadfac, nbeads;
+    Box box;
+    double fscale;
+    unsigned long long *fx, *fy, *fz;
+    double *epart;
+};
+
+struct CombineArgs {
+    int n;
+    double scale;                  // 1 / P for path-integral beads, else 1
+    const double *fsx, *fsy, *fsz; // pair short
+    const double *flx, *fly, *flz; // pair long
+    const double *recip;           // [nslice][3][n]
+    int nslice;
+    const unsigned long long *gx, *gy, *gz;    // fixed: slow scatter (ETERMS, long specials)
+    const unsigned long long *hlx, *hly, *hlz; // fixed: LOCALF specials
+    const unsigned long long *hbx, *hby, *hbz; // fixed: bonded
+    const unsigned long long *spx, *spy, *spz; // fixed: bead springs (unscaled) or null
+    double inv_scale;
+    const double *x, *y, *z;
+    const int *atom_mol;
+    const double *cx, *cy, *cz;
+    double *ox, *oy, *oz;          // F of the group
+    double *Gx, *Gy, *Gz;          // slow (optional)
+    double *Hx, *Hy, *Hz;          // fast (optional)
+    double *epart;
+};
+
+struct ReduceArgs {
+    int nrows;
+    const double *epart;
+    double *out;                   // Q_N
+};
 
 }  // namespace mdx

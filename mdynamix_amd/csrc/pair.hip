@@ -1,9 +1,9 @@
 // Real-space pair forces (replaces LOCALF forces.f:813-998 and FORCES
 // forces.f:1014-1202).
 //
-// k_pairs: one wavefront per i-cluster of 8 atoms; each iteration takes one
-// j-cluster of its list and lane (i_slot * 8 + j_slot) evaluates one atom
-// pair in FP32: Lennard-Jones plus the Abramowitz-Stegun erfc-screened
+// pairs_body: one wavefront per i-cluster of 8 atoms; each iteration takes
+// one j-cluster of its list and lane (i_slot * 8 + j_slot) evaluates one
+// atom pair in FP32: Lennard-Jones plus the Abramowitz-Stegun erfc-screened
 // Ewald real-space term (the reference's polynomial, prcm.h:27-29, with the
 // exact Gaussian in the force, forces.f:1146-1153).  Distance vectors are
 // formed from anchor-relative FP32 coordinates plus the cluster pair's
@@ -11,54 +11,70 @@
 // The list is a full list (no Newton's third law): every atom's force is
 // summed by the one wavefront that owns it, in a fixed order, with FP32
 // partial sums over 16 j-clusters flushed into FP64; energies and virials
-// (halved for the double visit) go to per-wavefront FP64 partials.  Pairs
-// keep the short/long class of the list build: short pairs feed the fast
-// forces H, long pairs the slow forces G.
+// (halved for the double visit) go to per-wavefront FP64 partials.  The
+// electrostatic energy is accumulated per unit q_i C with the exact q_j
+// (FP32 high part plus low part) and multiplied by the FP64 q_i C at the
+// end, so no charge rounding biases the cancelling sums.  Pairs keep the
+// short/long class of the list build: short pairs feed the fast forces H,
+// long pairs the slow forces G.
 //
-// k_special: the 1-4 and lambda-coupled intramolecular pairs in FP64 with
-// the reference's exact expressions (direct Coulomb, 1-4 scaling).
+// special_body: the 1-4 and lambda-coupled intramolecular pairs in FP64
+// with the reference's exact expressions (direct Coulomb, 1-4 scaling).
 #include "mdx_internal.h"
 
 namespace mdx {
 
-struct PairArgs {
-    const PairEntry *list;
-    const int *nlist;
-    int list_cap;
-    const float4 *xq;            // per slot: x, y, z relative to anchor; q (FP32)
-    const int2 *aux;             // per slot: LJ type | ee << 16; q - (float)q as float bits
-    const double *q;             // per atom charge (FP64)
-    float coulf_f;               // coulomb factor (FP32, forces)
-    double coulf;                // coulomb factor (FP64, energies)
-    const float2 *lj;            // nlj x nlj: (A6, B12)
-    int nlj;
-    const int *clu_atom;
-    float alpha;
-    float rc2;                   // evaluation cutoff^2 (Verlet) or +inf (reference)
-    float raex;
-    int wrap;                    // small box: re-apply the minimum image per pair
-    float lx, ly, lz, ilx, ily, ilz;
-    unsigned long long group_mask;   // restricts to short (FAST) or long (SLOW)
-    int group;                   // 0 ALL, 1 FAST (short only), 2 SLOW (long only)
-    double *fs_x, *fs_y, *fs_z;  // short-class pair forces (per atom)
-    double *fl_x, *fl_y, *fl_z;  // long-class pair forces (per atom)
-    double *epart;               // [nclu][Q_N]
-};
+// Anchor-relative packed coordinates of every cluster slot, then the
+// molecular centres of mass (GETCOM, service.f:471-500).
+__device__ inline void pack_body(const PackArgs &a, int b) {
+    const int nslot = a.nclu * kClu;
+    const int npack = (nslot + 255) / 256;
+    if (b < npack) {
+        const int s = b * 256 + threadIdx.x;
+        if (s >= nslot) return;
+        const int at = a.clu_atom[s];
+        const int c = s / kClu;
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (at >= 0) {
+            double d[3] = {a.x[at] - a.anchor[c * 3], a.y[at] - a.anchor[c * 3 + 1], a.z[at] - a.anchor[c * 3 + 2]};
+            for (int k = 0; k < 3; ++k) d[k] -= a.box.l[k] * rint(d[k] / a.box.l[k]);
+            o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)a.q[at]);
+        }
+        a.xq[s] = o;
+        return;
+    }
+    if (a.user_com) return;
+    const int m = (b - npack) * 256 + threadIdx.x;
+    if (m >= a.nmol) return;
+    double sx = 0, sy = 0, sz = 0, sm = 0;
+    for (int i = a.mstart[m]; i < a.mstart[m + 1]; ++i) {
+        sx = sx + a.mass[i] * a.x[i];
+        sy = sy + a.mass[i] * a.y[i];
+        sz = sz + a.mass[i] * a.z[i];
+        sm = sm + a.mass[i];
+    }
+    a.cx[m] = sx / sm;
+    a.cy[m] = sy / sm;
+    a.cz[m] = sz / sm;
+}
+
+__global__ __launch_bounds__(256) void k_pack(const PackArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    pack_body(args[m.x], m.y);
+}
 
 template <bool SOFT>
-__global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
-    __shared__ float2 s_lj[32 * 32];
-    const int ci = blockIdx.x;
+__device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int group) {
     const int lane = threadIdx.x;
-    const int il = lane >> 3;
     const int nlj2 = a.nlj * a.nlj;
     for (int t = lane; t < nlj2; t += 64) s_lj[t] = a.lj[t];
     __syncthreads();
+    if (ci >= a.nclu) return;
 
+    const int il = lane >> 3;
     const float4 pi = a.xq[ci * kClu + il];
     const int infoi = a.aux[ci * kClu + il].x;
     const int ti = infoi & 0xffff;
-    const float qic = pi.w * a.coulf_f;
     const float2 *ljrow = s_lj + ti * a.nlj;
     const int jl = lane & 7;
     const float A1 = (float)kA1, A2 = (float)kA2, A3 = (float)kA3, A4 = (float)kA4, A5 = (float)kA5;
@@ -66,9 +82,10 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
     const float alpha = a.alpha;
     const float nal2 = -alpha * alpha * 1.44269504088896341f;   // -alpha^2 log2(e)
     const float talpha = TOT * alpha;
+    const float qic = pi.w * a.coulf_f;
 
     double Fs[3] = {0, 0, 0}, Fl[3] = {0, 0, 0};
-    double E[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l, (unused)
+    double E[6] = {0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
     double Vs[3] = {0, 0, 0}, Vl[3] = {0, 0, 0};
     int cnt_s = 0, cnt_l = 0;
     const int n = a.nlist[ci];
@@ -81,8 +98,8 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
         for (int k = k0; k < k1; ++k) {
             const PairEntry e = L[k];
             unsigned long long m = e.mask;
-            if (a.group == 1) m &= e.mshort;
-            else if (a.group == 2) m &= ~e.mshort;
+            if (group == 1) m &= e.mshort;
+            else if (group == 2) m &= ~e.mshort;
             if (m == 0ull) continue;
             const float4 pj = a.xq[e.cj * kClu + jl];
             const int2 auxj = a.aux[e.cj * kClu + jl];
@@ -111,8 +128,6 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
             const float g = __builtin_amdgcn_exp2f(nal2 * r2);
             const float erfct = ((((A5 * t + A4) * t + A3) * t + A2) * t + A1) * t * g;
             const float ee1 = qic * pj.w * rei;
-            // energy per unit q_i C with the exact q_j = hi + lo (the i side
-            // is applied in FP64 at the end): no systematic charge rounding
             const float phi = erfct * rei;
             const float ees = pj.w * phi + __int_as_float(auxj.y) * phi;
             const float2 lj = ljrow[infoj & 0xffff];
@@ -175,25 +190,16 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
     }
 }
 
-template __global__ void k_pairs<false>(PairArgs);
-template __global__ void k_pairs<true>(PairArgs);
-
-// Anchor-relative packed coordinates of every cluster slot (each step).
-__global__ void k_pack(int nslot, const int *__restrict__ clu_atom, const double *__restrict__ anchor,
-                       const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-                       const double *__restrict__ q, Box b, float4 *__restrict__ xq) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslot) return;
-    const int a = clu_atom[s];
-    const int c = s / kClu;
-    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (a >= 0) {
-        double d[3] = {x[a] - anchor[c * 3], y[a] - anchor[c * 3 + 1], z[a] - anchor[c * 3 + 2]};
-        for (int k = 0; k < 3; ++k) d[k] -= b.l[k] * rint(d[k] / b.l[k]);
-        o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)q[a]);
-    }
-    xq[s] = o;
+template <bool SOFT>
+__global__ __launch_bounds__(64) void k_pairs(const PairArgs *__restrict__ args, const int2 *__restrict__ map,
+                                              int group) {
+    extern __shared__ float2 s_lj[];      // nlj x nlj (A6, B12)
+    const int2 m = map[blockIdx.x];
+    pairs_body<SOFT>(args[m.x], m.y, s_lj, group);
 }
+
+template __global__ void k_pairs<false>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<true>(const PairArgs *, const int2 *, int);
 
 // Per-slot LJ type, ee flag and the low part of the charge (at list build).
 __global__ void k_slot_info(int nslot, const int *__restrict__ clu_atom, const int *__restrict__ atom_ljtype,
@@ -211,36 +217,15 @@ __global__ void k_slot_info(int nslot, const int *__restrict__ clu_atom, const i
     aux[s] = v;
 }
 
-struct SpecialArgs {
-    int ns;
-    const int2 *pairs;
-    const signed char *cls;
-    const double *x, *y, *z;
-    const int *atom_moltype, *atom_ljtype;
-    const double *q0;            // CHARGE(NSITE)
-    const int *type_ee;
-    const double *c14el, *c14lj;
-    const double *a6_0, *b12_0, *a6_14, *b12_14;
-    int nlj;
-    double coulf, alpha, rc2;    // rc2: evaluation cutoff (Verlet) or +inf
-    Box box;
-    double fscale;
-    unsigned long long *fixh_x, *fixh_y, *fixh_z;   // short / 1-4 -> H (LOCALF)
-    unsigned long long *fixg_x, *fixg_y, *fixg_z;   // long -> G (FORCES)
-    int group;
-    double *epart;               // [gridDim][Q_N]
-};
-
 // One special pair per thread, reference expressions in FP64: LOCALF
 // (forces.f:835-986) for the short classes, FORCES (1039-1198) for long.
-__global__ __launch_bounds__(256) void k_special(SpecialArgs a) {
-    __shared__ double red[256];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void special_body(const SpecialArgs &a, int blk, double *red, int group) {
+    const int k = blk * blockDim.x + threadIdx.x;
     double q[Q_N];
     for (int i = 0; i < Q_N; ++i) q[i] = 0.0;
     if (k < a.ns) {
         const int c = a.cls[k];
-        const bool use = c != SP_NONE && (a.group == 0 || (a.group == 1) == (c != SP_LONG));
+        const bool use = c != SP_NONE && (group == 0 || (group == 1) == (c != SP_LONG));
         if (use) {
             const int isp = a.pairs[k].x, jsp = a.pairs[k].y;   // isp > jsp
             const int ityp = a.atom_moltype[isp];
@@ -262,28 +247,18 @@ __global__ __launch_bounds__(256) void k_special(SpecialArgs a) {
             if (c == SP_SHORT14 || rr <= a.rc2) {
                 const double r1 = sqrt(rr);
                 const bool localf = c != SP_LONG;
-                double ee1, ees, force;
-                const bool direct = c == SP_SHORT14 || a.type_ee[ityp] == 0;
-                if (direct) {
-                    if (localf) {
-                        const double r1i = 1.0 / r1;
-                        ee1 = qij * r1i;
-                        force = ee1 * (r1i * r1i);
-                    } else {
-                        ee1 = qij / r1;
-                        force = ee1 / (r1 * r1);
-                    }
-                    ees = ee1;
-                } else {
+                double ee1, force;
+                // special pairs are 1-4 pairs or pairs inside a lambda-coupled
+                // molecule: direct Coulomb (forces.f:917-924, 1127-1134)
+                if (localf) {
                     const double r1i = 1.0 / r1;
-                    const double alphar = a.alpha * r1;
-                    const double ttt = 1.0 / (1.0 + kB1 * alphar);
-                    const double exp2a = exp(-(alphar * alphar));
-                    const double erfct = ((((kA5 * ttt + kA4) * ttt + kA3) * ttt + kA2) * ttt + kA1) * ttt * exp2a;
                     ee1 = qij * r1i;
-                    ees = ee1 * erfct;
-                    force = ee1 * (kTOTPI * a.alpha * exp2a + erfct * r1i) * r1i;
+                    force = ee1 * (r1i * r1i);
+                } else {
+                    ee1 = qij / r1;
+                    force = ee1 / (r1 * r1);
                 }
+                const double ees = ee1;
                 const double r1i = 1.0 / r1;
                 const double r2i = r1i * r1i;
                 const double r6i = r2i * r2i * r2i;
@@ -321,9 +296,16 @@ __global__ __launch_bounds__(256) void k_special(SpecialArgs a) {
             if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
             __syncthreads();
         }
-        if (threadIdx.x == 0) a.epart[(size_t)blockIdx.x * Q_N + i] = red[0];
+        if (threadIdx.x == 0) a.epart[(size_t)blk * Q_N + i] = red[0];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void k_special(const SpecialArgs *__restrict__ args, const int2 *__restrict__ map,
+                                                 int group) {
+    __shared__ double red[256];
+    const int2 m = map[blockIdx.x];
+    special_body(args[m.x], m.y, red, group);
 }
 
 }  // namespace mdx

@@ -18,17 +18,11 @@
 
 namespace mdx {
 
-struct RecipRow {           // 16 bytes
-    int nx, ny, nz0, len;
-};
-
-__global__ void k_recip_prep(int n, const double *__restrict__ x, const double *__restrict__ y,
-                             const double *__restrict__ z, const double *__restrict__ q, Box b,
-                             double4 *__restrict__ fq, double4 *__restrict__ stepz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double fx = x[i] / b.l[0], fy = y[i] / b.l[1], fz = z[i] / b.l[2];
-    fq[i] = make_double4(fx, fy, fz, q[i]);
+__device__ inline void recip_prep_body(const RecipArgs &a, int blk) {
+    const int i = blk * 256 + threadIdx.x;
+    if (i >= a.natoms) return;
+    const double fx = a.x[i] / a.box.l[0], fy = a.y[i] / a.box.l[1], fz = a.z[i] / a.box.l[2];
+    a.fq[i] = make_double4(fx, fy, fz, a.q[i]);
     // exp(2 pi i f_z) and its 8th power, in FP64
     const double u = fz - rint(fz);
     double s1, c1, s8, c8;
@@ -36,7 +30,12 @@ __global__ void k_recip_prep(int n, const double *__restrict__ x, const double *
     double u8 = 8.0 * u;
     u8 -= rint(u8);
     sincospi(2.0 * u8, &s8, &c8);
-    stepz[i] = make_double4(c1, s1, c8, s8);
+    a.stepz[i] = make_double4(c1, s1, c8, s8);
+}
+
+__global__ __launch_bounds__(256) void k_recip_prep(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    recip_prep_body(args[m.x], m.y);
 }
 
 // exp(i 2 pi u) at the start of a row, u reduced to one turn in FP64.
@@ -50,29 +49,28 @@ __device__ inline void row_start(const RecipRow &r, double4 a, double &c, double
     s = (double)sf;
 }
 
-// The row's terms are generated in blocks of 8: an FP64 anchor advanced by
-// exp(16 pi i f_z) and an FP32 recurrence inside the block, so rounding of
-// the step cannot accumulate over more than 7 terms.
+// Structure factors of 64 rows over one atom chunk.  The row's terms are
+// generated in blocks of 8: an FP64 anchor advanced by exp(16 pi i f_z)
+// and an FP32 recurrence inside the block, so rounding of the step cannot
+// accumulate over more than 7 terms.
 template <int MAXLEN>
-__global__ __launch_bounds__(64) void k_sk(int nrows, const RecipRow *__restrict__ rows,
-                                           const int *__restrict__ rowoff, const int *__restrict__ wavelen,
-                                           int natoms, int chunk, const double4 *__restrict__ fq,
-                                           const double4 *__restrict__ stepz, int nkv, float2 *__restrict__ part) {
-    const int r = blockIdx.x * 64 + threadIdx.x;
-    const int a0 = blockIdx.y * chunk;
-    const int a1 = min(a0 + chunk, natoms);
-    const int wlen = wavelen[blockIdx.x];          // longest row of this wavefront
+__device__ inline void sk_body(const RecipArgs &a, int blk) {
+    const int rg = blk % a.nrowgroups, ch = blk / a.nrowgroups;
+    const int r = rg * 64 + threadIdx.x;
+    const int a0 = ch * a.chunk;
+    const int a1 = min(a0 + a.chunk, a.natoms);
+    const int wlen = a.wavelen[rg];          // longest row of this wavefront
     RecipRow row = {0, 0, 0, 0};
-    if (r < nrows) row = rows[r];
+    if (r < a.nrows) row = a.rows[r];
     float are[MAXLEN], aim[MAXLEN];
 #pragma unroll
     for (int j = 0; j < MAXLEN; ++j) { are[j] = 0.f; aim[j] = 0.f; }
     for (int i = a0; i < a1; ++i) {
-        const double4 a = fq[i];
-        const double4 st = stepz[i];
+        const double4 at = a.fq[i];
+        const double4 st = a.stepz[i];
         double s0, c0;
-        row_start(row, a, c0, s0);
-        double Ar = a.w * c0, Ai = a.w * s0;             // anchor, FP64
+        row_start(row, at, c0, s0);
+        double Ar = at.w * c0, Ai = at.w * s0;             // anchor, FP64
         const float sr = (float)st.x, si = (float)st.y;
 #pragma unroll
         for (int j0 = 0; j0 < MAXLEN; j0 += 8) {
@@ -91,48 +89,47 @@ __global__ __launch_bounds__(64) void k_sk(int nrows, const RecipRow *__restrict
             Ar = tr;
         }
     }
-    if (r < nrows) {
-        const int off = rowoff[r];
-        float2 *o = part + (size_t)blockIdx.y * nkv + off;
+    if (r < a.nrows) {
+        const int off = a.rowoff[r];
+        float2 *o = a.part + (size_t)ch * a.nkv + off;
 #pragma unroll
         for (int j = 0; j < MAXLEN; ++j)
             if (j < row.len) o[j] = make_float2(are[j], aim[j]);
     }
 }
 
-#define MDX_SK_INST(L) \
-    template __global__ void k_sk<L>(int, const RecipRow *, const int *, const int *, int, int, const double4 *, \
-                                     const double4 *, int, float2 *);
-MDX_SK_INST(16)
-MDX_SK_INST(32)
-MDX_SK_INST(64)
-MDX_SK_INST(96)
+template <int MAXLEN>
+__global__ __launch_bounds__(64) void k_sk(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    sk_body<MAXLEN>(args[m.x], m.y);
+}
+
+template __global__ void k_sk<16>(const RecipArgs *, const int2 *);
+template __global__ void k_sk<32>(const RecipArgs *, const int2 *);
+template __global__ void k_sk<64>(const RecipArgs *, const int2 *);
+template __global__ void k_sk<96>(const RecipArgs *, const int2 *);
 
 // Sum chunk partials in FP64 (fixed order), energy and virial terms of each
 // k (forces.f:1317-1331) and A_k = 2 AKC S(k).  kidx holds (nx, ny, nz) of
 // every k in row order.
-__global__ __launch_bounds__(256) void k_sk_reduce(int nkv, int nchunk, const float2 *__restrict__ part,
-                                                   const int4 *__restrict__ kidx, Box b, double alpha,
-                                                   double coulf, float2 *__restrict__ ak,
-                                                   double2 *__restrict__ sk, double *__restrict__ epart) {
-    __shared__ double red[4][256];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)[256]) {
+    const int k = blk * 256 + threadIdx.x;
     double e = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
-    if (k < nkv) {
+    if (k < a.nkv) {
         double sr = 0.0, si = 0.0;
-        for (int c = 0; c < nchunk; ++c) {
-            const float2 p = part[(size_t)c * nkv + k];
+        for (int c = 0; c < a.nchunk; ++c) {
+            const float2 p = a.part[(size_t)c * a.nkv + k];
             sr += (double)p.x;
             si += (double)p.y;
         }
-        const int4 id = kidx[k];
+        const int4 id = a.kidx[k];
         const double twopi = 2.0 * kPI;
-        const double rxv = twopi * id.x / b.l[0], ryv = twopi * id.y / b.l[1], rzv = twopi * id.z / b.l[2];
+        const double rxv = twopi * id.x / a.box.l[0], ryv = twopi * id.y / a.box.l[1], rzv = twopi * id.z / a.box.l[2];
         double rk2 = rxv * rxv + ryv * ryv;
         rk2 = rk2 + rzv * rzv;
-        const double alp2 = alpha * alpha;
-        const double vol = b.l[0] * b.l[1] * b.l[2];
-        const double cfur = 4.0 * kPI * coulf / vol;
+        const double alp2 = a.alpha * a.alpha;
+        const double vol = a.box.l[0] * a.box.l[1] * a.box.l[2];
+        const double cfur = 4.0 * kPI * a.coulf / vol;
         const double akc = cfur * exp(-0.25 * rk2 / alp2) / rk2;
         const double s2 = sr * sr + si * si;
         e = akc * s2;
@@ -140,8 +137,8 @@ __global__ __launch_bounds__(256) void k_sk_reduce(int nkv, int nchunk, const fl
         vx = e - rxv * rxv * akv * s2;
         vy = e - ryv * ryv * akv * s2;
         vz = e - rzv * rzv * akv * s2;
-        ak[k] = make_float2((float)(2.0 * akc * sr), (float)(2.0 * akc * si));
-        sk[k] = make_double2(sr, si);
+        a.ak[k] = make_float2((float)(2.0 * akc * sr), (float)(2.0 * akc * si));
+        a.sk[k] = make_double2(sr, si);
     }
     red[0][threadIdx.x] = e; red[1][threadIdx.x] = vx; red[2][threadIdx.x] = vy; red[3][threadIdx.x] = vz;
     __syncthreads();
@@ -151,7 +148,7 @@ __global__ __launch_bounds__(256) void k_sk_reduce(int nkv, int nchunk, const fl
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double *o = epart + (size_t)blockIdx.x * Q_N;
+        double *o = a.epart + (size_t)blk * Q_N;
         for (int q = 0; q < Q_N; ++q) o[q] = 0.0;
         o[Q_PELS1] = red[0][0];
         o[Q_VIRX] = red[1][0];
@@ -160,38 +157,45 @@ __global__ __launch_bounds__(256) void k_sk_reduce(int nkv, int nchunk, const fl
     }
 }
 
-// Reciprocal forces; blockIdx.y selects a slice of rows, each slice writes
-// its own partial force array (summed by the combine kernel).  Same
-// anchored recurrence as k_sk.
-__global__ __launch_bounds__(64) void k_recip_force(int natoms, int nrows, int rows_per_slice,
-                                                    const RecipRow *__restrict__ rows, const int *__restrict__ rowoff,
-                                                    const double4 *__restrict__ fq, const double4 *__restrict__ stepz,
-                                                    const float2 *__restrict__ ak, Box b,
-                                                    double *__restrict__ fpart) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int r0 = blockIdx.y * rows_per_slice;
-    const int r1 = min(r0 + rows_per_slice, nrows);
-    double4 a = make_double4(0, 0, 0, 0);
+__global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ double red[4][256];
+    const int2 m = map[blockIdx.x];
+    sk_reduce_body(args[m.x], m.y, red);
+}
+
+// Reciprocal forces; a block covers 64 atoms and one slice of rows, each
+// slice writes its own partial force array (summed by the combine kernel).
+// Same anchored recurrence as k_sk.
+__device__ inline void recip_force_body(const RecipArgs &a, int blk) {
+    const int ab = blk % a.natom_blocks, sl = blk / a.natom_blocks;
+    const int i = ab * 64 + threadIdx.x;
+    const int r0 = sl * a.rows_per_slice;
+    const int r1 = min(r0 + a.rows_per_slice, a.nrows);
+    double4 at = make_double4(0, 0, 0, 0);
     double4 st = make_double4(1.0, 0.0, 1.0, 0.0);
-    if (i < natoms) { a = fq[i]; st = stepz[i]; }
+    if (i < a.natoms) { at = a.fq[i]; st = a.stepz[i]; }
     const float sr = (float)st.x, si = (float)st.y;
-    const double tx = 2.0 * kPI / b.l[0], ty = 2.0 * kPI / b.l[1], tz = 2.0 * kPI / b.l[2];
+    const double tx = 2.0 * kPI / a.box.l[0], ty = 2.0 * kPI / a.box.l[1], tz = 2.0 * kPI / a.box.l[2];
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
     for (int r = r0; r < r1; ++r) {
-        const RecipRow row = rows[r];
-        const float2 *A = ak + rowoff[r];
+        const RecipRow row = a.rows[r];
+        const float2 *A = a.ak + a.rowoff[r];
         double s0, c0;
-        row_start(row, a, c0, s0);
-        double Wr = a.w * c0, Wi = a.w * s0;
+        row_start(row, at, c0, s0);
+        double Wr = at.w * c0, Wi = at.w * s0;
         float ts = 0.f, tzs = 0.f;
-        for (int j0 = 0; j0 < row.len; j0 += 8) {
+        int j0 = 0;
+        // full blocks of 8: the eight A_k are uniform, loaded together
+        for (; j0 + 8 <= row.len; j0 += 8) {
+            float2 Av[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Av[q] = A[j0 + q];
             float wr = (float)Wr, wi = (float)Wi;
-            const int j1 = min(j0 + 8, row.len);
-            for (int j = j0; j < j1; ++j) {
-                const float2 Ak = A[j];
-                const float t = wi * Ak.x - wr * Ak.y;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float t = wi * Av[q].x - wr * Av[q].y;
                 ts += t;
-                tzs += (float)j * t;
+                tzs += (float)(j0 + q) * t;
                 const float nr = wr * sr - wi * si;
                 wi = wr * si + wi * sr;
                 wr = nr;
@@ -200,16 +204,33 @@ __global__ __launch_bounds__(64) void k_recip_force(int natoms, int nrows, int r
             Wi = Wr * st.w + Wi * st.z;
             Wr = tr;
         }
+        {
+            float wr = (float)Wr, wi = (float)Wi;
+            for (int j = j0; j < row.len; ++j) {
+                const float2 Ak = A[j];
+                const float t = wi * Ak.x - wr * Ak.y;
+                ts += t;
+                tzs += (float)j * t;
+                const float nr = wr * sr - wi * si;
+                wi = wr * si + wi * sr;
+                wr = nr;
+            }
+        }
         Fx += tx * row.nx * (double)ts;
         Fy += ty * row.ny * (double)ts;
         Fz += tz * ((double)row.nz0 * (double)ts + (double)tzs);
     }
-    if (i < natoms) {
-        double *o = fpart + (size_t)blockIdx.y * 3 * natoms;
+    if (i < a.natoms) {
+        double *o = a.rpart + (size_t)sl * 3 * a.natoms;
         o[i] = Fx;
-        o[natoms + i] = Fy;
-        o[2 * natoms + i] = Fz;
+        o[a.natoms + i] = Fy;
+        o[2 * a.natoms + i] = Fz;
     }
+}
+
+__global__ __launch_bounds__(64) void k_recip_force(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    recip_force_body(args[m.x], m.y);
 }
 
 }  // namespace mdx

@@ -60,9 +60,21 @@ def load_library() -> C.CDLL:
     L.mdx_compute.argtypes = [pv, C.c_int32, C.c_int32] + [pd] * 9 + [pv]
     L.mdx_compute_device.argtypes = [pv, C.c_int32, C.c_int32, pv, pv, pv, pv]
     L.mdx_stage_times.argtypes = [pv, pd, C.c_int32]
+    L.mdx_get_forces.argtypes = [pv] + [pd] * 9
+    L.mdx_set_timing.argtypes = [pv, C.c_int32]
+    L.mdx_memory_bytes.argtypes = [pv, C.POINTER(C.c_int64)]
+    L.mdx_batch_create.argtypes = [C.POINTER(pv), C.c_int32, C.POINTER(pv)]
+    L.mdx_batch_set_ring.argtypes = [pv, C.c_double]
+    L.mdx_batch_use_graph.argtypes = [pv, C.c_int32]
+    L.mdx_batch_compute.argtypes = [pv, C.c_int32, C.c_int32, pv]
+    L.mdx_batch_last_error.argtypes = [pv]
+    L.mdx_batch_last_error.restype = C.c_char_p
+    L.mdx_batch_destroy.argtypes = [pv]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
                  "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_get_list", "mdx_compute",
-                 "mdx_compute_device", "mdx_stage_times", "mdx_device_info"):
+                 "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
+                 "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
+                 "mdx_batch_compute", "mdx_batch_destroy"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -185,6 +197,68 @@ class ForceEngine:
         ms = np.zeros(len(STAGES))
         self._check(self.lib.mdx_stage_times(self._h, as_ptr(ms), len(STAGES)))
         return dict(zip(STAGES, ms.tolist()))
+
+    def set_timing(self, on: bool) -> None:
+        """Per-stage event timing (eager launches) or, when off, replay of
+        the force call from a captured hipGraph."""
+        self._check(self.lib.mdx_set_timing(self._h, 1 if on else 0))
+
+    def get_forces(self):
+        """(F, G, H) of the last force call, e.g. after a batched call."""
+        n = self.n
+        out = np.zeros((9, n))
+        self._check(self.lib.mdx_get_forces(self._h, *[as_ptr(out[k]) for k in range(9)]))
+        return out[0:3].T.copy(), out[3:6].T.copy(), out[6:9].T.copy()
+
+    def memory_bytes(self) -> int:
+        b = C.c_int64()
+        self._check(self.lib.mdx_memory_bytes(self._h, C.byref(b)))
+        return b.value
+
+
+class ForceBatch:
+    """K engines evaluated by one set of fused launches per force call
+    (path-integral beads with `ring`, lambda states or independent jobs).
+    Each engine keeps its own positions and lists."""
+
+    def __init__(self, engines, ring_beadfac: Optional[float] = None, graph: bool = True):
+        self.lib = load_library()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        b = C.c_void_p()
+        st = self.lib.mdx_batch_create(arr, len(self.engines), C.byref(b))
+        self._b = b
+        if st != 0:
+            if b.value:
+                self.lib.mdx_batch_destroy(b)
+            self._b = C.c_void_p()
+            raise MdxError(st, "batch creation failed")
+        if ring_beadfac is not None:
+            self._check(self.lib.mdx_batch_set_ring(self._b, ring_beadfac))
+        self._check(self.lib.mdx_batch_use_graph(self._b, 1 if graph else 0))
+
+    def _check(self, st: int) -> None:
+        if st != 0:
+            raise MdxError(st, self.lib.mdx_batch_last_error(self._b).decode())
+
+    def compute(self, group: int = abi.MDX_FORCES_ALL, energies: bool = True):
+        k = len(self.engines)
+        e = (MdxEnergies * k)()
+        self._check(self.lib.mdx_batch_compute(self._b, group, 0, C.cast(e, C.c_void_p) if energies else None))
+        if not energies:
+            return None
+        return [e[s].as_dict(self.engines[s].topo.nmoltypes, self.engines[s].topo.ntypepairs) for s in range(k)]
+
+    def close(self) -> None:
+        if self._b and self._b.value:
+            self.lib.mdx_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_info() -> Dict:

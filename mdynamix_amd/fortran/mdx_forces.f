@@ -61,6 +61,7 @@
         integer(c_int64_t) :: npairs_short, npairs_long
         real(c_double) :: pote1(MAXTP), potl1(MAXTP)
         real(c_double) :: pes141(MAXT), psr141(MAXT), spee(MAXT)
+        real(c_double) :: qek, virpi
       end type
       interface
         integer(c_int) function mdx_create(t, p, h) bind(C)

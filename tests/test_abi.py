@@ -34,7 +34,7 @@ def test_struct_layout():
     # mdx_params: 3 box + rcut + rshort + skin + alpha + fexp + coulf (9 doubles), 2 ints, 2 doubles
     assert C.sizeof(abi.MdxParams) == 9 * 8 + 2 * 4 + 2 * 8
     # energies: 17 doubles, 2 int64, 2*MAX_TYPEPAIRS + 3*MAX_MOLTYPES doubles
-    assert C.sizeof(abi.MdxEnergies) == 8 * (17 + 2 + 2 * abi.MDX_MAX_TYPEPAIRS + 3 * abi.MDX_MAX_MOLTYPES)
+    assert C.sizeof(abi.MdxEnergies) == 8 * (17 + 2 + 2 * abi.MDX_MAX_TYPEPAIRS + 3 * abi.MDX_MAX_MOLTYPES + 2)
 
 
 def test_create_rejects_bad_input_without_device_work():

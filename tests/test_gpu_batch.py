@@ -1,0 +1,149 @@
+"""Fused batches and graph replay through the C ABI.
+
+* a force call replayed from its captured hipGraph gives the same bits as
+  the eagerly launched call, also after positions change and lists are
+  rebuilt (the graph's launch shapes are capacity based);
+* K different systems evaluated by one set of fused launches give the same
+  bits as each system alone (per-system argument structs, block maps and
+  fixed-order reductions);
+* a path-integral ring (PIFORCES, pimd.f:321-394; force scaling
+  pimd.f:61-69) equals the beads evaluated alone, scaled by 1/P, plus the
+  harmonic springs computed here in numpy from the reference formulas.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import load_golden, golden_positions
+from mdynamix_amd import abi, topology, systems
+from mdynamix_amd.engine import ForceEngine, ForceBatch
+
+pytestmark = pytest.mark.gpu
+
+GROUPS = (abi.MDX_FORCES_ALL, abi.MDX_FORCES_FAST, abi.MDX_FORCES_SLOW)
+KEYS = abi.MdxEnergies.SCALARS + ("npairs_short", "npairs_long")
+
+
+def _golden_engine(name):
+    d = load_golden(name)
+    topo, prm = topology.from_dump(d, lam=d["lam"])
+    eng = ForceEngine(topo, prm)
+    eng.set_positions(golden_positions(d))
+    eng.chcnb()
+    return d, eng
+
+
+def _same(a, b):
+    fa, ea = a
+    fb, eb = b
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y)
+    for k in KEYS:
+        assert ea[k] == eb[k], k
+
+
+def _alone(eng, group):
+    eng.set_timing(True)
+    f, g, h, e = eng.compute(group)
+    return (f, g, h), e
+
+
+def test_graph_replay_matches_eager():
+    d, eng = _golden_engine("water_508")
+    print("case water_508 graph replay", flush=True)
+    rng = np.random.default_rng(3)
+    pos = golden_positions(d)
+    for it in range(3):
+        for group in GROUPS:
+            ref = _alone(eng, group)
+            eng.set_timing(False)
+            for _ in range(2):
+                f, g, h, e = eng.compute(group)
+                _same(((f, g, h), e), ref)
+        pos = pos + rng.uniform(-0.05, 0.05, pos.shape)
+        eng.set_positions(pos)
+        if it == 1:
+            eng.chcnb()
+
+
+def test_batch_of_different_systems_matches_alone():
+    names = ("water_508", "chains_tiny", "ee_tiny")
+    engs = []
+    for nm in names:
+        d, eng = _golden_engine(nm)
+        engs.append((d, eng))
+    print("case batch", names, flush=True)
+    rng = np.random.default_rng(9)
+    batch = ForceBatch([e for _, e in engs])
+    for it in range(2):
+        for group in GROUPS:
+            alone = [_alone(eng, group) for _, eng in engs]
+            eb = batch.compute(group)
+            for s, (_, eng) in enumerate(engs):
+                _same((eng.get_forces(), eb[s]), alone[s])
+        d0, e0 = engs[0]
+        e0.set_positions(golden_positions(d0) + rng.uniform(-0.05, 0.05, (e0.n, 3)))
+        e0.chcnb()
+    batch.close()
+
+
+def _springs(beads, mass, box, beadfac):
+    """PIFORCES (pimd.f:321-394) for every bead: forces, the bead's QEK
+    share (half of both links) and VIRPI of its forward link."""
+    P = len(beads)
+    out = []
+    for k in range(P):
+        f = np.zeros_like(beads[k])
+        qek = 0.0
+        virpi = 0.0
+        virf = np.zeros(3)
+        for side, o in ((0, beads[(k + 1) % P]), (1, beads[(k - 1) % P])):
+            b = beads[k] - o
+            b -= np.where(np.abs(b) > 0.5 * box, np.sign(b) * box, 0.0)
+            bsq = (b * b).sum(1)
+            bbb = np.sqrt(bsq)
+            dbnd = bbb * beadfac * mass
+            forb = -2.0 * dbnd / bbb
+            f += b * forb[:, None]
+            qek += 0.5 * (dbnd * bbb).sum()
+            if side == 0:
+                virpi += (forb * bsq).sum()
+                virf += (forb[:, None] * b * b).sum(0) * P
+        out.append((f, qek, virpi, virf))
+    return out
+
+
+def test_pimd_ring_matches_beads_alone_plus_springs():
+    P = 4
+    beads = systems.pimd_beads(n_beads=P, n_water=400, box=22.9, seed=41)
+    engs = []
+    for s in beads:
+        topo, prm = topology.from_system(s)
+        eng = ForceEngine(topo, prm)
+        eng.set_positions(s.pos)
+        eng.chcnb()
+        engs.append(eng)
+    print("case pimd ring", P, engs[0].n, flush=True)
+    beadfac = 2.5e-4
+    box = np.array(beads[0].box, float)
+    spr = _springs([s.pos for s in beads], engs[0].topo.atom_mass, box, beadfac)
+    ring = ForceBatch(engs, ring_beadfac=beadfac)
+    for group in GROUPS:
+        alone = [_alone(eng, group) for eng in engs]
+        eb = ring.compute(group)
+        for k, eng in enumerate(engs):
+            (fa, ga, ha), ea = alone[k]
+            f, g, h = eng.get_forces()
+            sf, qek, virpi, virf = spr[k]
+            add = sf if group != abi.MDX_FORCES_SLOW else 0.0
+            scale = np.abs(fa / P + add).max()
+            assert np.abs(f - (fa / P + add)).max() <= 1e-12 * scale, (group, k)
+            assert np.abs(g - ga / P).max() <= 1e-12 * scale
+            assert np.abs(h - (ha / P + add)).max() <= 1e-12 * scale
+            for key in ("pe1", "pe2", "pels1", "pels2", "spe", "pint", "vir1", "vir2"):
+                assert eb[k][key] == ea[key], key
+            if group != abi.MDX_FORCES_SLOW:
+                assert eb[k]["qek"] == pytest.approx(qek, rel=1e-12)
+                assert eb[k]["virpi"] == pytest.approx(virpi, rel=1e-12)
+                for c, key in enumerate(("virfx", "virfy", "virfz")):
+                    assert eb[k][key] == pytest.approx(ea[key] + virf[c], rel=1e-11, abs=1e-9)
+    ring.close()
