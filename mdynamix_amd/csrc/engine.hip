@@ -47,6 +47,10 @@ struct mdx_engine {
     double fscale = 1.0;
     // device topology
     int *d_moltype = nullptr, *d_mol = nullptr, *d_ljtype = nullptr, *d_type_ee = nullptr;
+    int *d_type_pair = nullptr;
+    // per-type energy breakdown rows (allocated on the first detail call)
+    double *d_dpart = nullptr;
+    int ndet = 0, det_rows = 0;
     double *d_q = nullptr, *d_q0 = nullptr, *d_mass = nullptr, *d_c14el = nullptr, *d_c14lj = nullptr;
     double *d_a6_0 = nullptr, *d_b12_0 = nullptr, *d_a6_14 = nullptr, *d_b12_14 = nullptr;
     float2 *d_lj = nullptr;
@@ -173,6 +177,10 @@ struct mdx_batch {
     KPlan<SpringArgs> springs;
     KPlan<CombineArgs> comb;
     KPlan<ReduceArgs> red;
+    KPlan<DetailArgs> dred;
+    int nmt_max = 1;
+    double *d_det = nullptr;         // [K][kMaxDet]
+    double *h_det = nullptr;         // pinned copy
     double *d_energy = nullptr;      // [K][Q_N]
     double *h_energy = nullptr;      // pinned copy
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
@@ -425,6 +433,20 @@ void layout_rows(mdx_engine *h) {
 
 double *erow(mdx_engine *h, int row) { return h->d_epart + (size_t)row * Q_N; }
 
+void fill_args(mdx_engine *h);
+
+// Rows of the per-type breakdown: one per pair block, then one per
+// special block.
+mdx_status ensure_detail(mdx_engine *h) {
+    const int rows = h->nclu_cap + (h->nspecial ? cdiv(h->nspecial, 256) : 0);
+    if (h->d_dpart && rows == h->det_rows) return MDX_OK;
+    h->ndet = 2 * h->ntypepairs + 2 * h->ntypes;
+    CHK(dalloc(h, &h->d_dpart, (size_t)rows * h->ndet));
+    h->det_rows = rows;
+    fill_args(h);
+    return MDX_OK;
+}
+
 // Argument structs of every force kernel from the current state.
 void fill_args(mdx_engine *h) {
     const int n = h->n;
@@ -474,6 +496,13 @@ void fill_args(mdx_engine *h) {
     pa.fs_x = h->d_fs; pa.fs_y = h->d_fs + n; pa.fs_z = h->d_fs + 2 * n;
     pa.fl_x = h->d_fl; pa.fl_y = h->d_fl + n; pa.fl_z = h->d_fl + 2 * n;
     pa.epart = erow(h, h->row_pairs);
+    pa.atom_mol = h->d_mol;
+    pa.atom_moltype = h->d_moltype;
+    pa.type_pair = h->d_type_pair;
+    pa.nmt = h->ntypes;
+    pa.ntp = h->ntypepairs;
+    pa.ndet = h->ndet;
+    pa.dpart = h->d_dpart;
 
     SpecialArgs &sa = h->a_special;
     sa.ns = h->nspecial;
@@ -495,6 +524,10 @@ void fill_args(mdx_engine *h) {
     sa.fixh_x = fixHL; sa.fixh_y = fixHL + n; sa.fixh_z = fixHL + 2 * n;
     sa.fixg_x = fixG; sa.fixg_y = fixG + n; sa.fixg_z = fixG + 2 * n;
     sa.epart = erow(h, h->row_sp);
+    sa.nmt = h->ntypes;
+    sa.ntp = h->ntypepairs;
+    sa.ndet = h->ndet;
+    sa.dpart = h->d_dpart ? h->d_dpart + (size_t)h->nclu_cap * h->ndet : nullptr;
 
     RecipArgs &ra = h->a_recip;
     ra.natoms = n;
@@ -635,7 +668,8 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
     mdx_status s;
     if ((s = upload(h, &h->d_moltype, t->atom_moltype, n)) ||
         (s = upload(h, &h->d_mol, t->atom_mol, n)) || (s = upload(h, &h->d_ljtype, t->atom_ljtype, n)) ||
-        (s = upload(h, &h->d_type_ee, t->type_ee, h->ntypes)) || (s = upload(h, &h->d_q, t->atom_charge, n)) ||
+        (s = upload(h, &h->d_type_ee, t->type_ee, h->ntypes)) ||
+        (s = upload(h, &h->d_type_pair, t->type_pair, h->ntypes * h->ntypes)) || (s = upload(h, &h->d_q, t->atom_charge, n)) ||
         (s = upload(h, &h->d_q0, t->atom_charge0, n)) ||
         (s = upload(h, &h->d_c14el, t->type_c14el, h->ntypes)) ||
         (s = upload(h, &h->d_c14lj, t->type_c14lj, h->ntypes)) ||
@@ -1076,7 +1110,8 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     b->zero.clear(); b->pack.clear(); b->pairs.clear(); b->special.clear();
     b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
     b->et.clear(); b->bonds.clear(); b->angles.clear(); b->tors.clear();
-    b->springs.clear(); b->comb.clear(); b->red.clear();
+    b->springs.clear(); b->comb.clear(); b->red.clear(); b->dred.clear();
+    b->nmt_max = 1;
     b->maxlen = 16;
     b->nlj_max = 1;
     b->soft = false;
@@ -1084,6 +1119,9 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         mdx_engine *h = b->sys[s];
         const int n = h->n;
         b->maxlen = std::max(b->maxlen, h->maxlen);
+        b->nmt_max = std::max(b->nmt_max, h->ntypes);
+        if (h->d_dpart)
+            b->dred.add(DetailArgs{h->det_rows, h->ndet, h->d_dpart, b->d_det + (size_t)s * kMaxDet}, 1);
         b->nlj_max = std::max(b->nlj_max, h->nlj);
         b->soft = b->soft || h->p.ee_enabled;
         const long long nz1 = (long long)h->rows_total * Q_N;
@@ -1155,6 +1193,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     CHK(kplan_upload(b, b->angles, maps)); CHK(kplan_upload(b, b->tors, maps));
     CHK(kplan_upload(b, b->springs, maps)); CHK(kplan_upload(b, b->comb, maps));
     CHK(kplan_upload(b, b->red, maps));
+    CHK(kplan_upload(b, b->dred, maps));
     b->seen.assign(K, 0);
     for (int s = 0; s < K; ++s) b->seen[s] = b->sys[s]->version;
     b->planned = true;
@@ -1168,7 +1207,7 @@ void launch(mdx_batch *b, KPlan<T> &k, int nthreads, size_t lds, F kernel) {
 }
 
 // The kernel sequence of one force call over all systems of the batch.
-void launch_call(mdx_batch *b, int group, bool timing) {
+void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
     hipStream_t st = b->stream;
     auto mark = [&](int k) { if (timing) hipEventRecord(b->ev[k], st); };
     mark(0);
@@ -1176,13 +1215,24 @@ void launch_call(mdx_batch *b, int group, bool timing) {
     launch(b, b->pack, 256, 0, k_pack);
     mark(1);
     const size_t lj = sizeof(float2) * b->nlj_max * b->nlj_max;
+    const size_t ldet = lj + sizeof(double) * (b->nmt_max + 1) * 2 * 64 + sizeof(int) * 8;
     if (b->pairs.nblocks()) {
-        if (b->soft) k_pairs<true><<<b->pairs.nblocks(), 64, lj, st>>>(b->pairs.d_args, b->pairs.d_map, group);
-        else k_pairs<false><<<b->pairs.nblocks(), 64, lj, st>>>(b->pairs.d_args, b->pairs.d_map, group);
+        const int nb = b->pairs.nblocks();
+        const PairArgs *pa = b->pairs.d_args;
+        const int2 *pm = b->pairs.d_map;
+        if (detail) {
+            if (b->soft) k_pairs<true, true><<<nb, 64, ldet, st>>>(pa, pm, group);
+            else k_pairs<false, true><<<nb, 64, ldet, st>>>(pa, pm, group);
+        } else {
+            if (b->soft) k_pairs<true, false><<<nb, 64, lj, st>>>(pa, pm, group);
+            else k_pairs<false, false><<<nb, 64, lj, st>>>(pa, pm, group);
+        }
     }
     mark(2);
-    if (b->special.nblocks())
-        k_special<<<b->special.nblocks(), 256, 0, st>>>(b->special.d_args, b->special.d_map, group);
+    if (b->special.nblocks()) {
+        if (detail) k_special<true><<<b->special.nblocks(), 256, 0, st>>>(b->special.d_args, b->special.d_map, group);
+        else k_special<false><<<b->special.nblocks(), 256, 0, st>>>(b->special.d_args, b->special.d_map, group);
+    }
     mark(3);
     if (group != MDX_FORCES_FAST) {
         launch(b, b->rprep, 256, 0, k_recip_prep);
@@ -1210,11 +1260,23 @@ void launch_call(mdx_batch *b, int group, bool timing) {
     if (b->comb.nblocks()) k_combine<<<b->comb.nblocks(), 256, 0, st>>>(b->comb.d_args, b->comb.d_map, group);
     launch(b, b->red, 256, 0, k_ereduce);
     hipMemcpyAsync(b->h_energy, b->d_energy, sizeof(double) * Q_N * b->sys.size(), hipMemcpyDeviceToHost, st);
+    if (detail) {
+        launch(b, b->dred, 256, 0, k_detail_reduce);
+        hipMemcpyAsync(b->h_det, b->d_det, sizeof(double) * kMaxDet * b->sys.size(), hipMemcpyDeviceToHost, st);
+    }
     mark(9);
 }
 
-mdx_status batch_compute(mdx_batch *b, int group) {
+mdx_status batch_compute(mdx_batch *b, int group, bool detail) {
     if (group < 0 || group > 2) return MDX_ERR_ARG;
+    if (detail)
+        for (mdx_engine *h : b->sys) {
+            mdx_status s = ensure_detail(h);
+            if (s != MDX_OK) {
+                b->err = h->err;
+                return s;
+            }
+        }
     for (mdx_engine *h : b->sys)
         if (!h->have_list) {
             b->err = "mdx_compute before mdx_build_list";
@@ -1235,16 +1297,16 @@ mdx_status batch_compute(mdx_batch *b, int group) {
                                   b->zero.nblocks(), b->maxlen};
         if (before != after) drop_graphs(b);
     }
-    if (b->use_graph && !b->timing) {
+    if (b->use_graph && !b->timing && !detail) {
         if (!b->exec[group]) {
             HIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            launch_call(b, group, false);
+            launch_call(b, group, false, false);
             HIPCHK(b, hipStreamEndCapture(b->stream, &b->graph[group]));
             HIPCHK(b, hipGraphInstantiate(&b->exec[group], b->graph[group], nullptr, nullptr, 0));
         }
         HIPCHK(b, hipGraphLaunch(b->exec[group], b->stream));
     } else {
-        launch_call(b, group, b->timing);
+        launch_call(b, group, b->timing, detail);
     }
     HIPCHK(b, hipStreamSynchronize(b->stream));
     HIPCHK(b, hipGetLastError());
@@ -1258,7 +1320,7 @@ mdx_status batch_compute(mdx_batch *b, int group) {
     return MDX_OK;
 }
 
-void fill_energies(const mdx_batch *b, int s, int group, mdx_energies *e) {
+void fill_energies(const mdx_batch *b, int s, int group, bool detail, mdx_energies *e) {
     const mdx_engine *h = b->sys[s];
     const double *q = b->h_energy + (size_t)s * Q_N;
     std::memset(e, 0, sizeof(*e));
@@ -1275,6 +1337,12 @@ void fill_energies(const mdx_batch *b, int s, int group, mdx_energies *e) {
     e->qek = q[Q_QEK];
     e->virpi = q[Q_VIRPI];
     for (int t = 0; t < h->ntypes && group != MDX_FORCES_FAST; ++t) e->spee[t] = h->spee_self[t];
+    if (detail) {
+        const double *d = b->h_det + (size_t)s * kMaxDet;
+        const int ntp = h->ntypepairs, nmt = h->ntypes;
+        for (int k = 0; k < ntp; ++k) { e->pote1[k] = d[k]; e->potl1[k] = d[ntp + k]; }
+        for (int t = 0; t < nmt; ++t) { e->pes141[t] = d[2 * ntp + t]; e->psr141[t] = d[2 * ntp + nmt + t]; }
+    }
 }
 
 mdx_status copy_forces(mdx_engine *h, double *fx, double *fy, double *fz, double *gx, double *gy, double *gz,
@@ -1293,33 +1361,33 @@ extern "C" {
 
 mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want, double *fx, double *fy, double *fz, double *gx,
                        double *gy, double *gz, double *hx, double *hy, double *hz, mdx_energies *e) {
-    (void)want;
     if (!h) return MDX_ERR_ARG;
+    const bool detail = (want & MDX_WANT_DETAIL) != 0;
     mdx_batch *b = h->self;
     b->timing = h->timing;
-    mdx_status s = batch_compute(b, group);
+    mdx_status s = batch_compute(b, group, detail);
     if (s != MDX_OK) {
         h->err = b->err;
         return s;
     }
     for (int k = ST_PREP; k < ST_N; ++k) h->stage_ms[k] = b->stage_ms[k];
-    if (e) fill_energies(b, 0, group, e);
+    if (e) fill_energies(b, 0, group, detail, e);
     return copy_forces(h, fx, fy, fz, gx, gy, gz, hx, hy, hz, hipMemcpyDeviceToHost);
 }
 
 mdx_status mdx_compute_device(mdx_engine *h, int32_t group, int32_t want, double *dfx, double *dfy, double *dfz,
                               mdx_energies *e) {
-    (void)want;
     if (!h) return MDX_ERR_ARG;
+    const bool detail = (want & MDX_WANT_DETAIL) != 0;
     mdx_batch *b = h->self;
     b->timing = h->timing;
-    mdx_status s = batch_compute(b, group);
+    mdx_status s = batch_compute(b, group, detail);
     if (s != MDX_OK) {
         h->err = b->err;
         return s;
     }
     for (int k = ST_PREP; k < ST_N; ++k) h->stage_ms[k] = b->stage_ms[k];
-    if (e) fill_energies(b, 0, group, e);
+    if (e) fill_energies(b, 0, group, detail, e);
     return copy_forces(h, dfx, dfy, dfz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                        hipMemcpyDeviceToDevice);
 }
@@ -1368,7 +1436,9 @@ mdx_status mdx_batch_create(mdx_engine *const *systems, int32_t k, mdx_batch **o
     b->own_stream = true;
     for (int e = 0; e < 12; ++e) hipEventCreate(&b->ev[e]);
     if (hipMalloc((void **)&b->d_energy, sizeof(double) * Q_N * k) != hipSuccess ||
-        hipHostMalloc((void **)&b->h_energy, sizeof(double) * Q_N * k, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&b->h_energy, sizeof(double) * Q_N * k, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&b->d_det, sizeof(double) * kMaxDet * k) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_det, sizeof(double) * kMaxDet * k, hipHostMallocDefault) != hipSuccess) {
         *out = b;
         return MDX_ERR_NOMEM;
     }
@@ -1398,11 +1468,11 @@ mdx_status mdx_batch_use_graph(mdx_batch *b, int32_t on) {
 }
 
 mdx_status mdx_batch_compute(mdx_batch *b, int32_t group, int32_t want, mdx_energies *e) {
-    (void)want;
     if (!b) return MDX_ERR_ARG;
-    CHK(batch_compute(b, group));
+    const bool detail = (want & MDX_WANT_DETAIL) != 0;
+    CHK(batch_compute(b, group, detail));
     if (e)
-        for (size_t s = 0; s < b->sys.size(); ++s) fill_energies(b, (int)s, group, e + s);
+        for (size_t s = 0; s < b->sys.size(); ++s) fill_energies(b, (int)s, group, detail, e + s);
     return MDX_OK;
 }
 
@@ -1415,9 +1485,11 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->pairs); kplan_free(b->special);
     kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
     kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->angles); kplan_free(b->tors);
-    kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red);
+    kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);
     if (b->d_energy) hipFree(b->d_energy);
     if (b->h_energy) hipHostFree(b->h_energy);
+    if (b->d_det) hipFree(b->d_det);
+    if (b->h_det) hipHostFree(b->h_det);
     for (int e = 0; e < 12; ++e)
         if (b->ev[e]) hipEventDestroy(b->ev[e]);
     if (b->own_stream && b->stream) hipStreamDestroy(b->stream);

@@ -39,6 +39,9 @@ constexpr double kA4 = -1.453152027, kA5 = 1.061405429, kB1 = 0.3275911;
 constexpr double kPI = 3.1415926536, kTOTPI = 1.128379169;
 
 constexpr int kClu = 8;            // atoms per cluster
+// Per-type energy breakdown (MDX_WANT_DETAIL): per partial row
+// [POTE1(ntp) | POTL1(ntp) | PES141(nmt) | PSR141(nmt)]
+constexpr int kMaxDet = 2 * 137 + 2 * 16;
 constexpr int kWave = 64;          // wavefront size on gfx950
 
 // Energy/virial slots of the per-block partial sums.
@@ -125,6 +128,10 @@ struct PairArgs {
     double *fs_x, *fs_y, *fs_z;  // short-class pair forces (per atom)
     double *fl_x, *fl_y, *fl_z;  // long-class pair forces (per atom)
     double *epart;               // [nclu_cap][Q_N]
+    // per-type breakdown (detail calls only)
+    const int *atom_mol, *atom_moltype, *type_pair;
+    int nmt, ntp, ndet;
+    double *dpart;               // [nclu_cap][ndet]
 };
 
 struct SpecialArgs {
@@ -144,6 +151,8 @@ struct SpecialArgs {
     unsigned long long *fixh_x, *fixh_y, *fixh_z;   // short / 1-4 -> H (LOCALF)
     unsigned long long *fixg_x, *fixg_y, *fixg_z;   // long -> G (FORCES)
     double *epart;
+    int nmt, ntp, ndet;
+    double *dpart;               // detail rows of this kernel's blocks
 };
 
 struct RecipArgs {
@@ -239,6 +248,12 @@ struct CombineArgs {
     double *Gx, *Gy, *Gz;          // slow (optional)
     double *Hx, *Hy, *Hz;          // fast (optional)
     double *epart;
+};
+
+struct DetailArgs {
+    int nrows, ndet;
+    const double *dpart;           // [nrows][ndet]
+    double *out;                   // [ndet]
 };
 
 struct ReduceArgs {

@@ -63,11 +63,26 @@ __global__ __launch_bounds__(256) void k_pack(const PackArgs *__restrict__ args,
     pack_body(args[m.x], m.y);
 }
 
-template <bool SOFT>
+// Detail calls (MDX_WANT_DETAIL) also split the bare Coulomb energy
+// q_i q_j C / (r + RAE) and the LJ energy by molecule-type pair (POTE1,
+// POTL1) or, for pairs inside one molecule, by molecule type (PES141,
+// PSR141), as LOCALF/FORCES do (forces.f:945-974, 1155-1183).  Each lane
+// keeps its own FP64 buckets in LDS (bucket = type of the j molecule, or
+// the intramolecular bucket), so the sums stay in a fixed order.
+template <bool SOFT, bool DETAIL>
 __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int group) {
     const int lane = threadIdx.x;
     const int nlj2 = a.nlj * a.nlj;
     for (int t = lane; t < nlj2; t += 64) s_lj[t] = a.lj[t];
+    double *s_det = reinterpret_cast<double *>(s_lj + nlj2);   // [(nmt + 1) * 2][64]
+    int *s_ti = reinterpret_cast<int *>(s_det + (a.nmt + 1) * 2 * 64);
+    if (DETAIL) {
+        for (int b = 0; b < (a.nmt + 1) * 2; ++b) s_det[b * 64 + lane] = 0.0;
+        if (ci >= a.nclu) {
+            for (int o = lane; o < a.ndet; o += 64) a.dpart[(size_t)ci * a.ndet + o] = 0.0;
+            return;
+        }
+    }
     __syncthreads();
     if (ci >= a.nclu) return;
 
@@ -83,6 +98,12 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
     const float nal2 = -alpha * alpha * 1.44269504088896341f;   // -alpha^2 log2(e)
     const float talpha = TOT * alpha;
     const float qic = pi.w * a.coulf_f;
+    int moli = -1;
+    if (DETAIL) {
+        const int ai0 = a.clu_atom[ci * kClu + il];
+        moli = ai0 >= 0 ? a.atom_mol[ai0] : -1;
+        if (jl == 0) s_ti[il] = ai0 >= 0 ? a.atom_moltype[ai0] : 0;
+    }
 
     double Fs[3] = {0, 0, 0}, Fl[3] = {0, 0, 0};
     double E[6] = {0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
@@ -138,6 +159,12 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
             const float fsc = (ee1 * (talpha * g + erfct * rei) + fsr) * rinv;
             const float fxx = fsc * dx, fyy = fsc * dy, fzz = fsc * dz;
             const float vr = fsr * r;
+            if (DETAIL) {
+                const int aj = a.clu_atom[e.cj * kClu + jl];
+                const int b = a.atom_mol[aj] == moli ? a.nmt : a.atom_moltype[aj];
+                s_det[(b * 2) * 64 + lane] += (double)(pj.w * rei) + (double)(__int_as_float(auxj.y) * rei);
+                s_det[(b * 2 + 1) * 64 + lane] += (double)esr;
+            }
             if (isshort) {
                 fs[0] += fxx; fs[1] += fyy; fs[2] += fzz;
                 es += ees; ljs += esr; vs += vr;
@@ -188,18 +215,43 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
         o[Q_NPS] = 0.5 * q[12];
         o[Q_NPL] = 0.5 * q[13];
     }
+    if (DETAIL) {
+        // bare Coulomb buckets per unit q_i C -> energies; then each output
+        // column sums its (lane, bucket) entries in lane order (halved: the
+        // full list visits every pair twice)
+        for (int b = 0; b <= a.nmt; ++b) s_det[(b * 2) * 64 + lane] *= qc;
+        __syncthreads();
+        for (int o = lane; o < a.ndet; o += 64) {
+            double sum = 0.0;
+            for (int L = 0; L < 64; ++L) {
+                const int ti = s_ti[L >> 3];
+                if (o < 2 * a.ntp) {
+                    const int k = o < a.ntp ? 0 : 1, mtr = o < a.ntp ? o : o - a.ntp;
+                    for (int b = 0; b < a.nmt; ++b)
+                        if (a.type_pair[ti * a.nmt + b] == mtr) sum += s_det[(b * 2 + k) * 64 + L];
+                } else {
+                    const int r = o - 2 * a.ntp;
+                    const int k = r < a.nmt ? 0 : 1, t = r < a.nmt ? r : r - a.nmt;
+                    if (ti == t) sum += s_det[(a.nmt * 2 + k) * 64 + L];
+                }
+            }
+            a.dpart[(size_t)ci * a.ndet + o] = 0.5 * sum;
+        }
+    }
 }
 
-template <bool SOFT>
+template <bool SOFT, bool DETAIL>
 __global__ __launch_bounds__(64) void k_pairs(const PairArgs *__restrict__ args, const int2 *__restrict__ map,
                                               int group) {
-    extern __shared__ float2 s_lj[];      // nlj x nlj (A6, B12)
+    extern __shared__ float2 s_lj[];      // nlj x nlj (A6, B12), then detail buckets
     const int2 m = map[blockIdx.x];
-    pairs_body<SOFT>(args[m.x], m.y, s_lj, group);
+    pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, group);
 }
 
-template __global__ void k_pairs<false>(const PairArgs *, const int2 *, int);
-template __global__ void k_pairs<true>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<false, false>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<true, false>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<false, true>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<true, true>(const PairArgs *, const int2 *, int);
 
 // Per-slot LJ type, ee flag and the low part of the charge (at list build).
 __global__ void k_slot_info(int nslot, const int *__restrict__ clu_atom, const int *__restrict__ atom_ljtype,
@@ -219,10 +271,13 @@ __global__ void k_slot_info(int nslot, const int *__restrict__ clu_atom, const i
 
 // One special pair per thread, reference expressions in FP64: LOCALF
 // (forces.f:835-986) for the short classes, FORCES (1039-1198) for long.
+template <bool DETAIL>
 __device__ inline void special_body(const SpecialArgs &a, int blk, double *red, int group) {
     const int k = blk * blockDim.x + threadIdx.x;
     double q[Q_N];
     for (int i = 0; i < Q_N; ++i) q[i] = 0.0;
+    int dt = -1;                 // molecule type of an evaluated pair (detail)
+    double dcoul = 0.0, dlj = 0.0;
     if (k < a.ns) {
         const int c = a.cls[k];
         const bool use = c != SP_NONE && (group == 0 || (group == 1) == (c != SP_LONG));
@@ -266,6 +321,12 @@ __device__ inline void special_body(const SpecialArgs &a, int blk, double *red, 
                 const double fsr = (6.0 * a6 + 12.0 * b12 * r6i) * r6i * r1i;
                 if (b12 != 0.0) force = force + fsr * r1i;
                 const double fxx = force * dx, fyy = force * dy, fzz = force * dz;
+                if (DETAIL) {
+                    // special pairs lie inside one molecule: PES141 / PSR141
+                    dt = ityp;
+                    dcoul = ee1;
+                    dlj = b12 != 0.0 ? esr : 0.0;
+                }
                 if (localf) {
                     q[Q_NPS] = 1.0;
                     q[Q_PELS2] = ees;
@@ -299,13 +360,43 @@ __device__ inline void special_body(const SpecialArgs &a, int blk, double *red, 
         if (threadIdx.x == 0) a.epart[(size_t)blk * Q_N + i] = red[0];
         __syncthreads();
     }
+    if (DETAIL) {
+        double *o = a.dpart + (size_t)blk * a.ndet;
+        for (int c = threadIdx.x; c < 2 * a.ntp; c += blockDim.x) o[c] = 0.0;
+        for (int c = 0; c < 2 * a.nmt; ++c) {
+            const int t = c < a.nmt ? c : c - a.nmt;
+            red[threadIdx.x] = dt == t ? (c < a.nmt ? dcoul : dlj) : 0.0;
+            __syncthreads();
+            for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+                if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) o[2 * a.ntp + c] = red[0];
+            __syncthreads();
+        }
+    }
 }
 
+template <bool DETAIL>
 __global__ __launch_bounds__(256) void k_special(const SpecialArgs *__restrict__ args, const int2 *__restrict__ map,
                                                  int group) {
     __shared__ double red[256];
     const int2 m = map[blockIdx.x];
-    special_body(args[m.x], m.y, red, group);
+    special_body<DETAIL>(args[m.x], m.y, red, group);
+}
+
+template __global__ void k_special<false>(const SpecialArgs *, const int2 *, int);
+template __global__ void k_special<true>(const SpecialArgs *, const int2 *, int);
+
+// Column sums of the detail rows of one system, in row order.
+__global__ __launch_bounds__(256) void k_detail_reduce(const DetailArgs *__restrict__ args,
+                                                       const int2 *__restrict__ map) {
+    const DetailArgs a = args[map[blockIdx.x].x];
+    for (int c = threadIdx.x; c < a.ndet; c += blockDim.x) {
+        double s = 0.0;
+        for (int r = 0; r < a.nrows; ++r) s += a.dpart[(size_t)r * a.ndet + c];
+        a.out[c] = s;
+    }
 }
 
 }  // namespace mdx

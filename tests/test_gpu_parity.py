@@ -92,6 +92,35 @@ def test_forces_energies_vs_reference(name):
         check_energies(e, ref)
 
 
+@pytest.mark.parametrize("name", CASES)
+def test_energy_breakdown_vs_reference(name):
+    """MDX_WANT_DETAIL: bare Coulomb and LJ energy per molecule-type pair
+    (POTE1, POTL1) and per molecule type inside molecules (PES141, PSR141)
+    as LOCALF/FORCES accumulate them; each entry within 1e-5 relative (an
+    entry much smaller than the rest is held to 1e-5 of 1e-3 x the sum of
+    magnitudes)."""
+    d, topo, prm, eng = _engine_from_golden(name)
+    print("case", name, topo.natoms, flush=True)
+    eng.set_positions(golden_positions(d))
+    eng.chcnb()
+    double = int(d["SIZES"][0][0]) == 2
+    nt, npairs = topo.nmoltypes, topo.ntypepairs
+    for call in range(len(d["SX"])):
+        eng.set_positions(golden_positions(d, call))
+        eng.set_molecule_com(golden_com(d, call))
+        if double:
+            eng.slforces(want=abi.MDX_WANT_DETAIL)
+            _, _, _, e = eng.fforces(want=abi.MDX_WANT_DETAIL)   # FFORCES leaves the fast breakdown
+        else:
+            _, _, _, e = eng.allforce(want=abi.MDX_WANT_DETAIL)
+        refs = {"pote1": d["POTE1"][call][:npairs], "potl1": d["POTL1"][call][:npairs],
+                "pes141": d["PES141"][call][:nt], "psr141": d["PSR141"][call][:nt]}
+        scale = sum(np.abs(v).sum() for v in refs.values())
+        for key, ref in refs.items():
+            err = np.abs(e[key] - ref)
+            assert np.all(err <= COMPONENT_TOL * np.maximum(np.abs(ref), 1e-3 * scale)), (key, e[key], ref)
+
+
 def _system_engine(s, list_mode=None):
     topo, prm = topology.from_system(s)
     if list_mode is not None:
@@ -123,10 +152,9 @@ def test_verlet_mode_matches_fresh_reference_lists():
     # pairs that crossed R_short since then sit in the other class than in
     # a list rebuilt at pos2: compare the class sums.
     merged = lambda d: dict(d, pe1=d["pe1"] + d["pe2"], pe2=0.0, pels1=d["pels1"] + d["pels2"], pels2=0.0,
-                            vir1=d["vir1"] + d["vir2"], vir2=0.0)
-    check_energies(merged(e), merged(eo),
-                   keys=("pe1", "pels1", "spe", "pint", "vir1", "virb", "virx", "viry", "virz", "virfx", "virfy",
-                         "virfz", "wirs"))
+                            vir1=d["vir1"] + d["vir2"], vir2=0.0, virx=d["virx"] + d["virfx"],
+                            viry=d["viry"] + d["virfy"], virz=d["virz"] + d["virfz"])
+    check_energies(merged(e), merged(eo), keys=("pe1", "pels1", "spe", "pint", "vir1", "virb", "virx", "viry", "virz"))
     assert e["npairs_short"] + e["npairs_long"] == eo["npairs_short"] + eo["npairs_long"]
 
 
