@@ -258,6 +258,49 @@ mdx_status mdx_batch_compute(mdx_batch *b, int32_t group, int32_t want, mdx_ener
 const char *mdx_batch_last_error(const mdx_batch *b);
 mdx_status mdx_batch_destroy(mdx_batch *b);
 
+/*
+ * Device-resident MD segments: the leap-frog step of SULTAN
+ * (mdstep.f:29-260) with SHEJK bond constraints (mdstep.f:1105-1283), the
+ * Nose-Hoover thermostat of SCALING(3) (mdstep.f:559-589, 908-925) and
+ * GETCOM molecule wrapping (service.f:463-563), all in FP64 on the device;
+ * the list is rebuilt every rebuild_every steps (ICHNB) and the forces are
+ * the ALLFORCE of this engine.  Units are the caller's: acceleration =
+ * accel_factor * F / m, kinetic energy = ekin_factor * m v^2 / 2,
+ * T = 2 E_kin / (ndof * boltz).
+ */
+typedef struct mdx_md_params {
+    double dt;                /* TSTEP (time unit of the velocities)        */
+    double accel_factor;
+    double ekin_factor;
+    double boltz;             /* k_B in energy units per K                  */
+    double temperature;       /* TRTEMP                                     */
+    double tau_t;             /* Nose-Hoover QT (time units); 0: NVE        */
+    double sc0;               /* initial thermostat variable SC (1/time)    */
+    double ndof;              /* degrees of freedom FNST                    */
+    double shake_tol;         /* TOLER                                      */
+    double shake_bdtol;       /* BDTOL                                      */
+    int32_t shake_maxit;      /* MAXIT (2000)                               */
+    int32_t rebuild_every;    /* ICHNB                                      */
+    int32_t step0;            /* MSTEP before the first step (NSTTOT)       */
+    int32_t ncons;            /* constrained bonds, grouped by molecule,    */
+    const int32_t *cons_i;    /*   in the reference bond order (0-based)    */
+    const int32_t *cons_j;
+    const double *cons_len;   /* RB                                         */
+} mdx_md_params;
+
+/* Start a segment from the current positions and the given velocities:
+ * builds the list and evaluates the forces (SULTAN's CHCNB + ALLFORCE). */
+mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, const double *vy,
+                       const double *vz);
+/* nsteps steps; trace (NULL or nsteps x 4 doubles): per step the potential
+ * energy, kinetic energy, temperature and SC after the step. */
+mdx_status mdx_md_run(mdx_engine *h, int32_t nsteps, double *trace);
+mdx_status mdx_md_get_state(mdx_engine *h, double *x, double *y, double *z, double *vx, double *vy, double *vz);
+/* nsteps steps of every system of a batch (each initialised by
+ * mdx_md_init), one fused launch per kernel and step; trace: K x nsteps x 4
+ * or NULL. */
+mdx_status mdx_batch_md_run(mdx_batch *b, int32_t nsteps, double *trace);
+
 /* Timing of the last mdx_build_list and mdx_compute per stage, in ms (HIP
  * events on the handle's stream): list build, prep (pack + COM), pairs,
  * special pairs, k-space prep, structure factors, S(k) reduction,

@@ -68,6 +68,38 @@ class MdxParams(C.Structure):
     ]
 
 
+class MdxMdParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("accel_factor", C.c_double), ("ekin_factor", C.c_double), ("boltz", C.c_double),
+        ("temperature", C.c_double), ("tau_t", C.c_double), ("sc0", C.c_double), ("ndof", C.c_double),
+        ("shake_tol", C.c_double), ("shake_bdtol", C.c_double), ("shake_maxit", C.c_int32),
+        ("rebuild_every", C.c_int32), ("step0", C.c_int32), ("ncons", C.c_int32),
+        ("cons_i", C.POINTER(C.c_int32)), ("cons_j", C.POINTER(C.c_int32)), ("cons_len", C.POINTER(C.c_double)),
+    ]
+
+
+def make_md_params(d: Dict):
+    """MdxMdParams from a dict (topology.md_params); returns (struct, arrays
+    kept alive by the caller)."""
+    p = MdxMdParams()
+    keep = []
+    for name, ctype in MdxMdParams._fields_:
+        if name in ("cons_i", "cons_j", "cons_len"):
+            continue
+        if name == "ncons":
+            continue
+        setattr(p, name, d[name])
+    ci = np.ascontiguousarray(d["cons_i"], np.int32)
+    cj = np.ascontiguousarray(d["cons_j"], np.int32)
+    ln = np.ascontiguousarray(d["cons_len"], np.float64)
+    p.ncons = len(ci)
+    p.cons_i = ci.ctypes.data_as(C.POINTER(C.c_int32))
+    p.cons_j = cj.ctypes.data_as(C.POINTER(C.c_int32))
+    p.cons_len = ln.ctypes.data_as(C.POINTER(C.c_double))
+    keep += [ci, cj, ln]
+    return p, keep
+
+
 class MdxEnergies(C.Structure):
     _fields_ = [
         ("pe1", C.c_double), ("pe2", C.c_double), ("pels1", C.c_double), ("pels2", C.c_double),

@@ -24,6 +24,7 @@
 #include "pair.hip"
 #include "recip.hip"
 #include "bonded.hip"
+#include "md.hip"
 
 using namespace mdx;
 
@@ -133,6 +134,18 @@ struct mdx_engine {
     double stage_ms[ST_N] = {};
     hipEvent_t ev[2] = {};
     bool timing = true;
+    // device-resident MD segment
+    bool md_on = false;
+    mdx_md_params md{};
+    long long mstep = 0;
+    int ncons = 0;
+    double *d_v = nullptr, *d_o = nullptr, *d_u = nullptr;   // [3][n] each
+    double *d_minv = nullptr, *d_mdstate = nullptr, *d_kepart = nullptr;
+    int *d_cstart = nullptr;
+    int2 *d_cons = nullptr;
+    double *d_cons_len2 = nullptr;
+    std::vector<int> h_mstart;
+    std::vector<double> h_mass;
 };
 
 // A launch plan over K systems: per kernel an array of argument structs and
@@ -181,6 +194,10 @@ struct mdx_batch {
     int nmt_max = 1;
     double *d_det = nullptr;         // [K][kMaxDet]
     double *h_det = nullptr;         // pinned copy
+    // MD segments
+    KPlan<MdArgs> md_kd, md_sh, md_ke, md_th, md_gc, md_tr;
+    double *d_trace = nullptr;
+    size_t trace_cap = 0;
     double *d_energy = nullptr;      // [K][Q_N]
     double *h_energy = nullptr;      // pinned copy
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
@@ -698,6 +715,8 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
         mstart[m + 1] = i + 1;
     }
     if ((s = upload(h, &h->d_mstart, mstart.data(), h->nmol + 1))) return fail(s);
+    h->h_mstart = mstart;
+    h->h_mass = mass;
     // special-pair candidates and exclusion-correction pairs
     std::vector<int2> et;
     for (int i = 0; i < n; ++i) {
@@ -1267,7 +1286,9 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
     mark(9);
 }
 
-mdx_status batch_compute(mdx_batch *b, int group, bool detail) {
+// Queue one force call for every system of the batch on the batch stream
+// (no host synchronisation).  events: eager launches with stage events.
+mdx_status batch_launch(mdx_batch *b, int group, bool detail, bool events) {
     if (group < 0 || group > 2) return MDX_ERR_ARG;
     if (detail)
         for (mdx_engine *h : b->sys) {
@@ -1297,7 +1318,7 @@ mdx_status batch_compute(mdx_batch *b, int group, bool detail) {
                                   b->zero.nblocks(), b->maxlen};
         if (before != after) drop_graphs(b);
     }
-    if (b->use_graph && !b->timing && !detail) {
+    if (b->use_graph && !events && !detail) {
         if (!b->exec[group]) {
             HIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             launch_call(b, group, false, false);
@@ -1306,8 +1327,13 @@ mdx_status batch_compute(mdx_batch *b, int group, bool detail) {
         }
         HIPCHK(b, hipGraphLaunch(b->exec[group], b->stream));
     } else {
-        launch_call(b, group, b->timing, detail);
+        launch_call(b, group, events, detail);
     }
+    return MDX_OK;
+}
+
+mdx_status batch_compute(mdx_batch *b, int group, bool detail) {
+    CHK(batch_launch(b, group, detail, b->timing));
     HIPCHK(b, hipStreamSynchronize(b->stream));
     HIPCHK(b, hipGetLastError());
     if (b->timing) {
@@ -1352,6 +1378,122 @@ mdx_status copy_forces(mdx_engine *h, double *fx, double *fy, double *fz, double
         if (outs[k])
             HIPCHK(h, hipMemcpyAsync(outs[k], h->d_out + (size_t)k * h->n, sizeof(double) * h->n, kind, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MDX_OK;
+}
+
+
+MdArgs md_args(mdx_batch *b, int s, double *trace, int cap) {
+    mdx_engine *h = b->sys[s];
+    const int n = h->n;
+    MdArgs a{};
+    a.n = n;
+    a.nmol = h->nmol;
+    a.x = h->d_x; a.y = h->d_y; a.z = h->d_z;
+    a.vx = h->d_v; a.vy = h->d_v + n; a.vz = h->d_v + 2 * (size_t)n;
+    a.ox = h->d_o; a.oy = h->d_o + n; a.oz = h->d_o + 2 * (size_t)n;
+    a.ux = h->d_u; a.uy = h->d_u + n; a.uz = h->d_u + 2 * (size_t)n;
+    a.fx = h->d_out; a.fy = h->d_out + n; a.fz = h->d_out + 2 * (size_t)n;
+    a.minv = h->d_minv;
+    a.mass = h->d_mass;
+    a.mstart = h->d_mstart;
+    a.cstart = h->d_cstart;
+    a.cons = h->d_cons;
+    a.cons_len2 = h->d_cons_len2;
+    a.dt = h->md.dt;
+    a.accel = h->md.accel_factor;
+    a.tol2 = h->md.shake_tol * h->md.shake_tol;
+    a.bdtol = h->md.shake_bdtol;
+    a.maxit = h->md.shake_maxit;
+    a.nvt = h->md.tau_t > 0.0 ? 1 : 0;
+    a.box = h->box;
+    a.ekin_factor = h->md.ekin_factor;
+    a.boltz = h->md.boltz;
+    a.ndof = h->md.ndof;
+    a.t0 = h->md.temperature;
+    a.dqt = h->md.tau_t > 0.0 ? 1.0 / (h->md.tau_t * h->md.tau_t) : 0.0;
+    a.state = h->d_mdstate;
+    a.kepart = h->d_kepart;
+    a.nke_blocks = cdiv(n, 256);
+    a.energy = b->d_energy + (size_t)s * Q_N;
+    a.spei = h->spei;
+    a.trace = trace;
+    a.trace_cap = cap;
+    return a;
+}
+
+mdx_status md_plan(mdx_batch *b, int nsteps, bool want_trace) {
+    const int K = (int)b->sys.size();
+    if (want_trace && (size_t)K * nsteps * 4 > b->trace_cap) {
+        if (b->d_trace) hipFree(b->d_trace);
+        b->d_trace = nullptr;
+        b->trace_cap = 0;
+        if (hipMalloc((void **)&b->d_trace, sizeof(double) * (size_t)K * nsteps * 4) != hipSuccess) {
+            b->err = "hipMalloc of the MD trace failed";
+            return MDX_ERR_NOMEM;
+        }
+        b->trace_cap = (size_t)K * nsteps * 4;
+    }
+    b->md_kd.clear(); b->md_sh.clear(); b->md_ke.clear(); b->md_th.clear(); b->md_gc.clear(); b->md_tr.clear();
+    for (int s = 0; s < K; ++s) {
+        mdx_engine *h = b->sys[s];
+        const MdArgs a = md_args(b, s, want_trace ? b->d_trace + (size_t)s * nsteps * 4 : nullptr, nsteps);
+        b->md_kd.add(a, cdiv(h->n, 256));
+        b->md_sh.add(a, cdiv(h->nmol, 256));
+        b->md_ke.add(a, cdiv(h->n, 256));
+        b->md_th.add(a, 1);
+        b->md_gc.add(a, cdiv(h->nmol, 256));
+        b->md_tr.add(a, 1);
+    }
+    CHK(kplan_upload(b, b->md_kd, true)); CHK(kplan_upload(b, b->md_sh, true));
+    CHK(kplan_upload(b, b->md_ke, true)); CHK(kplan_upload(b, b->md_th, true));
+    CHK(kplan_upload(b, b->md_gc, true)); CHK(kplan_upload(b, b->md_tr, true));
+    return MDX_OK;
+}
+
+// nsteps leap-frog steps of every system; lists are rebuilt (on the
+// systems' own streams, after the batch stream drains) when a system's
+// MSTEP reaches a multiple of its ICHNB.
+mdx_status batch_md_run(mdx_batch *b, int nsteps, double *trace) {
+    if (nsteps < 0) return MDX_ERR_ARG;
+    for (mdx_engine *h : b->sys)
+        if (!h->md_on) {
+            b->err = "mdx_md_run before mdx_md_init";
+            return MDX_ERR_STATE;
+        }
+    for (mdx_engine *h : b->sys)
+        if (h->stream != b->stream) hipStreamSynchronize(h->stream);
+    CHK(md_plan(b, std::max(nsteps, 1), trace != nullptr));
+    for (mdx_engine *h : b->sys)
+        HIPCHK(b, hipMemsetAsync(h->d_mdstate + MD_STEP, 0, sizeof(double), b->stream));
+    hipStream_t st = b->stream;
+    for (int step = 0; step < nsteps; ++step) {
+        launch(b, b->md_kd, 256, 0, k_md_kick_drift);
+        launch(b, b->md_sh, 256, 0, k_md_shake);
+        launch(b, b->md_ke, 256, 0, k_md_scale_ke);
+        launch(b, b->md_th, 256, 0, k_md_thermo);
+        launch(b, b->md_gc, 256, 0, k_md_getcom);
+        bool synced = false;
+        for (mdx_engine *h : b->sys) {
+            h->mstep += 1;
+            if (h->md.rebuild_every > 0 && h->mstep % h->md.rebuild_every == 0) {
+                if (!synced) {
+                    HIPCHK(b, hipStreamSynchronize(st));
+                    synced = true;
+                }
+                mdx_status s = mdx_build_list(h);
+                if (s != MDX_OK) {
+                    b->err = h->err;
+                    return s;
+                }
+            }
+        }
+        CHK(batch_launch(b, MDX_FORCES_ALL, false, false));
+        launch(b, b->md_tr, 64, 0, k_md_trace);
+    }
+    HIPCHK(b, hipStreamSynchronize(st));
+    HIPCHK(b, hipGetLastError());
+    if (trace && nsteps > 0)
+        HIPCHK(b, hipMemcpy(trace, b->d_trace, sizeof(double) * b->sys.size() * nsteps * 4, hipMemcpyDeviceToHost));
     return MDX_OK;
 }
 
@@ -1416,6 +1558,96 @@ mdx_status mdx_memory_bytes(const mdx_engine *h, int64_t *bytes) {
     int64_t t = 0;
     for (auto &kv : h->dev_sizes) t += (int64_t)kv.second;
     *bytes = t;
+    return MDX_OK;
+}
+
+mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, const double *vy,
+                       const double *vz) {
+    if (!h || !p || !vx || !vy || !vz) return MDX_ERR_ARG;
+    if (!h->have_pos) {
+        h->err = "mdx_md_init before mdx_set_positions";
+        return MDX_ERR_STATE;
+    }
+    if (!(p->dt > 0.0) || !(p->ndof > 0.0) || !(p->boltz > 0.0) || p->ncons < 0 ||
+        (p->ncons > 0 && (!p->cons_i || !p->cons_j || !p->cons_len)) || p->shake_maxit < 0) {
+        h->err = "invalid MD parameters";
+        return MDX_ERR_ARG;
+    }
+    const int n = h->n;
+    // constraints grouped by molecule, both atoms in it, at most 64 atoms
+    std::vector<int> cstart(h->nmol + 1, 0);
+    std::vector<int2> cons(std::max(1, p->ncons));
+    std::vector<double> len2(std::max(1, p->ncons));
+    int prev = -1;
+    for (int c = 0; c < p->ncons; ++c) {
+        const int i = p->cons_i[c], j = p->cons_j[c];
+        if (i < 0 || i >= n || j < 0 || j >= n || i == j) return h->err = "constraint atom out of range", MDX_ERR_ARG;
+        const int m = h->h_mol[i];
+        if (h->h_mol[j] != m || m < prev) return h->err = "constraints must be grouped by molecule", MDX_ERR_ARG;
+        if (h->h_mstart[m + 1] - h->h_mstart[m] > 64)
+            return h->err = "constrained molecules may have at most 64 atoms", MDX_ERR_ARG;
+        prev = m;
+        cstart[m + 1] += 1;
+        cons[c] = make_int2(i, j);
+        len2[c] = p->cons_len[c] * p->cons_len[c];
+    }
+    for (int m = 0; m < h->nmol; ++m) cstart[m + 1] += cstart[m];
+    std::vector<double> minv(n);
+    for (int i = 0; i < n; ++i) minv[i] = 1.0 / h->h_mass[i];
+    std::vector<double> v(3 * (size_t)n);
+    std::memcpy(v.data(), vx, sizeof(double) * n);
+    std::memcpy(v.data() + n, vy, sizeof(double) * n);
+    std::memcpy(v.data() + 2 * (size_t)n, vz, sizeof(double) * n);
+    std::vector<double> state(MD_NSTATE, 0.0);
+    state[MD_SC] = p->sc0;
+    mdx_status s;
+    if ((s = upload(h, &h->d_v, v.data(), v.size())) || (s = dalloc(h, &h->d_o, 3 * (size_t)n)) ||
+        (s = dalloc(h, &h->d_u, 3 * (size_t)n)) || (s = upload(h, &h->d_minv, minv.data(), n)) ||
+        (s = upload(h, &h->d_mdstate, state.data(), MD_NSTATE)) || (s = dalloc(h, &h->d_kepart, cdiv(n, 256))) ||
+        (s = upload(h, &h->d_cstart, cstart.data(), cstart.size())) ||
+        (s = upload(h, &h->d_cons, cons.data(), cons.size())) ||
+        (s = upload(h, &h->d_cons_len2, len2.data(), len2.size())))
+        return s;
+    h->md = *p;
+    h->md.cons_i = h->md.cons_j = nullptr;
+    h->md.cons_len = nullptr;
+    h->ncons = p->ncons;
+    h->mstep = p->step0;
+    h->md_on = true;
+    // SULTAN's start: list and forces at the initial positions
+    CHK(mdx_build_list(h));
+    mdx_batch *b = h->self;
+    b->timing = false;
+    s = batch_compute(b, MDX_FORCES_ALL, false);
+    b->timing = h->timing;
+    if (s != MDX_OK) h->err = b->err;
+    return s;
+}
+
+mdx_status mdx_md_run(mdx_engine *h, int32_t nsteps, double *trace) {
+    if (!h) return MDX_ERR_ARG;
+    mdx_status s = batch_md_run(h->self, nsteps, trace);
+    if (s != MDX_OK) h->err = h->self->err;
+    return s;
+}
+
+mdx_status mdx_batch_md_run(mdx_batch *b, int32_t nsteps, double *trace) {
+    if (!b) return MDX_ERR_ARG;
+    return batch_md_run(b, nsteps, trace);
+}
+
+mdx_status mdx_md_get_state(mdx_engine *h, double *x, double *y, double *z, double *vx, double *vy, double *vz) {
+    if (!h) return MDX_ERR_ARG;
+    if (!h->md_on) {
+        h->err = "mdx_md_get_state before mdx_md_init";
+        return MDX_ERR_STATE;
+    }
+    const size_t b = sizeof(double) * h->n;
+    const double *src[6] = {h->d_x, h->d_y, h->d_z, h->d_v, h->d_v + h->n, h->d_v + 2 * (size_t)h->n};
+    double *dst[6] = {x, y, z, vx, vy, vz};
+    for (int k = 0; k < 6; ++k)
+        if (dst[k]) HIPCHK(h, hipMemcpyAsync(dst[k], src[k], b, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return MDX_OK;
 }
 
@@ -1486,6 +1718,9 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
     kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->angles); kplan_free(b->tors);
     kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);
+    kplan_free(b->md_kd); kplan_free(b->md_sh); kplan_free(b->md_ke); kplan_free(b->md_th);
+    kplan_free(b->md_gc); kplan_free(b->md_tr);
+    if (b->d_trace) hipFree(b->d_trace);
     if (b->d_energy) hipFree(b->d_energy);
     if (b->h_energy) hipHostFree(b->h_energy);
     if (b->d_det) hipFree(b->d_det);

@@ -1,0 +1,197 @@
+// Device-resident leap-frog integrator with bond constraints and the
+// Nose-Hoover thermostat of the reference's single-time-step algorithm
+// (SULTAN mdstep.f:29-260, SHEJK mdstep.f:1105-1283, SCALING(3)
+// mdstep.f:559-589 and 908-925, GETEMP service.f:35-92, GETCOM
+// service.f:463-563).  Everything in FP64; the step-by-step order is the
+// reference's:
+//   OX = S; V += dt a(F); S += dt V                (k_md_kick_drift)
+//   SHAKE each constrained molecule, V += dS/dt    (k_md_shake)
+//   V *= 1 - SC dt (NVT); kinetic energy           (k_md_scale_ke)
+//   T; SC += (T/T0 - 1) dt / QT^2                  (k_md_thermo)
+//   wrap molecules by their centre of mass         (k_md_getcom)
+//   [list rebuild every ICHNB steps, on the host]
+//   forces at t + dt (the force-call kernels)
+// Velocities are in length per time (the reference keeps momenta in
+// internal units; the update V += dt F/m, S += dt V is the same map).
+#include "mdx_internal.h"
+
+namespace mdx {
+
+__device__ inline void md_kick_drift_body(const MdArgs &a, int blk) {
+    const int i = blk * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const double s = a.dt * a.accel * a.minv[i];
+    double x = a.x[i], y = a.y[i], z = a.z[i];
+    a.ox[i] = x; a.oy[i] = y; a.oz[i] = z;
+    const double vx = a.vx[i] + s * a.fx[i], vy = a.vy[i] + s * a.fy[i], vz = a.vz[i] + s * a.fz[i];
+    a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
+    x = x + a.dt * vx; y = y + a.dt * vy; z = z + a.dt * vz;
+    a.x[i] = x; a.y[i] = y; a.z[i] = z;
+    a.ux[i] = x; a.uy[i] = y; a.uz[i] = z;
+}
+
+// One molecule per thread, the reference's iteration: every constraint
+// whose atoms moved in the previous sweep is corrected along the old bond
+// vector until |r^2 - RB^2| <= RB^2 TOLER^2 for all of them.
+__device__ inline void md_shake_body(const MdArgs &a, int blk) {
+#pragma clang fp contract(off)
+    const int m = blk * 256 + threadIdx.x;
+    if (m >= a.nmol) return;
+    const int c0 = a.cstart[m], c1 = a.cstart[m + 1];
+    if (c0 == c1) return;
+    const int a0 = a.mstart[m], a1 = a.mstart[m + 1];
+    unsigned long long moved = 0ull, moving = 0ull;   // bit (atom - a0)
+    for (int c = c0; c < c1; ++c) {
+        moved |= 1ull << (a.cons[c].x - a0);
+        moved |= 1ull << (a.cons[c].y - a0);
+    }
+    bool done = false;
+    int iter = 0;
+    while (!done && iter <= a.maxit) {
+        done = true;
+        for (int c = c0; c < c1; ++c) {
+            const int i = a.cons[c].x, j = a.cons[c].y;
+            const unsigned long long bi = 1ull << (i - a0), bj = 1ull << (j - a0);
+            if (!((moved & bi) || (moved & bj))) continue;
+            const double bndsq = a.cons_len2[c];
+            double bx = a.x[i] - a.x[j], by = a.y[i] - a.y[j], bz = a.z[i] - a.z[j];
+            min_image(bx, by, bz, a.box);
+            double bsq = bx * bx + by * by;
+            bsq = bsq + bz * bz;
+            const double diffsq = bndsq - bsq;
+            if (fabs(diffsq) > bndsq * a.tol2) {
+                double dx = a.ox[i] - a.ox[j], dy = a.oy[i] - a.oy[j], dz = a.oz[i] - a.oz[j];
+                min_image(dx, dy, dz, a.box);
+                double vdot = bx * dx + by * dy;
+                vdot = vdot + bz * dz;
+                if (vdot < bndsq * a.bdtol) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
+                const double rma = a.minv[i], rmb = a.minv[j];
+                const double gab = diffsq / (2.0 * (rma + rmb) * vdot);
+                dx = dx * gab; dy = dy * gab; dz = dz * gab;
+                a.x[i] = a.x[i] + rma * dx; a.y[i] = a.y[i] + rma * dy; a.z[i] = a.z[i] + rma * dz;
+                a.x[j] = a.x[j] - rmb * dx; a.y[j] = a.y[j] - rmb * dy; a.z[j] = a.z[j] - rmb * dz;
+                moving |= bi | bj;
+                done = false;
+            }
+        }
+        // atoms moved in this sweep are checked in the next one
+        moved = moving;
+        moving = 0ull;
+        ++iter;
+    }
+    if (!done) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
+    for (int i = a0; i < a1; ++i) {
+        a.vx[i] = a.vx[i] + (a.x[i] - a.ux[i]) / a.dt;
+        a.vy[i] = a.vy[i] + (a.y[i] - a.uy[i]) / a.dt;
+        a.vz[i] = a.vz[i] + (a.z[i] - a.uz[i]) / a.dt;
+    }
+}
+
+__device__ inline void md_scale_ke_body(const MdArgs &a, int blk, double *red) {
+    const int i = blk * 256 + threadIdx.x;
+    double v2 = 0.0;
+    if (i < a.n) {
+        double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+        if (a.nvt) {
+            const double scv = 1.0 - a.state[MD_SC] * a.dt;
+            vx *= scv; vy *= scv; vz *= scv;
+            a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
+        }
+        v2 = a.mass[i] * (vx * vx + vy * vy + vz * vz);
+    }
+    red[threadIdx.x] = v2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.kepart[blk] = red[0];
+}
+
+__device__ inline void md_thermo_body(const MdArgs &a, double *red) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.nke_blocks; b += 256) s += a.kepart[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double ekin = 0.5 * a.ekin_factor * red[0];
+        const double temp = 2.0 * ekin / (a.ndof * a.boltz);
+        a.state[MD_EKIN] = ekin;
+        a.state[MD_TEMP] = temp;
+        if (a.nvt) a.state[MD_SC] = a.state[MD_SC] + (temp / a.t0 - 1.0) * a.dqt * a.dt;
+    }
+}
+
+// GETCOM: S = (S - R_com) + PBC(R_com), per molecule.
+__device__ inline void md_getcom_body(const MdArgs &a, int blk) {
+#pragma clang fp contract(off)
+    const int m = blk * 256 + threadIdx.x;
+    if (m >= a.nmol) return;
+    const int a0 = a.mstart[m], a1 = a.mstart[m + 1];
+    double cx = 0.0, cy = 0.0, cz = 0.0, sm = 0.0;
+    for (int i = a0; i < a1; ++i) {
+        cx = cx + a.mass[i] * a.x[i];
+        cy = cy + a.mass[i] * a.y[i];
+        cz = cz + a.mass[i] * a.z[i];
+        sm = sm + a.mass[i];
+    }
+    cx = cx / sm; cy = cy / sm; cz = cz / sm;
+    double px = cx, py = cy, pz = cz;
+    min_image(px, py, pz, a.box);
+    for (int i = a0; i < a1; ++i) {
+        a.x[i] = (a.x[i] - cx) + px;
+        a.y[i] = (a.y[i] - cy) + py;
+        a.z[i] = (a.z[i] - cz) + pz;
+    }
+}
+
+__device__ inline void md_trace_body(const MdArgs &a) {
+    if (threadIdx.x != 0) return;
+    const int step = (int)a.state[MD_STEP];
+    a.state[MD_STEP] = step + 1;
+    if (!a.trace || step >= a.trace_cap) return;
+    const double *e = a.energy;
+    double *t = a.trace + (size_t)step * 4;
+    t[0] = e[Q_PE1] + e[Q_PE2] + e[Q_PELS1] + e[Q_PELS2] + e[Q_SPE] + a.spei + e[Q_PINT];
+    t[1] = a.state[MD_EKIN];
+    t[2] = a.state[MD_TEMP];
+    t[3] = a.state[MD_SC];
+}
+
+__global__ __launch_bounds__(256) void k_md_kick_drift(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_kick_drift_body(args[m.x], m.y);
+}
+
+__global__ __launch_bounds__(256) void k_md_shake(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_shake_body(args[m.x], m.y);
+}
+
+__global__ __launch_bounds__(256) void k_md_scale_ke(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ double red[256];
+    const int2 m = map[blockIdx.x];
+    md_scale_ke_body(args[m.x], m.y, red);
+}
+
+__global__ __launch_bounds__(256) void k_md_thermo(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ double red[256];
+    const int2 m = map[blockIdx.x];
+    md_thermo_body(args[m.x], red);
+}
+
+__global__ __launch_bounds__(256) void k_md_getcom(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_getcom_body(args[m.x], m.y);
+}
+
+__global__ void k_md_trace(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_trace_body(args[m.x]);
+}
+
+}  // namespace mdx

@@ -262,4 +262,31 @@ struct ReduceArgs {
     double *out;                   // Q_N
 };
 
+// Device-resident leap-frog step (SULTAN, mdstep.f:29-260) of one system.
+enum MdState { MD_SC = 0, MD_EKIN, MD_TEMP, MD_SHAKE_FAIL, MD_STEP, MD_NSTATE };
+
+struct MdArgs {
+    int n, nmol;
+    double *x, *y, *z, *vx, *vy, *vz;
+    double *ox, *oy, *oz;          // constrained positions at t (OX, mdstep.f:82-89)
+    double *ux, *uy, *uz;          // unconstrained positions at t + dt
+    const double *fx, *fy, *fz;    // total force F at t
+    const double *minv, *mass;     // 1/m and m per atom
+    const int *mstart;             // atoms of molecule m: [mstart[m], mstart[m+1])
+    const int *cstart;             // constraints of molecule m
+    const int2 *cons;              // constrained atom pairs
+    const double *cons_len2;       // RB^2
+    double dt, accel, tol2, bdtol;
+    int maxit, nvt;
+    Box box;
+    double ekin_factor, boltz, ndof, t0, dqt;
+    double *state;                 // MdState
+    double *kepart;                // per-block m v^2 partials
+    int nke_blocks;
+    const double *energy;          // Q_N energy slots of the last force call
+    double spei;
+    double *trace;                 // [step][4]: Epot, Ekin, T, SC (step = state[MD_STEP])
+    int trace_cap;
+};
+
 }  // namespace mdx

@@ -70,11 +70,16 @@ def load_library() -> C.CDLL:
     L.mdx_batch_last_error.argtypes = [pv]
     L.mdx_batch_last_error.restype = C.c_char_p
     L.mdx_batch_destroy.argtypes = [pv]
+    L.mdx_md_init.argtypes = [pv, pv, pd, pd, pd]
+    L.mdx_md_run.argtypes = [pv, C.c_int32, pv]
+    L.mdx_md_get_state.argtypes = [pv] + [pd] * 6
+    L.mdx_batch_md_run.argtypes = [pv, C.c_int32, pv]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
                  "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_get_list", "mdx_compute",
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
-                 "mdx_batch_compute", "mdx_batch_destroy"):
+                 "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
+                 "mdx_batch_md_run"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -210,6 +215,26 @@ class ForceEngine:
         self._check(self.lib.mdx_get_forces(self._h, *[as_ptr(out[k]) for k in range(9)]))
         return out[0:3].T.copy(), out[3:6].T.copy(), out[6:9].T.copy()
 
+    # -- device-resident MD segments ------------------------------------
+    def md_init(self, md: Dict, velocities: np.ndarray) -> None:
+        """Start an MD segment (leap-frog + constraints + thermostat on the
+        device) from the current positions; `md` as topology.md_params."""
+        p, keep = abi.make_md_params(md)
+        v = [np.ascontiguousarray(velocities[:, k], dtype=np.float64) for k in range(3)]
+        self._check(self.lib.mdx_md_init(self._h, C.byref(p), *[as_ptr(a) for a in v]))
+
+    def md_run(self, nsteps: int, trace: bool = True):
+        """nsteps steps; returns the per-step (Epot, Ekin, T, SC) array."""
+        tr = np.zeros((max(nsteps, 1), 4)) if trace else None
+        self._check(self.lib.mdx_md_run(self._h, nsteps, as_ptr(tr) if trace else None))
+        return tr[:nsteps] if trace else None
+
+    def md_state(self):
+        """(positions, velocities) as (n, 3) arrays."""
+        out = np.zeros((6, self.n))
+        self._check(self.lib.mdx_md_get_state(self._h, *[as_ptr(out[k]) for k in range(6)]))
+        return out[0:3].T.copy(), out[3:6].T.copy()
+
     def memory_bytes(self) -> int:
         b = C.c_int64()
         self._check(self.lib.mdx_memory_bytes(self._h, C.byref(b)))
@@ -248,6 +273,14 @@ class ForceBatch:
         if not energies:
             return None
         return [e[s].as_dict(self.engines[s].topo.nmoltypes, self.engines[s].topo.ntypepairs) for s in range(k)]
+
+    def md_run(self, nsteps: int, trace: bool = True):
+        """nsteps MD steps of every engine (each set up by md_init); returns
+        a (K, nsteps, 4) trace."""
+        k = len(self.engines)
+        tr = np.zeros((k, max(nsteps, 1), 4)) if trace else None
+        self._check(self.lib.mdx_batch_md_run(self._b, nsteps, as_ptr(tr) if trace else None))
+        return tr[:, :nsteps] if trace else None
 
     def close(self) -> None:
         if self._b and self._b.value:

@@ -269,3 +269,39 @@ def from_dump(d: Dict[str, List[np.ndarray]], lam: float = 1.0) -> Tuple[Topolog
 
 def params_struct(params: Dict):
     return make_params(**params)
+
+
+# prcm.h:22 (reference physical constants)
+AVSNO = 6.02252e23
+BOLTZ_J = 1.38054e-23
+BOLTZ_KJMOL = AVSNO * BOLTZ_J * 1e-3     # kJ/mol/K
+
+
+def md_params(system, tolerance: float = 1e-6) -> Dict:
+    """MD-segment parameters of a generated system in the host units
+    (A, fs, kJ/mol, g/mol): the constrained bonds of every molecule when
+    the system runs with SHAKE (all bonds with ID != 2 of constrained
+    molecule types, in .mmol order, as SHEJK walks them, mdstep.f:1115-1140),
+    the degrees of freedom FNST (main.f:297-348: 3 per atom, minus one per
+    constraint, minus 3) and the Nose-Hoover time constant."""
+    ci, cj, ln = [], [], []
+    off = 0
+    ndof = 0.0
+    for t, cnt in zip(system.types, system.counts):
+        for m in range(cnt):
+            if system.shake:
+                for b in t.bonds:
+                    if b[0] != 2:
+                        ci.append(off + b[1] - 1)
+                        cj.append(off + b[2] - 1)
+                        ln.append(b[3])
+            off += t.nsites
+        ndof += 3.0 * t.nsites * cnt
+        if system.shake:
+            ndof -= sum(1 for b in t.bonds if b[0] != 2) * cnt
+    ndof -= 3.0
+    return dict(dt=system.dt_fs, accel_factor=1e-4, ekin_factor=1e4, boltz=BOLTZ_KJMOL,
+                temperature=system.temperature, tau_t=30.0, sc0=0.0, ndof=ndof, shake_tol=tolerance,
+                shake_bdtol=float(np.float32(0.0001)), shake_maxit=2000, rebuild_every=system.rebuild_every,
+                step0=0, cons_i=np.array(ci, np.int32), cons_j=np.array(cj, np.int32),
+                cons_len=np.array(ln, np.float64))

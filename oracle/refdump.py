@@ -50,3 +50,24 @@ def run_reference(system, workdir: str | None = None, timeout: float = 600.0) ->
     out = read_dump(dump)
     out["_stdout"] = [np.array(res.stdout)]
     return out
+
+
+TRAJ_EXE = os.path.join(HERE, "_ref", "mdx_traj")
+
+
+def run_trajectory(system, nsteps: int, workdir: str | None = None, timeout: float = 600.0):
+    """Run the reference integrator (oracle/ref/mdx_traj.f around the
+    reference SULTAN) for nsteps leap-frog steps; returns the dump with the
+    start state, the end state and the last energies."""
+    if not os.path.exists(TRAJ_EXE):
+        raise FileNotFoundError("trajectory harness not built: run make -C oracle/ref")
+    own = workdir is None
+    d = tempfile.mkdtemp(prefix="mdxtrj_") if own else workdir
+    system.write_reference_inputs(d, nsteps=nsteps)
+    res = subprocess.run([TRAJ_EXE, "md.in"], cwd=d, capture_output=True, text=True, timeout=timeout)
+    dump = os.path.join(d, "mdx_traj.bin")
+    if not os.path.exists(dump):
+        raise RuntimeError("reference run failed:\n" + res.stdout[-4000:] + res.stderr[-2000:])
+    out = read_dump(dump)
+    out["_stdout"] = [np.array(res.stdout)]
+    return out

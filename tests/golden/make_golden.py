@@ -8,6 +8,7 @@ outputs (neighbour lists or their digest, forces F/G/H, every energy and
 virial scalar, per-type breakdowns).  Fixtures are data only.
 
     python tests/golden/make_golden.py        # rewrites tests/golden/*.npz
+    python tests/golden/make_golden.py traj   # trajectory fixtures only
 """
 from __future__ import annotations
 
@@ -50,7 +51,58 @@ def list_digest(a: np.ndarray, b: np.ndarray) -> str:
                           np.ascontiguousarray(b, "<i4").tobytes()).hexdigest()
 
 
+# trajectory fixtures: name -> (builder, step counts); the reference
+# integrator (oracle/ref/mdx_traj.f around SULTAN) from the reference's own
+# start state
+TRAJ_CASES = {
+    "traj_water_508": (systems.cfg_small, (1, 20)),
+}
+
+
+def parse_steps(stdout: str) -> np.ndarray:
+    """Per-step lines of SULTAN (mdstep.f:187-190): MSTEP, POTEN, PENS,
+    PEL, ETOT, TEMP, TRYCKM, DENS."""
+    rows = []
+    for line in stdout.splitlines():
+        f = line.split()
+        if len(f) == 8:
+            try:
+                rows.append([float(x) for x in f])
+            except ValueError:
+                pass
+    return np.array(rows)
+
+
+def main_traj() -> None:
+    for name, (builder, counts) in TRAJ_CASES.items():
+        s = builder()
+        out = {}
+        for n in counts:
+            d = refdump.run_trajectory(s, n)
+            if "TSIZES" not in out:
+                out["TSIZES"] = d["TSIZES"][0]
+                out["TSCAL"] = d["TSCAL"][0]
+                out["MASSDI"] = d["MASSDI"][0]
+                for k in ("SX", "SY", "SZ", "VX", "VY", "VZ"):
+                    out[k + "_0"] = d[k][0]
+            else:
+                for k in ("SX", "VX"):
+                    assert np.array_equal(out[k + "_0"], d[k][0]), "start state differs between runs"
+            for k in ("SX", "SY", "SZ", "VX", "VY", "VZ"):
+                out["%s_%d" % (k, n)] = d[k][1]
+            out["TEND_%d" % n] = d["TEND"][0]
+            out["STEPS_%d" % n] = parse_steps(str(d["_stdout"][0]))
+        out["COUNTS"] = np.array(counts)
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **out)
+        print("%-16s atoms=%d steps=%s -> %s (%d KB)" % (name, s.natoms, counts, path, os.path.getsize(path) // 1024))
+
+
 def main() -> None:
+    if sys.argv[1:] == ["traj"]:
+        main_traj()
+        return
+    main_traj()
     for name, (builder, lam, full_lists) in CASES.items():
         s = builder()
         d = refdump.run_reference(s)
