@@ -234,6 +234,9 @@ mdx_status mdx_get_forces(mdx_engine *h, double *fx, double *fy, double *fz, dou
 mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
 /* Device memory held by a handle, in bytes (admission by HBM budget). */
 mdx_status mdx_memory_bytes(const mdx_engine *h, int64_t *bytes);
+/* Device memory mdx_create would allocate for this system (plus the MD
+ * segment state when with_md), computed without allocating. */
+mdx_status mdx_estimate_bytes(const mdx_topology *t, const mdx_params *p, int32_t with_md, int64_t *bytes);
 
 /*
  * Batches: K systems (path-integral beads, expanded-ensemble lambda states,
@@ -292,14 +295,55 @@ typedef struct mdx_md_params {
  * builds the list and evaluates the forces (SULTAN's CHCNB + ALLFORCE). */
 mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, const double *vy,
                        const double *vz);
-/* nsteps steps; trace (NULL or nsteps x 4 doubles): per step the potential
- * energy, kinetic energy, temperature and SC after the step. */
+/* nsteps steps; trace (NULL or nsteps x 5 doubles): per step the potential
+ * energy, kinetic energy, temperature, SC and the number of pair
+ * interactions of the step's force call. */
 mdx_status mdx_md_run(mdx_engine *h, int32_t nsteps, double *trace);
 mdx_status mdx_md_get_state(mdx_engine *h, double *x, double *y, double *z, double *vx, double *vy, double *vz);
 /* nsteps steps of every system of a batch (each initialised by
- * mdx_md_init), one fused launch per kernel and step; trace: K x nsteps x 4
+ * mdx_md_init), one fused launch per kernel and step; trace: K x nsteps x 5
  * or NULL. */
 mdx_status mdx_batch_md_run(mdx_batch *b, int32_t nsteps, double *trace);
+
+/*
+ * Serving: MD-segment jobs admitted by an HBM budget and advanced together
+ * (continuous batching).  A worker thread owns the device; jobs submitted
+ * while others run join the batch as slots and memory free up.  No
+ * counterpart in the reference (one job per program run).
+ */
+typedef struct mdx_job_desc {
+    const mdx_topology *topology;     /* read at admission (mdx_create)     */
+    const mdx_params *params;
+    const mdx_md_params *md;
+    const double *x, *y, *z;          /* start coordinates                  */
+    const double *vx, *vy, *vz;       /* start velocities                   */
+    int32_t nsteps;                   /* segment length                     */
+    double *x_out, *y_out, *z_out;    /* final state (any may be NULL)      */
+    double *vx_out, *vy_out, *vz_out;
+} mdx_job_desc;                       /* every array stays valid until the
+                                         job's result is polled            */
+typedef struct mdx_job_result {
+    int64_t id;
+    int32_t status;                   /* mdx_status of the job              */
+    int32_t nsteps;                   /* steps completed                    */
+    double t_submit, t_admit, t_done; /* seconds on the server clock        */
+    double epot, ekin, temperature;   /* after the last step                */
+    int64_t pair_interactions;        /* over all of the job's force calls  */
+    int64_t bytes;                    /* device memory estimate admitted    */
+    char message[128];
+} mdx_job_result;
+typedef struct mdx_server mdx_server;
+/* quantum: steps per scheduling round (admission/retirement granularity). */
+mdx_status mdx_serve_create(int64_t hbm_budget_bytes, int32_t max_slots, int32_t quantum, mdx_server **out);
+mdx_status mdx_serve_submit(mdx_server *s, const mdx_job_desc *job, int64_t *id);
+/* Completed jobs (up to max), and the current active / queued counts. */
+mdx_status mdx_serve_poll(mdx_server *s, mdx_job_result *out, int32_t max, int32_t *n, int32_t *active,
+                          int32_t *queued);
+/* Block until no job is queued or active (MDX_ERR_STATE on timeout). */
+mdx_status mdx_serve_wait(mdx_server *s, double timeout_s);
+mdx_status mdx_serve_stats(mdx_server *s, int64_t *rounds, double *busy_s, double *clock_s);
+const char *mdx_serve_last_error(const mdx_server *s);
+mdx_status mdx_serve_destroy(mdx_server *s);
 
 /* Timing of the last mdx_build_list and mdx_compute per stage, in ms (HIP
  * events on the handle's stream): list build, prep (pack + COM), pairs,

@@ -146,6 +146,9 @@ struct mdx_engine {
     double *d_cons_len2 = nullptr;
     std::vector<int> h_mstart;
     std::vector<double> h_mass;
+    // sizing pass of mdx_estimate_bytes: allocations are only counted
+    bool dry = false;
+    uintptr_t dry_next = 0;
 };
 
 // A launch plan over K systems: per kernel an array of argument structs and
@@ -227,6 +230,14 @@ namespace {
 
 template <typename T>
 mdx_status dalloc(mdx_engine *h, T **p, size_t count) {
+    if (h->dry) {
+        if (*p) h->dev_sizes.erase((void *)*p);
+        if (count == 0) count = 1;
+        h->dry_next += ((sizeof(T) * count + 255) / 256 + 1) * 256;
+        *p = reinterpret_cast<T *>(h->dry_next);   // never dereferenced
+        h->dev_sizes[(void *)*p] = sizeof(T) * count;
+        return MDX_OK;
+    }
     if (*p) {
         h->dev_sizes.erase((void *)*p);
         hipFree(*p);
@@ -246,7 +257,7 @@ template <typename T>
 mdx_status upload(mdx_engine *h, T **p, const T *src, size_t count) {
     *p = nullptr;
     CHK(dalloc(h, p, count));
-    if (count && src) HIPCHK(h, hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    if (count && src && !h->dry) HIPCHK(h, hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyHostToDevice));
     h->owned.push_back((void *)*p);
     return MDX_OK;
 }
@@ -330,13 +341,13 @@ mdx_status setup_kspace(mdx_engine *h) {
     std::vector<int> wl((rows.size() + 63) / 64, 0);
     for (size_t r = 0; r < rows.size(); ++r) wl[r / 64] = std::max(wl[r / 64], rows[r].len);
     CHK(dalloc(h, &h->d_wavelen, wl.size()));
-    HIPCHK(h, hipMemcpy(h->d_wavelen, wl.data(), sizeof(int) * wl.size(), hipMemcpyHostToDevice));
+    if (!h->dry) HIPCHK(h, hipMemcpy(h->d_wavelen, wl.data(), sizeof(int) * wl.size(), hipMemcpyHostToDevice));
     CHK(dalloc(h, &h->d_rows, rows.size()));
     CHK(dalloc(h, &h->d_rowoff, rows.size()));
     CHK(dalloc(h, &h->d_kidx, kidx.size()));
-    HIPCHK(h, hipMemcpy(h->d_rows, rows.data(), sizeof(RecipRow) * rows.size(), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_rowoff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_kidx, kidx.data(), sizeof(int4) * kidx.size(), hipMemcpyHostToDevice));
+    if (!h->dry) HIPCHK(h, hipMemcpy(h->d_rows, rows.data(), sizeof(RecipRow) * rows.size(), hipMemcpyHostToDevice));
+    if (!h->dry) HIPCHK(h, hipMemcpy(h->d_rowoff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+    if (!h->dry) HIPCHK(h, hipMemcpy(h->d_kidx, kidx.data(), sizeof(int4) * kidx.size(), hipMemcpyHostToDevice));
     CHK(dalloc(h, &h->d_fq, h->n));
     CHK(dalloc(h, &h->d_stepz, h->n));
     h->nchunk = cdiv(h->n, h->chunk);
@@ -608,10 +619,11 @@ mdx_status mdx_device_info(int32_t *n_cu, int32_t *clock_mhz, char *name, int32_
 
 const char *mdx_last_error(const mdx_engine *h) { return h ? h->err.c_str() : "null handle"; }
 
-mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **out) {
+static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_engine **out, bool dry) {
     if (!t || !p || !out) return MDX_ERR_ARG;
     *out = nullptr;
     mdx_engine *h = new mdx_engine();
+    h->dry = dry;
     auto fail = [&](mdx_status s) { *out = h; return s; };
     h->n = t->natoms;
     h->nmol = t->nmols;
@@ -666,11 +678,13 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
             return fail(MDX_ERR_ARG);
         }
     }
-    if (hipStreamCreate(&h->stream) != hipSuccess) {
-        h->err = "hipStreamCreate failed";
-        return fail(MDX_ERR_HIP);
+    if (!dry) {
+        if (hipStreamCreate(&h->stream) != hipSuccess) {
+            h->err = "hipStreamCreate failed";
+            return fail(MDX_ERR_HIP);
+        }
+        for (int k = 0; k < 2; ++k) hipEventCreate(&h->ev[k]);
     }
-    for (int k = 0; k < 2; ++k) hipEventCreate(&h->ev[k]);
     const int n = h->n;
     // fixed-point scale: resolution ~1e-14 of coulomb_factor, range ~1e4 * coulomb_factor
     h->fscale = std::ldexp(1.0, (int)std::floor(std::log2(std::ldexp(1.0, 60) / (std::fabs(p->coulomb_factor) * 1e4 + 1e-300))));
@@ -800,6 +814,10 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
     layout_rows(h);
     if ((s = dalloc(h, &h->d_epart, (size_t)h->rows_total * Q_N))) return fail(s);
     fill_args(h);
+    if (dry) {
+        *out = h;
+        return MDX_OK;
+    }
     mdx_batch *self = nullptr;
     mdx_engine *one[1] = {h};
     if ((s = mdx_batch_create(one, 1, &self))) return fail(s);
@@ -813,8 +831,32 @@ mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **o
     return MDX_OK;
 }
 
+mdx_status mdx_create(const mdx_topology *t, const mdx_params *p, mdx_engine **out) {
+    return create_impl(t, p, out, false);
+}
+
+mdx_status mdx_estimate_bytes(const mdx_topology *t, const mdx_params *p, int32_t with_md, int64_t *bytes) {
+    if (!bytes) return MDX_ERR_ARG;
+    mdx_engine *h = nullptr;
+    const mdx_status s = create_impl(t, p, &h, true);
+    int64_t total = 0;
+    if (h) {
+        for (auto &kv : h->dev_sizes) total += (int64_t)kv.second;
+        // batch of one (energies, breakdown) and MD segment state
+        total += (int64_t)sizeof(double) * (Q_N + kMaxDet);
+        if (with_md) total += (int64_t)sizeof(double) * (10LL * h->n + MD_NSTATE + cdiv(h->n, 256) + h->nmol);
+        delete h;
+    }
+    *bytes = total;
+    return s;
+}
+
 mdx_status mdx_destroy(mdx_engine *h) {
     if (!h) return MDX_OK;
+    if (h->dry) {
+        delete h;
+        return MDX_OK;
+    }
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->self) mdx_batch_destroy(h->self);
     for (auto &kv : h->dev_sizes) hipFree(kv.first);
@@ -1423,20 +1465,20 @@ MdArgs md_args(mdx_batch *b, int s, double *trace, int cap) {
 
 mdx_status md_plan(mdx_batch *b, int nsteps, bool want_trace) {
     const int K = (int)b->sys.size();
-    if (want_trace && (size_t)K * nsteps * 4 > b->trace_cap) {
+    if (want_trace && (size_t)K * nsteps * kTrace > b->trace_cap) {
         if (b->d_trace) hipFree(b->d_trace);
         b->d_trace = nullptr;
         b->trace_cap = 0;
-        if (hipMalloc((void **)&b->d_trace, sizeof(double) * (size_t)K * nsteps * 4) != hipSuccess) {
+        if (hipMalloc((void **)&b->d_trace, sizeof(double) * (size_t)K * nsteps * kTrace) != hipSuccess) {
             b->err = "hipMalloc of the MD trace failed";
             return MDX_ERR_NOMEM;
         }
-        b->trace_cap = (size_t)K * nsteps * 4;
+        b->trace_cap = (size_t)K * nsteps * kTrace;
     }
     b->md_kd.clear(); b->md_sh.clear(); b->md_ke.clear(); b->md_th.clear(); b->md_gc.clear(); b->md_tr.clear();
     for (int s = 0; s < K; ++s) {
         mdx_engine *h = b->sys[s];
-        const MdArgs a = md_args(b, s, want_trace ? b->d_trace + (size_t)s * nsteps * 4 : nullptr, nsteps);
+        const MdArgs a = md_args(b, s, want_trace ? b->d_trace + (size_t)s * nsteps * kTrace : nullptr, nsteps);
         b->md_kd.add(a, cdiv(h->n, 256));
         b->md_sh.add(a, cdiv(h->nmol, 256));
         b->md_ke.add(a, cdiv(h->n, 256));
@@ -1493,7 +1535,7 @@ mdx_status batch_md_run(mdx_batch *b, int nsteps, double *trace) {
     HIPCHK(b, hipStreamSynchronize(st));
     HIPCHK(b, hipGetLastError());
     if (trace && nsteps > 0)
-        HIPCHK(b, hipMemcpy(trace, b->d_trace, sizeof(double) * b->sys.size() * nsteps * 4, hipMemcpyDeviceToHost));
+        HIPCHK(b, hipMemcpy(trace, b->d_trace, sizeof(double) * b->sys.size() * nsteps * kTrace, hipMemcpyDeviceToHost));
     return MDX_OK;
 }
 
@@ -1733,3 +1775,5 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
 }
 
 }  // extern "C"
+
+#include "serve.hip"

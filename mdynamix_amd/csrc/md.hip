@@ -155,11 +155,12 @@ __device__ inline void md_trace_body(const MdArgs &a) {
     a.state[MD_STEP] = step + 1;
     if (!a.trace || step >= a.trace_cap) return;
     const double *e = a.energy;
-    double *t = a.trace + (size_t)step * 4;
+    double *t = a.trace + (size_t)step * kTrace;
     t[0] = e[Q_PE1] + e[Q_PE2] + e[Q_PELS1] + e[Q_PELS2] + e[Q_SPE] + a.spei + e[Q_PINT];
     t[1] = a.state[MD_EKIN];
     t[2] = a.state[MD_TEMP];
     t[3] = a.state[MD_SC];
+    t[4] = e[Q_NPS] + e[Q_NPL];
 }
 
 __global__ __launch_bounds__(256) void k_md_kick_drift(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {

@@ -264,6 +264,7 @@ struct ReduceArgs {
 
 // Device-resident leap-frog step (SULTAN, mdstep.f:29-260) of one system.
 enum MdState { MD_SC = 0, MD_EKIN, MD_TEMP, MD_SHAKE_FAIL, MD_STEP, MD_NSTATE };
+constexpr int kTrace = 5;          // per step: Epot, Ekin, T, SC, pair interactions
 
 struct MdArgs {
     int n, nmol;
@@ -285,7 +286,7 @@ struct MdArgs {
     int nke_blocks;
     const double *energy;          // Q_N energy slots of the last force call
     double spei;
-    double *trace;                 // [step][4]: Epot, Ekin, T, SC (step = state[MD_STEP])
+    double *trace;                 // [step][kTrace] (step = state[MD_STEP])
     int trace_cap;
 };
 

@@ -224,8 +224,8 @@ class ForceEngine:
         self._check(self.lib.mdx_md_init(self._h, C.byref(p), *[as_ptr(a) for a in v]))
 
     def md_run(self, nsteps: int, trace: bool = True):
-        """nsteps steps; returns the per-step (Epot, Ekin, T, SC) array."""
-        tr = np.zeros((max(nsteps, 1), 4)) if trace else None
+        """nsteps steps; returns the per-step (Epot, Ekin, T, SC, pairs) array."""
+        tr = np.zeros((max(nsteps, 1), 5)) if trace else None
         self._check(self.lib.mdx_md_run(self._h, nsteps, as_ptr(tr) if trace else None))
         return tr[:nsteps] if trace else None
 
@@ -276,9 +276,9 @@ class ForceBatch:
 
     def md_run(self, nsteps: int, trace: bool = True):
         """nsteps MD steps of every engine (each set up by md_init); returns
-        a (K, nsteps, 4) trace."""
+        a (K, nsteps, 5) trace."""
         k = len(self.engines)
-        tr = np.zeros((k, max(nsteps, 1), 4)) if trace else None
+        tr = np.zeros((k, max(nsteps, 1), 5)) if trace else None
         self._check(self.lib.mdx_batch_md_run(self._b, nsteps, as_ptr(tr) if trace else None))
         return tr[:, :nsteps] if trace else None
 

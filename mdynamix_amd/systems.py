@@ -279,10 +279,12 @@ def ua_chain(n_sites: int = 46, seed: int = 7, n_charged: int = 4) -> MolType:
     return MolType("UACH", sites, bonds, angles, [], tors1, c14el=0.5, c14lj=0.5)
 
 
-def lj_solute(n_sites: int, seed: int) -> MolType:
-    """A rigid-ish charged LJ solute of `n_sites` sites (lambda-coupled in
+def lj_solute(n_sites: int, seed: int, bonded: bool = False) -> MolType:
+    """A charged LJ solute of `n_sites` sites (lambda-coupled in
     expanded-ensemble jobs).  Sites sit on a compact random cluster 1.5 A
-    apart with alternating charges summing to zero."""
+    apart with alternating charges summing to zero.  With `bonded`, every
+    site pair closer than 2.6 A is a bond (constrained under SHAKE), which
+    keeps the cluster's shape in MD and excludes its close pairs."""
     rng = np.random.default_rng(seed)
     pts = [np.zeros(3)]
     while len(pts) < n_sites:
@@ -295,7 +297,14 @@ def lj_solute(n_sites: int, seed: int) -> MolType:
     if n_sites % 2 == 1:
         q[-1] = 0.0
     sites = [Site("S%d" % (i + 1), tuple(pts[i]), 12.011, float(q[i]), 3.4, 0.4) for i in range(n_sites)]
-    return MolType("SOLU", sites)
+    bonds = []
+    if bonded:
+        for i in range(n_sites):
+            for j in range(i + 1, n_sites):
+                r = float(np.linalg.norm(pts[i] - pts[j]))
+                if r < 2.6:
+                    bonds.append((0, i + 1, j + 1, round(r, 6), 2000.0, 0.0, 0.0))
+    return MolType("SOLU", sites, bonds)
 
 
 # ----------------------------------------------------------------------
@@ -449,10 +458,11 @@ def pimd_beads(n_beads: int = 32, n_water: int = 4000, box: float = 49.3, seed: 
     return beads
 
 
-def ee_job(n_atoms_target: int, n_solute: int, lam: float, seed: int) -> System:
+def ee_job(n_atoms_target: int, n_solute: int, lam: float, seed: int, rigid_solute: bool = False) -> System:
     """Expanded-ensemble style job: SPC/E water + 0.15 M NaCl sized to about
     `n_atoms_target` atoms, plus one solute of `n_solute` sites coupled at
-    `lam` (LJ and charges scaled by lam^PW1, soft-core shift BEE(1-lam^PW2))."""
+    `lam` (LJ and charges scaled by lam^PW1, soft-core shift BEE(1-lam^PW2)).
+    `rigid_solute` gives the solute its bond network (for MD segments)."""
     rng = np.random.default_rng(seed)
     n_w = max(64, int((n_atoms_target - n_solute) / 3.0))
     n_ion = max(1, int(round(n_w * 0.15 / 55.5)))
@@ -460,7 +470,7 @@ def ee_job(n_atoms_target: int, n_solute: int, lam: float, seed: int) -> System:
     nmol = n_w + 2 * n_ion + 1
     box = (nmol / 0.0334) ** (1.0 / 3.0)
     box = max(box, 2.0 * 10.0 + 1.0)
-    sol = lj_solute(n_solute, seed + 11)
+    sol = lj_solute(n_solute, seed + 11, bonded=rigid_solute)
     centres = lattice_sites(nmol, (box, box, box), rng)
     # the solute takes the site nearest the box centre; water and ion sites
     # closer than 3.4 A to any solute atom are dropped
@@ -482,3 +492,15 @@ def ee_job(n_atoms_target: int, n_solute: int, lam: float, seed: int) -> System:
                     rcut=10.0, rshort=5.0, ewald_a=2.8, ewald_b=9.0, shake=True, skin=1.0)
     return dataclasses.replace(sysm, types=types, counts=counts, pos=np.concatenate([sysm.pos, sol_atoms]),
                                ee_type=3, ee_lambda=lam)
+
+
+def maxwell_velocities(system: System, seed: int, temperature: float | None = None) -> np.ndarray:
+    """Maxwell-Boltzmann velocities (A/fs) at the system temperature with
+    the total momentum removed."""
+    t = system.temperature if temperature is None else temperature
+    masses = np.concatenate([[s.mass for s in ty.sites] * c for ty, c in zip(system.types, system.counts)])
+    rng = np.random.default_rng(seed)
+    kt = AVSNO * BOLTZ * t * 1e-3                      # kJ/mol
+    v = rng.normal(size=(len(masses), 3)) * np.sqrt(kt / masses * 1e-4)[:, None]
+    v -= (masses[:, None] * v).sum(0) / masses.sum()
+    return v
