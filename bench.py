@@ -10,9 +10,13 @@ in HBM.  Synthetic seeded coordinates (jittered lattice, random
 orientations); nothing is loaded from disk.
 
 Prints one JSON line (metric pair-interactions/s; ns/day, per-stage times,
-roofline of the dominant kernel and the CPU baseline beside it).
+roofline of the dominant kernel and the CPU baseline beside it), plus the
+batched form (`batched`: 32 path-integral beads x 12,000 atoms per fused
+force call) and the served form (`served`: a Poisson stream of MD-segment
+jobs of 1.5k-60k atoms with continuous batching, p50/p99 latency and
+aggregate pair-interactions/s against one job at a time).
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-served]
 """
 from __future__ import annotations
 
@@ -93,6 +97,109 @@ def cpu_baseline(s, topo, prm, numtask: int = 12):
     }
 
 
+def bench_batched(steps: int, warmup: int, beads: int = 32, n_water: int = 4000) -> dict:
+    """Path-integral batch: `beads` replicas of an SPC-FW box (4,000 waters,
+    49.3 A) evaluated as one ring batch per force call (every kernel one
+    fused launch over all beads, replayed from a hipGraph), list rebuild
+    every 10 calls; beside it the same beads evaluated one at a time."""
+    from mdynamix_amd import systems, topology
+    from mdynamix_amd.engine import ForceEngine, ForceBatch
+    bs = systems.pimd_beads(n_beads=beads, n_water=n_water, box=49.3, seed=2027)
+    topo, prm = topology.from_system(bs[0])
+    engs = []
+    for b in bs:
+        e = ForceEngine(topo, prm)
+        e.set_positions(b.pos)
+        e.set_timing(False)
+        e.chcnb()
+        engs.append(e)
+    ring = ForceBatch(engs, ring_beadfac=1e-4)
+
+    def run(k0, n, batched):
+        pairs = 0
+        for k in range(k0, k0 + n):
+            if k % 10 == 0:
+                for e in engs:
+                    e.chcnb()
+            if batched:
+                es = ring.compute(energies=True)
+            else:
+                es = [e.compute_device(0, 0, 0, energies=True) for e in engs]
+            pairs += sum(x["npairs_short"] + x["npairs_long"] for x in es)
+        return pairs
+
+    run(0, warmup, True)
+    t0 = time.perf_counter()
+    pairs = run(warmup, steps, True)
+    tb = time.perf_counter() - t0
+    run(0, warmup, False)
+    t0 = time.perf_counter()
+    pairs_a = run(warmup, steps, False)
+    ta = time.perf_counter() - t0
+    ring.close()
+    for e in engs:
+        e.close()
+    return {"workload": "pimd_%dx%d_spcfw" % (beads, n_water), "beads": beads, "atoms_per_bead": topo.natoms,
+            "steps": steps, "ms_per_call": 1e3 * tb / steps, "pair_interactions_per_s": pairs / tb,
+            "one_at_a_time": {"ms_per_call": 1e3 * ta / steps, "pair_interactions_per_s": pairs_a / ta},
+            "speedup": ta / tb}
+
+
+def bench_served(n_jobs: int, nsteps: int, rate: float, seed: int = 77) -> dict:
+    """Served mixed-size stream: `n_jobs` MD segments of `nsteps` steps
+    (SPC/E + 0.15 M NaCl boxes of 1.5k-60k atoms, log-uniform, one rigid
+    10-60-site solute, at most n/50 sites, at lambda in {0, 0.05, ..., 1}),
+    Poisson arrivals at
+    `rate` jobs/s, served with continuous batching (up to 16 slots, 64 GiB
+    HBM budget, 10-step rounds) and, on the same arrival times, one job at
+    a time; latency = completion - submission."""
+    import threading
+    from mdynamix_amd import systems
+    from mdynamix_amd.serve import MDJob, Server
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(n_jobs):
+        n = int(np.exp(rng.uniform(np.log(1500), np.log(60000))))
+        # solute size capped at n/50 so it spans less than half the box
+        ns = min(int(rng.integers(10, 61)), max(10, n // 50))
+        lam = float(rng.integers(0, 21)) * 0.05
+        s = systems.ee_job(n, ns, lam, seed + j, rigid_solute=True)
+        jobs.append(MDJob(s, nsteps, systems.maxwell_velocities(s, seed + j)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+
+    def serve(slots):
+        srv = Server(hbm_budget_bytes=64 << 30, max_slots=slots, quantum=10)
+        t0 = time.perf_counter()
+        for j, job in enumerate(jobs):
+            dt = arrivals[j] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            srv.submit(job)
+        res = []
+        deadline = time.perf_counter() + 1800.0
+        while len(res) < n_jobs and time.perf_counter() < deadline:
+            srv.wait(15.0)
+            res += srv.poll()
+            print("served %d/%d jobs (slots %d)" % (len(res), n_jobs, slots), file=sys.stderr, flush=True)
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+        span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        pairs = sum(r["pair_interactions"] for r in res)
+        ok = all(r["status"] == 0 for r in res) and len(res) == n_jobs
+        return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                "mean_s": float(lat.mean()), "makespan_s": span, "pair_interactions_per_s": pairs / span,
+                "rounds": st["rounds"], "all_ok": ok}
+
+    served = serve(16)
+    alone = serve(1)
+    sizes = [j.natoms for j in jobs]
+    return {"workload": "ee_stream_%djobs_%dsteps" % (n_jobs, nsteps), "jobs": n_jobs, "steps_per_job": nsteps,
+            "arrival_rate_per_s": rate, "atoms_min": min(sizes), "atoms_max": max(sizes),
+            "atoms_total": int(sum(sizes)), "served": served, "one_at_a_time": alone,
+            "throughput_gain": served["pair_interactions_per_s"] / alone["pair_interactions_per_s"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +207,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-tasks", type=int, default=12)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=16)
+    ap.add_argument("--serve-steps", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU per box)", file=sys.stderr)
@@ -192,6 +304,13 @@ def main() -> int:
         "device": dev,
         "roofline": roof,
     }
+    eng.close()
+    print("plain: %.3f ms/step" % ms_step, file=sys.stderr, flush=True)
+    if not args.no_batched:
+        out["batched"] = bench_batched(max(args.steps // 2, 5), args.warmup)
+        print("batched: %.3f ms/call" % out["batched"]["ms_per_call"], file=sys.stderr, flush=True)
+    if not args.no_served:
+        out["served"] = bench_served(args.serve_jobs, args.serve_steps, args.serve_rate)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, topo, prm, args.cpu_tasks)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -281,30 +281,57 @@ def ua_chain(n_sites: int = 46, seed: int = 7, n_charged: int = 4) -> MolType:
 
 def lj_solute(n_sites: int, seed: int, bonded: bool = False) -> MolType:
     """A charged LJ solute of `n_sites` sites (lambda-coupled in
-    expanded-ensemble jobs).  Sites sit on a compact random cluster 1.5 A
-    apart with alternating charges summing to zero.  With `bonded`, every
-    site pair closer than 2.6 A is a bond (constrained under SHAKE), which
-    keeps the cluster's shape in MD and excludes its close pairs."""
+    expanded-ensemble jobs), alternating charges summing to zero.
+
+    Default: a compact random cluster, sites 1.5 A apart, no bonded terms
+    (force-evaluation fixtures).  With `bonded`: a branched tree for MD
+    segments -- bonds of 1.5 A (constrained under SHAKE: n - 1 constraints
+    on a tree, so the iteration converges in a few sweeps), a harmonic angle
+    on every pair of bonds sharing a site (which also excludes the 1-3
+    pairs), every other site pair at least 3.0 A apart."""
     rng = np.random.default_rng(seed)
-    pts = [np.zeros(3)]
-    while len(pts) < n_sites:
-        base = pts[rng.integers(len(pts))]
-        v = rng.normal(size=3)
-        cand = base + 1.5 * v / np.linalg.norm(v)
-        if min(np.linalg.norm(cand - p) for p in pts) > 1.4:
-            pts.append(cand)
     q = np.array([0.25 if i % 2 == 0 else -0.25 for i in range(n_sites)])
     if n_sites % 2 == 1:
         q[-1] = 0.0
+    if not bonded:
+        pts = [np.zeros(3)]
+        while len(pts) < n_sites:
+            base = pts[rng.integers(len(pts))]
+            v = rng.normal(size=3)
+            cand = base + 1.5 * v / np.linalg.norm(v)
+            if min(np.linalg.norm(cand - p) for p in pts) > 1.4:
+                pts.append(cand)
+        sites = [Site("S%d" % (i + 1), tuple(pts[i]), 12.011, float(q[i]), 3.4, 0.4) for i in range(n_sites)]
+        return MolType("SOLU", sites)
+    pts = [np.zeros(3)]
+    nbr = [[]]
+    while len(pts) < n_sites:
+        # grow from sites near the centre (compact tree)
+        c = np.mean(pts, axis=0)
+        pick = rng.integers(len(pts), size=4)
+        p = int(min(pick, key=lambda k: np.linalg.norm(pts[k] - c)))
+        if len(nbr[p]) >= 3:
+            continue
+        v = rng.normal(size=3)
+        cand = pts[p] + 1.5 * v / np.linalg.norm(v)
+        near = set(nbr[p]) | {p}
+        ok = all(np.linalg.norm(cand - pts[k]) >= 2.3 for k in nbr[p])
+        ok = ok and all(np.linalg.norm(cand - pts[k]) >= 3.0 for k in range(len(pts)) if k not in near)
+        if ok:
+            nbr[p].append(len(pts))
+            nbr.append([p])
+            pts.append(cand)
     sites = [Site("S%d" % (i + 1), tuple(pts[i]), 12.011, float(q[i]), 3.4, 0.4) for i in range(n_sites)]
-    bonds = []
-    if bonded:
-        for i in range(n_sites):
-            for j in range(i + 1, n_sites):
-                r = float(np.linalg.norm(pts[i] - pts[j]))
-                if r < 2.6:
-                    bonds.append((0, i + 1, j + 1, round(r, 6), 2000.0, 0.0, 0.0))
-    return MolType("SOLU", sites, bonds)
+    bonds = [(0, i + 1, j + 1, 1.5, 2000.0, 0.0, 0.0) for i in range(n_sites) for j in nbr[i] if i < j]
+    angles = []
+    for c in range(n_sites):
+        for a in range(len(nbr[c])):
+            for b in range(a + 1, len(nbr[c])):
+                i, k = nbr[c][a], nbr[c][b]
+                u, w = pts[i] - pts[c], pts[k] - pts[c]
+                th = math.degrees(math.acos(float(np.dot(u, w) / (np.linalg.norm(u) * np.linalg.norm(w)))))
+                angles.append((i + 1, c + 1, k + 1, round(th, 4), 200.0))
+    return MolType("SOLU", sites, bonds, angles)
 
 
 # ----------------------------------------------------------------------
