@@ -71,6 +71,49 @@ struct RecipRow {           // 16 bytes
 
 __host__ __device__ inline double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
 
+// Address-space views of the pointers kept in argument structs (which the
+// kernels read from HBM, so the compiler cannot infer where they point):
+// gld/gst for per-lane global accesses, sld for wavefront-uniform
+// read-only data, which then goes through scalar loads.
+#define MDX_GLOBAL __attribute__((address_space(1)))
+#define MDX_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ inline const MDX_GLOBAL T *gld(const T *p) { return (const MDX_GLOBAL T *)p; }
+template <typename T>
+__device__ inline MDX_GLOBAL T *gst(T *p) { return (MDX_GLOBAL T *)p; }
+template <typename T>
+__device__ inline const MDX_CONST T *sld(const T *p) { return (const MDX_CONST T *)p; }
+// Value of a struct or vector type through such a view, word by word (C++
+// copy constructors bind generic references only).
+template <typename T>
+__device__ inline T ldv(const MDX_CONST T *p) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized types only");
+    T v;
+    int *d = reinterpret_cast<int *>(&v);
+    const MDX_CONST int *s = reinterpret_cast<const MDX_CONST int *>(p);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) d[k] = s[k];
+    return v;
+}
+template <typename T>
+__device__ inline T ldv(const MDX_GLOBAL T *p) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized types only");
+    T v;
+    int *d = reinterpret_cast<int *>(&v);
+    const MDX_GLOBAL int *s = reinterpret_cast<const MDX_GLOBAL int *>(p);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) d[k] = s[k];
+    return v;
+}
+template <typename T>
+__device__ inline void stv(MDX_GLOBAL T *p, const T &v) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized types only");
+    const int *s = reinterpret_cast<const int *>(&v);
+    MDX_GLOBAL int *d = reinterpret_cast<MDX_GLOBAL int *>(p);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) d[k] = s[k];
+}
+
 // Single-shift minimum image of the reference pair loops (forces.f:1841-1867).
 __device__ inline void min_image(double &dx, double &dy, double &dz, const Box &b) {
     if (fabs(dx) > b.h[0]) dx = dx - fsign(b.l[0], dx);

@@ -73,7 +73,7 @@ template <bool SOFT, bool DETAIL>
 __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int group) {
     const int lane = threadIdx.x;
     const int nlj2 = a.nlj * a.nlj;
-    for (int t = lane; t < nlj2; t += 64) s_lj[t] = a.lj[t];
+    for (int t = lane; t < nlj2; t += 64) s_lj[t] = ldv(gld(a.lj) + t);
     double *s_det = reinterpret_cast<double *>(s_lj + nlj2);   // [(nmt + 1) * 2][64]
     int *s_ti = reinterpret_cast<int *>(s_det + (a.nmt + 1) * 2 * 64);
     if (DETAIL) {
@@ -87,8 +87,8 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
     if (ci >= a.nclu) return;
 
     const int il = lane >> 3;
-    const float4 pi = a.xq[ci * kClu + il];
-    const int infoi = a.aux[ci * kClu + il].x;
+    const float4 pi = ldv(gld(a.xq) + ci * kClu + il);
+    const int infoi = gld(a.aux)[ci * kClu + il].x;
     const int ti = infoi & 0xffff;
     const float2 *ljrow = s_lj + ti * a.nlj;
     const int jl = lane & 7;
@@ -109,21 +109,34 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
     double E[6] = {0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
     double Vs[3] = {0, 0, 0}, Vl[3] = {0, 0, 0};
     int cnt_s = 0, cnt_l = 0;
-    const int n = a.nlist[ci];
-    const PairEntry *L = a.list + (size_t)ci * a.list_cap;
+    const int n = sld(a.nlist)[ci];
+    const MDX_CONST PairEntry *L = sld(a.list) + (size_t)ci * a.list_cap;
+    const MDX_GLOBAL float4 *xq = gld(a.xq);
+    const MDX_GLOBAL int2 *aux = gld(a.aux);
+    // software pipeline: list entries two ahead (scalar loads), the
+    // j-cluster coordinates one ahead
+    PairEntry e1{}, e2{};
+    float4 pj1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    int2 aux1 = make_int2(0, 0);
+    if (n > 0) e1 = ldv(L);
+    if (n > 1) e2 = ldv(L + 1);
+    if (n > 0) { pj1 = ldv(xq + e1.cj * kClu + jl); aux1 = ldv(aux + e1.cj * kClu + jl); }
     for (int k0 = 0; k0 < n; k0 += 16) {
         float fs[3] = {0.f, 0.f, 0.f}, fl[3] = {0.f, 0.f, 0.f};
         float es = 0.f, el = 0.f, ljs = 0.f, ljl = 0.f, vs = 0.f, vl = 0.f;
         float vxs = 0.f, vys = 0.f, vzs = 0.f, vxl = 0.f, vyl = 0.f, vzl = 0.f;
         const int k1 = min(k0 + 16, n);
         for (int k = k0; k < k1; ++k) {
-            const PairEntry e = L[k];
+            const PairEntry e = e1;
+            const float4 pj = pj1;
+            const int2 auxj = aux1;
+            e1 = e2;
+            if (k + 2 < n) e2 = ldv(L + k + 2);
+            if (k + 1 < n) { pj1 = ldv(xq + e1.cj * kClu + jl); aux1 = ldv(aux + e1.cj * kClu + jl); }
             unsigned long long m = e.mask;
             if (group == 1) m &= e.mshort;
             else if (group == 2) m &= ~e.mshort;
             if (m == 0ull) continue;
-            const float4 pj = a.xq[e.cj * kClu + jl];
-            const int2 auxj = a.aux[e.cj * kClu + jl];
             const int infoj = auxj.x;
             const bool on = (m >> lane) & 1ull;
             const bool isshort = (e.mshort >> lane) & 1ull;
@@ -188,13 +201,13 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
             Fs[q] += __shfl_xor(Fs[q], off);
             Fl[q] += __shfl_xor(Fl[q], off);
         }
-    const int ai = a.clu_atom[ci * kClu + il];
-    const double qc = ai >= 0 ? a.q[ai] * a.coulf : 0.0;
+    const int ai = gld(a.clu_atom)[ci * kClu + il];
+    const double qc = ai >= 0 ? gld(a.q)[ai] * a.coulf : 0.0;
     E[0] *= qc;
     E[1] *= qc;
     if (jl == 0 && ai >= 0) {
-        if (a.fs_x) { a.fs_x[ai] = Fs[0]; a.fs_y[ai] = Fs[1]; a.fs_z[ai] = Fs[2]; }
-        if (a.fl_x) { a.fl_x[ai] = Fl[0]; a.fl_y[ai] = Fl[1]; a.fl_z[ai] = Fl[2]; }
+        if (a.fs_x) { gst(a.fs_x)[ai] = Fs[0]; gst(a.fs_y)[ai] = Fs[1]; gst(a.fs_z)[ai] = Fs[2]; }
+        if (a.fl_x) { gst(a.fl_x)[ai] = Fl[0]; gst(a.fl_y)[ai] = Fl[1]; gst(a.fl_z)[ai] = Fl[2]; }
     }
     // energies and virials: wavefront sums, halved for the full list
     double q[14] = {E[0], E[1], E[2], E[3], E[4], E[5], Vs[0], Vs[1], Vs[2], Vl[0], Vl[1], Vl[2],
@@ -202,7 +215,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
     for (int off = 1; off < 64; off <<= 1)
         for (int k = 0; k < 14; ++k) q[k] += __shfl_xor(q[k], off);
     if (lane == 0) {
-        double *o = a.epart + (size_t)ci * Q_N;
+        MDX_GLOBAL double *o = gst(a.epart) + (size_t)ci * Q_N;
         for (int k = 0; k < Q_N; ++k) o[k] = 0.0;
         o[Q_PELS2] = 0.5 * q[0];
         o[Q_PELS1] = 0.5 * q[1];

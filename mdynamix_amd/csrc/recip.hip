@@ -7,13 +7,14 @@
 // k_sk: lane = row, atoms of one chunk streamed with uniform (scalar) loads.
 //   For each atom the phase 2 pi (n_x f_x + n_y f_y + n_z0 f_z) is formed
 //   in FP64 fractional coordinates and reduced to one turn before sincospi;
-//   the remaining terms of the row follow by the complex recurrence
-//   P <- P exp(2 pi i f_z), re-anchored in FP64 every 8 terms.  Per-chunk FP32 sums are written as
+//   the remaining terms of the row follow by the recurrence
+//   P_{j+1} = 2 cos(theta) P_j - P_{j-1}, theta = 2 pi f_z, re-anchored in
+//   FP64 every 8 terms.  Per-chunk FP32 sums are written as
 //   partials and added in FP64 by k_sk_reduce, which also forms the energy
 //   QPE, the virial projections and A_k = 2 AKC S(k) for the force pass.
 // k_recip_force: lane = atom, rows streamed uniformly; the force on an atom
 //   is q sum_k k Im(exp(i k.r) conj(A_k)) (forces.f:1320-1326), again with
-//   one reduced-phase sincospi per row and a recurrence along n_z.
+//   one reduced-phase sincospi per row and the same recurrence along n_z.
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -49,52 +50,58 @@ __device__ inline void row_start(const RecipRow &r, double4 a, double &c, double
     s = (double)sf;
 }
 
-// Structure factors of 64 rows over one atom chunk.  The row's terms are
-// generated in blocks of 8: an FP64 anchor advanced by exp(16 pi i f_z)
-// and an FP32 recurrence inside the block, so rounding of the step cannot
-// accumulate over more than 7 terms.
+// Structure factors of 64 rows over one atom chunk.  Along a row the
+// terms P_j = q exp(i(phi0 + j theta)) follow the three-term recurrence
+// P_{j+1} = 2 cos(theta) P_j - P_{j-1} (2 FMA per term), in blocks of 8
+// seeded from an FP64 anchor advanced by exp(8 i theta), so rounding
+// cannot build up over more than 7 steps.  The atom data of the chunk is
+// wavefront-uniform and read with scalar loads, one atom ahead.
 template <int MAXLEN>
 __device__ inline void sk_body(const RecipArgs &a, int blk) {
     const int rg = blk % a.nrowgroups, ch = blk / a.nrowgroups;
     const int r = rg * 64 + threadIdx.x;
     const int a0 = ch * a.chunk;
     const int a1 = min(a0 + a.chunk, a.natoms);
-    const int wlen = a.wavelen[rg];          // longest row of this wavefront
+    const int wlen = sld(a.wavelen)[rg];          // longest row of this wavefront
+    const MDX_CONST double4 *fq = sld(a.fq);
+    const MDX_CONST double4 *stz = sld(a.stepz);
     RecipRow row = {0, 0, 0, 0};
-    if (r < a.nrows) row = a.rows[r];
+    if (r < a.nrows) row = ldv(gld(a.rows) + r);
     float are[MAXLEN], aim[MAXLEN];
 #pragma unroll
     for (int j = 0; j < MAXLEN; ++j) { are[j] = 0.f; aim[j] = 0.f; }
+    double4 at = ldv(fq + a0), st = ldv(stz + a0);
     for (int i = a0; i < a1; ++i) {
-        const double4 at = a.fq[i];
-        const double4 st = a.stepz[i];
+        const double4 atc = at, stc = st;
+        if (i + 1 < a1) { at = ldv(fq + i + 1); st = ldv(stz + i + 1); }
         double s0, c0;
-        row_start(row, at, c0, s0);
-        double Ar = at.w * c0, Ai = at.w * s0;             // anchor, FP64
-        const float sr = (float)st.x, si = (float)st.y;
+        row_start(row, atc, c0, s0);
+        double Ar = atc.w * c0, Ai = atc.w * s0;           // anchor, FP64
+        const float sr = (float)stc.x, si = (float)stc.y, c2 = 2.0f * sr;
 #pragma unroll
         for (int j0 = 0; j0 < MAXLEN; j0 += 8) {
             if (j0 >= wlen) break;
-            float pr = (float)Ar, pim = (float)Ai;
+            float pr0 = (float)Ar, pi0 = (float)Ai;
+            float pr1 = pr0 * sr - pi0 * si, pi1 = pr0 * si + pi0 * sr;
+            are[j0] += pr0; aim[j0] += pi0;
+            are[j0 + 1] += pr1; aim[j0 + 1] += pi1;
 #pragma unroll
-            for (int j = j0; j < j0 + 8; ++j) {
-                are[j] += pr;
-                aim[j] += pim;
-                const float tr = pr * sr - pim * si;
-                pim = pr * si + pim * sr;
-                pr = tr;
+            for (int j = j0 + 2; j < j0 + 8; ++j) {
+                const float pr2 = fmaf(c2, pr1, -pr0), pi2 = fmaf(c2, pi1, -pi0);
+                are[j] += pr2; aim[j] += pi2;
+                pr0 = pr1; pi0 = pi1; pr1 = pr2; pi1 = pi2;
             }
-            const double tr = Ar * st.z - Ai * st.w;
-            Ai = Ar * st.w + Ai * st.z;
+            const double tr = Ar * stc.z - Ai * stc.w;
+            Ai = Ar * stc.w + Ai * stc.z;
             Ar = tr;
         }
     }
     if (r < a.nrows) {
-        const int off = a.rowoff[r];
-        float2 *o = a.part + (size_t)ch * a.nkv + off;
+        const int off = gld(a.rowoff)[r];
+        MDX_GLOBAL float2 *o = gst(a.part) + (size_t)ch * a.nkv + off;
 #pragma unroll
         for (int j = 0; j < MAXLEN; ++j)
-            if (j < row.len) o[j] = make_float2(are[j], aim[j]);
+            if (j < row.len) stv(o + j, make_float2(are[j], aim[j]));
     }
 }
 
@@ -165,7 +172,9 @@ __global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__
 
 // Reciprocal forces; a block covers 64 atoms and one slice of rows, each
 // slice writes its own partial force array (summed by the combine kernel).
-// Same anchored recurrence as k_sk.
+// Rows and their A_k are wavefront-uniform (scalar loads, the next row's
+// header one row ahead); the phase factors follow the same anchored
+// three-term recurrence as k_sk.
 __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     const int ab = blk % a.natom_blocks, sl = blk / a.natom_blocks;
     const int i = ab * 64 + threadIdx.x;
@@ -173,55 +182,69 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     const int r1 = min(r0 + a.rows_per_slice, a.nrows);
     double4 at = make_double4(0, 0, 0, 0);
     double4 st = make_double4(1.0, 0.0, 1.0, 0.0);
-    if (i < a.natoms) { at = a.fq[i]; st = a.stepz[i]; }
-    const float sr = (float)st.x, si = (float)st.y;
+    if (i < a.natoms) { at = ldv(gld(a.fq) + i); st = ldv(gld(a.stepz) + i); }
+    const float sr = (float)st.x, si = (float)st.y, c2 = 2.0f * sr;
     const double tx = 2.0 * kPI / a.box.l[0], ty = 2.0 * kPI / a.box.l[1], tz = 2.0 * kPI / a.box.l[2];
+    const MDX_CONST RecipRow *rows = sld(a.rows);
+    const MDX_CONST int *rowoff = sld(a.rowoff);
+    const MDX_CONST float2 *ak = sld(a.ak);
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-    for (int r = r0; r < r1; ++r) {
-        const RecipRow row = a.rows[r];
-        const float2 *A = a.ak + a.rowoff[r];
-        double s0, c0;
-        row_start(row, at, c0, s0);
-        double Wr = at.w * c0, Wi = at.w * s0;
-        float ts = 0.f, tzs = 0.f;
-        int j0 = 0;
-        // full blocks of 8: the eight A_k are uniform, loaded together
-        for (; j0 + 8 <= row.len; j0 += 8) {
-            float2 Av[8];
+    if (r0 < r1) {
+        RecipRow nrow = ldv(rows + r0);
+        int noff = rowoff[r0];
+        for (int r = r0; r < r1; ++r) {
+            const RecipRow row = nrow;
+            const MDX_CONST float2 *A = ak + noff;
+            if (r + 1 < r1) { nrow = ldv(rows + r + 1); noff = rowoff[r + 1]; }
+            double s0, c0;
+            row_start(row, at, c0, s0);
+            double Wr = at.w * c0, Wi = at.w * s0;
+            float ts = 0.f, tzs = 0.f;
+            int j0 = 0;
+            // full blocks of 8: the eight A_k are uniform, loaded together
+            for (; j0 + 8 <= row.len; j0 += 8) {
+                float2 Av[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) Av[q] = A[j0 + q];
-            float wr = (float)Wr, wi = (float)Wi;
+                for (int q = 0; q < 8; ++q) Av[q] = ldv(A + j0 + q);
+                float wr0 = (float)Wr, wi0 = (float)Wi;
+                float wr1 = wr0 * sr - wi0 * si, wi1 = wr0 * si + wi0 * sr;
+                float t = wi0 * Av[0].x - wr0 * Av[0].y;
+                ts += t;
+                tzs = fmaf((float)j0, t, tzs);
+                t = wi1 * Av[1].x - wr1 * Av[1].y;
+                ts += t;
+                tzs = fmaf((float)(j0 + 1), t, tzs);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const float t = wi * Av[q].x - wr * Av[q].y;
-                ts += t;
-                tzs += (float)(j0 + q) * t;
-                const float nr = wr * sr - wi * si;
-                wi = wr * si + wi * sr;
-                wr = nr;
+                for (int q = 2; q < 8; ++q) {
+                    const float wr2 = fmaf(c2, wr1, -wr0), wi2 = fmaf(c2, wi1, -wi0);
+                    t = wi2 * Av[q].x - wr2 * Av[q].y;
+                    ts += t;
+                    tzs = fmaf((float)(j0 + q), t, tzs);
+                    wr0 = wr1; wi0 = wi1; wr1 = wr2; wi1 = wi2;
+                }
+                const double tr = Wr * st.z - Wi * st.w;
+                Wi = Wr * st.w + Wi * st.z;
+                Wr = tr;
             }
-            const double tr = Wr * st.z - Wi * st.w;
-            Wi = Wr * st.w + Wi * st.z;
-            Wr = tr;
-        }
-        {
-            float wr = (float)Wr, wi = (float)Wi;
-            for (int j = j0; j < row.len; ++j) {
-                const float2 Ak = A[j];
-                const float t = wi * Ak.x - wr * Ak.y;
-                ts += t;
-                tzs += (float)j * t;
-                const float nr = wr * sr - wi * si;
-                wi = wr * si + wi * sr;
-                wr = nr;
+            {
+                float wr = (float)Wr, wi = (float)Wi;
+                for (int j = j0; j < row.len; ++j) {
+                    const float2 Ak = ldv(A + j);
+                    const float t = wi * Ak.x - wr * Ak.y;
+                    ts += t;
+                    tzs = fmaf((float)j, t, tzs);
+                    const float nr = wr * sr - wi * si;
+                    wi = wr * si + wi * sr;
+                    wr = nr;
+                }
             }
+            Fx += tx * row.nx * (double)ts;
+            Fy += ty * row.ny * (double)ts;
+            Fz += tz * ((double)row.nz0 * (double)ts + (double)tzs);
         }
-        Fx += tx * row.nx * (double)ts;
-        Fy += ty * row.ny * (double)ts;
-        Fz += tz * ((double)row.nz0 * (double)ts + (double)tzs);
     }
     if (i < a.natoms) {
-        double *o = a.rpart + (size_t)sl * 3 * a.natoms;
+        MDX_GLOBAL double *o = gst(a.rpart) + (size_t)sl * 3 * a.natoms;
         o[i] = Fx;
         o[a.natoms + i] = Fy;
         o[2 * a.natoms + i] = Fz;
