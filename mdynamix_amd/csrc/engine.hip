@@ -976,7 +976,7 @@ mdx_status mdx_build_list(mdx_engine *h) {
     const int rx = (int)std::ceil((rl + cwx) / cwx), ry = (int)std::ceil((rl + cwy) / cwy);
     for (int attempt = 0; attempt < 4; ++attempt) {
         k_fill_i32<<<1, 64, 0, st>>>(1, h->d_overflow, 0);
-        k_candidates<<<cdiv(h->nclu, 128), 128, 0, st>>>(h->nclu, h->d_anchor, h->d_ext, h->d_clu_col, h->d_colclu,
+        k_candidates<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_anchor, h->d_ext, h->d_clu_col, h->d_colclu,
                                                           h->d_colclu_n, h->ncx, h->ncy, rx, ry, h->box, rl,
                                                           h->cand_cap, h->d_cand, h->d_ncand, h->d_overflow);
         k_masks<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_cand, h->d_ncand, h->cand_cap, h->d_clu_atom, h->d_anchor, h->d_x,

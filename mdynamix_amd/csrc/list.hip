@@ -86,19 +86,25 @@ __global__ void k_cluster_geom(int nclu, const int *__restrict__ clu_atom, const
 }
 
 // Candidate j-clusters of each i-cluster: bounding boxes within r_list.
-// One thread per i-cluster; neighbour columns within reach are visited once
-// each (periodic wrap).
-__global__ void k_candidates(int nclu, const double *__restrict__ anchor, const float *__restrict__ ext,
-                             const int *__restrict__ clu_col, const int *__restrict__ colclu,
-                             const int *__restrict__ colclu_n, int ncx, int ncy, int rx, int ry, Box b,
-                             double rlist, int cap, int *__restrict__ cand, int *__restrict__ ncand,
-                             int *__restrict__ overflow) {
-    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per i-cluster; the clusters of each neighbour column
+// (periodic wrap, each column once) are tested 64 at a time and the hits
+// compacted in column order with a ballot, so the candidate order is that
+// of a serial scan.
+__global__ __launch_bounds__(64) void k_candidates(int nclu, const double *__restrict__ anchor,
+                                                   const float *__restrict__ ext, const int *__restrict__ clu_col,
+                                                   const int *__restrict__ colclu, const int *__restrict__ colclu_n,
+                                                   int ncx, int ncy, int rx, int ry, Box b, double rlist, int cap,
+                                                   int *__restrict__ cand, int *__restrict__ ncand,
+                                                   int *__restrict__ overflow) {
+    const int ci = blockIdx.x;
     if (ci >= nclu) return;
+    const int lane = threadIdx.x;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const int col = clu_col[ci];
     const int cx0 = col % ncx, cy0 = col / ncx;
     const double ai[3] = {anchor[ci * 3], anchor[ci * 3 + 1], anchor[ci * 3 + 2]};
     const float ei[3] = {ext[ci * 3], ext[ci * 3 + 1], ext[ci * 3 + 2]};
+    const double il[3] = {1.0 / b.l[0], 1.0 / b.l[1], 1.0 / b.l[2]};
     const int nx = min(2 * rx + 1, ncx), ny = min(2 * ry + 1, ncy);
     const int sx = (2 * rx + 1 >= ncx) ? 0 : cx0 - rx + ncx;
     const int sy = (2 * ry + 1 >= ncy) ? 0 : cy0 - ry + ncy;
@@ -109,24 +115,31 @@ __global__ void k_candidates(int nclu, const double *__restrict__ anchor, const 
         for (int ox = 0; ox < nx; ++ox) {
             const int cx = (sx + ox) % ncx;
             const int c2 = cy * ncx + cx;
-            const int beg = colclu[c2], end = beg + colclu_n[c2];
-            for (int cj = beg; cj < end; ++cj) {
-                double d2 = 0.0;
-                for (int k = 0; k < 3; ++k) {
-                    double d = ai[k] - anchor[cj * 3 + k];
-                    d -= b.l[k] * rint(d / b.l[k]);
-                    double g = fabs(d) - (double)(ei[k] + ext[cj * 3 + k]);
-                    if (g > 0.0) d2 += g * g;
+            const int beg = colclu[c2], num = colclu_n[c2];
+            for (int base = 0; base < num; base += 64) {
+                const int cj = beg + base + lane;
+                bool hit = false;
+                if (base + lane < num) {
+                    double d2 = 0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        double d = ai[k] - anchor[cj * 3 + k];
+                        d -= b.l[k] * rint(d * il[k]);
+                        const double g = fabs(d) - (double)(ei[k] + ext[cj * 3 + k]);
+                        if (g > 0.0) d2 += g * g;
+                    }
+                    hit = d2 <= r2;
                 }
-                if (d2 <= r2) {
-                    if (cnt < cap) cand[(size_t)ci * cap + cnt] = cj;
-                    ++cnt;
-                }
+                const unsigned long long m = __ballot(hit);
+                const int pos = cnt + __popcll(m & below);
+                if (hit && pos < cap) cand[(size_t)ci * cap + pos] = cj;
+                cnt += __popcll(m);
             }
         }
     }
-    ncand[ci] = min(cnt, cap);
-    if (cnt > cap) atomicMax(overflow, cnt);
+    if (lane == 0) {
+        ncand[ci] = min(cnt, cap);
+        if (cnt > cap) atomicMax(overflow, cnt);
+    }
 }
 
 // Is j bound to i (returns 1 for 1-2/1-3, 2 for 1-4, 0 otherwise)?
@@ -161,13 +174,28 @@ __global__ __launch_bounds__(64) void k_masks(
     }
     const double aci[3] = {anchor[ci * 3], anchor[ci * 3 + 1], anchor[ci * 3 + 2]};
     const int nc = ncand[ci];
+    const int *cl = cand + (size_t)ci * cap;
+    // software pipeline: candidate indices two ahead, the j atom one ahead
+    // (its coordinates are requested one iteration before use)
+    int c1 = nc > 0 ? cl[0] : 0, c2 = nc > 1 ? cl[1] : 0;
+    int a1 = nc > 0 ? clu_atom[c1 * kClu + jl] : -1;
+    double x1 = 0.0, y1 = 0.0, z1 = 0.0;
+    if (a1 >= 0) { x1 = x[a1]; y1 = y[a1]; z1 = z[a1]; }
+    int a2 = nc > 1 ? clu_atom[c2 * kClu + jl] : -1;
     int cnt = 0;
     for (int k = 0; k < nc; ++k) {
-        const int cj = cand[(size_t)ci * cap + k];
-        const int aj = clu_atom[cj * kClu + jl];
+        const int cj = c1, aj = a1;
+        const double xj = x1, yj = y1, zj = z1;
+        c1 = c2;
+        a1 = a2;
+        if (a1 >= 0) { x1 = x[a1]; y1 = y[a1]; z1 = z[a1]; }
+        if (k + 2 < nc) {
+            c2 = cl[k + 2];
+            a2 = clu_atom[c2 * kClu + jl];
+        }
         bool listed = false, shrt = false;
         if (ai >= 0 && aj >= 0 && ai != aj) {
-            double dx = xi - x[aj], dy = yi - y[aj], dz = zi - z[aj];
+            double dx = xi - xj, dy = yi - yj, dz = zi - zj;
             min_image(dx, dy, dz, b);
             double rr2 = dx * dx + dy * dy;
             rr2 = rr2 + dz * dz;
