@@ -200,6 +200,25 @@ def bench_served(n_jobs: int, nsteps: int, rate: float, seed: int = 77) -> dict:
             "throughput_gain": served["pair_interactions_per_s"] / alone["pair_interactions_per_s"]}
 
 
+def copy_calibration(mib: int, reps: int = 5) -> dict:
+    """Device-to-device copies of a known size through the HIP runtime (its
+    copy kernel), so hardware traffic counters read in a profiler can be
+    scaled to bytes: each copy reads and writes `mib` MiB."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(reps):
+        hip.hipMemcpy(b, a, C.c_size_t(n), 3)          # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"copy_bytes_read": n, "copy_bytes_written": n, "reps": reps}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,7 +231,12 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (traffic counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration(args.copy_calibration)))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU per box)", file=sys.stderr)
         return 2
