@@ -111,6 +111,11 @@ struct mdx_engine {
     bool have_list = false;
     // reciprocal space
     int nkv = 0, nrows = 0, maxlen = 0, chunk = 512, nchunk = 0, nslice = 1, rows_per_slice = 0;
+    int nfrows = 0, npk = 0, mz = 0;
+    FRow *d_frows = nullptr;
+    int4 *d_fpk = nullptr;
+    float4 *d_fpack = nullptr;
+    float2 *d_fa0 = nullptr;
     double kbox[3] = {0, 0, 0};
     double spei = 0.0;
     std::vector<double> spee_self;
@@ -185,7 +190,8 @@ struct mdx_batch {
     KPlan<PackArgs> pack;
     KPlan<PairArgs> pairs;
     KPlan<SpecialArgs> special;
-    KPlan<RecipArgs> rprep, sk, skred, rforce;
+    KPlan<RecipArgs> rprep, sk, skred, fpk, rforce;
+    int mz = 1;
     KPlan<EtermArgs> et;
     KPlan<BondArgs> bonds;
     KPlan<AngleArgs> angles;
@@ -354,10 +360,57 @@ mdx_status setup_kspace(mdx_engine *h) {
     CHK(dalloc(h, &h->d_part, (size_t)h->nchunk * std::max(1, h->nkv)));
     CHK(dalloc(h, &h->d_ak, std::max(1, h->nkv)));
     CHK(dalloc(h, &h->d_sk, std::max(1, h->nkv)));
+    // force pass: rows in (n_x, n_y) order, +-n_z pairs (indices into the
+    // row-ordered k arrays above)
+    {
+        std::vector<int> order(rows.size());
+        for (size_t r = 0; r < rows.size(); ++r) order[r] = (int)r;
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            return rows[a].nx != rows[b].nx ? rows[a].nx < rows[b].nx : rows[a].ny < rows[b].ny;
+        });
+        std::vector<FRow> fr;
+        std::vector<int4> pk;
+        int mz = 0;
+        for (int r : order) {
+            const RecipRow &q = rows[r];
+            const int lo = q.nz0, hi = q.nz0 + q.len - 1;
+            const int m = std::max(std::abs(lo), std::abs(hi));
+            FRow f{};
+            f.ix = q.nx;
+            f.iy = q.ny;
+            f.m = m;
+            f.off = (int)pk.size();
+            f.k0 = (lo <= 0 && 0 <= hi) ? off[r] - lo : -1;
+            // entries padded to a multiple of 4 (zero pairs), so the force
+            // kernel reads whole blocks of 4 with unconditional loads
+            for (int j = 1; j <= (m + 3) / 4 * 4; ++j) {
+                const int kp = (j <= m && lo <= j && j <= hi) ? off[r] + j - lo : -1;
+                const int km = (j <= m && lo <= -j && -j <= hi) ? off[r] - j - lo : -1;
+                pk.push_back(make_int4(kp, km, j, 0));
+            }
+            mz = std::max(mz, m);
+            fr.push_back(f);
+        }
+        if (mz > 48) {
+            h->err = "k-vector rows with |n_z| above 48 are not supported";
+            return MDX_ERR_NKV;
+        }
+        h->nfrows = (int)fr.size();
+        h->npk = (int)pk.size();
+        h->mz = mz;
+        CHK(dalloc(h, &h->d_frows, fr.size()));
+        CHK(dalloc(h, &h->d_fpk, std::max<size_t>(1, pk.size())));
+        CHK(dalloc(h, &h->d_fpack, pk.size() + 8));   // + two spare blocks
+        CHK(dalloc(h, &h->d_fa0, fr.size()));
+        if (!h->dry) HIPCHK(h, hipMemcpy(h->d_frows, fr.data(), sizeof(FRow) * fr.size(), hipMemcpyHostToDevice));
+        if (!h->dry && !pk.empty())
+            HIPCHK(h, hipMemcpy(h->d_fpk, pk.data(), sizeof(int4) * pk.size(), hipMemcpyHostToDevice));
+    }
     // force pass: slices of rows so the grid fills the chip
+    // about 6,000 wavefronts: two rounds at three wavefronts per SIMD
     const int wav = cdiv(h->n, 64);
-    h->nslice = std::max(1, std::min(16, cdiv(4096, std::max(1, wav))));
-    h->rows_per_slice = cdiv(std::max(1, h->nrows), h->nslice);
+    h->nslice = std::max(1, std::min(16, cdiv(6000, std::max(1, wav))));
+    h->rows_per_slice = cdiv(std::max(1, h->nfrows), h->nslice);
     CHK(dalloc(h, &h->d_rpart, (size_t)h->nslice * 3 * h->n));
     return MDX_OK;
 }
@@ -579,7 +632,13 @@ void fill_args(mdx_engine *h) {
     ra.sk = h->d_sk;
     ra.nslice = h->nslice;
     ra.rows_per_slice = h->rows_per_slice;
-    ra.natom_blocks = cdiv(n, 64);
+    ra.nfrows = h->nfrows;
+    ra.npk = h->npk;
+    ra.frows = h->d_frows;
+    ra.fpk = h->d_fpk;
+    ra.fpack = h->d_fpack;
+    ra.fa0 = h->d_fa0;
+    ra.natom_blocks = cdiv(n, kForceBlock);
     ra.rpart = h->d_rpart;
     ra.epart = erow(h, h->row_sk);
 
@@ -1169,7 +1228,8 @@ void drop_graphs(mdx_batch *b) {
 mdx_status plan_fill(mdx_batch *b, bool maps) {
     const int K = (int)b->sys.size();
     b->zero.clear(); b->pack.clear(); b->pairs.clear(); b->special.clear();
-    b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
+    b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->fpk.clear(); b->rforce.clear();
+    b->mz = 1;
     b->et.clear(); b->bonds.clear(); b->angles.clear(); b->tors.clear();
     b->springs.clear(); b->comb.clear(); b->red.clear(); b->dred.clear();
     b->nmt_max = 1;
@@ -1195,7 +1255,9 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         b->rprep.add(h->a_recip, cdiv(n, 256));
         b->sk.add(h->a_recip, h->a_recip.nrowgroups * h->nchunk);
         b->skred.add(h->a_recip, cdiv(std::max(1, h->nkv), 256));
+        b->fpk.add(h->a_recip, cdiv(h->npk + h->nfrows, 256));
         b->rforce.add(h->a_recip, h->a_recip.natom_blocks * h->nslice);
+        b->mz = std::max(b->mz, h->mz);
         if (h->neterm) b->et.add(h->a_et, cdiv(h->neterm, 256));
         if (h->nb) b->bonds.add(h->a_bonds, cdiv(h->nb, 256));
         if (h->na) b->angles.add(h->a_angles, cdiv(h->na, 256));
@@ -1250,6 +1312,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->special, maps));
     CHK(kplan_upload(b, b->rprep, maps)); CHK(kplan_upload(b, b->sk, maps));
     CHK(kplan_upload(b, b->skred, maps)); CHK(kplan_upload(b, b->rforce, maps));
+    CHK(kplan_upload(b, b->fpk, maps));
     CHK(kplan_upload(b, b->et, maps)); CHK(kplan_upload(b, b->bonds, maps));
     CHK(kplan_upload(b, b->angles, maps)); CHK(kplan_upload(b, b->tors, maps));
     CHK(kplan_upload(b, b->springs, maps)); CHK(kplan_upload(b, b->comb, maps));
@@ -1304,8 +1367,11 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
         else launch(b, b->sk, 64, 0, k_sk<96>);
         mark(5);
         launch(b, b->skred, 256, 0, k_sk_reduce);
+        launch(b, b->fpk, 256, 0, k_fpack);
         mark(6);
-        launch(b, b->rforce, 64, 0, k_recip_force);
+        if (b->mz <= 16) launch(b, b->rforce, kForceBlock, 0, k_recip_force<16>);
+        else if (b->mz <= 32) launch(b, b->rforce, kForceBlock, 0, k_recip_force<32>);
+        else launch(b, b->rforce, kForceBlock, 0, k_recip_force<48>);
         mark(7);
         launch(b, b->et, 256, 0, k_eterms);
     } else {
@@ -1354,10 +1420,10 @@ mdx_status batch_launch(mdx_batch *b, int group, bool detail, bool events) {
         // launch shapes depend on capacities only; a new box (k set) or a
         // regrown capacity changes the maps and the captured graphs
         std::vector<int> before = {b->pairs.nblocks(), b->sk.nblocks(), b->rforce.nblocks(), b->skred.nblocks(),
-                                   b->zero.nblocks(), b->maxlen};
+                                   b->zero.nblocks(), b->maxlen, b->mz, b->fpk.nblocks(), b->nmt_max, b->nlj_max};
         CHK(plan_fill(b, true));
         std::vector<int> after = {b->pairs.nblocks(), b->sk.nblocks(), b->rforce.nblocks(), b->skred.nblocks(),
-                                  b->zero.nblocks(), b->maxlen};
+                                  b->zero.nblocks(), b->maxlen, b->mz, b->fpk.nblocks(), b->nmt_max, b->nlj_max};
         if (before != after) drop_graphs(b);
     }
     if (b->use_graph && !events && !detail) {
@@ -1757,7 +1823,7 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     drop_graphs(b);
     kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->pairs); kplan_free(b->special);
-    kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
+    kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce); kplan_free(b->fpk);
     kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->angles); kplan_free(b->tors);
     kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);
     kplan_free(b->md_kd); kplan_free(b->md_sh); kplan_free(b->md_ke); kplan_free(b->md_th);

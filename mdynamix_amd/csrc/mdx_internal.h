@@ -43,6 +43,7 @@ constexpr int kClu = 8;            // atoms per cluster
 // [POTE1(ntp) | POTL1(ntp) | PES141(nmt) | PSR141(nmt)]
 constexpr int kMaxDet = 2 * 137 + 2 * 16;
 constexpr int kWave = 64;          // wavefront size on gfx950
+constexpr int kForceBlock = 256;   // atoms per block of the reciprocal force pass
 
 // Energy/virial slots of the per-block partial sums.
 enum EQ {
@@ -67,6 +68,14 @@ struct Box {
 
 struct RecipRow {           // 16 bytes
     int nx, ny, nz0, len;
+};
+
+// Rows of the force pass, in (n_x, n_y) order so consecutive rows of one
+// n_x differ by one in n_y.  Entries j = 1..m pair the k vectors with
+// n_z = +j and -j (absent ones count as A = 0); k0 is the n_z = 0 vector
+// or -1.
+struct FRow {               // 32 bytes
+    int ix, iy, m, off, k0, pad0, pad1, pad2;
 };
 
 __host__ __device__ inline double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
@@ -216,6 +225,12 @@ struct RecipArgs {
     int nslice, rows_per_slice, natom_blocks;
     double *rpart;               // [nslice][3][natoms]
     double *epart;               // rows of k_sk_reduce
+    // force pass over (n_x, n_y) rows with +-n_z pairs
+    int nfrows, npk;
+    const FRow *frows;
+    const int4 *fpk;             // per pair entry: k(+j), k(-j), j
+    float4 *fpack;               // per pair entry: D = (A+r - A-r, A+r + A-r, A+i + A-i, A+i - A-i)
+    float2 *fa0;                 // per row: A of n_z = 0
 };
 
 struct ScatterArgs {

@@ -13,8 +13,9 @@
 //   partials and added in FP64 by k_sk_reduce, which also forms the energy
 //   QPE, the virial projections and A_k = 2 AKC S(k) for the force pass.
 // k_recip_force: lane = atom, rows streamed uniformly; the force on an atom
-//   is q sum_k k Im(exp(i k.r) conj(A_k)) (forces.f:1320-1326), again with
-//   one reduced-phase sincospi per row and the same recurrence along n_z.
+//   is q sum_k k Im(exp(i k.r) conj(A_k)) (forces.f:1320-1326), evaluated
+//   for the k vectors +-n_z together on a per-atom table of exp(2 pi i j f_z)
+//   (see recip_force_body).
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -170,77 +171,141 @@ __global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__
     sk_reduce_body(args[m.x], m.y, red);
 }
 
-// Reciprocal forces; a block covers 64 atoms and one slice of rows, each
-// slice writes its own partial force array (summed by the combine kernel).
-// Rows and their A_k are wavefront-uniform (scalar loads, the next row's
-// header one row ahead); the phase factors follow the same anchored
-// three-term recurrence as k_sk.
+// Pair packing for the force pass: for each (row, j) the sums and
+// differences of A_k = 2 AKC S(k) at n_z = +j and -j; per row the n_z = 0
+// value.
+__device__ inline void fpack_body(const RecipArgs &a, int blk) {
+    const int e = blk * 256 + threadIdx.x;
+    if (e < a.npk) {
+        const int4 k = ldv(gld(a.fpk) + e);
+        const float2 ap = k.x >= 0 ? ldv(gld(a.ak) + k.x) : make_float2(0.f, 0.f);
+        const float2 am = k.y >= 0 ? ldv(gld(a.ak) + k.y) : make_float2(0.f, 0.f);
+        stv(gst(a.fpack) + e, make_float4(ap.x - am.x, ap.x + am.x, ap.y + am.y, ap.y - am.y));
+    } else if (e < a.npk + a.nfrows) {
+        const int r = e - a.npk;
+        const int k0 = gld(a.frows)[r].k0;
+        stv(gst(a.fa0) + r, k0 >= 0 ? ldv(gld(a.ak) + k0) : make_float2(0.f, 0.f));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fpack(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    fpack_body(args[m.x], m.y);
+}
+
+// Reciprocal forces (forces.f:1320-1326); a block covers kForceBlock atoms
+// (four wavefronts on one CU, which share the row data through the scalar
+// cache) and one slice of rows; each slice writes its own partial force
+// array (summed by the combine kernel).  With b = q exp(2 pi i (n_x f_x + n_y f_y)) and
+// p_j = exp(2 pi i j f_z), the two vectors n_z = +-j contribute
+//   t(+j) + t(-j) = b_r (p_i D1 - p_r D3) + b_i (p_r D2 + p_i D4)
+//   t(+j) - t(-j) = b_r (p_i D2 - p_r D4) + b_i (p_r D1 + p_i D3)
+// with t = Im(exp(i k.r) conj(A_k)), so a row costs 10 VALU operations per
+// pair of k vectors on the per-atom table p_j (FP64 recurrence, kept in
+// registers) and the row's pair data (wavefront-uniform, scalar loads).  b advances
+// along n_y by exp(2 pi i f_y) in FP64 and restarts from an FP64 sincospi
+// at every new n_x.
+template <int MZ>
 __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     const int ab = blk % a.natom_blocks, sl = blk / a.natom_blocks;
-    const int i = ab * 64 + threadIdx.x;
+    const int i = ab * kForceBlock + threadIdx.x;
     const int r0 = sl * a.rows_per_slice;
-    const int r1 = min(r0 + a.rows_per_slice, a.nrows);
+    const int r1 = min(r0 + a.rows_per_slice, a.nfrows);
     double4 at = make_double4(0, 0, 0, 0);
     double4 st = make_double4(1.0, 0.0, 1.0, 0.0);
     if (i < a.natoms) { at = ldv(gld(a.fq) + i); st = ldv(gld(a.stepz) + i); }
-    const float sr = (float)st.x, si = (float)st.y, c2 = 2.0f * sr;
+    float pr[MZ], pim[MZ];
+    {
+        double wr = st.x, wi = st.y;
+#pragma unroll
+        for (int j = 0; j < MZ; ++j) {
+            pr[j] = (float)wr;
+            pim[j] = (float)wi;
+            const double t = wr * st.x - wi * st.y;
+            wi = wr * st.y + wi * st.x;
+            wr = t;
+        }
+    }
+    double eyr, eyi;
+    {
+        const double v = at.y - rint(at.y);
+        sincospi(2.0 * v, &eyi, &eyr);
+    }
     const double tx = 2.0 * kPI / a.box.l[0], ty = 2.0 * kPI / a.box.l[1], tz = 2.0 * kPI / a.box.l[2];
-    const MDX_CONST RecipRow *rows = sld(a.rows);
-    const MDX_CONST int *rowoff = sld(a.rowoff);
-    const MDX_CONST float2 *ak = sld(a.ak);
+    const MDX_CONST FRow *frows = sld(a.frows);
+    const MDX_CONST float4 *fpack = sld(a.fpack);
+    const MDX_CONST float2 *fa0 = sld(a.fa0);
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
     if (r0 < r1) {
-        RecipRow nrow = ldv(rows + r0);
-        int noff = rowoff[r0];
+        double Br = 0.0, Bi = 0.0;
+        int cix = -0x7fffffff, ciy = 0;
+        // the pair data of consecutive rows is contiguous: the blocks after a
+        // row's last one are the next row's first ones (the array ends with
+        // two spare blocks), so blocks are requested two blocks ahead into a
+        // three-block ring of registers whose roles rotate at compile time
+        FRow nfr = ldv(frows + r0);
+        float4 buf[3][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            buf[0][q] = ldv(fpack + nfr.off + q);
+            buf[1][q] = ldv(fpack + nfr.off + 4 + q);
+        }
         for (int r = r0; r < r1; ++r) {
-            const RecipRow row = nrow;
-            const MDX_CONST float2 *A = ak + noff;
-            if (r + 1 < r1) { nrow = ldv(rows + r + 1); noff = rowoff[r + 1]; }
-            double s0, c0;
-            row_start(row, at, c0, s0);
-            double Wr = at.w * c0, Wi = at.w * s0;
-            float ts = 0.f, tzs = 0.f;
-            int j0 = 0;
-            // full blocks of 8: the eight A_k are uniform, loaded together
-            for (; j0 + 8 <= row.len; j0 += 8) {
-                float2 Av[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) Av[q] = ldv(A + j0 + q);
-                float wr0 = (float)Wr, wi0 = (float)Wi;
-                float wr1 = wr0 * sr - wi0 * si, wi1 = wr0 * si + wi0 * sr;
-                float t = wi0 * Av[0].x - wr0 * Av[0].y;
-                ts += t;
-                tzs = fmaf((float)j0, t, tzs);
-                t = wi1 * Av[1].x - wr1 * Av[1].y;
-                ts += t;
-                tzs = fmaf((float)(j0 + 1), t, tzs);
-#pragma unroll
-                for (int q = 2; q < 8; ++q) {
-                    const float wr2 = fmaf(c2, wr1, -wr0), wi2 = fmaf(c2, wi1, -wi0);
-                    t = wi2 * Av[q].x - wr2 * Av[q].y;
-                    ts += t;
-                    tzs = fmaf((float)(j0 + q), t, tzs);
-                    wr0 = wr1; wi0 = wi1; wr1 = wr2; wi1 = wi2;
-                }
-                const double tr = Wr * st.z - Wi * st.w;
-                Wi = Wr * st.w + Wi * st.z;
-                Wr = tr;
+            const FRow fr = nfr;
+            if (r + 1 < r1) nfr = ldv(frows + r + 1);
+            if (fr.ix != cix || fr.iy != ciy + 1) {
+                double v = (double)fr.ix * at.x + (double)fr.iy * at.y;
+                v = v - rint(v);
+                double s, c;
+                sincospi(2.0 * v, &s, &c);
+                Br = at.w * c;
+                Bi = at.w * s;
+            } else {
+                const double t = Br * eyr - Bi * eyi;
+                Bi = Br * eyi + Bi * eyr;
+                Br = t;
             }
-            {
-                float wr = (float)Wr, wi = (float)Wi;
-                for (int j = j0; j < row.len; ++j) {
-                    const float2 Ak = ldv(A + j);
-                    const float t = wi * Ak.x - wr * Ak.y;
-                    ts += t;
-                    tzs = fmaf((float)j, t, tzs);
-                    const float nr = wr * sr - wi * si;
-                    wi = wr * si + wi * sr;
-                    wr = nr;
+            cix = fr.ix;
+            ciy = fr.iy;
+            const float2 a0 = ldv(fa0 + r);
+            const MDX_CONST float4 *P = fpack + fr.off;
+            const int nb = (fr.m + 3) >> 2;
+            float sx = 0.f, sy = 0.f, zx = 0.f, zy = 0.f;
+#pragma unroll
+            for (int jb = 0; jb < MZ / 4; ++jb) {
+                if (jb >= nb) break;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) buf[(jb + 2) % 3][q] = ldv(P + (jb + 2) * 4 + q);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int j = jb * 4 + q;
+                    const float4 d = buf[jb % 3][q];
+                    sx = fmaf(pim[j], d.x, sx);
+                    sx = fmaf(-pr[j], d.z, sx);
+                    sy = fmaf(pr[j], d.y, sy);
+                    sy = fmaf(pim[j], d.w, sy);
+                    // n_z-weighted: (j + 1) (p_i D2 - p_r D4), (j + 1) (p_r D1 + p_i D3)
+                    const float ux = fmaf(pim[j], d.y, -pr[j] * d.w);
+                    const float uy = fmaf(pr[j], d.x, pim[j] * d.z);
+                    zx = fmaf((float)(j + 1), ux, zx);
+                    zy = fmaf((float)(j + 1), uy, zy);
                 }
             }
-            Fx += tx * row.nx * (double)ts;
-            Fy += ty * row.ny * (double)ts;
-            Fz += tz * ((double)row.nz0 * (double)ts + (double)tzs);
+            // the next row's first two blocks sit in buf[nb % 3], buf[(nb + 1) % 3]
+            const int rot = nb % 3;
+            if (rot == 1) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { buf[0][q] = buf[1][q]; buf[1][q] = buf[2][q]; }
+            } else if (rot == 2) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { const float4 t = buf[0][q]; buf[0][q] = buf[2][q]; buf[1][q] = t; }
+            }
+            const float br = (float)Br, bi = (float)Bi;
+            const float ts = (bi * a0.x - br * a0.y) + (br * sx + bi * sy);
+            const float tzs = br * zx + bi * zy;
+            Fx += tx * fr.ix * (double)ts;
+            Fy += ty * fr.iy * (double)ts;
+            Fz += tz * (double)tzs;
         }
     }
     if (i < a.natoms) {
@@ -251,9 +316,14 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     }
 }
 
-__global__ __launch_bounds__(64) void k_recip_force(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+template <int MZ>
+__global__ __launch_bounds__(kForceBlock) void k_recip_force(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
-    recip_force_body(args[m.x], m.y);
+    recip_force_body<MZ>(args[m.x], m.y);
 }
+
+template __global__ void k_recip_force<16>(const RecipArgs *, const int2 *);
+template __global__ void k_recip_force<32>(const RecipArgs *, const int2 *);
+template __global__ void k_recip_force<48>(const RecipArgs *, const int2 *);
 
 }  // namespace mdx
