@@ -317,7 +317,7 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
 }
 
 template <int MZ>
-__global__ __launch_bounds__(kForceBlock) void k_recip_force(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+__global__ __launch_bounds__(kForceBlock, 4) void k_recip_force(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     recip_force_body<MZ>(args[m.x], m.y);
 }
