@@ -244,6 +244,9 @@ def main() -> int:
     from mdynamix_amd.engine import device_info
     s, topo, prm, eng = build_engine()
     dev = device_info()
+    eng.chcnb()
+    tuned = eng.autotune()
+    print("autotuned k-space shape: %s" % tuned, file=sys.stderr, flush=True)
     rebuild = s.rebuild_every
     stage_sum = {}
     list_ms = []
@@ -323,7 +326,7 @@ def main() -> int:
         "data": "synthetic (seeded jittered lattice, random orientations)",
         "config": {"workload": "spce_nacl_32k", "atoms": topo.natoms, "box_A": s.box[0], "rcut_A": s.rcut,
                    "skin_A": s.skin, "rebuild_every": rebuild, "kvectors": kv_count,
-                   "pairs_per_call": pairs_call, "list": "verlet-cluster-8"},
+                   "pairs_per_call": pairs_call, "list": "verlet-cluster-8", "kspace_shape": tuned},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "device": dev,
         "roofline": roof,

@@ -232,6 +232,14 @@ mdx_status mdx_get_forces(mdx_engine *h, double *fx, double *fy, double *fz, dou
 /* Per-stage HIP-event timing of mdx_compute (on by default; off replays the
  * call from a captured hipGraph). */
 mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
+/* Autotuner: times the reciprocal-space kernels of this system over a set
+ * of launch shapes (atoms per structure-factor block; row slices of the
+ * force pass) on the current coordinates and keeps the fastest; the choice
+ * is remembered for the process by size class (atoms, k vectors) and used
+ * by later mdx_create calls of that class.  The shape changes only how the
+ * sums are split into partial sums (results agree to rounding; each shape
+ * is deterministic).  Needs a built list. */
+mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk, int32_t *nslice);
 /* Device memory held by a handle, in bytes (admission by HBM budget). */
 mdx_status mdx_memory_bytes(const mdx_engine *h, int64_t *bytes);
 /* Device memory mdx_create would allocate for this system (plus the MD

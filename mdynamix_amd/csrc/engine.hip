@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -112,6 +113,7 @@ struct mdx_engine {
     // reciprocal space
     int nkv = 0, nrows = 0, maxlen = 0, chunk = 512, nchunk = 0, nslice = 1, rows_per_slice = 0;
     int nfrows = 0, npk = 0, mz = 0;
+    int tuned_nslice = 0;       // 0: heuristic
     FRow *d_frows = nullptr;
     int4 *d_fpk = nullptr;
     float4 *d_fpack = nullptr;
@@ -406,10 +408,10 @@ mdx_status setup_kspace(mdx_engine *h) {
         if (!h->dry && !pk.empty())
             HIPCHK(h, hipMemcpy(h->d_fpk, pk.data(), sizeof(int4) * pk.size(), hipMemcpyHostToDevice));
     }
-    // force pass: slices of rows so the grid fills the chip
-    // about 6,000 wavefronts: two rounds at three wavefronts per SIMD
+    // force pass: slices of rows so the grid fills the chip (about 6,000
+    // wavefronts: one and a half rounds at four per SIMD) unless tuned
     const int wav = cdiv(h->n, 64);
-    h->nslice = std::max(1, std::min(16, cdiv(6000, std::max(1, wav))));
+    h->nslice = h->tuned_nslice > 0 ? h->tuned_nslice : std::max(1, std::min(16, cdiv(6000, std::max(1, wav))));
     h->rows_per_slice = cdiv(std::max(1, h->nfrows), h->nslice);
     CHK(dalloc(h, &h->d_rpart, (size_t)h->nslice * 3 * h->n));
     return MDX_OK;
@@ -513,6 +515,37 @@ void layout_rows(mdx_engine *h) {
 }
 
 double *erow(mdx_engine *h, int row) { return h->d_epart + (size_t)row * Q_N; }
+
+void fill_args(mdx_engine *h);
+
+// k-space launch shape: atoms per structure-factor block and row slices of
+// the force pass (both change only the work split, not the sums' content
+// or order within a k vector and atom).
+mdx_status set_kspace_shape(mdx_engine *h, int chunk, int nslice) {
+    h->chunk = std::max(64, chunk);
+    h->nchunk = cdiv(h->n, h->chunk);
+    CHK(dalloc(h, &h->d_part, (size_t)h->nchunk * std::max(1, h->nkv)));
+    h->tuned_nslice = std::max(1, std::min(64, nslice));
+    h->nslice = h->tuned_nslice;
+    h->rows_per_slice = cdiv(std::max(1, h->nfrows), h->nslice);
+    CHK(dalloc(h, &h->d_rpart, (size_t)h->nslice * 3 * h->n));
+    fill_args(h);
+    return MDX_OK;
+}
+
+// Tuned shapes per size class (atoms, k vectors; quarter-octave buckets),
+// shared by every handle of the process.
+struct TuneKey {
+    int n, k;
+    bool operator<(const TuneKey &o) const { return n != o.n ? n < o.n : k < o.k; }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tune;
+
+TuneKey tune_key(const mdx_engine *h) {
+    return TuneKey{(int)std::lround(4.0 * std::log2(std::max(1, h->n))),
+                   (int)std::lround(4.0 * std::log2(std::max(1, h->nkv)))};
+}
 
 void fill_args(mdx_engine *h);
 
@@ -876,6 +909,15 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
     if (dry) {
         *out = h;
         return MDX_OK;
+    }
+    {
+        std::pair<int, int> shape(0, 0);
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(tune_key(h));
+            if (it != g_tune.end()) shape = it->second;
+        }
+        if (shape.first > 0 && (s = set_kspace_shape(h, shape.first, shape.second))) return fail(s);
     }
     mdx_batch *self = nullptr;
     mdx_engine *one[1] = {h};
@@ -1646,6 +1688,55 @@ mdx_status mdx_get_forces(mdx_engine *h, double *fx, double *fy, double *fz, dou
                           double *hx, double *hy, double *hz) {
     if (!h) return MDX_ERR_ARG;
     return copy_forces(h, fx, fy, fz, gx, gy, gz, hx, hy, hz, hipMemcpyDeviceToHost);
+}
+
+mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) {
+    if (!h) return MDX_ERR_ARG;
+    if (!h->have_list) {
+        h->err = "mdx_autotune before mdx_build_list";
+        return MDX_ERR_STATE;
+    }
+    mdx_batch *b = h->self;
+    const bool timing0 = b->timing;
+    b->timing = true;
+    auto trial = [&](int chunk, int nslice, double &sk_ms, double &force_ms) -> mdx_status {
+        CHK(set_kspace_shape(h, chunk, nslice));
+        sk_ms = force_ms = 1e30;
+        for (int rep = 0; rep < 4; ++rep) {
+            mdx_status st = batch_compute(b, MDX_FORCES_SLOW, false);
+            if (st != MDX_OK) {
+                h->err = b->err;
+                return st;
+            }
+            if (rep == 0) continue;     // warm-up (plan, allocation)
+            sk_ms = std::min(sk_ms, b->stage_ms[ST_SK] + b->stage_ms[ST_SKRED]);
+            force_ms = std::min(force_ms, b->stage_ms[ST_RFORCE] + b->stage_ms[ST_REDUCE]);
+        }
+        return MDX_OK;
+    };
+    const int wav = cdiv(h->n, 64);
+    int best_chunk = h->chunk, best_slice = h->nslice;
+    double best_sk = 1e30, best_force = 1e30, sk = 0.0, fo = 0.0;
+    for (int chunk : {256, 512, 1024, 2048}) {
+        if (chunk > 2 * h->n && chunk != 256) continue;
+        CHK(trial(chunk, best_slice, sk, fo));
+        if (sk < best_sk) { best_sk = sk; best_chunk = chunk; }
+    }
+    for (int waves : {2048, 3072, 4096, 6144, 8192, 12288, 16384}) {
+        const int ns = std::max(1, std::min(64, cdiv(waves, std::max(1, wav))));
+        if (ns == best_slice && best_force < 1e30) continue;
+        CHK(trial(best_chunk, ns, sk, fo));
+        if (fo < best_force) { best_force = fo; best_slice = ns; }
+    }
+    CHK(set_kspace_shape(h, best_chunk, best_slice));
+    b->timing = timing0;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[tune_key(h)] = std::make_pair(best_chunk, best_slice);
+    }
+    if (chunk_out) *chunk_out = best_chunk;
+    if (nslice_out) *nslice_out = best_slice;
+    return MDX_OK;
 }
 
 mdx_status mdx_set_timing(mdx_engine *h, int32_t on) {

@@ -193,3 +193,18 @@ def test_repeat_call_is_bit_identical():
     f2, _, _, e2 = eng.allforce()
     assert np.array_equal(f1, f2)
     assert all(e1[k] == e2[k] for k in abi.MdxEnergies.SCALARS)
+
+
+def test_autotuned_shape_keeps_parity():
+    """The autotuner only changes launch shapes: forces and energies stay
+    within the tolerances and the list is unchanged."""
+    d, topo, prm, eng = _engine_from_golden("water_508")
+    eng.set_positions(golden_positions(d))
+    eng.chcnb()
+    shape = eng.autotune()
+    print("case water_508 tuned", shape, flush=True)
+    assert shape["chunk"] > 0 and shape["nslice"] > 0
+    eng.set_molecule_com(golden_com(d, 0))
+    f, g, h, e = eng.allforce()
+    assert rms_rel(f, golden_forces(d, "F", 0)) <= FORCE_RMS_TOL
+    check_energies(e, dict(zip(abi.MdxEnergies.SCALARS, d["ENERGY"][0])))
