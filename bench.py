@@ -37,6 +37,19 @@ PAIR_FLOP = 74               # per pair within R_cut (forces.f:1071-1198 operati
 RECIP_FLOP = 21              # per (k, atom) term (forces.f:1288-1325 operation count)
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (lines 'traffic_bytes <kernel> <bytes>'), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_spce_nacl_32k.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 3 and f[0] == "traffic_bytes" and (f[1] == kernel or f[1].startswith(kernel + "<")):
+            return int(f[-1])
+    return None
+
+
 def build_engine(seed: int = 2025):
     from mdynamix_amd import systems, topology
     from mdynamix_amd.engine import ForceEngine
@@ -231,6 +244,8 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--kspace-shape", default="", metavar="CHUNK,NSLICE",
+                    help="fixed k-space launch shape instead of autotuning (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (traffic counter calibration)")
     args = ap.parse_args()
@@ -245,8 +260,13 @@ def main() -> int:
     s, topo, prm, eng = build_engine()
     dev = device_info()
     eng.chcnb()
-    tuned = eng.autotune()
-    print("autotuned k-space shape: %s" % tuned, file=sys.stderr, flush=True)
+    if args.kspace_shape:
+        c, n = (int(v) for v in args.kspace_shape.split(","))
+        eng.set_kspace_shape(c, n)
+        tuned = {"chunk": c, "nslice": n, "source": "--kspace-shape"}
+    else:
+        tuned = dict(eng.autotune(), source="autotune")
+    print("k-space shape: %s" % tuned, file=sys.stderr, flush=True)
     rebuild = s.rebuild_every
     stage_sum = {}
     list_ms = []
@@ -303,11 +323,14 @@ def main() -> int:
     dom = max(cand, key=lambda k: cand[k][0])
     dms, dflop = cand[dom]
     achieved = dflop / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
+    traffic = measured_traffic(dom)
     roof = {"kernel": dom, "bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "avg_ms": round(dms, 4), "algorithmic_flop": dflop,
             "note": ("k_pairs: 74 FLOP per pair within R_cut; k_sk / k_recip_force: half of the 21 FLOP "
-                     "per (k, atom) term of FURIR each")}
+                     "per (k, atom) term of FURIR each; peak is the packed-FP32 figure (unpacked FP32 "
+                     "and FP64 vector: 78.6 TFLOP/s); traffic: HBM bytes per launch from "
+                     "profiles/traffic_spce_nacl_32k.md (FETCH_SIZE/WRITE_SIZE, copy-calibrated)")}
     out = {
         "metric": "pair-interactions/s (32k SPC/E + NaCl, full force call incl. Ewald k-space)",
         "value": value,

@@ -240,6 +240,8 @@ mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
  * sums are split into partial sums (results agree to rounding; each shape
  * is deterministic).  Needs a built list. */
 mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk, int32_t *nslice);
+/* Set that shape directly (reproducible runs, profiling). */
+mdx_status mdx_set_kspace_shape(mdx_engine *h, int32_t chunk, int32_t nslice);
 /* Device memory held by a handle, in bytes (admission by HBM budget). */
 mdx_status mdx_memory_bytes(const mdx_engine *h, int64_t *bytes);
 /* Device memory mdx_create would allocate for this system (plus the MD

@@ -1739,6 +1739,11 @@ mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) 
     return MDX_OK;
 }
 
+mdx_status mdx_set_kspace_shape(mdx_engine *h, int32_t chunk, int32_t nslice) {
+    if (!h || chunk <= 0 || nslice <= 0) return MDX_ERR_ARG;
+    return set_kspace_shape(h, chunk, nslice);
+}
+
 mdx_status mdx_set_timing(mdx_engine *h, int32_t on) {
     if (!h) return MDX_ERR_ARG;
     h->timing = on != 0;

@@ -71,6 +71,7 @@ def load_library() -> C.CDLL:
     L.mdx_batch_last_error.restype = C.c_char_p
     L.mdx_batch_destroy.argtypes = [pv]
     L.mdx_autotune.argtypes = [pv, pi32, pi32]
+    L.mdx_set_kspace_shape.argtypes = [pv, C.c_int32, C.c_int32]
     L.mdx_md_init.argtypes = [pv, pv, pd, pd, pd]
     L.mdx_md_run.argtypes = [pv, C.c_int32, pv]
     L.mdx_md_get_state.argtypes = [pv] + [pd] * 6
@@ -80,7 +81,7 @@ def load_library() -> C.CDLL:
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
-                 "mdx_batch_md_run", "mdx_autotune"):
+                 "mdx_batch_md_run", "mdx_autotune", "mdx_set_kspace_shape"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -215,6 +216,9 @@ class ForceEngine:
         c, sl = C.c_int32(), C.c_int32()
         self._check(self.lib.mdx_autotune(self._h, C.byref(c), C.byref(sl)))
         return {"chunk": c.value, "nslice": sl.value}
+
+    def set_kspace_shape(self, chunk: int, nslice: int) -> None:
+        self._check(self.lib.mdx_set_kspace_shape(self._h, int(chunk), int(nslice)))
 
     def get_forces(self):
         """(F, G, H) of the last force call, e.g. after a batched call."""
