@@ -358,6 +358,13 @@ mdx_status setup_kspace(mdx_engine *h) {
     if (!h->dry) HIPCHK(h, hipMemcpy(h->d_kidx, kidx.data(), sizeof(int4) * kidx.size(), hipMemcpyHostToDevice));
     CHK(dalloc(h, &h->d_fq, h->n));
     CHK(dalloc(h, &h->d_stepz, h->n));
+    if (h->tuned_nslice == 0) {
+        // about 2,048 structure-factor blocks (row groups x atom chunks)
+        const long long want = (long long)h->n * cdiv(std::max(1, h->nrows), 64) / 2048;
+        int c = 64;
+        while (c < want && c < 1024) c *= 2;
+        h->chunk = c;
+    }
     h->nchunk = cdiv(h->n, h->chunk);
     CHK(dalloc(h, &h->d_part, (size_t)h->nchunk * std::max(1, h->nkv)));
     CHK(dalloc(h, &h->d_ak, std::max(1, h->nkv)));

@@ -32,7 +32,68 @@ __device__ inline void md_kick_drift_body(const MdArgs &a, int blk) {
 
 // One molecule per thread, the reference's iteration: every constraint
 // whose atoms moved in the previous sweep is corrected along the old bond
-// vector until |r^2 - RB^2| <= RB^2 TOLER^2 for all of them.
+// vector until |r^2 - RB^2| <= RB^2 TOLER^2 for all of them.  The molecule's
+// coordinates live in registers (up to kShakeRegAtoms atoms, e.g. water) or
+// are read and written in HBM (larger molecules).
+constexpr int kShakeRegAtoms = 4;
+
+template <int NA>
+__device__ inline void md_shake_regs(const MdArgs &a, int c0, int c1, int a0) {
+#pragma clang fp contract(off)
+    double x[NA], y[NA], z[NA], ox[NA], oy[NA], oz[NA], mi[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        const int i = a0 + k;
+        x[k] = a.x[i]; y[k] = a.y[i]; z[k] = a.z[i];
+        ox[k] = a.ox[i]; oy[k] = a.oy[i]; oz[k] = a.oz[i];
+        mi[k] = a.minv[i];
+    }
+    unsigned moved = 0u, moving = 0u;
+    for (int c = c0; c < c1; ++c) moved |= (1u << (a.cons[c].x - a0)) | (1u << (a.cons[c].y - a0));
+    bool done = false;
+    int iter = 0;
+    while (!done && iter <= a.maxit) {
+        done = true;
+        for (int c = c0; c < c1; ++c) {
+            const int i = a.cons[c].x - a0, j = a.cons[c].y - a0;
+            const unsigned bi = 1u << i, bj = 1u << j;
+            if (!((moved & bi) || (moved & bj))) continue;
+            const double bndsq = a.cons_len2[c];
+            double bx = x[i] - x[j], by = y[i] - y[j], bz = z[i] - z[j];
+            min_image(bx, by, bz, a.box);
+            double bsq = bx * bx + by * by;
+            bsq = bsq + bz * bz;
+            const double diffsq = bndsq - bsq;
+            if (fabs(diffsq) > bndsq * a.tol2) {
+                double dx = ox[i] - ox[j], dy = oy[i] - oy[j], dz = oz[i] - oz[j];
+                min_image(dx, dy, dz, a.box);
+                double vdot = bx * dx + by * dy;
+                vdot = vdot + bz * dz;
+                if (vdot < bndsq * a.bdtol) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
+                const double rma = mi[i], rmb = mi[j];
+                const double gab = diffsq / (2.0 * (rma + rmb) * vdot);
+                dx = dx * gab; dy = dy * gab; dz = dz * gab;
+                x[i] = x[i] + rma * dx; y[i] = y[i] + rma * dy; z[i] = z[i] + rma * dz;
+                x[j] = x[j] - rmb * dx; y[j] = y[j] - rmb * dy; z[j] = z[j] - rmb * dz;
+                moving |= bi | bj;
+                done = false;
+            }
+        }
+        moved = moving;
+        moving = 0u;
+        ++iter;
+    }
+    if (!done) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        const int i = a0 + k;
+        a.x[i] = x[k]; a.y[i] = y[k]; a.z[i] = z[k];
+        a.vx[i] = a.vx[i] + (x[k] - a.ux[i]) / a.dt;
+        a.vy[i] = a.vy[i] + (y[k] - a.uy[i]) / a.dt;
+        a.vz[i] = a.vz[i] + (z[k] - a.uz[i]) / a.dt;
+    }
+}
+
 __device__ inline void md_shake_body(const MdArgs &a, int blk) {
 #pragma clang fp contract(off)
     const int m = blk * 256 + threadIdx.x;
@@ -40,6 +101,12 @@ __device__ inline void md_shake_body(const MdArgs &a, int blk) {
     const int c0 = a.cstart[m], c1 = a.cstart[m + 1];
     if (c0 == c1) return;
     const int a0 = a.mstart[m], a1 = a.mstart[m + 1];
+    switch (a1 - a0) {
+        case 2: md_shake_regs<2>(a, c0, c1, a0); return;
+        case 3: md_shake_regs<3>(a, c0, c1, a0); return;
+        case 4: md_shake_regs<4>(a, c0, c1, a0); return;
+        default: break;
+    }
     unsigned long long moved = 0ull, moving = 0ull;   // bit (atom - a0)
     for (int c = c0; c < c1; ++c) {
         moved |= 1ull << (a.cons[c].x - a0);

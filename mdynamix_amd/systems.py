@@ -123,6 +123,8 @@ class System:
     ewald_b: float = 9.0             # FEXP
     dt_fs: float = 2.0
     shake: bool = True
+    # molecule types constrained under SHAKE (None: every type with bonds)
+    shake_types: Optional[Tuple[int, ...]] = None
     nfreq: int = 10                  # double time step ratio when not SHAKE
     temperature: float = 300.0
     skin: float = 0.0
@@ -167,7 +169,11 @@ class System:
                   "Nose_thermostat %.3f 30." % self.temperature,
                   "Time_step %.6f" % self.dt_fs, "Number_steps %d" % nsteps]
         if self.shake:
-            lines.append("Constraints 1.d-6")
+            if self.shake_types is None:
+                lines.append("Constraints 1.d-6")
+            else:
+                flags = " ".join("1" if k in self.shake_types else "0" for k in range(len(self.types)))
+                lines.append("Constraints 1.d-6 %s" % flags)
         else:
             lines.append("Double_timestep %d" % self.nfreq)
         lines += ["R_cutoff %.10f" % self.rcut, "R_short %.10f" % self.rshort,
@@ -489,7 +495,8 @@ def ee_job(n_atoms_target: int, n_solute: int, lam: float, seed: int, rigid_solu
     """Expanded-ensemble style job: SPC/E water + 0.15 M NaCl sized to about
     `n_atoms_target` atoms, plus one solute of `n_solute` sites coupled at
     `lam` (LJ and charges scaled by lam^PW1, soft-core shift BEE(1-lam^PW2)).
-    `rigid_solute` gives the solute its bond network (for MD segments)."""
+    `rigid_solute` gives the solute its bond network (for MD segments); the
+    bonds are then flexible (harmonic) and only the waters are constrained."""
     rng = np.random.default_rng(seed)
     n_w = max(64, int((n_atoms_target - n_solute) / 3.0))
     n_ion = max(1, int(round(n_w * 0.15 / 55.5)))
@@ -518,7 +525,7 @@ def ee_job(n_atoms_target: int, n_solute: int, lam: float, seed: int, rigid_solu
     sysm = assemble("ee%d" % seed, types[:3], counts[:3], centres[:-1], rng, box=(box, box, box), seed=seed,
                     rcut=10.0, rshort=5.0, ewald_a=2.8, ewald_b=9.0, shake=True, skin=1.0)
     return dataclasses.replace(sysm, types=types, counts=counts, pos=np.concatenate([sysm.pos, sol_atoms]),
-                               ee_type=3, ee_lambda=lam)
+                               ee_type=3, ee_lambda=lam, shake_types=(0,) if rigid_solute else None)
 
 
 def maxwell_velocities(system: System, seed: int, temperature: float | None = None) -> np.ndarray:

@@ -280,16 +280,18 @@ BOLTZ_KJMOL = AVSNO * BOLTZ_J * 1e-3     # kJ/mol/K
 def md_params(system, tolerance: float = 1e-6) -> Dict:
     """MD-segment parameters of a generated system in the host units
     (A, fs, kJ/mol, g/mol): the constrained bonds of every molecule when
-    the system runs with SHAKE (all bonds with ID != 2 of constrained
-    molecule types, in .mmol order, as SHEJK walks them, mdstep.f:1115-1140),
+    the system runs with SHAKE (all bonds with ID != 2 of the constrained
+    molecule types -- ISHEJK, input.f:292-296 -- in .mmol order, as SHEJK
+    walks them, mdstep.f:1115-1140),
     the degrees of freedom FNST (main.f:297-348: 3 per atom, minus one per
     constraint, minus 3) and the Nose-Hoover time constant."""
     ci, cj, ln = [], [], []
     off = 0
     ndof = 0.0
-    for t, cnt in zip(system.types, system.counts):
+    for k, (t, cnt) in enumerate(zip(system.types, system.counts)):
+        con = system.shake and (system.shake_types is None or k in system.shake_types)
         for m in range(cnt):
-            if system.shake:
+            if con:
                 for b in t.bonds:
                     if b[0] != 2:
                         ci.append(off + b[1] - 1)
@@ -297,7 +299,7 @@ def md_params(system, tolerance: float = 1e-6) -> Dict:
                         ln.append(b[3])
             off += t.nsites
         ndof += 3.0 * t.nsites * cnt
-        if system.shake:
+        if con:
             ndof -= sum(1 for b in t.bonds if b[0] != 2) * cnt
     ndof -= 3.0
     return dict(dt=system.dt_fs, accel_factor=1e-4, ekin_factor=1e4, boltz=BOLTZ_KJMOL,
