@@ -207,6 +207,8 @@ struct mdx_batch {
     double *h_det = nullptr;         // pinned copy
     // MD segments
     KPlan<MdArgs> md_kd, md_sh, md_ke, md_th, md_gc, md_tr;
+    hipGraph_t md_graph = nullptr;   // one MD step (no list rebuild)
+    hipGraphExec_t md_exec = nullptr;
     double *d_trace = nullptr;
     size_t trace_cap = 0;
     double *d_energy = nullptr;      // [K][Q_N]
@@ -359,8 +361,10 @@ mdx_status setup_kspace(mdx_engine *h) {
     CHK(dalloc(h, &h->d_fq, h->n));
     CHK(dalloc(h, &h->d_stepz, h->n));
     if (h->tuned_nslice == 0) {
-        // about 2,048 structure-factor blocks (row groups x atom chunks)
-        const long long want = (long long)h->n * cdiv(std::max(1, h->nrows), 64) / 2048;
+        // about 2,048 structure-factor blocks (row groups x atom chunks), at
+        // most ~48 chunks per k vector for the reduction
+        const long long want = std::max((long long)h->n * cdiv(std::max(1, h->nrows), 64) / 2048,
+                                        (long long)h->n / 48);
         int c = 64;
         while (c < want && c < 1024) c *= 2;
         h->chunk = c;
@@ -1270,6 +1274,10 @@ void drop_graphs(mdx_batch *b) {
         b->exec[g] = nullptr;
         b->graph[g] = nullptr;
     }
+    if (b->md_exec) hipGraphExecDestroy(b->md_exec);
+    if (b->md_graph) hipGraphDestroy(b->md_graph);
+    b->md_exec = nullptr;
+    b->md_graph = nullptr;
 }
 
 // Argument arrays of every kernel from the systems' current structs.  With
@@ -1443,9 +1451,10 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
     mark(9);
 }
 
-// Queue one force call for every system of the batch on the batch stream
-// (no host synchronisation).  events: eager launches with stage events.
-mdx_status batch_launch(mdx_batch *b, int group, bool detail, bool events) {
+// Checks and plan refresh before a force call (argument structs of systems
+// whose lists or box changed are uploaded again; captured graphs are dropped
+// when a launch shape changed).
+mdx_status batch_prepare(mdx_batch *b, int group, bool detail) {
     if (group < 0 || group > 2) return MDX_ERR_ARG;
     if (detail)
         for (mdx_engine *h : b->sys) {
@@ -1475,6 +1484,13 @@ mdx_status batch_launch(mdx_batch *b, int group, bool detail, bool events) {
                                   b->zero.nblocks(), b->maxlen, b->mz, b->fpk.nblocks(), b->nmt_max, b->nlj_max};
         if (before != after) drop_graphs(b);
     }
+    return MDX_OK;
+}
+
+// Queue one force call for every system of the batch on the batch stream
+// (no host synchronisation).  events: eager launches with stage events.
+mdx_status batch_launch(mdx_batch *b, int group, bool detail, bool events) {
+    CHK(batch_prepare(b, group, detail));
     if (b->use_graph && !events && !detail) {
         if (!b->exec[group]) {
             HIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1595,15 +1611,26 @@ mdx_status md_plan(mdx_batch *b, int nsteps, bool want_trace) {
         mdx_engine *h = b->sys[s];
         const MdArgs a = md_args(b, s, want_trace ? b->d_trace + (size_t)s * nsteps * kTrace : nullptr, nsteps);
         b->md_kd.add(a, cdiv(h->n, 256));
-        b->md_sh.add(a, cdiv(h->nmol, 256));
+        b->md_sh.add(a, cdiv(h->nmol, 64));
         b->md_ke.add(a, cdiv(h->n, 256));
         b->md_th.add(a, 1);
         b->md_gc.add(a, cdiv(h->nmol, 256));
         b->md_tr.add(a, 1);
     }
+    const void *before[6] = {b->md_kd.d_args, b->md_sh.d_args, b->md_ke.d_args, b->md_th.d_args, b->md_gc.d_args,
+                             b->md_tr.d_args};
     CHK(kplan_upload(b, b->md_kd, true)); CHK(kplan_upload(b, b->md_sh, true));
     CHK(kplan_upload(b, b->md_ke, true)); CHK(kplan_upload(b, b->md_th, true));
     CHK(kplan_upload(b, b->md_gc, true)); CHK(kplan_upload(b, b->md_tr, true));
+    const void *after[6] = {b->md_kd.d_args, b->md_sh.d_args, b->md_ke.d_args, b->md_th.d_args, b->md_gc.d_args,
+                            b->md_tr.d_args};
+    for (int k = 0; k < 6; ++k)
+        if (before[k] != after[k] && b->md_exec) {
+            hipGraphExecDestroy(b->md_exec);
+            hipGraphDestroy(b->md_graph);
+            b->md_exec = nullptr;
+            b->md_graph = nullptr;
+        }
     return MDX_OK;
 }
 
@@ -1623,12 +1650,33 @@ mdx_status batch_md_run(mdx_batch *b, int nsteps, double *trace) {
     for (mdx_engine *h : b->sys)
         HIPCHK(b, hipMemsetAsync(h->d_mdstate + MD_STEP, 0, sizeof(double), b->stream));
     hipStream_t st = b->stream;
-    for (int step = 0; step < nsteps; ++step) {
+    auto md_kernels = [&]() {
         launch(b, b->md_kd, 256, 0, k_md_kick_drift);
-        launch(b, b->md_sh, 256, 0, k_md_shake);
+        launch(b, b->md_sh, 64, 0, k_md_shake);
         launch(b, b->md_ke, 256, 0, k_md_scale_ke);
         launch(b, b->md_th, 256, 0, k_md_thermo);
         launch(b, b->md_gc, 256, 0, k_md_getcom);
+    };
+    for (int step = 0; step < nsteps; ++step) {
+        bool rebuild = false;
+        for (mdx_engine *h : b->sys)
+            rebuild = rebuild || (h->md.rebuild_every > 0 && (h->mstep + 1) % h->md.rebuild_every == 0);
+        if (!rebuild && b->use_graph) {
+            // the whole step (integration, force call, trace) as one graph
+            CHK(batch_prepare(b, MDX_FORCES_ALL, false));
+            if (!b->md_exec) {
+                HIPCHK(b, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                md_kernels();
+                launch_call(b, MDX_FORCES_ALL, false, false);
+                launch(b, b->md_tr, 64, 0, k_md_trace);
+                HIPCHK(b, hipStreamEndCapture(st, &b->md_graph));
+                HIPCHK(b, hipGraphInstantiate(&b->md_exec, b->md_graph, nullptr, nullptr, 0));
+            }
+            HIPCHK(b, hipGraphLaunch(b->md_exec, st));
+            for (mdx_engine *h : b->sys) h->mstep += 1;
+            continue;
+        }
+        md_kernels();
         bool synced = false;
         for (mdx_engine *h : b->sys) {
             h->mstep += 1;

@@ -33,9 +33,11 @@ __device__ inline void md_kick_drift_body(const MdArgs &a, int blk) {
 // One molecule per thread, the reference's iteration: every constraint
 // whose atoms moved in the previous sweep is corrected along the old bond
 // vector until |r^2 - RB^2| <= RB^2 TOLER^2 for all of them.  The molecule's
-// coordinates live in registers (up to kShakeRegAtoms atoms, e.g. water) or
-// are read and written in HBM (larger molecules).
+// coordinates and constraints live in registers (up to kShakeRegAtoms atoms
+// and kShakeRegCons constraints, e.g. water) or are read and written in HBM
+// (larger molecules).
 constexpr int kShakeRegAtoms = 4;
+constexpr int kShakeRegCons = 6;
 
 template <int NA>
 __device__ inline void md_shake_regs(const MdArgs &a, int c0, int c1, int a0) {
@@ -48,17 +50,33 @@ __device__ inline void md_shake_regs(const MdArgs &a, int c0, int c1, int a0) {
         ox[k] = a.ox[i]; oy[k] = a.oy[i]; oz[k] = a.oz[i];
         mi[k] = a.minv[i];
     }
+    // constraints of the molecule in registers (at most kShakeRegCons)
+    const int nc = c1 - c0;
+    int ci[kShakeRegCons], cj[kShakeRegCons];
+    double cl[kShakeRegCons];
     unsigned moved = 0u, moving = 0u;
-    for (int c = c0; c < c1; ++c) moved |= (1u << (a.cons[c].x - a0)) | (1u << (a.cons[c].y - a0));
+#pragma unroll
+    for (int c = 0; c < kShakeRegCons; ++c) {
+        ci[c] = 0; cj[c] = 0; cl[c] = 1.0;
+        if (c < nc) {
+            const int2 p = a.cons[c0 + c];
+            ci[c] = p.x - a0;
+            cj[c] = p.y - a0;
+            cl[c] = a.cons_len2[c0 + c];
+            moved |= (1u << ci[c]) | (1u << cj[c]);
+        }
+    }
     bool done = false;
     int iter = 0;
     while (!done && iter <= a.maxit) {
         done = true;
-        for (int c = c0; c < c1; ++c) {
-            const int i = a.cons[c].x - a0, j = a.cons[c].y - a0;
+#pragma unroll
+        for (int c = 0; c < kShakeRegCons; ++c) {
+            if (c >= nc) break;
+            const int i = ci[c], j = cj[c];
             const unsigned bi = 1u << i, bj = 1u << j;
             if (!((moved & bi) || (moved & bj))) continue;
-            const double bndsq = a.cons_len2[c];
+            const double bndsq = cl[c];
             double bx = x[i] - x[j], by = y[i] - y[j], bz = z[i] - z[j];
             min_image(bx, by, bz, a.box);
             double bsq = bx * bx + by * by;
@@ -96,16 +114,18 @@ __device__ inline void md_shake_regs(const MdArgs &a, int c0, int c1, int a0) {
 
 __device__ inline void md_shake_body(const MdArgs &a, int blk) {
 #pragma clang fp contract(off)
-    const int m = blk * 256 + threadIdx.x;
+    const int m = blk * 64 + threadIdx.x;
     if (m >= a.nmol) return;
     const int c0 = a.cstart[m], c1 = a.cstart[m + 1];
     if (c0 == c1) return;
     const int a0 = a.mstart[m], a1 = a.mstart[m + 1];
-    switch (a1 - a0) {
-        case 2: md_shake_regs<2>(a, c0, c1, a0); return;
-        case 3: md_shake_regs<3>(a, c0, c1, a0); return;
-        case 4: md_shake_regs<4>(a, c0, c1, a0); return;
-        default: break;
+    if (c1 - c0 <= kShakeRegCons) {
+        switch (a1 - a0) {
+            case 2: md_shake_regs<2>(a, c0, c1, a0); return;
+            case 3: md_shake_regs<3>(a, c0, c1, a0); return;
+            case 4: md_shake_regs<4>(a, c0, c1, a0); return;
+            default: break;
+        }
     }
     unsigned long long moved = 0ull, moving = 0ull;   // bit (atom - a0)
     for (int c = c0; c < c1; ++c) {
@@ -235,7 +255,7 @@ __global__ __launch_bounds__(256) void k_md_kick_drift(const MdArgs *__restrict_
     md_kick_drift_body(args[m.x], m.y);
 }
 
-__global__ __launch_bounds__(256) void k_md_shake(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+__global__ __launch_bounds__(64) void k_md_shake(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     md_shake_body(args[m.x], m.y);
 }
