@@ -208,3 +208,37 @@ def test_autotuned_shape_keeps_parity():
     f, g, h, e = eng.allforce()
     assert rms_rel(f, golden_forces(d, "F", 0)) <= FORCE_RMS_TOL
     check_energies(e, dict(zip(abi.MdxEnergies.SCALARS, d["ENERGY"][0])))
+
+
+def test_all_bonded_kinds_vs_oracle():
+    """Morse bonds and cosine, Ryckaert-Bellemans and improper torsions
+    (the oracle's forms are pinned by tests/test_oracle_math.py) on a box of
+    charged 6-site chains, all three force groups."""
+    rng = np.random.default_rng(8)
+    geo = systems._chain_geometry(6, rng, 1.53, 112.0)
+    q = [0.2, -0.1, 0.0, 0.1, -0.3, 0.1]
+    sites = [systems.Site("C%d" % (i + 1), tuple(geo[i]), 14.0, q[i], 3.6, 0.4) for i in range(6)]
+    bonds = [(1 if i == 2 else 0, i + 1, i + 2, 1.53 + 0.01 * i, 1200.0 + 10 * i, 300.0, 1.9) for i in range(5)]
+    angles = [(i + 1, i + 2, i + 3, 112.0 + i, 250.0 + 5 * i) for i in range(4)]
+    tors = [(i + 1, i + 2, i + 3, i + 4, 15.0 * (i + 1), 2.0 + i, 3) for i in range(3)]
+    mt = systems.MolType("CH6", sites, bonds, angles, tors, c14el=0.5, c14lj=0.5)
+    centres = systems.lattice_sites(27, (24.0, 24.0, 24.0), rng)
+    s = systems.assemble("kinds", [mt], [27], centres, rng, box=(24.0, 24.0, 24.0), rcut=9.0, rshort=4.5,
+                         ewald_a=2.8, ewald_b=9.0, shake=False)
+    topo, prm = topology.from_system(s)
+    assert len(topo.tp_kind) >= 3 and len(topo.bp_morse) >= 3
+    topo.bp_morse[:] = 0
+    topo.bp_morse[2] = 1
+    topo.tp_kind[:3] = [0, 5, -1]
+    topo.tp_k1[1], topo.tp_k2[1], topo.tp_k3[1], topo.tp_k4[1], topo.tp_k5[1] = 3.0, -2.0, 1.5, -0.5, 0.25
+    eng = ForceEngine(topo, prm)
+    eng.set_positions(s.pos)
+    eng.chcnb()
+    print("case bonded kinds", topo.natoms, flush=True)
+    for group in (abi.MDX_FORCES_ALL, abi.MDX_FORCES_FAST, abi.MDX_FORCES_SLOW):
+        f, g, h, e = eng.compute(group)
+        fo, go, ho, eo = _oracle(topo, prm, s.pos, group)
+        assert rms_rel(f, fo) <= FORCE_RMS_TOL, group
+        check_energies(e, eo)
+        if group != abi.MDX_FORCES_SLOW:
+            assert abs(e["pint"] - eo["pint"]) <= 1e-9 * max(abs(eo["pint"]), 1.0)
