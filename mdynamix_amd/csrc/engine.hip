@@ -111,7 +111,7 @@ struct mdx_engine {
     int2 *d_aux = nullptr;
     bool have_list = false;
     // reciprocal space
-    int nkv = 0, nrows = 0, maxlen = 0, chunk = 512, nchunk = 0, nslice = 1, rows_per_slice = 0;
+    int nkv = 0, nrows = 0, chunk = 512, nchunk = 0, nslice = 1, rows_per_slice = 0;
     int nfrows = 0, npk = 0, mz = 0;
     int tuned_nslice = 0;       // 0: heuristic
     FRow *d_frows = nullptr;
@@ -121,14 +121,15 @@ struct mdx_engine {
     double kbox[3] = {0, 0, 0};
     double spei = 0.0;
     std::vector<double> spee_self;
-    RecipRow *d_rows = nullptr;
-    int *d_rowoff = nullptr;
-    int4 *d_kidx = nullptr;
+    FRow *d_srows = nullptr;
+    float2 *d_tx = nullptr, *d_ty = nullptr, *d_tz = nullptr;
+    int kx = 0, ky = 0;
+    float4 *d_ppart = nullptr;
+    float2 *d_s0part = nullptr;
+    int mzp = 8, nsgroups = 1;
     double4 *d_fq = nullptr;
     double4 *d_stepz = nullptr;
-    float2 *d_part = nullptr, *d_ak = nullptr;
-    int *d_wavelen = nullptr;
-    double2 *d_sk = nullptr;
+
     double *d_rpart = nullptr;
     std::vector<int4> h_kidx;   // (nx, ny, nz) in INIFUR order
     // forces
@@ -186,13 +187,13 @@ struct mdx_batch {
     double beadfac = 0.0;
     bool use_graph = true;
     bool planned = false;
-    int maxlen = 16, nlj_max = 1;
+    int nlj_max = 1;
     bool soft = false;
     KPlan<ZeroSeg> zero;
     KPlan<PackArgs> pack;
     KPlan<PairArgs> pairs;
     KPlan<SpecialArgs> special;
-    KPlan<RecipArgs> rprep, sk, skred, fpk, rforce;
+    KPlan<RecipArgs> rprep, sk, skred, rforce;
     int mz = 1;
     KPlan<EtermArgs> et;
     KPlan<BondArgs> bonds;
@@ -296,6 +297,17 @@ void enumerate_k(const mdx_params &p, std::vector<int4> &out) {
         }
 }
 
+// Atom chunks of the structure-factor pass: a multiple of 8 chunks (the
+// blocks of one chunk share an XCD) of a multiple of kSkSub atoms, about
+// `want` atoms each; the last chunks may be short or empty.
+mdx_status set_chunks(mdx_engine *h, int want) {
+    h->nchunk = cdiv(cdiv(h->n, std::max(kSkSub, want)), 8) * 8;
+    h->chunk = cdiv(cdiv(h->n, h->nchunk), kSkSub) * kSkSub;
+    CHK(dalloc(h, &h->d_ppart, (size_t)h->nchunk * std::max(1, h->npk)));
+    CHK(dalloc(h, &h->d_s0part, (size_t)h->nchunk * std::max(1, h->nfrows)));
+    return MDX_OK;
+}
+
 mdx_status setup_kspace(mdx_engine *h) {
     std::vector<int4> ks;
     enumerate_k(h->p, ks);
@@ -326,53 +338,19 @@ mdx_status setup_kspace(mdx_engine *h) {
         r.len = kv.second.second - kv.second.first + 1;
         rows.push_back(r);
     }
-    std::stable_sort(rows.begin(), rows.end(), [](const RecipRow &a, const RecipRow &b) { return a.len > b.len; });
     long long total = 0;
     std::vector<int> off(rows.size());
-    std::vector<int4> kidx;
-    int maxlen = 0;
     for (size_t r = 0; r < rows.size(); ++r) {
         off[r] = (int)total;
-        for (int j = 0; j < rows[r].len; ++j) kidx.push_back(make_int4(rows[r].nx, rows[r].ny, rows[r].nz0 + j, 0));
         total += rows[r].len;
-        maxlen = std::max(maxlen, rows[r].len);
     }
     if (total != h->nkv) {
         h->err = "k-vector rows are not contiguous";
         return MDX_ERR_NKV;
     }
-    if (maxlen > 96) {
-        h->err = "k-vector rows longer than 96 (kmax > 47) are not supported";
-        return MDX_ERR_NKV;
-    }
     h->nrows = (int)rows.size();
-    h->maxlen = maxlen;
-    // longest row of each wavefront of 64 rows (rows are sorted by length)
-    std::vector<int> wl((rows.size() + 63) / 64, 0);
-    for (size_t r = 0; r < rows.size(); ++r) wl[r / 64] = std::max(wl[r / 64], rows[r].len);
-    CHK(dalloc(h, &h->d_wavelen, wl.size()));
-    if (!h->dry) HIPCHK(h, hipMemcpy(h->d_wavelen, wl.data(), sizeof(int) * wl.size(), hipMemcpyHostToDevice));
-    CHK(dalloc(h, &h->d_rows, rows.size()));
-    CHK(dalloc(h, &h->d_rowoff, rows.size()));
-    CHK(dalloc(h, &h->d_kidx, kidx.size()));
-    if (!h->dry) HIPCHK(h, hipMemcpy(h->d_rows, rows.data(), sizeof(RecipRow) * rows.size(), hipMemcpyHostToDevice));
-    if (!h->dry) HIPCHK(h, hipMemcpy(h->d_rowoff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
-    if (!h->dry) HIPCHK(h, hipMemcpy(h->d_kidx, kidx.data(), sizeof(int4) * kidx.size(), hipMemcpyHostToDevice));
     CHK(dalloc(h, &h->d_fq, h->n));
     CHK(dalloc(h, &h->d_stepz, h->n));
-    if (h->tuned_nslice == 0) {
-        // about 2,048 structure-factor blocks (row groups x atom chunks), at
-        // most ~48 chunks per k vector for the reduction
-        const long long want = std::max((long long)h->n * cdiv(std::max(1, h->nrows), 64) / 2048,
-                                        (long long)h->n / 48);
-        int c = 64;
-        while (c < want && c < 1024) c *= 2;
-        h->chunk = c;
-    }
-    h->nchunk = cdiv(h->n, h->chunk);
-    CHK(dalloc(h, &h->d_part, (size_t)h->nchunk * std::max(1, h->nkv)));
-    CHK(dalloc(h, &h->d_ak, std::max(1, h->nkv)));
-    CHK(dalloc(h, &h->d_sk, std::max(1, h->nkv)));
     // force pass: rows in (n_x, n_y) order, +-n_z pairs (indices into the
     // row-ordered k arrays above)
     {
@@ -399,7 +377,7 @@ mdx_status setup_kspace(mdx_engine *h) {
             for (int j = 1; j <= (m + 3) / 4 * 4; ++j) {
                 const int kp = (j <= m && lo <= j && j <= hi) ? off[r] + j - lo : -1;
                 const int km = (j <= m && lo <= -j && -j <= hi) ? off[r] - j - lo : -1;
-                pk.push_back(make_int4(kp, km, j, 0));
+                pk.push_back(make_int4(kp, km, j, (int)fr.size()));
             }
             mz = std::max(mz, m);
             fr.push_back(f);
@@ -411,6 +389,24 @@ mdx_status setup_kspace(mdx_engine *h) {
         h->nfrows = (int)fr.size();
         h->npk = (int)pk.size();
         h->mz = mz;
+        // phase tables: n_z = 0..mzp-1 (whole N tiles of 8), n_x = 0..kx, |n_y| = 0..ky
+        h->mzp = (mz + 1 + 7) / 8 * 8;
+        h->kx = h->ky = 0;
+        for (const FRow &f : fr) {
+            h->kx = std::max(h->kx, std::abs(f.ix));
+            h->ky = std::max(h->ky, std::abs(f.iy));
+        }
+        // structure-factor rows: the same rows sorted by pair count (longest
+        // first); pad0 is the row's index in the force-pass order
+        std::vector<FRow> sr(fr);
+        for (size_t r = 0; r < sr.size(); ++r) sr[r].pad0 = (int)r;
+        std::stable_sort(sr.begin(), sr.end(), [](const FRow &a, const FRow &b) { return a.m > b.m; });
+        h->nsgroups = cdiv(std::max<size_t>(1, sr.size()), kSkRows);
+        CHK(dalloc(h, &h->d_srows, sr.size()));
+        if (!h->dry) HIPCHK(h, hipMemcpy(h->d_srows, sr.data(), sizeof(FRow) * sr.size(), hipMemcpyHostToDevice));
+        CHK(dalloc(h, &h->d_tx, (size_t)h->n * (h->kx + 1)));
+        CHK(dalloc(h, &h->d_ty, (size_t)h->n * (h->ky + 1)));
+        CHK(dalloc(h, &h->d_tz, (size_t)h->n * h->mzp));
         CHK(dalloc(h, &h->d_frows, fr.size()));
         CHK(dalloc(h, &h->d_fpk, std::max<size_t>(1, pk.size())));
         CHK(dalloc(h, &h->d_fpack, pk.size() + 8));   // + two spare blocks
@@ -419,6 +415,8 @@ mdx_status setup_kspace(mdx_engine *h) {
         if (!h->dry && !pk.empty())
             HIPCHK(h, hipMemcpy(h->d_fpk, pk.data(), sizeof(int4) * pk.size(), hipMemcpyHostToDevice));
     }
+    // about 2,048 structure-factor blocks (row groups x atom chunks) unless tuned
+    CHK(set_chunks(h, h->tuned_nslice > 0 ? h->chunk : std::max(256, cdiv(h->n * h->nsgroups, 2048))));
     // force pass: slices of rows so the grid fills the chip (about 6,000
     // wavefronts: one and a half rounds at four per SIMD) unless tuned
     const int wav = cdiv(h->n, 64);
@@ -515,7 +513,7 @@ void layout_rows(mdx_engine *h) {
     int r = 0;
     h->row_pairs = r; r += h->nclu_cap;
     h->row_sp = r; r += h->nspecial ? cdiv(h->nspecial, 256) : 0;
-    h->row_sk = r; r += cdiv(std::max(1, h->nkv), 256);
+    h->row_sk = r; r += cdiv(h->npk + h->nfrows, 256);
     h->row_et = r; r += h->neterm ? cdiv(h->neterm, 256) : 0;
     h->row_b = r; r += h->nb ? cdiv(h->nb, 256) : 0;
     h->row_a = r; r += h->na ? cdiv(h->na, 256) : 0;
@@ -533,9 +531,7 @@ void fill_args(mdx_engine *h);
 // the force pass (both change only the work split, not the sums' content
 // or order within a k vector and atom).
 mdx_status set_kspace_shape(mdx_engine *h, int chunk, int nslice) {
-    h->chunk = std::max(64, chunk);
-    h->nchunk = cdiv(h->n, h->chunk);
-    CHK(dalloc(h, &h->d_part, (size_t)h->nchunk * std::max(1, h->nkv)));
+    CHK(set_chunks(h, chunk));
     h->tuned_nslice = std::max(1, std::min(64, nslice));
     h->nslice = h->tuned_nslice;
     h->rows_per_slice = cdiv(std::max(1, h->nfrows), h->nslice);
@@ -660,20 +656,21 @@ void fill_args(mdx_engine *h) {
     ra.box = h->box;
     ra.fq = h->d_fq;
     ra.stepz = h->d_stepz;
-    ra.nrows = h->nrows;
-    ra.nrowgroups = cdiv(std::max(1, h->nrows), 64);
+    ra.tx = h->d_tx;
+    ra.ty = h->d_ty;
+    ra.tz = h->d_tz;
+    ra.kx = h->kx;
+    ra.ky = h->ky;
+    ra.mzp = h->mzp;
     ra.chunk = h->chunk;
     ra.nchunk = h->nchunk;
     ra.nkv = h->nkv;
-    ra.rows = h->d_rows;
-    ra.rowoff = h->d_rowoff;
-    ra.wavelen = h->d_wavelen;
-    ra.part = h->d_part;
-    ra.kidx = h->d_kidx;
     ra.alpha = h->p.alpha;
     ra.coulf = h->p.coulomb_factor;
-    ra.ak = h->d_ak;
-    ra.sk = h->d_sk;
+    ra.nsgroups = h->nsgroups;
+    ra.srows = h->d_srows;
+    ra.ppart = h->d_ppart;
+    ra.s0part = h->d_s0part;
     ra.nslice = h->nslice;
     ra.rows_per_slice = h->rows_per_slice;
     ra.nfrows = h->nfrows;
@@ -1285,18 +1282,16 @@ void drop_graphs(mdx_batch *b) {
 mdx_status plan_fill(mdx_batch *b, bool maps) {
     const int K = (int)b->sys.size();
     b->zero.clear(); b->pack.clear(); b->pairs.clear(); b->special.clear();
-    b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->fpk.clear(); b->rforce.clear();
+    b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
     b->mz = 1;
     b->et.clear(); b->bonds.clear(); b->angles.clear(); b->tors.clear();
     b->springs.clear(); b->comb.clear(); b->red.clear(); b->dred.clear();
     b->nmt_max = 1;
-    b->maxlen = 16;
     b->nlj_max = 1;
     b->soft = false;
     for (int s = 0; s < K; ++s) {
         mdx_engine *h = b->sys[s];
         const int n = h->n;
-        b->maxlen = std::max(b->maxlen, h->maxlen);
         b->nmt_max = std::max(b->nmt_max, h->ntypes);
         if (h->d_dpart)
             b->dred.add(DetailArgs{h->det_rows, h->ndet, h->d_dpart, b->d_det + (size_t)s * kMaxDet}, 1);
@@ -1310,9 +1305,8 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         b->pairs.add(h->a_pair, h->nclu_cap);
         if (h->nspecial) b->special.add(h->a_special, cdiv(h->nspecial, 256));
         b->rprep.add(h->a_recip, cdiv(n, 256));
-        b->sk.add(h->a_recip, h->a_recip.nrowgroups * h->nchunk);
-        b->skred.add(h->a_recip, cdiv(std::max(1, h->nkv), 256));
-        b->fpk.add(h->a_recip, cdiv(h->npk + h->nfrows, 256));
+        b->sk.add(h->a_recip, h->nsgroups * h->nchunk);
+        b->skred.add(h->a_recip, cdiv(h->npk + h->nfrows, 256));
         b->rforce.add(h->a_recip, h->a_recip.natom_blocks * h->nslice);
         b->mz = std::max(b->mz, h->mz);
         if (h->neterm) b->et.add(h->a_et, cdiv(h->neterm, 256));
@@ -1361,15 +1355,10 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         b->comb.add(ca, cdiv(n, 256));
         b->red.add(ReduceArgs{h->rows_total, h->d_epart, b->d_energy + (size_t)s * Q_N}, Q_N);
     }
-    if (b->maxlen > 96) {
-        b->err = "k-vector rows longer than 96 are not supported";
-        return MDX_ERR_NKV;
-    }
     CHK(kplan_upload(b, b->zero, maps)); CHK(kplan_upload(b, b->pack, maps));
     CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->special, maps));
     CHK(kplan_upload(b, b->rprep, maps)); CHK(kplan_upload(b, b->sk, maps));
     CHK(kplan_upload(b, b->skred, maps)); CHK(kplan_upload(b, b->rforce, maps));
-    CHK(kplan_upload(b, b->fpk, maps));
     CHK(kplan_upload(b, b->et, maps)); CHK(kplan_upload(b, b->bonds, maps));
     CHK(kplan_upload(b, b->angles, maps)); CHK(kplan_upload(b, b->tors, maps));
     CHK(kplan_upload(b, b->springs, maps)); CHK(kplan_upload(b, b->comb, maps));
@@ -1418,13 +1407,11 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
     if (group != MDX_FORCES_FAST) {
         launch(b, b->rprep, 256, 0, k_recip_prep);
         mark(4);
-        if (b->maxlen <= 16) launch(b, b->sk, 64, 0, k_sk<16>);
-        else if (b->maxlen <= 32) launch(b, b->sk, 64, 0, k_sk<32>);
-        else if (b->maxlen <= 64) launch(b, b->sk, 64, 0, k_sk<64>);
-        else launch(b, b->sk, 64, 0, k_sk<96>);
+        if (b->mz <= 16) launch(b, b->sk, 256, 0, k_sk<3>);
+        else if (b->mz <= 32) launch(b, b->sk, 256, 0, k_sk<5>);
+        else launch(b, b->sk, 256, 0, k_sk<7>);
         mark(5);
         launch(b, b->skred, 256, 0, k_sk_reduce);
-        launch(b, b->fpk, 256, 0, k_fpack);
         mark(6);
         if (b->mz <= 16) launch(b, b->rforce, kForceBlock, 0, k_recip_force<16>);
         else if (b->mz <= 32) launch(b, b->rforce, kForceBlock, 0, k_recip_force<32>);
@@ -1478,10 +1465,10 @@ mdx_status batch_prepare(mdx_batch *b, int group, bool detail) {
         // launch shapes depend on capacities only; a new box (k set) or a
         // regrown capacity changes the maps and the captured graphs
         std::vector<int> before = {b->pairs.nblocks(), b->sk.nblocks(), b->rforce.nblocks(), b->skred.nblocks(),
-                                   b->zero.nblocks(), b->maxlen, b->mz, b->fpk.nblocks(), b->nmt_max, b->nlj_max};
+                                   b->zero.nblocks(), b->mz, b->nmt_max, b->nlj_max};
         CHK(plan_fill(b, true));
         std::vector<int> after = {b->pairs.nblocks(), b->sk.nblocks(), b->rforce.nblocks(), b->skred.nblocks(),
-                                  b->zero.nblocks(), b->maxlen, b->mz, b->fpk.nblocks(), b->nmt_max, b->nlj_max};
+                                  b->zero.nblocks(), b->mz, b->nmt_max, b->nlj_max};
         if (before != after) drop_graphs(b);
     }
     return MDX_OK;
@@ -1772,8 +1759,8 @@ mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) 
     const int wav = cdiv(h->n, 64);
     int best_chunk = h->chunk, best_slice = h->nslice;
     double best_sk = 1e30, best_force = 1e30, sk = 0.0, fo = 0.0;
-    for (int chunk : {256, 512, 1024, 2048}) {
-        if (chunk > 2 * h->n && chunk != 256) continue;
+    for (int chunk : {512, 1024, 2048, 4096}) {
+        if (chunk > 2 * h->n && chunk != 512) continue;
         CHK(trial(chunk, best_slice, sk, fo));
         if (sk < best_sk) { best_sk = sk; best_chunk = chunk; }
     }
@@ -1974,7 +1961,7 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     drop_graphs(b);
     kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->pairs); kplan_free(b->special);
-    kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce); kplan_free(b->fpk);
+    kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
     kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->angles); kplan_free(b->tors);
     kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);
     kplan_free(b->md_kd); kplan_free(b->md_sh); kplan_free(b->md_ke); kplan_free(b->md_th);

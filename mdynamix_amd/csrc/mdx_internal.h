@@ -44,6 +44,8 @@ constexpr int kClu = 8;            // atoms per cluster
 constexpr int kMaxDet = 2 * 137 + 2 * 16;
 constexpr int kWave = 64;          // wavefront size on gfx950
 constexpr int kForceBlock = 256;   // atoms per block of the reciprocal force pass
+constexpr int kSkRows = 64;        // k rows per structure-factor block (16 per wavefront)
+constexpr int kSkSub = 32;         // atoms per LDS stage of the structure-factor block
 
 // Energy/virial slots of the per-block partial sums.
 enum EQ {
@@ -213,22 +215,25 @@ struct RecipArgs {
     Box box;
     double4 *fq;                 // fractional coordinates and charge
     double4 *stepz;              // exp(2 pi i f_z), exp(16 pi i f_z)
-    int nrows, nrowgroups, chunk, nchunk, nkv;
-    const RecipRow *rows;
-    const int *rowoff;
-    const int *wavelen;
-    float2 *part;                // [nchunk][nkv]
-    const int4 *kidx;
+    // phase tables, [index][natoms] (FP64 recurrences rounded to FP32):
+    // tx = q exp(2 pi i n_x f_x), n_x = 0..kx; ty = exp(2 pi i n_y f_y),
+    // n_y = 0..ky; tz = exp(2 pi i j f_z), j = 0..mzp-1
+    float2 *tx, *ty, *tz;
+    int kx, ky, mzp;
+    int chunk, nchunk, nkv;
     double alpha, coulf;
-    float2 *ak;                  // 2 AKC S(k)
-    double2 *sk;
     int nslice, rows_per_slice, natom_blocks;
     double *rpart;               // [nslice][3][natoms]
     double *epart;               // rows of k_sk_reduce
-    // force pass over (n_x, n_y) rows with +-n_z pairs
-    int nfrows, npk;
+    // (n_x, n_y) rows with +-n_z pairs: frows in (n_x, n_y) order (force
+    // pass), srows the same rows sorted by pair count (structure factors,
+    // groups of 64)
+    int nfrows, npk, nsgroups;
     const FRow *frows;
-    const int4 *fpk;             // per pair entry: k(+j), k(-j), j
+    const FRow *srows;           // pad0: index of the row in frows
+    const int4 *fpk;             // per pair entry: k(+j), k(-j), j, frow
+    float4 *ppart;               // [nchunk][npk]: partial U_j, V_j (nchunk a multiple of 8)
+    float2 *s0part;              // [nchunk][nfrows]: partial S(n_z = 0)
     float4 *fpack;               // per pair entry: D = (A+r - A-r, A+r + A-r, A+i + A-i, A+i - A-i)
     float2 *fa0;                 // per row: A of n_z = 0
 };

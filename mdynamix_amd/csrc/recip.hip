@@ -1,21 +1,17 @@
 // Ewald reciprocal sum (replaces FURIR, forces.f:1217-1339).
 //
 // The half-space k set of INIFUR (forces.f:1356-1427) is regrouped into
-// rows of fixed (n_x, n_y) whose n_z run over a contiguous range; rows are
-// sorted by length so the 64 rows of one wavefront have similar lengths.
-//
-// k_sk: lane = row, atoms of one chunk streamed with uniform (scalar) loads.
-//   For each atom the phase 2 pi (n_x f_x + n_y f_y + n_z0 f_z) is formed
-//   in FP64 fractional coordinates and reduced to one turn before sincospi;
-//   the remaining terms of the row follow by the recurrence
-//   P_{j+1} = 2 cos(theta) P_j - P_{j-1}, theta = 2 pi f_z, re-anchored in
-//   FP64 every 8 terms.  Per-chunk FP32 sums are written as
-//   partials and added in FP64 by k_sk_reduce, which also forms the energy
-//   QPE, the virial projections and A_k = 2 AKC S(k) for the force pass.
-// k_recip_force: lane = atom, rows streamed uniformly; the force on an atom
-//   is q sum_k k Im(exp(i k.r) conj(A_k)) (forces.f:1320-1326), evaluated
-//   for the k vectors +-n_z together on a per-atom table of exp(2 pi i j f_z)
-//   (see recip_force_body).
+// rows of fixed (n_x, n_y); within a row the vectors n_z = +j and -j are
+// taken together (the k set is symmetric in n_z except the (0, 0) row).
+// k_recip_prep: per-atom phase tables along x, y and z (FP64 recurrences);
+// k_sk: S(+-j) = U_j +- i V_j, with U_j, V_j sums of B p_j over the atoms
+//   of a chunk (B = q exp(2 pi i (n_x f_x + n_y f_y)), p_j = exp(2 pi i j
+//   f_z)), as a matrix product on the matrix cores;
+// k_sk_reduce: FP64 sums of the chunk partials, energy, virial, and the
+//   per-pair data of the force pass;
+// k_recip_force: lane = atom, rows uniform, the force
+//   q sum_k k Im(exp(i k.r) conj(A_k)) (forces.f:1320-1326) evaluated for
+//   +-j together (see recip_force_body).
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -24,7 +20,8 @@ __device__ inline void recip_prep_body(const RecipArgs &a, int blk) {
     const int i = blk * 256 + threadIdx.x;
     if (i >= a.natoms) return;
     const double fx = a.x[i] / a.box.l[0], fy = a.y[i] / a.box.l[1], fz = a.z[i] / a.box.l[2];
-    a.fq[i] = make_double4(fx, fy, fz, a.q[i]);
+    const double q = a.q[i];
+    stv(gst(a.fq) + i, make_double4(fx, fy, fz, q));
     // exp(2 pi i f_z) and its 8th power, in FP64
     const double u = fz - rint(fz);
     double s1, c1, s8, c8;
@@ -32,7 +29,43 @@ __device__ inline void recip_prep_body(const RecipArgs &a, int blk) {
     double u8 = 8.0 * u;
     u8 -= rint(u8);
     sincospi(2.0 * u8, &s8, &c8);
-    a.stepz[i] = make_double4(c1, s1, c8, s8);
+    stv(gst(a.stepz) + i, make_double4(c1, s1, c8, s8));
+    // phase tables (FP64 recurrences from FP64 sincospi steps, FP32 values)
+    const size_t n = (size_t)a.natoms;
+    {
+        double sr = 0.0, si = 0.0;
+        sincospi(2.0 * (fx - rint(fx)), &si, &sr);
+        double pr = q, pi = 0.0;
+        MDX_GLOBAL float2 *t = gst(a.tx) + i;
+        for (int k = 0; k <= a.kx; ++k) {
+            stv(t + k * n, make_float2((float)pr, (float)pi));
+            const double w = pr * sr - pi * si;
+            pi = pr * si + pi * sr;
+            pr = w;
+        }
+    }
+    {
+        double sr = 0.0, si = 0.0;
+        sincospi(2.0 * (fy - rint(fy)), &si, &sr);
+        double pr = 1.0, pi = 0.0;
+        MDX_GLOBAL float2 *t = gst(a.ty) + i;
+        for (int k = 0; k <= a.ky; ++k) {
+            stv(t + k * n, make_float2((float)pr, (float)pi));
+            const double w = pr * sr - pi * si;
+            pi = pr * si + pi * sr;
+            pr = w;
+        }
+    }
+    {
+        double pr = 1.0, pi = 0.0;
+        MDX_GLOBAL float2 *t = gst(a.tz) + i;
+        for (int k = 0; k < a.mzp; ++k) {
+            stv(t + k * n, make_float2((float)pr, (float)pi));
+            const double w = pr * c1 - pi * s1;
+            pi = pr * s1 + pi * c1;
+            pr = w;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_recip_prep(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
@@ -40,115 +73,208 @@ __global__ __launch_bounds__(256) void k_recip_prep(const RecipArgs *__restrict_
     recip_prep_body(args[m.x], m.y);
 }
 
-// exp(i 2 pi u) at the start of a row, u reduced to one turn in FP64.
-__device__ inline void row_start(const RecipRow &r, double4 a, double &c, double &s) {
-    double u = (double)r.nx * a.x + (double)r.ny * a.y;
-    u = u + (double)r.nz0 * a.z;
-    u = u - rint(u);
-    float sf, cf;
-    sincospif((float)(2.0 * u), &sf, &cf);
-    c = (double)cf;
-    s = (double)sf;
-}
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-// Structure factors of 64 rows over one atom chunk.  Along a row the
-// terms P_j = q exp(i(phi0 + j theta)) follow the three-term recurrence
-// P_{j+1} = 2 cos(theta) P_j - P_{j-1} (2 FMA per term), in blocks of 8
-// seeded from an FP64 anchor advanced by exp(8 i theta), so rounding
-// cannot build up over more than 7 steps.  The atom data of the chunk is
-// wavefront-uniform and read with scalar loads, one atom ahead.
-template <int MAXLEN>
-__device__ inline void sk_body(const RecipArgs &a, int blk) {
-    const int rg = blk % a.nrowgroups, ch = blk / a.nrowgroups;
-    const int r = rg * 64 + threadIdx.x;
+// Structure factors on the matrix cores.  For a row (n_x, n_y) with
+// B_a = q_a exp(2 pi i (n_x f_x + n_y f_y)) and p_j = exp(2 pi i j f_z) =
+// C_j + i D_j, the vectors n_z = +-j have S(+-j) = U_j +- i V_j with
+// U_j = sum_a B_a C_j(a), V_j = sum_a B_a D_j(a); n_z = 0 is U_0 (C_0 = 1).
+// Over the atoms of a chunk this is the real product
+//   [Re B; Im B] (rows x atoms)  x  [C_j, D_j] (atoms x j)
+// evaluated with v_mfma_f32_16x16x4_f32: an M tile is 8 rows x (Re, Im),
+// an N tile 8 values of j x (C, D), K = 4 atoms.  A block holds 64 rows
+// (sorted by pair count; a wavefront takes 16 rows = 2 M tiles and as many
+// N tiles as its first, longest row needs) and one chunk of atoms, staged
+// through LDS 32 atoms at a time: B = tx[n_x] ty[|n_y|] (conjugated for
+// n_y < 0) and p_j = tz[j].  Operand layout of the instruction (lane l):
+// A[l % 16][l / 16], B[l / 16][l % 16]; result D[4 (l / 16) + v][l % 16].
+// Each block writes its FP32 chunk sums per pair entry (U_j, V_j) and per
+// row (S(n_z = 0)); k_sk_reduce adds the chunks in FP64.
+// Blocks are numbered so that the row groups of one chunk share an XCD
+// (block index mod 8), whose L2 then serves that chunk's table columns.
+constexpr int kSkStride = kSkSub + 4;      // float2 per LDS row (bank spread)
+
+template <int NT>
+__device__ inline void sk_body(const RecipArgs &a, int blk, float2 *wl, float2 *zl) {
+    const int G = a.nsgroups;
+    const int x8 = blk & 7, s8 = blk >> 3;
+    const int g = s8 % G, ch = (s8 / G) * 8 + x8;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int a0 = ch * a.chunk;
     const int a1 = min(a0 + a.chunk, a.natoms);
-    const int wlen = sld(a.wavelen)[rg];          // longest row of this wavefront
-    const MDX_CONST double4 *fq = sld(a.fq);
-    const MDX_CONST double4 *stz = sld(a.stepz);
-    RecipRow row = {0, 0, 0, 0};
-    if (r < a.nrows) row = ldv(gld(a.rows) + r);
-    float are[MAXLEN], aim[MAXLEN];
+    const int rbase = g * kSkRows;
+    const size_t n = (size_t)a.natoms;
+    // staging assignment: atom tid % 32 of the stage, rows / j values tid / 32 + 8 i
+    const int sa = tid & 31, sr0 = tid >> 5;
+    int offx[8], offy[8];
+    bool conj[8];
 #pragma unroll
-    for (int j = 0; j < MAXLEN; ++j) { are[j] = 0.f; aim[j] = 0.f; }
-    double4 at = ldv(fq + a0), st = ldv(stz + a0);
-    for (int i = a0; i < a1; ++i) {
-        const double4 atc = at, stc = st;
-        if (i + 1 < a1) { at = ldv(fq + i + 1); st = ldv(stz + i + 1); }
-        double s0, c0;
-        row_start(row, atc, c0, s0);
-        double Ar = atc.w * c0, Ai = atc.w * s0;           // anchor, FP64
-        const float sr = (float)stc.x, si = (float)stc.y, c2 = 2.0f * sr;
+    for (int i = 0; i < 8; ++i) {
+        const int r = rbase + sr0 + 8 * i;
+        offx[i] = -1;
+        offy[i] = 0;
+        conj[i] = false;
+        if (r < a.nfrows) {
+            const FRow fr = ldv(gld(a.srows) + r);
+            offx[i] = fr.ix * a.natoms;
+            offy[i] = abs(fr.iy) * a.natoms;
+            conj[i] = fr.iy < 0;
+        }
+    }
+    // N tiles: the group's first row is its longest, a wavefront's first row is its longest
+    const int rw = rbase + 16 * w;
+    const int ntw = rw < a.nfrows ? min(NT, (ldv(gld(a.srows) + rw).m + 8) >> 3) : 0;
+    const int ntg = rbase < a.nfrows ? min(NT, (ldv(gld(a.srows) + rbase).m + 8) >> 3) : 0;
+    mfma_f4 acc[2][NT];
 #pragma unroll
-        for (int j0 = 0; j0 < MAXLEN; j0 += 8) {
-            if (j0 >= wlen) break;
-            float pr0 = (float)Ar, pi0 = (float)Ai;
-            float pr1 = pr0 * sr - pi0 * si, pi1 = pr0 * si + pi0 * sr;
-            are[j0] += pr0; aim[j0] += pi0;
-            are[j0 + 1] += pr1; aim[j0 + 1] += pi1;
+    for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-            for (int j = j0 + 2; j < j0 + 8; ++j) {
-                const float pr2 = fmaf(c2, pr1, -pr0), pi2 = fmaf(c2, pi1, -pi0);
-                are[j] += pr2; aim[j] += pi2;
-                pr0 = pr1; pi0 = pi1; pr1 = pr2; pi1 = pi2;
+        for (int t = 0; t < NT; ++t) acc[mt][t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    const MDX_GLOBAL float2 *tx = gld(a.tx), *ty = gld(a.ty), *tz = gld(a.tz);
+    const int rA = lane & 7, cA = (lane >> 3) & 1, kA = lane >> 4;
+    const float *wlf = reinterpret_cast<const float *>(wl);
+    const float *zlf = reinterpret_cast<const float *>(zl);
+    for (int sb = a0; sb < a1; sb += kSkSub) {
+        const int ia = sb + sa;
+        const bool ok = ia < a1;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            float2 b = make_float2(0.f, 0.f);
+            if (ok && offx[i] >= 0) {
+                const float2 X = ldv(tx + offx[i] + ia);
+                float2 Y = ldv(ty + offy[i] + ia);
+                if (conj[i]) Y.y = -Y.y;
+                b = make_float2(X.x * Y.x - X.y * Y.y, X.x * Y.y + X.y * Y.x);
             }
-            const double tr = Ar * stc.z - Ai * stc.w;
-            Ai = Ar * stc.w + Ai * stc.z;
-            Ar = tr;
+            wl[(sr0 + 8 * i) * kSkStride + sa] = b;
         }
-    }
-    if (r < a.nrows) {
-        const int off = gld(a.rowoff)[r];
-        MDX_GLOBAL float2 *o = gst(a.part) + (size_t)ch * a.nkv + off;
 #pragma unroll
-        for (int j = 0; j < MAXLEN; ++j)
-            if (j < row.len) stv(o + j, make_float2(are[j], aim[j]));
-    }
-}
-
-template <int MAXLEN>
-__global__ __launch_bounds__(64) void k_sk(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
-    const int2 m = map[blockIdx.x];
-    sk_body<MAXLEN>(args[m.x], m.y);
-}
-
-template __global__ void k_sk<16>(const RecipArgs *, const int2 *);
-template __global__ void k_sk<32>(const RecipArgs *, const int2 *);
-template __global__ void k_sk<64>(const RecipArgs *, const int2 *);
-template __global__ void k_sk<96>(const RecipArgs *, const int2 *);
-
-// Sum chunk partials in FP64 (fixed order), energy and virial terms of each
-// k (forces.f:1317-1331) and A_k = 2 AKC S(k).  kidx holds (nx, ny, nz) of
-// every k in row order.
-__device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)[256]) {
-    const int k = blk * 256 + threadIdx.x;
-    double e = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
-    if (k < a.nkv) {
-        double sr = 0.0, si = 0.0;
-        for (int c = 0; c < a.nchunk; ++c) {
-            const float2 p = a.part[(size_t)c * a.nkv + k];
-            sr += (double)p.x;
-            si += (double)p.y;
+        for (int i = 0; i < NT; ++i) {
+            const int j = sr0 + 8 * i;
+            float2 p = make_float2(0.f, 0.f);
+            if (ok && i < ntg) p = ldv(tz + (size_t)j * n + ia);
+            zl[j * kSkStride + sa] = p;
         }
-        const int4 id = a.kidx[k];
-        const double twopi = 2.0 * kPI;
-        const double rxv = twopi * id.x / a.box.l[0], ryv = twopi * id.y / a.box.l[1], rzv = twopi * id.z / a.box.l[2];
-        double rk2 = rxv * rxv + ryv * ryv;
-        rk2 = rk2 + rzv * rzv;
-        const double alp2 = a.alpha * a.alpha;
-        const double vol = a.box.l[0] * a.box.l[1] * a.box.l[2];
-        const double cfur = 4.0 * kPI * a.coulf / vol;
-        const double akc = cfur * exp(-0.25 * rk2 / alp2) / rk2;
-        const double s2 = sr * sr + si * si;
-        e = akc * s2;
-        const double akv = 0.5 * akc * (4.0 * alp2 + rk2) / (alp2 * rk2);
-        vx = e - rxv * rxv * akv * s2;
-        vy = e - ryv * ryv * akv * s2;
-        vz = e - rzv * rzv * akv * s2;
-        a.ak[k] = make_float2((float)(2.0 * akc * sr), (float)(2.0 * akc * si));
-        a.sk[k] = make_double2(sr, si);
+        __syncthreads();
+        if (ntw > 0) {
+#pragma unroll
+            for (int s = 0; s < kSkSub / 4; ++s) {
+                const int at = 4 * s + kA;
+                const float A0 = wlf[2 * ((16 * w + rA) * kSkStride + at) + cA];
+                const float A1 = wlf[2 * ((16 * w + 8 + rA) * kSkStride + at) + cA];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    if (t < ntw) {
+                        const float Bv = zlf[2 * ((8 * t + rA) * kSkStride + at) + cA];
+                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0, Bv, acc[0][t], 0, 0, 0);
+                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1, Bv, acc[1][t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();
     }
-    red[0][threadIdx.x] = e; red[1][threadIdx.x] = vx; red[2][threadIdx.x] = vy; red[3][threadIdx.x] = vz;
+    // results: through LDS so that one lane holds (Re U, Im U, Re V, Im V)
+    // of one (row, j): M = 4 (l / 16) + v is (row r = M % 8, Re/Im = M / 8),
+    // N = l % 16 is (j = N % 8, C/D = N / 8)
+    float *scr = reinterpret_cast<float *>(wl) + w * 256;
+    const int r = lane >> 3, jj = lane & 7;
+    const int q0 = (r & 3) * 64 + 16 * (r >> 2) + jj;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            if (t < ntw)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) scr[v * 64 + lane] = acc[mt][t][v];
+            __syncthreads();
+            const int row = rw + 8 * mt + r;
+            if (t < ntw && row < a.nfrows) {
+                const float ur = scr[q0], ui = scr[q0 + 32], vr = scr[q0 + 8], vi = scr[q0 + 40];
+                const FRow fr = ldv(gld(a.srows) + row);
+                const int j = 8 * t + jj;
+                if (j == 0) stv(gst(a.s0part) + (size_t)ch * a.nfrows + fr.pad0, make_float2(ur, ui));
+                else if (j <= fr.m) stv(gst(a.ppart) + (size_t)ch * a.npk + fr.off + j - 1, make_float4(ur, ui, vr, vi));
+            }
+            __syncthreads();
+        }
+}
+
+template <int NT>
+__global__ __launch_bounds__(256, 2) void k_sk(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ float2 wl[kSkRows * kSkStride];
+    __shared__ float2 zl[NT * 8 * kSkStride];
+    const int2 m = map[blockIdx.x];
+    sk_body<NT>(args[m.x], m.y, wl, zl);
+}
+
+template __global__ void k_sk<3>(const RecipArgs *, const int2 *);
+template __global__ void k_sk<5>(const RecipArgs *, const int2 *);
+template __global__ void k_sk<7>(const RecipArgs *, const int2 *);
+
+// Energy and virial terms of one k vector (forces.f:1317-1331); returns
+// A_k = 2 AKC S(k).
+__device__ inline float2 k_terms(const RecipArgs &a, int ix, int iy, int iz, double sr, double si, double (&acc)[4]) {
+    const double twopi = 2.0 * kPI;
+    const double rxv = twopi * ix / a.box.l[0], ryv = twopi * iy / a.box.l[1], rzv = twopi * iz / a.box.l[2];
+    double rk2 = rxv * rxv + ryv * ryv;
+    rk2 = rk2 + rzv * rzv;
+    const double alp2 = a.alpha * a.alpha;
+    const double vol = a.box.l[0] * a.box.l[1] * a.box.l[2];
+    const double cfur = 4.0 * kPI * a.coulf / vol;
+    const double akc = cfur * exp(-0.25 * rk2 / alp2) / rk2;
+    const double s2 = sr * sr + si * si;
+    const double e = akc * s2;
+    const double akv = 0.5 * akc * (4.0 * alp2 + rk2) / (alp2 * rk2);
+    acc[0] += e;
+    acc[1] += e - rxv * rxv * akv * s2;
+    acc[2] += e - ryv * ryv * akv * s2;
+    acc[3] += e - rzv * rzv * akv * s2;
+    return make_float2((float)(2.0 * akc * sr), (float)(2.0 * akc * si));
+}
+
+// Sum the chunk partials in FP64 (fixed order), form S(+-j) = U_j +- i V_j,
+// the energy QPE and virial projections of every k (forces.f:1317-1331)
+// and the force-pass data: per pair entry D = (A+r - A-r, A+r + A-r,
+// A+i + A-i, A+i - A-i), per row A of n_z = 0.
+__device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)[256]) {
+    const int e = blk * 256 + threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (e < a.npk) {
+        const int4 k = ldv(gld(a.fpk) + e);
+        float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (k.x >= 0 || k.y >= 0) {
+            double ur = 0.0, ui = 0.0, vr = 0.0, vi = 0.0;
+            for (int c = 0; c < a.nchunk; ++c) {
+                const float4 p = ldv(gld(a.ppart) + (size_t)c * a.npk + e);
+                ur += (double)p.x;
+                ui += (double)p.y;
+                vr += (double)p.z;
+                vi += (double)p.w;
+            }
+            const FRow fr = ldv(gld(a.frows) + k.w);
+            float2 ap = make_float2(0.f, 0.f), am = make_float2(0.f, 0.f);
+            if (k.x >= 0) ap = k_terms(a, fr.ix, fr.iy, k.z, ur - vi, ui + vr, acc);
+            if (k.y >= 0) am = k_terms(a, fr.ix, fr.iy, -k.z, ur + vi, ui - vr, acc);
+            d = make_float4(ap.x - am.x, ap.x + am.x, ap.y + am.y, ap.y - am.y);
+        }
+        stv(gst(a.fpack) + e, d);
+    } else if (e < a.npk + a.nfrows) {
+        const int r = e - a.npk;
+        const FRow fr = ldv(gld(a.frows) + r);
+        float2 a0 = make_float2(0.f, 0.f);
+        if (fr.k0 >= 0) {
+            double sr = 0.0, si = 0.0;
+            for (int c = 0; c < a.nchunk; ++c) {
+                const float2 p = ldv(gld(a.s0part) + (size_t)c * a.nfrows + r);
+                sr += (double)p.x;
+                si += (double)p.y;
+            }
+            a0 = k_terms(a, fr.ix, fr.iy, 0, sr, si, acc);
+        }
+        stv(gst(a.fa0) + r, a0);
+    }
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = acc[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s)
@@ -156,7 +282,7 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double *o = a.epart + (size_t)blk * Q_N;
+        MDX_GLOBAL double *o = gst(a.epart) + (size_t)blk * Q_N;
         for (int q = 0; q < Q_N; ++q) o[q] = 0.0;
         o[Q_PELS1] = red[0][0];
         o[Q_VIRX] = red[1][0];
@@ -169,28 +295,6 @@ __global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__
     __shared__ double red[4][256];
     const int2 m = map[blockIdx.x];
     sk_reduce_body(args[m.x], m.y, red);
-}
-
-// Pair packing for the force pass: for each (row, j) the sums and
-// differences of A_k = 2 AKC S(k) at n_z = +j and -j; per row the n_z = 0
-// value.
-__device__ inline void fpack_body(const RecipArgs &a, int blk) {
-    const int e = blk * 256 + threadIdx.x;
-    if (e < a.npk) {
-        const int4 k = ldv(gld(a.fpk) + e);
-        const float2 ap = k.x >= 0 ? ldv(gld(a.ak) + k.x) : make_float2(0.f, 0.f);
-        const float2 am = k.y >= 0 ? ldv(gld(a.ak) + k.y) : make_float2(0.f, 0.f);
-        stv(gst(a.fpack) + e, make_float4(ap.x - am.x, ap.x + am.x, ap.y + am.y, ap.y - am.y));
-    } else if (e < a.npk + a.nfrows) {
-        const int r = e - a.npk;
-        const int k0 = gld(a.frows)[r].k0;
-        stv(gst(a.fa0) + r, k0 >= 0 ? ldv(gld(a.ak) + k0) : make_float2(0.f, 0.f));
-    }
-}
-
-__global__ __launch_bounds__(256) void k_fpack(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
-    const int2 m = map[blockIdx.x];
-    fpack_body(args[m.x], m.y);
 }
 
 // Reciprocal forces (forces.f:1320-1326); a block covers kForceBlock atoms
