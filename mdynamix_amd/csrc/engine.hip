@@ -111,12 +111,13 @@ struct mdx_engine {
     int2 *d_aux = nullptr;
     bool have_list = false;
     // reciprocal space
-    int nkv = 0, nrows = 0, chunk = 512, nchunk = 0, nslice = 1, rows_per_slice = 0;
+    int nkv = 0, nrows = 0, chunk = 512, nchunk = 0, nslice = 1, tiles_per_slice = 0, ntiles = 0;
     int nfrows = 0, npk = 0, mz = 0;
     int tuned_nslice = 0;       // 0: heuristic
     FRow *d_frows = nullptr;
     int4 *d_fpk = nullptr;
-    float4 *d_fpack = nullptr;
+    FTile *d_ftiles = nullptr;
+    float *d_wm = nullptr;
     float2 *d_fa0 = nullptr;
     double kbox[3] = {0, 0, 0};
     double spei = 0.0;
@@ -407,9 +408,36 @@ mdx_status setup_kspace(mdx_engine *h) {
         CHK(dalloc(h, &h->d_tx, (size_t)h->n * (h->kx + 1)));
         CHK(dalloc(h, &h->d_ty, (size_t)h->n * (h->ky + 1)));
         CHK(dalloc(h, &h->d_tz, (size_t)h->n * h->mzp));
+        // force-pass tiles: up to 4 rows of one n_x with consecutive n_y;
+        // the matrix operand of a tile is ceil(longest row / 2) steps of 64
+        // floats, zero where a row has no pair entry
+        std::vector<FTile> tl;
+        int woff = 0;
+        for (size_t r = 0; r < fr.size(); ++r) {
+            FTile *cur = tl.empty() ? nullptr : &tl.back();
+            if (!cur || cur->nrow == 4 || fr[r].ix != cur->ix || fr[r].iy != cur->iy0 + cur->nrow) {
+                if (cur) woff += cur->nstep * 64;
+                FTile t{};
+                t.ix = fr[r].ix;
+                t.iy0 = fr[r].iy;
+                t.r0 = (int)r;
+                tl.push_back(t);
+                cur = &tl.back();
+                cur->woff = woff;
+            }
+            fr[r].pad1 = cur->woff;
+            fr[r].pad2 = cur->nrow++;
+            cur->nstep = std::max(cur->nstep, (fr[r].m + 1) / 2);
+        }
+        if (!tl.empty()) woff += tl.back().nstep * 64;
+        h->ntiles = (int)tl.size();
+        CHK(dalloc(h, &h->d_ftiles, std::max<size_t>(1, tl.size())));
+        CHK(dalloc(h, &h->d_wm, std::max(64, woff)));
+        if (!h->dry && !tl.empty())
+            HIPCHK(h, hipMemcpy(h->d_ftiles, tl.data(), sizeof(FTile) * tl.size(), hipMemcpyHostToDevice));
+        if (!h->dry) HIPCHK(h, hipMemset(h->d_wm, 0, sizeof(float) * std::max(64, woff)));
         CHK(dalloc(h, &h->d_frows, fr.size()));
         CHK(dalloc(h, &h->d_fpk, std::max<size_t>(1, pk.size())));
-        CHK(dalloc(h, &h->d_fpack, pk.size() + 8));   // + two spare blocks
         CHK(dalloc(h, &h->d_fa0, fr.size()));
         if (!h->dry) HIPCHK(h, hipMemcpy(h->d_frows, fr.data(), sizeof(FRow) * fr.size(), hipMemcpyHostToDevice));
         if (!h->dry && !pk.empty())
@@ -417,11 +445,11 @@ mdx_status setup_kspace(mdx_engine *h) {
     }
     // about 2,048 structure-factor blocks (row groups x atom chunks) unless tuned
     CHK(set_chunks(h, h->tuned_nslice > 0 ? h->chunk : std::max(256, cdiv(h->n * h->nsgroups, 2048))));
-    // force pass: slices of rows so the grid fills the chip (about 6,000
-    // wavefronts: one and a half rounds at four per SIMD) unless tuned
+    // force pass: slices of tiles so the grid fills the chip (about 6,000
+    // wavefronts of 64 atoms) unless tuned
     const int wav = cdiv(h->n, 64);
     h->nslice = h->tuned_nslice > 0 ? h->tuned_nslice : std::max(1, std::min(16, cdiv(6000, std::max(1, wav))));
-    h->rows_per_slice = cdiv(std::max(1, h->nfrows), h->nslice);
+    h->tiles_per_slice = cdiv(std::max(1, h->ntiles), h->nslice);
     CHK(dalloc(h, &h->d_rpart, (size_t)h->nslice * 3 * h->n));
     return MDX_OK;
 }
@@ -534,7 +562,7 @@ mdx_status set_kspace_shape(mdx_engine *h, int chunk, int nslice) {
     CHK(set_chunks(h, chunk));
     h->tuned_nslice = std::max(1, std::min(64, nslice));
     h->nslice = h->tuned_nslice;
-    h->rows_per_slice = cdiv(std::max(1, h->nfrows), h->nslice);
+    h->tiles_per_slice = cdiv(std::max(1, h->ntiles), h->nslice);
     CHK(dalloc(h, &h->d_rpart, (size_t)h->nslice * 3 * h->n));
     fill_args(h);
     return MDX_OK;
@@ -672,12 +700,14 @@ void fill_args(mdx_engine *h) {
     ra.ppart = h->d_ppart;
     ra.s0part = h->d_s0part;
     ra.nslice = h->nslice;
-    ra.rows_per_slice = h->rows_per_slice;
+    ra.tiles_per_slice = h->tiles_per_slice;
     ra.nfrows = h->nfrows;
     ra.npk = h->npk;
     ra.frows = h->d_frows;
     ra.fpk = h->d_fpk;
-    ra.fpack = h->d_fpack;
+    ra.ntiles = h->ntiles;
+    ra.ftiles = h->d_ftiles;
+    ra.wm = h->d_wm;
     ra.fa0 = h->d_fa0;
     ra.natom_blocks = cdiv(n, kForceBlock);
     ra.rpart = h->d_rpart;

@@ -80,6 +80,12 @@ struct FRow {               // 32 bytes
     int ix, iy, m, off, k0, pad0, pad1, pad2;
 };
 
+// Force-pass tile: up to 4 rows of one n_x with consecutive n_y; its matrix
+// operand (nstep blocks of 64 floats) starts at woff.
+struct FTile {              // 32 bytes
+    int ix, iy0, nrow, nstep, woff, r0, pad0, pad1;
+};
+
 __host__ __device__ inline double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
 
 // Address-space views of the pointers kept in argument structs (which the
@@ -222,7 +228,7 @@ struct RecipArgs {
     int kx, ky, mzp;
     int chunk, nchunk, nkv;
     double alpha, coulf;
-    int nslice, rows_per_slice, natom_blocks;
+    int nslice, tiles_per_slice, natom_blocks;
     double *rpart;               // [nslice][3][natoms]
     double *epart;               // rows of k_sk_reduce
     // (n_x, n_y) rows with +-n_z pairs: frows in (n_x, n_y) order (force
@@ -234,8 +240,12 @@ struct RecipArgs {
     const int4 *fpk;             // per pair entry: k(+j), k(-j), j, frow
     float4 *ppart;               // [nchunk][npk]: partial U_j, V_j (nchunk a multiple of 8)
     float2 *s0part;              // [nchunk][nfrows]: partial S(n_z = 0)
-    float4 *fpack;               // per pair entry: D = (A+r - A-r, A+r + A-r, A+i + A-i, A+i - A-i)
     float2 *fa0;                 // per row: A of n_z = 0
+    // force pass: tiles of 4 rows and their matrix operand (frows: pad1 =
+    // woff of the row's tile, pad2 = row within the tile)
+    int ntiles;
+    const FTile *ftiles;
+    float *wm;
 };
 
 struct ScatterArgs {

@@ -235,13 +235,18 @@ __device__ inline float2 k_terms(const RecipArgs &a, int ix, int iy, int iz, dou
 
 // Sum the chunk partials in FP64 (fixed order), form S(+-j) = U_j +- i V_j,
 // the energy QPE and virial projections of every k (forces.f:1317-1331)
-// and the force-pass data: per pair entry D = (A+r - A-r, A+r + A-r,
-// A+i + A-i, A+i - A-i), per row A of n_z = 0.
+// and the force-pass data: per row A of n_z = 0, and per pair entry, from
+// D = (A+r - A-r, A+r + A-r, A+i + A-i, A+i - A-i), the coefficients of
+// (Re p_j, Im p_j) in the four sums of recip_force_body, written in the
+// operand order of its matrix instruction: lane K 16 + 4 row + s, with
+// K = 2 ((j - 1) mod 2) + (0: Re p_j, 1: Im p_j) and s = 0..3 for
+//   S1: (-D3, D1)   S2: (D2, D4)   Z1: j (-D4, D2)   Z2: j (D1, D3).
 __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)[256]) {
     const int e = blk * 256 + threadIdx.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (e < a.npk) {
         const int4 k = ldv(gld(a.fpk) + e);
+        const FRow fr = ldv(gld(a.frows) + k.w);
         float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
         if (k.x >= 0 || k.y >= 0) {
             double ur = 0.0, ui = 0.0, vr = 0.0, vi = 0.0;
@@ -252,13 +257,18 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
                 vr += (double)p.z;
                 vi += (double)p.w;
             }
-            const FRow fr = ldv(gld(a.frows) + k.w);
             float2 ap = make_float2(0.f, 0.f), am = make_float2(0.f, 0.f);
             if (k.x >= 0) ap = k_terms(a, fr.ix, fr.iy, k.z, ur - vi, ui + vr, acc);
             if (k.y >= 0) am = k_terms(a, fr.ix, fr.iy, -k.z, ur + vi, ui - vr, acc);
             d = make_float4(ap.x - am.x, ap.x + am.x, ap.y + am.y, ap.y - am.y);
         }
-        stv(gst(a.fpack) + e, d);
+        if (k.z <= fr.m) {
+            const float fj = (float)k.z;
+            const int jm = k.z - 1;
+            MDX_GLOBAL float *w = gst(a.wm) + fr.pad1 + (jm >> 1) * 64 + (jm & 1) * 32 + 4 * fr.pad2;
+            stv(reinterpret_cast<MDX_GLOBAL float4 *>(w), make_float4(-d.z, d.y, -fj * d.w, fj * d.x));
+            stv(reinterpret_cast<MDX_GLOBAL float4 *>(w + 16), make_float4(d.x, d.w, fj * d.y, fj * d.z));
+        }
     } else if (e < a.npk + a.nfrows) {
         const int r = e - a.npk;
         const FRow fr = ldv(gld(a.frows) + r);
@@ -297,131 +307,134 @@ __global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__
     sk_reduce_body(args[m.x], m.y, red);
 }
 
-// Reciprocal forces (forces.f:1320-1326); a block covers kForceBlock atoms
-// (four wavefronts on one CU, which share the row data through the scalar
-// cache) and one slice of rows; each slice writes its own partial force
-// array (summed by the combine kernel).  With b = q exp(2 pi i (n_x f_x + n_y f_y)) and
-// p_j = exp(2 pi i j f_z), the two vectors n_z = +-j contribute
+// Reciprocal forces (forces.f:1320-1326).  With b = q exp(2 pi i (n_x f_x +
+// n_y f_y)) and p_j = exp(2 pi i j f_z), the two vectors n_z = +-j of a row
+// contribute
 //   t(+j) + t(-j) = b_r (p_i D1 - p_r D3) + b_i (p_r D2 + p_i D4)
 //   t(+j) - t(-j) = b_r (p_i D2 - p_r D4) + b_i (p_r D1 + p_i D3)
-// with t = Im(exp(i k.r) conj(A_k)), so a row costs 10 VALU operations per
-// pair of k vectors on the per-atom table p_j (FP64 recurrence, kept in
-// registers) and the row's pair data (wavefront-uniform, scalar loads).  b advances
-// along n_y by exp(2 pi i f_y) in FP64 and restarts from an FP64 sincospi
-// at every new n_x.
+// (t = Im(exp(i k.r) conj(A_k)), D from k_sk_reduce), so per atom and row
+// the four sums over j
+//   S1 = sum (p_i D1 - p_r D3)    S2 = sum (p_r D2 + p_i D4)
+//   Z1 = sum j (p_i D2 - p_r D4)  Z2 = sum j (p_r D1 + p_i D3)
+// are a matrix product (4 rows x 4 sums) x (2 j x (Re, Im)) x (16 atoms),
+// v_mfma_f32_16x16x4_f32 with the row coefficients as A (laid out by
+// k_sk_reduce, one float per lane and step) and the atoms' p_j as B (FP64
+// recurrence, kept in registers for the whole row loop).  Lane l then holds
+// S1, S2, Z1, Z2 of row l / 16 of the tile and atom l % 16, and adds
+//   t = b_i A0_r - b_r A0_i + b_r S1 + b_i S2,  t_z = b_r Z1 + b_i Z2
+// (A0: n_z = 0) into F_x += n_x t, F_y += n_y t, F_z += t_z in FP64; b
+// advances over the 4-row tiles of one n_x by exp(8 pi i f_y) (FP64) and
+// restarts from an FP64 sincospi at every new n_x.  A wavefront takes 4
+// tiles of 16 atoms, a block 4 wavefronts, each block one slice of the
+// tiles; the 4 lanes of an atom are summed at the end and every slice
+// writes its own partial force array (summed by the combine kernel).
 template <int MZ>
 __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
+    constexpr int SM = MZ / 2;
     const int ab = blk % a.natom_blocks, sl = blk / a.natom_blocks;
-    const int i = ab * kForceBlock + threadIdx.x;
-    const int r0 = sl * a.rows_per_slice;
-    const int r1 = min(r0 + a.rows_per_slice, a.nfrows);
-    double4 at = make_double4(0, 0, 0, 0);
-    double4 st = make_double4(1.0, 0.0, 1.0, 0.0);
-    if (i < a.natoms) { at = ldv(gld(a.fq) + i); st = ldv(gld(a.stepz) + i); }
-    float pr[MZ], pim[MZ];
-    {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int K = lane >> 4, jo = K >> 1, cm = K & 1;
+    const int t0 = sl * a.tiles_per_slice;
+    const int t1 = min(t0 + a.tiles_per_slice, a.ntiles);
+    const int abase = ab * kForceBlock + w * 64 + (lane & 15);
+    float P[4][SM];
+    double fx[4], fy[4], q[4], w4r[4], w4i[4], br[4], bi[4];
+    double Fx[4], Fy[4], Fz[4];
+#pragma unroll
+    for (int ta = 0; ta < 4; ++ta) {
+        const int i = abase + 16 * ta;
+        double4 at = make_double4(0, 0, 0, 0);
+        double4 st = make_double4(1.0, 0.0, 1.0, 0.0);
+        if (i < a.natoms) { at = ldv(gld(a.fq) + i); st = ldv(gld(a.stepz) + i); }
+        fx[ta] = at.x;
+        fy[ta] = at.y;
+        q[ta] = at.w;
+        // p_j for j = 2 s + 1 + jo, component cm
         double wr = st.x, wi = st.y;
 #pragma unroll
-        for (int j = 0; j < MZ; ++j) {
-            pr[j] = (float)wr;
-            pim[j] = (float)wi;
+        for (int j = 1; j <= MZ; ++j) {
+            if (((j - 1) & 1) == jo) P[ta][(j - 1) >> 1] = (float)(cm ? wi : wr);
             const double t = wr * st.x - wi * st.y;
             wi = wr * st.y + wi * st.x;
             wr = t;
         }
+        double s4, c4;
+        {
+            double v = 4.0 * (at.y - rint(at.y));
+            v -= rint(v);
+            sincospi(2.0 * v, &s4, &c4);
+        }
+        w4r[ta] = c4;
+        w4i[ta] = s4;
+        br[ta] = bi[ta] = 0.0;
+        Fx[ta] = Fy[ta] = Fz[ta] = 0.0;
     }
-    double eyr, eyi;
-    {
-        const double v = at.y - rint(at.y);
-        sincospi(2.0 * v, &eyi, &eyr);
+    const MDX_CONST FTile *tiles = sld(a.ftiles);
+    const MDX_GLOBAL float *wm = gld(a.wm) + lane;
+    const MDX_GLOBAL float2 *fa0 = gld(a.fa0);
+    int cix = -0x7fffffff, ciy = 0;
+    for (int t = t0; t < t1; ++t) {
+        const FTile T = ldv(tiles + t);
+        const int ny = T.iy0 + K;
+        const bool restart = T.ix != cix || T.iy0 != ciy + 4;
+        cix = T.ix;
+        ciy = T.iy0;
+        float A[SM];
+#pragma unroll
+        for (int s = 0; s < SM; ++s) A[s] = s < T.nstep ? ldv(wm + T.woff + 64 * s) : 0.f;
+        const float2 a0 = K < T.nrow ? ldv(fa0 + T.r0 + K) : make_float2(0.f, 0.f);
+        mfma_f4 acc[4];
+#pragma unroll
+        for (int ta = 0; ta < 4; ++ta) acc[ta] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < SM; ++s) {
+            if (s < T.nstep) {
+#pragma unroll
+                for (int ta = 0; ta < 4; ++ta)
+                    acc[ta] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s], P[ta][s], acc[ta], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int ta = 0; ta < 4; ++ta) {
+            if (restart) {
+                double v = (double)T.ix * fx[ta] + (double)ny * fy[ta];
+                v = v - rint(v);
+                double sv, cv;
+                sincospi(2.0 * v, &sv, &cv);
+                br[ta] = q[ta] * cv;
+                bi[ta] = q[ta] * sv;
+            } else {
+                const double u = br[ta] * w4r[ta] - bi[ta] * w4i[ta];
+                bi[ta] = br[ta] * w4i[ta] + bi[ta] * w4r[ta];
+                br[ta] = u;
+            }
+            const float bre = (float)br[ta], bim = (float)bi[ta];
+            const float ts = (bim * a0.x - bre * a0.y) + (bre * acc[ta][0] + bim * acc[ta][1]);
+            const float tzs = bre * acc[ta][2] + bim * acc[ta][3];
+            Fx[ta] += (double)T.ix * (double)ts;
+            Fy[ta] += (double)ny * (double)ts;
+            Fz[ta] += (double)tzs;
+        }
     }
     const double tx = 2.0 * kPI / a.box.l[0], ty = 2.0 * kPI / a.box.l[1], tz = 2.0 * kPI / a.box.l[2];
-    const MDX_CONST FRow *frows = sld(a.frows);
-    const MDX_CONST float4 *fpack = sld(a.fpack);
-    const MDX_CONST float2 *fa0 = sld(a.fa0);
-    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-    if (r0 < r1) {
-        double Br = 0.0, Bi = 0.0;
-        int cix = -0x7fffffff, ciy = 0;
-        // the pair data of consecutive rows is contiguous: the blocks after a
-        // row's last one are the next row's first ones (the array ends with
-        // two spare blocks), so blocks are requested two blocks ahead into a
-        // three-block ring of registers whose roles rotate at compile time
-        FRow nfr = ldv(frows + r0);
-        float4 buf[3][4];
+    MDX_GLOBAL double *o = gst(a.rpart) + (size_t)sl * 3 * a.natoms;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            buf[0][q] = ldv(fpack + nfr.off + q);
-            buf[1][q] = ldv(fpack + nfr.off + 4 + q);
+    for (int ta = 0; ta < 4; ++ta) {
+        // the atom's four lanes (rows 0..3 of every tile), fixed order
+        double x = Fx[ta], y = Fy[ta], z = Fz[ta];
+        x += __shfl_xor(x, 16); y += __shfl_xor(y, 16); z += __shfl_xor(z, 16);
+        x += __shfl_xor(x, 32); y += __shfl_xor(y, 32); z += __shfl_xor(z, 32);
+        const int i = abase + 16 * ta;
+        if (K == 0 && i < a.natoms) {
+            o[i] = tx * x;
+            o[a.natoms + i] = ty * y;
+            o[2 * a.natoms + i] = tz * z;
         }
-        for (int r = r0; r < r1; ++r) {
-            const FRow fr = nfr;
-            if (r + 1 < r1) nfr = ldv(frows + r + 1);
-            if (fr.ix != cix || fr.iy != ciy + 1) {
-                double v = (double)fr.ix * at.x + (double)fr.iy * at.y;
-                v = v - rint(v);
-                double s, c;
-                sincospi(2.0 * v, &s, &c);
-                Br = at.w * c;
-                Bi = at.w * s;
-            } else {
-                const double t = Br * eyr - Bi * eyi;
-                Bi = Br * eyi + Bi * eyr;
-                Br = t;
-            }
-            cix = fr.ix;
-            ciy = fr.iy;
-            const float2 a0 = ldv(fa0 + r);
-            const MDX_CONST float4 *P = fpack + fr.off;
-            const int nb = (fr.m + 3) >> 2;
-            float sx = 0.f, sy = 0.f, zx = 0.f, zy = 0.f;
-#pragma unroll
-            for (int jb = 0; jb < MZ / 4; ++jb) {
-                if (jb >= nb) break;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) buf[(jb + 2) % 3][q] = ldv(P + (jb + 2) * 4 + q);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int j = jb * 4 + q;
-                    const float4 d = buf[jb % 3][q];
-                    sx = fmaf(pim[j], d.x, sx);
-                    sx = fmaf(-pr[j], d.z, sx);
-                    sy = fmaf(pr[j], d.y, sy);
-                    sy = fmaf(pim[j], d.w, sy);
-                    // n_z-weighted: (j + 1) (p_i D2 - p_r D4), (j + 1) (p_r D1 + p_i D3)
-                    const float ux = fmaf(pim[j], d.y, -pr[j] * d.w);
-                    const float uy = fmaf(pr[j], d.x, pim[j] * d.z);
-                    zx = fmaf((float)(j + 1), ux, zx);
-                    zy = fmaf((float)(j + 1), uy, zy);
-                }
-            }
-            // the next row's first two blocks sit in buf[nb % 3], buf[(nb + 1) % 3]
-            const int rot = nb % 3;
-            if (rot == 1) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) { buf[0][q] = buf[1][q]; buf[1][q] = buf[2][q]; }
-            } else if (rot == 2) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) { const float4 t = buf[0][q]; buf[0][q] = buf[2][q]; buf[1][q] = t; }
-            }
-            const float br = (float)Br, bi = (float)Bi;
-            const float ts = (bi * a0.x - br * a0.y) + (br * sx + bi * sy);
-            const float tzs = br * zx + bi * zy;
-            Fx += tx * fr.ix * (double)ts;
-            Fy += ty * fr.iy * (double)ts;
-            Fz += tz * (double)tzs;
-        }
-    }
-    if (i < a.natoms) {
-        MDX_GLOBAL double *o = gst(a.rpart) + (size_t)sl * 3 * a.natoms;
-        o[i] = Fx;
-        o[a.natoms + i] = Fy;
-        o[2 * a.natoms + i] = Fz;
     }
 }
 
 template <int MZ>
-__global__ __launch_bounds__(kForceBlock, 4) void k_recip_force(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
+__global__ __launch_bounds__(256, 2) void k_recip_force(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     recip_force_body<MZ>(args[m.x], m.y);
 }
