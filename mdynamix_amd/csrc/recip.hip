@@ -92,10 +92,18 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 // row (S(n_z = 0)); k_sk_reduce adds the chunks in FP64.
 // Blocks are numbered so that the row groups of one chunk share an XCD
 // (block index mod 8), whose L2 then serves that chunk's table columns.
-constexpr int kSkStride = kSkSub + 4;      // float2 per LDS row (bank spread)
+constexpr int kSkStride = kSkSub + 4;      // floats per LDS plane row (bank spread)
 
+// LDS stage: W planes [64 rows][Re, Im][kSkStride] then Z planes
+// [8 NT values of j][C, D][kSkStride]; two stages alternate, so the next
+// one is fetched into registers while the current one feeds the matrix
+// instructions and one barrier per stage suffices.  Within a stage, step
+// (h, e) of the 8 takes atom 16 h + 4 K + e as its K index K, so a lane
+// reads its four steps' operands with one 16-byte LDS load.
 template <int NT>
-__device__ inline void sk_body(const RecipArgs &a, int blk, float2 *wl, float2 *zl) {
+__device__ inline void sk_body(const RecipArgs &a, int blk, float *lds) {
+    constexpr int WSZ = kSkRows * 2 * kSkStride;
+    constexpr int BUF = WSZ + NT * 8 * 2 * kSkStride;
     const int G = a.nsgroups;
     const int x8 = blk & 7, s8 = blk >> 3;
     const int g = s8 % G, ch = (s8 / G) * 8 + x8;
@@ -131,53 +139,79 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, float2 *wl, float2 *
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[mt][t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
     const MDX_GLOBAL float2 *tx = gld(a.tx), *ty = gld(a.ty), *tz = gld(a.tz);
-    const int rA = lane & 7, cA = (lane >> 3) & 1, kA = lane >> 4;
-    const float *wlf = reinterpret_cast<const float *>(wl);
-    const float *zlf = reinterpret_cast<const float *>(zl);
-    for (int sb = a0; sb < a1; sb += kSkSub) {
+    float2 X[8], Y[8], Zv[NT];
+    auto fetch = [&](int sb) {
         const int ia = sb + sa;
         const bool ok = ia < a1;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            float2 b = make_float2(0.f, 0.f);
+            X[i] = make_float2(0.f, 0.f);
+            Y[i] = make_float2(0.f, 0.f);
             if (ok && offx[i] >= 0) {
-                const float2 X = ldv(tx + offx[i] + ia);
-                float2 Y = ldv(ty + offy[i] + ia);
-                if (conj[i]) Y.y = -Y.y;
-                b = make_float2(X.x * Y.x - X.y * Y.y, X.x * Y.y + X.y * Y.x);
+                X[i] = ldv(tx + offx[i] + ia);
+                Y[i] = ldv(ty + offy[i] + ia);
             }
-            wl[(sr0 + 8 * i) * kSkStride + sa] = b;
+        }
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            Zv[i] = make_float2(0.f, 0.f);
+            if (ok && i < ntg) Zv[i] = ldv(tz + (size_t)(sr0 + 8 * i) * n + ia);
+        }
+    };
+    auto store = [&](float *buf) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float yi = conj[i] ? -Y[i].y : Y[i].y;
+            const int r = sr0 + 8 * i;
+            buf[(2 * r) * kSkStride + sa] = X[i].x * Y[i].x - X[i].y * yi;
+            buf[(2 * r + 1) * kSkStride + sa] = X[i].x * yi + X[i].y * Y[i].x;
         }
 #pragma unroll
         for (int i = 0; i < NT; ++i) {
             const int j = sr0 + 8 * i;
-            float2 p = make_float2(0.f, 0.f);
-            if (ok && i < ntg) p = ldv(tz + (size_t)j * n + ia);
-            zl[j * kSkStride + sa] = p;
+            buf[WSZ + (2 * j) * kSkStride + sa] = Zv[i].x;
+            buf[WSZ + (2 * j + 1) * kSkStride + sa] = Zv[i].y;
         }
-        __syncthreads();
+    };
+    const int rA = lane & 7, cA = (lane >> 3) & 1, kA = lane >> 4;
+    const int nsub = a1 > a0 ? (a1 - a0 + kSkSub - 1) / kSkSub : 0;
+    if (nsub > 0) {
+        fetch(a0);
+        store(lds);
+    }
+    __syncthreads();
+    for (int k = 0; k < nsub; ++k) {
+        const float *cur = lds + (k & 1) * BUF;
+        if (k + 1 < nsub) fetch(a0 + (k + 1) * kSkSub);
         if (ntw > 0) {
 #pragma unroll
-            for (int s = 0; s < kSkSub / 4; ++s) {
-                const int at = 4 * s + kA;
-                const float A0 = wlf[2 * ((16 * w + rA) * kSkStride + at) + cA];
-                const float A1 = wlf[2 * ((16 * w + 8 + rA) * kSkStride + at) + cA];
+            for (int h = 0; h < 2; ++h) {
+                const int col = 16 * h + 4 * kA;
+                const float4 A0 = *reinterpret_cast<const float4 *>(cur + (2 * (16 * w + rA) + cA) * kSkStride + col);
+                const float4 A1 = *reinterpret_cast<const float4 *>(cur + (2 * (16 * w + 8 + rA) + cA) * kSkStride + col);
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                     if (t < ntw) {
-                        const float Bv = zlf[2 * ((8 * t + rA) * kSkStride + at) + cA];
-                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0, Bv, acc[0][t], 0, 0, 0);
-                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1, Bv, acc[1][t], 0, 0, 0);
+                        const float4 Bv = *reinterpret_cast<const float4 *>(cur + WSZ + (2 * (8 * t + rA) + cA) * kSkStride + col);
+                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0.x, Bv.x, acc[0][t], 0, 0, 0);
+                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1.x, Bv.x, acc[1][t], 0, 0, 0);
+                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0.y, Bv.y, acc[0][t], 0, 0, 0);
+                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1.y, Bv.y, acc[1][t], 0, 0, 0);
+                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0.z, Bv.z, acc[0][t], 0, 0, 0);
+                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1.z, Bv.z, acc[1][t], 0, 0, 0);
+                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0.w, Bv.w, acc[0][t], 0, 0, 0);
+                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1.w, Bv.w, acc[1][t], 0, 0, 0);
                     }
                 }
             }
         }
+        if (k + 1 < nsub) store(lds + ((k + 1) & 1) * BUF);
         __syncthreads();
     }
     // results: through LDS so that one lane holds (Re U, Im U, Re V, Im V)
     // of one (row, j): M = 4 (l / 16) + v is (row r = M % 8, Re/Im = M / 8),
     // N = l % 16 is (j = N % 8, C/D = N / 8)
-    float *scr = reinterpret_cast<float *>(wl) + w * 256;
+    float *scr = lds + w * 256;
     const int r = lane >> 3, jj = lane & 7;
     const int q0 = (r & 3) * 64 + 16 * (r >> 2) + jj;
 #pragma unroll
@@ -202,10 +236,9 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, float2 *wl, float2 *
 
 template <int NT>
 __global__ __launch_bounds__(256, 2) void k_sk(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
-    __shared__ float2 wl[kSkRows * kSkStride];
-    __shared__ float2 zl[NT * 8 * kSkStride];
+    __shared__ __attribute__((aligned(16))) float lds[2 * (kSkRows * 2 * kSkStride + NT * 8 * 2 * kSkStride)];
     const int2 m = map[blockIdx.x];
-    sk_body<NT>(args[m.x], m.y, wl, zl);
+    sk_body<NT>(args[m.x], m.y, lds);
 }
 
 template __global__ void k_sk<3>(const RecipArgs *, const int2 *);
@@ -373,16 +406,32 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     const MDX_GLOBAL float *wm = gld(a.wm) + lane;
     const MDX_GLOBAL float2 *fa0 = gld(a.fa0);
     int cix = -0x7fffffff, ciy = 0;
+    // the next tile's operand is requested while the current one computes
+    float An[SM];
+    float2 a0n = make_float2(0.f, 0.f);
+    FTile Tn{};
+    if (t0 < t1) {
+        Tn = ldv(tiles + t0);
+#pragma unroll
+        for (int s = 0; s < SM; ++s) An[s] = s < Tn.nstep ? ldv(wm + Tn.woff + 64 * s) : 0.f;
+        if (K < Tn.nrow) a0n = ldv(fa0 + Tn.r0 + K);
+    }
     for (int t = t0; t < t1; ++t) {
-        const FTile T = ldv(tiles + t);
+        const FTile T = Tn;
+        float A[SM];
+#pragma unroll
+        for (int s = 0; s < SM; ++s) A[s] = An[s];
+        const float2 a0 = a0n;
+        if (t + 1 < t1) {
+            Tn = ldv(tiles + t + 1);
+#pragma unroll
+            for (int s = 0; s < SM; ++s) An[s] = s < Tn.nstep ? ldv(wm + Tn.woff + 64 * s) : 0.f;
+            a0n = K < Tn.nrow ? ldv(fa0 + Tn.r0 + K) : make_float2(0.f, 0.f);
+        }
         const int ny = T.iy0 + K;
         const bool restart = T.ix != cix || T.iy0 != ciy + 4;
         cix = T.ix;
         ciy = T.iy0;
-        float A[SM];
-#pragma unroll
-        for (int s = 0; s < SM; ++s) A[s] = s < T.nstep ? ldv(wm + T.woff + 64 * s) : 0.f;
-        const float2 a0 = K < T.nrow ? ldv(fa0 + T.r0 + K) : make_float2(0.f, 0.f);
         mfma_f4 acc[4];
 #pragma unroll
         for (int ta = 0; ta < 4; ++ta) acc[ta] = mfma_f4{0.f, 0.f, 0.f, 0.f};
