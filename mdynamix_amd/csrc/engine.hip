@@ -416,7 +416,7 @@ mdx_status setup_kspace(mdx_engine *h) {
         for (size_t r = 0; r < fr.size(); ++r) {
             FTile *cur = tl.empty() ? nullptr : &tl.back();
             if (!cur || cur->nrow == 4 || fr[r].ix != cur->ix || fr[r].iy != cur->iy0 + cur->nrow) {
-                if (cur) woff += cur->nstep * 64;
+                if (cur) woff += (cur->nstep + 1) / 2 * 128;
                 FTile t{};
                 t.ix = fr[r].ix;
                 t.iy0 = fr[r].iy;
@@ -429,16 +429,19 @@ mdx_status setup_kspace(mdx_engine *h) {
             fr[r].pad2 = cur->nrow++;
             cur->nstep = std::max(cur->nstep, (fr[r].m + 1) / 2);
         }
-        if (!tl.empty()) woff += tl.back().nstep * 64;
+        if (!tl.empty()) woff += (tl.back().nstep + 1) / 2 * 128;
+        // spare steps after the last region: the force pass reads a whole
+        // register set of steps from every region
+        woff += 24 * 64;
         h->ntiles = (int)tl.size();
         CHK(dalloc(h, &h->d_ftiles, std::max<size_t>(1, tl.size())));
-        CHK(dalloc(h, &h->d_wm, std::max(64, woff)));
+        CHK(dalloc(h, &h->d_wm, woff));
         if (!h->dry && !tl.empty())
             HIPCHK(h, hipMemcpy(h->d_ftiles, tl.data(), sizeof(FTile) * tl.size(), hipMemcpyHostToDevice));
-        if (!h->dry) HIPCHK(h, hipMemset(h->d_wm, 0, sizeof(float) * std::max(64, woff)));
+        if (!h->dry) HIPCHK(h, hipMemset(h->d_wm, 0, sizeof(float) * woff));
         CHK(dalloc(h, &h->d_frows, fr.size()));
         CHK(dalloc(h, &h->d_fpk, std::max<size_t>(1, pk.size())));
-        CHK(dalloc(h, &h->d_fa0, fr.size()));
+        CHK(dalloc(h, &h->d_fa0, fr.size() + 4));   // + the force pass's unused reads past the last tile
         if (!h->dry) HIPCHK(h, hipMemcpy(h->d_frows, fr.data(), sizeof(FRow) * fr.size(), hipMemcpyHostToDevice));
         if (!h->dry && !pk.empty())
             HIPCHK(h, hipMemcpy(h->d_fpk, pk.data(), sizeof(int4) * pk.size(), hipMemcpyHostToDevice));
@@ -447,7 +450,7 @@ mdx_status setup_kspace(mdx_engine *h) {
     CHK(set_chunks(h, h->tuned_nslice > 0 ? h->chunk : std::max(256, cdiv(h->n * h->nsgroups, 2048))));
     // force pass: slices of tiles so the grid fills the chip (about 6,000
     // wavefronts of 64 atoms) unless tuned
-    const int wav = cdiv(h->n, 64);
+    const int wav = cdiv(h->n, 16 * kForceTiles);
     h->nslice = h->tuned_nslice > 0 ? h->tuned_nslice : std::max(1, std::min(16, cdiv(6000, std::max(1, wav))));
     h->tiles_per_slice = cdiv(std::max(1, h->ntiles), h->nslice);
     CHK(dalloc(h, &h->d_rpart, (size_t)h->nslice * 3 * h->n));
@@ -1443,9 +1446,9 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
         mark(5);
         launch(b, b->skred, 256, 0, k_sk_reduce);
         mark(6);
-        if (b->mz <= 16) launch(b, b->rforce, kForceBlock, 0, k_recip_force<16>);
-        else if (b->mz <= 32) launch(b, b->rforce, kForceBlock, 0, k_recip_force<32>);
-        else launch(b, b->rforce, kForceBlock, 0, k_recip_force<48>);
+        if (b->mz <= 16) launch(b, b->rforce, 256, 0, k_recip_force<16>);
+        else if (b->mz <= 32) launch(b, b->rforce, 256, 0, k_recip_force<32>);
+        else launch(b, b->rforce, 256, 0, k_recip_force<48>);
         mark(7);
         launch(b, b->et, 256, 0, k_eterms);
     } else {
@@ -1786,7 +1789,7 @@ mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) 
         }
         return MDX_OK;
     };
-    const int wav = cdiv(h->n, 64);
+    const int wav = cdiv(h->n, 16 * kForceTiles);
     int best_chunk = h->chunk, best_slice = h->nslice;
     double best_sk = 1e30, best_force = 1e30, sk = 0.0, fo = 0.0;
     for (int chunk : {512, 1024, 2048, 4096}) {

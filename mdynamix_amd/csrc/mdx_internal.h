@@ -43,7 +43,8 @@ constexpr int kClu = 8;            // atoms per cluster
 // [POTE1(ntp) | POTL1(ntp) | PES141(nmt) | PSR141(nmt)]
 constexpr int kMaxDet = 2 * 137 + 2 * 16;
 constexpr int kWave = 64;          // wavefront size on gfx950
-constexpr int kForceBlock = 256;   // atoms per block of the reciprocal force pass
+constexpr int kForceTiles = 4;     // 16-atom tiles per wavefront of the reciprocal force pass
+constexpr int kForceBlock = 64 * kForceTiles;   // atoms per block (4 wavefronts) of that pass
 constexpr int kSkRows = 64;        // k rows per structure-factor block (16 per wavefront)
 constexpr int kSkSub = 32;         // atoms per LDS stage of the structure-factor block
 

@@ -357,9 +357,9 @@ __global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__
 //   t = b_i A0_r - b_r A0_i + b_r S1 + b_i S2,  t_z = b_r Z1 + b_i Z2
 // (A0: n_z = 0) into F_x += n_x t, F_y += n_y t, F_z += t_z in FP64; b
 // advances over the 4-row tiles of one n_x by exp(8 pi i f_y) (FP64) and
-// restarts from an FP64 sincospi at every new n_x.  A wavefront takes 4
-// tiles of 16 atoms, a block 4 wavefronts, each block one slice of the
-// tiles; the 4 lanes of an atom are summed at the end and every slice
+// restarts from an FP64 sincospi at every new n_x.  A wavefront takes
+// kForceTiles tiles of 16 atoms, a block 4 wavefronts, each block one slice
+// of the tiles; the 4 lanes of an atom are summed at the end and every slice
 // writes its own partial force array (summed by the combine kernel).
 template <int MZ>
 __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
@@ -369,19 +369,16 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     const int K = lane >> 4, jo = K >> 1, cm = K & 1;
     const int t0 = sl * a.tiles_per_slice;
     const int t1 = min(t0 + a.tiles_per_slice, a.ntiles);
-    const int abase = ab * kForceBlock + w * 64 + (lane & 15);
-    float P[4][SM];
-    double fx[4], fy[4], q[4], w4r[4], w4i[4], br[4], bi[4];
-    double Fx[4], Fy[4], Fz[4];
+    const int abase = ab * kForceBlock + w * 16 * kForceTiles + (lane & 15);
+    float P[kForceTiles][SM];
+    double w4r[kForceTiles], w4i[kForceTiles], br[kForceTiles], bi[kForceTiles];
+    double Fx[kForceTiles], Fy[kForceTiles], Fz[kForceTiles];
 #pragma unroll
-    for (int ta = 0; ta < 4; ++ta) {
+    for (int ta = 0; ta < kForceTiles; ++ta) {
         const int i = abase + 16 * ta;
         double4 at = make_double4(0, 0, 0, 0);
         double4 st = make_double4(1.0, 0.0, 1.0, 0.0);
         if (i < a.natoms) { at = ldv(gld(a.fq) + i); st = ldv(gld(a.stepz) + i); }
-        fx[ta] = at.x;
-        fy[ta] = at.y;
-        q[ta] = at.w;
         // p_j for j = 2 s + 1 + jo, component cm
         double wr = st.x, wi = st.y;
 #pragma unroll
@@ -406,57 +403,70 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     const MDX_GLOBAL float *wm = gld(a.wm) + lane;
     const MDX_GLOBAL float2 *fa0 = gld(a.fa0);
     int cix = -0x7fffffff, ciy = 0;
-    // the next tile's operand is requested while the current one computes
-    float An[SM];
+    // the next tile's operand is requested while the current one computes;
+    // operand regions hold an even number of steps (zero padded) and the
+    // array ends with SM spare steps, so the loads need no bounds
+    float A[SM], An[SM];
     float2 a0n = make_float2(0.f, 0.f);
     FTile Tn{};
     if (t0 < t1) {
         Tn = ldv(tiles + t0);
 #pragma unroll
-        for (int s = 0; s < SM; ++s) An[s] = s < Tn.nstep ? ldv(wm + Tn.woff + 64 * s) : 0.f;
-        if (K < Tn.nrow) a0n = ldv(fa0 + Tn.r0 + K);
+        for (int s = 0; s < SM; ++s) An[s] = ldv(wm + Tn.woff + 64 * s);
+        a0n = ldv(fa0 + Tn.r0 + K);
     }
     for (int t = t0; t < t1; ++t) {
         const FTile T = Tn;
-        float A[SM];
 #pragma unroll
         for (int s = 0; s < SM; ++s) A[s] = An[s];
-        const float2 a0 = a0n;
+        const float2 a0 = K < T.nrow ? a0n : make_float2(0.f, 0.f);
         if (t + 1 < t1) {
             Tn = ldv(tiles + t + 1);
 #pragma unroll
-            for (int s = 0; s < SM; ++s) An[s] = s < Tn.nstep ? ldv(wm + Tn.woff + 64 * s) : 0.f;
-            a0n = K < Tn.nrow ? ldv(fa0 + Tn.r0 + K) : make_float2(0.f, 0.f);
+            for (int s = 0; s < SM; ++s) An[s] = ldv(wm + Tn.woff + 64 * s);
+            a0n = ldv(fa0 + Tn.r0 + K);
         }
         const int ny = T.iy0 + K;
         const bool restart = T.ix != cix || T.iy0 != ciy + 4;
         cix = T.ix;
         ciy = T.iy0;
-        mfma_f4 acc[4];
+        mfma_f4 acc[kForceTiles];
 #pragma unroll
-        for (int ta = 0; ta < 4; ++ta) acc[ta] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+        for (int ta = 0; ta < kForceTiles; ++ta) acc[ta] = mfma_f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < SM; ++s) {
+        for (int s = 0; s < SM; s += 2) {
             if (s < T.nstep) {
 #pragma unroll
-                for (int ta = 0; ta < 4; ++ta)
+                for (int ta = 0; ta < kForceTiles; ++ta)
                     acc[ta] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s], P[ta][s], acc[ta], 0, 0, 0);
+#pragma unroll
+                for (int ta = 0; ta < kForceTiles; ++ta)
+                    acc[ta] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s + 1], P[ta][s + 1], acc[ta], 0, 0, 0);
             }
         }
+        if (restart) {
+            // (f_x, f_y, q) again from memory: rare, and not worth registers
 #pragma unroll
-        for (int ta = 0; ta < 4; ++ta) {
-            if (restart) {
-                double v = (double)T.ix * fx[ta] + (double)ny * fy[ta];
+            for (int ta = 0; ta < kForceTiles; ++ta) {
+                const int i = abase + 16 * ta;
+                const double4 at = i < a.natoms ? ldv(gld(a.fq) + i) : make_double4(0, 0, 0, 0);
+                double v = (double)T.ix * at.x + (double)ny * at.y;
                 v = v - rint(v);
                 double sv, cv;
                 sincospi(2.0 * v, &sv, &cv);
-                br[ta] = q[ta] * cv;
-                bi[ta] = q[ta] * sv;
-            } else {
+                br[ta] = at.w * cv;
+                bi[ta] = at.w * sv;
+            }
+        } else {
+#pragma unroll
+            for (int ta = 0; ta < kForceTiles; ++ta) {
                 const double u = br[ta] * w4r[ta] - bi[ta] * w4i[ta];
                 bi[ta] = br[ta] * w4i[ta] + bi[ta] * w4r[ta];
                 br[ta] = u;
             }
+        }
+#pragma unroll
+        for (int ta = 0; ta < kForceTiles; ++ta) {
             const float bre = (float)br[ta], bim = (float)bi[ta];
             const float ts = (bim * a0.x - bre * a0.y) + (bre * acc[ta][0] + bim * acc[ta][1]);
             const float tzs = bre * acc[ta][2] + bim * acc[ta][3];
@@ -468,7 +478,7 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     const double tx = 2.0 * kPI / a.box.l[0], ty = 2.0 * kPI / a.box.l[1], tz = 2.0 * kPI / a.box.l[2];
     MDX_GLOBAL double *o = gst(a.rpart) + (size_t)sl * 3 * a.natoms;
 #pragma unroll
-    for (int ta = 0; ta < 4; ++ta) {
+    for (int ta = 0; ta < kForceTiles; ++ta) {
         // the atom's four lanes (rows 0..3 of every tile), fixed order
         double x = Fx[ta], y = Fy[ta], z = Fz[ta];
         x += __shfl_xor(x, 16); y += __shfl_xor(y, 16); z += __shfl_xor(z, 16);
