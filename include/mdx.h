@@ -299,13 +299,30 @@ typedef struct mdx_md_params {
     const int32_t *cons_i;    /*   in the reference bond order (0-based)    */
     const int32_t *cons_j;
     const double *cons_len;   /* RB                                         */
+    int32_t integrator;       /* MDX_MD_LEAPFROG (SULTAN) or MDX_MD_RESPA   */
+    int32_t nfreq;            /* NFREQ: short steps per long step (RESPA)   */
 } mdx_md_params;
+
+/* Integrators of mdx_md_run (mdx_md_params.integrator):
+ * MDX_MD_LEAPFROG  SULTAN (mdstep.f:29-260): leap-frog, SHEJK constraints,
+ *                  SCALING(3), one force call (ALLFORCE) per step;
+ * MDX_MD_RESPA     DOUBLE (mdstep.f:281-530): multiple time step, dt the
+ *                  long step; per long step SCALING(1), a half kick with
+ *                  the slow forces G, nfreq short steps (half kick with the
+ *                  fast forces H, drift, GETCOM, fast forces -- slow and
+ *                  fast at the last short step --, half kick), a half kick
+ *                  with G, SCALING(2); the list is rebuilt before the force
+ *                  call of short step nfreq/2 + 1 of every ICHNB-th long
+ *                  step; no constraints (ncons must be 0). */
+#define MDX_MD_LEAPFROG 0
+#define MDX_MD_RESPA 1
 
 /* Start a segment from the current positions and the given velocities:
  * builds the list and evaluates the forces (SULTAN's CHCNB + ALLFORCE). */
 mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, const double *vy,
                        const double *vz);
-/* nsteps steps; trace (NULL or nsteps x 5 doubles): per step the potential
+/* nsteps steps (long steps under MDX_MD_RESPA); trace (NULL or nsteps x 5
+ * doubles): per step the potential
  * energy, kinetic energy, temperature, SC and the number of pair
  * interactions of the step's force call. */
 mdx_status mdx_md_run(mdx_engine *h, int32_t nsteps, double *trace);

@@ -26,6 +26,8 @@ STATUS_NAMES = {0: "MDX_OK", 1: "MDX_ERR_ARG", 2: "MDX_ERR_LIST_OVERFLOW", 3: "M
 MDX_LIST_REFERENCE = 0
 MDX_LIST_VERLET = 1
 MDX_FORCES_ALL = 0
+MDX_MD_LEAPFROG = 0                  # SULTAN: leap-frog with SHEJK constraints
+MDX_MD_RESPA = 1                     # DOUBLE: multiple time step (NFREQ short steps)
 MDX_FORCES_FAST = 1
 MDX_FORCES_SLOW = 2
 MDX_WANT_DETAIL = 1
@@ -75,6 +77,7 @@ class MdxMdParams(C.Structure):
         ("shake_tol", C.c_double), ("shake_bdtol", C.c_double), ("shake_maxit", C.c_int32),
         ("rebuild_every", C.c_int32), ("step0", C.c_int32), ("ncons", C.c_int32),
         ("cons_i", C.POINTER(C.c_int32)), ("cons_j", C.POINTER(C.c_int32)), ("cons_len", C.POINTER(C.c_double)),
+        ("integrator", C.c_int32), ("nfreq", C.c_int32),
     ]
 
 
@@ -87,6 +90,9 @@ def make_md_params(d: Dict):
         if name in ("cons_i", "cons_j", "cons_len"):
             continue
         if name == "ncons":
+            continue
+        if name in ("integrator", "nfreq"):
+            setattr(p, name, int(d.get(name, MDX_MD_LEAPFROG if name == "integrator" else 1)))
             continue
         setattr(p, name, d[name])
     ci = np.ascontiguousarray(d["cons_i"], np.int32)

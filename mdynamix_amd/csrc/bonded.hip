@@ -285,11 +285,14 @@ __device__ inline void combine_body(const CombineArgs &a, int blk, int group) {
         const double wx = a.x[i] - a.cx[m], wy = a.y[i] - a.cy[m], wz = a.z[i] - a.cz[m];
         v[0] = -G[0] * wx - G[1] * wy - G[2] * wz;
         v[1] = -HL[0] * wx - HL[1] * wy - HL[2] * wz;
+        // F is the evaluated classes' sum; a class not evaluated keeps its
+        // previous G or H, as FFORCES leaves GX and SLFORCES leaves HX
+        // (forces.f:36-73)
         if (a.ox) {
             a.ox[i] = G[0] + H[0]; a.oy[i] = G[1] + H[1]; a.oz[i] = G[2] + H[2];
         }
-        if (a.Gx) { a.Gx[i] = G[0]; a.Gy[i] = G[1]; a.Gz[i] = G[2]; }
-        if (a.Hx) { a.Hx[i] = H[0]; a.Hy[i] = H[1]; a.Hz[i] = H[2]; }
+        if (a.Gx && group != 1) { a.Gx[i] = G[0]; a.Gy[i] = G[1]; a.Gz[i] = G[2]; }
+        if (a.Hx && group != 2) { a.Hx[i] = H[0]; a.Hy[i] = H[1]; a.Hz[i] = H[2]; }
     }
     double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);

@@ -211,6 +211,7 @@ struct mdx_batch {
     KPlan<MdArgs> md_kd, md_sh, md_ke, md_th, md_gc, md_tr;
     hipGraph_t md_graph = nullptr;   // one MD step (no list rebuild)
     hipGraphExec_t md_exec = nullptr;
+    int md_exec_kind = -1;           // integrator * 65536 + nfreq of md_exec
     double *d_trace = nullptr;
     size_t trace_cap = 0;
     double *d_energy = nullptr;      // [K][Q_N]
@@ -1586,6 +1587,8 @@ MdArgs md_args(mdx_batch *b, int s, double *trace, int cap) {
     a.ox = h->d_o; a.oy = h->d_o + n; a.oz = h->d_o + 2 * (size_t)n;
     a.ux = h->d_u; a.uy = h->d_u + n; a.uz = h->d_u + 2 * (size_t)n;
     a.fx = h->d_out; a.fy = h->d_out + n; a.fz = h->d_out + 2 * (size_t)n;
+    a.gx = h->d_out + 3 * (size_t)n; a.gy = a.gx + n; a.gz = a.gx + 2 * (size_t)n;
+    a.hx = h->d_out + 6 * (size_t)n; a.hy = a.hx + n; a.hz = a.hx + 2 * (size_t)n;
     a.minv = h->d_minv;
     a.mass = h->d_mass;
     a.mstart = h->d_mstart;
@@ -1593,6 +1596,8 @@ MdArgs md_args(mdx_batch *b, int s, double *trace, int cap) {
     a.cons = h->d_cons;
     a.cons_len2 = h->d_cons_len2;
     a.dt = h->md.dt;
+    a.dts = h->md.integrator == MDX_MD_RESPA ? 0.5 * h->md.dt : h->md.dt;
+    a.dti = h->md.dt / std::max(1, h->md.nfreq);
     a.accel = h->md.accel_factor;
     a.tol2 = h->md.shake_tol * h->md.shake_tol;
     a.bdtol = h->md.shake_bdtol;
@@ -1654,6 +1659,84 @@ mdx_status md_plan(mdx_batch *b, int nsteps, bool want_trace) {
     return MDX_OK;
 }
 
+// nsteps long steps of the multiple-time-step algorithm DOUBLE
+// (mdstep.f:281-530) for every system; a list rebuild falls before the
+// force call of short step nfreq / 2 + 1 of every ICHNB-th long step
+// (mdstep.f:389-390).  Without a rebuild the whole long step (nfreq force
+// calls) is one captured hipGraph.
+mdx_status batch_respa_run(mdx_batch *b, int nsteps, int nfreq, double *trace) {
+    hipStream_t st = b->stream;
+    const int nf2 = nfreq / 2 + 1;
+    auto scaling1 = [&]() {
+        launch(b, b->md_ke, 256, 0, k_md_ke<false>);
+        launch(b, b->md_th, 256, 0, k_md_thermo);
+        launch(b, b->md_kd, 256, 0, k_md_scale);
+    };
+    auto scaling2 = [&]() {
+        launch(b, b->md_ke, 256, 0, k_md_ke<true>);
+        launch(b, b->md_th, 256, 0, k_md_thermo);
+    };
+    for (int step = 0; step < nsteps; ++step) {
+        bool rebuild = false;
+        for (mdx_engine *h : b->sys)
+            rebuild = rebuild || (h->md.rebuild_every > 0 && (h->mstep + 1) % h->md.rebuild_every == 0);
+        if (!rebuild && b->use_graph) {
+            CHK(batch_prepare(b, MDX_FORCES_ALL, false));
+            if (!b->md_exec) {
+                HIPCHK(b, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                scaling1();
+                launch(b, b->md_kd, 256, 0, k_md_kick_slow);
+                for (int k = 1; k <= nfreq; ++k) {
+                    launch(b, b->md_kd, 256, 0, k_md_kick_drift_fast);
+                    launch(b, b->md_gc, 256, 0, k_md_getcom);
+                    launch_call(b, k < nfreq ? MDX_FORCES_FAST : MDX_FORCES_ALL, false, false);
+                    launch(b, b->md_kd, 256, 0, k_md_kick_fast);
+                }
+                launch(b, b->md_kd, 256, 0, k_md_kick_slow);
+                scaling2();
+                launch(b, b->md_tr, 64, 0, k_md_trace);
+                HIPCHK(b, hipStreamEndCapture(st, &b->md_graph));
+                HIPCHK(b, hipGraphInstantiate(&b->md_exec, b->md_graph, nullptr, nullptr, 0));
+            }
+            HIPCHK(b, hipGraphLaunch(b->md_exec, st));
+            for (mdx_engine *h : b->sys) h->mstep += 1;
+            continue;
+        }
+        scaling1();
+        launch(b, b->md_kd, 256, 0, k_md_kick_slow);
+        for (int k = 1; k <= nfreq; ++k) {
+            launch(b, b->md_kd, 256, 0, k_md_kick_drift_fast);
+            launch(b, b->md_gc, 256, 0, k_md_getcom);
+            if (k == nf2) {
+                bool synced = false;
+                for (mdx_engine *h : b->sys)
+                    if (h->md.rebuild_every > 0 && (h->mstep + 1) % h->md.rebuild_every == 0) {
+                        if (!synced) {
+                            HIPCHK(b, hipStreamSynchronize(st));
+                            synced = true;
+                        }
+                        mdx_status s = mdx_build_list(h);
+                        if (s != MDX_OK) {
+                            b->err = h->err;
+                            return s;
+                        }
+                    }
+            }
+            CHK(batch_launch(b, k < nfreq ? MDX_FORCES_FAST : MDX_FORCES_ALL, false, false));
+            launch(b, b->md_kd, 256, 0, k_md_kick_fast);
+        }
+        launch(b, b->md_kd, 256, 0, k_md_kick_slow);
+        scaling2();
+        launch(b, b->md_tr, 64, 0, k_md_trace);
+        for (mdx_engine *h : b->sys) h->mstep += 1;
+    }
+    HIPCHK(b, hipStreamSynchronize(st));
+    HIPCHK(b, hipGetLastError());
+    if (trace && nsteps > 0)
+        HIPCHK(b, hipMemcpy(trace, b->d_trace, sizeof(double) * b->sys.size() * nsteps * kTrace, hipMemcpyDeviceToHost));
+    return MDX_OK;
+}
+
 // nsteps leap-frog steps of every system; lists are rebuilt (on the
 // systems' own streams, after the batch stream drains) when a system's
 // MSTEP reaches a multiple of its ICHNB.
@@ -1666,14 +1749,29 @@ mdx_status batch_md_run(mdx_batch *b, int nsteps, double *trace) {
         }
     for (mdx_engine *h : b->sys)
         if (h->stream != b->stream) hipStreamSynchronize(h->stream);
+    const int integrator = b->sys[0]->md.integrator, nfreq = b->sys[0]->md.nfreq;
+    for (mdx_engine *h : b->sys)
+        if (h->md.integrator != integrator || (integrator == MDX_MD_RESPA && h->md.nfreq != nfreq)) {
+            b->err = "the systems of a batch must share the integrator and nfreq";
+            return MDX_ERR_ARG;
+        }
     CHK(md_plan(b, std::max(nsteps, 1), trace != nullptr));
+    const int kind = integrator * 65536 + (integrator == MDX_MD_RESPA ? nfreq : 0);
+    if (b->md_exec && b->md_exec_kind != kind) {
+        hipGraphExecDestroy(b->md_exec);
+        hipGraphDestroy(b->md_graph);
+        b->md_exec = nullptr;
+        b->md_graph = nullptr;
+    }
+    b->md_exec_kind = kind;
     for (mdx_engine *h : b->sys)
         HIPCHK(b, hipMemsetAsync(h->d_mdstate + MD_STEP, 0, sizeof(double), b->stream));
+    if (integrator == MDX_MD_RESPA) return batch_respa_run(b, nsteps, nfreq, trace);
     hipStream_t st = b->stream;
     auto md_kernels = [&]() {
         launch(b, b->md_kd, 256, 0, k_md_kick_drift);
         launch(b, b->md_sh, 64, 0, k_md_shake);
-        launch(b, b->md_ke, 256, 0, k_md_scale_ke);
+        launch(b, b->md_ke, 256, 0, k_md_ke<true>);
         launch(b, b->md_th, 256, 0, k_md_thermo);
         launch(b, b->md_gc, 256, 0, k_md_getcom);
     };
@@ -1848,7 +1946,9 @@ mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, 
         return MDX_ERR_STATE;
     }
     if (!(p->dt > 0.0) || !(p->ndof > 0.0) || !(p->boltz > 0.0) || p->ncons < 0 ||
-        (p->ncons > 0 && (!p->cons_i || !p->cons_j || !p->cons_len)) || p->shake_maxit < 0) {
+        (p->ncons > 0 && (!p->cons_i || !p->cons_j || !p->cons_len)) || p->shake_maxit < 0 ||
+        (p->integrator != MDX_MD_LEAPFROG && p->integrator != MDX_MD_RESPA) ||
+        (p->integrator == MDX_MD_RESPA && (p->nfreq < 1 || p->nfreq > 1000 || p->ncons > 0))) {
         h->err = "invalid MD parameters";
         return MDX_ERR_ARG;
     }
@@ -1893,7 +1993,7 @@ mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, 
     h->ncons = p->ncons;
     h->mstep = p->step0;
     h->md_on = true;
-    // SULTAN's start: list and forces at the initial positions
+    // SULTAN's and DOUBLE's start: list and forces at the initial positions
     CHK(mdx_build_list(h));
     mdx_batch *b = h->self;
     b->timing = false;

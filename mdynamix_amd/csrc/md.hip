@@ -13,6 +13,18 @@
 //   forces at t + dt (the force-call kernels)
 // Velocities are in length per time (the reference keeps momenta in
 // internal units; the update V += dt F/m, S += dt V is the same map).
+//
+// The multiple-time-step algorithm DOUBLE (mdstep.f:281-530) uses the same
+// thermostat kernels with the half long step as their step (SCALING(1):
+// k_md_ke<false>, k_md_thermo, k_md_scale; SCALING(2): k_md_ke<true>,
+// k_md_thermo; mdstep.f:673-679, 776-791 and 806-818, 895-901 for the
+// Nose-Hoover case) and
+//   V += dt/2 a(G)                                  (k_md_kick_slow)
+//   nfreq times: V += dti/2 a(H); S += dti V        (k_md_kick_drift_fast)
+//                GETCOM, fast (or, last, all) forces
+//                V += dti/2 a(H)                    (k_md_kick_fast)
+//   V += dt/2 a(G)                                  (k_md_kick_slow)
+// with G the slow and H the fast forces, dti = dt / nfreq.
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -174,13 +186,48 @@ __device__ inline void md_shake_body(const MdArgs &a, int blk) {
     }
 }
 
+// Multiple time step kicks and drift (DOUBLE, mdstep.f:346-353, 364-376,
+// 404-411, 423-430).
+__device__ inline void md_kick_slow_body(const MdArgs &a, int blk) {
+    const int i = blk * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const double s = 0.5 * a.dt * a.accel * a.minv[i];
+    a.vx[i] = a.vx[i] + s * a.gx[i];
+    a.vy[i] = a.vy[i] + s * a.gy[i];
+    a.vz[i] = a.vz[i] + s * a.gz[i];
+}
+
+__device__ inline void md_kick_fast_body(const MdArgs &a, int blk, bool drift) {
+    const int i = blk * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const double s = 0.5 * a.dti * a.accel * a.minv[i];
+    const double vx = a.vx[i] + s * a.hx[i], vy = a.vy[i] + s * a.hy[i], vz = a.vz[i] + s * a.hz[i];
+    a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
+    if (drift) {
+        a.x[i] = vx * a.dti + a.x[i];
+        a.y[i] = vy * a.dti + a.y[i];
+        a.z[i] = vz * a.dti + a.z[i];
+    }
+}
+
+// V *= 1 - SC dts (Nose-Hoover velocity scaling, mdstep.f:781-789, 812-819).
+__device__ inline void md_scale_body(const MdArgs &a, int blk) {
+    const int i = blk * 256 + threadIdx.x;
+    if (i >= a.n || !a.nvt) return;
+    const double scv = 1.0 - a.state[MD_SC] * a.dts;
+    a.vx[i] *= scv; a.vy[i] *= scv; a.vz[i] *= scv;
+}
+
+// Kinetic energy partials; with SCALE the Nose-Hoover velocity scaling
+// first.
+template <bool SCALE>
 __device__ inline void md_scale_ke_body(const MdArgs &a, int blk, double *red) {
     const int i = blk * 256 + threadIdx.x;
     double v2 = 0.0;
     if (i < a.n) {
         double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-        if (a.nvt) {
-            const double scv = 1.0 - a.state[MD_SC] * a.dt;
+        if (SCALE && a.nvt) {
+            const double scv = 1.0 - a.state[MD_SC] * a.dts;
             vx *= scv; vy *= scv; vz *= scv;
             a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
         }
@@ -209,7 +256,7 @@ __device__ inline void md_thermo_body(const MdArgs &a, double *red) {
         const double temp = 2.0 * ekin / (a.ndof * a.boltz);
         a.state[MD_EKIN] = ekin;
         a.state[MD_TEMP] = temp;
-        if (a.nvt) a.state[MD_SC] = a.state[MD_SC] + (temp / a.t0 - 1.0) * a.dqt * a.dt;
+        if (a.nvt) a.state[MD_SC] = a.state[MD_SC] + (temp / a.t0 - 1.0) * a.dqt * a.dts;
     }
 }
 
@@ -260,10 +307,32 @@ __global__ __launch_bounds__(64) void k_md_shake(const MdArgs *__restrict__ args
     md_shake_body(args[m.x], m.y);
 }
 
-__global__ __launch_bounds__(256) void k_md_scale_ke(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+template <bool SCALE>
+__global__ __launch_bounds__(256) void k_md_ke(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
     __shared__ double red[256];
     const int2 m = map[blockIdx.x];
-    md_scale_ke_body(args[m.x], m.y, red);
+    md_scale_ke_body<SCALE>(args[m.x], m.y, red);
+}
+
+__global__ __launch_bounds__(256) void k_md_scale(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_scale_body(args[m.x], m.y);
+}
+
+__global__ __launch_bounds__(256) void k_md_kick_slow(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_kick_slow_body(args[m.x], m.y);
+}
+
+__global__ __launch_bounds__(256) void k_md_kick_drift_fast(const MdArgs *__restrict__ args,
+                                                            const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_kick_fast_body(args[m.x], m.y, true);
+}
+
+__global__ __launch_bounds__(256) void k_md_kick_fast(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_kick_fast_body(args[m.x], m.y, false);
 }
 
 __global__ __launch_bounds__(256) void k_md_thermo(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {

@@ -346,12 +346,16 @@ struct MdArgs {
     double *ox, *oy, *oz;          // constrained positions at t (OX, mdstep.f:82-89)
     double *ux, *uy, *uz;          // unconstrained positions at t + dt
     const double *fx, *fy, *fz;    // total force F at t
+    const double *gx, *gy, *gz;    // slow forces G (multiple time step)
+    const double *hx, *hy, *hz;    // fast forces H
     const double *minv, *mass;     // 1/m and m per atom
     const int *mstart;             // atoms of molecule m: [mstart[m], mstart[m+1])
     const int *cstart;             // constraints of molecule m
     const int2 *cons;              // constrained atom pairs
     const double *cons_len2;       // RB^2
     double dt, accel, tol2, bdtol;
+    double dts;                    // thermostat step: dt (leap-frog), dt / 2 (multiple time step)
+    double dti;                    // short step dt / nfreq (multiple time step)
     int maxit, nvt;
     Box box;
     double ekin_factor, boltz, ndof, t0, dqt;

@@ -406,6 +406,14 @@ def cfg_small(seed: int = 2024) -> System:
                       ewald_a=2.8, ewald_b=9.95, shake=True)
 
 
+def cfg_flex_small(seed: int = 2030) -> System:
+    """216 flexible SPC-FW waters + 2 Na+ + 2 Cl- in a 19 A box, R_cut 9 A,
+    R_short 5 A, no constraints: the multiple-time-step integrator with a
+    2 fs long step split into 4 short steps."""
+    return water_nacl(216, 2, 19.0, seed, name="f216", water=spcfw_water(), rcut=9.0, rshort=5.0,
+                      ewald_a=2.8, ewald_b=9.0, shake=False, nfreq=4, dt_fs=2.0)
+
+
 def cfg_large(seed: int = 2025) -> System:
     """32,000 SPC/E + 87 Na+ + 87 Cl- (96,174 atoms) in a 98.7 A box,
     R_cut 12 A, skin 1 A, alpha*R_cut=3.2 and B=11.5 for ~1e-5 relative

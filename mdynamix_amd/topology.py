@@ -284,7 +284,9 @@ def md_params(system, tolerance: float = 1e-6) -> Dict:
     molecule types -- ISHEJK, input.f:292-296 -- in .mmol order, as SHEJK
     walks them, mdstep.f:1115-1140),
     the degrees of freedom FNST (main.f:297-348: 3 per atom, minus one per
-    constraint, minus 3) and the Nose-Hoover time constant."""
+    constraint, minus 3) and the Nose-Hoover time constant.  Without SHAKE
+    the integrator is the multiple-time-step DOUBLE with the system's NFREQ
+    (the reference chooses it the same way, main.f:918-926)."""
     ci, cj, ln = [], [], []
     off = 0
     ndof = 0.0
@@ -306,4 +308,5 @@ def md_params(system, tolerance: float = 1e-6) -> Dict:
                 temperature=system.temperature, tau_t=30.0, sc0=0.0, ndof=ndof, shake_tol=tolerance,
                 shake_bdtol=float(np.float32(0.0001)), shake_maxit=2000, rebuild_every=system.rebuild_every,
                 step0=0, cons_i=np.array(ci, np.int32), cons_j=np.array(cj, np.int32),
-                cons_len=np.array(ln, np.float64))
+                cons_len=np.array(ln, np.float64),
+                integrator=0 if system.shake else 1, nfreq=1 if system.shake else int(system.nfreq))

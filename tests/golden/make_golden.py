@@ -9,6 +9,7 @@ virial scalar, per-type breakdowns).  Fixtures are data only.
 
     python tests/golden/make_golden.py        # rewrites tests/golden/*.npz
     python tests/golden/make_golden.py traj   # trajectory fixtures only
+    python tests/golden/make_golden.py traj traj_spcfw_respa   # one of them
 """
 from __future__ import annotations
 
@@ -52,16 +53,18 @@ def list_digest(a: np.ndarray, b: np.ndarray) -> str:
 
 
 # trajectory fixtures: name -> (builder, step counts); the reference
-# integrator (oracle/ref/mdx_traj.f around SULTAN) from the reference's own
-# start state
+# integrator (oracle/ref/mdx_traj.f around SULTAN with constraints, DOUBLE
+# without) from the reference's own start state
 TRAJ_CASES = {
     "traj_water_508": (systems.cfg_small, (1, 20)),
+    "traj_spcfw_respa": (systems.cfg_flex_small, (1, 20)),
 }
 
 
 def parse_steps(stdout: str) -> np.ndarray:
-    """Per-step lines of SULTAN (mdstep.f:187-190): MSTEP, POTEN, PENS,
-    PEL, ETOT, TEMP, TRYCKM, DENS."""
+    """Per-step lines of SULTAN (mdstep.f:187-190) or DOUBLE
+    (mdstep.f:489-490): MSTEP, POTEN, PENS, PEL, ETOT, TEMP, TRYCK(M),
+    DENS."""
     rows = []
     for line in stdout.splitlines():
         f = line.split()
@@ -73,8 +76,10 @@ def parse_steps(stdout: str) -> np.ndarray:
     return np.array(rows)
 
 
-def main_traj() -> None:
+def main_traj(names=None) -> None:
     for name, (builder, counts) in TRAJ_CASES.items():
+        if names and name not in names:
+            continue
         s = builder()
         out = {}
         for n in counts:
@@ -99,8 +104,8 @@ def main_traj() -> None:
 
 
 def main() -> None:
-    if sys.argv[1:] == ["traj"]:
-        main_traj()
+    if sys.argv[1:2] == ["traj"]:
+        main_traj(sys.argv[2:])
         return
     main_traj()
     for name, (builder, lam, full_lists) in CASES.items():
