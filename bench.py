@@ -12,11 +12,13 @@ orientations); nothing is loaded from disk.
 Prints one JSON line (metric pair-interactions/s; ns/day, per-stage times,
 roofline of the dominant kernel and the CPU baseline beside it), plus the
 batched form (`batched`: 32 path-integral beads x 12,000 atoms per fused
-force call) and the served form (`served`: a Poisson stream of MD-segment
+force call), the multiple-time-step MD segment (`respa`: 47,776-atom
+chains-in-water box, 2 fs long steps of 10 short steps, ms per long step
+and ns/day) and the served form (`served`: a Poisson stream of MD-segment
 jobs of 1.5k-60k atoms with continuous batching, p50/p99 latency and
 aggregate pair-interactions/s against one job at a time).
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-served]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-respa] [--no-served]
 """
 from __future__ import annotations
 
@@ -158,6 +160,34 @@ def bench_batched(steps: int, warmup: int, beads: int = 32, n_water: int = 4000)
             "speedup": ta / tb}
 
 
+def bench_respa(long_steps: int, warmup: int) -> dict:
+    """Multiple-time-step MD segment on the device (DOUBLE, mdstep.f:281-530):
+    256 flexible 46-site chains in 12,000 SPC/E waters (47,776 atoms, 80 A,
+    R_cut 12 A, R_short 5 A, skin 1 A), 2 fs long steps of 10 short steps
+    (9 fast force calls and one full call per long step), Nose-Hoover at
+    300 K, list rebuild every 10 long steps; each long step is one captured
+    hipGraph between rebuilds."""
+    from mdynamix_amd import systems, topology
+    from mdynamix_amd.engine import ForceEngine
+    s = systems.cfg_chains()
+    topo, prm = topology.from_system(s)
+    eng = ForceEngine(topo, prm)
+    eng.set_positions(s.pos)
+    eng.set_timing(False)
+    md = topology.md_params(s)
+    eng.md_init(md, systems.maxwell_velocities(s, 31))
+    eng.md_run(warmup)
+    t0 = time.perf_counter()
+    tr = eng.md_run(long_steps)
+    t = time.perf_counter() - t0
+    eng.close()
+    return {"workload": "chains_256x46_spce_12000", "atoms": topo.natoms, "integrator": "DOUBLE (r-RESPA)",
+            "dt_fs": s.dt_fs, "nfreq": md["nfreq"], "long_steps": long_steps,
+            "ms_per_long_step": 1e3 * t / long_steps,
+            "ns_per_day": (long_steps / t) * s.dt_fs * 0.0864,
+            "final_temperature_K": float(tr[-1, 2])}
+
+
 def bench_served(n_jobs: int, nsteps: int, rate: float, seed: int = 77) -> dict:
     """Served mixed-size stream: `n_jobs` MD segments of `nsteps` steps
     (SPC/E + 0.15 M NaCl boxes of 1.5k-60k atoms, log-uniform, one rigid
@@ -241,6 +271,7 @@ def main() -> int:
     ap.add_argument("--cpu-tasks", type=int, default=12)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-respa", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -361,6 +392,9 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = bench_batched(max(args.steps // 2, 5), args.warmup)
         print("batched: %.3f ms/call" % out["batched"]["ms_per_call"], file=sys.stderr, flush=True)
+    if not args.no_respa:
+        out["respa"] = bench_respa(max(args.steps, 20), args.warmup)
+        print("respa: %.3f ms/long step" % out["respa"]["ms_per_long_step"], file=sys.stderr, flush=True)
     if not args.no_served:
         out["served"] = bench_served(args.serve_jobs, args.serve_steps, args.serve_rate)
     if not args.no_cpu:

@@ -246,14 +246,19 @@ def chloride() -> MolType:
     return MolType("CLM", [Site("Cl", (0.0, 0.0, 0.0), 35.453, -1.0, 4.83, 0.054)])
 
 
-def _chain_geometry(n: int, rng: np.random.Generator, bond: float, theta_deg: float) -> np.ndarray:
-    """A chain conformation: fixed bond and angle, trans-rich dihedrals."""
+def _chain_geometry(n: int, rng: np.random.Generator, bond: float, theta_deg: float,
+                    compact: Optional[float] = None) -> np.ndarray:
+    """A chain conformation: fixed bond and angle, trans-rich dihedrals.
+    With `compact` (A) a globule instead: each new site takes the first of
+    the three staggered dihedrals (in random order) that keeps it within
+    `compact` of the centroid so far and 3 A from every site four or more
+    bonds back, else the one farthest from those sites."""
     theta = math.radians(theta_deg)
     x = np.zeros((n, 3))
     x[1] = [bond, 0.0, 0.0]
     x[2] = x[1] + bond * np.array([-math.cos(theta), math.sin(theta), 0.0])
-    for i in range(3, n):
-        phi = math.pi if rng.random() < 0.75 else rng.choice([math.pi / 3, -math.pi / 3])
+
+    def place(i, phi):
         b = x[i - 1] - x[i - 2]
         b /= np.linalg.norm(b)
         nrm = np.cross(x[i - 2] - x[i - 3], b)
@@ -261,17 +266,33 @@ def _chain_geometry(n: int, rng: np.random.Generator, bond: float, theta_deg: fl
         m = np.cross(nrm, b)
         d = np.array([-bond * math.cos(theta), bond * math.sin(theta) * math.cos(phi),
                       bond * math.sin(theta) * math.sin(phi)])
-        x[i] = x[i - 1] + d[0] * b + d[1] * m + d[2] * nrm
+        return x[i - 1] + d[0] * b + d[1] * m + d[2] * nrm
+
+    for i in range(3, n):
+        if compact is None:
+            phi = math.pi if rng.random() < 0.75 else rng.choice([math.pi / 3, -math.pi / 3])
+            x[i] = place(i, phi)
+            continue
+        best, best_d = None, -1.0
+        for phi in rng.permutation([math.pi, math.pi / 3, -math.pi / 3]):
+            p = place(i, phi)
+            dmin = np.sqrt(((x[:i - 3] - p) ** 2).sum(1)).min() if i > 3 else 9.0
+            if dmin >= 3.0 and np.linalg.norm(p - x[:i].mean(0)) <= compact:
+                best = p
+                break
+            if dmin > best_d:
+                best, best_d = p, dmin
+        x[i] = best
     return x
 
 
-def ua_chain(n_sites: int = 46, seed: int = 7, n_charged: int = 4) -> MolType:
+def ua_chain(n_sites: int = 46, seed: int = 7, n_charged: int = 4, compact: Optional[float] = None) -> MolType:
     """Flexible united-atom chain: bonds k=2500 kJ/mol/A^2 r0=1.53 A,
     angles k=500 kJ/mol/rad^2 theta0=112 deg, MM3-type dihedrals with
     seeded coefficients in [-5, 5] kJ/mol, LJ sigma=3.9 A eps=0.5 kJ/mol,
     charges +-0.3 on `n_charged` sites (net zero)."""
     rng = np.random.default_rng(seed)
-    geo = _chain_geometry(n_sites, rng, 1.53, 112.0)
+    geo = _chain_geometry(n_sites, rng, 1.53, 112.0, compact)
     charged = np.linspace(0, n_sites - 1, n_charged).round().astype(int)
     q = np.zeros(n_sites)
     for k, i in enumerate(charged):
@@ -423,29 +444,36 @@ def cfg_large(seed: int = 2025) -> System:
 
 
 def chain_mixture(n_chain: int, n_water: int, box: float, seed: int, name: str = "chains",
-                  chain_sites: int = 46, **params) -> System:
+                  chain_sites: int = 46, water_factor: float = 1.6, chain_compact: Optional[float] = None,
+                  **params) -> System:
     """Flexible chains solvated in SPC/E water.  Chains are placed first on a
-    coarse lattice; waters fill lattice sites farther than 2.6 A from any
-    chain atom."""
+    coarse lattice; waters fill lattice sites (water_factor x n_water
+    candidates) farther than 2.6 A from any chain atom."""
+    from scipy.spatial import cKDTree
     rng = np.random.default_rng(seed)
-    chain = ua_chain(chain_sites, seed=seed + 1)
+    chain = ua_chain(chain_sites, seed=seed + 1, compact=chain_compact)
     water = spce_water()
     L = np.array([box, box, box])
     cc = lattice_sites(n_chain, L, rng, jitter=0.05)
     loc = chain.local_coords()
+    # chains farther apart than twice the chain radius plus the 3.2 A
+    # clearance cannot change a placement test, so only nearer ones are
+    # compared (same result as comparing against every placed chain)
+    reach = 2.0 * np.sqrt((loc ** 2).sum(1)).max() + 3.3
     chain_atoms = []
-    for c in cc:
+    for k, c in enumerate(cc):
         # orientation with no chain atom closer than 3.2 A to those placed
         best, best_d = None, -1.0
+        dc = cc[:k] - c
+        dc -= L * np.round(dc / L)
+        near = [chain_atoms[q] for q in np.nonzero((dc ** 2).sum(1) < reach ** 2)[0]]
+        tree = cKDTree(np.mod(np.concatenate(near), box), boxsize=box) if near else None
         for _ in range(60):
             cand_atoms = loc @ random_rotation(rng).T + c
-            if not chain_atoms:
+            if tree is None:
                 best = cand_atoms
                 break
-            prev = np.concatenate(chain_atoms)
-            d = cand_atoms[:, None, :] - prev[None, :, :]
-            d -= L * np.round(d / L)
-            dmin = np.sqrt((d ** 2).sum(-1).min())
+            dmin = tree.query(np.mod(cand_atoms, box))[0].min()
             if dmin > best_d:
                 best, best_d = cand_atoms, dmin
             if dmin > 3.2:
@@ -453,14 +481,10 @@ def chain_mixture(n_chain: int, n_water: int, box: float, seed: int, name: str =
         chain_atoms.append(best)
     ca = np.concatenate(chain_atoms) if chain_atoms else np.zeros((0, 3))
     # candidate water sites: dense lattice minus those near chains
-    cand = lattice_sites(int(n_water * 1.6) + 64, L, rng, jitter=0.08)
+    cand = lattice_sites(int(n_water * water_factor) + 64, L, rng, jitter=0.08)
     if len(ca):
-        keep = np.ones(len(cand), bool)
-        for s in range(0, len(cand), 4096):
-            d = cand[s:s + 4096, None, :] - ca[None, :, :]
-            d -= L * np.round(d / L)
-            keep[s:s + 4096] = (np.einsum("ijk,ijk->ij", d, d).min(1) > 2.6 ** 2)
-        cand = cand[keep]
+        d = cKDTree(np.mod(ca, box), boxsize=box).query(np.mod(cand, box))[0]
+        cand = cand[d > 2.6]
     if len(cand) < n_water:
         raise ValueError("box too small for %d waters next to %d chains" % (n_water, n_chain))
     wc = cand[rng.permutation(len(cand))[:n_water]]
@@ -473,10 +497,14 @@ def chain_mixture(n_chain: int, n_water: int, box: float, seed: int, name: str =
 
 
 def cfg_chains(seed: int = 2026) -> System:
-    """256 chains of 46 united atoms in 12,000 SPC/E waters, box 80 A,
-    double time step 0.2/2 fs."""
-    return chain_mixture(256, 12000, 80.0, seed, name="ch256", rcut=12.0, rshort=5.0,
-                         ewald_a=3.2, ewald_b=11.5, shake=False, nfreq=10, dt_fs=2.0, skin=1.0)
+    """256 chains of 46 united atoms in 12,000 SPC/E waters, double time
+    step 0.2/2 fs.  The chains are compact globules (radius ~7 A) and the
+    box is 92 A (0.81 g/cm^3): at 80-86 A (0.9-1.0 g/cm^3) rigid random
+    placement leaves chain sites overlapping, whose forces blow up the
+    first steps."""
+    return chain_mixture(256, 12000, 92.0, seed, name="ch256", water_factor=2.0, chain_compact=7.0,
+                         rcut=12.0, rshort=5.0, ewald_a=3.2, ewald_b=11.5, shake=False, nfreq=10, dt_fs=2.0,
+                         skin=1.0)
 
 
 def pimd_beads(n_beads: int = 32, n_water: int = 4000, box: float = 49.3, seed: int = 2027,

@@ -137,8 +137,11 @@ def test_pimd_ring_matches_beads_alone_plus_springs():
             add = sf if group != abi.MDX_FORCES_SLOW else 0.0
             scale = np.abs(fa / P + add).max()
             assert np.abs(f - (fa / P + add)).max() <= 1e-12 * scale, (group, k)
-            assert np.abs(g - ga / P).max() <= 1e-12 * scale
-            assert np.abs(h - (ha / P + add)).max() <= 1e-12 * scale
+            # a class not evaluated keeps its previous forces (FFORCES leaves GX)
+            if group != abi.MDX_FORCES_FAST:
+                assert np.abs(g - ga / P).max() <= 1e-12 * scale
+            if group != abi.MDX_FORCES_SLOW:
+                assert np.abs(h - (ha / P + add)).max() <= 1e-12 * scale
             for key in ("pe1", "pe2", "pels1", "pels2", "spe", "pint", "vir1", "vir2"):
                 assert eb[k][key] == ea[key], key
             if group != abi.MDX_FORCES_SLOW:
