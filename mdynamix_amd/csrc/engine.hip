@@ -105,7 +105,7 @@ struct mdx_engine {
     double *d_anchor = nullptr;
     float *d_ext = nullptr;
     int cand_cap = 0, list_cap = 0;
-    int *d_cand = nullptr, *d_ncand = nullptr, *d_nlist = nullptr, *d_overflow = nullptr;
+    int *d_cand = nullptr, *d_ncand = nullptr, *d_nlist = nullptr, *d_nshort = nullptr, *d_overflow = nullptr;
     PairEntry *d_list = nullptr;
     float4 *d_xq = nullptr;
     int2 *d_aux = nullptr;
@@ -505,6 +505,7 @@ mdx_status alloc_clusters(mdx_engine *h) {
     CHK(dalloc(h, &h->d_ext, (size_t)h->nclu_cap * 3));
     CHK(dalloc(h, &h->d_ncand, h->nclu_cap));
     CHK(dalloc(h, &h->d_nlist, h->nclu_cap));
+    CHK(dalloc(h, &h->d_nshort, h->nclu_cap));
     CHK(dalloc(h, &h->d_overflow, 1));
     CHK(dalloc(h, &h->d_xq, (size_t)h->nclu_cap * kClu));
     CHK(dalloc(h, &h->d_aux, (size_t)h->nclu_cap * kClu));
@@ -623,6 +624,7 @@ void fill_args(mdx_engine *h) {
     pa.nclu = h->have_list ? h->nclu : 0;
     pa.list = h->d_list;
     pa.nlist = h->d_nlist;
+    pa.nshort = h->d_nshort;
     pa.list_cap = h->list_cap;
     pa.xq = h->d_xq;
     pa.aux = h->d_aux;
@@ -1125,8 +1127,8 @@ mdx_status mdx_build_list(mdx_engine *h) {
         k_masks<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_cand, h->d_ncand, h->cand_cap, h->d_clu_atom, h->d_anchor, h->d_x,
                                         h->d_y, h->d_z, h->d_mol, h->d_moltype, h->d_type_ee, h->d_excl_start,
                                         h->d_excl_atom, h->d_excl_is14, h->box, rl2,
-                                        h->p.rshort * h->p.rshort, h->d_list, h->d_nlist, h->list_cap,
-                                        h->d_overflow);
+                                        h->p.rshort * h->p.rshort, h->d_list, h->d_nlist, h->d_nshort,
+                                        h->list_cap, h->d_overflow);
         int ovf = 0;
         HIPCHK(h, hipMemcpyAsync(&ovf, h->d_overflow, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(h, hipStreamSynchronize(st));
@@ -1161,19 +1163,24 @@ mdx_status mdx_build_list(mdx_engine *h) {
     return MDX_OK;
 }
 
+// Entry k of an i-cluster's list: the ns entries with a short-class pair
+// from the front, the rest from the back (k_masks).
+inline size_t list_slot(int k, int ns, int cap) { return k < ns ? (size_t)k : (size_t)(cap - 1 - (k - ns)); }
+
 mdx_status mdx_list_counts(mdx_engine *h, int64_t *n_short, int64_t *n_long) {
     if (!h) return MDX_ERR_ARG;
     if (!h->have_list) return MDX_ERR_STATE;
-    std::vector<int> nl(h->nclu);
+    std::vector<int> nl(h->nclu), ns(h->nclu);
     std::vector<PairEntry> L((size_t)h->nclu * h->list_cap);
     std::vector<signed char> cls(std::max(1, h->nspecial));
     HIPCHK(h, hipMemcpy(nl.data(), h->d_nlist, sizeof(int) * h->nclu, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ns.data(), h->d_nshort, sizeof(int) * h->nclu, hipMemcpyDeviceToHost));
     HIPCHK(h, hipMemcpy(L.data(), h->d_list, sizeof(PairEntry) * L.size(), hipMemcpyDeviceToHost));
     if (h->nspecial) HIPCHK(h, hipMemcpy(cls.data(), h->d_sp_cls, h->nspecial, hipMemcpyDeviceToHost));
     int64_t s = 0, l = 0;
     for (int c = 0; c < h->nclu; ++c)
         for (int k = 0; k < nl[c]; ++k) {
-            const PairEntry &e = L[(size_t)c * h->list_cap + k];
+            const PairEntry &e = L[(size_t)c * h->list_cap + list_slot(k, ns[c], h->list_cap)];
             s += __builtin_popcountll(e.mask & e.mshort);
             l += __builtin_popcountll(e.mask & ~e.mshort);
         }
@@ -1192,10 +1199,11 @@ mdx_status mdx_get_list(mdx_engine *h, int32_t *nbs1, int32_t *nbs2, int64_t cap
                         int32_t *nbl2, int64_t cap_long) {
     if (!h) return MDX_ERR_ARG;
     if (!h->have_list) return MDX_ERR_STATE;
-    std::vector<int> nl(h->nclu), ca((size_t)h->nclu * kClu);
+    std::vector<int> nl(h->nclu), ns(h->nclu), ca((size_t)h->nclu * kClu);
     std::vector<PairEntry> L((size_t)h->nclu * h->list_cap);
     std::vector<signed char> cls(std::max(1, h->nspecial));
     HIPCHK(h, hipMemcpy(nl.data(), h->d_nlist, sizeof(int) * h->nclu, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ns.data(), h->d_nshort, sizeof(int) * h->nclu, hipMemcpyDeviceToHost));
     HIPCHK(h, hipMemcpy(ca.data(), h->d_clu_atom, sizeof(int) * ca.size(), hipMemcpyDeviceToHost));
     HIPCHK(h, hipMemcpy(L.data(), h->d_list, sizeof(PairEntry) * L.size(), hipMemcpyDeviceToHost));
     if (h->nspecial) HIPCHK(h, hipMemcpy(cls.data(), h->d_sp_cls, h->nspecial, hipMemcpyDeviceToHost));
@@ -1213,7 +1221,7 @@ mdx_status mdx_get_list(mdx_engine *h, int32_t *nbs1, int32_t *nbs2, int64_t cap
     };
     for (int c = 0; c < h->nclu; ++c)
         for (int k = 0; k < nl[c]; ++k) {
-            const PairEntry &e = L[(size_t)c * h->list_cap + k];
+            const PairEntry &e = L[(size_t)c * h->list_cap + list_slot(k, ns[c], h->list_cap)];
             for (int lane = 0; lane < 64; ++lane) {
                 if (!((e.mask >> lane) & 1ull)) continue;
                 const int ai = ca[c * kClu + (lane >> 3)], aj = ca[e.cj * kClu + (lane & 7)];

@@ -158,8 +158,8 @@ __global__ __launch_bounds__(64) void k_masks(
     const double *__restrict__ y, const double *__restrict__ z, const int *__restrict__ atom_mol,
     const int *__restrict__ atom_moltype, const int *__restrict__ type_ee, const int *__restrict__ excl_start,
     const int *__restrict__ excl_atom, const signed char *__restrict__ excl_is14, Box b, double rl2,
-    double short2, PairEntry *__restrict__ list, int *__restrict__ nlist, int list_cap,
-    int *__restrict__ overflow) {
+    double short2, PairEntry *__restrict__ list, int *__restrict__ nlist, int *__restrict__ nshort,
+    int list_cap, int *__restrict__ overflow) {
 #pragma clang fp contract(off)
     const int ci = blockIdx.x;
     const int lane = threadIdx.x;
@@ -182,7 +182,10 @@ __global__ __launch_bounds__(64) void k_masks(
     double x1 = 0.0, y1 = 0.0, z1 = 0.0;
     if (a1 >= 0) { x1 = x[a1]; y1 = y[a1]; z1 = z[a1]; }
     int a2 = nc > 1 ? clu_atom[c2 * kClu + jl] : -1;
-    int cnt = 0;
+    // entries with a short-class pair fill the list from the front, the
+    // others from the back (in discovery order both), so a fast-class call
+    // reads only the front segment
+    int cnt = 0, cnt_s = 0;
     for (int k = 0; k < nc; ++k) {
         const int cj = c1, aj = a1;
         const double xj = x1, yj = y1, zj = z1;
@@ -209,6 +212,7 @@ __global__ __launch_bounds__(64) void k_masks(
         const unsigned long long m = __ballot(listed);
         const unsigned long long ms = __ballot(listed && shrt);
         if (m != 0ull) {
+            const int slot = ms != 0ull ? cnt_s : list_cap - 1 - (cnt - cnt_s);
             if (lane == 0 && cnt < list_cap) {
                 PairEntry e;
                 e.cj = cj;
@@ -220,13 +224,15 @@ __global__ __launch_bounds__(64) void k_masks(
                 e.sx = (float)s[0]; e.sy = (float)s[1]; e.sz = (float)s[2];
                 e.mask = m;
                 e.mshort = ms;
-                list[(size_t)ci * list_cap + cnt] = e;
+                list[(size_t)ci * list_cap + slot] = e;
             }
             ++cnt;
+            if (ms != 0ull) ++cnt_s;
         }
     }
     if (lane == 0) {
         nlist[ci] = min(cnt, list_cap);
+        nshort[ci] = min(cnt_s, list_cap);
         if (cnt > list_cap) atomicMax(overflow, cnt);
     }
 }

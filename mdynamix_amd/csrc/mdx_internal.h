@@ -169,8 +169,9 @@ struct PackArgs {
 
 struct PairArgs {
     int nclu;
-    const PairEntry *list;
-    const int *nlist;
+    const PairEntry *list;       // per i-cluster: nshort entries with a short-class pair from
+    const int *nlist;            //   the front, the other nlist - nshort from the back
+    const int *nshort;
     int list_cap;
     const float4 *xq;            // per slot: x, y, z relative to anchor; q (FP32)
     const int2 *aux;             // per slot: LJ type | ee << 16; q - (float)q as float bits

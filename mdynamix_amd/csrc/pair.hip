@@ -109,7 +109,12 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
     double E[6] = {0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
     double Vs[3] = {0, 0, 0}, Vl[3] = {0, 0, 0};
     int cnt_s = 0, cnt_l = 0;
-    const int n = sld(a.nlist)[ci];
+    // a fast-class call reads only the entries with a short-class pair (the
+    // front segment); logical entry k >= ns sits at list_cap - 1 - (k - ns)
+    const int ns = sld(a.nshort)[ci];
+    const int n = group == 1 ? ns : sld(a.nlist)[ci];
+    const int cap = a.list_cap;
+    auto ent = [ns, cap](int k) { return k < ns ? k : cap - 1 - (k - ns); };
     const MDX_CONST PairEntry *L = sld(a.list) + (size_t)ci * a.list_cap;
     const MDX_GLOBAL float4 *xq = gld(a.xq);
     const MDX_GLOBAL int2 *aux = gld(a.aux);
@@ -118,8 +123,8 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
     PairEntry e1{}, e2{};
     float4 pj1 = make_float4(0.f, 0.f, 0.f, 0.f);
     int2 aux1 = make_int2(0, 0);
-    if (n > 0) e1 = ldv(L);
-    if (n > 1) e2 = ldv(L + 1);
+    if (n > 0) e1 = ldv(L + ent(0));
+    if (n > 1) e2 = ldv(L + ent(1));
     if (n > 0) { pj1 = ldv(xq + e1.cj * kClu + jl); aux1 = ldv(aux + e1.cj * kClu + jl); }
     for (int k0 = 0; k0 < n; k0 += 16) {
         float fs[3] = {0.f, 0.f, 0.f}, fl[3] = {0.f, 0.f, 0.f};
@@ -131,7 +136,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
             const float4 pj = pj1;
             const int2 auxj = aux1;
             e1 = e2;
-            if (k + 2 < n) e2 = ldv(L + k + 2);
+            if (k + 2 < n) e2 = ldv(L + ent(k + 2));
             if (k + 1 < n) { pj1 = ldv(xq + e1.cj * kClu + jl); aux1 = ldv(aux + e1.cj * kClu + jl); }
             unsigned long long m = e.mask;
             if (group == 1) m &= e.mshort;
