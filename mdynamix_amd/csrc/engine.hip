@@ -59,6 +59,8 @@ struct mdx_engine {
     int *d_excl_start = nullptr, *d_excl_atom = nullptr;
     signed char *d_excl_is14 = nullptr;
     int *d_mstart = nullptr;
+    int *d_comgrp = nullptr;    // molecule groups of the centre-of-mass kernels
+    int ncomgrp = 0;
     std::vector<int> h_excl_start, h_excl_atom, h_mol, h_moltype, h_type_ee;
     std::vector<signed char> h_excl_is14;
     // specials and exclusion corrections
@@ -191,7 +193,7 @@ struct mdx_batch {
     int nlj_max = 1;
     bool soft = false;
     KPlan<ZeroSeg> zero;
-    KPlan<PackArgs> pack;
+    KPlan<PackArgs> pack, com;
     KPlan<PairArgs> pairs;
     KPlan<SpecialArgs> special;
     KPlan<RecipArgs> rprep, sk, skred, rforce;
@@ -617,6 +619,7 @@ void fill_args(mdx_engine *h) {
     pk.nmol = h->nmol;
     pk.user_com = h->user_com ? 1 : 0;
     pk.mstart = h->d_mstart;
+    pk.comgrp = h->d_comgrp;
     pk.mass = h->d_mass;
     pk.cx = h->d_cx; pk.cy = h->d_cy; pk.cz = h->d_cz;
 
@@ -866,6 +869,18 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
     }
     if ((s = upload(h, &h->d_mstart, mstart.data(), h->nmol + 1))) return fail(s);
     h->h_mstart = mstart;
+    // centre-of-mass groups: consecutive molecules, at most 64 of them and
+    // at most kComAtoms atoms (a larger molecule alone)
+    {
+        std::vector<int> g(1, 0);
+        for (int m = 0; m < h->nmol; ++m) {
+            const int first = g.back();
+            if (m > first && (m - first == 64 || mstart[m + 1] - mstart[first] > kComAtoms)) g.push_back(m);
+        }
+        g.push_back(h->nmol);
+        h->ncomgrp = (int)g.size() - 1;
+        if ((s = upload(h, &h->d_comgrp, g.data(), g.size()))) return fail(s);
+    }
     h->h_mass = mass;
     // special-pair candidates and exclusion-correction pairs
     std::vector<int2> et;
@@ -1323,7 +1338,7 @@ void drop_graphs(mdx_batch *b) {
 // maps = true the block maps are rebuilt too (first use, box changes).
 mdx_status plan_fill(mdx_batch *b, bool maps) {
     const int K = (int)b->sys.size();
-    b->zero.clear(); b->pack.clear(); b->pairs.clear(); b->special.clear();
+    b->zero.clear(); b->pack.clear(); b->com.clear(); b->pairs.clear(); b->special.clear();
     b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
     b->mz = 1;
     b->et.clear(); b->bonds.clear(); b->angles.clear(); b->tors.clear();
@@ -1343,7 +1358,8 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         b->zero.add(ZeroSeg{(unsigned long long *)h->d_epart, nz1}, cdiv(nz1, 256));
         b->zero.add(ZeroSeg{h->d_fix, 9LL * n}, cdiv(9LL * n, 256));
         b->zero.add(ZeroSeg{h->d_fixsp, 3LL * n}, cdiv(3LL * n, 256));
-        b->pack.add(h->a_pack, cdiv((long long)h->nclu_cap * kClu, 256) + cdiv(h->nmol, 256));
+        b->pack.add(h->a_pack, cdiv((long long)h->nclu_cap * kClu, 256));
+        b->com.add(h->a_pack, h->ncomgrp);
         b->pairs.add(h->a_pair, h->nclu_cap);
         if (h->nspecial) b->special.add(h->a_special, cdiv(h->nspecial, 256));
         b->rprep.add(h->a_recip, cdiv(n, 256));
@@ -1397,7 +1413,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         b->comb.add(ca, cdiv(n, 256));
         b->red.add(ReduceArgs{h->rows_total, h->d_epart, b->d_energy + (size_t)s * Q_N}, Q_N);
     }
-    CHK(kplan_upload(b, b->zero, maps)); CHK(kplan_upload(b, b->pack, maps));
+    CHK(kplan_upload(b, b->zero, maps)); CHK(kplan_upload(b, b->pack, maps)); CHK(kplan_upload(b, b->com, maps));
     CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->special, maps));
     CHK(kplan_upload(b, b->rprep, maps)); CHK(kplan_upload(b, b->sk, maps));
     CHK(kplan_upload(b, b->skred, maps)); CHK(kplan_upload(b, b->rforce, maps));
@@ -1425,6 +1441,7 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
     mark(0);
     launch(b, b->zero, 256, 0, k_zero);
     launch(b, b->pack, 256, 0, k_pack);
+    launch(b, b->com, 64, 0, k_com);
     mark(1);
     const size_t lj = sizeof(float2) * b->nlj_max * b->nlj_max;
     const size_t ldet = lj + sizeof(double) * (b->nmt_max + 1) * 2 * 64 + sizeof(int) * 8;
@@ -1600,6 +1617,8 @@ MdArgs md_args(mdx_batch *b, int s, double *trace, int cap) {
     a.minv = h->d_minv;
     a.mass = h->d_mass;
     a.mstart = h->d_mstart;
+    a.comgrp = h->d_comgrp;
+    a.atom_mol = h->d_mol;
     a.cstart = h->d_cstart;
     a.cons = h->d_cons;
     a.cons_len2 = h->d_cons_len2;
@@ -1647,7 +1666,7 @@ mdx_status md_plan(mdx_batch *b, int nsteps, bool want_trace) {
         b->md_sh.add(a, cdiv(h->nmol, 64));
         b->md_ke.add(a, cdiv(h->n, 256));
         b->md_th.add(a, 1);
-        b->md_gc.add(a, cdiv(h->nmol, 256));
+        b->md_gc.add(a, h->ncomgrp);
         b->md_tr.add(a, 1);
     }
     const void *before[6] = {b->md_kd.d_args, b->md_sh.d_args, b->md_ke.d_args, b->md_th.d_args, b->md_gc.d_args,
@@ -1696,7 +1715,7 @@ mdx_status batch_respa_run(mdx_batch *b, int nsteps, int nfreq, double *trace) {
                 launch(b, b->md_kd, 256, 0, k_md_kick_slow);
                 for (int k = 1; k <= nfreq; ++k) {
                     launch(b, b->md_kd, 256, 0, k_md_kick_drift_fast);
-                    launch(b, b->md_gc, 256, 0, k_md_getcom);
+                    launch(b, b->md_gc, 64, 0, k_md_getcom);
                     launch_call(b, k < nfreq ? MDX_FORCES_FAST : MDX_FORCES_ALL, false, false);
                     launch(b, b->md_kd, 256, 0, k_md_kick_fast);
                 }
@@ -1714,7 +1733,7 @@ mdx_status batch_respa_run(mdx_batch *b, int nsteps, int nfreq, double *trace) {
         launch(b, b->md_kd, 256, 0, k_md_kick_slow);
         for (int k = 1; k <= nfreq; ++k) {
             launch(b, b->md_kd, 256, 0, k_md_kick_drift_fast);
-            launch(b, b->md_gc, 256, 0, k_md_getcom);
+            launch(b, b->md_gc, 64, 0, k_md_getcom);
             if (k == nf2) {
                 bool synced = false;
                 for (mdx_engine *h : b->sys)
@@ -1781,7 +1800,7 @@ mdx_status batch_md_run(mdx_batch *b, int nsteps, double *trace) {
         launch(b, b->md_sh, 64, 0, k_md_shake);
         launch(b, b->md_ke, 256, 0, k_md_ke<true>);
         launch(b, b->md_th, 256, 0, k_md_thermo);
-        launch(b, b->md_gc, 256, 0, k_md_getcom);
+        launch(b, b->md_gc, 64, 0, k_md_getcom);
     };
     for (int step = 0; step < nsteps; ++step) {
         bool rebuild = false;
@@ -2101,7 +2120,7 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     if (!b) return MDX_OK;
     if (b->stream) hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->pairs); kplan_free(b->special);
+    kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->com); kplan_free(b->pairs); kplan_free(b->special);
     kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
     kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->angles); kplan_free(b->tors);
     kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);

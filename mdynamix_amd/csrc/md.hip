@@ -260,26 +260,66 @@ __device__ inline void md_thermo_body(const MdArgs &a, double *red) {
     }
 }
 
-// GETCOM: S = (S - R_com) + PBC(R_com), per molecule.
-__device__ inline void md_getcom_body(const MdArgs &a, int blk) {
+// GETCOM: S = (S - R_com) + PBC(R_com), per molecule.  A wavefront takes
+// one group of consecutive molecules (at most 64, at most kComAtoms atoms
+// unless one molecule is larger): their atoms pass through LDS in coalesced
+// stages, each lane adds its molecule's atoms in atom order (the
+// reference's serial sums), then the lanes move the atoms, lane = atom,
+// with their molecule's centre and image shift from LDS.
+__device__ inline void md_getcom_body(const MdArgs &a, int blk, double *sh) {
 #pragma clang fp contract(off)
-    const int m = blk * 256 + threadIdx.x;
-    if (m >= a.nmol) return;
-    const int a0 = a.mstart[m], a1 = a.mstart[m + 1];
+    const int lane = threadIdx.x;
+    const int m0 = a.comgrp[blk], m1 = a.comgrp[blk + 1];
+    const int m = m0 + lane;
+    const int A0 = a.mstart[m0], A1 = a.mstart[m1];
+    const int b0 = m < m1 ? a.mstart[m] : A1, b1 = m < m1 ? a.mstart[m + 1] : A1;
+    double *sm_ = sh, *sx = sh + kComAtoms, *sy = sh + 2 * kComAtoms, *sz = sh + 3 * kComAtoms;
     double cx = 0.0, cy = 0.0, cz = 0.0, sm = 0.0;
-    for (int i = a0; i < a1; ++i) {
-        cx = cx + a.mass[i] * a.x[i];
-        cy = cy + a.mass[i] * a.y[i];
-        cz = cz + a.mass[i] * a.z[i];
-        sm = sm + a.mass[i];
+    for (int c0 = A0; c0 < A1; c0 += kComAtoms) {
+        const int c1 = min(c0 + kComAtoms, A1);
+        for (int i = c0 + lane; i < c1; i += 64) {
+            sm_[i - c0] = a.mass[i]; sx[i - c0] = a.x[i]; sy[i - c0] = a.y[i]; sz[i - c0] = a.z[i];
+        }
+        __syncthreads();
+        for (int i = max(b0, c0); i < min(b1, c1); ++i) {
+            const double mi = sm_[i - c0];
+            cx = cx + mi * sx[i - c0];
+            cy = cy + mi * sy[i - c0];
+            cz = cz + mi * sz[i - c0];
+            sm = sm + mi;
+        }
+        __syncthreads();
     }
-    cx = cx / sm; cy = cy / sm; cz = cz / sm;
-    double px = cx, py = cy, pz = cz;
-    min_image(px, py, pz, a.box);
-    for (int i = a0; i < a1; ++i) {
-        a.x[i] = (a.x[i] - cx) + px;
-        a.y[i] = (a.y[i] - cy) + py;
-        a.z[i] = (a.z[i] - cz) + pz;
+    // per molecule: centre and image-shifted centre
+    double *ccx = sh, *ccy = sh + 64, *ccz = sh + 128, *cpx = sh + 192, *cpy = sh + 256, *cpz = sh + 320;
+    if (m < m1) {
+        cx = cx / sm; cy = cy / sm; cz = cz / sm;
+        double px = cx, py = cy, pz = cz;
+        min_image(px, py, pz, a.box);
+        ccx[lane] = cx; ccy[lane] = cy; ccz[lane] = cz;
+        cpx[lane] = px; cpy[lane] = py; cpz[lane] = pz;
+    }
+    __syncthreads();
+    // 8 atoms per lane per pass: every load of a pass before its stores
+    for (int i0 = A0; i0 < A1; i0 += 8 * 64) {
+        double xx[8], yy[8], zz[8];
+        int kk[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = min(i0 + u * 64 + lane, A1 - 1);
+            xx[u] = a.x[i]; yy[u] = a.y[i]; zz[u] = a.z[i];
+            kk[u] = a.atom_mol[i] - m0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + u * 64 + lane;
+            if (i < A1) {
+                const int k = kk[u];
+                a.x[i] = (xx[u] - ccx[k]) + cpx[k];
+                a.y[i] = (yy[u] - ccy[k]) + cpy[k];
+                a.z[i] = (zz[u] - ccz[k]) + cpz[k];
+            }
+        }
     }
 }
 
@@ -341,9 +381,10 @@ __global__ __launch_bounds__(256) void k_md_thermo(const MdArgs *__restrict__ ar
     md_thermo_body(args[m.x], red);
 }
 
-__global__ __launch_bounds__(256) void k_md_getcom(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+__global__ __launch_bounds__(64) void k_md_getcom(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ double sh[4 * kComAtoms];
     const int2 m = map[blockIdx.x];
-    md_getcom_body(args[m.x], m.y);
+    md_getcom_body(args[m.x], m.y, sh);
 }
 
 __global__ void k_md_trace(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {

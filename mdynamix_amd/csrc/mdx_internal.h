@@ -45,6 +45,7 @@ constexpr int kMaxDet = 2 * 137 + 2 * 16;
 constexpr int kWave = 64;          // wavefront size on gfx950
 constexpr int kForceTiles = 4;     // 16-atom tiles per wavefront of the reciprocal force pass
 constexpr int kForceBlock = 64 * kForceTiles;   // atoms per block (4 wavefronts) of that pass
+constexpr int kComAtoms = 512;     // LDS stage (atoms) of the centre-of-mass kernels
 constexpr int kSkRows = 64;        // k rows per structure-factor block (16 per wavefront)
 constexpr int kSkSub = 32;         // atoms per LDS stage of the structure-factor block
 
@@ -163,6 +164,7 @@ struct PackArgs {
     // molecular centres (skipped when user_com)
     int nmol, user_com;
     const int *mstart;
+    const int *comgrp;           // molecule groups [comgrp[g], comgrp[g+1]) of the COM kernel
     const double *mass;
     double *cx, *cy, *cz;
 };
@@ -351,6 +353,8 @@ struct MdArgs {
     const double *hx, *hy, *hz;    // fast forces H
     const double *minv, *mass;     // 1/m and m per atom
     const int *mstart;             // atoms of molecule m: [mstart[m], mstart[m+1])
+    const int *atom_mol;           // molecule of every atom
+    const int *comgrp;             // molecule groups of k_md_getcom
     const int *cstart;             // constraints of molecule m
     const int2 *cons;              // constrained atom pairs
     const double *cons_len2;       // RB^2

@@ -24,43 +24,66 @@
 
 namespace mdx {
 
-// Anchor-relative packed coordinates of every cluster slot, then the
-// molecular centres of mass (GETCOM, service.f:471-500).
+// Anchor-relative packed coordinates of every cluster slot.
 __device__ inline void pack_body(const PackArgs &a, int b) {
-    const int nslot = a.nclu * kClu;
-    const int npack = (nslot + 255) / 256;
-    if (b < npack) {
-        const int s = b * 256 + threadIdx.x;
-        if (s >= nslot) return;
-        const int at = a.clu_atom[s];
-        const int c = s / kClu;
-        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (at >= 0) {
-            double d[3] = {a.x[at] - a.anchor[c * 3], a.y[at] - a.anchor[c * 3 + 1], a.z[at] - a.anchor[c * 3 + 2]};
-            for (int k = 0; k < 3; ++k) d[k] -= a.box.l[k] * rint(d[k] / a.box.l[k]);
-            o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)a.q[at]);
-        }
-        a.xq[s] = o;
-        return;
+    const int s = b * 256 + threadIdx.x;
+    if (s >= a.nclu * kClu) return;
+    const int at = a.clu_atom[s];
+    const int c = s / kClu;
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (at >= 0) {
+        double d[3] = {a.x[at] - a.anchor[c * 3], a.y[at] - a.anchor[c * 3 + 1], a.z[at] - a.anchor[c * 3 + 2]};
+        for (int k = 0; k < 3; ++k) d[k] -= a.box.l[k] * rint(d[k] / a.box.l[k]);
+        o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)a.q[at]);
     }
-    if (a.user_com) return;
-    const int m = (b - npack) * 256 + threadIdx.x;
-    if (m >= a.nmol) return;
-    double sx = 0, sy = 0, sz = 0, sm = 0;
-    for (int i = a.mstart[m]; i < a.mstart[m + 1]; ++i) {
-        sx = sx + a.mass[i] * a.x[i];
-        sy = sy + a.mass[i] * a.y[i];
-        sz = sz + a.mass[i] * a.z[i];
-        sm = sm + a.mass[i];
-    }
-    a.cx[m] = sx / sm;
-    a.cy[m] = sy / sm;
-    a.cz[m] = sz / sm;
+    a.xq[s] = o;
 }
 
 __global__ __launch_bounds__(256) void k_pack(const PackArgs *__restrict__ args, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
     pack_body(args[m.x], m.y);
+}
+
+// Molecular centres of mass (GETCOM, service.f:471-500) for the molecular
+// virial.  A wavefront takes one group of consecutive molecules (at most
+// 64, at most kComAtoms atoms unless one molecule is larger): their atoms
+// pass through LDS in coalesced stages, and each lane adds its molecule's
+// atoms in atom order.
+__device__ inline void com_body(const PackArgs &a, int blk, double *sh) {
+    if (a.user_com) return;
+    const int lane = threadIdx.x;
+    const int m0 = a.comgrp[blk], m1 = a.comgrp[blk + 1];
+    const int m = m0 + lane;
+    const int A0 = a.mstart[m0], A1 = a.mstart[m1];
+    const int b0 = m < m1 ? a.mstart[m] : A1, b1 = m < m1 ? a.mstart[m + 1] : A1;
+    double *sm_ = sh, *sx = sh + kComAtoms, *sy = sh + 2 * kComAtoms, *sz = sh + 3 * kComAtoms;
+    double cx = 0.0, cy = 0.0, cz = 0.0, sm = 0.0;
+    for (int c0 = A0; c0 < A1; c0 += kComAtoms) {
+        const int c1 = min(c0 + kComAtoms, A1);
+        for (int i = c0 + lane; i < c1; i += 64) {
+            sm_[i - c0] = a.mass[i]; sx[i - c0] = a.x[i]; sy[i - c0] = a.y[i]; sz[i - c0] = a.z[i];
+        }
+        __syncthreads();
+        for (int i = max(b0, c0); i < min(b1, c1); ++i) {
+            const double mi = sm_[i - c0];
+            cx = cx + mi * sx[i - c0];
+            cy = cy + mi * sy[i - c0];
+            cz = cz + mi * sz[i - c0];
+            sm = sm + mi;
+        }
+        __syncthreads();
+    }
+    if (m < m1) {
+        a.cx[m] = cx / sm;
+        a.cy[m] = cy / sm;
+        a.cz[m] = cz / sm;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_com(const PackArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ double sh[4 * kComAtoms];
+    const int2 m = map[blockIdx.x];
+    com_body(args[m.x], m.y, sh);
 }
 
 // Detail calls (MDX_WANT_DETAIL) also split the bare Coulomb energy
