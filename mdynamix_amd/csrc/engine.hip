@@ -126,7 +126,7 @@ struct mdx_engine {
     std::vector<double> spee_self;
     FRow *d_srows = nullptr;
     float2 *d_tx = nullptr, *d_ty = nullptr, *d_tz = nullptr;
-    int kx = 0, ky = 0;
+    int kx = 0, ky = 0, tstride = 64;
     float4 *d_ppart = nullptr;
     float2 *d_s0part = nullptr;
     int mzp = 8, nsgroups = 1;
@@ -408,9 +408,10 @@ mdx_status setup_kspace(mdx_engine *h) {
         h->nsgroups = cdiv(std::max<size_t>(1, sr.size()), kSkRows);
         CHK(dalloc(h, &h->d_srows, sr.size()));
         if (!h->dry) HIPCHK(h, hipMemcpy(h->d_srows, sr.data(), sizeof(FRow) * sr.size(), hipMemcpyHostToDevice));
-        CHK(dalloc(h, &h->d_tx, (size_t)h->n * (h->kx + 1)));
-        CHK(dalloc(h, &h->d_ty, (size_t)h->n * (h->ky + 1)));
-        CHK(dalloc(h, &h->d_tz, (size_t)h->n * h->mzp));
+        h->tstride = cdiv(h->n, 64) * 64;
+        CHK(dalloc(h, &h->d_tx, (size_t)h->tstride * (h->kx + 1)));
+        CHK(dalloc(h, &h->d_ty, (size_t)h->tstride * (h->ky + 1)));
+        CHK(dalloc(h, &h->d_tz, (size_t)h->tstride * h->mzp));
         // force-pass tiles: up to 4 rows of one n_x with consecutive n_y;
         // the matrix operand of a tile is ceil(longest row / 2) steps of 64
         // floats, zero where a row has no pair entry
@@ -697,6 +698,7 @@ void fill_args(mdx_engine *h) {
     ra.ty = h->d_ty;
     ra.tz = h->d_tz;
     ra.kx = h->kx;
+    ra.tstride = h->tstride;
     ra.ky = h->ky;
     ra.mzp = h->mzp;
     ra.chunk = h->chunk;

@@ -230,6 +230,7 @@ struct RecipArgs {
     // n_y = 0..ky; tz = exp(2 pi i j f_z), j = 0..mzp-1
     float2 *tx, *ty, *tz;
     int kx, ky, mzp;
+    int tstride;                 // table row stride (atoms, a multiple of 64)
     int chunk, nchunk, nkv;
     double alpha, coulf;
     int nslice, tiles_per_slice, natom_blocks;

@@ -18,7 +18,17 @@ namespace mdx {
 
 __device__ inline void recip_prep_body(const RecipArgs &a, int blk) {
     const int i = blk * 256 + threadIdx.x;
-    if (i >= a.natoms) return;
+    if (i >= a.natoms) {
+        // the tables' padding atoms are zero (the structure-factor pass
+        // reads atoms in pairs)
+        if (i < a.tstride) {
+            const size_t n = (size_t)a.tstride;
+            for (int k = 0; k <= a.kx; ++k) stv(gst(a.tx) + i + k * n, make_float2(0.f, 0.f));
+            for (int k = 0; k <= a.ky; ++k) stv(gst(a.ty) + i + k * n, make_float2(0.f, 0.f));
+            for (int k = 0; k < a.mzp; ++k) stv(gst(a.tz) + i + k * n, make_float2(0.f, 0.f));
+        }
+        return;
+    }
     const double fx = a.x[i] / a.box.l[0], fy = a.y[i] / a.box.l[1], fz = a.z[i] / a.box.l[2];
     const double q = a.q[i];
     stv(gst(a.fq) + i, make_double4(fx, fy, fz, q));
@@ -31,7 +41,7 @@ __device__ inline void recip_prep_body(const RecipArgs &a, int blk) {
     sincospi(2.0 * u8, &s8, &c8);
     stv(gst(a.stepz) + i, make_double4(c1, s1, c8, s8));
     // phase tables (FP64 recurrences from FP64 sincospi steps, FP32 values)
-    const size_t n = (size_t)a.natoms;
+    const size_t n = (size_t)a.tstride;
     {
         double sr = 0.0, si = 0.0;
         sincospi(2.0 * (fx - rint(fx)), &si, &sr);
@@ -92,18 +102,56 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 // row (S(n_z = 0)); k_sk_reduce adds the chunks in FP64.
 // Blocks are numbered so that the row groups of one chunk share an XCD
 // (block index mod 8), whose L2 then serves that chunk's table columns.
-constexpr int kSkStride = kSkSub + 4;      // floats per LDS plane row (bank spread)
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-// LDS stage: W planes [64 rows][Re, Im][kSkStride] then Z planes
-// [8 NT values of j][C, D][kSkStride]; two stages alternate, so the next
-// one is fetched into registers while the current one feeds the matrix
-// instructions and one barrier per stage suffices.  Within a stage, step
-// (h, e) of the 8 takes atom 16 h + 4 K + e as its K index K, so a lane
-// reads its four steps' operands with one 16-byte LDS load.
+// f16 pair of an FP32 value: hi = f16(v), lo = f16(v - hi); hi + lo keeps
+// 22 significant bits (absolute error below 2^-24 for |v| <= 1, the range
+// of B and p_j), and the three products hi*hi + hi*lo + lo*hi of two such
+// pairs reproduce an FP32 product to about 2^-21 relative.
+__device__ inline void split_f16(float v0, float v1, half2_t &hi, half2_t &lo) {
+    const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
+    hi = half2_t{h0, h1};
+    lo = half2_t{(_Float16)(v0 - (float)h0), (_Float16)(v1 - (float)h1)};
+}
+
+// LDS stage of the structure-factor block: per plane row (a k row or a
+// value of j, times Re/Im or C/D) 32 atoms of f16 (64 B) plus 16 B of
+// padding; four planes per stage: W hi, W lo, Z hi, Z lo.
+constexpr int kSkRowB = 80;                 // bytes per plane row
+
 template <int NT>
-__device__ inline void sk_body(const RecipArgs &a, int blk, float *lds) {
-    constexpr int WSZ = kSkRows * 2 * kSkStride;
-    constexpr int BUF = WSZ + NT * 8 * 2 * kSkStride;
+struct SkStage {
+    static constexpr int WROWS = kSkRows * 2;
+    static constexpr int ZROWS = NT * 8 * 2;
+    static constexpr int WH = 0, WL = WROWS * kSkRowB, ZH = 2 * WROWS * kSkRowB, ZL = ZH + ZROWS * kSkRowB;
+    static constexpr int BYTES = ZL + ZROWS * kSkRowB;
+};
+
+// Structure factors on the matrix cores.  For a row (n_x, n_y) with
+// B_a = q_a exp(2 pi i (n_x f_x + n_y f_y)) and p_j = exp(2 pi i j f_z) =
+// C_j + i D_j, the vectors n_z = +-j have S(+-j) = U_j +- i V_j with
+// U_j = sum_a B_a C_j(a), V_j = sum_a B_a D_j(a); n_z = 0 is U_0 (C_0 = 1).
+// Over the atoms of a chunk this is the real product
+//   [Re B; Im B] (rows x atoms)  x  [C_j, D_j] (atoms x j)
+// evaluated with v_mfma_f32_16x16x32_f16 on f16 hi/lo pairs of the FP32
+// operands (three products per tile, FP32 accumulation): an M tile is 8
+// rows x (Re, Im), an N tile 8 values of j x (C, D), K = the 32 atoms of an
+// LDS stage.  A block holds 64 rows (sorted by pair count; a wavefront
+// takes 16 rows = 2 M tiles and as many N tiles as its first, longest row
+// needs) and one chunk of atoms, staged through LDS 32 atoms at a time:
+// B = tx[n_x] ty[|n_y|] (conjugated for n_y < 0) formed in FP32, then
+// split, and p_j = tz[j] split.  Operand layout of the instruction (lane
+// l): A[l % 16][8 (l / 16) + k], B[8 (l / 16) + k][l % 16], k = 0..7;
+// result D[4 (l / 16) + v][l % 16].  Stages alternate between two LDS
+// buffers, the next one fetched into registers while the current one feeds
+// the matrix instructions.  Each block writes its FP32 chunk sums per pair
+// entry (U_j, V_j) and per row (S(n_z = 0)); k_sk_reduce adds the chunks in
+// FP64.  Blocks are numbered so that the row groups of one chunk share an
+// XCD (block index mod 8), whose L2 then serves that chunk's table columns.
+template <int NT>
+__device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) {
+    using St = SkStage<NT>;
     const int G = a.nsgroups;
     const int x8 = blk & 7, s8 = blk >> 3;
     const int g = s8 % G, ch = (s8 / G) * 8 + x8;
@@ -111,21 +159,22 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, float *lds) {
     const int a0 = ch * a.chunk;
     const int a1 = min(a0 + a.chunk, a.natoms);
     const int rbase = g * kSkRows;
-    const size_t n = (size_t)a.natoms;
-    // staging assignment: atom tid % 32 of the stage, rows / j values tid / 32 + 8 i
-    const int sa = tid & 31, sr0 = tid >> 5;
-    int offx[8], offy[8];
-    bool conj[8];
+    const size_t ts = (size_t)a.tstride;
+    // staging assignment: atom pair tid % 16 (atoms 2p, 2p + 1 of the
+    // stage), rows tid / 16 + 16 i
+    const int sp = tid & 15, sr0 = tid >> 4;
+    int offx[4], offy[4];
+    bool conj[4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int r = rbase + sr0 + 8 * i;
+    for (int i = 0; i < 4; ++i) {
+        const int r = rbase + sr0 + 16 * i;
         offx[i] = -1;
         offy[i] = 0;
         conj[i] = false;
         if (r < a.nfrows) {
             const FRow fr = ldv(gld(a.srows) + r);
-            offx[i] = fr.ix * a.natoms;
-            offy[i] = abs(fr.iy) * a.natoms;
+            offx[i] = fr.ix * a.tstride;
+            offy[i] = abs(fr.iy) * a.tstride;
             conj[i] = fr.iy < 0;
         }
     }
@@ -138,39 +187,65 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, float *lds) {
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[mt][t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
-    const MDX_GLOBAL float2 *tx = gld(a.tx), *ty = gld(a.ty), *tz = gld(a.tz);
-    float2 X[8], Y[8], Zv[NT];
+    const MDX_GLOBAL float4 *tx = reinterpret_cast<const MDX_GLOBAL float4 *>(gld(a.tx));
+    const MDX_GLOBAL float4 *ty = reinterpret_cast<const MDX_GLOBAL float4 *>(gld(a.ty));
+    const MDX_GLOBAL float4 *tz = reinterpret_cast<const MDX_GLOBAL float4 *>(gld(a.tz));
+    // Z items of a stage: (j, atom pair) for j < 8 ntg, tid + 256 k
+    constexpr int ZIT = (NT * 8 * 16 + 255) / 256;
+    float4 X[4], Y[4], Zv[ZIT];
     auto fetch = [&](int sb) {
-        const int ia = sb + sa;
+        // two atoms per float4; the tables' rows are padded to a multiple
+        // of 64 atoms, so the pair never leaves its row
+        const int ia = sb + 2 * sp;
         const bool ok = ia < a1;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            X[i] = make_float2(0.f, 0.f);
-            Y[i] = make_float2(0.f, 0.f);
+        for (int i = 0; i < 4; ++i) {
+            X[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            Y[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (ok && offx[i] >= 0) {
-                X[i] = ldv(tx + offx[i] + ia);
-                Y[i] = ldv(ty + offy[i] + ia);
+                X[i] = ldv(tx + ((size_t)offx[i] + ia) / 2);
+                Y[i] = ldv(ty + ((size_t)offy[i] + ia) / 2);
             }
         }
 #pragma unroll
-        for (int i = 0; i < NT; ++i) {
-            Zv[i] = make_float2(0.f, 0.f);
-            if (ok && i < ntg) Zv[i] = ldv(tz + (size_t)(sr0 + 8 * i) * n + ia);
+        for (int k = 0; k < ZIT; ++k) {
+            const int it = tid + 256 * k;
+            const int j = it >> 4, p = it & 15;
+            const int iz = sb + 2 * p;
+            Zv[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (j < 8 * ntg && iz < a1) Zv[k] = ldv(tz + ((size_t)j * ts + iz) / 2);
         }
     };
-    auto store = [&](float *buf) {
+    auto store = [&](unsigned char *buf) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const float yi = conj[i] ? -Y[i].y : Y[i].y;
-            const int r = sr0 + 8 * i;
-            buf[(2 * r) * kSkStride + sa] = X[i].x * Y[i].x - X[i].y * yi;
-            buf[(2 * r + 1) * kSkStride + sa] = X[i].x * yi + X[i].y * Y[i].x;
+        for (int i = 0; i < 4; ++i) {
+            const int r = sr0 + 16 * i;
+            const float y1 = conj[i] ? -Y[i].y : Y[i].y, y3 = conj[i] ? -Y[i].w : Y[i].w;
+            const float re0 = X[i].x * Y[i].x - X[i].y * y1, im0 = X[i].x * y1 + X[i].y * Y[i].x;
+            const float re1 = X[i].z * Y[i].z - X[i].w * y3, im1 = X[i].z * y3 + X[i].w * Y[i].z;
+            half2_t hr, lr, hi, li;
+            split_f16(re0, re1, hr, lr);
+            split_f16(im0, im1, hi, li);
+            const int o = (2 * r) * kSkRowB + 4 * sp;
+            *reinterpret_cast<half2_t *>(buf + St::WH + o) = hr;
+            *reinterpret_cast<half2_t *>(buf + St::WH + o + kSkRowB) = hi;
+            *reinterpret_cast<half2_t *>(buf + St::WL + o) = lr;
+            *reinterpret_cast<half2_t *>(buf + St::WL + o + kSkRowB) = li;
         }
 #pragma unroll
-        for (int i = 0; i < NT; ++i) {
-            const int j = sr0 + 8 * i;
-            buf[WSZ + (2 * j) * kSkStride + sa] = Zv[i].x;
-            buf[WSZ + (2 * j + 1) * kSkStride + sa] = Zv[i].y;
+        for (int k = 0; k < ZIT; ++k) {
+            const int it = tid + 256 * k;
+            const int j = it >> 4, p = it & 15;
+            if (j < NT * 8) {
+                half2_t hc, lc, hd, ld;
+                split_f16(Zv[k].x, Zv[k].z, hc, lc);
+                split_f16(Zv[k].y, Zv[k].w, hd, ld);
+                const int o = (2 * j) * kSkRowB + 4 * p;
+                *reinterpret_cast<half2_t *>(buf + St::ZH + o) = hc;
+                *reinterpret_cast<half2_t *>(buf + St::ZH + o + kSkRowB) = hd;
+                *reinterpret_cast<half2_t *>(buf + St::ZL + o) = lc;
+                *reinterpret_cast<half2_t *>(buf + St::ZL + o + kSkRowB) = ld;
+            }
         }
     };
     const int rA = lane & 7, cA = (lane >> 3) & 1, kA = lane >> 4;
@@ -181,37 +256,38 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, float *lds) {
     }
     __syncthreads();
     for (int k = 0; k < nsub; ++k) {
-        const float *cur = lds + (k & 1) * BUF;
+        const unsigned char *cur = lds + (k & 1) * St::BYTES;
         if (k + 1 < nsub) fetch(a0 + (k + 1) * kSkSub);
         if (ntw > 0) {
+            const int col = 16 * kA;                // bytes: atoms 8 kA .. 8 kA + 7
+            const int ra0 = (2 * (16 * w + rA) + cA) * kSkRowB + col;
+            const int ra1 = (2 * (16 * w + 8 + rA) + cA) * kSkRowB + col;
+            const half8_t Ah0 = *reinterpret_cast<const half8_t *>(cur + St::WH + ra0);
+            const half8_t Al0 = *reinterpret_cast<const half8_t *>(cur + St::WL + ra0);
+            const half8_t Ah1 = *reinterpret_cast<const half8_t *>(cur + St::WH + ra1);
+            const half8_t Al1 = *reinterpret_cast<const half8_t *>(cur + St::WL + ra1);
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int col = 16 * h + 4 * kA;
-                const float4 A0 = *reinterpret_cast<const float4 *>(cur + (2 * (16 * w + rA) + cA) * kSkStride + col);
-                const float4 A1 = *reinterpret_cast<const float4 *>(cur + (2 * (16 * w + 8 + rA) + cA) * kSkStride + col);
-#pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    if (t < ntw) {
-                        const float4 Bv = *reinterpret_cast<const float4 *>(cur + WSZ + (2 * (8 * t + rA) + cA) * kSkStride + col);
-                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0.x, Bv.x, acc[0][t], 0, 0, 0);
-                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1.x, Bv.x, acc[1][t], 0, 0, 0);
-                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0.y, Bv.y, acc[0][t], 0, 0, 0);
-                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1.y, Bv.y, acc[1][t], 0, 0, 0);
-                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0.z, Bv.z, acc[0][t], 0, 0, 0);
-                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1.z, Bv.z, acc[1][t], 0, 0, 0);
-                        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A0.w, Bv.w, acc[0][t], 0, 0, 0);
-                        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A1.w, Bv.w, acc[1][t], 0, 0, 0);
-                    }
+            for (int t = 0; t < NT; ++t) {
+                if (t < ntw) {
+                    const int rb = (2 * (8 * t + rA) + cA) * kSkRowB + col;
+                    const half8_t Bh = *reinterpret_cast<const half8_t *>(cur + St::ZH + rb);
+                    const half8_t Bl = *reinterpret_cast<const half8_t *>(cur + St::ZL + rb);
+                    acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah0, Bh, acc[0][t], 0, 0, 0);
+                    acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah1, Bh, acc[1][t], 0, 0, 0);
+                    acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah0, Bl, acc[0][t], 0, 0, 0);
+                    acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah1, Bl, acc[1][t], 0, 0, 0);
+                    acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al0, Bh, acc[0][t], 0, 0, 0);
+                    acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al1, Bh, acc[1][t], 0, 0, 0);
                 }
             }
         }
-        if (k + 1 < nsub) store(lds + ((k + 1) & 1) * BUF);
+        if (k + 1 < nsub) store(lds + ((k + 1) & 1) * St::BYTES);
         __syncthreads();
     }
     // results: through LDS so that one lane holds (Re U, Im U, Re V, Im V)
     // of one (row, j): M = 4 (l / 16) + v is (row r = M % 8, Re/Im = M / 8),
     // N = l % 16 is (j = N % 8, C/D = N / 8)
-    float *scr = lds + w * 256;
+    float *scr = reinterpret_cast<float *>(lds) + w * 256;
     const int r = lane >> 3, jj = lane & 7;
     const int q0 = (r & 3) * 64 + 16 * (r >> 2) + jj;
 #pragma unroll
@@ -236,7 +312,7 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, float *lds) {
 
 template <int NT>
 __global__ __launch_bounds__(256, 2) void k_sk(const RecipArgs *__restrict__ args, const int2 *__restrict__ map) {
-    __shared__ __attribute__((aligned(16))) float lds[2 * (kSkRows * 2 * kSkStride + NT * 8 * 2 * kSkStride)];
+    __shared__ __attribute__((aligned(16))) unsigned char lds[2 * SkStage<NT>::BYTES];
     const int2 m = map[blockIdx.x];
     sk_body<NT>(args[m.x], m.y, lds);
 }
