@@ -119,7 +119,8 @@ struct mdx_engine {
     FRow *d_frows = nullptr;
     int4 *d_fpk = nullptr;
     FTile *d_ftiles = nullptr;
-    float *d_wm = nullptr;
+    uint16_t *d_wm = nullptr;   // f16 operand of the force pass
+    double qabs = 0.0;          // sum |q| (bounds of the force-pass operand)
     float2 *d_fa0 = nullptr;
     double kbox[3] = {0, 0, 0};
     double spei = 0.0;
@@ -413,36 +414,62 @@ mdx_status setup_kspace(mdx_engine *h) {
         CHK(dalloc(h, &h->d_ty, (size_t)h->tstride * (h->ky + 1)));
         CHK(dalloc(h, &h->d_tz, (size_t)h->tstride * h->mzp));
         // force-pass tiles: up to 4 rows of one n_x with consecutive n_y;
-        // the matrix operand of a tile is ceil(longest row / 2) steps of 64
-        // floats, zero where a row has no pair entry
+        // the matrix operand of a tile is ceil(longest row / 16) steps of
+        // kForceStepH halves (f16 hi and lo of 64 lanes x 8 values), zero
+        // where a row has no pair entry, scaled by 2^e so that the bound
+        // |j D| <= 4 j AKC(k) sum|q| of its rows stays below 2^13
+        const double vol = h->p.box[0] * h->p.box[1] * h->p.box[2];
+        const double alp2 = h->p.alpha * h->p.alpha;
+        auto row_bound = [&](const FRow &f) {
+            double b = 0.0;
+            for (int j = 1; j <= f.m; ++j) {
+                const double kx = 2.0 * kPI * f.ix / h->p.box[0], ky = 2.0 * kPI * f.iy / h->p.box[1],
+                             kz = 2.0 * kPI * j / h->p.box[2];
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                const double akc = 4.0 * kPI * h->p.coulomb_factor / vol * std::exp(-0.25 * k2 / alp2) / k2;
+                b = std::max(b, 4.0 * j * akc * h->qabs);
+            }
+            return b;
+        };
         std::vector<FTile> tl;
+        std::vector<double> tbound;
         int woff = 0;
         for (size_t r = 0; r < fr.size(); ++r) {
             FTile *cur = tl.empty() ? nullptr : &tl.back();
             if (!cur || cur->nrow == 4 || fr[r].ix != cur->ix || fr[r].iy != cur->iy0 + cur->nrow) {
-                if (cur) woff += (cur->nstep + 1) / 2 * 128;
+                if (cur) woff += cur->nstep * kForceStepH;
                 FTile t{};
                 t.ix = fr[r].ix;
                 t.iy0 = fr[r].iy;
                 t.r0 = (int)r;
                 tl.push_back(t);
+                tbound.push_back(0.0);
                 cur = &tl.back();
                 cur->woff = woff;
             }
             fr[r].pad1 = cur->woff;
             fr[r].pad2 = cur->nrow++;
-            cur->nstep = std::max(cur->nstep, (fr[r].m + 1) / 2);
+            cur->nstep = std::max(cur->nstep, (fr[r].m + 15) / 16);
+            tbound.back() = std::max(tbound.back(), row_bound(fr[r]));
         }
-        if (!tl.empty()) woff += (tl.back().nstep + 1) / 2 * 128;
+        if (!tl.empty()) woff += tl.back().nstep * kForceStepH;
+        // scale exponents: per tile (FTile.pad0) and per row (bits 8.. of pad2)
+        for (size_t t = 0; t < tl.size(); ++t) {
+            int e = 0;
+            if (tbound[t] > 0.0) e = 13 - (int)std::ceil(std::log2(tbound[t]));
+            e = std::max(-120, std::min(120, e));
+            tl[t].pad0 = e;
+            for (int q = 0; q < tl[t].nrow; ++q) fr[tl[t].r0 + q].pad2 |= (e + 512) << 8;
+        }
         // spare steps after the last region: the force pass reads a whole
         // register set of steps from every region
-        woff += 24 * 64;
+        woff += 3 * kForceStepH;
         h->ntiles = (int)tl.size();
         CHK(dalloc(h, &h->d_ftiles, std::max<size_t>(1, tl.size())));
         CHK(dalloc(h, &h->d_wm, woff));
         if (!h->dry && !tl.empty())
             HIPCHK(h, hipMemcpy(h->d_ftiles, tl.data(), sizeof(FTile) * tl.size(), hipMemcpyHostToDevice));
-        if (!h->dry) HIPCHK(h, hipMemset(h->d_wm, 0, sizeof(float) * woff));
+        if (!h->dry) HIPCHK(h, hipMemset(h->d_wm, 0, sizeof(uint16_t) * woff));
         CHK(dalloc(h, &h->d_frows, fr.size()));
         CHK(dalloc(h, &h->d_fpk, std::max<size_t>(1, pk.size())));
         CHK(dalloc(h, &h->d_fa0, fr.size() + 4));   // + the force pass's unused reads past the last tile
@@ -961,6 +988,8 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
         (s = dalloc(h, &h->d_cz, h->nmol)) || (s = dalloc(h, &h->d_energy, Q_N)))
         return fail(s);
     self_energy(h, t);
+    h->qabs = 0.0;
+    for (int i = 0; i < n; ++i) h->qabs += std::fabs(t->atom_charge[i]);
     if ((s = setup_kspace(h))) return fail(s);
     if ((s = alloc_clusters(h))) return fail(s);
     if ((s = dalloc(h, &h->d_fixsp, (size_t)3 * n))) return fail(s);

@@ -83,10 +83,12 @@ struct FRow {               // 32 bytes
 };
 
 // Force-pass tile: up to 4 rows of one n_x with consecutive n_y; its matrix
-// operand (nstep blocks of 64 floats) starts at woff.
+// operand (nstep steps of 16 values of j, kForceStepH halves each) starts at
+// half woff and is scaled by 2^pad0.
 struct FTile {              // 32 bytes
     int ix, iy0, nrow, nstep, woff, r0, pad0, pad1;
 };
+constexpr int kForceStepH = 1024;  // halves per force-pass step: 64 lanes x 8 (hi), then 64 x 8 (lo)
 
 __host__ __device__ inline double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
 
@@ -250,7 +252,7 @@ struct RecipArgs {
     // woff of the row's tile, pad2 = row within the tile)
     int ntiles;
     const FTile *ftiles;
-    float *wm;
+    uint16_t *wm;                // f16 hi/lo operand, kForceStepH halves per step
 };
 
 struct ScatterArgs {

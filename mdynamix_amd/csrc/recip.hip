@@ -346,10 +346,12 @@ __device__ inline float2 k_terms(const RecipArgs &a, int ix, int iy, int iz, dou
 // the energy QPE and virial projections of every k (forces.f:1317-1331)
 // and the force-pass data: per row A of n_z = 0, and per pair entry, from
 // D = (A+r - A-r, A+r + A-r, A+i + A-i, A+i - A-i), the coefficients of
-// (Re p_j, Im p_j) in the four sums of recip_force_body, written in the
-// operand order of its matrix instruction: lane K 16 + 4 row + s, with
-// K = 2 ((j - 1) mod 2) + (0: Re p_j, 1: Im p_j) and s = 0..3 for
-//   S1: (-D3, D1)   S2: (D2, D4)   Z1: j (-D4, D2)   Z2: j (D1, D3).
+// (Re p_j, Im p_j) in the four sums of recip_force_body,
+//   S1: (-D3, D1)   S2: (D2, D4)   Z1: j (-D4, D2)   Z2: j (D1, D3),
+// scaled by the tile's 2^e, split into f16 hi/lo and written in the operand
+// order of its matrix instruction: step (j - 1) / 16; with u = (j - 1) mod
+// 16, lane 16 (u / 4) + 4 row + s (s = 0..3 the sum), element 2 (u mod 4)
+// + (0: Re p_j, 1: Im p_j); hi halves first, then lo.
 __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)[256]) {
     const int e = blk * 256 + threadIdx.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -372,11 +374,19 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
             d = make_float4(ap.x - am.x, ap.x + am.x, ap.y + am.y, ap.y - am.y);
         }
         if (k.z <= fr.m) {
-            const float fj = (float)k.z;
-            const int jm = k.z - 1;
-            MDX_GLOBAL float *w = gst(a.wm) + fr.pad1 + (jm >> 1) * 64 + (jm & 1) * 32 + 4 * fr.pad2;
-            stv(reinterpret_cast<MDX_GLOBAL float4 *>(w), make_float4(-d.z, d.y, -fj * d.w, fj * d.x));
-            stv(reinterpret_cast<MDX_GLOBAL float4 *>(w + 16), make_float4(d.x, d.w, fj * d.y, fj * d.z));
+            const int jm = k.z - 1, u = jm & 15;
+            const int rl = fr.pad2 & 255, e = (fr.pad2 >> 8) - 512;
+            const float sc = ldexpf(1.f, e), fj = (float)k.z * sc;
+            const float c[2][4] = {{-d.z * sc, d.y * sc, -fj * d.w, fj * d.x},
+                                   {d.x * sc, d.w * sc, fj * d.y, fj * d.z}};
+            MDX_GLOBAL _Float16 *w = reinterpret_cast<MDX_GLOBAL _Float16 *>(gst(a.wm)) + fr.pad1 +
+                                     (size_t)(jm >> 4) * kForceStepH + 8 * (16 * (u >> 2) + 4 * rl) + 2 * (u & 3);
+            for (int s = 0; s < 4; ++s)
+                for (int cm = 0; cm < 2; ++cm) {
+                    const _Float16 hi = (_Float16)c[cm][s];
+                    w[8 * s + cm] = hi;
+                    w[512 + 8 * s + cm] = (_Float16)(c[cm][s] - (float)hi);
+                }
         }
     } else if (e < a.npk + a.nfrows) {
         const int r = e - a.npk;
@@ -425,28 +435,32 @@ __global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__
 // the four sums over j
 //   S1 = sum (p_i D1 - p_r D3)    S2 = sum (p_r D2 + p_i D4)
 //   Z1 = sum j (p_i D2 - p_r D4)  Z2 = sum j (p_r D1 + p_i D3)
-// are a matrix product (4 rows x 4 sums) x (2 j x (Re, Im)) x (16 atoms),
-// v_mfma_f32_16x16x4_f32 with the row coefficients as A (laid out by
-// k_sk_reduce, one float per lane and step) and the atoms' p_j as B (FP64
-// recurrence, kept in registers for the whole row loop).  Lane l then holds
-// S1, S2, Z1, Z2 of row l / 16 of the tile and atom l % 16, and adds
+// are a matrix product (4 rows x 4 sums) x (16 j x (Re, Im)) x (16 atoms),
+// v_mfma_f32_16x16x32_f16 on f16 hi/lo pairs (three products, FP32
+// accumulation): the row coefficients are the A operand (laid out and
+// scaled by 2^e per tile by k_sk_reduce), the atoms' p_j (FP64 recurrence,
+// split once) the B operand, kept in registers for the whole row loop.
+// Lane l then holds 2^e (S1, S2, Z1, Z2) of row l / 16 of the tile and atom
+// l % 16, and adds
 //   t = b_i A0_r - b_r A0_i + b_r S1 + b_i S2,  t_z = b_r Z1 + b_i Z2
 // (A0: n_z = 0) into F_x += n_x t, F_y += n_y t, F_z += t_z in FP64; b
 // advances over the 4-row tiles of one n_x by exp(8 pi i f_y) (FP64) and
 // restarts from an FP64 sincospi at every new n_x.  A wavefront takes
 // kForceTiles tiles of 16 atoms, a block 4 wavefronts, each block one slice
-// of the tiles; the 4 lanes of an atom are summed at the end and every slice
-// writes its own partial force array (summed by the combine kernel).
+// of the tiles; the 4 lanes of an atom are summed at the end and every
+// slice writes its own partial force array (summed by the combine kernel).
 template <int MZ>
 __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
-    constexpr int SM = MZ / 2;
+    constexpr int S16 = MZ / 16;
     const int ab = blk % a.natom_blocks, sl = blk / a.natom_blocks;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int K = lane >> 4, jo = K >> 1, cm = K & 1;
+    const int K = lane >> 4;
     const int t0 = sl * a.tiles_per_slice;
     const int t1 = min(t0 + a.tiles_per_slice, a.ntiles);
     const int abase = ab * kForceBlock + w * 16 * kForceTiles + (lane & 15);
-    float P[kForceTiles][SM];
+    // B operand: element 2 q + c of step st is p_j, j = 16 st + 4 K + q + 1,
+    // component c (Re, Im)
+    half8_t Ph[kForceTiles][S16], Pl[kForceTiles][S16];
     double w4r[kForceTiles], w4i[kForceTiles], br[kForceTiles], bi[kForceTiles];
     double Fx[kForceTiles], Fy[kForceTiles], Fz[kForceTiles];
 #pragma unroll
@@ -455,15 +469,27 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         double4 at = make_double4(0, 0, 0, 0);
         double4 st = make_double4(1.0, 0.0, 1.0, 0.0);
         if (i < a.natoms) { at = ldv(gld(a.fq) + i); st = ldv(gld(a.stepz) + i); }
-        // p_j for j = 2 s + 1 + jo, component cm
+        float pv[S16][8];
         double wr = st.x, wi = st.y;
 #pragma unroll
         for (int j = 1; j <= MZ; ++j) {
-            if (((j - 1) & 1) == jo) P[ta][(j - 1) >> 1] = (float)(cm ? wi : wr);
+            const int u = (j - 1) & 15, stp = (j - 1) >> 4;
+            if ((u >> 2) == K) {
+                pv[stp][2 * (u & 3)] = (float)wr;
+                pv[stp][2 * (u & 3) + 1] = (float)wi;
+            }
             const double t = wr * st.x - wi * st.y;
             wi = wr * st.y + wi * st.x;
             wr = t;
         }
+#pragma unroll
+        for (int stp = 0; stp < S16; ++stp)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const _Float16 hi = (_Float16)pv[stp][k];
+                Ph[ta][stp][k] = hi;
+                Pl[ta][stp][k] = (_Float16)(pv[stp][k] - (float)hi);
+            }
         double s4, c4;
         {
             double v = 4.0 * (at.y - rint(at.y));
@@ -476,30 +502,35 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         Fx[ta] = Fy[ta] = Fz[ta] = 0.0;
     }
     const MDX_CONST FTile *tiles = sld(a.ftiles);
-    const MDX_GLOBAL float *wm = gld(a.wm) + lane;
+    const MDX_GLOBAL half8_t *wm = reinterpret_cast<const MDX_GLOBAL half8_t *>(gld(a.wm)) + lane;
     const MDX_GLOBAL float2 *fa0 = gld(a.fa0);
     int cix = -0x7fffffff, ciy = 0;
     // the next tile's operand is requested while the current one computes;
-    // operand regions hold an even number of steps (zero padded) and the
-    // array ends with SM spare steps, so the loads need no bounds
-    float A[SM], An[SM];
+    // the array ends with spare steps, so the loads need no bounds
+    half8_t Ah[S16], Al[S16], Ahn[S16], Aln[S16];
     float2 a0n = make_float2(0.f, 0.f);
     FTile Tn{};
     if (t0 < t1) {
         Tn = ldv(tiles + t0);
 #pragma unroll
-        for (int s = 0; s < SM; ++s) An[s] = ldv(wm + Tn.woff + 64 * s);
+        for (int s = 0; s < S16; ++s) {
+            Ahn[s] = wm[Tn.woff / 8 + s * (kForceStepH / 8)];
+            Aln[s] = wm[Tn.woff / 8 + s * (kForceStepH / 8) + 64];
+        }
         a0n = ldv(fa0 + Tn.r0 + K);
     }
     for (int t = t0; t < t1; ++t) {
         const FTile T = Tn;
 #pragma unroll
-        for (int s = 0; s < SM; ++s) A[s] = An[s];
+        for (int s = 0; s < S16; ++s) { Ah[s] = Ahn[s]; Al[s] = Aln[s]; }
         const float2 a0 = K < T.nrow ? a0n : make_float2(0.f, 0.f);
         if (t + 1 < t1) {
             Tn = ldv(tiles + t + 1);
 #pragma unroll
-            for (int s = 0; s < SM; ++s) An[s] = ldv(wm + Tn.woff + 64 * s);
+            for (int s = 0; s < S16; ++s) {
+                Ahn[s] = wm[Tn.woff / 8 + s * (kForceStepH / 8)];
+                Aln[s] = wm[Tn.woff / 8 + s * (kForceStepH / 8) + 64];
+            }
             a0n = ldv(fa0 + Tn.r0 + K);
         }
         const int ny = T.iy0 + K;
@@ -510,14 +541,14 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
 #pragma unroll
         for (int ta = 0; ta < kForceTiles; ++ta) acc[ta] = mfma_f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < SM; s += 2) {
+        for (int s = 0; s < S16; ++s) {
             if (s < T.nstep) {
 #pragma unroll
-                for (int ta = 0; ta < kForceTiles; ++ta)
-                    acc[ta] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s], P[ta][s], acc[ta], 0, 0, 0);
-#pragma unroll
-                for (int ta = 0; ta < kForceTiles; ++ta)
-                    acc[ta] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s + 1], P[ta][s + 1], acc[ta], 0, 0, 0);
+                for (int ta = 0; ta < kForceTiles; ++ta) {
+                    acc[ta] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[s], Ph[ta][s], acc[ta], 0, 0, 0);
+                    acc[ta] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[s], Pl[ta][s], acc[ta], 0, 0, 0);
+                    acc[ta] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[s], Ph[ta][s], acc[ta], 0, 0, 0);
+                }
             }
         }
         if (restart) {
@@ -541,11 +572,12 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
                 br[ta] = u;
             }
         }
+        const float isc = ldexpf(1.f, -T.pad0);
 #pragma unroll
         for (int ta = 0; ta < kForceTiles; ++ta) {
             const float bre = (float)br[ta], bim = (float)bi[ta];
-            const float ts = (bim * a0.x - bre * a0.y) + (bre * acc[ta][0] + bim * acc[ta][1]);
-            const float tzs = bre * acc[ta][2] + bim * acc[ta][3];
+            const float ts = (bim * a0.x - bre * a0.y) + (bre * acc[ta][0] + bim * acc[ta][1]) * isc;
+            const float tzs = (bre * acc[ta][2] + bim * acc[ta][3]) * isc;
             Fx[ta] += (double)T.ix * (double)ts;
             Fy[ta] += (double)ny * (double)ts;
             Fz[ta] += (double)tzs;
