@@ -257,17 +257,18 @@ __device__ inline void combine_body(const CombineArgs &a, int blk, int group) {
     double v[2] = {0, 0};
     if (i < a.n) {
         double G[3] = {0, 0, 0}, HL[3] = {0, 0, 0}, H[3] = {0, 0, 0};
+        const unsigned long long *jf = a.jf + (size_t)a.atom_slot[i] * 8;
         if (group != 1) {
-            G[0] = a.flx[i] + fixd(a.gx, i, a.inv_scale);
-            G[1] = a.fly[i] + fixd(a.gy, i, a.inv_scale);
-            G[2] = a.flz[i] + fixd(a.gz, i, a.inv_scale);
+            G[0] = a.flx[i] + (fixd(a.gx, i, a.inv_scale) + fixd(jf, 3, a.inv_scale));
+            G[1] = a.fly[i] + (fixd(a.gy, i, a.inv_scale) + fixd(jf, 4, a.inv_scale));
+            G[2] = a.flz[i] + (fixd(a.gz, i, a.inv_scale) + fixd(jf, 5, a.inv_scale));
             for (int s = 0; s < (group != 1 ? a.nslice : 0); ++s)
                 for (int d = 0; d < 3; ++d) G[d] += a.recip[((size_t)s * 3 + d) * a.n + i];
         }
         if (group != 2) {
-            HL[0] = a.fsx[i] + fixd(a.hlx, i, a.inv_scale);
-            HL[1] = a.fsy[i] + fixd(a.hly, i, a.inv_scale);
-            HL[2] = a.fsz[i] + fixd(a.hlz, i, a.inv_scale);
+            HL[0] = a.fsx[i] + (fixd(a.hlx, i, a.inv_scale) + fixd(jf, 0, a.inv_scale));
+            HL[1] = a.fsy[i] + (fixd(a.hly, i, a.inv_scale) + fixd(jf, 1, a.inv_scale));
+            HL[2] = a.fsz[i] + (fixd(a.hlz, i, a.inv_scale) + fixd(jf, 2, a.inv_scale));
             H[0] = HL[0] + fixd(a.hbx, i, a.inv_scale);
             H[1] = HL[1] + fixd(a.hby, i, a.inv_scale);
             H[2] = HL[2] + fixd(a.hbz, i, a.inv_scale);

@@ -139,6 +139,7 @@ struct mdx_engine {
     // forces
     double *d_fs = nullptr, *d_fl = nullptr;     // [3][n] each
     unsigned long long *d_fix = nullptr;          // [9][n]: G, HL, HB
+    unsigned long long *d_fixj = nullptr;         // [slot][8]: pair j reactions HL xyz, G xyz, pad
     double *d_out = nullptr;                      // [9][n]: F, G, H
     double *d_epart = nullptr;
     int epart_rows = 0;
@@ -530,6 +531,8 @@ mdx_status alloc_clusters(mdx_engine *h) {
     CHK(dalloc(h, &h->d_colclu, h->ncol));
     CHK(dalloc(h, &h->d_clu_atom, (size_t)h->nclu_cap * kClu));
     CHK(dalloc(h, &h->d_atom_slot, h->n));
+    // the force combine reads atom_slot; valid (zero) before the first list
+    if (!h->dry) HIPCHK(h, hipMemset(h->d_atom_slot, 0, sizeof(int) * h->n));
     CHK(dalloc(h, &h->d_clu_col, h->nclu_cap));
     CHK(dalloc(h, &h->d_anchor, (size_t)h->nclu_cap * 3));
     CHK(dalloc(h, &h->d_ext, (size_t)h->nclu_cap * 3));
@@ -559,6 +562,7 @@ mdx_status alloc_clusters(mdx_engine *h) {
     CHK(dalloc(h, &h->d_fs, (size_t)3 * h->n));
     CHK(dalloc(h, &h->d_fl, (size_t)3 * h->n));
     CHK(dalloc(h, &h->d_fix, (size_t)9 * h->n));
+    CHK(dalloc(h, &h->d_fixj, (size_t)h->nclu_cap * kClu * 8));
     CHK(dalloc(h, &h->d_out, (size_t)9 * h->n));
     return MDX_OK;
 }
@@ -681,6 +685,8 @@ void fill_args(mdx_engine *h) {
     }
     pa.fs_x = h->d_fs; pa.fs_y = h->d_fs + n; pa.fs_z = h->d_fs + 2 * n;
     pa.fl_x = h->d_fl; pa.fl_y = h->d_fl + n; pa.fl_z = h->d_fl + 2 * n;
+    pa.jf = h->d_fixj;
+    pa.fscale = h->fscale;
     pa.epart = erow(h, h->row_pairs);
     pa.atom_mol = h->d_mol;
     pa.atom_moltype = h->d_moltype;
@@ -1230,8 +1236,6 @@ mdx_status mdx_list_counts(mdx_engine *h, int64_t *n_short, int64_t *n_long) {
             s += __builtin_popcountll(e.mask & e.mshort);
             l += __builtin_popcountll(e.mask & ~e.mshort);
         }
-    s /= 2;
-    l /= 2;
     for (int k = 0; k < h->nspecial; ++k) {
         if (cls[k] == SP_SHORT14 || cls[k] == SP_SHORT) ++s;
         else if (cls[k] == SP_LONG) ++l;
@@ -1270,8 +1274,9 @@ mdx_status mdx_get_list(mdx_engine *h, int32_t *nbs1, int32_t *nbs2, int64_t cap
             const PairEntry &e = L[(size_t)c * h->list_cap + list_slot(k, ns[c], h->list_cap)];
             for (int lane = 0; lane < 64; ++lane) {
                 if (!((e.mask >> lane) & 1ull)) continue;
-                const int ai = ca[c * kClu + (lane >> 3)], aj = ca[e.cj * kClu + (lane & 7)];
-                if (ai < aj) continue;
+                // the half list holds each pair once, in either orientation
+                const int u = ca[c * kClu + (lane >> 3)], v = ca[e.cj * kClu + (lane & 7)];
+                const int ai = std::max(u, v), aj = std::min(u, v);
                 P q{keyof(ai, aj), ai + 1, aj + 1};
                 if ((e.mshort >> lane) & 1ull) S.push_back(q);
                 else Lg.push_back(q);
@@ -1388,6 +1393,8 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         const long long nz1 = (long long)h->rows_total * Q_N;
         b->zero.add(ZeroSeg{(unsigned long long *)h->d_epart, nz1}, cdiv(nz1, 256));
         b->zero.add(ZeroSeg{h->d_fix, 9LL * n}, cdiv(9LL * n, 256));
+        const long long nzj = (long long)h->nclu_cap * kClu * 8;
+        b->zero.add(ZeroSeg{h->d_fixj, nzj}, cdiv(nzj, 256));
         b->zero.add(ZeroSeg{h->d_fixsp, 3LL * n}, cdiv(3LL * n, 256));
         b->pack.add(h->a_pack, cdiv((long long)h->nclu_cap * kClu, 256));
         b->com.add(h->a_pack, h->ncomgrp);
@@ -1429,6 +1436,8 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         ca.gx = fixG; ca.gy = fixG + n; ca.gz = fixG + 2 * n;
         ca.hlx = fixHL; ca.hly = fixHL + n; ca.hlz = fixHL + 2 * n;
         ca.hbx = fixHB; ca.hby = fixHB + n; ca.hbz = fixHB + 2 * n;
+        ca.jf = h->d_fixj;
+        ca.atom_slot = h->d_atom_slot;
         const bool spr = b->ring && K > 1;
         ca.spx = spr ? h->d_fixsp : nullptr;
         ca.spy = spr ? h->d_fixsp + n : nullptr;

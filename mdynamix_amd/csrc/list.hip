@@ -3,7 +3,7 @@
 // The reference scans all N^2/2 pairs.  Here atoms are binned into x-y
 // columns, sorted by z inside each column and cut into clusters of 8; an
 // i-cluster lists every j-cluster whose bounding box comes within the list
-// radius (full list: both (i,j) and (j,i) appear).  For each listed cluster
+// radius (half list: every cluster pair once, cj >= ci).  For each listed cluster
 // pair a wavefront tests all 64 atom pairs with the reference's arithmetic
 // (FP64 differences of the caller's coordinates, one-shift minimum image,
 // rr2 = (dx^2 + dy^2) + dz^2 without FMA contraction, inclusive <= against
@@ -85,11 +85,12 @@ __global__ void k_cluster_geom(int nclu, const int *__restrict__ clu_atom, const
     }
 }
 
-// Candidate j-clusters of each i-cluster: bounding boxes within r_list.
-// One wavefront per i-cluster; the clusters of each neighbour column
-// (periodic wrap, each column once) are tested 64 at a time and the hits
-// compacted in column order with a ballot, so the candidate order is that
-// of a serial scan.
+// Candidate j-clusters of each i-cluster: bounding boxes within r_list,
+// j >= i (a half list: every cluster pair once, the i-cluster's own pairs
+// with i slot < j slot).  One wavefront per i-cluster; the clusters of each
+// neighbour column (periodic wrap, each column once) are tested 64 at a
+// time and the hits compacted in column order with a ballot, so the
+// candidate order is that of a serial scan.
 __global__ __launch_bounds__(64) void k_candidates(int nclu, const double *__restrict__ anchor,
                                                    const float *__restrict__ ext, const int *__restrict__ clu_col,
                                                    const int *__restrict__ colclu, const int *__restrict__ colclu_n,
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(64) void k_candidates(int nclu, const double *__res
                         const double g = fabs(d) - (double)(ei[k] + ext[cj * 3 + k]);
                         if (g > 0.0) d2 += g * g;
                     }
-                    hit = d2 <= r2;
+                    hit = d2 <= r2 && cj >= ci;
                 }
                 const unsigned long long m = __ballot(hit);
                 const int pos = cnt + __popcll(m & below);
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(64) void k_masks(
             a2 = clu_atom[c2 * kClu + jl];
         }
         bool listed = false, shrt = false;
-        if (ai >= 0 && aj >= 0 && ai != aj) {
+        if (ai >= 0 && aj >= 0 && ai != aj && (cj != ci || il < jl)) {
             double dx = xi - xj, dy = yi - yj, dz = zi - zj;
             min_image(dx, dy, dz, b);
             double rr2 = dx * dx + dy * dy;

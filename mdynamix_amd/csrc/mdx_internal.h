@@ -148,6 +148,13 @@ __device__ inline void fx_add(unsigned long long *p, double v, double scale) {
     atomicAdd(p, (unsigned long long)q);
 }
 
+// The same on a global-memory pointer (a global atomic, not a flat one,
+// so it does not also count against the scalar-memory wait counter).
+__device__ inline void fx_add_g(MDX_GLOBAL unsigned long long *p, double v, double scale) {
+    const long long q = __double2ll_rn(v * scale);
+    __hip_atomic_fetch_add(p, (unsigned long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------------
 // Argument structs of the force-call kernels (one per system).
 
@@ -193,6 +200,12 @@ struct PairArgs {
     float lx, ly, lz, ilx, ily, ilz;
     double *fs_x, *fs_y, *fs_z;  // short-class pair forces (per atom)
     double *fl_x, *fl_y, *fl_z;  // long-class pair forces (per atom)
+    // j-atom reactions of the half list, 64-bit fixed point (scale fscale),
+    // per cluster slot 8 words: short x, y, z (fast forces H), long x, y, z
+    // (slow forces G), 2 unused, so one list entry's 48 values are 512
+    // contiguous bytes
+    unsigned long long *jf;
+    double fscale;
     double *epart;               // [nclu_cap][Q_N]
     // per-type breakdown (detail calls only)
     const int *atom_mol, *atom_moltype, *type_pair;
@@ -320,6 +333,8 @@ struct CombineArgs {
     const unsigned long long *hlx, *hly, *hlz; // fixed: LOCALF specials
     const unsigned long long *hbx, *hby, *hbz; // fixed: bonded
     const unsigned long long *spx, *spy, *spz; // fixed: bead springs (unscaled) or null
+    const unsigned long long *jf;              // fixed: pair j reactions [slot][8]
+    const int *atom_slot;
     double inv_scale;
     const double *x, *y, *z;
     const int *atom_mol;

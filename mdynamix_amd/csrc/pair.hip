@@ -8,10 +8,13 @@
 // exact Gaussian in the force, forces.f:1146-1153).  Distance vectors are
 // formed from anchor-relative FP32 coordinates plus the cluster pair's
 // image shift, so no FP32 absolute coordinate enters a difference.
-// The list is a full list (no Newton's third law): every atom's force is
-// summed by the one wavefront that owns it, in a fixed order, with FP32
-// partial sums over 16 j-clusters flushed into FP64; energies and virials
-// (halved for the double visit) go to per-wavefront FP64 partials.  The
+// The list is a half list (each atom pair once): the i atoms' forces are
+// summed by the wavefront that owns the i-cluster, in a fixed order, with
+// FP32 partial sums over 16 j-clusters flushed into FP64; the j atoms'
+// reactions (Newton's third law) are summed over the 8 i slots in a fixed
+// butterfly and added into 64-bit fixed-point accumulators, whose integer
+// sums do not depend on the order of arrival.  Energies and virials go to
+// per-wavefront FP64 partials.  The
 // electrostatic energy is accumulated per unit q_i C with the exact q_j
 // (FP32 high part plus low part) and multiplied by the FP64 q_i C at the
 // end, so no charge rounding biases the cancelling sums.  Pairs keep the
@@ -86,6 +89,25 @@ __global__ __launch_bounds__(64) void k_com(const PackArgs *__restrict__ args, c
     com_body(args[m.x], m.y, sh);
 }
 
+// Reduce-scatter steps on the gfx950 lane-swap instructions.
+// swap_sum32(x, y): lanes 0-31 get x + x(lane + 32), lanes 32-63 get
+// y(lane - 32) + y; v_permlane32_swap exchanges x's upper half with y's
+// lower half.
+__device__ inline float swap_sum32(float x, float y) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// swap_sum16(x, y): even rows of 16 lanes get x + x(lane + 16), odd rows
+// y(lane - 16) + y (v_permlane16_swap: x's odd rows with y's even rows).
+__device__ inline float swap_sum16(float x, float y) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// x(lane ^ 8): a rotation by 8 inside each row of 16 (DPP row_ror:8).
+__device__ inline float xor8(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x128, 0xf, 0xf, false));
+}
+
 // Detail calls (MDX_WANT_DETAIL) also split the bare Coulomb energy
 // q_i q_j C / (r + RAE) and the LJ energy by molecule-type pair (POTE1,
 // POTL1) or, for pairs inside one molecule, by molecule type (PES141,
@@ -93,7 +115,7 @@ __global__ __launch_bounds__(64) void k_com(const PackArgs *__restrict__ args, c
 // keeps its own FP64 buckets in LDS (bucket = type of the j molecule, or
 // the intramolecular bucket), so the sums stay in a fixed order.
 template <bool SOFT, bool DETAIL>
-__device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int group) {
+__device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float *s_jf, int *s_jc, int group) {
     const int lane = threadIdx.x;
     const int nlj2 = a.nlj * a.nlj;
     for (int t = lane; t < nlj2; t += 64) s_lj[t] = ldv(gld(a.lj) + t);
@@ -141,6 +163,11 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
     const MDX_CONST PairEntry *L = sld(a.list) + (size_t)ci * a.list_cap;
     const MDX_GLOBAL float4 *xq = gld(a.xq);
     const MDX_GLOBAL int2 *aux = gld(a.aux);
+    MDX_GLOBAL unsigned long long *const jf_base = gst(a.jf) + (size_t)jl * 8 + il;
+    const double fscale = a.fscale;
+    const bool wrap = a.wrap;
+    const float bl[3] = {a.lx, a.ly, a.lz}, bi[3] = {a.ilx, a.ily, a.ilz};
+    const float rc2 = a.rc2, raex = a.raex;
     // software pipeline: list entries two ahead (scalar loads), the
     // j-cluster coordinates one ahead
     PairEntry e1{}, e2{};
@@ -157,33 +184,43 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
         for (int k = k0; k < k1; ++k) {
             const PairEntry e = e1;
             const float4 pj = pj1;
-            const int2 auxj = aux1;
+            int2 auxj = aux1;
+            // keeps the LJ-type mask below from being hoisted next to the
+            // prefetch load, which would wait for that load at once
+            asm volatile("" : "+v"(auxj.x));
             e1 = e2;
-            if (k + 2 < n) e2 = ldv(L + ent(k + 2));
-            if (k + 1 < n) { pj1 = ldv(xq + e1.cj * kClu + jl); aux1 = ldv(aux + e1.cj * kClu + jl); }
+            // unconditional prefetch (past the end it re-reads the last
+            // entry and its j-cluster: valid addresses, values unused)
+            e2 = ldv(L + ent(min(k + 2, n - 1)));
+            pj1 = ldv(xq + e1.cj * kClu + jl);
+            aux1 = ldv(aux + e1.cj * kClu + jl);
             unsigned long long m = e.mask;
             if (group == 1) m &= e.mshort;
             else if (group == 2) m &= ~e.mshort;
-            if (m == 0ull) continue;
+            // branch-free body: a lane without a pair inside the cutoff
+            // takes rinv = 0, which makes every contribution exactly zero,
+            // and the class split multiplies by exact 0/1 weights, so the
+            // accumulators stay in place (no divergent paths to merge)
             const int infoj = auxj.x;
             const bool on = (m >> lane) & 1ull;
             const bool isshort = (e.mshort >> lane) & 1ull;
+            const float ws = isshort ? 1.f : 0.f, wl = 1.f - ws;
             float dx = (pi.x - pj.x) + e.sx;
             float dy = (pi.y - pj.y) + e.sy;
             float dz = (pi.z - pj.z) + e.sz;
-            if (a.wrap) {
-                dx = fmaf(-a.lx, rintf(dx * a.ilx), dx);
-                dy = fmaf(-a.ly, rintf(dy * a.ily), dy);
-                dz = fmaf(-a.lz, rintf(dz * a.ilz), dz);
+            if (wrap) {
+                dx = fmaf(-bl[0], rintf(dx * bi[0]), dx);
+                dy = fmaf(-bl[1], rintf(dy * bi[1]), dy);
+                dz = fmaf(-bl[2], rintf(dz * bi[2]), dz);
             }
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!on || !(r2 <= a.rc2)) continue;
-            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const bool in = on && r2 <= rc2;
+            const float rinv = in ? __builtin_amdgcn_rsqf(r2) : 0.f;
             const float r = r2 * rinv;
             float rei = rinv;
             if (SOFT) {
                 const int eej = infoj >> 16, eei = infoi >> 16;
-                if ((eei & eej) == 0) rei = __builtin_amdgcn_rcpf(r + a.raex);
+                rei = (in && (eei & eej) == 0) ? __builtin_amdgcn_rcpf(r + raex) : rinv;
             }
             const float ar = alpha * r;
             const float t = __builtin_amdgcn_rcpf(1.0f + B1 * ar);
@@ -200,24 +237,50 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
             const float fsc = (ee1 * (talpha * g + erfct * rei) + fsr) * rinv;
             const float fxx = fsc * dx, fyy = fsc * dy, fzz = fsc * dz;
             const float vr = fsr * r;
-            if (DETAIL) {
+            if (DETAIL && in) {
                 const int aj = a.clu_atom[e.cj * kClu + jl];
                 const int b = a.atom_mol[aj] == moli ? a.nmt : a.atom_moltype[aj];
                 s_det[(b * 2) * 64 + lane] += (double)(pj.w * rei) + (double)(__int_as_float(auxj.y) * rei);
                 s_det[(b * 2 + 1) * 64 + lane] += (double)esr;
             }
-            if (isshort) {
-                fs[0] += fxx; fs[1] += fyy; fs[2] += fzz;
-                es += ees; ljs += esr; vs += vr;
-                vxs += fxx * dx; vys += fyy * dy; vzs += fzz * dz;
-                ++cnt_s;
-            } else {
-                fl[0] += fxx; fl[1] += fyy; fl[2] += fzz;
-                el += ees; ljl += esr; vl += vr;
-                vxl += fxx * dx; vyl += fyy * dy; vzl += fzz * dz;
-                ++cnt_l;
+            const float vxx = fxx * dx, vyy = fyy * dy, vzz = fzz * dz;
+            fs[0] = fmaf(ws, fxx, fs[0]); fs[1] = fmaf(ws, fyy, fs[1]); fs[2] = fmaf(ws, fzz, fs[2]);
+            fl[0] = fmaf(wl, fxx, fl[0]); fl[1] = fmaf(wl, fyy, fl[1]); fl[2] = fmaf(wl, fzz, fl[2]);
+            es = fmaf(ws, ees, es); ljs = fmaf(ws, esr, ljs); vs = fmaf(ws, vr, vs);
+            el = fmaf(wl, ees, el); ljl = fmaf(wl, esr, ljl); vl = fmaf(wl, vr, vl);
+            vxs = fmaf(ws, vxx, vxs); vys = fmaf(ws, vyy, vys); vzs = fmaf(ws, vzz, vzs);
+            vxl = fmaf(wl, vxx, vxl); vyl = fmaf(wl, vyy, vyl); vzl = fmaf(wl, vzz, vzl);
+            cnt_s += (in && isshort) ? 1 : 0;
+            cnt_l += (in && !isshort) ? 1 : 0;
+            const float jx = -fxx, jy = -fyy, jz = -fzz;   // force on this lane's j atom
+            // Newton's third law: the j reactions (short x, y, z, long x, y,
+            // z) summed over the 8 i slots by a reduce-scatter butterfly;
+            // lane (il, jl) ends with component il of j slot jl
+            {
+                float v[8];
+                for (int q = 0; q < 3; ++q) {
+                    const float f = q == 0 ? jx : (q == 1 ? jy : jz);
+                    v[q] = ws * f;
+                    v[3 + q] = wl * f;
+                }
+                v[6] = 0.f; v[7] = 0.f;
+                float w[4], u[2];
+                for (int t = 0; t < 4; ++t) w[t] = swap_sum32(v[t], v[4 + t]);
+                for (int t = 0; t < 2; ++t) u[t] = swap_sum16(w[t], w[2 + t]);
+                const float send = (lane & 8) ? u[0] : u[1], keep = (lane & 8) ? u[1] : u[0];
+                s_jf[(k - k0) * 64 + lane] = keep + xor8(send);
+                if (lane == 0) s_jc[k - k0] = e.cj;
             }
         }
+        // the block's j reactions into the 64-bit fixed-point accumulators:
+        // one atomic per (entry, component, j slot), batched so the wave
+        // waits for them once per block
+        __syncthreads();
+        for (int k = k0; k < k1; ++k) {
+            const float r = s_jf[(k - k0) * 64 + lane];
+            if (il < 6 && r != 0.f) fx_add_g(jf_base + (size_t)s_jc[k - k0] * (kClu * 8), (double)r, fscale);
+        }
+        __syncthreads();
         for (int q = 0; q < 3; ++q) { Fs[q] += fs[q]; Fl[q] += fl[q]; }
         E[0] += es; E[1] += el; E[2] += ljs; E[3] += ljl; E[4] += vs; E[5] += vl;
         Vs[0] += vxs; Vs[1] += vys; Vs[2] += vzs;
@@ -237,7 +300,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
         if (a.fs_x) { gst(a.fs_x)[ai] = Fs[0]; gst(a.fs_y)[ai] = Fs[1]; gst(a.fs_z)[ai] = Fs[2]; }
         if (a.fl_x) { gst(a.fl_x)[ai] = Fl[0]; gst(a.fl_y)[ai] = Fl[1]; gst(a.fl_z)[ai] = Fl[2]; }
     }
-    // energies and virials: wavefront sums, halved for the full list
+    // energies and virials: wavefront sums (every pair once)
     double q[14] = {E[0], E[1], E[2], E[3], E[4], E[5], Vs[0], Vs[1], Vs[2], Vl[0], Vl[1], Vl[2],
                     (double)cnt_s, (double)cnt_l};
     for (int off = 1; off < 64; off <<= 1)
@@ -245,21 +308,21 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
     if (lane == 0) {
         MDX_GLOBAL double *o = gst(a.epart) + (size_t)ci * Q_N;
         for (int k = 0; k < Q_N; ++k) o[k] = 0.0;
-        o[Q_PELS2] = 0.5 * q[0];
-        o[Q_PELS1] = 0.5 * q[1];
-        o[Q_PE2] = 0.5 * q[2];
-        o[Q_PE1] = 0.5 * q[3];
-        o[Q_VIR2] = 0.5 * q[4];
-        o[Q_VIR1] = 0.5 * q[5];
-        o[Q_VIRFX] = 0.5 * q[6]; o[Q_VIRFY] = 0.5 * q[7]; o[Q_VIRFZ] = 0.5 * q[8];
-        o[Q_VIRX] = 0.5 * q[9]; o[Q_VIRY] = 0.5 * q[10]; o[Q_VIRZ] = 0.5 * q[11];
-        o[Q_NPS] = 0.5 * q[12];
-        o[Q_NPL] = 0.5 * q[13];
+        o[Q_PELS2] = q[0];
+        o[Q_PELS1] = q[1];
+        o[Q_PE2] = q[2];
+        o[Q_PE1] = q[3];
+        o[Q_VIR2] = q[4];
+        o[Q_VIR1] = q[5];
+        o[Q_VIRFX] = q[6]; o[Q_VIRFY] = q[7]; o[Q_VIRFZ] = q[8];
+        o[Q_VIRX] = q[9]; o[Q_VIRY] = q[10]; o[Q_VIRZ] = q[11];
+        o[Q_NPS] = q[12];
+        o[Q_NPL] = q[13];
     }
     if (DETAIL) {
         // bare Coulomb buckets per unit q_i C -> energies; then each output
-        // column sums its (lane, bucket) entries in lane order (halved: the
-        // full list visits every pair twice)
+        // column sums its (lane, bucket) entries in lane order; the pair
+        // type index is symmetric, so each pair counts once in its bucket
         for (int b = 0; b <= a.nmt; ++b) s_det[(b * 2) * 64 + lane] *= qc;
         __syncthreads();
         for (int o = lane; o < a.ndet; o += 64) {
@@ -276,7 +339,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, int g
                     if (ti == t) sum += s_det[(a.nmt * 2 + k) * 64 + L];
                 }
             }
-            a.dpart[(size_t)ci * a.ndet + o] = 0.5 * sum;
+            a.dpart[(size_t)ci * a.ndet + o] = sum;
         }
     }
 }
@@ -285,8 +348,10 @@ template <bool SOFT, bool DETAIL>
 __global__ __launch_bounds__(64) void k_pairs(const PairArgs *__restrict__ args, const int2 *__restrict__ map,
                                               int group) {
     extern __shared__ float2 s_lj[];      // nlj x nlj (A6, B12), then detail buckets
+    __shared__ float s_jf[16 * 64];       // j reactions of a block of 16 list entries
+    __shared__ int s_jc[16];              // their j-clusters
     const int2 m = map[blockIdx.x];
-    pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, group);
+    pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, s_jf, s_jc, group);
 }
 
 template __global__ void k_pairs<false, false>(const PairArgs *, const int2 *, int);
