@@ -159,7 +159,6 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
     const int a0 = ch * a.chunk;
     const int a1 = min(a0 + a.chunk, a.natoms);
     const int rbase = g * kSkRows;
-    const size_t ts = (size_t)a.tstride;
     // staging assignment: atom pair tid % 16 (atoms 2p, 2p + 1 of the
     // stage), rows tid / 16 + 16 i
     const int sp = tid & 15, sr0 = tid >> 4;
@@ -167,8 +166,10 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
     bool conj[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+        // rows past the last one stage row 0's tables; their sums are
+        // never written
         const int r = rbase + sr0 + 16 * i;
-        offx[i] = -1;
+        offx[i] = 0;
         offy[i] = 0;
         conj[i] = false;
         if (r < a.nfrows) {
@@ -193,27 +194,23 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
     // Z items of a stage: (j, atom pair) for j < 8 ntg, tid + 256 k
     constexpr int ZIT = (NT * 8 * 16 + 255) / 256;
     float4 X[4], Y[4], Zv[ZIT];
+    // Unconditional loads: a chunk's stages never pass the chunk's end
+    // except in the last chunk, whose last stage ends at or before the
+    // 64-atom padded row end, and the padding is zero; rows of j past the
+    // group's N tiles re-read its last row (no matrix instruction uses them).
+    const int jlast = max(0, 8 * ntg - 1);
     auto fetch = [&](int sb) {
-        // two atoms per float4; the tables' rows are padded to a multiple
-        // of 64 atoms, so the pair never leaves its row
-        const int ia = sb + 2 * sp;
-        const bool ok = ia < a1;
+        const unsigned ia = (unsigned)(sb + 2 * sp);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            X[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            Y[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ok && offx[i] >= 0) {
-                X[i] = ldv(tx + ((size_t)offx[i] + ia) / 2);
-                Y[i] = ldv(ty + ((size_t)offy[i] + ia) / 2);
-            }
+            X[i] = ldv(tx + ((unsigned)offx[i] + ia) / 2);
+            Y[i] = ldv(ty + ((unsigned)offy[i] + ia) / 2);
         }
 #pragma unroll
         for (int k = 0; k < ZIT; ++k) {
             const int it = tid + 256 * k;
-            const int j = it >> 4, p = it & 15;
-            const int iz = sb + 2 * p;
-            Zv[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (j < 8 * ntg && iz < a1) Zv[k] = ldv(tz + ((size_t)j * ts + iz) / 2);
+            const unsigned j = (unsigned)min(it >> 4, jlast), p = it & 15;
+            Zv[k] = ldv(tz + (j * (unsigned)a.tstride + (unsigned)sb + 2 * p) / 2);
         }
     };
     auto store = [&](unsigned char *buf) {
@@ -257,7 +254,10 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
     __syncthreads();
     for (int k = 0; k < nsub; ++k) {
         const unsigned char *cur = lds + (k & 1) * St::BYTES;
-        if (k + 1 < nsub) fetch(a0 + (k + 1) * kSkSub);
+        // the last iteration re-fetches and re-stores its own stage into
+        // the idle buffer (unconditional, so the loads stay in flight
+        // across the matrix instructions instead of being copied at once)
+        fetch(a0 + min(k + 1, nsub - 1) * kSkSub);
         if (ntw > 0) {
             const int col = 16 * kA;                // bytes: atoms 8 kA .. 8 kA + 7
             const int ra0 = (2 * (16 * w + rA) + cA) * kSkRowB + col;
@@ -281,7 +281,7 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
                 }
             }
         }
-        if (k + 1 < nsub) store(lds + ((k + 1) & 1) * St::BYTES);
+        store(lds + ((k + 1) & 1) * St::BYTES);
         __syncthreads();
     }
     // results: through LDS so that one lane holds (Re U, Im U, Re V, Im V)
