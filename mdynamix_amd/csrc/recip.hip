@@ -505,34 +505,23 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     const MDX_GLOBAL half8_t *wm = reinterpret_cast<const MDX_GLOBAL half8_t *>(gld(a.wm)) + lane;
     const MDX_GLOBAL float2 *fa0 = gld(a.fa0);
     int cix = -0x7fffffff, ciy = 0;
-    // the next tile's operand is requested while the current one computes;
-    // the array ends with spare steps, so the loads need no bounds
-    half8_t Ah[S16], Al[S16], Ahn[S16], Aln[S16];
-    float2 a0n = make_float2(0.f, 0.f);
-    FTile Tn{};
-    if (t0 < t1) {
-        Tn = ldv(tiles + t0);
+    // Two operand sets in ping-pong: the next tile's operand is requested
+    // (unconditionally: past the slice end the last tile is re-read, and
+    // the array ends with spare steps) while the current one computes.
+    half8_t Ah0[S16], Al0[S16], Ah1[S16], Al1[S16];
+    float2 a00 = make_float2(0.f, 0.f), a01 = make_float2(0.f, 0.f);
+    FTile T0{}, T1{};
+    auto load = [&](int tn, FTile &Tx, half8_t (&Ahx)[S16], half8_t (&Alx)[S16], float2 &a0x) {
+        Tx = ldv(tiles + tn);
 #pragma unroll
         for (int s = 0; s < S16; ++s) {
-            Ahn[s] = wm[Tn.woff / 8 + s * (kForceStepH / 8)];
-            Aln[s] = wm[Tn.woff / 8 + s * (kForceStepH / 8) + 64];
+            Ahx[s] = wm[Tx.woff / 8 + s * (kForceStepH / 8)];
+            Alx[s] = wm[Tx.woff / 8 + s * (kForceStepH / 8) + 64];
         }
-        a0n = ldv(fa0 + Tn.r0 + K);
-    }
-    for (int t = t0; t < t1; ++t) {
-        const FTile T = Tn;
-#pragma unroll
-        for (int s = 0; s < S16; ++s) { Ah[s] = Ahn[s]; Al[s] = Aln[s]; }
-        const float2 a0 = K < T.nrow ? a0n : make_float2(0.f, 0.f);
-        if (t + 1 < t1) {
-            Tn = ldv(tiles + t + 1);
-#pragma unroll
-            for (int s = 0; s < S16; ++s) {
-                Ahn[s] = wm[Tn.woff / 8 + s * (kForceStepH / 8)];
-                Aln[s] = wm[Tn.woff / 8 + s * (kForceStepH / 8) + 64];
-            }
-            a0n = ldv(fa0 + Tn.r0 + K);
-        }
+        a0x = ldv(fa0 + Tx.r0 + K);
+    };
+    auto tile = [&](const FTile &T, const half8_t (&Ah)[S16], const half8_t (&Al)[S16], float2 a0raw) {
+        const float2 a0 = K < T.nrow ? a0raw : make_float2(0.f, 0.f);
         const int ny = T.iy0 + K;
         const bool restart = T.ix != cix || T.iy0 != ciy + 4;
         cix = T.ix;
@@ -582,6 +571,14 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
             Fy[ta] += (double)ny * (double)ts;
             Fz[ta] += (double)tzs;
         }
+    };
+    if (t0 < t1) load(t0, T0, Ah0, Al0, a00);
+    for (int t = t0; t < t1; t += 2) {
+        load(min(t + 1, t1 - 1), T1, Ah1, Al1, a01);
+        tile(T0, Ah0, Al0, a00);
+        if (t + 1 >= t1) break;
+        load(min(t + 2, t1 - 1), T0, Ah0, Al0, a00);
+        tile(T1, Ah1, Al1, a01);
     }
     const double tx = 2.0 * kPI / a.box.l[0], ty = 2.0 * kPI / a.box.l[1], tz = 2.0 * kPI / a.box.l[2];
     MDX_GLOBAL double *o = gst(a.rpart) + (size_t)sl * 3 * a.natoms;
