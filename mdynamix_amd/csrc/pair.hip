@@ -25,6 +25,8 @@
 // with the reference's exact expressions (direct Coulomb, 1-4 scaling).
 #include "mdx_internal.h"
 
+#include <type_traits>
+
 namespace mdx {
 
 // Anchor-relative packed coordinates of every cluster slot.
@@ -153,7 +155,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
     double Fs[3] = {0, 0, 0}, Fl[3] = {0, 0, 0};
     double E[6] = {0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
     double Vs[3] = {0, 0, 0}, Vl[3] = {0, 0, 0};
-    int cnt_s = 0, cnt_l = 0;
+    int cnt_s = 0, cnt_l = 0;   // wave totals (ballot counts)
     // a fast-class call reads only the entries with a short-class pair (the
     // front segment); logical entry k >= ns sits at list_cap - 1 - (k - ns)
     const int ns = sld(a.nshort)[ci];
@@ -181,29 +183,20 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
         float es = 0.f, el = 0.f, ljs = 0.f, ljl = 0.f, vs = 0.f, vl = 0.f;
         float vxs = 0.f, vys = 0.f, vzs = 0.f, vxl = 0.f, vyl = 0.f, vzl = 0.f;
         const int k1 = min(k0 + 16, n);
-        for (int k = k0; k < k1; ++k) {
-            const PairEntry e = e1;
-            const float4 pj = pj1;
-            int2 auxj = aux1;
-            // keeps the LJ-type mask below from being hoisted next to the
-            // prefetch load, which would wait for that load at once
-            asm volatile("" : "+v"(auxj.x));
-            e1 = e2;
-            // unconditional prefetch (past the end it re-reads the last
-            // entry and its j-cluster: valid addresses, values unused)
-            e2 = ldv(L + ent(min(k + 2, n - 1)));
-            pj1 = ldv(xq + e1.cj * kClu + jl);
-            aux1 = ldv(aux + e1.cj * kClu + jl);
+        // one list entry; MIXED: an entry of the front segment (it holds a
+        // short-class pair, so both classes are split by exact 0/1 weights);
+        // otherwise every pair is long-class and only the long sums move
+        auto entry = [&](auto mixed, int kk, const PairEntry &e, const float4 &pj, const int2 &auxj) {
+            constexpr bool MIXED = decltype(mixed)::value;
             unsigned long long m = e.mask;
             if (group == 1) m &= e.mshort;
             else if (group == 2) m &= ~e.mshort;
             // branch-free body: a lane without a pair inside the cutoff
-            // takes rinv = 0, which makes every contribution exactly zero,
-            // and the class split multiplies by exact 0/1 weights, so the
-            // accumulators stay in place (no divergent paths to merge)
+            // takes rinv = 0, which makes every contribution exactly zero
+            // (no divergent paths to merge)
             const int infoj = auxj.x;
             const bool on = (m >> lane) & 1ull;
-            const bool isshort = (e.mshort >> lane) & 1ull;
+            const bool isshort = MIXED && ((e.mshort >> lane) & 1ull);
             const float ws = isshort ? 1.f : 0.f, wl = 1.f - ws;
             float dx = (pi.x - pj.x) + e.sx;
             float dy = (pi.y - pj.y) + e.sy;
@@ -244,33 +237,57 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
                 s_det[(b * 2 + 1) * 64 + lane] += (double)esr;
             }
             const float vxx = fxx * dx, vyy = fyy * dy, vzz = fzz * dz;
-            fs[0] = fmaf(ws, fxx, fs[0]); fs[1] = fmaf(ws, fyy, fs[1]); fs[2] = fmaf(ws, fzz, fs[2]);
-            fl[0] = fmaf(wl, fxx, fl[0]); fl[1] = fmaf(wl, fyy, fl[1]); fl[2] = fmaf(wl, fzz, fl[2]);
-            es = fmaf(ws, ees, es); ljs = fmaf(ws, esr, ljs); vs = fmaf(ws, vr, vs);
-            el = fmaf(wl, ees, el); ljl = fmaf(wl, esr, ljl); vl = fmaf(wl, vr, vl);
-            vxs = fmaf(ws, vxx, vxs); vys = fmaf(ws, vyy, vys); vzs = fmaf(ws, vzz, vzs);
-            vxl = fmaf(wl, vxx, vxl); vyl = fmaf(wl, vyy, vyl); vzl = fmaf(wl, vzz, vzl);
-            cnt_s += (in && isshort) ? 1 : 0;
-            cnt_l += (in && !isshort) ? 1 : 0;
-            const float jx = -fxx, jy = -fyy, jz = -fzz;   // force on this lane's j atom
-            // Newton's third law: the j reactions (short x, y, z, long x, y,
-            // z) summed over the 8 i slots by a reduce-scatter butterfly;
-            // lane (il, jl) ends with component il of j slot jl
-            {
-                float v[8];
-                for (int q = 0; q < 3; ++q) {
-                    const float f = q == 0 ? jx : (q == 1 ? jy : jz);
-                    v[q] = ws * f;
-                    v[3 + q] = wl * f;
-                }
-                v[6] = 0.f; v[7] = 0.f;
-                float w[4], u[2];
-                for (int t = 0; t < 4; ++t) w[t] = swap_sum32(v[t], v[4 + t]);
-                for (int t = 0; t < 2; ++t) u[t] = swap_sum16(w[t], w[2 + t]);
-                const float send = (lane & 8) ? u[0] : u[1], keep = (lane & 8) ? u[1] : u[0];
-                s_jf[(k - k0) * 64 + lane] = keep + xor8(send);
-                if (lane == 0) s_jc[k - k0] = e.cj;
+            const unsigned long long inm = __ballot(in);
+            if (MIXED) {
+                fs[0] = fmaf(ws, fxx, fs[0]); fs[1] = fmaf(ws, fyy, fs[1]); fs[2] = fmaf(ws, fzz, fs[2]);
+                fl[0] = fmaf(wl, fxx, fl[0]); fl[1] = fmaf(wl, fyy, fl[1]); fl[2] = fmaf(wl, fzz, fl[2]);
+                es = fmaf(ws, ees, es); ljs = fmaf(ws, esr, ljs); vs = fmaf(ws, vr, vs);
+                el = fmaf(wl, ees, el); ljl = fmaf(wl, esr, ljl); vl = fmaf(wl, vr, vl);
+                vxs = fmaf(ws, vxx, vxs); vys = fmaf(ws, vyy, vys); vzs = fmaf(ws, vzz, vzs);
+                vxl = fmaf(wl, vxx, vxl); vyl = fmaf(wl, vyy, vyl); vzl = fmaf(wl, vzz, vzl);
+                cnt_s += __popcll(inm & e.mshort);
+                cnt_l += __popcll(inm & ~e.mshort);
+            } else {
+                fl[0] += fxx; fl[1] += fyy; fl[2] += fzz;
+                el += ees; ljl += esr; vl += vr;
+                vxl += vxx; vyl += vyy; vzl += vzz;
+                cnt_l += __popcll(inm);
             }
+            // Newton's third law: the j reactions (short x, y, z, long x, y,
+            // z, two zeros) summed over the 8 i slots by a reduce-scatter
+            // butterfly; lane (il, jl) ends with component il of j slot jl
+            float w[4], u[2];
+            if (MIXED) {
+                w[0] = swap_sum32(-ws * fxx, -wl * fyy);
+                w[1] = swap_sum32(-ws * fyy, -wl * fzz);
+                w[2] = swap_sum32(-ws * fzz, 0.f);
+                w[3] = swap_sum32(-wl * fxx, 0.f);
+            } else {
+                w[0] = swap_sum32(0.f, -fyy);
+                w[1] = swap_sum32(0.f, -fzz);
+                w[2] = 0.f;
+                w[3] = swap_sum32(-fxx, 0.f);
+            }
+            for (int q = 0; q < 2; ++q) u[q] = swap_sum16(w[q], w[2 + q]);
+            const float send = (lane & 8) ? u[0] : u[1], keep = (lane & 8) ? u[1] : u[0];
+            s_jf[kk * 64 + lane] = keep + xor8(send);
+            if (lane == 0) s_jc[kk] = e.cj;
+        };
+        for (int k = k0; k < k1; ++k) {
+            const PairEntry e = e1;
+            const float4 pj = pj1;
+            int2 auxj = aux1;
+            // keeps the LJ-type mask below from being hoisted next to the
+            // prefetch load, which would wait for that load at once
+            asm volatile("" : "+v"(auxj.x));
+            e1 = e2;
+            // unconditional prefetch (past the end it re-reads the last
+            // entry and its j-cluster: valid addresses, values unused)
+            e2 = ldv(L + ent(min(k + 2, n - 1)));
+            pj1 = ldv(xq + e1.cj * kClu + jl);
+            aux1 = ldv(aux + e1.cj * kClu + jl);
+            if (k < ns) entry(std::true_type{}, k - k0, e, pj, auxj);
+            else entry(std::false_type{}, k - k0, e, pj, auxj);
         }
         // the block's j reactions into the 64-bit fixed-point accumulators:
         // one atomic per (entry, component, j slot), batched so the wave
@@ -302,7 +319,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
     }
     // energies and virials: wavefront sums (every pair once)
     double q[14] = {E[0], E[1], E[2], E[3], E[4], E[5], Vs[0], Vs[1], Vs[2], Vl[0], Vl[1], Vl[2],
-                    (double)cnt_s, (double)cnt_l};
+                    lane == 0 ? (double)cnt_s : 0.0, lane == 0 ? (double)cnt_l : 0.0};
     for (int off = 1; off < 64; off <<= 1)
         for (int k = 0; k < 14; ++k) q[k] += __shfl_xor(q[k], off);
     if (lane == 0) {
