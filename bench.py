@@ -355,15 +355,19 @@ def main() -> int:
     dms, dflop = cand[dom]
     achieved = dflop / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
     traffic = measured_traffic(dom)
-    roof = {"kernel": dom, "bound": "valu" if dom == "k_pairs" else "mfma", "achieved": round(achieved, 3),
-            "peak": FP32_PEAK_TFLOPS,
+    # every candidate is compute-bound (a few hundred FLOP per HBM byte), so
+    # the roof is the FP32 compute peak ("mfma" = the compute roof)
+    roof = {"kernel": dom, "bound": "mfma", "compute_unit": "valu" if dom == "k_pairs" else "mfma",
+            "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "avg_ms": round(dms, 4), "algorithmic_flop": dflop,
-            "note": ("k_pairs: 74 FLOP per pair within R_cut (VALU); k_sk / k_recip_force: half of the 21 "
-                     "FLOP per (k, atom) term of FURIR each, computed as FP32 matrix products "
-                     "(v_mfma_f32_16x16x4_f32); peak: MI355X FP32 157.3 TFLOP/s (dense matrix, or packed "
-                     "vector; unpacked FP32 and FP64 vector: 78.6 TFLOP/s); traffic: HBM bytes per launch "
-                     "from profiles/traffic_spce_nacl_32k.md (FETCH_SIZE/WRITE_SIZE, copy-calibrated)")}
+            "note": ("k_pairs: 74 FLOP per pair within R_cut on the vector ALU (FP32, unpacked: its own "
+                     "issue peak is 78.6 TFLOP/s); k_sk / k_recip_force: half of the 21 FLOP per (k, atom) "
+                     "term of FURIR each, computed on the matrix cores as FP32 products split into f16 "
+                     "hi/lo pairs (v_mfma_f32_16x16x32_f16, three products, FP32 accumulation); peak: "
+                     "MI355X FP32 157.3 TFLOP/s (dense matrix or packed vector); traffic: HBM bytes per "
+                     "launch from profiles/traffic_spce_nacl_32k.md (FETCH_SIZE/WRITE_SIZE, "
+                     "copy-calibrated)")}
     out = {
         "metric": "pair-interactions/s (32k SPC/E + NaCl, full force call incl. Ewald k-space)",
         "value": value,
