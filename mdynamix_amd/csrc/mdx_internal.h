@@ -240,9 +240,11 @@ struct RecipArgs {
     Box box;
     double4 *fq;                 // fractional coordinates and charge
     double4 *stepz;              // exp(2 pi i f_z), exp(16 pi i f_z)
-    // phase tables, [index][natoms] (FP64 recurrences rounded to FP32):
+    // phase tables, [index][tstride] (FP64 recurrences rounded to FP32,
+    // rows zero-padded to tstride = a multiple of 64 atoms):
     // tx = q exp(2 pi i n_x f_x), n_x = 0..kx; ty = exp(2 pi i n_y f_y),
-    // n_y = 0..ky; tz = exp(2 pi i j f_z), j = 0..mzp-1
+    // n_y = 0..ky; tz = exp(2 pi i j f_z), j = 0..mzp-1, stored split into
+    // f16 hi/lo: per atom pair 16 B (C hi, D hi, C lo, D lo as half2)
     float2 *tx, *ty, *tz;
     int kx, ky, mzp;
     int tstride;                 // table row stride (atoms, a multiple of 64)

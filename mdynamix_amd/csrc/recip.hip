@@ -18,59 +18,80 @@ namespace mdx {
 
 __device__ inline void recip_prep_body(const RecipArgs &a, int blk) {
     const int i = blk * 256 + threadIdx.x;
-    if (i >= a.natoms) {
+    const bool have = i < a.natoms;
+    const size_t n = (size_t)a.tstride;
+    double c1 = 1.0, s1 = 0.0;
+    if (!have) {
         // the tables' padding atoms are zero (the structure-factor pass
-        // reads atoms in pairs)
+        // reads atoms in pairs and stages up to the padded row end)
         if (i < a.tstride) {
-            const size_t n = (size_t)a.tstride;
             for (int k = 0; k <= a.kx; ++k) stv(gst(a.tx) + i + k * n, make_float2(0.f, 0.f));
             for (int k = 0; k <= a.ky; ++k) stv(gst(a.ty) + i + k * n, make_float2(0.f, 0.f));
-            for (int k = 0; k < a.mzp; ++k) stv(gst(a.tz) + i + k * n, make_float2(0.f, 0.f));
         }
-        return;
-    }
-    const double fx = a.x[i] / a.box.l[0], fy = a.y[i] / a.box.l[1], fz = a.z[i] / a.box.l[2];
-    const double q = a.q[i];
-    stv(gst(a.fq) + i, make_double4(fx, fy, fz, q));
-    // exp(2 pi i f_z) and its 8th power, in FP64
-    const double u = fz - rint(fz);
-    double s1, c1, s8, c8;
-    sincospi(2.0 * u, &s1, &c1);
-    double u8 = 8.0 * u;
-    u8 -= rint(u8);
-    sincospi(2.0 * u8, &s8, &c8);
-    stv(gst(a.stepz) + i, make_double4(c1, s1, c8, s8));
-    // phase tables (FP64 recurrences from FP64 sincospi steps, FP32 values)
-    const size_t n = (size_t)a.tstride;
-    {
-        double sr = 0.0, si = 0.0;
-        sincospi(2.0 * (fx - rint(fx)), &si, &sr);
-        double pr = q, pi = 0.0;
-        MDX_GLOBAL float2 *t = gst(a.tx) + i;
-        for (int k = 0; k <= a.kx; ++k) {
-            stv(t + k * n, make_float2((float)pr, (float)pi));
-            const double w = pr * sr - pi * si;
-            pi = pr * si + pi * sr;
-            pr = w;
+    } else {
+        const double fx = a.x[i] / a.box.l[0], fy = a.y[i] / a.box.l[1], fz = a.z[i] / a.box.l[2];
+        const double q = a.q[i];
+        stv(gst(a.fq) + i, make_double4(fx, fy, fz, q));
+        // exp(2 pi i f_z) and its 8th power, in FP64
+        const double u = fz - rint(fz);
+        double s8, c8;
+        sincospi(2.0 * u, &s1, &c1);
+        double u8 = 8.0 * u;
+        u8 -= rint(u8);
+        sincospi(2.0 * u8, &s8, &c8);
+        stv(gst(a.stepz) + i, make_double4(c1, s1, c8, s8));
+        // phase tables (FP64 recurrences from FP64 sincospi steps, FP32 values)
+        {
+            double sr = 0.0, si = 0.0;
+            sincospi(2.0 * (fx - rint(fx)), &si, &sr);
+            double pr = q, pi = 0.0;
+            MDX_GLOBAL float2 *t = gst(a.tx) + i;
+            for (int k = 0; k <= a.kx; ++k) {
+                stv(t + k * n, make_float2((float)pr, (float)pi));
+                const double w = pr * sr - pi * si;
+                pi = pr * si + pi * sr;
+                pr = w;
+            }
+        }
+        {
+            double sr = 0.0, si = 0.0;
+            sincospi(2.0 * (fy - rint(fy)), &si, &sr);
+            double pr = 1.0, pi = 0.0;
+            MDX_GLOBAL float2 *t = gst(a.ty) + i;
+            for (int k = 0; k <= a.ky; ++k) {
+                stv(t + k * n, make_float2((float)pr, (float)pi));
+                const double w = pr * sr - pi * si;
+                pi = pr * si + pi * sr;
+                pr = w;
+            }
         }
     }
+    // z table, already split for the matrix instructions: per (j, atom
+    // pair) 16 bytes = half2 (C hi), (D hi), (C lo), (D lo) of the pair's
+    // two atoms, p_j = C + i D = exp(2 pi i j f_z) from the FP64 recurrence
+    // rounded to FP32, hi = f16(v), lo = f16(v - hi); the even lane of each
+    // pair writes the record (every lane of the wavefront takes part)
     {
-        double sr = 0.0, si = 0.0;
-        sincospi(2.0 * (fy - rint(fy)), &si, &sr);
         double pr = 1.0, pi = 0.0;
-        MDX_GLOBAL float2 *t = gst(a.ty) + i;
-        for (int k = 0; k <= a.ky; ++k) {
-            stv(t + k * n, make_float2((float)pr, (float)pi));
-            const double w = pr * sr - pi * si;
-            pi = pr * si + pi * sr;
-            pr = w;
-        }
-    }
-    {
-        double pr = 1.0, pi = 0.0;
-        MDX_GLOBAL float2 *t = gst(a.tz) + i;
+        uint4 *t = reinterpret_cast<uint4 *>(a.tz);
+        const bool odd = i & 1;
         for (int k = 0; k < a.mzp; ++k) {
-            stv(t + k * n, make_float2((float)pr, (float)pi));
+            const float c = have ? (float)pr : 0.f, d = have ? (float)pi : 0.f;
+            const _Float16 hc = (_Float16)c, hd = (_Float16)d;
+            const _Float16 lc = (_Float16)(c - (float)hc), ld = (_Float16)(d - (float)hd);
+            const unsigned hi2 = (unsigned)__builtin_bit_cast(unsigned short, hc) |
+                                 ((unsigned)__builtin_bit_cast(unsigned short, hd) << 16);
+            const unsigned lo2 = (unsigned)__builtin_bit_cast(unsigned short, lc) |
+                                 ((unsigned)__builtin_bit_cast(unsigned short, ld) << 16);
+            const unsigned ohi = __shfl_xor(hi2, 1), olo = __shfl_xor(lo2, 1);
+            if (!odd && i < a.tstride) {
+                uint4 r;
+                r.x = (hi2 & 0xffffu) | (ohi << 16);            // C hi of atoms i, i + 1
+                r.y = (hi2 >> 16) | (ohi & 0xffff0000u);         // D hi
+                r.z = (lo2 & 0xffffu) | (olo << 16);            // C lo
+                r.w = (lo2 >> 16) | (olo & 0xffff0000u);         // D lo
+                stv(gst(t) + (k * n + i) / 2, r);
+            }
             const double w = pr * c1 - pi * s1;
             pi = pr * s1 + pi * c1;
             pr = w;
@@ -141,7 +162,7 @@ struct SkStage {
 // takes 16 rows = 2 M tiles and as many N tiles as its first, longest row
 // needs) and one chunk of atoms, staged through LDS 32 atoms at a time:
 // B = tx[n_x] ty[|n_y|] (conjugated for n_y < 0) formed in FP32, then
-// split, and p_j = tz[j] split.  Operand layout of the instruction (lane
+// split; p_j = tz[j] comes split from k_recip_prep.  Operand layout of the instruction (lane
 // l): A[l % 16][8 (l / 16) + k], B[8 (l / 16) + k][l % 16], k = 0..7;
 // result D[4 (l / 16) + v][l % 16].  Stages alternate between two LDS
 // buffers, the next one fetched into registers while the current one feeds
@@ -199,19 +220,32 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
     // 64-atom padded row end, and the padding is zero; rows of j past the
     // group's N tiles re-read its last row (no matrix instruction uses them).
     const int jlast = max(0, 8 * ntg - 1);
+    // per-thread byte offsets (16 B = two atoms) are fixed for the whole
+    // chunk; a stage only moves the uniform base (scalar base + 32-bit
+    // vector offset addressing; every table is far below 4 GiB)
+    unsigned ox[4], oy[4], oz[ZIT];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        ox[i] = ((unsigned)offx[i] / 2 + sp) * 16u;
+        oy[i] = ((unsigned)offy[i] / 2 + sp) * 16u;
+    }
+#pragma unroll
+    for (int k = 0; k < ZIT; ++k) {
+        const int it = tid + 256 * k;
+        oz[k] = ((unsigned)min(it >> 4, jlast) * ((unsigned)a.tstride / 2) + (it & 15)) * 16u;
+    }
+    auto at = [](const MDX_GLOBAL float4 *base, unsigned off) {
+        return ldv(reinterpret_cast<const MDX_GLOBAL float4 *>(reinterpret_cast<const MDX_GLOBAL char *>(base) + off));
+    };
     auto fetch = [&](int sb) {
-        const unsigned ia = (unsigned)(sb + 2 * sp);
+        const MDX_GLOBAL float4 *txs = tx + sb / 2, *tys = ty + sb / 2, *tzs = tz + sb / 2;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            X[i] = ldv(tx + ((unsigned)offx[i] + ia) / 2);
-            Y[i] = ldv(ty + ((unsigned)offy[i] + ia) / 2);
+            X[i] = at(txs, ox[i]);
+            Y[i] = at(tys, oy[i]);
         }
 #pragma unroll
-        for (int k = 0; k < ZIT; ++k) {
-            const int it = tid + 256 * k;
-            const unsigned j = (unsigned)min(it >> 4, jlast), p = it & 15;
-            Zv[k] = ldv(tz + (j * (unsigned)a.tstride + (unsigned)sb + 2 * p) / 2);
-        }
+        for (int k = 0; k < ZIT; ++k) Zv[k] = at(tzs, oz[k]);
     };
     auto store = [&](unsigned char *buf) {
 #pragma unroll
@@ -234,9 +268,9 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
             const int it = tid + 256 * k;
             const int j = it >> 4, p = it & 15;
             if (j < NT * 8) {
-                half2_t hc, lc, hd, ld;
-                split_f16(Zv[k].x, Zv[k].z, hc, lc);
-                split_f16(Zv[k].y, Zv[k].w, hd, ld);
+                // the z table is stored split (k_recip_prep): C hi, D hi, C lo, D lo
+                const half2_t hc = __builtin_bit_cast(half2_t, Zv[k].x), hd = __builtin_bit_cast(half2_t, Zv[k].y);
+                const half2_t lc = __builtin_bit_cast(half2_t, Zv[k].z), ld = __builtin_bit_cast(half2_t, Zv[k].w);
                 const int o = (2 * j) * kSkRowB + 4 * p;
                 *reinterpret_cast<half2_t *>(buf + St::ZH + o) = hc;
                 *reinterpret_cast<half2_t *>(buf + St::ZH + o + kSkRowB) = hd;
