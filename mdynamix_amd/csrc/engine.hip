@@ -104,6 +104,7 @@ struct mdx_engine {
     void *d_cub = nullptr;
     size_t cub_bytes = 0;
     int *d_clu_atom = nullptr, *d_atom_slot = nullptr, *d_clu_col = nullptr;
+    double4 *d_spos = nullptr;                    // per slot: FP64 x, y, z, molecule (list build)
     double *d_anchor = nullptr;
     float *d_ext = nullptr;
     int cand_cap = 0, list_cap = 0;
@@ -530,6 +531,7 @@ mdx_status alloc_clusters(mdx_engine *h) {
     CHK(dalloc(h, &h->d_colclu_n, h->ncol));
     CHK(dalloc(h, &h->d_colclu, h->ncol));
     CHK(dalloc(h, &h->d_clu_atom, (size_t)h->nclu_cap * kClu));
+    CHK(dalloc(h, &h->d_spos, (size_t)h->nclu_cap * kClu));
     CHK(dalloc(h, &h->d_atom_slot, h->n));
     // the force combine reads atom_slot; valid (zero) before the first list
     if (!h->dry) HIPCHK(h, hipMemset(h->d_atom_slot, 0, sizeof(int) * h->n));
@@ -1167,6 +1169,7 @@ mdx_status mdx_build_list(mdx_engine *h) {
                                           h->d_atom_slot, h->d_clu_col);
     k_cluster_geom<<<cdiv(h->nclu, 256), 256, 0, st>>>(h->nclu, h->d_clu_atom, h->d_x, h->d_y, h->d_z, h->box,
                                                         h->d_anchor, h->d_ext);
+    k_slot_pos<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, h->d_x, h->d_y, h->d_z, h->d_mol, h->d_spos);
     const double rl2 = list_radius2(h);
     const double rl = std::sqrt(rl2);
     const double cwx = h->p.box[0] / h->ncx, cwy = h->p.box[1] / h->ncy;
@@ -1176,8 +1179,8 @@ mdx_status mdx_build_list(mdx_engine *h) {
         k_candidates<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_anchor, h->d_ext, h->d_clu_col, h->d_colclu,
                                                           h->d_colclu_n, h->ncx, h->ncy, rx, ry, h->box, rl,
                                                           h->cand_cap, h->d_cand, h->d_ncand, h->d_overflow);
-        k_masks<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_cand, h->d_ncand, h->cand_cap, h->d_clu_atom, h->d_anchor, h->d_x,
-                                        h->d_y, h->d_z, h->d_mol, h->d_moltype, h->d_type_ee, h->d_excl_start,
+        k_masks<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_cand, h->d_ncand, h->cand_cap, h->d_clu_atom, h->d_spos,
+                                        h->d_anchor, h->d_moltype, h->d_type_ee, h->d_excl_start,
                                         h->d_excl_atom, h->d_excl_is14, h->box, rl2,
                                         h->p.rshort * h->p.rshort, h->d_list, h->d_nlist, h->d_nshort,
                                         h->list_cap, h->d_overflow);

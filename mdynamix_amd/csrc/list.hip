@@ -155,8 +155,7 @@ __device__ inline int bound_kind(int i, int j, const int *__restrict__ excl_star
 // i-cluster, lane = i_slot * 8 + j_slot.
 __global__ __launch_bounds__(64) void k_masks(
     int nclu, const int *__restrict__ cand, const int *__restrict__ ncand, int cap,
-    const int *__restrict__ clu_atom, const double *__restrict__ anchor, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const int *__restrict__ atom_mol,
+    const int *__restrict__ clu_atom, const double4 *__restrict__ spos, const double *__restrict__ anchor,
     const int *__restrict__ atom_moltype, const int *__restrict__ type_ee, const int *__restrict__ excl_start,
     const int *__restrict__ excl_atom, const signed char *__restrict__ excl_is14, Box b, double rl2,
     double short2, PairEntry *__restrict__ list, int *__restrict__ nlist, int *__restrict__ nshort,
@@ -166,46 +165,41 @@ __global__ __launch_bounds__(64) void k_masks(
     const int lane = threadIdx.x;
     const int il = lane >> 3, jl = lane & 7;
     const int ai = clu_atom[ci * kClu + il];
-    double xi = 0, yi = 0, zi = 0;
-    int moli = -1, eei = 1;
-    if (ai >= 0) {
-        xi = x[ai]; yi = y[ai]; zi = z[ai];
-        moli = atom_mol[ai];
-        eei = type_ee[atom_moltype[ai]];
-    }
+    const double4 pi = spos[ci * kClu + il];
+    const int moli = (int)pi.w;
+    const int eei = ai >= 0 ? type_ee[atom_moltype[ai]] : 1;
     const double aci[3] = {anchor[ci * 3], anchor[ci * 3 + 1], anchor[ci * 3 + 2]};
     const int nc = ncand[ci];
-    const int *cl = cand + (size_t)ci * cap;
-    // software pipeline: candidate indices two ahead, the j atom one ahead
-    // (its coordinates are requested one iteration before use)
-    int c1 = nc > 0 ? cl[0] : 0, c2 = nc > 1 ? cl[1] : 0;
-    int a1 = nc > 0 ? clu_atom[c1 * kClu + jl] : -1;
-    double x1 = 0.0, y1 = 0.0, z1 = 0.0;
-    if (a1 >= 0) { x1 = x[a1]; y1 = y[a1]; z1 = z[a1]; }
-    int a2 = nc > 1 ? clu_atom[c2 * kClu + jl] : -1;
+    const MDX_CONST int *cl = sld(cand) + (size_t)ci * cap;
+    // software pipeline: candidate indices two ahead (scalar loads), the j
+    // slot's coordinates one ahead; the loads are unconditional (past the
+    // end they re-read the last candidate) so none is waited for at once
+    const int nl = max(nc - 1, 0);
+    int c1 = nc > 0 ? cl[0] : ci, c2 = cl[min(1, nl)];
+    if (nc == 0) c2 = ci;
+    double4 p1 = spos[c1 * kClu + jl];
     // entries with a short-class pair fill the list from the front, the
     // others from the back (in discovery order both), so a fast-class call
     // reads only the front segment
     int cnt = 0, cnt_s = 0;
     for (int k = 0; k < nc; ++k) {
-        const int cj = c1, aj = a1;
-        const double xj = x1, yj = y1, zj = z1;
+        const int cj = c1;
+        const double4 pj = p1;
         c1 = c2;
-        a1 = a2;
-        if (a1 >= 0) { x1 = x[a1]; y1 = y[a1]; z1 = z[a1]; }
-        if (k + 2 < nc) {
-            c2 = cl[k + 2];
-            a2 = clu_atom[c2 * kClu + jl];
-        }
+        p1 = spos[c1 * kClu + jl];
+        c2 = cl[min(k + 2, nl)];
         bool listed = false, shrt = false;
-        if (ai >= 0 && aj >= 0 && ai != aj && (cj != ci || il < jl)) {
-            double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+        // an empty slot has molecule -1; (cj != ci || il < jl) keeps each
+        // pair once and leaves out an atom with itself
+        if (moli >= 0 && pj.w >= 0.0 && (cj != ci || il < jl)) {
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             min_image(dx, dy, dz, b);
             double rr2 = dx * dx + dy * dy;
             rr2 = rr2 + dz * dz;
             listed = rr2 <= rl2;
             shrt = rr2 <= short2;
-            if (listed && moli == atom_mol[aj]) {
+            if (listed && moli == (int)pj.w) {
+                const int aj = clu_atom[cj * kClu + jl];
                 if (rr2 <= 64.0 && bound_kind(ai, aj, excl_start, excl_atom, excl_is14) != 0) listed = false;
                 else if (eei == 0) listed = false;
             }
@@ -236,6 +230,17 @@ __global__ __launch_bounds__(64) void k_masks(
         nshort[ci] = min(cnt_s, list_cap);
         if (cnt > list_cap) atomicMax(overflow, cnt);
     }
+}
+
+// Per-slot FP64 coordinates (the caller's, as the reference differences
+// them) and molecule index (-1: empty slot) for the mask pass.
+__global__ void k_slot_pos(int nslot, const int *__restrict__ clu_atom, const double *__restrict__ x,
+                           const double *__restrict__ y, const double *__restrict__ z,
+                           const int *__restrict__ atom_mol, double4 *__restrict__ spos) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    const int a = clu_atom[s];
+    spos[s] = a >= 0 ? make_double4(x[a], y[a], z[a], (double)atom_mol[a]) : make_double4(0.0, 0.0, 0.0, -1.0);
 }
 
 // Special pairs (1-4 pairs, intramolecular pairs of lambda-coupled
