@@ -15,19 +15,26 @@
 
 namespace mdx {
 
+// Block sums of NQ values in a fixed order: a butterfly inside each
+// wavefront (xor 32, 16, ..., 1), then the wavefronts' sums in wavefront
+// order (blocks of at most 1024 threads); one barrier for all NQ values.
 template <int NQ>
 __device__ inline void block_reduce_store(double (&q)[NQ], const int (&slot)[NQ], double *out) {
-    __shared__ double red[256];
+    __shared__ double wred[16][NQ];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = 0; i < NQ; ++i) {
-        red[threadIdx.x] = q[i];
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[slot[i]] = red[0];
-        __syncthreads();
+        double v = q[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) wred[w][i] = v;
     }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double t = wred[0][threadIdx.x];
+        for (int k = 1; k < nw; ++k) t += wred[k][threadIdx.x];
+        out[slot[threadIdx.x]] = t;
+    }
+    __syncthreads();
 }
 
 __device__ inline void zero_slots(double *o) {
@@ -301,12 +308,24 @@ __device__ inline void combine_body(const CombineArgs &a, int blk, int group) {
     block_reduce_store<2>(v, slots, o);
 }
 
-// Sum rows [0, nrows) of epart; block q sums slot q with a fixed
-// strided-then-tree order (256 threads).
+// Sum rows [0, nrows) of epart; block q sums slot q in a fixed order:
+// thread t keeps four running sums over rows t + 256 (4 i + u), u = 0..3
+// (four loads in flight per thread), adds them as ((s0 + s1) + s2) + s3,
+// then a tree over the 256 threads.
 __device__ inline void ereduce_body(const ReduceArgs &a, int q, double *red) {
-    double s = 0.0;
-    for (int r = threadIdx.x; r < a.nrows; r += 256) s += a.epart[(size_t)r * Q_N + q];
-    red[threadIdx.x] = s;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const double *e = a.epart + q;
+    int r = threadIdx.x;
+    for (; r + 768 < a.nrows; r += 1024) {
+        s0 += e[(size_t)r * Q_N];
+        s1 += e[(size_t)(r + 256) * Q_N];
+        s2 += e[(size_t)(r + 512) * Q_N];
+        s3 += e[(size_t)(r + 768) * Q_N];
+    }
+    if (r < a.nrows) s0 += e[(size_t)r * Q_N];
+    if (r + 256 < a.nrows) s1 += e[(size_t)(r + 256) * Q_N];
+    if (r + 512 < a.nrows) s2 += e[(size_t)(r + 512) * Q_N];
+    red[threadIdx.x] = ((s0 + s1) + s2) + s3;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
