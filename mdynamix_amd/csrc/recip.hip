@@ -477,7 +477,8 @@ __global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__
 // Lane l then holds 2^e (S1, S2, Z1, Z2) of row l / 16 of the tile and atom
 // l % 16, and adds
 //   t = b_i A0_r - b_r A0_i + b_r S1 + b_i S2,  t_z = b_r Z1 + b_i Z2
-// (A0: n_z = 0) into F_x += n_x t, F_y += n_y t, F_z += t_z in FP64; b
+// (A0: n_z = 0) into F_x += n_x t, F_y += n_y t, F_z += t_z (FP32 sums
+// over a run of tiles of one n_x, added into FP64 when the run ends); b
 // advances over the 4-row tiles of one n_x by exp(8 pi i f_y) (FP64) and
 // restarts from an FP64 sincospi at every new n_x.  A wavefront takes
 // kForceTiles tiles of 16 atoms, a block 4 wavefronts, each block one slice
@@ -497,6 +498,9 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     half8_t Ph[kForceTiles][S16], Pl[kForceTiles][S16];
     double w4r[kForceTiles], w4i[kForceTiles], br[kForceTiles], bi[kForceTiles];
     double Fx[kForceTiles], Fy[kForceTiles], Fz[kForceTiles];
+    // FP32 sums over the current run of tiles of one n_x (sum t, sum n_y t,
+    // sum t_z), added into the FP64 forces when the run ends
+    float Sx[kForceTiles], Sy[kForceTiles], Sz[kForceTiles];
 #pragma unroll
     for (int ta = 0; ta < kForceTiles; ++ta) {
         const int i = abase + 16 * ta;
@@ -534,6 +538,7 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         w4i[ta] = s4;
         br[ta] = bi[ta] = 0.0;
         Fx[ta] = Fy[ta] = Fz[ta] = 0.0;
+        Sx[ta] = Sy[ta] = Sz[ta] = 0.f;
     }
     const MDX_CONST FTile *tiles = sld(a.ftiles);
     const MDX_GLOBAL half8_t *wm = reinterpret_cast<const MDX_GLOBAL half8_t *>(gld(a.wm)) + lane;
@@ -558,6 +563,16 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         const float2 a0 = K < T.nrow ? a0raw : make_float2(0.f, 0.f);
         const int ny = T.iy0 + K;
         const bool restart = T.ix != cix || T.iy0 != ciy + 4;
+        if (restart) {
+            // the run of tiles of one n_x ends: its FP32 sums into FP64
+#pragma unroll
+            for (int ta = 0; ta < kForceTiles; ++ta) {
+                Fx[ta] += (double)cix * (double)Sx[ta];
+                Fy[ta] += (double)Sy[ta];
+                Fz[ta] += (double)Sz[ta];
+                Sx[ta] = Sy[ta] = Sz[ta] = 0.f;
+            }
+        }
         cix = T.ix;
         ciy = T.iy0;
         mfma_f4 acc[kForceTiles];
@@ -601,9 +616,9 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
             const float bre = (float)br[ta], bim = (float)bi[ta];
             const float ts = (bim * a0.x - bre * a0.y) + (bre * acc[ta][0] + bim * acc[ta][1]) * isc;
             const float tzs = (bre * acc[ta][2] + bim * acc[ta][3]) * isc;
-            Fx[ta] += (double)T.ix * (double)ts;
-            Fy[ta] += (double)ny * (double)ts;
-            Fz[ta] += (double)tzs;
+            Sx[ta] += ts;
+            Sy[ta] = fmaf((float)ny, ts, Sy[ta]);
+            Sz[ta] += tzs;
         }
     };
     if (t0 < t1) load(t0, T0, Ah0, Al0, a00);
@@ -613,6 +628,12 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         if (t + 1 >= t1) break;
         load(min(t + 2, t1 - 1), T0, Ah0, Al0, a00);
         tile(T1, Ah1, Al1, a01);
+    }
+#pragma unroll
+    for (int ta = 0; ta < kForceTiles; ++ta) {
+        Fx[ta] += (double)cix * (double)Sx[ta];
+        Fy[ta] += (double)Sy[ta];
+        Fz[ta] += (double)Sz[ta];
     }
     const double tx = 2.0 * kPI / a.box.l[0], ty = 2.0 * kPI / a.box.l[1], tz = 2.0 * kPI / a.box.l[2];
     MDX_GLOBAL double *o = gst(a.rpart) + (size_t)sl * 3 * a.natoms;
