@@ -117,7 +117,8 @@ __device__ inline float xor8(float x) {
 // keeps its own FP64 buckets in LDS (bucket = type of the j molecule, or
 // the intramolecular bucket), so the sums stay in a fixed order.
 template <bool SOFT, bool DETAIL>
-__device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float *s_jf, int *s_jc, int group) {
+__device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float *s_jf, int *s_jc, double *s_acc,
+                                  int group) {
     const int lane = threadIdx.x;
     const int nlj2 = a.nlj * a.nlj;
     for (int t = lane; t < nlj2; t += 64) s_lj[t] = ldv(gld(a.lj) + t);
@@ -154,7 +155,9 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
 
     double Fs[3] = {0, 0, 0}, Fl[3] = {0, 0, 0};
     double E[6] = {0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
-    double Vs[3] = {0, 0, 0}, Vl[3] = {0, 0, 0};
+    // the FP64 virial sums per lane live in LDS (touched once per block of
+    // 16 entries; registers stay free for the pair loop): short xyz, long xyz
+    for (int q = 0; q < 6; ++q) s_acc[q * 64 + lane] = 0.0;
     int cnt_s = 0, cnt_l = 0;   // wave totals (ballot counts)
     // a fast-class call reads only the entries with a short-class pair (the
     // front segment); logical entry k >= ns sits at list_cap - 1 - (k - ns)
@@ -300,8 +303,10 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
         __syncthreads();
         for (int q = 0; q < 3; ++q) { Fs[q] += fs[q]; Fl[q] += fl[q]; }
         E[0] += es; E[1] += el; E[2] += ljs; E[3] += ljl; E[4] += vs; E[5] += vl;
-        Vs[0] += vxs; Vs[1] += vys; Vs[2] += vzs;
-        Vl[0] += vxl; Vl[1] += vyl; Vl[2] += vzl;
+        {
+            const float part[6] = {vxs, vys, vzs, vxl, vyl, vzl};
+            for (int q = 0; q < 6; ++q) s_acc[q * 64 + lane] += (double)part[q];
+        }
     }
     // i forces: sum over the 8 lanes of one i slot
     for (int off = 1; off < 8; off <<= 1)
@@ -311,6 +316,8 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
         }
     const int ai = gld(a.clu_atom)[ci * kClu + il];
     const double qc = ai >= 0 ? gld(a.q)[ai] * a.coulf : 0.0;
+    double Vs[3], Vl[3];
+    for (int q = 0; q < 3; ++q) { Vs[q] = s_acc[q * 64 + lane]; Vl[q] = s_acc[(3 + q) * 64 + lane]; }
     E[0] *= qc;
     E[1] *= qc;
     if (jl == 0 && ai >= 0) {
@@ -367,8 +374,9 @@ __global__ __launch_bounds__(64) void k_pairs(const PairArgs *__restrict__ args,
     extern __shared__ float2 s_lj[];      // nlj x nlj (A6, B12), then detail buckets
     __shared__ float s_jf[16 * 64];       // j reactions of a block of 16 list entries
     __shared__ int s_jc[16];              // their j-clusters
+    __shared__ double s_acc[6 * 64];      // per-lane FP64 virial sums
     const int2 m = map[blockIdx.x];
-    pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, s_jf, s_jc, group);
+    pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, s_jf, s_jc, s_acc, group);
 }
 
 template __global__ void k_pairs<false, false>(const PairArgs *, const int2 *, int);
