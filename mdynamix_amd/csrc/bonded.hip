@@ -308,30 +308,29 @@ __device__ inline void combine_body(const CombineArgs &a, int blk, int group) {
     block_reduce_store<2>(v, slots, o);
 }
 
-// Sum rows [0, nrows) of epart; block q sums slot q in a fixed order:
-// thread t keeps four running sums over rows t + 256 (4 i + u), u = 0..3
-// (four loads in flight per thread), adds them as ((s0 + s1) + s2) + s3,
-// then a tree over the 256 threads.
-__device__ inline void ereduce_body(const ReduceArgs &a, int q, double *red) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+// Sum rows [0, nrows) of epart; block q (1,024 threads) sums slot q in a
+// fixed order: thread t keeps eight running sums over rows t + 1024 (8 i +
+// u), u = 0..7 (eight loads in flight per thread), adds them pairwise,
+// then butterflies inside each wavefront and the 16 wavefronts' sums in
+// order.
+__device__ inline void ereduce_body(const ReduceArgs &a, int q, double *wred) {
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const double *e = a.epart + q;
-    int r = threadIdx.x;
-    for (; r + 768 < a.nrows; r += 1024) {
-        s0 += e[(size_t)r * Q_N];
-        s1 += e[(size_t)(r + 256) * Q_N];
-        s2 += e[(size_t)(r + 512) * Q_N];
-        s3 += e[(size_t)(r + 768) * Q_N];
-    }
-    if (r < a.nrows) s0 += e[(size_t)r * Q_N];
-    if (r + 256 < a.nrows) s1 += e[(size_t)(r + 256) * Q_N];
-    if (r + 512 < a.nrows) s2 += e[(size_t)(r + 512) * Q_N];
-    red[threadIdx.x] = ((s0 + s1) + s2) + s3;
+    for (int r0 = threadIdx.x; r0 < a.nrows; r0 += 8192)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int r = r0 + 1024 * u;
+            if (r < a.nrows) s[u] += e[(size_t)r * Q_N];
+        }
+    double v = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = wred[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t += wred[w];
+        a.out[q] = t;
     }
-    if (threadIdx.x == 0) a.out[q] = red[0];
 }
 
 // Clear accumulators (a kernel, not a memset node, so the call sequence
@@ -402,10 +401,10 @@ __global__ __launch_bounds__(256) void k_combine(const CombineArgs *__restrict__
     combine_body(args[m.x], m.y, group);
 }
 
-__global__ __launch_bounds__(256) void k_ereduce(const ReduceArgs *__restrict__ args, const int2 *__restrict__ map) {
-    __shared__ double red[256];
+__global__ __launch_bounds__(1024) void k_ereduce(const ReduceArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ double wred[16];
     const int2 m = map[blockIdx.x];
-    ereduce_body(args[m.x], m.y, red);
+    ereduce_body(args[m.x], m.y, wred);
 }
 
 __global__ void k_fill_i32(int n, int *__restrict__ p, int v) {

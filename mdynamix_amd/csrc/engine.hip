@@ -1531,7 +1531,7 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
     }
     mark(8);
     if (b->comb.nblocks()) k_combine<<<b->comb.nblocks(), 256, 0, st>>>(b->comb.d_args, b->comb.d_map, group);
-    launch(b, b->red, 256, 0, k_ereduce);
+    launch(b, b->red, 1024, 0, k_ereduce);
     hipMemcpyAsync(b->h_energy, b->d_energy, sizeof(double) * Q_N * b->sys.size(), hipMemcpyDeviceToHost, st);
     if (detail) {
         launch(b, b->dred, 256, 0, k_detail_reduce);
