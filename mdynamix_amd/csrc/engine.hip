@@ -318,6 +318,9 @@ mdx_status set_chunks(mdx_engine *h, int want) {
 mdx_status setup_kspace(mdx_engine *h) {
     std::vector<int4> ks;
     enumerate_k(h->p, ks);
+    // FURIR returns at once for a single molecule (NTYPES = 1 and
+    // NSPEC(1) = 1, forces.f:1266): no reciprocal term at all
+    if (h->ntypes == 1 && h->nmol == 1) ks.clear();
     h->nkv = (int)ks.size();
     if (h->nkv > 4000000) {
         h->err = "too many k-vectors";

@@ -363,7 +363,7 @@ int64_t orc_inifur(const mdx_topology *t, const mdx_params *p, double *rkx, doub
 void orc_furir_task(const mdx_topology *t, const mdx_params *p, const double *x, const double *y,
                     const double *z, const double *rkx, const double *rky, const double *rkz, int64_t nkv,
                     int taskid, int numtask, double *gx, double *gy, double *gz, mdx_energies *e) {
-    if (nkv == 0 || t->nmols == 1) return;
+    if (nkv == 0 || (t->nmoltypes == 1 && t->nmols == 1)) return;   /* forces.f:1266 */
     const int n = t->natoms;
     double *ts = (double *)malloc(sizeof(double) * n), *tc = (double *)malloc(sizeof(double) * n);
     const double alp2 = p->alpha * p->alpha;
