@@ -10,7 +10,11 @@ in HBM.  Synthetic seeded coordinates (jittered lattice, random
 orientations); nothing is loaded from disk.
 
 Prints one JSON line (metric pair-interactions/s; ns/day, per-stage times,
-roofline of the dominant kernel and the CPU baseline beside it), plus the
+roofline of the dominant kernel and the CPU baseline beside it: the
+reference's own code, built from its sources by oracle/ref/Makefile, on
+one host core and a 1/12 replicated-data stripe, with the FP64 port on the
+same stripe beside it; the port alone when the reference build is
+absent), plus the
 batched form (`batched`: 32 path-integral beads x 12,000 atoms per fused
 force call), the multiple-time-step MD segment (`respa`: 47,776-atom
 chains-in-water box, 2 fs long steps of 10 short steps, ms per long step
@@ -60,6 +64,33 @@ def build_engine(seed: int = 2025):
     eng = ForceEngine(topo, prm)
     eng.set_positions(s.pos)
     return s, topo, prm, eng
+
+
+def cpu_reference(s, numtask: int = 12):
+    """The reference itself (its forces.f, main.f set-up and serial MPI
+    stubs compiled from source by oracle/ref/Makefile with amdflang -O3,
+    no fast-math) on one host core: CHCNB and one ALLFORCE of rank 0 of
+    `numtask` replicated-data ranks (the reference's own striding:
+    forces.f:1825-1830, 1273-1277, 1501-1502), wall-clocked inside the
+    program and scaled by numtask to a full force call, with the list
+    amortised over its 10-step rebuild interval."""
+    from oracle import refdump
+    r = refdump.time_reference_stripe(s, numtask, timeout=600.0)
+    t_call = numtask * r["t_force"] + numtask * r["t_list"] / 10.0
+    pairs_call = (r["mbsh"] + r["mbln"]) * numtask
+    return {
+        "value": pairs_call / t_call,
+        "unit": "pair-interactions/s",
+        "cores": 1,
+        "kind": "reference",
+        "ns_per_day": (1.0 / t_call) * 2.0 * 0.0864,
+        "seconds_per_step": t_call,
+        "sample": ("reference forces.f (amdflang -O3 serial build, oracle/_ref/mdx_time) on rank 0 of %d "
+                   "replicated-data stripes of the same 96,174-atom box, one core: CHCNB %.2f s "
+                   "(%d + %d listed pairs), ALLFORCE %.2f s (%.1f s sampled, %.1f s with the program's "
+                   "start-up), scaled x%d" % (numtask, r["t_list"], r["mbsh"], r["mbln"], r["t_force"],
+                                            r["t_list"] + r["t_force"], r["t_total"], numtask)),
+    }
 
 
 def cpu_baseline(s, topo, prm, numtask: int = 12):
@@ -402,7 +433,20 @@ def main() -> int:
     if not args.no_served:
         out["served"] = bench_served(args.serve_jobs, args.serve_steps, args.serve_rate)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, topo, prm, args.cpu_tasks)
+        # the reference itself when its timing build is present (built from
+        # the reference sources by oracle/ref/Makefile), beside the FP64
+        # port (the oracle) on the same stripe
+        port = cpu_baseline(s, topo, prm, args.cpu_tasks)
+        try:
+            ref = cpu_reference(s, args.cpu_tasks)
+            ref["port"] = {k: port[k] for k in ("value", "seconds_per_step", "sample")}
+            out["cpu_baseline"] = ref
+        except Exception as exc:   # binary absent or the run failed: the port alone
+            port["reference_unavailable"] = str(exc).splitlines()[0][:200]
+            out["cpu_baseline"] = port
+        print("cpu baseline (%s): %.3g pair-interactions/s" % (out["cpu_baseline"]["kind"],
+                                                               out["cpu_baseline"]["value"]),
+              file=sys.stderr, flush=True)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out))
     return 0

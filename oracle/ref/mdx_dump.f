@@ -18,6 +18,7 @@
 *
       SUBROUTINE SULTAN
       include "common_inc.f"
+      call MDXTIM
       open(unit=77,file='mdx_dump.bin',access='stream',
      +     form='unformatted',status='replace')
       call CHCNB(.false.)
@@ -38,6 +39,7 @@
 *
       SUBROUTINE DOUBLE
       include "common_inc.f"
+      call MDXTIM
       open(unit=77,file='mdx_dump.bin',access='stream',
      +     form='unformatted',status='replace')
       call CHCNB(.false.)
@@ -48,6 +50,32 @@
       call MDXLST
       call MDXFRC
       close(77)
+      stop
+      END
+*
+*  Timing mode (CPU baseline): with MDX_TIME_STRIPES=N in the environment
+*  the run takes the share of rank 0 of N replicated-data ranks (the
+*  reference's own striding: CHCNB outer atoms, forces.f:1825-1830; FURIR
+*  k vectors, forces.f:1273-1277; ETERMS, forces.f:1501-1502), times
+*  CHCNB and one ALLFORCE with the wall clock, prints
+*  "MDXTIME N MBSH MBLN t_list t_force" and stops.  No dump is written.
+      SUBROUTINE MDXTIM
+      include "common_inc.f"
+      character*16 ENVV
+      integer LENV,NSTR,IST
+      integer*8 IC0,IC1,IC2,IRATE
+      call GET_ENVIRONMENT_VARIABLE('MDX_TIME_STRIPES',ENVV,LENV,IST)
+      if(IST.ne.0.or.LENV.le.0)return
+      read(ENVV,*)NSTR
+      NUMTASK=NSTR
+      TASKID=0
+      call SYSTEM_CLOCK(IC0,IRATE)
+      call CHCNB(.false.)
+      call SYSTEM_CLOCK(IC1)
+      call ALLFORCE
+      call SYSTEM_CLOCK(IC2)
+      write(*,'(a,3i12,2f14.6)')' MDXTIME ',NSTR,MBSH,MBLN,
+     +  dble(IC1-IC0)/dble(IRATE),dble(IC2-IC1)/dble(IRATE)
       stop
       END
 *

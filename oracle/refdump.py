@@ -1,7 +1,8 @@
 """Reader for the binary dump written by the reference harness
 (oracle/ref/mdx_dump.f) and driver that runs the reference program on a
-generated system.  Test infrastructure: used only by tests/ and by the
-fixture generator under tests/golden/.
+generated system.  Test infrastructure: used only by tests/, by the
+fixture generator under tests/golden/ and by the cpu_baseline leg of
+bench.py (time_reference_stripe).
 
 Record layout: 8-byte name, int32 kind (1 = int32, 2 = float64),
 int32 count, count values.  Names that occur twice (per-call arrays of the
@@ -71,3 +72,30 @@ def run_trajectory(system, nsteps: int, workdir: str | None = None, timeout: flo
     out = read_dump(dump)
     out["_stdout"] = [np.array(res.stdout)]
     return out
+
+
+TIME_EXE = os.path.join(HERE, "_ref", "mdx_time")
+
+
+def time_reference_stripe(system, stripes: int, workdir: str | None = None, timeout: float = 300.0) -> Dict:
+    """Run the reference (timing build of the dump harness, -O3) on rank 0's
+    share of `stripes` replicated-data ranks: its own CHCNB and one
+    ALLFORCE, timed inside the program with the wall clock.  Returns
+    {"t_list", "t_force", "mbsh", "mbln", "t_total"}; raises if the binary
+    is missing or the run fails."""
+    import time
+    if not os.path.exists(TIME_EXE):
+        raise FileNotFoundError("reference timing binary not built: run make -C oracle/ref")
+    d = tempfile.mkdtemp(prefix="mdxtime_") if workdir is None else workdir
+    system.write_reference_inputs(d)
+    t0 = time.perf_counter()
+    res = subprocess.run([TIME_EXE, "md.in"], cwd=d, capture_output=True, text=True, timeout=timeout,
+                         env=dict(os.environ, MDX_TIME_STRIPES=str(int(stripes))))
+    t_total = time.perf_counter() - t0
+    for line in res.stdout.splitlines():
+        f = line.split()
+        if len(f) == 6 and f[0] == "MDXTIME":
+            return {"stripes": int(f[1]), "mbsh": int(f[2]), "mbln": int(f[3]), "t_list": float(f[4]),
+                    "t_force": float(f[5]), "t_total": t_total}
+    raise RuntimeError("reference timing run failed:\n" + res.stdout[-2000:] + res.stderr[-1000:])
+
