@@ -93,6 +93,32 @@ def cpu_reference(s, numtask: int = 12):
     }
 
 
+def cpu_reference_mpi(s, ranks: int):
+    """The reference's MPI build (its forces.f with mpi_double.f, amdflang
+    -O3 against MPICH, oracle/_ref/mdx_time_mpi) as `ranks` processes on the
+    host cores: one full CHCNB and ALLFORCE of the same system with its
+    replicated-data reductions, wall-clocked between barriers on rank 0;
+    the list amortised over its 10-step rebuild interval."""
+    from oracle import refdump
+    r = refdump.time_reference_mpi(s, ranks, timeout=600.0)
+    t_call = r["t_force"] + r["t_list"] / 10.0
+    pairs_call = (r["mbsh"] + r["mbln"]) * r["ranks"]
+    return {
+        "value": pairs_call / t_call,
+        "unit": "pair-interactions/s",
+        "cores": r["ranks"],
+        "kind": "reference",
+        "ns_per_day": (1.0 / t_call) * 2.0 * 0.0864,
+        "seconds_per_step": t_call,
+        "sample": ("reference MDynaMix MPI build (forces.f + mpi_double.f, amdflang -O3, MPICH; "
+                   "oracle/_ref/mdx_time_mpi) as %d ranks on %d host cores: one full CHCNB %.2f s and "
+                   "ALLFORCE %.2f s of the same 96,174-atom box (%.1f s with the program's start-up); "
+                   "pairs = rank 0's %d + %d listed pairs x %d" %
+                   (r["ranks"], r["ranks"], r["t_list"], r["t_force"], r["t_total"], r["mbsh"], r["mbln"],
+                    r["ranks"])),
+    }
+
+
 def cpu_baseline(s, topo, prm, numtask: int = 12):
     """The FP64 oracle (a scalar C port of the reference force path) on one
     host core, on the share of one of `numtask` replicated-data ranks
@@ -300,6 +326,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-tasks", type=int, default=12)
+    ap.add_argument("--cpu-ranks", type=int, default=16,
+                    help="MPI ranks of the reference CPU baseline (the host cores of one GPU's share)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-respa", action="store_true")
@@ -437,13 +465,25 @@ def main() -> int:
         # the reference sources by oracle/ref/Makefile), beside the FP64
         # port (the oracle) on the same stripe
         port = cpu_baseline(s, topo, prm, args.cpu_tasks)
+        brief = lambda r: {k: r[k] for k in ("value", "cores", "seconds_per_step", "sample")}
+        single = mpi = None
+        notes = []
         try:
-            ref = cpu_reference(s, args.cpu_tasks)
-            ref["port"] = {k: port[k] for k in ("value", "seconds_per_step", "sample")}
-            out["cpu_baseline"] = ref
-        except Exception as exc:   # binary absent or the run failed: the port alone
-            port["reference_unavailable"] = str(exc).splitlines()[0][:200]
-            out["cpu_baseline"] = port
+            single = cpu_reference(s, args.cpu_tasks)
+        except Exception as exc:   # binary absent or the run failed
+            notes.append("single-core reference: " + str(exc).splitlines()[0][:160])
+        try:
+            mpi = cpu_reference_mpi(s, min(args.cpu_ranks, os.cpu_count() or 1))
+        except Exception as exc:
+            notes.append("MPI reference: " + str(exc).splitlines()[0][:160])
+        cb = mpi or single or port
+        if cb is not single and single is not None:
+            cb["single_core"] = brief(single)
+        if cb is not port:
+            cb["port"] = brief(port)
+        if notes:
+            cb["unavailable"] = notes
+        out["cpu_baseline"] = cb
         print("cpu baseline (%s): %.3g pair-interactions/s" % (out["cpu_baseline"]["kind"],
                                                                out["cpu_baseline"]["value"]),
               file=sys.stderr, flush=True)

@@ -53,12 +53,15 @@
       stop
       END
 *
-*  Timing mode (CPU baseline): with MDX_TIME_STRIPES=N in the environment
-*  the run takes the share of rank 0 of N replicated-data ranks (the
-*  reference's own striding: CHCNB outer atoms, forces.f:1825-1830; FURIR
-*  k vectors, forces.f:1273-1277; ETERMS, forces.f:1501-1502), times
-*  CHCNB and one ALLFORCE with the wall clock, prints
-*  "MDXTIME N MBSH MBLN t_list t_force" and stops.  No dump is written.
+*  Timing mode (CPU baseline), set by MDX_TIME_STRIPES=N in the
+*  environment.  N > 0 (serial build): the run takes the share of rank 0
+*  of N replicated-data ranks (the reference's own striding: CHCNB outer
+*  atoms, forces.f:1825-1830; FURIR k vectors, forces.f:1273-1277; ETERMS,
+*  forces.f:1501-1502).  N = 0 (MPI build): the ranks of the MPI job.
+*  Times CHCNB and one ALLFORCE (with its reductions) with the wall clock
+*  between barriers; rank 0 prints
+*  "MDXTIME ranks MBSH MBLN t_list t_force" (its own list lengths); then
+*  the program ends.  No dump is written.
       SUBROUTINE MDXTIM
       include "common_inc.f"
       character*16 ENVV
@@ -67,16 +70,20 @@
       call GET_ENVIRONMENT_VARIABLE('MDX_TIME_STRIPES',ENVV,LENV,IST)
       if(IST.ne.0.or.LENV.le.0)return
       read(ENVV,*)NSTR
-      NUMTASK=NSTR
-      TASKID=0
+      if(NSTR.gt.0)then
+        NUMTASK=NSTR
+        TASKID=0
+      end if
+      call BARRIER
       call SYSTEM_CLOCK(IC0,IRATE)
       call CHCNB(.false.)
       call SYSTEM_CLOCK(IC1)
       call ALLFORCE
+      call BARRIER
       call SYSTEM_CLOCK(IC2)
-      write(*,'(a,3i12,2f14.6)')' MDXTIME ',NSTR,MBSH,MBLN,
-     +  dble(IC1-IC0)/dble(IRATE),dble(IC2-IC1)/dble(IRATE)
-      stop
+      if(TASKID.eq.0)write(*,'(a,3i12,2f14.6)')' MDXTIME ',NUMTASK,
+     +  MBSH,MBLN,dble(IC1-IC0)/dble(IRATE),dble(IC2-IC1)/dble(IRATE)
+      call FINAL
       END
 *
 *  Unused integrator pieces of mdstep.f: never reached from this harness.

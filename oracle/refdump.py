@@ -75,13 +75,41 @@ def run_trajectory(system, nsteps: int, workdir: str | None = None, timeout: flo
 
 
 TIME_EXE = os.path.join(HERE, "_ref", "mdx_time")
+TIME_MPI_EXE = os.path.join(HERE, "_ref", "mdx_time_mpi")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def _parse_time(res, t_total) -> Dict:
+    for line in res.stdout.splitlines():
+        f = line.split()
+        if len(f) == 6 and f[0] == "MDXTIME":
+            return {"ranks": int(f[1]), "mbsh": int(f[2]), "mbln": int(f[3]), "t_list": float(f[4]),
+                    "t_force": float(f[5]), "t_total": t_total}
+    raise RuntimeError("reference timing run failed:\n" + res.stdout[-2000:] + res.stderr[-1000:])
+
+
+def time_reference_mpi(system, ranks: int, workdir: str | None = None, timeout: float = 300.0) -> Dict:
+    """Run the reference's MPI build (timing harness linked to mpi_double.f
+    and MPICH) as `ranks` processes on the host cores: its own CHCNB and
+    one ALLFORCE with the replicated-data reductions, timed between
+    barriers on rank 0.  Returns {"ranks", "t_list", "t_force", "mbsh",
+    "mbln" (rank 0's list lengths), "t_total"}."""
+    import time
+    if not (os.path.exists(TIME_MPI_EXE) and os.path.exists(MPIEXEC)):
+        raise FileNotFoundError("reference MPI timing binary or mpiexec missing")
+    d = tempfile.mkdtemp(prefix="mdxmpi_") if workdir is None else workdir
+    system.write_reference_inputs(d)
+    t0 = time.perf_counter()
+    res = subprocess.run([MPIEXEC, "-n", str(int(ranks)), TIME_MPI_EXE, "md.in"], cwd=d, capture_output=True,
+                         text=True, timeout=timeout, env=dict(os.environ, MDX_TIME_STRIPES="0"))
+    return _parse_time(res, time.perf_counter() - t0)
 
 
 def time_reference_stripe(system, stripes: int, workdir: str | None = None, timeout: float = 300.0) -> Dict:
     """Run the reference (timing build of the dump harness, -O3) on rank 0's
     share of `stripes` replicated-data ranks: its own CHCNB and one
     ALLFORCE, timed inside the program with the wall clock.  Returns
-    {"t_list", "t_force", "mbsh", "mbln", "t_total"}; raises if the binary
+    {"ranks", "t_list", "t_force", "mbsh", "mbln", "t_total"}; raises if the binary
     is missing or the run fails."""
     import time
     if not os.path.exists(TIME_EXE):
@@ -91,11 +119,5 @@ def time_reference_stripe(system, stripes: int, workdir: str | None = None, time
     t0 = time.perf_counter()
     res = subprocess.run([TIME_EXE, "md.in"], cwd=d, capture_output=True, text=True, timeout=timeout,
                          env=dict(os.environ, MDX_TIME_STRIPES=str(int(stripes))))
-    t_total = time.perf_counter() - t0
-    for line in res.stdout.splitlines():
-        f = line.split()
-        if len(f) == 6 and f[0] == "MDXTIME":
-            return {"stripes": int(f[1]), "mbsh": int(f[2]), "mbln": int(f[3]), "t_list": float(f[4]),
-                    "t_force": float(f[5]), "t_total": t_total}
-    raise RuntimeError("reference timing run failed:\n" + res.stdout[-2000:] + res.stderr[-1000:])
+    return _parse_time(res, time.perf_counter() - t0)
 
