@@ -1457,7 +1457,10 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         ca.Hx = h->d_out + 6 * (size_t)n; ca.Hy = ca.Hx + n; ca.Hz = ca.Hx + 2 * n;
         ca.epart = erow(h, h->row_c);
         b->comb.add(ca, cdiv(n, 256));
-        b->red.add(ReduceArgs{h->rows_total, h->d_epart, b->d_energy + (size_t)s * Q_N}, Q_N);
+        // 8 blocks per energy slot, of which only every eighth works: the
+        // working blocks of one system then share an XCD (block index mod
+        // 8) and its L2, which reads the interleaved rows from HBM once
+        b->red.add(ReduceArgs{h->rows_total, h->d_epart, b->d_energy + (size_t)s * Q_N}, Q_N * 8);
     }
     CHK(kplan_upload(b, b->zero, maps)); CHK(kplan_upload(b, b->pack, maps)); CHK(kplan_upload(b, b->com, maps));
     CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->special, maps));
