@@ -116,6 +116,9 @@ __global__ __launch_bounds__(64) void k_candidates(int nclu, const double *__res
         for (int ox = 0; ox < nx; ++ox) {
             const int cx = (sx + ox) % ncx;
             const int c2 = cy * ncx + cx;
+            // clusters are numbered column by column, so a column below the
+            // i-cluster's own holds only j < i: none of its pairs is kept
+            if (c2 < col) continue;
             const int beg = colclu[c2], num = colclu_n[c2];
             for (int base = 0; base < num; base += 64) {
                 const int cj = beg + base + lane;
