@@ -172,6 +172,7 @@ __global__ __launch_bounds__(64) void k_masks(
     const int moli = (int)pi.w;
     const int eei = ai >= 0 ? type_ee[atom_moltype[ai]] : 1;
     const double aci[3] = {anchor[ci * 3], anchor[ci * 3 + 1], anchor[ci * 3 + 2]};
+    const double ilb[3] = {1.0 / b.l[0], 1.0 / b.l[1], 1.0 / b.l[2]};
     const int nc = ncand[ci];
     const MDX_CONST int *cl = sld(cand) + (size_t)ci * cap;
     // software pipeline: candidate indices two ahead (scalar loads), the j
@@ -216,8 +217,11 @@ __global__ __launch_bounds__(64) void k_masks(
                 e.cj = cj;
                 double s[3];
                 for (int q = 0; q < 3; ++q) {
+                    // any image of the anchor difference serves: the pair
+                    // kernel re-applies the minimum image per pair whenever
+                    // half the box comes near the list radius
                     double d = aci[q] - anchor[cj * 3 + q];
-                    s[q] = d - b.l[q] * rint(d / b.l[q]);
+                    s[q] = d - b.l[q] * rint(d * ilb[q]);
                 }
                 e.sx = (float)s[0]; e.sy = (float)s[1]; e.sz = (float)s[2];
                 e.mask = m;
