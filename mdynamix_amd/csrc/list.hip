@@ -217,9 +217,10 @@ __global__ __launch_bounds__(64) void k_masks(
                 e.cj = cj;
                 double s[3];
                 for (int q = 0; q < 3; ++q) {
-                    // any image of the anchor difference serves: the pair
-                    // kernel re-applies the minimum image per pair whenever
-                    // half the box comes near the list radius
+                    // d * (1/L) and d / L round to the same image unless the
+                    // anchors are near half a box apart; then half the box is
+                    // near the list radius and the pair kernel re-applies the
+                    // minimum image per pair, so either image serves
                     double d = aci[q] - anchor[cj * 3 + q];
                     s[q] = d - b.l[q] * rint(d * ilb[q]);
                 }
