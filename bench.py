@@ -75,7 +75,7 @@ def cpu_reference(s, numtask: int = 12):
     program and scaled by numtask to a full force call, with the list
     amortised over its 10-step rebuild interval."""
     from oracle import refdump
-    r = refdump.time_reference_stripe(s, numtask, timeout=600.0)
+    r = refdump.time_reference_stripe(s, numtask, timeout=180.0)
     t_call = numtask * r["t_force"] + numtask * r["t_list"] / 10.0
     pairs_call = (r["mbsh"] + r["mbln"]) * numtask
     return {
@@ -100,7 +100,7 @@ def cpu_reference_mpi(s, ranks: int):
     replicated-data reductions, wall-clocked between barriers on rank 0;
     the list amortised over its 10-step rebuild interval."""
     from oracle import refdump
-    r = refdump.time_reference_mpi(s, ranks, timeout=600.0)
+    r = refdump.time_reference_mpi(s, ranks, timeout=180.0)
     t_call = r["t_force"] + r["t_list"] / 10.0
     pairs_call = (r["mbsh"] + r["mbln"]) * r["ranks"]
     return {
