@@ -138,6 +138,17 @@ typedef struct mdx_topology {
     const double  *tp_k;          /* FT                                      */
     const double  *tp_delta;      /* RT, radians                             */
     const double  *tp_k1, *tp_k2, *tp_k3, *tp_k4, *tp_k5;   /* FT1..FT5      */
+    /* flexible SPC water of the "fSPC" kind (STRBND, forces.f:530-800,
+       called from GETBND for types with IPOT = 1 or 2, whose bonds and
+       angles are then not listed, setup.f:297): one record per molecule,
+       atoms O, H1, H2 and 12 parameters R_OH1, R_OH2, R_HH (RB of the
+       type's bonds 1-3), A, B (DB of bonds 1, 2 for IPOT 2, FB for 1),
+       C (FB of bond 3), rho1, rho2 (RO of bonds 1, 2), E, G (the fixed
+       cross constants), IPOT, LMOVE; energies in the units of the other
+       bonded parameters.  nspc = 0 when no such type exists. */
+    int32_t nspc;
+    const int32_t *spc_atom;      /* [3 nspc] */
+    const double  *spc_par;       /* [12 nspc] */
 } mdx_topology;
 
 /* Run parameters (COMMON /SIZES/, /ENERGY/ COULF, /MDEE/ RAEX). */

@@ -58,6 +58,7 @@ class MdxTopology(C.Structure):
         ("tors_param", _pi), ("ntorsparams", C.c_int32), ("tp_kind", _pi), ("tp_mult", _pi),
         ("tp_k", _pd), ("tp_delta", _pd), ("tp_k1", _pd), ("tp_k2", _pd), ("tp_k3", _pd),
         ("tp_k4", _pd), ("tp_k5", _pd),
+        ("nspc", C.c_int32), ("spc_atom", _pi), ("spc_par", _pd),
     ]
 
 
@@ -197,6 +198,9 @@ class TopologyArrays:
     tp_k3: np.ndarray
     tp_k4: np.ndarray
     tp_k5: np.ndarray
+    # fSPC water (STRBND): per molecule atoms O, H1, H2 and 12 parameters
+    spc_atom: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3), np.int32))
+    spc_par: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 12)))
     _keep: Optional[list] = None
 
     @property
@@ -250,6 +254,9 @@ class TopologyArrays:
         s.nangleparams = len(self.ap_k)
         s.ntors = len(self.tors_i)
         s.ntorsparams = len(self.tp_kind)
+        s.nspc = int(len(self.spc_atom))
+        s.spc_atom = ptr(np.asarray(self.spc_atom).ravel(), "i")
+        s.spc_par = ptr(np.asarray(self.spc_par).ravel(), "d")
         self._keep = keep
         return s
 

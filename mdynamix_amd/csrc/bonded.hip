@@ -2,6 +2,7 @@
 //
 // k_eterms   ETERMS  forces.f:1482-1676 (one bound/intramolecular pair per thread)
 // k_bonds    GETBND  forces.f:113-204
+// k_strbnd   STRBND  forces.f:530-800 (fSPC water, one molecule per thread)
 // k_angles   GETANG  forces.f:216-309
 // k_tors     GETTRS  forces.f:323-513
 // All in FP64 with the reference's expressions; forces are scattered in
@@ -111,6 +112,74 @@ __device__ inline void bonds_body(const BondArgs &t, int blk) {
         fx_add(a.fz + i, -bz * forb, a.fscale);
         fx_add(a.fx + j, bx * forb, a.fscale); fx_add(a.fy + j, by * forb, a.fscale);
         fx_add(a.fz + j, bz * forb, a.fscale);
+    }
+    double *o = a.epart + (size_t)blk * Q_N;
+    zero_slots(o);
+    const int slots[5] = {Q_PINT, Q_VIRB, Q_VIRFX, Q_VIRFY, Q_VIRFZ};
+    block_reduce_store<5>(v, slots, o);
+}
+
+
+// Morse (IPOT 2, minimum image) or harmonic (IPOT 1, no image) O-H
+// stretches, harmonic H-H and the cross terms E sa sh, E sb sh, G sa sb.
+// Beyond R_OH > 1.3 A (BBMA, a REAL*4 constant) the Morse force takes the
+// harmonic form -2 s A rho1^2 while the energy stays Morse; the reference
+// uses rho1 for both bonds there, and so does this kernel.
+__device__ inline void strbnd_body(const SpcArgs &t, int blk) {
+    const ScatterArgs &a = t.s;
+    const int n = blk * blockDim.x + threadIdx.x;
+    double v[5] = {0, 0, 0, 0, 0};
+    if (n < t.n) {
+        const int i = t.atom[3 * n], j = t.atom[3 * n + 1], k = t.atom[3 * n + 2];
+        const double *q = t.par + 12 * (size_t)n;
+        const double roh1 = q[0], roh2 = q[1], rhh = q[2], aaa = q[3], bbb = q[4], ccc = q[5];
+        const double rho1 = q[6], rho2 = q[7], eee = q[8], ggg = q[9];
+        const bool morse = q[10] == 2.0, lmove = q[11] != 0.0;
+        double ax = a.x[j] - a.x[i], ay = a.y[j] - a.y[i], az = a.z[j] - a.z[i];
+        double bx = a.x[k] - a.x[i], by = a.y[k] - a.y[i], bz = a.z[k] - a.z[i];
+        double cx = a.x[k] - a.x[j], cy = a.y[k] - a.y[j], cz = a.z[k] - a.z[j];
+        if (morse) {
+            min_image(ax, ay, az, a.box); min_image(bx, by, bz, a.box); min_image(cx, cy, cz, a.box);
+        }
+        const double ass = sqrt(ax * ax + ay * ay + az * az);
+        const double bss = sqrt(bx * bx + by * by + bz * bz);
+        const double css = sqrt(cx * cx + cy * cy + cz * cz);
+        const double sa = ass - roh1, sb = bss - roh2, sh = css - rhh;
+        double wa, wb, fa, fb;
+        if (morse) {
+            const double bbm = (double)1.3f;
+            const double ea = exp(-rho1 * sa), eam = 1.0 - ea;
+            const double eb = exp(-rho2 * sb), ebm = 1.0 - eb;
+            wa = aaa * (eam * eam) + eee * sa * sh;
+            wb = bbb * (ebm * ebm) + eee * sb * sh;
+            const double ad = aaa * rho1, bd = bbb * rho2;
+            fa = ass > bbm ? -2. * sa * ad * rho1 : -2. * ad * ea * eam;
+            fb = bss > bbm ? -2. * sb * bd * rho1 : -2. * bd * eb * ebm;
+        } else {
+            wa = aaa * sa * sa + eee * sa * sh;
+            wb = bbb * sb * sb + eee * sb * sh;
+            fa = -2.0 * aaa * sa;
+            fb = -2.0 * bbb * sb;
+        }
+        fa = (fa - eee * sh - ggg * sb) / ass;
+        fb = (fb - eee * sh - ggg * sa) / bss;
+        const double fc = (-2.0 * ccc * sh - eee * sa - eee * sb) / css;
+        v[0] = wa + wb + (ccc * sh * sh + ggg * sa * sb);
+        if (lmove) {
+            const double fax = fa * ax, fay = fa * ay, faz = fa * az;
+            const double fbx = fb * bx, fby = fb * by, fbz = fb * bz;
+            const double fcx = fc * cx, fcy = fc * cy, fcz = fc * cz;
+            fx_add(a.fx + i, -fax - fbx, a.fscale); fx_add(a.fy + i, -fay - fby, a.fscale);
+            fx_add(a.fz + i, -faz - fbz, a.fscale);
+            fx_add(a.fx + j, fax - fcx, a.fscale); fx_add(a.fy + j, fay - fcy, a.fscale);
+            fx_add(a.fz + j, faz - fcz, a.fscale);
+            fx_add(a.fx + k, fbx + fcx, a.fscale); fx_add(a.fy + k, fby + fcy, a.fscale);
+            fx_add(a.fz + k, fbz + fcz, a.fscale);
+            v[2] = ax * fax + bx * fbx + cx * fcx;
+            v[3] = ay * fay + by * fby + cy * fcy;
+            v[4] = az * faz + bz * fbz + cz * fcz;
+            v[1] = v[2] + v[3] + v[4];
+        }
     }
     double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
@@ -390,6 +459,7 @@ __device__ inline void springs_body(const SpringArgs &a, int blk) {
 
 MDX_BATCHED(k_eterms, EtermArgs, eterms_body, 256)
 MDX_BATCHED(k_bonds, BondArgs, bonds_body, 256)
+MDX_BATCHED(k_strbnd, SpcArgs, strbnd_body, 256)
 MDX_BATCHED(k_angles, AngleArgs, angles_body, 256)
 MDX_BATCHED(k_tors, TorsArgs, tors_body, 256)
 MDX_BATCHED(k_springs, SpringArgs, springs_body, 256)

@@ -72,7 +72,7 @@ struct mdx_engine {
     int neterm = 0;
     int2 *d_et_pairs = nullptr;
     // bonded
-    int nb = 0, na = 0, nt = 0;
+    int nb = 0, na = 0, nt = 0, nspc = 0;
     std::vector<void *> owned;
     // argument structs of the force kernels, refreshed whenever a list
     // rebuild or box change moves a pointer or count (version counts those)
@@ -82,11 +82,12 @@ struct mdx_engine {
     RecipArgs a_recip{};
     EtermArgs a_et{};
     BondArgs a_bonds{};
+    SpcArgs a_spc{};
     AngleArgs a_angles{};
     TorsArgs a_tors{};
     long long version = 1;
     // per-system rows of the energy partials (fixed by capacities)
-    int row_pairs = 0, row_sp = 0, row_sk = 0, row_et = 0, row_b = 0, row_a = 0, row_t = 0, row_spr = 0,
+    int row_pairs = 0, row_sp = 0, row_sk = 0, row_et = 0, row_b = 0, row_s = 0, row_a = 0, row_t = 0, row_spr = 0,
         row_c = 0, rows_total = 0;
     unsigned long long *d_fixsp = nullptr;        // [3][n] bead springs
     std::map<void *, size_t> dev_sizes;           // live device allocations
@@ -203,6 +204,7 @@ struct mdx_batch {
     int mz = 1;
     KPlan<EtermArgs> et;
     KPlan<BondArgs> bonds;
+    KPlan<SpcArgs> spc;
     KPlan<AngleArgs> angles;
     KPlan<TorsArgs> tors;
     KPlan<SpringArgs> springs;
@@ -588,6 +590,7 @@ void layout_rows(mdx_engine *h) {
     h->row_sk = r; r += cdiv(h->npk + h->nfrows, 256);
     h->row_et = r; r += h->neterm ? cdiv(h->neterm, 256) : 0;
     h->row_b = r; r += h->nb ? cdiv(h->nb, 256) : 0;
+    h->row_s = r; r += h->nspc ? cdiv(h->nspc, 256) : 0;
     h->row_a = r; r += h->na ? cdiv(h->na, 256) : 0;
     h->row_t = r; r += h->nt ? cdiv(h->nt, 256) : 0;
     h->row_spr = r; r += cdiv(h->n, 256);
@@ -772,6 +775,8 @@ void fill_args(mdx_engine *h) {
     ScatterArgs fast{h->d_x, h->d_y, h->d_z, h->box, h->fscale, fixHB, fixHB + n, fixHB + 2 * n, nullptr};
     h->a_bonds.s = fast;
     h->a_bonds.s.epart = erow(h, h->row_b);
+    h->a_spc.s = fast;
+    h->a_spc.s.epart = erow(h, h->row_s);
     h->a_angles.s = fast;
     h->a_angles.s.epart = erow(h, h->row_a);
     h->a_tors.s = fast;
@@ -834,6 +839,11 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
         for (int k = 0; k < t->nbonds; ++k)
             if (t->bond_i[k] < 0 || t->bond_i[k] >= n || t->bond_j[k] < 0 || t->bond_j[k] >= n ||
                 t->bond_param[k] < 0 || t->bond_param[k] >= t->nbondparams) return bad("bond");
+        if (t->nspc < 0) return bad("nspc");
+        for (int k = 0; k < 3 * t->nspc; ++k)
+            if (t->spc_atom[k] < 0 || t->spc_atom[k] >= n) return bad("spc_atom");
+        for (int k = 0; k < t->nspc; ++k)
+            if (t->spc_par[12 * k + 10] != 1.0 && t->spc_par[12 * k + 10] != 2.0) return bad("spc_par IPOT");
         for (int k = 0; k < t->nangles; ++k)
             if (t->angle_i[k] < 0 || t->angle_i[k] >= n || t->angle_j[k] < 0 || t->angle_j[k] >= n ||
                 t->angle_k[k] < 0 || t->angle_k[k] >= n || t->angle_param[k] < 0 ||
@@ -971,6 +981,10 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
     if ((s = upload(h, &dp, t->bp_k, t->nbondparams))) return fail(s); h->a_bonds.k = dp;
     if ((s = upload(h, &dp, t->bp_d, t->nbondparams))) return fail(s); h->a_bonds.d = dp;
     if ((s = upload(h, &dp, t->bp_rho, t->nbondparams))) return fail(s); h->a_bonds.rho = dp;
+    h->nspc = t->nspc;
+    h->a_spc.n = h->nspc;
+    if ((s = upload(h, &ip, t->spc_atom, 3 * (size_t)h->nspc))) return fail(s); h->a_spc.atom = ip;
+    if ((s = upload(h, &dp, t->spc_par, 12 * (size_t)h->nspc))) return fail(s); h->a_spc.par = dp;
     h->a_angles.n = h->na;
     if ((s = upload(h, &ip, t->angle_i, h->na))) return fail(s); h->a_angles.ai = ip;
     if ((s = upload(h, &ip, t->angle_j, h->na))) return fail(s); h->a_angles.aj = ip;
@@ -1383,7 +1397,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     b->zero.clear(); b->pack.clear(); b->com.clear(); b->pairs.clear(); b->special.clear();
     b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
     b->mz = 1;
-    b->et.clear(); b->bonds.clear(); b->angles.clear(); b->tors.clear();
+    b->et.clear(); b->bonds.clear(); b->spc.clear(); b->angles.clear(); b->tors.clear();
     b->springs.clear(); b->comb.clear(); b->red.clear(); b->dred.clear();
     b->nmt_max = 1;
     b->nlj_max = 1;
@@ -1413,6 +1427,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         b->mz = std::max(b->mz, h->mz);
         if (h->neterm) b->et.add(h->a_et, cdiv(h->neterm, 256));
         if (h->nb) b->bonds.add(h->a_bonds, cdiv(h->nb, 256));
+        if (h->nspc) b->spc.add(h->a_spc, cdiv(h->nspc, 256));
         if (h->na) b->angles.add(h->a_angles, cdiv(h->na, 256));
         if (h->nt) b->tors.add(h->a_tors, cdiv(h->nt, 256));
         if (b->ring && K > 1) {
@@ -1466,7 +1481,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->special, maps));
     CHK(kplan_upload(b, b->rprep, maps)); CHK(kplan_upload(b, b->sk, maps));
     CHK(kplan_upload(b, b->skred, maps)); CHK(kplan_upload(b, b->rforce, maps));
-    CHK(kplan_upload(b, b->et, maps)); CHK(kplan_upload(b, b->bonds, maps));
+    CHK(kplan_upload(b, b->et, maps)); CHK(kplan_upload(b, b->bonds, maps)); CHK(kplan_upload(b, b->spc, maps));
     CHK(kplan_upload(b, b->angles, maps)); CHK(kplan_upload(b, b->tors, maps));
     CHK(kplan_upload(b, b->springs, maps)); CHK(kplan_upload(b, b->comb, maps));
     CHK(kplan_upload(b, b->red, maps));
@@ -1530,6 +1545,7 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
         mark(4); mark(5); mark(6); mark(7);
     }
     if (group != MDX_FORCES_SLOW) {
+        launch(b, b->spc, 256, 0, k_strbnd);
         launch(b, b->bonds, 256, 0, k_bonds);
         launch(b, b->angles, 256, 0, k_angles);
         launch(b, b->tors, 256, 0, k_tors);
@@ -2171,7 +2187,7 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     drop_graphs(b);
     kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->com); kplan_free(b->pairs); kplan_free(b->special);
     kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
-    kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->angles); kplan_free(b->tors);
+    kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->spc); kplan_free(b->angles); kplan_free(b->tors);
     kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);
     kplan_free(b->md_kd); kplan_free(b->md_sh); kplan_free(b->md_ke); kplan_free(b->md_th);
     kplan_free(b->md_gc); kplan_free(b->md_tr);

@@ -294,6 +294,16 @@ struct BondArgs {
     ScatterArgs s;
 };
 
+// Flexible SPC water (STRBND, forces.f:530-800): one molecule per record,
+// atoms O, H1, H2 and 12 parameters (R_OH1, R_OH2, R_HH, A, B, C, rho1,
+// rho2, E, G, IPOT, LMOVE).
+struct SpcArgs {
+    int n;
+    const int *atom;
+    const double *par;
+    ScatterArgs s;
+};
+
 struct AngleArgs {
     int n;
     const int *ai, *aj, *ak, *apar;

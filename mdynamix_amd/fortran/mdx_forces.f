@@ -47,6 +47,8 @@
         integer(c_int32_t) :: ntorsparams
         type(c_ptr) :: tp_kind, tp_mult, tp_k, tp_delta
         type(c_ptr) :: tp_k1, tp_k2, tp_k3, tp_k4, tp_k5
+        integer(c_int32_t) :: nspc
+        type(c_ptr) :: spc_atom, spc_par
       end type
       type, bind(C) :: mdx_params_t
         real(c_double) :: box(3), rcut, rshort, skin, alpha, fexp
@@ -135,13 +137,14 @@
       integer(c_int32_t), allocatable, target, save :: amtyp(:),
      +  amol(:), aljt(:), tee(:), tmov(:), tpair(:), exs(:), exa(:),
      +  bi(:), bj(:), bp(:), bmo(:), ai(:), aj(:), ak(:), ap(:),
-     +  ti(:), tj(:), tk(:), tl(:), tp(:), tkind(:), tmult(:)
+     +  ti(:), tj(:), tk(:), tl(:), tp(:), tkind(:), tmult(:),
+     +  sat(:)
       integer(c_int8_t), allocatable, target, save :: ex14(:)
       real(c_double), allocatable, target, save :: aq(:), aq0(:),
      +  am(:), c14e(:), c14l(:), ta6(:), tb12(:), ta60(:), tb120(:),
      +  ta614(:), tb1214(:), br0(:), bk(:), bd(:), brho(:), ath(:),
      +  akk(:), tk0(:), tdel(:), tk1(:), tk2(:), tk3(:), tk4(:),
-     +  tk5(:)
+     +  tk5(:), spar(:)
       end module mdxc
 *
 *  Report an engine error and stop the run through FINAL.
@@ -273,6 +276,43 @@
         tk4(K)=FT4(K)
         tk5(K)=FT5(K)
       end do
+*  fSPC water (IPOT 1 or 2): STRBND records, forces.f:555-605
+      NSPCM=0
+      do ITYP=1,NTYPES
+        if(IPOT(ITYP).eq.1.or.IPOT(ITYP).eq.2)NSPCM=NSPCM+NSPEC(ITYP)
+      end do
+      allocate(sat(max(3*NSPCM,1)),spar(max(12*NSPCM,1)))
+      K=0
+      do ITYP=1,NTYPES
+        if(IPOT(ITYP).eq.1.or.IPOT(ITYP).eq.2)then
+          I1=IADB(ITYP)
+          do M=1,NSPEC(ITYP)
+            K=K+1
+            IML=ISADDR(ITYP)+(M-1)*NSITS(ITYP)
+            sat(3*K-2)=IML
+            sat(3*K-1)=IML+1
+            sat(3*K)=IML+2
+            spar(12*K-11)=RB(I1)
+            spar(12*K-10)=RB(I1+1)
+            spar(12*K-9)=RB(I1+2)
+            if(IPOT(ITYP).eq.2)then
+              spar(12*K-8)=DB(I1)
+              spar(12*K-7)=DB(I1+1)
+            else
+              spar(12*K-8)=FB(I1)
+              spar(12*K-7)=FB(I1+1)
+            end if
+            spar(12*K-6)=FB(I1+2)
+            spar(12*K-5)=RO(I1)
+            spar(12*K-4)=RO(I1+1)
+            spar(12*K-3)=-884.63D0*1.D3/AVSNO/UNITE
+            spar(12*K-2)=467.31D0*1.D3/AVSNO/UNITE
+            spar(12*K-1)=dble(IPOT(ITYP))
+            spar(12*K)=0.d0
+            if(LMOVE(ITYP))spar(12*K)=1.d0
+          end do
+        end if
+      end do
       topo%natoms=N
       topo%nmols=NOP
       topo%nmoltypes=NTYPES
@@ -332,6 +372,9 @@
       topo%tp_k3=c_loc(tk3)
       topo%tp_k4=c_loc(tk4)
       topo%tp_k5=c_loc(tk5)
+      topo%nspc=NSPCM
+      topo%spc_atom=c_loc(sat)
+      topo%spc_par=c_loc(spar)
       prm%box(1)=BOXL
       prm%box(2)=BOYL
       prm%box(3)=BOZL

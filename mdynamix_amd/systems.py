@@ -66,6 +66,9 @@ class MolType:
     tors1: List[Tuple[int, int, int, int, float, float, float]] = dataclasses.field(default_factory=list)
     c14el: Optional[float] = None
     c14lj: Optional[float] = None
+    # "fSPC" keyword (input.f:1304-1313): flexible SPC water, whose
+    # intramolecular terms STRBND computes from the first three bonds
+    fspc: bool = False
 
     @property
     def nsites(self) -> int:
@@ -105,6 +108,8 @@ class MolType:
             out.append("sel14 %.8f" % self.c14el)
         if self.c14lj is not None:
             out.append("slj14 %.8f" % self.c14lj)
+        if self.fspc:
+            out.append("fSPC")
         out.append("End")
         return "\n".join(out) + "\n"
 
@@ -223,6 +228,22 @@ def spce_water() -> MolType:
     bonds = [(0, 1, 2, 1.0, 2811.0, 0.0, 0.0), (0, 1, 3, 1.0, 2811.0, 0.0, 0.0),
              (0, 2, 3, hh, 687.0, 0.0, 0.0)]
     return MolType("SPCE", sites, bonds)
+
+
+def fspc_water() -> MolType:
+    """Flexible SPC water of the "fSPC" kind (STRBND, forces.f:530-800):
+    the molecule must be named H2O*; Morse O-H bonds (depth D, range rho)
+    and a harmonic H-H term from the first three bonds, plus STRBND's fixed
+    cross terms.  Parameter values in the spirit of Toukan and Rahman
+    (kJ/mol, A)."""
+    g = _water_geometry(1.0, 109.47)
+    sites = [Site("OW", g[0], 15.9994, -0.82, 3.165492, 0.6502),
+             Site("HW1", g[1], 1.008, 0.41, 0.0, 0.0),
+             Site("HW2", g[2], 1.008, 0.41, 0.0, 0.0)]
+    rhh = float(np.linalg.norm(np.asarray(g[1]) - np.asarray(g[2])))
+    bonds = [(1, 1, 2, 1.0, 2000.0, 432.58, 2.566), (1, 1, 3, 1.0, 2000.0, 432.58, 2.566),
+             (0, 2, 3, rhh, 228.3, 0.0, 0.0)]
+    return MolType("H2O", sites, bonds, [], fspc=True)
 
 
 def spcfw_water() -> MolType:

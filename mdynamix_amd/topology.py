@@ -22,6 +22,7 @@ from .abi import TopologyArrays, make_params, MDX_LIST_REFERENCE, MDX_LIST_VERLE
 from .systems import COULF_KJ, System
 
 TODGR = 180.0 / 3.1415926536          # prcm.h:31
+FSPC_EEP, FSPC_GGP = -884.63, 467.31  # STRBND cross constants, kJ/mol/A^2 (forces.f:537)
 
 
 def _lb(s1, s2, e1, e2) -> Tuple[float, float]:
@@ -170,6 +171,7 @@ def from_system(system: System, lam: float | None = None) -> Tuple[TopologyArray
     ai, aj, ak, apar, ath, akk = [], [], [], [], [], []
     ti, tj, tk, tl, tpar = [], [], [], [], []
     tkind, tmult, tkk, tdel, tk1, tk2, tk3, tk4, tk5 = [], [], [], [], [], [], [], [], []
+    spc_at, spc_pr = [], []
     base = 0
     for t, c in zip(types, counts):
         b0, a0, t0 = len(br0), len(ath), len(tkind)
@@ -187,6 +189,14 @@ def from_system(system: System, lam: float | None = None) -> Tuple[TopologyArray
         ns = t.nsites
         for m in range(c):
             off = base + m * ns
+            if t.fspc:
+                # STRBND replaces the type's listed bonds, angles and
+                # torsions (setup.f:297): one record per molecule
+                b1, b2, b3 = t.bonds[0], t.bonds[1], t.bonds[2]
+                spc_at.append((off, off + 1, off + 2))
+                spc_pr.append((b1[3], b2[3], b3[3], b1[5], b2[5], b3[4], b1[6], b2[6],
+                               FSPC_EEP, FSPC_GGP, 2.0, 1.0))
+                continue
             for kb, b in enumerate(t.bonds):
                 bi.append(off + b[1] - 1); bj.append(off + b[2] - 1); bpar.append(b0 + kb)
             for ka, a in enumerate(t.angles):
@@ -214,7 +224,8 @@ def from_system(system: System, lam: float | None = None) -> Tuple[TopologyArray
         tors_l=np.array(tl, np.int32), tors_param=np.array(tpar, np.int32),
         tp_kind=np.array(tkind, np.int32), tp_mult=np.array(tmult, np.int32), tp_k=np.array(tkk),
         tp_delta=np.array(tdel), tp_k1=np.array(tk1), tp_k2=np.array(tk2), tp_k3=np.array(tk3),
-        tp_k4=np.array(tk4), tp_k5=np.array(tk5))
+        tp_k4=np.array(tk4), tp_k5=np.array(tk5),
+        spc_atom=np.array(spc_at, np.int32).reshape(-1, 3), spc_par=np.array(spc_pr).reshape(-1, 12))
     raex = system.ee_bee * (1.0 - lam ** system.ee_pw2) if system.ee_type is not None else 0.0
     params = dict(box=system.box, rcut=system.rcut, rshort=system.rshort, alpha=system.alpha,
                   fexp=system.ewald_b, coulomb_factor=COULF_KJ,
@@ -222,6 +233,31 @@ def from_system(system: System, lam: float | None = None) -> Tuple[TopologyArray
                   skin=system.skin, ee_enabled=system.ee_type is not None, ee_raex=raex,
                   ee_self_scale=lam)
     return topo, params
+
+
+def _spc_records(d: Dict[str, List[np.ndarray]]) -> Tuple[np.ndarray, np.ndarray]:
+    """STRBND records (forces.f:555-590) of every type with IPOT 1 or 2:
+    atoms ISADDR+3(n-1)+1..3 of molecule n, parameters from the type's
+    first three bond records (IADB), cross constants EEP, GGP times
+    FACTOR = 1e3/AVSNO/UNITE."""
+    g = lambda k: d[k][0]
+    ipot, iadb, nspec, nsits = g("IPOT"), g("IADB"), g("NSPEC"), g("NSITS")
+    rb, fb, db, ro, lmove = g("RB"), g("FB"), g("DB"), g("RO"), g("LMOVE")
+    factor = 1.0e3 / 6.02252e23 / g("SCALARS")[11]
+    at, pr = [], []
+    base = 0
+    for it in range(len(ipot)):
+        if ipot[it] in (1, 2):
+            i1 = iadb[it] - 1
+            i2, i3 = i1 + 1, i1 + 2
+            a_, b_ = (db[i1], db[i2]) if ipot[it] == 2 else (fb[i1], fb[i2])
+            for m in range(nspec[it]):
+                off = base + m * nsits[it]
+                at.append((off, off + 1, off + 2))
+                pr.append((rb[i1], rb[i2], rb[i3], a_, b_, fb[i3], ro[i1], ro[i2], FSPC_EEP * factor,
+                           FSPC_GGP * factor, float(ipot[it]), float(lmove[it] != 0)))
+        base += nspec[it] * nsits[it]
+    return np.array(at, np.int32).reshape(-1, 3), np.array(pr, float).reshape(-1, 12)
 
 
 def from_dump(d: Dict[str, List[np.ndarray]], lam: float = 1.0) -> Tuple[TopologyArrays, Dict]:
@@ -260,6 +296,8 @@ def from_dump(d: Dict[str, List[np.ndarray]], lam: float = 1.0) -> Tuple[Topolog
         tp_delta=g("RT"), tp_k1=g("FT1"), tp_k2=g("FT2"), tp_k3=g("FT3"), tp_k4=g("FT4"),
         tp_k5=g("FT5"))
     assert nbond == len(topo.bond_i)
+    if "IPOT" in d:
+        topo.spc_atom, topo.spc_par = _spc_records(d)
     ee_enabled = bool(sizes[11])
     params = dict(box=(sc[0], sc[1], sc[2]), rcut=sc[3], rshort=sc[5], alpha=sc[7], fexp=sc[8],
                   coulomb_factor=sc[9], list_mode=MDX_LIST_REFERENCE, skin=0.0,

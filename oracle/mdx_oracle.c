@@ -17,6 +17,7 @@
  *   orc_inifur    INIFUR   forces.f:1356-1449
  *   orc_furir     FURIR    forces.f:1217-1339 (orc_furir_task: one rank's k stripe)
  *   orc_eterms    ETERMS   forces.f:1482-1676
+ *   orc_strbnd    STRBND   forces.f:530-800 (fSPC water)
  *   orc_getbnd    GETBND   forces.f:113-204
  *   orc_getang    GETANG   forces.f:216-309
  *   orc_gettrs    GETTRS   forces.f:323-513
@@ -508,6 +509,70 @@ void orc_getbnd(const mdx_topology *t, const mdx_params *p, const double *x, con
     }
 }
 
+/* STRBND (forces.f:530-800): flexible SPC water, Morse (IPOT 2) or
+ * harmonic (IPOT 1) O-H terms, harmonic H-H and fixed cross terms, per
+ * molecule.  The Morse energy is kept beyond R_OH > BBMA (a REAL*4 1.3)
+ * while the force switches to the harmonic form -2 s D rho1^2 -- for
+ * both bonds with rho1, as the reference writes it; the harmonic variant
+ * takes no periodic image.  Pinned by: the reference's HX/PINT/VIRB of an
+ * fSPC box (golden dump fspc_tiny, IPOT 2) and finite differences of the
+ * energy away from the switch (tests); IPOT 1 cannot be set by an input
+ * (input.f:1304-1306) and is parity unpinned.                           */
+void orc_strbnd(const mdx_topology *t, const mdx_params *p, const double *x, const double *y,
+                const double *z, double *hx, double *hy, double *hz, mdx_energies *e) {
+    const double bbm = (double)1.3f;
+    for (int n = 0; n < t->nspc; ++n) {
+        const int i = t->spc_atom[3 * n], j = t->spc_atom[3 * n + 1], k = t->spc_atom[3 * n + 2];
+        const double *q = t->spc_par + 12 * (size_t)n;
+        const double roh1 = q[0], roh2 = q[1], rhh = q[2], aaa = q[3], bbb = q[4], ccc = q[5];
+        const double rho1 = q[6], rho2 = q[7], eee = q[8], ggg = q[9];
+        const int ipt = (int)q[10], lmove = q[11] != 0.0;
+        const double fff = eee;
+        double ax = x[j] - x[i], ay = y[j] - y[i], az = z[j] - z[i];
+        double bx = x[k] - x[i], by = y[k] - y[i], bz = z[k] - z[i];
+        double cx = x[k] - x[j], cy = y[k] - y[j], cz = z[k] - z[j];
+        if (ipt == 2) { pbc(&ax, &ay, &az, p); pbc(&bx, &by, &bz, p); pbc(&cx, &cy, &cz, p); }
+        const double rra = ax * ax + ay * ay + az * az;
+        const double rrb = bx * bx + by * by + bz * bz;
+        const double rrc = cx * cx + cy * cy + cz * cz;
+        const double ass = sqrt(rra), bss = sqrt(rrb), css = sqrt(rrc);
+        const double sa = ass - roh1, sb = bss - roh2, sh = css - rhh;
+        double wb1, wb2, fa, fb;
+        if (ipt == 2) {
+            const double expa = exp(-rho1 * sa), expam = 1.0 - expa;
+            const double expb = exp(-rho2 * sb), expbm = 1.0 - expb;
+            wb1 = aaa * (expam * expam) + eee * sa * sh;
+            wb2 = bbb * (expbm * expbm) + fff * sb * sh;
+            const double aadd = aaa * rho1, bbdd = bbb * rho2;
+            fa = ass > bbm ? -2. * sa * aadd * rho1 : -2. * aadd * expa * expam;
+            fa = fa - eee * sh - ggg * sb;
+            fb = bss > bbm ? -2. * sb * bbdd * rho1 : -2. * bbdd * expb * expbm;
+            fb = fb - fff * sh - ggg * sa;
+        } else {
+            wb1 = aaa * sa * sa + eee * sa * sh;
+            wb2 = bbb * sb * sb + fff * sb * sh;
+            fa = -2.0 * aaa * sa - eee * sh - ggg * sb;
+            fb = -2.0 * bbb * sb - fff * sh - ggg * sa;
+        }
+        const double wb3 = ccc * sh * sh + ggg * sa * sb;
+        e->pint = e->pint + wb1 + wb2 + wb3;
+        if (!lmove) continue;
+        double fc = -2.0 * ccc * sh - eee * sa - fff * sb;
+        fa = fa / ass; fb = fb / bss; fc = fc / css;
+        const double axx = fa * ax, ayy = fa * ay, azz = fa * az;
+        const double bxx = fb * bx, byy = fb * by, bzz = fb * bz;
+        const double cxx = fc * cx, cyy = fc * cy, czz = fc * cz;
+        hx[i] = hx[i] - axx - bxx; hy[i] = hy[i] - ayy - byy; hz[i] = hz[i] - azz - bzz;
+        hx[j] = hx[j] + axx - cxx; hy[j] = hy[j] + ayy - cyy; hz[j] = hz[j] + azz - czz;
+        hx[k] = hx[k] + bxx + cxx; hy[k] = hy[k] + byy + cyy; hz[k] = hz[k] + bzz + czz;
+        const double vix = ax * axx + bx * bxx + cx * cxx;
+        const double viy = ay * ayy + by * byy + cy * cyy;
+        const double viz = az * azz + bz * bzz + cz * czz;
+        e->virb = e->virb + vix + viy + viz;
+        e->virfx += vix; e->virfy += viy; e->virfz += viz;
+    }
+}
+
 /* GETANG (forces.f:216-309).  Pinned as GETBND.                         */
 void orc_getang(const mdx_topology *t, const mdx_params *p, const double *x, const double *y,
                 const double *z, double *hx, double *hy, double *hz, mdx_energies *e) {
@@ -703,6 +768,7 @@ int orc_compute(const mdx_topology *t, const mdx_params *p, const double *x, con
         memset(hx, 0, sizeof(double) * n); memset(hy, 0, sizeof(double) * n); memset(hz, 0, sizeof(double) * n);
         orc_localf(t, p, x, y, z, nbs1, nbs2, mbsh, hx, hy, hz, e);
         e->wirss = mol_virial(t, x, y, z, hx, hy, hz, cx, cy, cz);
+        orc_strbnd(t, p, x, y, z, hx, hy, hz, e);   /* GETBND calls STRBND first */
         orc_getbnd(t, p, x, y, z, hx, hy, hz, e);
         orc_getang(t, p, x, y, z, hx, hy, hz, e);
         orc_gettrs(t, p, x, y, z, hx, hy, hz, e);

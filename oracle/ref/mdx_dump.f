@@ -186,6 +186,8 @@
       call MDXWI('LMOVE   ',NTYPES,IW)
       call MDXWD('C14EL   ',NTYPES,C14EL)
       call MDXWD('C14LJ   ',NTYPES,C14LJ)
+      call MDXWI('IPOT    ',NTYPES,IPOT)
+      call MDXWI('IADB    ',NTYPES,IADB)
       call MDXWI('NSPEC   ',NTYPES,NSPEC)
       call MDXWI('NSITS   ',NTYPES,NSITS)
       call MDXWD('SPEEI   ',NTYPES,SPEEI)

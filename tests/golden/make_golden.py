@@ -10,8 +10,11 @@ virial scalar, per-type breakdowns).  Fixtures are data only.
     python tests/golden/make_golden.py        # rewrites tests/golden/*.npz
     python tests/golden/make_golden.py traj   # trajectory fixtures only
     python tests/golden/make_golden.py traj traj_spcfw_respa   # one of them
+    python tests/golden/make_golden.py force fspc_tiny         # one force fixture
 """
 from __future__ import annotations
+
+import dataclasses
 
 import hashlib
 import os
@@ -35,10 +38,20 @@ CASES = {
                                                   rcut=8.0, rshort=4.0, ewald_a=2.8, ewald_b=9.0,
                                                   shake=False, nfreq=10), 1.0, True),
     "ee_tiny": (lambda: systems.ee_job(400, 10, 0.35, 9), 0.35, True),
+    "fspc_tiny": (lambda: _jitter(systems.water_nacl(60, 1, 13.0, 17, name="fspc", water=systems.fspc_water(),
+                                                     rcut=6.0, rshort=3.5, ewald_a=2.8, ewald_b=9.0,
+                                                     shake=False, nfreq=4), 0.04, 23), 1.0, True),
 }
 
+def _jitter(s, sigma, seed):
+    """Atoms displaced by N(0, sigma) A: bonds away from equilibrium, so
+    the intramolecular terms are not zero."""
+    rng = np.random.default_rng(seed)
+    return dataclasses.replace(s, pos=s.pos + rng.normal(0.0, sigma, s.pos.shape))
+
+
 STATIC = ["SIZES", "SCALARS", "ITYPE", "NNUM", "NSITE", "INBTA", "CHARGEA", "Q", "MASSDA", "MASSA",
-          "IEE", "LMOVE", "C14EL", "C14LJ", "NSPEC", "NSITS", "SPEEI", "MDX", "A6LJ", "B12LJ",
+          "IEE", "LMOVE", "C14EL", "C14LJ", "IPOT", "IADB", "NSPEC", "NSITS", "SPEEI", "MDX", "A6LJ", "B12LJ",
           "A6LJ0", "B12LJ0", "A6_14", "B12_14", "NNBB", "INBB", "RKX", "RKY", "RKZ", "IBI", "JBJ",
           "IBUK", "ID", "RB", "FB", "DB", "RO", "IAI", "JAJ", "KAK", "IAUK", "RA", "FA", "ITI", "JTJ",
           "KTK", "LTL", "ITUK", "ITORS", "NMUL", "FT", "RT", "FT1", "FT2", "FT3", "FT4", "FT5",
@@ -107,13 +120,20 @@ def main() -> None:
     if sys.argv[1:2] == ["traj"]:
         main_traj(sys.argv[2:])
         return
-    main_traj()
+    only = None
+    if sys.argv[1:2] == ["force"]:
+        only = set(sys.argv[2:])
+    else:
+        main_traj()
     for name, (builder, lam, full_lists) in CASES.items():
+        if only and name not in only:
+            continue
         s = builder()
         d = refdump.run_reference(s)
         out = {"lam": np.array(lam)}
         for k in STATIC:
-            out[k] = d[k][0]
+            if k in d:
+                out[k] = d[k][0]
         ncall = len(d["SX"])
         out["NCALL"] = np.array(ncall)
         for c in range(ncall):

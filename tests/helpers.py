@@ -9,7 +9,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CASES = ("water_tiny", "water_508", "chains_tiny", "ee_tiny")
+CASES = ("water_tiny", "water_508", "chains_tiny", "ee_tiny", "fspc_tiny")
 
 
 def load_golden(name):

@@ -149,3 +149,33 @@ def test_dense_droplet_in_large_box_regrows_lists():
     s = systems.water_nacl(650, 1, 27.0, 73, rcut=8.0, rshort=4.0, ewald_a=2.8, ewald_b=6.0, shake=False)
     s = dataclasses.replace(s, box=(100.0, 100.0, 100.0))
     _run_vs_oracle(s, "droplet", expect_growth=True, groups=(abi.MDX_FORCES_ALL,), skip=("wirs", "wirss"))
+
+
+@pytest.mark.parametrize("ipot", [2, 1])
+def test_fspc_water_stretched_bond(ipot):
+    """fSPC water (STRBND, forces.f:530-800) with one O-H bond pulled to
+    1.45 A, beyond the 1.3 A switch of the Morse force (IPOT 2), and the
+    harmonic variant (IPOT 1, records edited: no input sets it)."""
+    s = systems.water_nacl(60, 1, 13.0, 17, name="fspc", water=systems.fspc_water(), rcut=6.0, rshort=3.5,
+                           ewald_a=2.8, ewald_b=9.0, shake=False)
+    pos = s.pos.copy()
+    d = pos[1] - pos[0]
+    pos[1] = pos[0] + d * (1.45 / np.linalg.norm(d))
+    s = dataclasses.replace(s, pos=pos)
+    topo, prm = topology.from_system(s)
+    assert len(topo.spc_atom) == 60
+    topo.spc_par[:, 10] = ipot
+    prm["list_mode"] = abi.MDX_LIST_REFERENCE
+    eng = ForceEngine(topo, prm)
+    try:
+        eng.set_positions(s.pos)
+        eng.chcnb()
+        ts, ps = topo.struct(), topology.params_struct(dict(prm, list_mode=0, skin=0.0))
+        lists = orc.chcnb(ts, ps, s.pos)
+        print("case fspc ipot", ipot, flush=True)
+        f, g, h, e = eng.compute(abi.MDX_FORCES_ALL)
+        fo, go, ho, eo = orc.compute(topo, ts, ps, s.pos, lists, abi.MDX_FORCES_ALL)
+        assert _rms_rel(f, fo) <= FORCE_RMS_TOL and _rms_rel(h, ho) <= FORCE_RMS_TOL
+        _check(e, eo)
+    finally:
+        eng.close()

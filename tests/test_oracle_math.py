@@ -156,3 +156,36 @@ def test_harmonic_angle(theta):
     s, topo, prm = _molecule(coords, angles=[(1, 2, 3, 112.0, 50.0)])
     e = _fd_check(topo, prm, s.pos)
     assert e == pytest.approx(topo.ap_k[0] * (t - math.radians(112.0)) ** 2, rel=1e-9, abs=1e-12)
+
+
+@pytest.mark.parametrize("ipot", [2, 1])
+def test_strbnd_forces_are_energy_gradients(ipot):
+    """STRBND (forces.f:530-800) alone (charges and LJ off): forces equal
+    minus the central difference of PINT for both variants, away from the
+    1.3 A switch of the Morse force."""
+    import dataclasses
+    from mdynamix_amd import systems, topology
+    s = systems.water_nacl(8, 0, 12.0, 5, name="fspc", water=systems.fspc_water(), rcut=5.0, rshort=3.0,
+                           ewald_a=2.8, ewald_b=4.0, shake=False)
+    rng = np.random.default_rng(3)
+    pos = s.pos + rng.normal(0.0, 0.05, s.pos.shape)
+    topo, prm = topology.from_system(dataclasses.replace(s, pos=pos))
+    topo.spc_par[:, 10] = ipot
+    topo.atom_charge[:] = 0.0
+    topo.atom_charge0[:] = 0.0
+    topo.a6[:] = 0.0
+    topo.b12[:] = 0.0
+    ts, ps = topo.struct(), topology.params_struct(dict(prm, list_mode=0, skin=0.0))
+    lists = orc.chcnb(ts, ps, pos)
+    _, _, h, e = orc.compute(topo, ts, ps, pos, lists, abi.MDX_FORCES_FAST)
+    assert e["pint"] != 0.0
+    step = 1e-6
+    for atom in (0, 1, 2, 7):
+        for c in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[atom, c] += step
+            pm[atom, c] -= step
+            ep = orc.compute(topo, ts, ps, pp, lists, abi.MDX_FORCES_FAST)[3]["pint"]
+            em = orc.compute(topo, ts, ps, pm, lists, abi.MDX_FORCES_FAST)[3]["pint"]
+            fd = -(ep - em) / (2 * step)
+            assert abs(h[atom, c] - fd) <= 1e-6 * max(1.0, abs(fd)), (atom, c, h[atom, c], fd)
