@@ -515,9 +515,11 @@ void orc_getbnd(const mdx_topology *t, const mdx_params *p, const double *x, con
  * while the force switches to the harmonic form -2 s D rho1^2 -- for
  * both bonds with rho1, as the reference writes it; the harmonic variant
  * takes no periodic image.  Pinned by: the reference's HX/PINT/VIRB of an
- * fSPC box (golden dump fspc_tiny, IPOT 2) and finite differences of the
- * energy away from the switch (tests); IPOT 1 cannot be set by an input
- * (input.f:1304-1306) and is parity unpinned.                           */
+ * fSPC box (golden dump fspc_tiny, IPOT 2, bonds off equilibrium) and, for
+ * both variants, forces equal to minus the central difference of PINT
+ * away from the switch (tests/test_oracle_math.py).  IPOT 1 cannot be set
+ * by an input (input.f:1304-1306): no reference output exists for it, so
+ * the gradient check is its only pin.                                   */
 void orc_strbnd(const mdx_topology *t, const mdx_params *p, const double *x, const double *y,
                 const double *z, double *hx, double *hy, double *hz, mdx_energies *e) {
     const double bbm = (double)1.3f;
