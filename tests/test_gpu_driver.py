@@ -51,3 +51,21 @@ def test_reference_driver_on_gpu_engine_matches_cpu_reference():
     for k in range(3):
         for col in (1, 3, 4, 5):
             assert abs(gpu[k, col] - ref[k, col]) <= 2e-3 + 2e-5 * abs(ref[k, col]), (k, col, gpu[k], ref[k])
+
+
+@pytest.mark.skipif(not (os.path.exists(MD) and os.path.exists(DRV)),
+                    reason="reference binaries not built (make -C oracle/ref all drv needs /root/reference)")
+def test_reference_driver_on_gpu_engine_fspc_water():
+    """Flexible fSPC water ("fSPC" keyword, IPOT 2): the shim hands the
+    engine STRBND records built from IPOT/IADB (forces.f:555-605) and
+    lists none of the type's bonds (setup.f:297); 182 atoms, no SHAKE."""
+    s = systems.water_nacl(60, 1, 13.0, 17, name="fspc", water=systems.fspc_water(), rcut=6.0, rshort=3.5,
+                           ewald_a=2.8, ewald_b=9.0, shake=False, nfreq=4)
+    print("driver: 182 atoms fSPC water, 5 steps", flush=True)
+    ref = run_md(MD, s, 5)
+    gpu = run_md(DRV, s, 5)
+    assert ref.shape == gpu.shape == (5, 8)
+    assert abs(ref[0, 2]) > 1e-3, ref[0]      # intramolecular energy is not zero
+    for k in range(3):
+        for col in (1, 2, 3, 4, 5):
+            assert abs(gpu[k, col] - ref[k, col]) <= 2e-3 + 2e-5 * abs(ref[k, col]), (k, col, gpu[k], ref[k])
