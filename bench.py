@@ -414,16 +414,16 @@ def main() -> int:
     dms, dflop = cand[dom]
     achieved = dflop / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
     traffic = measured_traffic(dom)
-    # every candidate is compute-bound (a few hundred FLOP per HBM byte), so
-    # the roof is the FP32 compute peak ("mfma" = the compute roof)
-    roof = {"kernel": dom, "bound": "mfma", "compute_unit": "valu" if dom == "k_pairs" else "mfma",
+    # every candidate is compute-bound (a few hundred FLOP per HBM byte):
+    # k_pairs on the vector ALU, the k-space passes on the matrix cores
+    roof = {"kernel": dom, "bound": "valu" if dom == "k_pairs" else "mfma",
             "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "avg_ms": round(dms, 4), "algorithmic_flop": dflop,
             "note": ("k_pairs: 74 FLOP per pair within R_cut on the vector ALU (FP32, unpacked: its own "
                      "issue peak is 78.6 TFLOP/s); k_sk / k_recip_force: half of the 21 FLOP per (k, atom) "
                      "term of FURIR each, computed on the matrix cores as FP32 products split into f16 "
-                     "hi/lo pairs (v_mfma_f32_16x16x32_f16, three products, FP32 accumulation); peak: "
+                     "hi/lo pairs (v_mfma_f32_16x16x32_f16, three products), FP64 sums; peak: "
                      "MI355X FP32 157.3 TFLOP/s (dense matrix or packed vector); traffic: HBM bytes per "
                      "launch from profiles/traffic_spce_nacl_32k.md (FETCH_SIZE/WRITE_SIZE, "
                      "copy-calibrated)")}
@@ -441,7 +441,10 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair/erfc arithmetic, fp64 accumulation and k-space sums",
+        "dtype": ("fp32 pair and erfc arithmetic (FP32 sums over <= 16 list entries, then fp64 / 64-bit "
+                  "fixed point); k space: fp32 phase products as f16 hi/lo pairs (22-bit) on the matrix "
+                  "cores, each 32-atom stage's fp32 result summed in fp64 (structure factors), per-k-row "
+                  "fp32 terms summed in fp64 (forces); fp64 tables, energies, virials"),
         "data": "synthetic (seeded jittered lattice, random orientations)",
         "config": {"workload": "spce_nacl_32k", "atoms": topo.natoms, "box_A": s.box[0], "rcut_A": s.rcut,
                    "skin_A": s.skin, "rebuild_every": rebuild, "kvectors": kv_count,

@@ -251,6 +251,17 @@ mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
  * sums are split into partial sums (results agree to rounding; each shape
  * is deterministic).  Needs a built list. */
 mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk, int32_t *nslice);
+/* The k vectors of the reciprocal sum in INIFUR order (forces.f:1375-1427,
+ * rectangular cell): integer triples (n_x, n_y, n_z) with k = 2 pi n / L,
+ * i.e. RKX = 2 pi n_x as INIFUR stores it.  *nkv receives the count; the
+ * arrays (cap entries, may be NULL when cap is 0) the vectors.  A single
+ * molecule has none (FURIR returns at once, forces.f:1266). */
+mdx_status mdx_get_kvectors(const mdx_engine *h, int32_t *nx, int32_t *ny, int32_t *nz, int64_t cap,
+                            int64_t *nkv);
+/* The same enumeration from parameters alone (box, alpha, fexp), on the
+ * host, without a device or a handle. */
+mdx_status mdx_enumerate_kvectors(const mdx_params *p, int32_t *nx, int32_t *ny, int32_t *nz, int64_t cap,
+                                  int64_t *nkv);
 /* Set that shape directly (reproducible runs, profiling). */
 mdx_status mdx_set_kspace_shape(mdx_engine *h, int32_t chunk, int32_t nslice);
 /* Device memory held by a handle, in bytes (admission by HBM budget). */
@@ -338,6 +349,12 @@ mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, 
  * interactions of the step's force call. */
 mdx_status mdx_md_run(mdx_engine *h, int32_t nsteps, double *trace);
 mdx_status mdx_md_get_state(mdx_engine *h, double *x, double *y, double *z, double *vx, double *vy, double *vz);
+/* Constraint failures since mdx_md_init: molecules whose SHEJK sweeps hit
+ * MAXIT without convergence, plus bond updates whose projection on the
+ * old bond fell below BDTOL (the reference prints "NO CONVERGENCE" and
+ * continues, mdstep.f:1264-1268; the device run continues too and counts
+ * them here).  0 in a healthy run. */
+mdx_status mdx_md_shake_failures(mdx_engine *h, int64_t *count);
 /* nsteps steps of every system of a batch (each initialised by
  * mdx_md_init), one fused launch per kernel and step; trace: K x nsteps x 5
  * or NULL. */

@@ -130,8 +130,8 @@ struct mdx_engine {
     FRow *d_srows = nullptr;
     float2 *d_tx = nullptr, *d_ty = nullptr, *d_tz = nullptr;
     int kx = 0, ky = 0, tstride = 64;
-    float4 *d_ppart = nullptr;
-    float2 *d_s0part = nullptr;
+    double4 *d_ppart = nullptr;
+    double2 *d_s0part = nullptr;
     int mzp = 8, nsgroups = 1;
     double4 *d_fq = nullptr;
     double4 *d_stepz = nullptr;
@@ -679,15 +679,19 @@ void fill_args(mdx_engine *h) {
     pa.clu_atom = h->d_clu_atom;
     pa.alpha = (float)h->p.alpha;
     pa.rc2 = verlet ? (float)rc2 : INFINITY;
+    {
+        // FP32 r^2 from anchor-relative coordinates and an FP32 image shift
+        // (|shift| <= L/2): |error| <= about 2 r_c L 2^-24 + 3 r_c^2 2^-24;
+        // the band is eight times that
+        const double lmax = std::max(h->p.box[0], std::max(h->p.box[1], h->p.box[2]));
+        pa.band = verlet ? (float)(1e-6 * h->p.rcut * (lmax + 3.0 * h->p.rcut)) : 0.f;
+        pa.rc2d = rc2;
+        pa.x = h->d_x; pa.y = h->d_y; pa.z = h->d_z;
+        pa.box = h->box;
+    }
     pa.raex = (float)h->p.ee_raex;
     pa.soft = h->p.ee_enabled ? 1 : 0;
     {
-        // Cluster image shifts are minimum images of the anchors; with
-        // clusters up to ~5 A across, a pair can need another image only
-        // when half the box is within ~10 A of the list radius.
-        const double rl = std::sqrt(list_radius2(h));
-        const double hmin = 0.5 * std::min(h->p.box[0], std::min(h->p.box[1], h->p.box[2]));
-        pa.wrap = hmin < rl + 10.0;
         pa.lx = (float)h->p.box[0]; pa.ly = (float)h->p.box[1]; pa.lz = (float)h->p.box[2];
         pa.ilx = (float)(1.0 / h->p.box[0]); pa.ily = (float)(1.0 / h->p.box[1]); pa.ilz = (float)(1.0 / h->p.box[2]);
     }
@@ -1197,7 +1201,7 @@ mdx_status mdx_build_list(mdx_engine *h) {
                                                           h->d_colclu_n, h->ncx, h->ncy, rx, ry, h->box, rl,
                                                           h->cand_cap, h->d_cand, h->d_ncand, h->d_overflow);
         k_masks<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_cand, h->d_ncand, h->cand_cap, h->d_clu_atom, h->d_spos,
-                                        h->d_anchor, h->d_moltype, h->d_type_ee, h->d_excl_start,
+                                        h->d_anchor, h->d_ext, rl, h->d_moltype, h->d_type_ee, h->d_excl_start,
                                         h->d_excl_atom, h->d_excl_is14, h->box, rl2,
                                         h->p.rshort * h->p.rshort, h->d_list, h->d_nlist, h->d_nshort,
                                         h->list_cap, h->d_overflow);
@@ -1295,7 +1299,8 @@ mdx_status mdx_get_list(mdx_engine *h, int32_t *nbs1, int32_t *nbs2, int64_t cap
             for (int lane = 0; lane < 64; ++lane) {
                 if (!((e.mask >> lane) & 1ull)) continue;
                 // the half list holds each pair once, in either orientation
-                const int u = ca[c * kClu + (lane >> 3)], v = ca[e.cj * kClu + (lane & 7)];
+                const int cj = (int)((unsigned)e.cj & ~kEntryWrap);
+                const int u = ca[c * kClu + (lane >> 3)], v = ca[cj * kClu + (lane & 7)];
                 const int ai = std::max(u, v), aj = std::min(u, v);
                 P q{keyof(ai, aj), ai + 1, aj + 1};
                 if ((e.mshort >> lane) & 1ull) S.push_back(q);
@@ -2004,6 +2009,35 @@ mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) 
     return MDX_OK;
 }
 
+static mdx_status export_kvectors(const std::vector<int4> &ks, int32_t *nx, int32_t *ny, int32_t *nz, int64_t cap,
+                                  int64_t *nkv) {
+    if (!nkv || cap < 0 || (cap > 0 && (!nx || !ny || !nz))) return MDX_ERR_ARG;
+    *nkv = (int64_t)ks.size();
+    if ((int64_t)ks.size() > cap && cap > 0) return MDX_ERR_ARG;
+    for (size_t k = 0; k < ks.size() && (int64_t)k < cap; ++k) {
+        nx[k] = ks[k].x;
+        ny[k] = ks[k].y;
+        nz[k] = ks[k].z;
+    }
+    return MDX_OK;
+}
+
+mdx_status mdx_get_kvectors(const mdx_engine *h, int32_t *nx, int32_t *ny, int32_t *nz, int64_t cap,
+                            int64_t *nkv) {
+    if (!h) return MDX_ERR_ARG;
+    return export_kvectors(h->h_kidx, nx, ny, nz, cap, nkv);
+}
+
+mdx_status mdx_enumerate_kvectors(const mdx_params *p, int32_t *nx, int32_t *ny, int32_t *nz, int64_t cap,
+                                  int64_t *nkv) {
+    if (!p || !(p->alpha > 0.0) || !(p->fexp > 0.0)) return MDX_ERR_ARG;
+    for (int k = 0; k < 3; ++k)
+        if (!(p->box[k] > 0.0)) return MDX_ERR_ARG;
+    std::vector<int4> ks;
+    enumerate_k(*p, ks);
+    return export_kvectors(ks, nx, ny, nz, cap, nkv);
+}
+
 mdx_status mdx_set_kspace_shape(mdx_engine *h, int32_t chunk, int32_t nslice) {
     if (!h || chunk <= 0 || nslice <= 0) return MDX_ERR_ARG;
     return set_kspace_shape(h, chunk, nslice);
@@ -2119,6 +2153,19 @@ mdx_status mdx_md_get_state(mdx_engine *h, double *x, double *y, double *z, doub
     for (int k = 0; k < 6; ++k)
         if (dst[k]) HIPCHK(h, hipMemcpyAsync(dst[k], src[k], b, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MDX_OK;
+}
+
+mdx_status mdx_md_shake_failures(mdx_engine *h, int64_t *count) {
+    if (!h || !count) return MDX_ERR_ARG;
+    if (!h->md_on) {
+        h->err = "mdx_md_shake_failures before mdx_md_init";
+        return MDX_ERR_STATE;
+    }
+    double v = 0.0;
+    HIPCHK(h, hipMemcpyAsync(&v, h->d_mdstate + MD_SHAKE_FAIL, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *count = (int64_t)v;
     return MDX_OK;
 }
 

@@ -159,7 +159,7 @@ __device__ inline int bound_kind(int i, int j, const int *__restrict__ excl_star
 __global__ __launch_bounds__(64) void k_masks(
     int nclu, const int *__restrict__ cand, const int *__restrict__ ncand, int cap,
     const int *__restrict__ clu_atom, const double4 *__restrict__ spos, const double *__restrict__ anchor,
-    const int *__restrict__ atom_moltype, const int *__restrict__ type_ee, const int *__restrict__ excl_start,
+    const float *__restrict__ ext, double rlist, const int *__restrict__ atom_moltype, const int *__restrict__ type_ee, const int *__restrict__ excl_start,
     const int *__restrict__ excl_atom, const signed char *__restrict__ excl_is14, Box b, double rl2,
     double short2, PairEntry *__restrict__ list, int *__restrict__ nlist, int *__restrict__ nshort,
     int list_cap, int *__restrict__ overflow) {
@@ -172,6 +172,7 @@ __global__ __launch_bounds__(64) void k_masks(
     const int moli = (int)pi.w;
     const int eei = ai >= 0 ? type_ee[atom_moltype[ai]] : 1;
     const double aci[3] = {anchor[ci * 3], anchor[ci * 3 + 1], anchor[ci * 3 + 2]};
+    const double eci[3] = {ext[ci * 3], ext[ci * 3 + 1], ext[ci * 3 + 2]};
     const double ilb[3] = {1.0 / b.l[0], 1.0 / b.l[1], 1.0 / b.l[2]};
     const int nc = ncand[ci];
     const MDX_CONST int *cl = sld(cand) + (size_t)ci * cap;
@@ -214,16 +215,17 @@ __global__ __launch_bounds__(64) void k_masks(
             const int slot = ms != 0ull ? cnt_s : list_cap - 1 - (cnt - cnt_s);
             if (lane == 0 && cnt < list_cap) {
                 PairEntry e;
-                e.cj = cj;
                 double s[3];
+                bool wrap = false;
                 for (int q = 0; q < 3; ++q) {
                     // d * (1/L) and d / L round to the same image unless the
-                    // anchors are near half a box apart; then half the box is
-                    // near the list radius and the pair kernel re-applies the
-                    // minimum image per pair, so either image serves
+                    // anchors are near half a box apart; such entries take the
+                    // minimum image per pair (below), so either image serves
                     double d = aci[q] - anchor[cj * 3 + q];
                     s[q] = d - b.l[q] * rint(d * ilb[q]);
+                    wrap = wrap || rlist + eci[q] + (double)ext[cj * 3 + q] + kImageMargin >= b.h[q];
                 }
+                e.cj = (int)((unsigned)cj | (wrap ? kEntryWrap : 0u));
                 e.sx = (float)s[0]; e.sy = (float)s[1]; e.sz = (float)s[2];
                 e.mask = m;
                 e.mshort = ms;

@@ -7,9 +7,10 @@
 //              float4 (x-anchor, y-anchor, z-anchor, q) plus an int2 (LJ type
 //              | ee flag << 16, low part of q); empty slots hold q = 0 and
 //              never appear in a pair mask
-//   pair list  full cluster-pair list (each i-cluster owns its j-clusters),
-//              32 bytes per entry: j-cluster, FP32 image shift, 64-bit mask
-//              of listed pairs and 64-bit mask of short-class pairs (bit
+//   pair list  half cluster-pair list (each cluster pair once, under the
+//              lower-numbered i-cluster), 32 bytes per entry: j-cluster (and
+//              the per-pair image flag), FP32 image shift, 64-bit mask of
+//              listed pairs and 64-bit mask of short-class pairs (bit
 //              i_slot * 8 + j_slot)
 //   specials   atom pairs the cluster kernel does not handle: 1-4 pairs and
 //              intramolecular pairs of lambda-coupled molecules, classified
@@ -57,11 +58,20 @@ enum EQ {
 };
 
 struct PairEntry {          // 32 bytes, read with scalar loads
-    int32_t cj;
+    int32_t cj;             // j-cluster; bit 31 (kEntryWrap): minimum image per pair
     float sx, sy, sz;
     uint64_t mask;          // pairs in the list
     uint64_t mshort;        // of those, the short (fast) class
 };
+
+// A list entry takes the minimum image per pair (bit 31 of PairEntry.cj)
+// unless, on every axis, r_list + the two clusters' half extents +
+// kImageMargin stays below half the box: then every atom pair within reach
+// has the image of the two cluster anchors, also after each atom has moved
+// up to kImageMargin / 4 since the build (the bound adds twice the motion
+// to the pair distance and twice to the extents).
+constexpr uint32_t kEntryWrap = 0x80000000u;
+constexpr double kImageMargin = 4.0;
 
 // Special pair classes.
 enum { SP_NONE = 0, SP_SHORT14 = 1, SP_SHORT = 2, SP_LONG = 3 };
@@ -194,9 +204,17 @@ struct PairArgs {
     const int *clu_atom;
     float alpha;
     float rc2;                   // evaluation cutoff^2 (Verlet) or +inf (reference)
+    // Verlet lists: pairs whose FP32 r^2 lies within `band` of rc2 take the
+    // cutoff test again in FP64 with the reference arithmetic (CHCNB's
+    // one-shift minimum image, forces.f:1841-1868, and <= against R_cut^2),
+    // so the set of evaluated pairs is that of a list built at the current
+    // positions; band = 0 for reference lists (no cutoff test)
+    float band;
+    double rc2d;
+    const double *x, *y, *z;     // caller coordinates (FP64)
+    Box box;
     float raex;
     int soft;                    // lambda-coupled types present
-    int wrap;                    // small box: re-apply the minimum image per pair
     float lx, ly, lz, ilx, ily, ilz;
     double *fs_x, *fs_y, *fs_z;  // short-class pair forces (per atom)
     double *fl_x, *fl_y, *fl_z;  // long-class pair forces (per atom)
@@ -260,8 +278,8 @@ struct RecipArgs {
     const FRow *frows;
     const FRow *srows;           // pad0: index of the row in frows
     const int4 *fpk;             // per pair entry: k(+j), k(-j), j, frow
-    float4 *ppart;               // [nchunk][npk]: partial U_j, V_j (nchunk a multiple of 8)
-    float2 *s0part;              // [nchunk][nfrows]: partial S(n_z = 0)
+    double4 *ppart;              // [nchunk][npk]: partial U_j, V_j (nchunk a multiple of 8)
+    double2 *s0part;             // [nchunk][nfrows]: partial S(n_z = 0)
     float2 *fa0;                 // per row: A of n_z = 0
     // force pass: tiles of 4 rows and their matrix operand (frows: pad1 =
     // woff of the row's tile, pad2 = row within the tile)

@@ -170,9 +170,19 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
     const MDX_GLOBAL int2 *aux = gld(a.aux);
     MDX_GLOBAL unsigned long long *const jf_base = gst(a.jf) + (size_t)jl * 8 + il;
     const double fscale = a.fscale;
-    const bool wrap = a.wrap;
     const float bl[3] = {a.lx, a.ly, a.lz}, bi[3] = {a.ilx, a.ily, a.ilz};
-    const float rc2 = a.rc2, raex = a.raex;
+    const float rc2 = a.rc2, raex = a.raex, band = a.band;
+    // FP64 cutoff test of this lane's pair with the reference arithmetic
+    auto exact_in = [&](int cj) {
+#pragma clang fp contract(off)
+        const int ai = gld(a.clu_atom)[ci * kClu + il], aj = gld(a.clu_atom)[cj * kClu + jl];
+        double dx = gld(a.x)[ai] - gld(a.x)[aj], dy = gld(a.y)[ai] - gld(a.y)[aj];
+        double dz = gld(a.z)[ai] - gld(a.z)[aj];
+        min_image(dx, dy, dz, a.box);
+        double rr2 = dx * dx + dy * dy;
+        rr2 = rr2 + dz * dz;
+        return rr2 <= a.rc2d;
+    };
     // software pipeline: list entries two ahead (scalar loads), the
     // j-cluster coordinates one ahead
     PairEntry e1{}, e2{};
@@ -180,7 +190,8 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
     int2 aux1 = make_int2(0, 0);
     if (n > 0) e1 = ldv(L + ent(0));
     if (n > 1) e2 = ldv(L + ent(1));
-    if (n > 0) { pj1 = ldv(xq + e1.cj * kClu + jl); aux1 = ldv(aux + e1.cj * kClu + jl); }
+    auto jclu = [](const PairEntry &e) { return (int)((unsigned)e.cj & ~kEntryWrap); };
+    if (n > 0) { pj1 = ldv(xq + jclu(e1) * kClu + jl); aux1 = ldv(aux + jclu(e1) * kClu + jl); }
     for (int k0 = 0; k0 < n; k0 += 16) {
         float fs[3] = {0.f, 0.f, 0.f}, fl[3] = {0.f, 0.f, 0.f};
         float es = 0.f, el = 0.f, ljs = 0.f, ljl = 0.f, vs = 0.f, vl = 0.f;
@@ -204,13 +215,20 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
             float dx = (pi.x - pj.x) + e.sx;
             float dy = (pi.y - pj.y) + e.sy;
             float dz = (pi.z - pj.z) + e.sz;
-            if (wrap) {
+            // entries whose clusters reach near half the box: minimum image
+            // per pair (the flag is uniform over the wavefront)
+            if (e.cj < 0) {
                 dx = fmaf(-bl[0], rintf(dx * bi[0]), dx);
                 dy = fmaf(-bl[1], rintf(dy * bi[1]), dy);
                 dz = fmaf(-bl[2], rintf(dz * bi[2]), dz);
             }
             const float r2 = dx * dx + dy * dy + dz * dz;
-            const bool in = on && r2 <= rc2;
+            bool in = on && r2 <= rc2;
+            if (band > 0.f) {
+                // near the cutoff: the reference's FP64 test decides
+                const bool amb = on && fabsf(r2 - rc2) <= band;
+                if (__ballot(amb) != 0ull && amb) in = exact_in(jclu(e));
+            }
             const float rinv = in ? __builtin_amdgcn_rsqf(r2) : 0.f;
             const float r = r2 * rinv;
             float rei = rinv;
@@ -234,7 +252,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
             const float fxx = fsc * dx, fyy = fsc * dy, fzz = fsc * dz;
             const float vr = fsr * r;
             if (DETAIL && in) {
-                const int aj = a.clu_atom[e.cj * kClu + jl];
+                const int aj = a.clu_atom[jclu(e) * kClu + jl];
                 const int b = a.atom_mol[aj] == moli ? a.nmt : a.atom_moltype[aj];
                 s_det[(b * 2) * 64 + lane] += (double)(pj.w * rei) + (double)(__int_as_float(auxj.y) * rei);
                 s_det[(b * 2 + 1) * 64 + lane] += (double)esr;
@@ -274,7 +292,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
             for (int q = 0; q < 2; ++q) u[q] = swap_sum16(w[q], w[2 + q]);
             const float send = (lane & 8) ? u[0] : u[1], keep = (lane & 8) ? u[1] : u[0];
             s_jf[kk * 64 + lane] = keep + xor8(send);
-            if (lane == 0) s_jc[kk] = e.cj;
+            if (lane == 0) s_jc[kk] = jclu(e);
         };
         for (int k = k0; k < k1; ++k) {
             const PairEntry e = e1;
@@ -287,8 +305,8 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
             // unconditional prefetch (past the end it re-reads the last
             // entry and its j-cluster: valid addresses, values unused)
             e2 = ldv(L + ent(min(k + 2, n - 1)));
-            pj1 = ldv(xq + e1.cj * kClu + jl);
-            aux1 = ldv(aux + e1.cj * kClu + jl);
+            pj1 = ldv(xq + jclu(e1) * kClu + jl);
+            aux1 = ldv(aux + jclu(e1) * kClu + jl);
             if (k < ns) entry(std::true_type{}, k - k0, e, pj, auxj);
             else entry(std::false_type{}, k - k0, e, pj, auxj);
         }

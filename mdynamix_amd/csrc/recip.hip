@@ -6,7 +6,8 @@
 // k_recip_prep: per-atom phase tables along x, y and z (FP64 recurrences);
 // k_sk: S(+-j) = U_j +- i V_j, with U_j, V_j sums of B p_j over the atoms
 //   of a chunk (B = q exp(2 pi i (n_x f_x + n_y f_y)), p_j = exp(2 pi i j
-//   f_z)), as a matrix product on the matrix cores;
+//   f_z)), as a matrix product on the matrix cores whose result for each
+//   32-atom stage is added into FP64 sums;
 // k_sk_reduce: FP64 sums of the chunk partials, energy, virial, and the
 //   per-pair data of the force pass;
 // k_recip_force: lane = atom, rows uniform, the force
@@ -15,6 +16,22 @@
 #include "mdx_internal.h"
 
 namespace mdx {
+
+// f16 pair of an FP32 value: hi = f16(v), lo = f16(v - hi); hi + lo keeps
+// 22 significant bits (absolute error below 2^-24 for |v| <= 1, the range
+// of B and p_j), and the three products hi*hi + hi*lo + lo*hi of two such
+// pairs reproduce an FP32 product to about 2^-21 relative.
+//
+// The value is taken through an empty asm first.  Otherwise FP contraction
+// may fuse the product that formed v into both uses with different
+// roundings (hi from the FP32-rounded product, lo = f16(exact product -
+// hi) in one mixed FMA); where the two sit on either side of an f16
+// rounding midpoint, hi + lo then misses v by a whole f16 ulp.
+__device__ inline void split1(float v, _Float16 &hi, _Float16 &lo) {
+    asm volatile("" : "+v"(v));
+    hi = (_Float16)v;
+    lo = (_Float16)(v - (float)hi);
+}
 
 __device__ inline void recip_prep_body(const RecipArgs &a, int blk) {
     const int i = blk * 256 + threadIdx.x;
@@ -77,8 +94,9 @@ __device__ inline void recip_prep_body(const RecipArgs &a, int blk) {
         const bool odd = i & 1;
         for (int k = 0; k < a.mzp; ++k) {
             const float c = have ? (float)pr : 0.f, d = have ? (float)pi : 0.f;
-            const _Float16 hc = (_Float16)c, hd = (_Float16)d;
-            const _Float16 lc = (_Float16)(c - (float)hc), ld = (_Float16)(d - (float)hd);
+            _Float16 hc, hd, lc, ld;
+            split1(c, hc, lc);
+            split1(d, hd, ld);
             const unsigned hi2 = (unsigned)__builtin_bit_cast(unsigned short, hc) |
                                  ((unsigned)__builtin_bit_cast(unsigned short, hd) << 16);
             const unsigned lo2 = (unsigned)__builtin_bit_cast(unsigned short, lc) |
@@ -105,35 +123,16 @@ __global__ __launch_bounds__(256) void k_recip_prep(const RecipArgs *__restrict_
 }
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-
-// Structure factors on the matrix cores.  For a row (n_x, n_y) with
-// B_a = q_a exp(2 pi i (n_x f_x + n_y f_y)) and p_j = exp(2 pi i j f_z) =
-// C_j + i D_j, the vectors n_z = +-j have S(+-j) = U_j +- i V_j with
-// U_j = sum_a B_a C_j(a), V_j = sum_a B_a D_j(a); n_z = 0 is U_0 (C_0 = 1).
-// Over the atoms of a chunk this is the real product
-//   [Re B; Im B] (rows x atoms)  x  [C_j, D_j] (atoms x j)
-// evaluated with v_mfma_f32_16x16x4_f32: an M tile is 8 rows x (Re, Im),
-// an N tile 8 values of j x (C, D), K = 4 atoms.  A block holds 64 rows
-// (sorted by pair count; a wavefront takes 16 rows = 2 M tiles and as many
-// N tiles as its first, longest row needs) and one chunk of atoms, staged
-// through LDS 32 atoms at a time: B = tx[n_x] ty[|n_y|] (conjugated for
-// n_y < 0) and p_j = tz[j].  Operand layout of the instruction (lane l):
-// A[l % 16][l / 16], B[l / 16][l % 16]; result D[4 (l / 16) + v][l % 16].
-// Each block writes its FP32 chunk sums per pair entry (U_j, V_j) and per
-// row (S(n_z = 0)); k_sk_reduce adds the chunks in FP64.
-// Blocks are numbered so that the row groups of one chunk share an XCD
-// (block index mod 8), whose L2 then serves that chunk's table columns.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-// f16 pair of an FP32 value: hi = f16(v), lo = f16(v - hi); hi + lo keeps
-// 22 significant bits (absolute error below 2^-24 for |v| <= 1, the range
-// of B and p_j), and the three products hi*hi + hi*lo + lo*hi of two such
-// pairs reproduce an FP32 product to about 2^-21 relative.
+// f16 hi/lo pairs of two FP32 values (split1 each).
 __device__ inline void split_f16(float v0, float v1, half2_t &hi, half2_t &lo) {
-    const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
+    _Float16 h0, h1, l0, l1;
+    split1(v0, h0, l0);
+    split1(v1, h1, l1);
     hi = half2_t{h0, h1};
-    lo = half2_t{(_Float16)(v0 - (float)h0), (_Float16)(v1 - (float)h1)};
+    lo = half2_t{l0, l1};
 }
 
 // LDS stage of the structure-factor block: per plane row (a k row or a
@@ -156,9 +155,11 @@ struct SkStage {
 // Over the atoms of a chunk this is the real product
 //   [Re B; Im B] (rows x atoms)  x  [C_j, D_j] (atoms x j)
 // evaluated with v_mfma_f32_16x16x32_f16 on f16 hi/lo pairs of the FP32
-// operands (three products per tile, FP32 accumulation): an M tile is 8
-// rows x (Re, Im), an N tile 8 values of j x (C, D), K = the 32 atoms of an
-// LDS stage.  A block holds 64 rows (sorted by pair count; a wavefront
+// operands (three products per tile, hi*hi + hi*lo + lo*hi): an M tile is
+// 8 rows x (Re, Im), an N tile 8 values of j x (C, D), K = the 32 atoms of
+// an LDS stage.  The instruction sums its 32 atoms' products in FP32; each
+// stage's result is added into FP64 accumulators, so the sum over the
+// chunk's atoms is an FP64 sum of 32-atom partials.  A block holds 64 rows (sorted by pair count; a wavefront
 // takes 16 rows = 2 M tiles and as many N tiles as its first, longest row
 // needs) and one chunk of atoms, staged through LDS 32 atoms at a time:
 // B = tx[n_x] ty[|n_y|] (conjugated for n_y < 0) formed in FP32, then
@@ -166,7 +167,7 @@ struct SkStage {
 // l): A[l % 16][8 (l / 16) + k], B[8 (l / 16) + k][l % 16], k = 0..7;
 // result D[4 (l / 16) + v][l % 16].  Stages alternate between two LDS
 // buffers, the next one fetched into registers while the current one feeds
-// the matrix instructions.  Each block writes its FP32 chunk sums per pair
+// the matrix instructions.  Each block writes its FP64 chunk sums per pair
 // entry (U_j, V_j) and per row (S(n_z = 0)); k_sk_reduce adds the chunks in
 // FP64.  Blocks are numbered so that the row groups of one chunk share an
 // XCD (block index mod 8), whose L2 then serves that chunk's table columns.
@@ -204,11 +205,14 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
     const int rw = rbase + 16 * w;
     const int ntw = rw < a.nfrows ? min(NT, (ldv(gld(a.srows) + rw).m + 8) >> 3) : 0;
     const int ntg = rbase < a.nfrows ? min(NT, (ldv(gld(a.srows) + rbase).m + 8) >> 3) : 0;
-    mfma_f4 acc[2][NT];
+    double acc[2][NT][4];           // FP64 sums over the chunk
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[mt][t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[mt][t][v] = 0.0;
+    const mfma_f4 zero4 = {0.f, 0.f, 0.f, 0.f};
     const MDX_GLOBAL float4 *tx = reinterpret_cast<const MDX_GLOBAL float4 *>(gld(a.tx));
     const MDX_GLOBAL float4 *ty = reinterpret_cast<const MDX_GLOBAL float4 *>(gld(a.ty));
     const MDX_GLOBAL float4 *tz = reinterpret_cast<const MDX_GLOBAL float4 *>(gld(a.tz));
@@ -306,12 +310,17 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
                     const int rb = (2 * (8 * t + rA) + cA) * kSkRowB + col;
                     const half8_t Bh = *reinterpret_cast<const half8_t *>(cur + St::ZH + rb);
                     const half8_t Bl = *reinterpret_cast<const half8_t *>(cur + St::ZL + rb);
-                    acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah0, Bh, acc[0][t], 0, 0, 0);
-                    acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah1, Bh, acc[1][t], 0, 0, 0);
-                    acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah0, Bl, acc[0][t], 0, 0, 0);
-                    acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah1, Bl, acc[1][t], 0, 0, 0);
-                    acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al0, Bh, acc[0][t], 0, 0, 0);
-                    acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al1, Bh, acc[1][t], 0, 0, 0);
+                    mfma_f4 p0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah0, Bh, zero4, 0, 0, 0);
+                    mfma_f4 p1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah1, Bh, zero4, 0, 0, 0);
+                    p0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah0, Bl, p0, 0, 0, 0);
+                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah1, Bl, p1, 0, 0, 0);
+                    p0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al0, Bh, p0, 0, 0, 0);
+                    p1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al1, Bh, p1, 0, 0, 0);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        acc[0][t][v] += (double)p0[v];
+                        acc[1][t][v] += (double)p1[v];
+                    }
                 }
             }
         }
@@ -321,7 +330,7 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
     // results: through LDS so that one lane holds (Re U, Im U, Re V, Im V)
     // of one (row, j): M = 4 (l / 16) + v is (row r = M % 8, Re/Im = M / 8),
     // N = l % 16 is (j = N % 8, C/D = N / 8)
-    float *scr = reinterpret_cast<float *>(lds) + w * 256;
+    double *scr = reinterpret_cast<double *>(lds) + w * 256;
     const int r = lane >> 3, jj = lane & 7;
     const int q0 = (r & 3) * 64 + 16 * (r >> 2) + jj;
 #pragma unroll
@@ -334,11 +343,11 @@ __device__ inline void sk_body(const RecipArgs &a, int blk, unsigned char *lds) 
             __syncthreads();
             const int row = rw + 8 * mt + r;
             if (t < ntw && row < a.nfrows) {
-                const float ur = scr[q0], ui = scr[q0 + 32], vr = scr[q0 + 8], vi = scr[q0 + 40];
+                const double ur = scr[q0], ui = scr[q0 + 32], vr = scr[q0 + 8], vi = scr[q0 + 40];
                 const FRow fr = ldv(gld(a.srows) + row);
                 const int j = 8 * t + jj;
-                if (j == 0) stv(gst(a.s0part) + (size_t)ch * a.nfrows + fr.pad0, make_float2(ur, ui));
-                else if (j <= fr.m) stv(gst(a.ppart) + (size_t)ch * a.npk + fr.off + j - 1, make_float4(ur, ui, vr, vi));
+                if (j == 0) stv(gst(a.s0part) + (size_t)ch * a.nfrows + fr.pad0, make_double2(ur, ui));
+                else if (j <= fr.m) stv(gst(a.ppart) + (size_t)ch * a.npk + fr.off + j - 1, make_double4(ur, ui, vr, vi));
             }
             __syncthreads();
         }
@@ -396,11 +405,11 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
         if (k.x >= 0 || k.y >= 0) {
             double ur = 0.0, ui = 0.0, vr = 0.0, vi = 0.0;
             for (int c = 0; c < a.nchunk; ++c) {
-                const float4 p = ldv(gld(a.ppart) + (size_t)c * a.npk + e);
-                ur += (double)p.x;
-                ui += (double)p.y;
-                vr += (double)p.z;
-                vi += (double)p.w;
+                const double4 p = ldv(gld(a.ppart) + (size_t)c * a.npk + e);
+                ur += p.x;
+                ui += p.y;
+                vr += p.z;
+                vi += p.w;
             }
             float2 ap = make_float2(0.f, 0.f), am = make_float2(0.f, 0.f);
             if (k.x >= 0) ap = k_terms(a, fr.ix, fr.iy, k.z, ur - vi, ui + vr, acc);
@@ -417,9 +426,10 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
                                      (size_t)(jm >> 4) * kForceStepH + 8 * (16 * (u >> 2) + 4 * rl) + 2 * (u & 3);
             for (int s = 0; s < 4; ++s)
                 for (int cm = 0; cm < 2; ++cm) {
-                    const _Float16 hi = (_Float16)c[cm][s];
+                    _Float16 hi, lo;
+                    split1(c[cm][s], hi, lo);
                     w[8 * s + cm] = hi;
-                    w[512 + 8 * s + cm] = (_Float16)(c[cm][s] - (float)hi);
+                    w[512 + 8 * s + cm] = lo;
                 }
         }
     } else if (e < a.npk + a.nfrows) {
@@ -429,9 +439,9 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
         if (fr.k0 >= 0) {
             double sr = 0.0, si = 0.0;
             for (int c = 0; c < a.nchunk; ++c) {
-                const float2 p = ldv(gld(a.s0part) + (size_t)c * a.nfrows + r);
-                sr += (double)p.x;
-                si += (double)p.y;
+                const double2 p = ldv(gld(a.s0part) + (size_t)c * a.nfrows + r);
+                sr += p.x;
+                si += p.y;
             }
             a0 = k_terms(a, fr.ix, fr.iy, 0, sr, si, acc);
         }
@@ -477,10 +487,11 @@ __global__ __launch_bounds__(256) void k_sk_reduce(const RecipArgs *__restrict__
 // Lane l then holds 2^e (S1, S2, Z1, Z2) of row l / 16 of the tile and atom
 // l % 16, and adds
 //   t = b_i A0_r - b_r A0_i + b_r S1 + b_i S2,  t_z = b_r Z1 + b_i Z2
-// (A0: n_z = 0) into F_x += n_x t, F_y += n_y t, F_z += t_z (FP32 sums
-// over a run of tiles of one n_x, added into FP64 when the run ends); b
-// advances over the 4-row tiles of one n_x by exp(8 pi i f_y) (FP64) and
-// restarts from an FP64 sincospi at every new n_x.  A wavefront takes
+// (A0: n_z = 0) into the FP64 sums F_x += n_x t, F_y += n_y t, F_z += t_z
+// (t and t_z are FP32 terms of one atom and k row: their sums over n_z run
+// inside the matrix instruction); b advances over the 4-row tiles of one
+// n_x by exp(8 pi i f_y) (FP64) and restarts from an FP64 sincospi at
+// every new n_x.  A wavefront takes
 // kForceTiles tiles of 16 atoms, a block 4 wavefronts, each block one slice
 // of the tiles; the 4 lanes of an atom are summed at the end and every
 // slice writes its own partial force array (summed by the combine kernel).
@@ -498,9 +509,6 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
     half8_t Ph[kForceTiles][S16], Pl[kForceTiles][S16];
     double w4r[kForceTiles], w4i[kForceTiles], br[kForceTiles], bi[kForceTiles];
     double Fx[kForceTiles], Fy[kForceTiles], Fz[kForceTiles];
-    // FP32 sums over the current run of tiles of one n_x (sum t, sum n_y t,
-    // sum t_z), added into the FP64 forces when the run ends
-    float Sx[kForceTiles], Sy[kForceTiles], Sz[kForceTiles];
 #pragma unroll
     for (int ta = 0; ta < kForceTiles; ++ta) {
         const int i = abase + 16 * ta;
@@ -524,9 +532,10 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         for (int stp = 0; stp < S16; ++stp)
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const _Float16 hi = (_Float16)pv[stp][k];
+                _Float16 hi, lo;
+                split1(pv[stp][k], hi, lo);
                 Ph[ta][stp][k] = hi;
-                Pl[ta][stp][k] = (_Float16)(pv[stp][k] - (float)hi);
+                Pl[ta][stp][k] = lo;
             }
         double s4, c4;
         {
@@ -538,7 +547,6 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         w4i[ta] = s4;
         br[ta] = bi[ta] = 0.0;
         Fx[ta] = Fy[ta] = Fz[ta] = 0.0;
-        Sx[ta] = Sy[ta] = Sz[ta] = 0.f;
     }
     const MDX_CONST FTile *tiles = sld(a.ftiles);
     const MDX_GLOBAL half8_t *wm = reinterpret_cast<const MDX_GLOBAL half8_t *>(gld(a.wm)) + lane;
@@ -563,16 +571,6 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         const float2 a0 = K < T.nrow ? a0raw : make_float2(0.f, 0.f);
         const int ny = T.iy0 + K;
         const bool restart = T.ix != cix || T.iy0 != ciy + 4;
-        if (restart) {
-            // the run of tiles of one n_x ends: its FP32 sums into FP64
-#pragma unroll
-            for (int ta = 0; ta < kForceTiles; ++ta) {
-                Fx[ta] += (double)cix * (double)Sx[ta];
-                Fy[ta] += (double)Sy[ta];
-                Fz[ta] += (double)Sz[ta];
-                Sx[ta] = Sy[ta] = Sz[ta] = 0.f;
-            }
-        }
         cix = T.ix;
         ciy = T.iy0;
         mfma_f4 acc[kForceTiles];
@@ -616,9 +614,10 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
             const float bre = (float)br[ta], bim = (float)bi[ta];
             const float ts = (bim * a0.x - bre * a0.y) + (bre * acc[ta][0] + bim * acc[ta][1]) * isc;
             const float tzs = (bre * acc[ta][2] + bim * acc[ta][3]) * isc;
-            Sx[ta] += ts;
-            Sy[ta] = fmaf((float)ny, ts, Sy[ta]);
-            Sz[ta] += tzs;
+            const double td = (double)ts;
+            Fx[ta] = fma((double)T.ix, td, Fx[ta]);
+            Fy[ta] = fma((double)ny, td, Fy[ta]);
+            Fz[ta] += (double)tzs;
         }
     };
     if (t0 < t1) load(t0, T0, Ah0, Al0, a00);
@@ -628,12 +627,6 @@ __device__ inline void recip_force_body(const RecipArgs &a, int blk) {
         if (t + 1 >= t1) break;
         load(min(t + 2, t1 - 1), T0, Ah0, Al0, a00);
         tile(T1, Ah1, Al1, a01);
-    }
-#pragma unroll
-    for (int ta = 0; ta < kForceTiles; ++ta) {
-        Fx[ta] += (double)cix * (double)Sx[ta];
-        Fy[ta] += (double)Sy[ta];
-        Fz[ta] += (double)Sz[ta];
     }
     const double tx = 2.0 * kPI / a.box.l[0], ty = 2.0 * kPI / a.box.l[1], tz = 2.0 * kPI / a.box.l[2];
     MDX_GLOBAL double *o = gst(a.rpart) + (size_t)sl * 3 * a.natoms;

@@ -72,16 +72,20 @@ def load_library() -> C.CDLL:
     L.mdx_batch_destroy.argtypes = [pv]
     L.mdx_autotune.argtypes = [pv, pi32, pi32]
     L.mdx_set_kspace_shape.argtypes = [pv, C.c_int32, C.c_int32]
+    L.mdx_get_kvectors.argtypes = [pv, pi32, pi32, pi32, C.c_int64, C.POINTER(C.c_int64)]
+    L.mdx_enumerate_kvectors.argtypes = [pv, pi32, pi32, pi32, C.c_int64, C.POINTER(C.c_int64)]
     L.mdx_md_init.argtypes = [pv, pv, pd, pd, pd]
     L.mdx_md_run.argtypes = [pv, C.c_int32, pv]
     L.mdx_md_get_state.argtypes = [pv] + [pd] * 6
     L.mdx_batch_md_run.argtypes = [pv, C.c_int32, pv]
+    L.mdx_md_shake_failures.argtypes = [pv, C.POINTER(C.c_int64)]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
                  "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_get_list", "mdx_compute",
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
-                 "mdx_batch_md_run", "mdx_autotune", "mdx_set_kspace_shape"):
+                 "mdx_batch_md_run", "mdx_autotune", "mdx_set_kspace_shape", "mdx_get_kvectors",
+                 "mdx_enumerate_kvectors", "mdx_md_shake_failures"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -217,6 +221,11 @@ class ForceEngine:
         self._check(self.lib.mdx_autotune(self._h, C.byref(c), C.byref(sl)))
         return {"chunk": c.value, "nslice": sl.value}
 
+    def kvectors(self) -> np.ndarray:
+        """The engine's k set as an (nkv, 3) array of integer triples n in
+        INIFUR order (RKX = 2 pi n_x, forces.f:1375-1427)."""
+        return _kvectors(lambda *a: self.lib.mdx_get_kvectors(self._h, *a), self._check)
+
     def set_kspace_shape(self, chunk: int, nslice: int) -> None:
         self._check(self.lib.mdx_set_kspace_shape(self._h, int(chunk), int(nslice)))
 
@@ -246,6 +255,12 @@ class ForceEngine:
         out = np.zeros((6, self.n))
         self._check(self.lib.mdx_md_get_state(self._h, *[as_ptr(out[k]) for k in range(6)]))
         return out[0:3].T.copy(), out[3:6].T.copy()
+
+    def md_shake_failures(self) -> int:
+        """Constraint failures counted by the device SHAKE since md_init."""
+        c = C.c_int64()
+        self._check(self.lib.mdx_md_shake_failures(self._h, C.byref(c)))
+        return c.value
 
     def memory_bytes(self) -> int:
         b = C.c_int64()
@@ -304,6 +319,27 @@ class ForceBatch:
             self.close()
         except Exception:
             pass
+
+
+def _kvectors(call, check) -> np.ndarray:
+    n = C.c_int64()
+    check(call(None, None, None, 0, C.byref(n)))
+    out = np.zeros((3, max(n.value, 1)), np.int32)
+    pi = C.c_int32
+    check(call(as_ptr(out[0], pi), as_ptr(out[1], pi), as_ptr(out[2], pi), out.shape[1], C.byref(n)))
+    return out[:, :n.value].T.copy()
+
+
+def enumerate_kvectors(params: Dict) -> np.ndarray:
+    """INIFUR's k set for `params` (a dict of mdx_params fields) from the
+    engine's host-side enumeration; needs the library but no device."""
+    L = load_library()
+    ps = abi.make_params(**params)
+
+    def check(st):
+        if st != 0:
+            raise MdxError(st, "mdx_enumerate_kvectors")
+    return _kvectors(lambda *a: L.mdx_enumerate_kvectors(C.byref(ps), *a), check)
 
 
 def device_info() -> Dict:

@@ -397,6 +397,13 @@
         write(*,*)'!!! GPU force engine: rectangular cell, Ewald only'
         call FINAL
       end if
+C  per-term bonded averages (Average_int yes, input.f:372-374) need the
+C  BR/BE/AR/AE/TR/TE sums of GETBND/GETANG/GETTRS, which this shim does
+C  not fill: stop rather than let GETAVR average zeros (aver.f:43-112)
+      if(ISTAV.gt.0)then
+        write(*,*)'!!! GPU force engine: Average_int yes not supported'
+        call FINAL
+      end if
       IST=mdx_create(topo,prm,hdl)
       if(IST.ne.0)call MDXERR(IST,'mdx_create')
       lastbox=prm%box

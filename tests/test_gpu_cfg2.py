@@ -1,0 +1,117 @@
+"""The 32k SPC/E + 0.15 M NaCl box at its full size and parameters (96,174
+atoms, 98.7 A, R_cut 12 A, R_short 5 A, alpha R_cut 3.2, FEXP 11.5,
+48,052 k vectors), the configuration the benchmark measures, against the
+FP64 oracle's fixture (tests/golden/make_cfg2.py).
+
+Tolerances (the engine's precision contract): force RMS relative error
+<= 1e-5 against FP64 (F, and G and H, whose fixture copies are FP32, i.e.
+6e-8 relative), total potential energy <= 1e-6 relative, each component
+and virial <= 1e-5 (a component much smaller than the total is held to
+1e-5 of 1e-3 x the sum of component magnitudes); lists and the k set
+bit-exact.  The reciprocal sum is checked at the default launch shape and
+at the 4,096-atom structure-factor chunks the autotuner picks for this
+box."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from tests.helpers import GOLDEN
+from tests.test_gpu_parity import check_energies, rms_rel, FORCE_RMS_TOL, ENERGY_KEYS
+from mdynamix_amd import abi, systems, topology
+from mdynamix_amd.engine import ForceEngine, enumerate_kvectors
+
+pytestmark = pytest.mark.gpu
+
+TWOPI = 2.0 * 3.1415926536
+
+
+def _fixture():
+    z = np.load(os.path.join(GOLDEN, "spce_nacl_32k.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _digest(a, b):
+    return hashlib.sha256(np.ascontiguousarray(a, "<i4").tobytes() + np.ascontiguousarray(b, "<i4").tobytes()).hexdigest()
+
+
+def _system(fx):
+    s = systems.cfg_large(int(fx["SEED"]))
+    assert _sha(s.pos.astype("<f8")) == str(fx["POS_SHA_0"]), "generator positions changed"
+    return s
+
+
+def test_cfg2_reference_lists_kset_and_forces():
+    fx = _fixture()
+    s = _system(fx)
+    topo, prm = topology.from_system(s)
+    prm = dict(prm, list_mode=abi.MDX_LIST_REFERENCE, skin=0.0)
+    eng = ForceEngine(topo, prm)
+    try:
+        eng.set_positions(s.pos)
+        print("case cfg2 reference lists", topo.natoms, flush=True)
+        eng.chcnb()
+        assert eng.list_counts() == tuple(int(v) for v in fx["LISTLEN_0"])
+        s1, s2, l1, l2 = eng.get_list()
+        assert _digest(s1, s2) == str(fx["DIGEST_SHORT_0"])
+        assert _digest(l1, l2) == str(fx["DIGEST_LONG_0"])
+        del s1, s2, l1, l2
+        kv = eng.kvectors()
+        assert len(kv) == int(fx["NKV_0"]) == 48052
+        assert np.array_equal(kv, enumerate_kvectors(prm))
+        assert _sha((TWOPI * kv.astype(np.float64)).astype("<f8")) == str(fx["KSET_SHA_0"])
+        ref = dict(zip(abi.MdxEnergies.SCALARS, fx["ENERGY_0"]))
+        for shape in (None, (4096, 6)):
+            if shape is not None:
+                eng.set_kspace_shape(*shape)
+            print("case cfg2 k-space shape", shape, flush=True)
+            f, g, h, e = eng.allforce()
+            assert rms_rel(f, fx["F_0"]) <= FORCE_RMS_TOL, shape
+            assert rms_rel(g, fx["G_0"].astype(np.float64)) <= FORCE_RMS_TOL, shape
+            assert rms_rel(h, fx["H_0"].astype(np.float64)) <= FORCE_RMS_TOL, shape
+            assert (e["npairs_short"], e["npairs_long"]) == tuple(int(v) for v in fx["NPAIRS_0"])
+            check_energies(e, ref)
+            print("cfg2 shape %s: F rms %.2e, total energy rel %.2e" % (
+                shape, rms_rel(f, fx["F_0"]),
+                abs(sum(e[k] for k in ENERGY_KEYS) - sum(ref[k] for k in ENERGY_KEYS)) /
+                abs(sum(ref[k] for k in ENERGY_KEYS))), flush=True)
+    finally:
+        eng.close()
+
+
+def test_cfg2_verlet_list_at_displaced_positions():
+    """The benchmark's list (Verlet, 1 A skin) built at the generator's
+    positions and evaluated after every atom moved up to 0.25 A per axis:
+    the pairs within R_cut are those of a reference list built at the new
+    positions (same count), forces and total energy within the tolerances.
+    The short/long class split stays as built, so only class sums are
+    compared."""
+    fx = _fixture()
+    s = _system(fx)
+    topo, prm = topology.from_system(s)
+    assert prm["list_mode"] == abi.MDX_LIST_VERLET and prm["skin"] == 1.0
+    pos2 = s.pos + np.random.default_rng(int(fx["DISPLACE_SEED"])).uniform(
+        -float(fx["DISPLACE"]), float(fx["DISPLACE"]), s.pos.shape)
+    assert _sha(pos2.astype("<f8")) == str(fx["POS_SHA_1"])
+    eng = ForceEngine(topo, prm)
+    try:
+        eng.set_positions(s.pos)
+        eng.chcnb()
+        eng.set_positions(pos2)
+        print("case cfg2 verlet displaced", flush=True)
+        f, g, h, e = eng.allforce()
+        ref = dict(zip(abi.MdxEnergies.SCALARS, fx["ENERGY_1"]))
+        assert e["npairs_short"] + e["npairs_long"] == int(fx["NPAIRS_1"].sum())
+        assert rms_rel(f, fx["F_1"]) <= FORCE_RMS_TOL
+        merged = lambda d: dict(d, pe1=d["pe1"] + d["pe2"], pe2=0.0, pels1=d["pels1"] + d["pels2"], pels2=0.0,
+                                vir1=d["vir1"] + d["vir2"], vir2=0.0, virx=d["virx"] + d["virfx"],
+                                viry=d["viry"] + d["virfy"], virz=d["virz"] + d["virfz"])
+        check_energies(merged(e), merged(ref),
+                       keys=("pe1", "pels1", "spe", "pint", "vir1", "virb", "virx", "viry", "virz"))
+    finally:
+        eng.close()

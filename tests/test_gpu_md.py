@@ -94,6 +94,28 @@ def test_md_segment_matches_reference_integrator():
     assert res["T20"] <= 1e-6 and res["SC20"] <= 1e-5
     assert res["bond1"] <= 1e-11 and res["bond20"] <= 1e-11
     assert np.all(np.isfinite(tr))
+    assert eng.md_shake_failures() == 0
+
+
+def test_shake_failures_are_counted():
+    """With MAXIT = 0 (one SHEJK sweep) every molecule stops before the
+    tolerance is met (the limit is set from the host, every kernel exits
+    normally): the run continues, as the reference's does after its
+    NO CONVERGENCE message (mdstep.f:1264-1268), and the count says so."""
+    s = systems.cfg_small()
+    topo, prm = topology.from_system(s)
+    eng = ForceEngine(topo, prm)
+    try:
+        eng.set_positions(s.pos)
+        md = topology.md_params(s)
+        md.update(shake_maxit=0)
+        eng.md_init(md, systems.maxwell_velocities(s, 5))
+        print("case shake maxit 0", flush=True)
+        tr = eng.md_run(2)
+        assert np.all(np.isfinite(tr))
+        assert eng.md_shake_failures() > 0
+    finally:
+        eng.close()
 
 
 def test_batched_md_matches_alone():
