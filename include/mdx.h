@@ -262,6 +262,26 @@ mdx_status mdx_get_kvectors(const mdx_engine *h, int32_t *nx, int32_t *ny, int32
  * host, without a device or a handle. */
 mdx_status mdx_enumerate_kvectors(const mdx_params *p, int32_t *nx, int32_t *ny, int32_t *nz, int64_t cap,
                                   int64_t *nkv);
+/*
+ * FURIR in FP64 on device arrays, without a handle: the reciprocal Ewald
+ * forces (forces.f:1320-1326), the energy QPE (forces.f:1317-1318) and the
+ * virial projections (forces.f:1328-1335) of natoms charges over the k set
+ * INIFUR enumerates from p (box, alpha, fexp; rectangular cell,
+ * forces.f:1375-1427).  x, y, z, q and the outputs fx, fy, fz (forces,
+ * written) and energy4 (QPE, VIRX, VIRY, VIRZ) are device arrays; the
+ * caller's workspace (device memory of mdx_furir_workspace_bytes bytes)
+ * holds the k set, the structure factors and the partial sums.  Work is
+ * queued on `stream` (a hipStream_t; NULL: the default stream) and not
+ * waited for.  Every sum runs in a fixed order, so repeated calls give the
+ * same bits.  FURIR's single-molecule exit (forces.f:1266) is the
+ * caller's: this routine always sums.  Replaces FURIR's arithmetic
+ * (forces.f:1271-1335) for callers holding coordinates in HBM; the engine
+ * uses it in its FP64 precision mode.
+ */
+mdx_status mdx_furir_workspace_bytes(const mdx_params *p, int32_t natoms, int64_t *bytes);
+mdx_status mdx_furir_device(const mdx_params *p, int32_t natoms, const double *x, const double *y, const double *z,
+                            const double *q, double *fx, double *fy, double *fz, double *energy4, void *workspace,
+                            int64_t workspace_bytes, void *stream);
 /* Set that shape directly (reproducible runs, profiling). */
 mdx_status mdx_set_kspace_shape(mdx_engine *h, int32_t chunk, int32_t nslice);
 /* Device memory held by a handle, in bytes (admission by HBM budget). */

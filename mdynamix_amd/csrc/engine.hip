@@ -26,6 +26,7 @@
 #include "recip.hip"
 #include "bonded.hip"
 #include "md.hip"
+#include "furir64.hip"
 
 using namespace mdx;
 
@@ -304,6 +305,73 @@ void enumerate_k(const mdx_params &p, std::vector<int4> &out) {
                 if (rk2 <= cutk2) out.push_back(make_int4(ikx, iky, ikz, 0));
             }
         }
+}
+
+// INIFUR's cube (KMAX, KMAY, KMAZ, forces.f:1370-1372) and cut CUTK2.
+void inifur_cube(const mdx_params &p, int &kx, int &ky, int &kz, double &cutk2) {
+    cutk2 = 4.0 * (p.alpha * p.alpha) * p.fexp;
+    const double cutk = std::sqrt(cutk2);
+    const double twopi = 2.0 * kPI;
+    kx = (int)(cutk * p.box[0] / twopi + 1.0);
+    ky = (int)(cutk * p.box[1] / twopi + 1.0);
+    kz = (int)(cutk * p.box[2] / twopi + 1.0);
+}
+
+// Size of INIFUR's k set, by the same loops as enumerate_k without
+// storing the vectors (no host allocation).
+long long count_k(const mdx_params &p) {
+    int kx, ky, kz;
+    double cutk2;
+    inifur_cube(p, kx, ky, kz, cutk2);
+    const double twopi = 2.0 * kPI;
+    long long n = 0;
+    for (int ikz = -kz; ikz <= kz; ++ikz)
+        for (int ikx = 0; ikx <= kx; ++ikx) {
+            const int ky1 = ikx == 0 ? (ikz > 0 ? 0 : 1) : -ky;
+            for (int iky = ky1; iky <= ky; ++iky) {
+                const double ax = twopi * ikx / p.box[0], ay = twopi * iky / p.box[1], az = twopi * ikz / p.box[2];
+                double rk2 = ax * ax + ay * ay;
+                rk2 = rk2 + az * az;
+                if (rk2 <= cutk2) ++n;
+            }
+        }
+    return n;
+}
+
+bool fur_params_ok(const mdx_params *p) {
+    if (!p || !(p->alpha > 0.0) || !(p->fexp > 0.0)) return false;
+    for (int k = 0; k < 3; ++k)
+        if (!(p->box[k] > 0.0)) return false;
+    return true;
+}
+
+// Queue the FP64 FURIR kernels on `st` with every buffer in `ws`
+// (fur_layout(n, nkv).total bytes).
+void fur_launch(const mdx_params &p, int n, long long nkv, const double *x, const double *y, const double *z,
+                const double *q, double *fx, double *fy, double *fz, double *e4, char *ws, hipStream_t st) {
+    const FurLayout L = fur_layout(n, (int)nkv);
+    FurArgs a{};
+    a.n = n;
+    a.nkv = (int)nkv;
+    a.nchunk = cdiv(std::max(n, 1), kFurChunk);
+    inifur_cube(p, a.kmax, a.kmay, a.kmaz, a.cutk2);
+    a.lx = p.box[0]; a.ly = p.box[1]; a.lz = p.box[2];
+    a.alp2 = p.alpha * p.alpha;
+    a.cfur = 4.0 * kPI * p.coulomb_factor / (p.box[0] * p.box[1] * p.box[2]);
+    a.x = x; a.y = y; a.z = z; a.q = q;
+    a.fx = fx; a.fy = fy; a.fz = fz; a.e4 = e4;
+    a.kv = reinterpret_cast<int4 *>(ws + L.kv);
+    a.spart = reinterpret_cast<double2 *>(ws + L.spart);
+    a.ak = reinterpret_cast<double2 *>(ws + L.ak);
+    a.epart = reinterpret_cast<double4 *>(ws + L.epart);
+    a.count = reinterpret_cast<int *>(ws + L.count);
+    const int nkb = cdiv(std::max<long long>(nkv, 1), kFurT);
+    k_fur_kset<<<1, 1024, 0, st>>>(a);
+    if (nkv > 0 && n > 0) k_fur_sk<<<dim3(nkb, a.nchunk), kFurT, 0, st>>>(a);
+    else a.nchunk = 0;
+    k_fur_sred<<<nkb, kFurT, 0, st>>>(a);
+    if (n > 0) k_fur_force<<<cdiv(n, kFurT), kFurT, 0, st>>>(a);
+    k_fur_esum<<<1, kFurT, 0, st>>>(a);
 }
 
 // Atom chunks of the structure-factor pass: a multiple of 8 chunks (the
@@ -2036,6 +2104,27 @@ mdx_status mdx_enumerate_kvectors(const mdx_params *p, int32_t *nx, int32_t *ny,
     std::vector<int4> ks;
     enumerate_k(*p, ks);
     return export_kvectors(ks, nx, ny, nz, cap, nkv);
+}
+
+mdx_status mdx_furir_workspace_bytes(const mdx_params *p, int32_t natoms, int64_t *bytes) {
+    if (!fur_params_ok(p) || natoms < 0 || !bytes) return MDX_ERR_ARG;
+    const long long nkv = count_k(*p);
+    if (nkv > 4000000) return MDX_ERR_NKV;
+    *bytes = (int64_t)fur_layout(natoms, (int)nkv).total;
+    return MDX_OK;
+}
+
+mdx_status mdx_furir_device(const mdx_params *p, int32_t natoms, const double *x, const double *y, const double *z,
+                            const double *q, double *fx, double *fy, double *fz, double *energy4, void *workspace,
+                            int64_t workspace_bytes, void *stream) {
+    if (!fur_params_ok(p) || natoms < 0 || !energy4 || !workspace) return MDX_ERR_ARG;
+    if (natoms > 0 && (!x || !y || !z || !q || !fx || !fy || !fz)) return MDX_ERR_ARG;
+    const long long nkv = count_k(*p);
+    if (nkv > 4000000) return MDX_ERR_NKV;
+    if ((int64_t)fur_layout(natoms, (int)nkv).total > workspace_bytes) return MDX_ERR_ARG;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    fur_launch(*p, natoms, nkv, x, y, z, q, fx, fy, fz, energy4, static_cast<char *>(workspace), st);
+    return hipGetLastError() == hipSuccess ? MDX_OK : MDX_ERR_HIP;
 }
 
 mdx_status mdx_set_kspace_shape(mdx_engine *h, int32_t chunk, int32_t nslice) {

@@ -79,13 +79,17 @@ def load_library() -> C.CDLL:
     L.mdx_md_get_state.argtypes = [pv] + [pd] * 6
     L.mdx_batch_md_run.argtypes = [pv, C.c_int32, pv]
     L.mdx_md_shake_failures.argtypes = [pv, C.POINTER(C.c_int64)]
+    L.mdx_furir_workspace_bytes.argtypes = [pv, C.c_int32, C.POINTER(C.c_int64)]
+    # p, natoms, x, y, z, q, fx, fy, fz, energy4, [workspace, bytes,] stream
+    L.mdx_furir_device.argtypes = [pv, C.c_int32] + [pv] * 8 + [pv, C.c_int64, pv]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
                  "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_get_list", "mdx_compute",
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
                  "mdx_batch_md_run", "mdx_autotune", "mdx_set_kspace_shape", "mdx_get_kvectors",
-                 "mdx_enumerate_kvectors", "mdx_md_shake_failures"):
+                 "mdx_enumerate_kvectors", "mdx_md_shake_failures", "mdx_furir_workspace_bytes",
+                 "mdx_furir_device"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -340,6 +344,31 @@ def enumerate_kvectors(params: Dict) -> np.ndarray:
         if st != 0:
             raise MdxError(st, "mdx_enumerate_kvectors")
     return _kvectors(lambda *a: L.mdx_enumerate_kvectors(C.byref(ps), *a), check)
+
+
+def furir_workspace_bytes(params: Dict, natoms: int) -> int:
+    """Device workspace mdx_furir_device needs for natoms atoms."""
+    L = load_library()
+    ps = abi.make_params(**params)
+    b = C.c_int64()
+    st = L.mdx_furir_workspace_bytes(C.byref(ps), natoms, C.byref(b))
+    if st != 0:
+        raise MdxError(st, "mdx_furir_workspace_bytes")
+    return b.value
+
+
+def furir_device(params: Dict, natoms: int, xyzq, forces, energy4: int, workspace: int, workspace_bytes: int,
+                 stream: int = 0) -> None:
+    """FURIR in FP64 on device arrays (addresses as ints): xyzq = (x, y, z,
+    q), forces = (fx, fy, fz), energy4 = QPE and the three virial
+    projections; queued on `stream` (0: default), not waited for."""
+    L = load_library()
+    ps = abi.make_params(**params)
+    v = lambda p: C.c_void_p(p) if p else None
+    st = L.mdx_furir_device(C.byref(ps), natoms, *[v(p) for p in xyzq], *[v(p) for p in forces], v(energy4),
+                            v(workspace), workspace_bytes, v(stream))
+    if st != 0:
+        raise MdxError(st, "mdx_furir_device")
 
 
 def device_info() -> Dict:

@@ -41,3 +41,44 @@ def golden_com(d, call=0):
 
 def golden_forces(d, key, call=0):
     return np.stack([d[key + "X"][call], d[key + "Y"][call], d[key + "Z"][call]], 1)
+
+
+class DeviceArrays:
+    """Device buffers from the HIP runtime the engine library links
+    (libamdhip64.so.7, already loaded with it), for tests of entry points that
+    take device pointers.  A second HIP runtime in the process (the one a
+    PyTorch wheel bundles) would compete with it for the device."""
+
+    def __init__(self):
+        import ctypes as C
+        from mdynamix_amd.engine import load_library
+        load_library()
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.ptrs = []
+
+    def alloc(self, nbytes):
+        p = self.C.c_void_p()
+        assert self.hip.hipMalloc(self.C.byref(p), self.C.c_size_t(max(int(nbytes), 1))) == 0
+        self.ptrs.append(p.value)
+        return p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        assert self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0      # host to device
+        return p
+
+    def download(self, p, n, dtype=np.float64):
+        out = np.zeros(n, dtype)
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def close(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
