@@ -45,7 +45,9 @@ typedef enum mdx_status {
     MDX_ERR_NKV = 3,            /* too many k-vectors (forces.f:1414)           */
     MDX_ERR_HIP = 4,            /* HIP runtime error                            */
     MDX_ERR_NOMEM = 5,          /* device allocation failed                     */
-    MDX_ERR_STATE = 6           /* call order (e.g. compute before a list)      */
+    MDX_ERR_STATE = 6,          /* call order (e.g. compute before a list)      */
+    MDX_ERR_LIMIT = 7           /* beyond the fixed scratch of a call that
+                                   allocates nothing                            */
 } mdx_status;
 
 /* Neighbour-list semantics. */
@@ -282,6 +284,23 @@ mdx_status mdx_furir_workspace_bytes(const mdx_params *p, int32_t natoms, int64_
 mdx_status mdx_furir_device(const mdx_params *p, int32_t natoms, const double *x, const double *y, const double *z,
                             const double *q, double *fx, double *fy, double *fz, double *energy4, void *workspace,
                             int64_t workspace_bytes, void *stream);
+/*
+ * mdx_furir_device without the workspace argument, for callers that cannot
+ * allocate device memory (code under hipGraph stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The k set, structure factors and partial sums live in a
+ * static device array of MDX_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; calls whose mdx_furir_workspace_bytes exceeds it return
+ * MDX_ERR_LIMIT before any device work.  The call allocates nothing (no
+ * hipMalloc, hipFree or allocating library routine).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two host
+ * threads, or two streams in flight) race and must be serialised by the
+ * caller; consecutive calls on one stream are safe.
+ */
+#define MDX_NOALLOC_SCRATCH_BYTES (64LL << 20)
+mdx_status mdx_furir_device_noalloc(const mdx_params *p, int32_t natoms, const double *x, const double *y,
+                                    const double *z, const double *q, double *fx, double *fy, double *fz,
+                                    double *energy4, void *stream);
 /* Set that shape directly (reproducible runs, profiling). */
 mdx_status mdx_set_kspace_shape(mdx_engine *h, int32_t chunk, int32_t nslice);
 /* Device memory held by a handle, in bytes (admission by HBM budget). */

@@ -20,8 +20,10 @@ MDX_ERR_NKV = 3
 MDX_ERR_HIP = 4
 MDX_ERR_NOMEM = 5
 MDX_ERR_STATE = 6
+MDX_ERR_LIMIT = 7
 STATUS_NAMES = {0: "MDX_OK", 1: "MDX_ERR_ARG", 2: "MDX_ERR_LIST_OVERFLOW", 3: "MDX_ERR_NKV",
-                4: "MDX_ERR_HIP", 5: "MDX_ERR_NOMEM", 6: "MDX_ERR_STATE"}
+                4: "MDX_ERR_HIP", 5: "MDX_ERR_NOMEM", 6: "MDX_ERR_STATE", 7: "MDX_ERR_LIMIT"}
+MDX_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 MDX_LIST_REFERENCE = 0
 MDX_LIST_VERLET = 1

@@ -2127,6 +2127,49 @@ mdx_status mdx_furir_device(const mdx_params *p, int32_t natoms, const double *x
     return hipGetLastError() == hipSuccess ? MDX_OK : MDX_ERR_HIP;
 }
 
+}  // extern "C"
+
+// Scratch of mdx_furir_device_noalloc: one array per GPU in the code
+// object's device memory.  In host code `scratch` names a host-side stand-in,
+// not a device address: kernels get the address hipGetSymbolAddress returns
+// for the current device, looked up once per device on its first call.
+static __device__ __align__(256) unsigned char scratch[MDX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+mdx_status scratch_of_current_device(char **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return MDX_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MDX_ERR_HIP;
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = static_cast<char *>(g_scratch_ptr[dev]);
+    return MDX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+mdx_status mdx_furir_device_noalloc(const mdx_params *p, int32_t natoms, const double *x, const double *y,
+                                    const double *z, const double *q, double *fx, double *fy, double *fz,
+                                    double *energy4, void *stream) {
+    if (!fur_params_ok(p) || natoms < 0 || !energy4) return MDX_ERR_ARG;
+    if (natoms > 0 && (!x || !y || !z || !q || !fx || !fy || !fz)) return MDX_ERR_ARG;
+    const long long nkv = count_k(*p);
+    if (nkv > 4000000) return MDX_ERR_NKV;
+    if ((long long)fur_layout(natoms, (int)nkv).total > MDX_NOALLOC_SCRATCH_BYTES) return MDX_ERR_LIMIT;
+    char *ws = nullptr;
+    CHK(scratch_of_current_device(&ws));
+    fur_launch(*p, natoms, nkv, x, y, z, q, fx, fy, fz, energy4, ws, reinterpret_cast<hipStream_t>(stream));
+    return hipGetLastError() == hipSuccess ? MDX_OK : MDX_ERR_HIP;
+}
+
 mdx_status mdx_set_kspace_shape(mdx_engine *h, int32_t chunk, int32_t nslice) {
     if (!h || chunk <= 0 || nslice <= 0) return MDX_ERR_ARG;
     return set_kspace_shape(h, chunk, nslice);

@@ -82,6 +82,7 @@ def load_library() -> C.CDLL:
     L.mdx_furir_workspace_bytes.argtypes = [pv, C.c_int32, C.POINTER(C.c_int64)]
     # p, natoms, x, y, z, q, fx, fy, fz, energy4, [workspace, bytes,] stream
     L.mdx_furir_device.argtypes = [pv, C.c_int32] + [pv] * 8 + [pv, C.c_int64, pv]
+    L.mdx_furir_device_noalloc.argtypes = [pv, C.c_int32] + [pv] * 8 + [pv]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
                  "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_get_list", "mdx_compute",
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
@@ -89,7 +90,7 @@ def load_library() -> C.CDLL:
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
                  "mdx_batch_md_run", "mdx_autotune", "mdx_set_kspace_shape", "mdx_get_kvectors",
                  "mdx_enumerate_kvectors", "mdx_md_shake_failures", "mdx_furir_workspace_bytes",
-                 "mdx_furir_device"):
+                 "mdx_furir_device", "mdx_furir_device_noalloc"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -369,6 +370,18 @@ def furir_device(params: Dict, natoms: int, xyzq, forces, energy4: int, workspac
                             v(workspace), workspace_bytes, v(stream))
     if st != 0:
         raise MdxError(st, "mdx_furir_device")
+
+
+def furir_device_noalloc(params: Dict, natoms: int, xyzq, forces, energy4: int, stream: int = 0) -> None:
+    """mdx_furir_device on the library's static per-GPU scratch (no device
+    allocation); raises MdxError(MDX_ERR_LIMIT) beyond its size."""
+    L = load_library()
+    ps = abi.make_params(**params)
+    v = lambda p: C.c_void_p(p) if p else None
+    st = L.mdx_furir_device_noalloc(C.byref(ps), natoms, *[v(p) for p in xyzq], *[v(p) for p in forces],
+                                    v(energy4), v(stream))
+    if st != 0:
+        raise MdxError(st, "mdx_furir_device_noalloc")
 
 
 def device_info() -> Dict:
