@@ -5,8 +5,8 @@
 // k_strbnd   STRBND  forces.f:530-800 (fSPC water, one molecule per thread)
 // k_angles   GETANG  forces.f:216-309
 // k_tors     GETTRS  forces.f:323-513
-// All in FP64 with the reference's expressions; forces are scattered in
-// 64-bit fixed point (order independent), energies reduced per block in a
+// All in FP64 (the bonded forms derived in each body); forces are scattered
+// in 64-bit fixed point (order independent), energies reduced per block in a
 // fixed tree order.
 // k_combine  F = G + H (CM_ADDLA, scalar.f:82-116) and the molecular virial
 //            sums WIRS/WIRSS (scalar.f:72-76, forces.f:991-995)
@@ -80,38 +80,43 @@ __device__ inline void eterms_body(const EtermArgs &t, int blk) {
 }
 
 
+// Bond stretch of GETBND (forces.f:113-204): with d = r_j - r_i (minimum
+// image) and u = |d| - r0, E = K u^2, or the Morse well D (1 - e^{-rho u})^2
+// when the bond type is Morse and u <= r0 (the harmonic form beyond).  The
+// pair force follows from dE/dr: F_j = -dE/dr d/|d| = -F_i.  Virials: VIRB
+// gets d . F_j = -|d| dE/dr, the projections d_x F_jx etc.
 __device__ inline void bonds_body(const BondArgs &t, int blk) {
     const ScatterArgs &a = t.s;
     const int n = blk * blockDim.x + threadIdx.x;
     double v[5] = {0, 0, 0, 0, 0};
     if (n < t.n) {
         const int i = t.bi[n], j = t.bj[n], m = t.bpar[n];
-        const double req = t.r0[m];
-        double bx = a.x[j] - a.x[i], by = a.y[j] - a.y[i], bz = a.z[j] - a.z[i];
-        min_image(bx, by, bz, a.box);
-        double bsq = bx * bx + by * by;
-        bsq = bsq + bz * bz;
-        const double bd = sqrt(bsq);
-        const double r1i = 1.0 / bd;
-        const double bbb = bd - req;
-        double ebdd, forb;
-        if (!t.morse[m] || bbb > req) {
-            const double dbnd = bbb * t.k[m];
-            ebdd = dbnd * bbb;
-            forb = -2.0 * dbnd * r1i;
+        double d[3] = {a.x[j] - a.x[i], a.y[j] - a.y[i], a.z[j] - a.z[i]};
+        min_image(d[0], d[1], d[2], a.box);
+        double len2 = d[0] * d[0] + d[1] * d[1];
+        len2 = len2 + d[2] * d[2];
+        const double len = sqrt(len2);
+        const double u = len - t.r0[m];
+        double energy, slope;                  // E and dE/dr
+        if (t.morse[m] && u <= t.r0[m]) {
+            const double w = exp(-t.rho[m] * u);
+            energy = t.d[m] * (1.0 - w) * (1.0 - w);
+            slope = 2.0 * t.d[m] * t.rho[m] * w * (1.0 - w);
         } else {
-            const double exm = exp(-t.rho[m] * bbb);
-            const double exmp = 1.0 - exm;
-            ebdd = t.d[m] * (exmp * exmp);
-            forb = -2.0 * t.d[m] * t.rho[m] * exm * exmp * r1i;
+            energy = t.k[m] * u * u;
+            slope = 2.0 * t.k[m] * u;
         }
-        v[0] = ebdd;
-        v[1] = forb * bsq;
-        v[2] = forb * (bx * bx); v[3] = forb * (by * by); v[4] = forb * (bz * bz);
-        fx_add(a.fx + i, -bx * forb, a.fscale); fx_add(a.fy + i, -by * forb, a.fscale);
-        fx_add(a.fz + i, -bz * forb, a.fscale);
-        fx_add(a.fx + j, bx * forb, a.fscale); fx_add(a.fy + j, by * forb, a.fscale);
-        fx_add(a.fz + j, bz * forb, a.fscale);
+        const double g = slope / len;          // F_j = -g d, F_i = +g d
+        double fj[3];
+        for (int c = 0; c < 3; ++c) fj[c] = -g * d[c];
+        v[0] = energy;
+        v[1] = -slope * len;
+        for (int c = 0; c < 3; ++c) v[2 + c] = d[c] * fj[c];
+        unsigned long long *F[3] = {a.fx, a.fy, a.fz};
+        for (int c = 0; c < 3; ++c) {
+            fx_add(F[c] + i, -fj[c], a.fscale);
+            fx_add(F[c] + j, fj[c], a.fscale);
+        }
     }
     double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
@@ -188,45 +193,49 @@ __device__ inline void strbnd_body(const SpcArgs &t, int blk) {
 }
 
 
+// Bending of GETANG (forces.f:216-309): E = K (theta - theta0)^2 with the
+// angle theta at j between p = r_i - r_j and s = r_k - r_j (minimum images).
+// With c = cos theta = p.s / (|p| |s|), d c / d p = (s/|s| - c p/|p|) / |p|
+// and d theta = -d c / sin theta, so
+//   F_i = (dE/dtheta / sin theta) (s/|s| - c p/|p|) / |p|,
+//   F_k = (dE/dtheta / sin theta) (p/|p| - c s/|s|) / |s|,  F_j = -F_i - F_k.
+// c is clamped to [-1, 1] as the reference does; sin theta is sqrt(1 - c^2).
+// Virial projections: p_x F_ix + s_x F_kx (positions relative to j).
 __device__ inline void angles_body(const AngleArgs &t, int blk) {
     const ScatterArgs &a = t.s;
     const int n = blk * blockDim.x + threadIdx.x;
     double v[4] = {0, 0, 0, 0};
     if (n < t.n) {
         const int i = t.ai[n], j = t.aj[n], k = t.ak[n], m = t.apar[n];
-        const double fcn = t.k[m], aeq = t.theta0[m];
-        double ax = a.x[i] - a.x[j], ay = a.y[i] - a.y[j], az = a.z[i] - a.z[j];
-        min_image(ax, ay, az, a.box);
-        double asq = ax * ax + ay * ay; asq = asq + az * az;
-        const double asqi = 1.0 / asq;
-        double bx = a.x[k] - a.x[j], by = a.y[k] - a.y[j], bz = a.z[k] - a.z[j];
-        min_image(bx, by, bz, a.box);
-        double bsq = bx * bx + by * by; bsq = bsq + bz * bz;
-        const double bsqi = 1.0 / bsq;
-        const double ab = sqrt(asq * bsq);
-        const double abi = 1.0 / ab;
-        double cosa = ax * bx + ay * by; cosa = cosa + az * bz;
-        cosa = cosa * abi;
-        cosa = fmin(cosa, 1.0);
-        cosa = fmax(cosa, -1.0);
-        const double alf = acos(cosa);
-        const double sinai = 1.0 / sin(alf);
-        const double dalf = alf - aeq;
-        const double abc = dalf * fcn;
-        v[0] = abc * dalf;
-        const double sab = -2.0 * abc * sinai;
-        const double cab = sab * cosa;
-        const double fab = sab * abi;
-        const double faa = cab * asqi, fbb = cab * bsqi;
-        const double fax = fab * bx - faa * ax, fay = fab * by - faa * ay, faz = fab * bz - faa * az;
-        const double fbx = fab * ax - fbb * bx, fby = fab * ay - fbb * by, fbz = fab * az - fbb * bz;
-        fx_add(a.fx + i, -fax, a.fscale); fx_add(a.fy + i, -fay, a.fscale); fx_add(a.fz + i, -faz, a.fscale);
-        fx_add(a.fx + j, fax + fbx, a.fscale); fx_add(a.fy + j, fay + fby, a.fscale);
-        fx_add(a.fz + j, faz + fbz, a.fscale);
-        fx_add(a.fx + k, -fbx, a.fscale); fx_add(a.fy + k, -fby, a.fscale); fx_add(a.fz + k, -fbz, a.fscale);
-        v[1] = -ax * fax - bx * fbx;
-        v[2] = -ay * fay - by * fby;
-        v[3] = -az * faz - bz * fbz;
+        double p[3] = {a.x[i] - a.x[j], a.y[i] - a.y[j], a.z[i] - a.z[j]};
+        double s[3] = {a.x[k] - a.x[j], a.y[k] - a.y[j], a.z[k] - a.z[j]};
+        min_image(p[0], p[1], p[2], a.box);
+        min_image(s[0], s[1], s[2], a.box);
+        double pp = p[0] * p[0] + p[1] * p[1];
+        pp = pp + p[2] * p[2];
+        double ss = s[0] * s[0] + s[1] * s[1];
+        ss = ss + s[2] * s[2];
+        double ps = p[0] * s[0] + p[1] * s[1];
+        ps = ps + p[2] * s[2];
+        const double lp = sqrt(pp), ls = sqrt(ss);
+        const double c = fmax(-1.0, fmin(1.0, ps / (lp * ls)));
+        const double theta = acos(c);
+        const double dev = theta - t.theta0[m];
+        v[0] = t.k[m] * dev * dev;
+        const double h = 2.0 * t.k[m] * dev / sqrt(1.0 - c * c);   // dE/dtheta / sin theta
+        double fi[3], fk[3];
+        for (int q = 0; q < 3; ++q) {
+            const double ph = p[q] / lp, sh = s[q] / ls;
+            fi[q] = h * (sh - c * ph) / lp;
+            fk[q] = h * (ph - c * sh) / ls;
+        }
+        unsigned long long *F[3] = {a.fx, a.fy, a.fz};
+        for (int q = 0; q < 3; ++q) {
+            fx_add(F[q] + i, fi[q], a.fscale);
+            fx_add(F[q] + j, -(fi[q] + fk[q]), a.fscale);
+            fx_add(F[q] + k, fk[q], a.fscale);
+            v[1 + q] = p[q] * fi[q] + s[q] * fk[q];
+        }
     }
     double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
@@ -235,85 +244,90 @@ __device__ inline void angles_body(const AngleArgs &t, int blk) {
 }
 
 
+// Dihedral terms of GETTRS (forces.f:323-513).  Bond vectors b1 = r_j - r_i,
+// b2 = r_k - r_j, b3 = r_l - r_k (minimum images), plane normals m = b1 x b2,
+// nn = b2 x b3; the IUPAC angle (trans = 180 deg, sign of b1 . nn) is
+// phi = atan2(|b2| b1 . nn, m . nn).  Energies of phi by the type's form
+// (ITORS 0: K (1 + cos(M phi - delta)); 1: K1 (1 + cos phi) + K2 (1 -
+// cos 2 phi) + K3 (1 + cos 3 phi); 5: sum_n C_n cos^n(phi - delta),
+// n = 0..5; -1: K (phi - phi0)^2), and the gradient of phi after Blondel
+// and Karplus (J. Comput. Chem. 17, 1132 (1996)):
+//   d phi / d r_i = -|b2| / |m|^2 m,     d phi / d r_l = |b2| / |nn|^2 nn,
+// and with s1 = b1.b2 / |b2|^2, s3 = b3.b2 / |b2|^2 (projections of the
+// outer bonds on the middle one, in the directions of b1 and b3)
+//   d phi / d r_j = -(1 + s1) d phi/d r_i + s3 d phi/d r_l,
+//   d phi / d r_k = s1 d phi/d r_i - (1 + s3) d phi/d r_l
+// (the four sum to zero); F = -dE/dphi d phi / d r, and no 1 / sin phi
+// appears.  Nearly collinear
+// bonds (|m|^2 |nn|^2 <= 1e-10) contribute nothing, as in the reference.
+// Virial projections: sum over the atoms of (r - r_j) F.
 __device__ inline void tors_body(const TorsArgs &t, int blk) {
     const ScatterArgs &a = t.s;
     const int n = blk * blockDim.x + threadIdx.x;
     double v[4] = {0, 0, 0, 0};
     if (n < t.n) {
         const int i = t.ti[n], j = t.tj[n], k = t.tk[n], l = t.tl[n], m = t.tpar[n];
-        double ax = a.x[j] - a.x[i], ay = a.y[j] - a.y[i], az = a.z[j] - a.z[i];
-        double bx = a.x[k] - a.x[j], by = a.y[k] - a.y[j], bz = a.z[k] - a.z[j];
-        double cx = a.x[l] - a.x[k], cy = a.y[l] - a.y[k], cz = a.z[l] - a.z[k];
-        min_image(ax, ay, az, a.box); min_image(bx, by, bz, a.box); min_image(cx, cy, cz, a.box);
-        double ab = ax * bx + ay * by; ab = ab + az * bz;
-        double bc = bx * cx + by * cy; bc = bc + bz * cz;
-        double ac = ax * cx + ay * cy; ac = ac + az * cz;
-        double at = ax * ax + ay * ay; at = at + az * az;
-        double bt = bx * bx + by * by; bt = bt + bz * bz;
-        double ct = cx * cx + cy * cy; ct = ct + cz * cz;
-        double axb = (at * bt) - (ab * ab);
-        double bxc = (bt * ct) - (bc * bc);
-        const double fnum = (ab * bc) - (ac * bt);
-        double den = axb * bxc;
-        if (den > 1.0e-10) {
-            den = sqrt(den);
-            double co = fnum / den;
-            co = fmin(co, 1.0);
-            co = fmax(co, -1.0);
-            double signum = ax * (by * cz - cy * bz) + ay * (bz * cx - cz * bx);
-            signum = signum + az * (bx * cy - cx * by);
-            const double arg = fsign(acos(co), signum);
-            const double sin1f = sin(arg);
-            double si = sin1f;
-            if (fabs(si) < 1.0e-10) si = fsign(1.0e-10, si);
-            double pott = 0.0, deri = 0.0;
-            const int kd = t.kind[m];
-            if (kd == 0) {
-                const double earg = t.mult[m] * arg - t.delta[m];
-                pott = t.k[m] * (1.0 + cos(earg));
-                deri = -t.k[m] * t.mult[m] * sin(earg);
-            } else if (kd == 1) {
-                const double c1 = cos(arg);
-                const double c2 = 2.0 * (c1 * c1) - 1.0;
-                const double c3 = c1 * (2.0 * c2 - 1.0);
-                const double s2 = 2.0 * sin1f * c1;
-                const double s3 = sin1f * c2 + s2 * c1;
-                pott = t.k1[m] * (1.0 + c1) + t.k2[m] * (1.0 - c2) + t.k3[m] * (1.0 + c3);
-                deri = -t.k1[m] * sin1f + 2.0 * t.k2[m] * s2 - 3.0 * t.k3[m] * s3;
-            } else if (kd == 5) {
-                const double earg = arg - t.delta[m];
-                const double c1 = cos(earg), c2 = c1 * c1, c3 = c1 * c2, c4 = c1 * c3, c5 = c1 * c4;
-                pott = t.k[m] + t.k1[m] * c1 + t.k2[m] * c2 + t.k3[m] * c3 + t.k4[m] * c4 + t.k5[m] * c5;
-                deri = -sin(earg) * (t.k1[m] + 2.0 * t.k2[m] * c1 + 3.0 * t.k3[m] * c2 + 4.0 * t.k4[m] * c3 +
-                                     5.0 * t.k5[m] * c4);
-            } else if (kd == -1) {
-                const double earg = arg - t.delta[m];
-                pott = t.k[m] * (earg * earg);
-                deri = 2.0 * t.k[m] * earg;
+        double b1[3] = {a.x[j] - a.x[i], a.y[j] - a.y[i], a.z[j] - a.z[i]};
+        double b2[3] = {a.x[k] - a.x[j], a.y[k] - a.y[j], a.z[k] - a.z[j]};
+        double b3[3] = {a.x[l] - a.x[k], a.y[l] - a.y[k], a.z[l] - a.z[k]};
+        min_image(b1[0], b1[1], b1[2], a.box);
+        min_image(b2[0], b2[1], b2[2], a.box);
+        min_image(b3[0], b3[1], b3[2], a.box);
+        auto cross = [](const double *u, const double *w, double *o) {
+            o[0] = u[1] * w[2] - u[2] * w[1];
+            o[1] = u[2] * w[0] - u[0] * w[2];
+            o[2] = u[0] * w[1] - u[1] * w[0];
+        };
+        auto dot = [](const double *u, const double *w) { return u[0] * w[0] + u[1] * w[1] + u[2] * w[2]; };
+        double mv[3], nv[3];
+        cross(b1, b2, mv);
+        cross(b2, b3, nv);
+        const double mm = dot(mv, mv), nn = dot(nv, nv), bb = dot(b2, b2);
+        if (mm * nn > 1.0e-10) {
+            const double lb = sqrt(bb);
+            const double phi = atan2(lb * dot(b1, nv), dot(mv, nv));
+            double energy = 0.0, dedp = 0.0;   // E and dE/dphi
+            const int kind = t.kind[m];
+            if (kind == 0) {
+                const double w = t.mult[m] * phi - t.delta[m];
+                energy = t.k[m] * (1.0 + cos(w));
+                dedp = -t.k[m] * t.mult[m] * sin(w);
+            } else if (kind == 1) {
+                energy = t.k1[m] * (1.0 + cos(phi)) + t.k2[m] * (1.0 - cos(2.0 * phi)) + t.k3[m] * (1.0 + cos(3.0 * phi));
+                dedp = -t.k1[m] * sin(phi) + 2.0 * t.k2[m] * sin(2.0 * phi) - 3.0 * t.k3[m] * sin(3.0 * phi);
+            } else if (kind == 5) {
+                const double w = phi - t.delta[m], c = cos(w);
+                const double coef[6] = {t.k[m], t.k1[m], t.k2[m], t.k3[m], t.k4[m], t.k5[m]};
+                double e = 0.0, de = 0.0;          // Horner in c for E and dE/dc
+                for (int q = 5; q >= 0; --q) {
+                    de = de * c + e;
+                    e = e * c + coef[q];
+                }
+                energy = e;
+                dedp = -sin(w) * de;
+            } else if (kind == -1) {
+                const double w = phi - t.delta[m];
+                energy = t.k[m] * w * w;
+                dedp = 2.0 * t.k[m] * w;
             }
-            v[0] = pott;
-            const double de1 = deri / den / si;
-            axb = axb / den * co;
-            bxc = bxc / den * co;
-            double f[3][4];
-            const double A[3] = {ax, ay, az}, B[3] = {bx, by, bz}, Cc[3] = {cx, cy, cz};
-            for (int d = 0; d < 3; ++d) {
-                double dnum = Cc[d] * bt - B[d] * bc;
-                double dden = (ab * B[d] - A[d] * bt) * bxc;
-                f[d][0] = (dnum - dden) * de1;
-                dnum = ((B[d] - A[d]) * bc - ab * Cc[d]) + (2.0 * ac * B[d] - Cc[d] * bt);
-                dden = axb * (bc * Cc[d] - B[d] * ct) + (A[d] * bt - at * B[d] - ab * (B[d] - A[d])) * bxc;
-                f[d][1] = (dnum - dden) * de1;
-                dnum = ab * B[d] - A[d] * bt;
-                dden = axb * (bt * Cc[d] - bc * B[d]);
-                f[d][3] = (dnum - dden) * de1;
-                f[d][2] = -(f[d][0] + f[d][1] + f[d][3]);
+            v[0] = energy;
+            const double s1 = dot(b1, b2) / bb, s3 = dot(b3, b2) / bb;
+            double fi[3], fj[3], fk[3], fl[3];
+            for (int q = 0; q < 3; ++q) {
+                const double gi = -lb / mm * mv[q], gl = lb / nn * nv[q];
+                fi[q] = -dedp * gi;
+                fl[q] = -dedp * gl;
+                fj[q] = -dedp * (s3 * gl - (1.0 + s1) * gi);
+                fk[q] = -dedp * (s1 * gi - (1.0 + s3) * gl);
             }
             unsigned long long *F[3] = {a.fx, a.fy, a.fz};
-            const int at4[4] = {i, j, k, l};
-            for (int d = 0; d < 3; ++d)
-                for (int q = 0; q < 4; ++q) fx_add(F[d] + at4[q], f[d][q], a.fscale);
-            for (int d = 0; d < 3; ++d) v[1 + d] = -(A[d] + B[d]) * f[d][0] - B[d] * f[d][1] + Cc[d] * f[d][3];
+            for (int q = 0; q < 3; ++q) {
+                fx_add(F[q] + i, fi[q], a.fscale);
+                fx_add(F[q] + j, fj[q], a.fscale);
+                fx_add(F[q] + k, fk[q], a.fscale);
+                fx_add(F[q] + l, fl[q], a.fscale);
+                v[1 + q] = -b1[q] * fi[q] + b2[q] * fk[q] + (b2[q] + b3[q]) * fl[q];
+            }
         }
     }
     double *o = a.epart + (size_t)blk * Q_N;

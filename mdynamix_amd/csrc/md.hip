@@ -88,23 +88,23 @@ __device__ inline void md_shake_regs(const MdArgs &a, int c0, int c1, int a0) {
             const int i = ci[c], j = cj[c];
             const unsigned bi = 1u << i, bj = 1u << j;
             if (!((moved & bi) || (moved & bj))) continue;
-            const double bndsq = cl[c];
+            const double target2 = cl[c];
             double bx = x[i] - x[j], by = y[i] - y[j], bz = z[i] - z[j];
             min_image(bx, by, bz, a.box);
-            double bsq = bx * bx + by * by;
-            bsq = bsq + bz * bz;
-            const double diffsq = bndsq - bsq;
-            if (fabs(diffsq) > bndsq * a.tol2) {
+            double cur2 = bx * bx + by * by;
+            cur2 = cur2 + bz * bz;
+            const double gap2 = target2 - cur2;
+            if (fabs(gap2) > target2 * a.tol2) {
                 double dx = ox[i] - ox[j], dy = oy[i] - oy[j], dz = oz[i] - oz[j];
                 min_image(dx, dy, dz, a.box);
-                double vdot = bx * dx + by * dy;
-                vdot = vdot + bz * dz;
-                if (vdot < bndsq * a.bdtol) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
-                const double rma = mi[i], rmb = mi[j];
-                const double gab = diffsq / (2.0 * (rma + rmb) * vdot);
-                dx = dx * gab; dy = dy * gab; dz = dz * gab;
-                x[i] = x[i] + rma * dx; y[i] = y[i] + rma * dy; z[i] = z[i] + rma * dz;
-                x[j] = x[j] - rmb * dx; y[j] = y[j] - rmb * dy; z[j] = z[j] - rmb * dz;
+                double proj = bx * dx + by * dy;
+                proj = proj + bz * dz;
+                if (proj < target2 * a.bdtol) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
+                const double wi = mi[i], wj = mi[j];
+                const double lam = gap2 / (2.0 * (wi + wj) * proj);
+                dx = dx * lam; dy = dy * lam; dz = dz * lam;
+                x[i] = x[i] + wi * dx; y[i] = y[i] + wi * dy; z[i] = z[i] + wi * dz;
+                x[j] = x[j] - wj * dx; y[j] = y[j] - wj * dy; z[j] = z[j] - wj * dz;
                 moving |= bi | bj;
                 done = false;
             }
@@ -152,23 +152,23 @@ __device__ inline void md_shake_body(const MdArgs &a, int blk) {
             const int i = a.cons[c].x, j = a.cons[c].y;
             const unsigned long long bi = 1ull << (i - a0), bj = 1ull << (j - a0);
             if (!((moved & bi) || (moved & bj))) continue;
-            const double bndsq = a.cons_len2[c];
+            const double target2 = a.cons_len2[c];
             double bx = a.x[i] - a.x[j], by = a.y[i] - a.y[j], bz = a.z[i] - a.z[j];
             min_image(bx, by, bz, a.box);
-            double bsq = bx * bx + by * by;
-            bsq = bsq + bz * bz;
-            const double diffsq = bndsq - bsq;
-            if (fabs(diffsq) > bndsq * a.tol2) {
+            double cur2 = bx * bx + by * by;
+            cur2 = cur2 + bz * bz;
+            const double gap2 = target2 - cur2;
+            if (fabs(gap2) > target2 * a.tol2) {
                 double dx = a.ox[i] - a.ox[j], dy = a.oy[i] - a.oy[j], dz = a.oz[i] - a.oz[j];
                 min_image(dx, dy, dz, a.box);
-                double vdot = bx * dx + by * dy;
-                vdot = vdot + bz * dz;
-                if (vdot < bndsq * a.bdtol) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
-                const double rma = a.minv[i], rmb = a.minv[j];
-                const double gab = diffsq / (2.0 * (rma + rmb) * vdot);
-                dx = dx * gab; dy = dy * gab; dz = dz * gab;
-                a.x[i] = a.x[i] + rma * dx; a.y[i] = a.y[i] + rma * dy; a.z[i] = a.z[i] + rma * dz;
-                a.x[j] = a.x[j] - rmb * dx; a.y[j] = a.y[j] - rmb * dy; a.z[j] = a.z[j] - rmb * dz;
+                double proj = bx * dx + by * dy;
+                proj = proj + bz * dz;
+                if (proj < target2 * a.bdtol) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
+                const double wi = a.minv[i], wj = a.minv[j];
+                const double lam = gap2 / (2.0 * (wi + wj) * proj);
+                dx = dx * lam; dy = dy * lam; dz = dz * lam;
+                a.x[i] = a.x[i] + wi * dx; a.y[i] = a.y[i] + wi * dy; a.z[i] = a.z[i] + wi * dz;
+                a.x[j] = a.x[j] - wj * dx; a.y[j] = a.y[j] - wj * dy; a.z[j] = a.z[j] - wj * dz;
                 moving |= bi | bj;
                 done = false;
             }

@@ -6,6 +6,7 @@ FP32 with FP64 accumulation; force RMS relative error <= 1e-5 against FP64,
 energies <= 1e-6 relative; neighbour lists bit-exact (same pairs, order
 and 1-4 signs after export in the reference layout).
 """
+import dataclasses
 import hashlib
 
 import numpy as np
@@ -225,6 +226,9 @@ def test_all_bonded_kinds_vs_oracle():
     centres = systems.lattice_sites(27, (24.0, 24.0, 24.0), rng)
     s = systems.assemble("kinds", [mt], [27], centres, rng, box=(24.0, 24.0, 24.0), rcut=9.0, rshort=4.5,
                          ewald_a=2.8, ewald_b=9.0, shake=False)
+    # off the built geometry's exact trans dihedrals (sin phi = 0, where the
+    # reference's force formula degenerates; see the test below)
+    s = dataclasses.replace(s, pos=s.pos + rng.normal(0.0, 0.03, s.pos.shape))
     topo, prm = topology.from_system(s)
     assert len(topo.tp_kind) >= 3 and len(topo.bp_morse) >= 3
     topo.bp_morse[:] = 0
@@ -242,3 +246,62 @@ def test_all_bonded_kinds_vs_oracle():
         check_energies(e, eo)
         if group != abi.MDX_FORCES_SLOW:
             assert abs(e["pint"] - eo["pint"]) <= 1e-9 * max(abs(eo["pint"]), 1.0)
+
+
+@pytest.mark.parametrize("kind", [0, 5])
+def test_dihedral_at_exact_trans_is_the_energy_gradient(kind):
+    """A 4-site chain at exactly 180 degrees (sin phi = 0) with a periodic
+    dihedral form whose dE/dphi is not zero there (cosine with delta = 15
+    deg, the cos^n series with delta = 30 deg; the improper form K (phi -
+    phi0)^2 has its branch cut at 180 degrees).  The
+    reference's GETTRS force divides d(cos phi)/dr, which vanishes at 180
+    degrees, by sin phi clamped to 1e-10 (forces.f:383-388) and loses the
+    torque there; the engine's gradient of phi has no 1/sin phi, so its
+    force is the energy gradient.  The chain lies in the z = 0 plane with
+    its bonds and angles at their equilibria: the torque is along z and
+    checked against central differences of the oracle's bonded energy (no
+    charges, no Lennard-Jones, a single molecule) to 1e-6 relative, and the
+    in-plane forces vanish by symmetry (in-plane differences of the
+    oracle's energy are not used: its phi comes from acos near -1, whose
+    rounding moves phi by ~1e-8 rad under in-plane steps)."""
+    sites = [systems.Site("C%d" % (i + 1), (0.0, 0.0, 0.0), 14.0, 0.0, 0.0, 0.0) for i in range(4)]
+    bonds = [(0, i + 1, i + 2, 1.53, 1200.0, 0.0, 0.0) for i in range(3)]
+    angles = [(i + 1, i + 2, i + 3, 112.0, 250.0) for i in range(2)]
+    delta = {0: 15.0, 5: 30.0}[kind]
+    tors = [(1, 2, 3, 4, delta, 2.0, 3)]
+    mt = systems.MolType("C4T", sites, bonds, angles, tors, c14el=0.0, c14lj=0.0)
+    th = np.radians(112.0)
+    pos = np.array([[0.0, 0.0, 0.0], [1.53, 0.0, 0.0],
+                    [1.53 - 1.53 * np.cos(th), 1.53 * np.sin(th), 0.0],
+                    [3.06 - 1.53 * np.cos(th), 1.53 * np.sin(th), 0.0]])
+    pos[3] = pos[2] + (pos[1] - pos[0])          # anti (trans) in the plane
+    pos += 10.0
+    s = systems.System("trans", [mt], [1], (20.0, 20.0, 20.0), pos, rcut=8.0, rshort=4.0, ewald_a=2.8,
+                       ewald_b=9.0, shake=False)
+    topo, prm = topology.from_system(s)
+    topo.tp_kind[0] = kind
+    if kind == 5:
+        topo.tp_k1[0], topo.tp_k2[0], topo.tp_k3[0], topo.tp_k4[0], topo.tp_k5[0] = 3.0, -2.0, 1.5, -0.5, 0.25
+    eng = ForceEngine(topo, prm)
+    try:
+        eng.set_positions(pos)
+        eng.chcnb()
+        f, _, _, e = eng.fforces()
+    finally:
+        eng.close()
+    ts, ps = topo.struct(), topology.params_struct(dict(prm, list_mode=0, skin=0.0))
+
+    def energy(p):
+        return orc.compute(topo, ts, ps, p, orc.chcnb(ts, ps, p), abi.MDX_FORCES_FAST)[3]["pint"]
+    h = 1e-5
+    gz = np.zeros(4)
+    for a in range(4):
+        pp, pm = pos.copy(), pos.copy()
+        pp[a, 2] += h
+        pm[a, 2] -= h
+        gz[a] = (energy(pp) - energy(pm)) / (2 * h)
+    scale = np.abs(gz).max()
+    print("case trans kind", kind, "max |F_z + dE/dz|", np.abs(f[:, 2] + gz).max(), "of", scale, flush=True)
+    assert scale > 1e-3
+    assert np.abs(f[:, 2] + gz).max() <= 1e-6 * scale
+    assert np.abs(f[:, :2]).max() <= 1e-6 * scale
