@@ -112,6 +112,10 @@ struct mdx_engine {
     int cand_cap = 0, list_cap = 0;
     int *d_cand = nullptr, *d_ncand = nullptr, *d_nlist = nullptr, *d_nshort = nullptr, *d_overflow = nullptr;
     PairEntry *d_list = nullptr;
+    // j-reaction records per list slot and the transposed list (k_jgather)
+    float *d_rec = nullptr;
+    int2 *d_jcnt = nullptr;
+    unsigned *d_jstart = nullptr, *d_jdest = nullptr;
     float4 *d_xq = nullptr;
     int2 *d_aux = nullptr;
     bool have_list = false;
@@ -248,6 +252,8 @@ struct mdx_batch {
     } while (0)
 
 namespace {
+
+mdx_status alloc_list(mdx_engine *h);
 
 template <typename T>
 mdx_status dalloc(mdx_engine *h, T **p, size_t count) {
@@ -632,13 +638,30 @@ mdx_status alloc_clusters(mdx_engine *h) {
     h->cand_cap = ((int)est + 63) / 64 * 64;
     h->list_cap = h->cand_cap;
     CHK(dalloc(h, &h->d_cand, (size_t)h->nclu_cap * h->cand_cap));
-    CHK(dalloc(h, &h->d_list, (size_t)h->nclu_cap * h->list_cap));
+    CHK(alloc_list(h));
+    CHK(dalloc(h, &h->d_jcnt, h->nclu_cap));
+    CHK(dalloc(h, &h->d_jstart, h->nclu_cap));
     // per-call buffers
     CHK(dalloc(h, &h->d_fs, (size_t)3 * h->n));
     CHK(dalloc(h, &h->d_fl, (size_t)3 * h->n));
     CHK(dalloc(h, &h->d_fix, (size_t)9 * h->n));
     CHK(dalloc(h, &h->d_fixj, (size_t)h->nclu_cap * kClu * 8));
     CHK(dalloc(h, &h->d_out, (size_t)9 * h->n));
+    return MDX_OK;
+}
+
+
+// Pair list storage at the current capacity: entries, their j-reaction
+// records and the transposed list (list slots are 31-bit indices).
+mdx_status alloc_list(mdx_engine *h) {
+    const size_t slots = (size_t)h->nclu_cap * h->list_cap;
+    if (slots >= (size_t)1 << 31) {
+        h->err = "pair list above 2^31 entries";
+        return MDX_ERR_LIMIT;
+    }
+    CHK(dalloc(h, &h->d_list, slots));
+    CHK(dalloc(h, &h->d_rec, slots * 2 * kRecUnit));
+    CHK(dalloc(h, &h->d_jdest, slots));
     return MDX_OK;
 }
 
@@ -753,6 +776,8 @@ void fill_args(mdx_engine *h) {
         // the band is eight times that
         const double lmax = std::max(h->p.box[0], std::max(h->p.box[1], h->p.box[2]));
         pa.band = verlet ? (float)(1e-6 * h->p.rcut * (lmax + 3.0 * h->p.rcut)) : 0.f;
+        pa.rlo = pa.band > 0.f ? std::nextafter((float)(rc2 - pa.band), 0.f) : pa.rc2;
+        pa.rhi = pa.band > 0.f ? std::nextafter((float)(rc2 + pa.band), INFINITY) : pa.rc2;
         pa.rc2d = rc2;
         pa.x = h->d_x; pa.y = h->d_y; pa.z = h->d_z;
         pa.box = h->box;
@@ -765,6 +790,10 @@ void fill_args(mdx_engine *h) {
     }
     pa.fs_x = h->d_fs; pa.fs_y = h->d_fs + n; pa.fs_z = h->d_fs + 2 * n;
     pa.fl_x = h->d_fl; pa.fl_y = h->d_fl + n; pa.fl_z = h->d_fl + 2 * n;
+    pa.rec = h->d_rec;
+    pa.jstart = h->d_jstart;
+    pa.jcnt = h->d_jcnt;
+    pa.jdest = h->d_jdest;
     pa.jf = h->d_fixj;
     pa.fscale = h->fscale;
     pa.epart = erow(h, h->row_pairs);
@@ -977,7 +1006,7 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
     if (t->atom_mass) mass.assign(t->atom_mass, t->atom_mass + n);
     if ((s = upload(h, &h->d_mass, mass.data(), n))) return fail(s);
     std::vector<float2> lj((size_t)h->nlj * h->nlj);
-    for (size_t k = 0; k < lj.size(); ++k) lj[k] = make_float2((float)t->a6[k], (float)t->b12[k]);
+    for (size_t k = 0; k < lj.size(); ++k) lj[k] = make_float2((float)(6.0 * t->a6[k]), (float)(12.0 * t->b12[k]));
     if ((s = upload(h, &h->d_lj, lj.data(), lj.size()))) return fail(s);
     // molecules are contiguous atom ranges
     std::vector<int> mstart(h->nmol + 1, 0);
@@ -1282,12 +1311,19 @@ mdx_status mdx_build_list(mdx_engine *h) {
         h->cand_cap = std::max(h->cand_cap, cap);
         h->list_cap = h->cand_cap;
         CHK(dalloc(h, &h->d_cand, (size_t)h->nclu_cap * h->cand_cap));
-        CHK(dalloc(h, &h->d_list, (size_t)h->nclu_cap * h->list_cap));
+        CHK(alloc_list(h));
         if (attempt == 3) {
             h->err = "pair list overflow";
             return MDX_ERR_LIST_OVERFLOW;
         }
     }
+    // transposed list for the j-reaction sums
+    k_fill_i32<<<cdiv(2 * h->nclu, 256), 256, 0, st>>>(2 * h->nclu, (int *)h->d_jcnt, 0);
+    k_jcount<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_list, h->d_nlist, h->d_nshort, h->list_cap, h->d_jcnt,
+                                     h->d_jdest);
+    k_scan_excl<<<1, 1024, 0, st>>>(h->nclu, h->d_jcnt, h->d_jstart);
+    k_jplace<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_list, h->d_nlist, h->d_nshort, h->list_cap, h->d_jstart,
+                                     h->d_jcnt, h->d_jdest);
     k_slot_info<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, h->d_ljtype, h->d_moltype, h->d_type_ee,
                                                    h->d_q, h->d_aux);
     if (h->nspecial > 0)
@@ -1486,8 +1522,6 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         const long long nz1 = (long long)h->rows_total * Q_N;
         b->zero.add(ZeroSeg{(unsigned long long *)h->d_epart, nz1}, cdiv(nz1, 256));
         b->zero.add(ZeroSeg{h->d_fix, 9LL * n}, cdiv(9LL * n, 256));
-        const long long nzj = (long long)h->nclu_cap * kClu * 8;
-        b->zero.add(ZeroSeg{h->d_fixj, nzj}, cdiv(nzj, 256));
         b->zero.add(ZeroSeg{h->d_fixsp, 3LL * n}, cdiv(3LL * n, 256));
         b->pack.add(h->a_pack, cdiv((long long)h->nclu_cap * kClu, 256));
         b->com.add(h->a_pack, h->ncomgrp);
@@ -1593,6 +1627,7 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
             if (b->soft) k_pairs<true, false><<<nb, 64, lj, st>>>(pa, pm, group);
             else k_pairs<false, false><<<nb, 64, lj, st>>>(pa, pm, group);
         }
+        k_jgather<<<nb, 64, 0, st>>>(pa, pm, group);
     }
     mark(2);
     if (b->special.nblocks()) {

@@ -242,6 +242,74 @@ __global__ __launch_bounds__(64) void k_masks(
     }
 }
 
+// Transposed list: the entries that name cluster c as their j-cluster get
+// consecutive records from jstart[c] (in units of 24 floats), so the pair
+// kernel writes each entry's j-reaction record where k_jgather streams it
+// for the j atoms, without atomics.  A segment holds the records of the
+// front-segment entries first (both classes, 2 units each, jcnt.x of them),
+// then those of the long-class entries (1 unit each, jcnt.y).  k_jcount
+// gives each entry its rank in its part (integer atomics, at rebuilds
+// only), k_scan_excl turns the unit counts into segment starts, and
+// k_jplace turns each entry's rank into its record position (jdest, per
+// list slot).  The order inside each part follows the atomics; the sums
+// over it are integer sums, so results do not depend on that order.
+__global__ __launch_bounds__(64) void k_jcount(int nclu, const PairEntry *__restrict__ list,
+                                               const int *__restrict__ nlist, const int *__restrict__ nshort,
+                                               int list_cap, int2 *__restrict__ jcnt, unsigned *__restrict__ jrank) {
+    const int ci = blockIdx.x;
+    if (ci >= nclu) return;
+    const int n = nlist[ci], ns = nshort[ci];
+    for (int k = threadIdx.x; k < n; k += 64) {
+        const size_t p = (size_t)ci * list_cap + (k < ns ? k : list_cap - 1 - (k - ns));
+        const int cj = (int)((unsigned)list[p].cj & ~kEntryWrap);
+        jrank[p] = (unsigned)atomicAdd(k < ns ? &jcnt[cj].x : &jcnt[cj].y, 1);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_jplace(int nclu, const PairEntry *__restrict__ list,
+                                               const int *__restrict__ nlist, const int *__restrict__ nshort,
+                                               int list_cap, const unsigned *__restrict__ jstart,
+                                               const int2 *__restrict__ jcnt, unsigned *__restrict__ jdest) {
+    const int ci = blockIdx.x;
+    if (ci >= nclu) return;
+    const int n = nlist[ci], ns = nshort[ci];
+    for (int k = threadIdx.x; k < n; k += 64) {
+        const size_t p = (size_t)ci * list_cap + (k < ns ? k : list_cap - 1 - (k - ns));
+        const int cj = (int)((unsigned)list[p].cj & ~kEntryWrap);
+        const unsigned r = jdest[p];
+        jdest[p] = jstart[cj] + (k < ns ? 2u * r : 2u * (unsigned)jcnt[cj].x + r);
+    }
+}
+
+// Exclusive prefix sum of the record units 2 x + y of n clusters, one block
+// of 1,024 threads: thread t
+// sums a contiguous chunk, the chunk sums are scanned across the block
+// (wavefront scans, then the 16 wavefront totals), and each thread writes
+// its chunk's prefixes.
+__global__ __launch_bounds__(1024) void k_scan_excl(int n, const int2 *__restrict__ in, unsigned *__restrict__ out) {
+    __shared__ unsigned wtot[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (n + 1023) / 1024;
+    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
+    unsigned sum = 0;
+    for (int i = i0; i < i1; ++i) sum += 2u * (unsigned)in[i].x + (unsigned)in[i].y;
+    unsigned inc = sum;   // inclusive scan inside the wavefront
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned v = __shfl_up(inc, off);
+        if (lane >= off) inc += v;
+    }
+    if (lane == 63) wtot[w] = inc;
+    __syncthreads();
+    unsigned base = 0;
+    for (int k = 0; k < w; ++k) base += wtot[k];
+    unsigned run = base + inc - sum;
+    for (int i = i0; i < i1; ++i) {
+        const unsigned v = 2u * (unsigned)in[i].x + (unsigned)in[i].y;
+        out[i] = run;
+        run += v;
+    }
+}
+
 // Per-slot FP64 coordinates (the caller's, as the reference differences
 // them) and molecule index (-1: empty slot) for the mask pass.
 __global__ void k_slot_pos(int nslot, const int *__restrict__ clu_atom, const double *__restrict__ x,

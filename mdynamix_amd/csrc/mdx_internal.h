@@ -40,6 +40,7 @@ constexpr double kA4 = -1.453152027, kA5 = 1.061405429, kB1 = 0.3275911;
 constexpr double kPI = 3.1415926536, kTOTPI = 1.128379169;
 
 constexpr int kClu = 8;            // atoms per cluster
+constexpr int kRecUnit = 24;       // floats per unit of the j-reaction records
 // Per-type energy breakdown (MDX_WANT_DETAIL): per partial row
 // [POTE1(ntp) | POTL1(ntp) | PES141(nmt) | PSR141(nmt)]
 constexpr int kMaxDet = 2 * 137 + 2 * 16;
@@ -199,7 +200,7 @@ struct PairArgs {
     const double *q;             // per atom charge (FP64)
     float coulf_f;               // coulomb factor (FP32, forces)
     double coulf;                // coulomb factor (FP64, energies)
-    const float2 *lj;            // nlj x nlj: (A6, B12)
+    const float2 *lj;            // nlj x nlj: (6 A6, 12 B12)
     int nlj;
     const int *clu_atom;
     float alpha;
@@ -210,6 +211,7 @@ struct PairArgs {
     // so the set of evaluated pairs is that of a list built at the current
     // positions; band = 0 for reference lists (no cutoff test)
     float band;
+    float rlo, rhi;              // rc2 - band, rc2 + band, rounded outwards
     double rc2d;
     const double *x, *y, *z;     // caller coordinates (FP64)
     Box box;
@@ -218,10 +220,17 @@ struct PairArgs {
     float lx, ly, lz, ilx, ily, ilz;
     double *fs_x, *fs_y, *fs_z;  // short-class pair forces (per atom)
     double *fl_x, *fl_y, *fl_z;  // long-class pair forces (per atom)
-    // j-atom reactions of the half list, 64-bit fixed point (scale fscale),
-    // per cluster slot 8 words: short x, y, z (fast forces H), long x, y, z
-    // (slow forces G), 2 unused, so one list entry's 48 values are 512
-    // contiguous bytes
+    // j-atom reactions of the half list: k_pairs writes one record per list
+    // entry at unit jdest[list slot] of rec (a unit is 24 floats: a
+    // long-class entry's record is one unit, a front-segment entry's two);
+    // the records naming cluster c as j-cluster are consecutive from unit
+    // jstart[c]: jcnt[c].x front-segment records, then jcnt[c].y long-class
+    // ones; k_jgather sums them into 64-bit fixed point (scale fscale), per
+    // cluster slot 8 words: short x, y, z (fast forces H), long x, y, z (slow
+    // forces G), 2 unused
+    float *rec;
+    const unsigned *jstart, *jdest;
+    const int2 *jcnt;
     unsigned long long *jf;
     double fscale;
     double *epart;               // [nclu_cap][Q_N]

@@ -105,10 +105,46 @@ __device__ inline float swap_sum16(float x, float y) {
     const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
     return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
-// x(lane ^ 8): a rotation by 8 inside each row of 16 (DPP row_ror:8).
-__device__ inline float xor8(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x128, 0xf, 0xf, false));
+// a * b rounded on its own: never contracted into a later add
+__device__ inline float mul_unfused(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
 }
+
+// x + x(lane ^ 8): the two halves of each row of 16 lanes added (DPP
+// row_ror:8 folded into the add).
+__device__ inline float add_xor8(float x) {
+    return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x128, 0xf, 0xf, false));
+}
+
+// j reactions of the half list leave the pair kernel as a record per list
+// entry at the entry's record position (PairArgs::rec, jdest): plain
+// stores, marked non-temporal (streamed to memory, not kept in the caches,
+// where the next call's stores to the same lines would find them), no
+// atomics; k_jgather sums them per j atom.  The butterflies below leave
+// the 8 j slots' sums in lanes 16 row + j_slot of each row of 16 lanes,
+// written to record floats 8 (row) + j_slot, and 32 + 8 (row / 2) + j_slot
+// for the second value of a mixed entry:
+//   long-class entry:  rows 0, 1, 2 = long x, z, y             (floats 0-23)
+//   mixed entry:       first value rows 0-3 = long x, z, y, short x;
+//                      second value rows 0, 2 = short y, z     (floats 0-47)
+constexpr unsigned long long kPatLong = 0x000000ff00ff00ffull;
+constexpr unsigned long long kPatMix0 = 0x00ff00ff00ff00ffull;
+constexpr unsigned long long kPatMix1 = 0x000000ff000000ffull;
+constexpr double kMagic = 6755399441055744.0;              // 1.5 * 2^52
+constexpr int kPairBlock = 16;      // list entries per FP32 partial sum
+
+// FP32 partial sums of one block of list entries (per lane).
+template <bool MIXED>
+struct PairAcc {
+    float fl[3] = {0.f, 0.f, 0.f}, vl[3] = {0.f, 0.f, 0.f};
+    float el = 0.f, el_lo = 0.f, lj = 0.f, vr = 0.f;          // long class
+    // long-class entries: 12 E_LJ as the sums of 6 A6 / r^6 and of r F_LJ
+    // (= vr except for soft-core pairs, which keep their own w6 sum)
+    float a6 = 0.f, w6 = 0.f;
+    float fs[MIXED ? 3 : 1] = {}, vs[MIXED ? 3 : 1] = {};
+    float es = 0.f, ljs = 0.f, vrs = 0.f;                      // short class
+};
 
 // Detail calls (MDX_WANT_DETAIL) also split the bare Coulomb energy
 // q_i q_j C / (r + RAE) and the LJ energy by molecule-type pair (POTE1,
@@ -117,8 +153,7 @@ __device__ inline float xor8(float x) {
 // keeps its own FP64 buckets in LDS (bucket = type of the j molecule, or
 // the intramolecular bucket), so the sums stay in a fixed order.
 template <bool SOFT, bool DETAIL>
-__device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float *s_jf, int *s_jc, double *s_acc,
-                                  int group) {
+__device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, double *s_acc, int group) {
     const int lane = threadIdx.x;
     const int nlj2 = a.nlj * a.nlj;
     for (int t = lane; t < nlj2; t += 64) s_lj[t] = ldv(gld(a.lj) + t);
@@ -135,16 +170,15 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
     if (ci >= a.nclu) return;
 
     const int il = lane >> 3;
+    const int jl = lane & 7;
     const float4 pi = ldv(gld(a.xq) + ci * kClu + il);
     const int infoi = gld(a.aux)[ci * kClu + il].x;
-    const int ti = infoi & 0xffff;
-    const float2 *ljrow = s_lj + ti * a.nlj;
-    const int jl = lane & 7;
+    const float2 *ljrow = s_lj + (infoi & 0xffff) * a.nlj;   // (6 A6, 12 B12) against type j
     const float A1 = (float)kA1, A2 = (float)kA2, A3 = (float)kA3, A4 = (float)kA4, A5 = (float)kA5;
-    const float B1 = (float)kB1, TOT = (float)kTOTPI;
     const float alpha = a.alpha;
+    const float b1a = (float)kB1 * alpha;
     const float nal2 = -alpha * alpha * 1.44269504088896341f;   // -alpha^2 log2(e)
-    const float talpha = TOT * alpha;
+    const float talpha = (float)kTOTPI * alpha;
     const float qic = pi.w * a.coulf_f;
     int moli = -1;
     if (DETAIL) {
@@ -156,22 +190,36 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
     double Fs[3] = {0, 0, 0}, Fl[3] = {0, 0, 0};
     double E[6] = {0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
     // the FP64 virial sums per lane live in LDS (touched once per block of
-    // 16 entries; registers stay free for the pair loop): short xyz, long xyz
+    // entries; registers stay free for the pair loop): short xyz, long xyz
     for (int q = 0; q < 6; ++q) s_acc[q * 64 + lane] = 0.0;
     int cnt_s = 0, cnt_l = 0;   // wave totals (ballot counts)
-    // a fast-class call reads only the entries with a short-class pair (the
-    // front segment); logical entry k >= ns sits at list_cap - 1 - (k - ns)
     const int ns = sld(a.nshort)[ci];
-    const int n = group == 1 ? ns : sld(a.nlist)[ci];
-    const int cap = a.list_cap;
-    auto ent = [ns, cap](int k) { return k < ns ? k : cap - 1 - (k - ns); };
+    const int n = sld(a.nlist)[ci];
     const MDX_CONST PairEntry *L = sld(a.list) + (size_t)ci * a.list_cap;
-    const MDX_GLOBAL float4 *xq = gld(a.xq);
-    const MDX_GLOBAL int2 *aux = gld(a.aux);
-    MDX_GLOBAL unsigned long long *const jf_base = gst(a.jf) + (size_t)jl * 8 + il;
-    const double fscale = a.fscale;
     const float bl[3] = {a.lx, a.ly, a.lz}, bi[3] = {a.ilx, a.ily, a.ilz};
-    const float rc2 = a.rc2, raex = a.raex, band = a.band;
+    const float rlo = a.rlo, rhi = a.rhi, raex = a.raex;
+    // j-cluster data through buffer descriptors: lane offset in a VGPR,
+    // cluster offset in an SGPR
+    const __amdgpu_buffer_rsrc_t rs_xq =
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.xq, (short)0, a.nclu * kClu * 16, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs_aux =
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.aux, (short)0, a.nclu * kClu * 8, 0x00020000);
+    const int voff_xq = jl * 16, voff_aux = jl * 8;
+    auto ld_xq = [&](int cj) {
+        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs_xq, voff_xq, cj * (kClu * 16), 0));
+    };
+    auto ld_aux = [&](int cj) {
+        return __builtin_bit_cast(int2, __builtin_amdgcn_raw_buffer_load_b64(rs_aux, voff_aux, cj * (kClu * 8), 0));
+    };
+    auto jclu = [](const PairEntry &e) { return (int)((unsigned)e.cj & ~kEntryWrap); };
+
+    // j-reaction records: record base address in SGPRs, lane float in a VGPR
+    const int fr0 = (lane >> 4) * 8 + jl, fr1 = 32 + (lane >> 5) * 8 + jl;
+    const MDX_CONST unsigned *jdest = sld(a.jdest) + (size_t)ci * a.list_cap;
+    auto put = [&](float v, unsigned long long pat, int f, unsigned dest) {
+        if (__builtin_amdgcn_inverse_ballot_w64(pat))
+            __builtin_nontemporal_store(v, gst(a.rec + (size_t)dest * kRecUnit) + f);
+    };
     // FP64 cutoff test of this lane's pair with the reference arithmetic
     auto exact_in = [&](int cj) {
 #pragma clang fp contract(off)
@@ -183,149 +231,154 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
         rr2 = rr2 + dz * dz;
         return rr2 <= a.rc2d;
     };
-    // software pipeline: list entries two ahead (scalar loads), the
-    // j-cluster coordinates one ahead
-    PairEntry e1{}, e2{};
-    float4 pj1 = make_float4(0.f, 0.f, 0.f, 0.f);
-    int2 aux1 = make_int2(0, 0);
-    if (n > 0) e1 = ldv(L + ent(0));
-    if (n > 1) e2 = ldv(L + ent(1));
-    auto jclu = [](const PairEntry &e) { return (int)((unsigned)e.cj & ~kEntryWrap); };
-    if (n > 0) { pj1 = ldv(xq + jclu(e1) * kClu + jl); aux1 = ldv(aux + jclu(e1) * kClu + jl); }
-    for (int k0 = 0; k0 < n; k0 += 16) {
-        float fs[3] = {0.f, 0.f, 0.f}, fl[3] = {0.f, 0.f, 0.f};
-        float es = 0.f, el = 0.f, ljs = 0.f, ljl = 0.f, vs = 0.f, vl = 0.f;
-        float vxs = 0.f, vys = 0.f, vzs = 0.f, vxl = 0.f, vyl = 0.f, vzl = 0.f;
-        const int k1 = min(k0 + 16, n);
-        // one list entry; MIXED: an entry of the front segment (it holds a
-        // short-class pair, so both classes are split by exact 0/1 weights);
-        // otherwise every pair is long-class and only the long sums move
-        auto entry = [&](auto mixed, int kk, const PairEntry &e, const float4 &pj, const int2 &auxj) {
-            constexpr bool MIXED = decltype(mixed)::value;
-            unsigned long long m = e.mask;
-            if (group == 1) m &= e.mshort;
-            else if (group == 2) m &= ~e.mshort;
-            // branch-free body: a lane without a pair inside the cutoff
-            // takes rinv = 0, which makes every contribution exactly zero
-            // (no divergent paths to merge)
-            const int infoj = auxj.x;
-            const bool on = (m >> lane) & 1ull;
-            const bool isshort = MIXED && ((e.mshort >> lane) & 1ull);
+
+    // One list entry.  MIXED: an entry of the front segment (it holds a
+    // short-class pair, so both classes are split by exact 0/1 weights);
+    // otherwise every pair is long-class and only the long sums move.
+    // Branch-free body: a lane without a pair inside the cutoff takes
+    // rinv = 0, which makes every contribution exactly zero.
+    auto entry = [&](auto mixed, PairAcc<decltype(mixed)::value> &s, const PairEntry &e, float4 pj, int2 auxj,
+                     unsigned slot) {   // slot: the entry's record position (units)
+        constexpr bool MIXED = decltype(mixed)::value;
+        // uses of the j data stay here, not next to their prefetch
+        __builtin_amdgcn_sched_barrier(0);
+        unsigned long long m = e.mask;
+        if (group == 1) m &= e.mshort;
+        else if (group == 2) m &= ~e.mshort;
+        float dx = (pi.x - pj.x) + e.sx;
+        float dy = (pi.y - pj.y) + e.sy;
+        float dz = (pi.z - pj.z) + e.sz;
+        // entries whose clusters reach near half the box: minimum image
+        // per pair (the flag is uniform over the wavefront)
+        if (e.cj < 0) {
+            dx = fmaf(-bl[0], rintf(dx * bi[0]), dx);
+            dy = fmaf(-bl[1], rintf(dy * bi[1]), dy);
+            dz = fmaf(-bl[2], rintf(dz * bi[2]), dz);
+        }
+        const float dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+        const float r2 = dx2 + dy2 + dz2;
+        // lanes in the list and inside the cutoff, as a wave mask; pairs
+        // within the FP32 error band of the cutoff take the reference's
+        // FP64 test (no band for reference lists: rlo = rhi)
+        unsigned long long inm = __ballot(r2 <= rlo) & m;
+        const unsigned long long amb = __ballot(r2 <= rhi) & m & ~inm;
+        if (amb != 0ull) {
+            const bool mine = (amb >> lane) & 1ull;
+            inm |= __ballot(mine && exact_in(jclu(e)));
+        }
+        const bool in = __builtin_amdgcn_inverse_ballot_w64(inm);
+        const float rinv = in ? __builtin_amdgcn_rsqf(r2) : 0.f;
+        const float r = r2 * rinv;
+        float rei = rinv;
+        bool softp = false;      // a soft-core pair (lambda-coupled types)
+        if (SOFT) {
+            const int eej = auxj.x >> 16, eei = infoi >> 16;
+            softp = in && (eei & eej) == 0;
+            rei = softp ? __builtin_amdgcn_rcpf(r + raex) : rinv;
+        }
+        const float t = __builtin_amdgcn_rcpf(fmaf(b1a, r, 1.0f));
+        const float g = __builtin_amdgcn_exp2f(nal2 * r2);
+        const float pt = ((((A5 * t + A4) * t + A3) * t + A2) * t + A1) * t;
+        const float phi = pt * (g * rei);                   // erfc(alpha r) / r
+        const float2 lj = ljrow[auxj.x & 0xffff];
+        const float r2i = rei * rei;
+        const float r6i = r2i * r2i * r2i;
+        // r F_LJ; a rounded product (not fused into a later sum), so both
+        // soft-core instantiations form the same sums for ordinary pairs
+        const float w6 = mul_unfused(fmaf(lj.y, r6i, lj.x), r6i);
+        const float e12 = fmaf(lj.x, r6i, w6);              // 12 E_LJ
+        const float qq = qic * pj.w;
+        const float t2 = fmaf(qq, fmaf(talpha, g, phi), w6);
+        const float fsc = (SOFT ? rinv * rei : r2i) * t2;   // |F| / r
+        // every other pair takes the same operations in both instantiations
+        const float vr = softp ? w6 * (rei * r) : w6;
+        // reactions on j (minus the force on i)
+        const float nx = -fsc * dx, ny = -fsc * dy, nz = -fsc * dz;
+        const float qlo = __int_as_float(auxj.y);
+        if (DETAIL && in) {
+            const int aj = a.clu_atom[jclu(e) * kClu + jl];
+            const int b = a.atom_mol[aj] == moli ? a.nmt : a.atom_moltype[aj];
+            s_det[(b * 2) * 64 + lane] += (double)(pj.w * rei) + (double)(qlo * rei);
+            s_det[(b * 2 + 1) * 64 + lane] += (double)e12 * (1.0 / 12.0);
+        }
+        if (MIXED) {
+            const bool isshort = __builtin_amdgcn_inverse_ballot_w64(e.mshort);
             const float ws = isshort ? 1.f : 0.f, wl = 1.f - ws;
-            float dx = (pi.x - pj.x) + e.sx;
-            float dy = (pi.y - pj.y) + e.sy;
-            float dz = (pi.z - pj.z) + e.sz;
-            // entries whose clusters reach near half the box: minimum image
-            // per pair (the flag is uniform over the wavefront)
-            if (e.cj < 0) {
-                dx = fmaf(-bl[0], rintf(dx * bi[0]), dx);
-                dy = fmaf(-bl[1], rintf(dy * bi[1]), dy);
-                dz = fmaf(-bl[2], rintf(dz * bi[2]), dz);
+            const float sx = ws * nx, sy = ws * ny, sz = ws * nz;
+            const float lx = nx - sx, ly = ny - sy, lz = nz - sz;
+            const float fss = ws * fsc, fsl = fsc - fss;
+            s.fs[0] = fmaf(fss, dx, s.fs[0]); s.fs[1] = fmaf(fss, dy, s.fs[1]); s.fs[2] = fmaf(fss, dz, s.fs[2]);
+            s.fl[0] = fmaf(fsl, dx, s.fl[0]); s.fl[1] = fmaf(fsl, dy, s.fl[1]); s.fl[2] = fmaf(fsl, dz, s.fl[2]);
+            s.vs[0] = fmaf(fss, dx2, s.vs[0]); s.vs[1] = fmaf(fss, dy2, s.vs[1]); s.vs[2] = fmaf(fss, dz2, s.vs[2]);
+            s.vl[0] = fmaf(fsl, dx2, s.vl[0]); s.vl[1] = fmaf(fsl, dy2, s.vl[1]); s.vl[2] = fmaf(fsl, dz2, s.vl[2]);
+            const float ees = fmaf(qlo, phi, pj.w * phi);
+            s.es = fmaf(ws, ees, s.es); s.ljs = fmaf(ws, e12, s.ljs); s.vrs = fmaf(ws, vr, s.vrs);
+            s.el = fmaf(wl, ees, s.el); s.lj = fmaf(wl, e12, s.lj); s.vr = fmaf(wl, vr, s.vr);
+            cnt_s += __popcll(inm & e.mshort);
+            cnt_l += __popcll(inm & ~e.mshort);
+            const float d0 = add_xor8(swap_sum16(swap_sum32(lx, ly), swap_sum32(lz, sx)));
+            const float d1 = add_xor8(swap_sum16(swap_sum32(sy, sz), 0.f));
+            put(d0, kPatMix0, fr0, slot);
+            put(d1, kPatMix1, fr1, slot);
+        } else {
+            s.fl[0] = fmaf(fsc, dx, s.fl[0]); s.fl[1] = fmaf(fsc, dy, s.fl[1]); s.fl[2] = fmaf(fsc, dz, s.fl[2]);
+            s.vl[0] = fmaf(fsc, dx2, s.vl[0]); s.vl[1] = fmaf(fsc, dy2, s.vl[1]); s.vl[2] = fmaf(fsc, dz2, s.vl[2]);
+            s.el = fmaf(pj.w, phi, s.el); s.el_lo = fmaf(qlo, phi, s.el_lo);
+            s.a6 = fmaf(lj.x, r6i, s.a6);
+            if (SOFT) s.w6 += w6;
+            s.vr += vr;
+            cnt_l += __popcll(inm);
+            const float d = add_xor8(swap_sum16(swap_sum32(nx, ny), swap_sum32(nz, 0.f)));
+            put(d, kPatLong, fr0, slot);
+        }
+    };
+
+    // One segment of the list: entries P[0], P[step], ..., cnt of them,
+    // two per iteration (the j data of each entry is loaded one entry
+    // ahead, the entries themselves two ahead, in fixed registers)
+    auto segment = [&](auto mixed, int slot0, int step, int cnt) {
+        const MDX_CONST PairEntry *P = L + slot0;
+        constexpr bool MIXED = decltype(mixed)::value;
+        if (cnt <= 0) return;
+        auto at = [&](int k) { return ldv(P + (ptrdiff_t)step * min(k, cnt - 1)); };
+        PairEntry eA = at(0), eB = at(1);
+        float4 pjA = ld_xq(jclu(eA));
+        int2 auxA = ld_aux(jclu(eA));
+        for (int k0 = 0; k0 < cnt; k0 += kPairBlock) {
+            PairAcc<MIXED> s;
+            const int k1 = min(k0 + kPairBlock, cnt);
+            for (int k = k0; k < k1; k += 2) {
+                const float4 pjB = ld_xq(jclu(eB));
+                const int2 auxB = ld_aux(jclu(eB));
+                const PairEntry eC = at(k + 2);
+                entry(mixed, s, eA, pjA, auxA, ldv(jdest + slot0 + step * k));
+                pjA = ld_xq(jclu(eC));
+                auxA = ld_aux(jclu(eC));
+                const PairEntry eD = at(k + 3);
+                if (k + 1 < k1) entry(mixed, s, eB, pjB, auxB, ldv(jdest + slot0 + step * (k + 1)));
+                eA = eC;
+                eB = eD;
             }
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            bool in = on && r2 <= rc2;
-            if (band > 0.f) {
-                // near the cutoff: the reference's FP64 test decides
-                const bool amb = on && fabsf(r2 - rc2) <= band;
-                if (__ballot(amb) != 0ull && amb) in = exact_in(jclu(e));
-            }
-            const float rinv = in ? __builtin_amdgcn_rsqf(r2) : 0.f;
-            const float r = r2 * rinv;
-            float rei = rinv;
-            if (SOFT) {
-                const int eej = infoj >> 16, eei = infoi >> 16;
-                rei = (in && (eei & eej) == 0) ? __builtin_amdgcn_rcpf(r + raex) : rinv;
-            }
-            const float ar = alpha * r;
-            const float t = __builtin_amdgcn_rcpf(1.0f + B1 * ar);
-            const float g = __builtin_amdgcn_exp2f(nal2 * r2);
-            const float erfct = ((((A5 * t + A4) * t + A3) * t + A2) * t + A1) * t * g;
-            const float ee1 = qic * pj.w * rei;
-            const float phi = erfct * rei;
-            const float ees = pj.w * phi + __int_as_float(auxj.y) * phi;
-            const float2 lj = ljrow[infoj & 0xffff];
-            const float r2i = rei * rei;
-            const float r6i = r2i * r2i * r2i;
-            const float esr = (lj.x + lj.y * r6i) * r6i;
-            const float fsr = (6.0f * lj.x + 12.0f * lj.y * r6i) * r6i * rei;
-            const float fsc = (ee1 * (talpha * g + erfct * rei) + fsr) * rinv;
-            const float fxx = fsc * dx, fyy = fsc * dy, fzz = fsc * dz;
-            const float vr = fsr * r;
-            if (DETAIL && in) {
-                const int aj = a.clu_atom[jclu(e) * kClu + jl];
-                const int b = a.atom_mol[aj] == moli ? a.nmt : a.atom_moltype[aj];
-                s_det[(b * 2) * 64 + lane] += (double)(pj.w * rei) + (double)(__int_as_float(auxj.y) * rei);
-                s_det[(b * 2 + 1) * 64 + lane] += (double)esr;
-            }
-            const float vxx = fxx * dx, vyy = fyy * dy, vzz = fzz * dz;
-            const unsigned long long inm = __ballot(in);
+            for (int q = 0; q < 3; ++q) Fl[q] += (double)s.fl[q];
+            E[1] += (double)s.el + (double)s.el_lo;
+            E[3] += ((double)s.lj + (double)s.a6 + (double)(MIXED ? 0.f : SOFT ? s.w6 : s.vr)) * (1.0 / 12.0);
+            E[5] += (double)s.vr;
+            for (int q = 0; q < 3; ++q) s_acc[(3 + q) * 64 + lane] += (double)s.vl[q];
             if (MIXED) {
-                fs[0] = fmaf(ws, fxx, fs[0]); fs[1] = fmaf(ws, fyy, fs[1]); fs[2] = fmaf(ws, fzz, fs[2]);
-                fl[0] = fmaf(wl, fxx, fl[0]); fl[1] = fmaf(wl, fyy, fl[1]); fl[2] = fmaf(wl, fzz, fl[2]);
-                es = fmaf(ws, ees, es); ljs = fmaf(ws, esr, ljs); vs = fmaf(ws, vr, vs);
-                el = fmaf(wl, ees, el); ljl = fmaf(wl, esr, ljl); vl = fmaf(wl, vr, vl);
-                vxs = fmaf(ws, vxx, vxs); vys = fmaf(ws, vyy, vys); vzs = fmaf(ws, vzz, vzs);
-                vxl = fmaf(wl, vxx, vxl); vyl = fmaf(wl, vyy, vyl); vzl = fmaf(wl, vzz, vzl);
-                cnt_s += __popcll(inm & e.mshort);
-                cnt_l += __popcll(inm & ~e.mshort);
-            } else {
-                fl[0] += fxx; fl[1] += fyy; fl[2] += fzz;
-                el += ees; ljl += esr; vl += vr;
-                vxl += vxx; vyl += vyy; vzl += vzz;
-                cnt_l += __popcll(inm);
+                for (int q = 0; q < 3; ++q) Fs[q] += (double)s.fs[q];
+                E[0] += (double)s.es;
+                E[2] += (double)s.ljs * (1.0 / 12.0);
+                E[4] += (double)s.vrs;
+                for (int q = 0; q < 3; ++q) s_acc[q * 64 + lane] += (double)s.vs[q];
             }
-            // Newton's third law: the j reactions (short x, y, z, long x, y,
-            // z, two zeros) summed over the 8 i slots by a reduce-scatter
-            // butterfly; lane (il, jl) ends with component il of j slot jl
-            float w[4], u[2];
-            if (MIXED) {
-                w[0] = swap_sum32(-ws * fxx, -wl * fyy);
-                w[1] = swap_sum32(-ws * fyy, -wl * fzz);
-                w[2] = swap_sum32(-ws * fzz, 0.f);
-                w[3] = swap_sum32(-wl * fxx, 0.f);
-            } else {
-                w[0] = swap_sum32(0.f, -fyy);
-                w[1] = swap_sum32(0.f, -fzz);
-                w[2] = 0.f;
-                w[3] = swap_sum32(-fxx, 0.f);
-            }
-            for (int q = 0; q < 2; ++q) u[q] = swap_sum16(w[q], w[2 + q]);
-            const float send = (lane & 8) ? u[0] : u[1], keep = (lane & 8) ? u[1] : u[0];
-            s_jf[kk * 64 + lane] = keep + xor8(send);
-            if (lane == 0) s_jc[kk] = jclu(e);
-        };
-        for (int k = k0; k < k1; ++k) {
-            const PairEntry e = e1;
-            const float4 pj = pj1;
-            int2 auxj = aux1;
-            // keeps the LJ-type mask below from being hoisted next to the
-            // prefetch load, which would wait for that load at once
-            asm volatile("" : "+v"(auxj.x));
-            e1 = e2;
-            // unconditional prefetch (past the end it re-reads the last
-            // entry and its j-cluster: valid addresses, values unused)
-            e2 = ldv(L + ent(min(k + 2, n - 1)));
-            pj1 = ldv(xq + jclu(e1) * kClu + jl);
-            aux1 = ldv(aux + jclu(e1) * kClu + jl);
-            if (k < ns) entry(std::true_type{}, k - k0, e, pj, auxj);
-            else entry(std::false_type{}, k - k0, e, pj, auxj);
         }
-        // the block's j reactions into the 64-bit fixed-point accumulators:
-        // one atomic per (entry, component, j slot), batched so the wave
-        // waits for them once per block
-        __syncthreads();
-        for (int k = k0; k < k1; ++k) {
-            const float r = s_jf[(k - k0) * 64 + lane];
-            if (il < 6 && r != 0.f) fx_add_g(jf_base + (size_t)s_jc[k - k0] * (kClu * 8), (double)r, fscale);
-        }
-        __syncthreads();
-        for (int q = 0; q < 3; ++q) { Fs[q] += fs[q]; Fl[q] += fl[q]; }
-        E[0] += es; E[1] += el; E[2] += ljs; E[3] += ljl; E[4] += vs; E[5] += vl;
-        {
-            const float part[6] = {vxs, vys, vzs, vxl, vyl, vzl};
-            for (int q = 0; q < 6; ++q) s_acc[q * 64 + lane] += (double)part[q];
-        }
-    }
+    };
+    // the front segment holds the entries with a short-class pair; a
+    // fast-class call reads only these; the others sit at the back of the
+    // cluster's list space, last first
+    segment(std::true_type{}, 0, 1, ns);
+    if (group != 1) segment(std::false_type{}, a.list_cap - 1, -1, n - ns);
+
     // i forces: sum over the 8 lanes of one i slot
     for (int off = 1; off < 8; off <<= 1)
         for (int q = 0; q < 3; ++q) {
@@ -389,18 +442,91 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, float
 template <bool SOFT, bool DETAIL>
 __global__ __launch_bounds__(64) void k_pairs(const PairArgs *__restrict__ args, const int2 *__restrict__ map,
                                               int group) {
-    extern __shared__ float2 s_lj[];      // nlj x nlj (A6, B12), then detail buckets
-    __shared__ float s_jf[16 * 64];       // j reactions of a block of 16 list entries
-    __shared__ int s_jc[16];              // their j-clusters
+    extern __shared__ float2 s_lj[];      // nlj x nlj (6 A6, 12 B12), then detail buckets
     __shared__ double s_acc[6 * 64];      // per-lane FP64 virial sums
     const int2 m = map[blockIdx.x];
-    pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, s_jf, s_jc, s_acc, group);
+    pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, s_acc, group);
 }
 
 template __global__ void k_pairs<false, false>(const PairArgs *, const int2 *, int);
 template __global__ void k_pairs<true, false>(const PairArgs *, const int2 *, int);
 template __global__ void k_pairs<false, true>(const PairArgs *, const int2 *, int);
 template __global__ void k_pairs<true, true>(const PairArgs *, const int2 *, int);
+
+// j reactions per atom: one wavefront per j-cluster, lanes 0-47 (48-63
+// idle), each on one record float: the front-segment records (two units:
+// long x, z, y, short x, y, z, 8 j slots each) one per step with lane =
+// float; then the long-class records (one unit: long x, z, y) two per step,
+// lane = 24 (record of the pair) + float.  Every lane sums its floats as
+// 64-bit fixed point (round(v fscale) as the low bits of v fscale +
+// 1.5 2^52, exact for |v fscale| < 2^51, else converted directly), so the
+// integer totals do not depend on the record order; lanes 0-23 then add
+// the long sums of lanes 24-47 and every word is stored once: no atomics,
+// no clearing.  A fast-class call (group 1) sums the short floats of the
+// front-segment records only, a slow-class call (group 2) the long ones.
+__device__ inline long long to_fixed(float v, double fscale, float vlim) {
+    long long qv = __double_as_longlong(fma((double)v, fscale, kMagic)) - __double_as_longlong(kMagic);
+    const bool big = fabsf(v) >= vlim;
+    if (__ballot(big) != 0ull) {
+        const long long slow = __double2ll_rn((double)v * fscale);
+        qv = big ? slow : qv;
+        asm volatile("" : "+v"(qv));    // a real branch: this path is rare
+    }
+    return qv;
+}
+
+__device__ inline void jgather_body(const PairArgs &a, int c, int group) {
+    if (c >= a.nclu) return;
+    const int lane = threadIdx.x;
+    const bool live = lane < 2 * kRecUnit;
+    const bool is_long = lane < kRecUnit;             // front records: long floats
+    const bool want1 = live && (is_long ? group != 1 : group != 2);
+    const unsigned u0 = sld(a.jstart)[c];
+    const int2 nc = ldv(sld(a.jcnt) + c);
+    const MDX_GLOBAL float *rec = gld(a.rec) + (size_t)u0 * kRecUnit + lane;
+    const double fscale = a.fscale;
+    const float vlim = (float)(2251799813685248.0 / fscale);   // 2^51 / fscale
+    long long acc1 = 0, acc2 = 0;
+    constexpr int U = 8;       // loads in flight per lane
+    // front-segment records: 2 units each
+    for (int t0 = 0; t0 < nc.x; t0 += U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            v[u] = (t0 + u < nc.x && want1) ? __builtin_nontemporal_load(rec + (size_t)(t0 + u) * 2 * kRecUnit) : 0.f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc1 += to_fixed(v[u], fscale, vlim);
+    }
+    // long-class records: 1 unit each, two per step
+    if (group != 1) {
+        const MDX_GLOBAL float *rl = rec + (size_t)nc.x * 2 * kRecUnit;
+        const int half = lane < kRecUnit ? 0 : 1;
+        for (int t0 = 0; t0 < nc.y; t0 += 2 * U) {
+            float v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                v[u] = (live && t0 + 2 * u + half < nc.y) ? __builtin_nontemporal_load(rl + (size_t)(t0 + 2 * u) * kRecUnit)
+                                                          : 0.f;
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc2 += to_fixed(v[u], fscale, vlim);
+        }
+    }
+    // lane l < 24 adds the long-record sums of lane l + 24
+    const long long up = (long long)__shfl_down((unsigned long long)acc2, kRecUnit);
+    const int jl = lane & 7;
+    if (is_long && group != 1) {
+        constexpr int word_long[3] = {3, 5, 4};           // long x, z, y
+        gst(a.jf)[((size_t)c * kClu + jl) * 8 + word_long[lane >> 3]] = (unsigned long long)(acc1 + acc2 + up);
+    } else if (live && !is_long && group != 2) {
+        gst(a.jf)[((size_t)c * kClu + jl) * 8 + (lane >> 3) - 3] = (unsigned long long)acc1;   // short x, y, z
+    }
+}
+
+__global__ __launch_bounds__(64) void k_jgather(const PairArgs *__restrict__ args, const int2 *__restrict__ map,
+                                                int group) {
+    const int2 m = map[blockIdx.x];
+    jgather_body(args[m.x], m.y, group);
+}
 
 // Per-slot LJ type, ee flag and the low part of the charge (at list build).
 __global__ void k_slot_info(int nslot, const int *__restrict__ clu_atom, const int *__restrict__ atom_ljtype,
