@@ -169,6 +169,69 @@ def cpu_baseline(s, topo, prm, numtask: int = 12):
     }
 
 
+def bench_detail(eng, calls: int) -> dict:
+    """The call the Fortran drop-in makes at every step (ALLFORCE with
+    MDX_WANT_DETAIL, mdx_forces.f): forces F, G, H and the per-type energy
+    breakdown copied back to host arrays, so the time includes the PCIe
+    copies of 9 x 96,174 doubles."""
+    import ctypes as C
+    from mdynamix_amd import abi
+    from mdynamix_amd.abi import MdxEnergies, as_ptr
+    # the caller's arrays, allocated once as a Fortran program holds them
+    out = np.zeros((9, eng.n))
+    ptrs = [as_ptr(out[k]) for k in range(9)]
+    e = MdxEnergies()
+
+    def timed(want):
+        def call():
+            st = eng.lib.mdx_compute(eng._h, abi.MDX_FORCES_ALL, want, *ptrs, C.byref(e))
+            if st != 0:
+                raise RuntimeError("mdx_compute failed: %d" % st)
+        call()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            call()
+        return 1e3 * (time.perf_counter() - t0) / calls
+    return {"calls": calls, "ms_per_call": timed(abi.MDX_WANT_DETAIL), "ms_per_call_no_detail": timed(0),
+            "what": "mdx_compute(MDX_FORCES_ALL, MDX_WANT_DETAIL) into host F, G, H arrays (PCIe included); "
+                    "beside it the same host-array call without the breakdown"}
+
+
+def bench_md(s, topo, prm, steps: int, warmup: int) -> dict:
+    """MD on the device at the headline workload: the reference's
+    single-time-step leap-frog with SHAKE (SULTAN, mdstep.f:29-260) for the
+    rigid SPC/E waters, Nose-Hoover at 300 K, list rebuild every 10 steps,
+    each step a full force call; between rebuilds a step is one captured
+    hipGraph.  ns/day from the system's 2 fs step."""
+    from mdynamix_amd import systems, topology
+    from mdynamix_amd.engine import ForceEngine
+    eng = ForceEngine(topo, prm)
+    eng.set_positions(s.pos)
+    eng.set_timing(False)
+    md = topology.md_params(s)
+    eng.md_init(md, systems.maxwell_velocities(s, 41))
+    eng.md_run(warmup)
+    t0 = time.perf_counter()
+    tr = eng.md_run(steps)
+    t = time.perf_counter() - t0
+    fails = eng.md_shake_failures()
+    eng.close()
+    return {"workload": "spce_nacl_32k", "integrator": "SULTAN + SHAKE (single time step)",
+            "dt_fs": s.dt_fs, "steps": steps, "ms_per_step": 1e3 * t / steps,
+            "ns_per_day": (steps / t) * s.dt_fs * 0.0864, "final_temperature_K": float(tr[-1, 2]),
+            "shake_failures": int(fails)}
+
+
+def call_roofline(pairs_per_s: float, kterms_per_s: float) -> dict:
+    """Whole-call FLOP rate of a batched or served form against the FP32
+    peak: 74 FLOP per pair within R_cut and 21 FLOP per (k vector, atom)
+    term of the reciprocal sum, per second of wall time."""
+    flops = PAIR_FLOP * pairs_per_s + RECIP_FLOP * kterms_per_s
+    return {"bound": "valu+mfma", "achieved": round(flops / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(flops / 1e12 / FP32_PEAK_TFLOPS, 4),
+            "note": "whole force calls per wall second (pairs x 74 FLOP + k terms x 21 FLOP)"}
+
+
 def bench_batched(steps: int, warmup: int, beads: int = 32, n_water: int = 4000) -> dict:
     """Path-integral batch: `beads` replicas of an SPC-FW box (4,000 waters,
     49.3 A) evaluated as one ring batch per force call (every kernel one
@@ -211,10 +274,11 @@ def bench_batched(steps: int, warmup: int, beads: int = 32, n_water: int = 4000)
     ring.close()
     for e in engs:
         e.close()
+    kterms = _count_k(prm) * topo.natoms * beads * steps
     return {"workload": "pimd_%dx%d_spcfw" % (beads, n_water), "beads": beads, "atoms_per_bead": topo.natoms,
             "steps": steps, "ms_per_call": 1e3 * tb / steps, "pair_interactions_per_s": pairs / tb,
             "one_at_a_time": {"ms_per_call": 1e3 * ta / steps, "pair_interactions_per_s": pairs_a / ta},
-            "speedup": ta / tb}
+            "speedup": ta / tb, "roofline": call_roofline(pairs / tb, kterms / tb)}
 
 
 def bench_respa(long_steps: int, warmup: int) -> dict:
@@ -256,15 +320,24 @@ def bench_served(n_jobs: int, nsteps: int, rate: float, seed: int = 77) -> dict:
     import threading
     from mdynamix_amd import systems
     from mdynamix_amd.serve import MDJob, Server
+    from mdynamix_amd import topology
     rng = np.random.default_rng(seed)
     jobs = []
+    kterms = 0.0
+    # sizes: log-spaced over 1.5k-60k atoms (the generator's atom count is
+    # corrected once toward each target), in a shuffled arrival order
+    targets = np.geomspace(1500, 60000, n_jobs)[rng.permutation(n_jobs)]
     for j in range(n_jobs):
-        n = int(np.exp(rng.uniform(np.log(1500), np.log(60000))))
+        n = int(targets[j])
         # solute size capped at n/50 so it spans less than half the box
         ns = min(int(rng.integers(10, 61)), max(10, n // 50))
         lam = float(rng.integers(0, 21)) * 0.05
         s = systems.ee_job(n, ns, lam, seed + j, rigid_solute=True)
+        if s.pos.shape[0] != n:
+            s = systems.ee_job(int(round(n * n / s.pos.shape[0])), ns, lam, seed + j, rigid_solute=True)
         jobs.append(MDJob(s, nsteps, systems.maxwell_velocities(s, seed + j)))
+        _, jprm = topology.from_system(s)
+        kterms += float(_count_k(jprm)) * s.pos.shape[0] * nsteps
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
 
     def serve(slots):
@@ -289,7 +362,7 @@ def bench_served(n_jobs: int, nsteps: int, rate: float, seed: int = 77) -> dict:
         ok = all(r["status"] == 0 for r in res) and len(res) == n_jobs
         return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
                 "mean_s": float(lat.mean()), "makespan_s": span, "pair_interactions_per_s": pairs / span,
-                "rounds": st["rounds"], "all_ok": ok}
+                "rounds": st["rounds"], "all_ok": ok, "roofline": call_roofline(pairs / span, kterms / span)}
 
     served = serve(16)
     alone = serve(1)
@@ -413,15 +486,27 @@ def main() -> int:
     dom = max(cand, key=lambda k: cand[k][0])
     dms, dflop = cand[dom]
     achieved = dflop / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
-    traffic = measured_traffic(dom)
+    if dom == "k_pairs":
+        # the pair stage: k_pairs and the gather of its j-reaction records
+        tp, tg = measured_traffic("k_pairs"), measured_traffic("k_jgather")
+        traffic = tp + tg if tp is not None and tg is not None else None
+        dom = "k_pairs+k_jgather"
+    else:
+        traffic = measured_traffic(dom)
+    entries, front = eng.list_entries()
+    lanes = {"list_entries": entries, "front_entries": front, "pairs_in_cutoff": pairs_call,
+             "lane_occupancy": round(pairs_call / (64.0 * entries), 4) if entries else None}
     # every candidate is compute-bound (a few hundred FLOP per HBM byte):
     # k_pairs on the vector ALU, the k-space passes on the matrix cores
-    roof = {"kernel": dom, "bound": "valu" if dom == "k_pairs" else "mfma",
+    roof = {"kernel": dom, "bound": "valu" if dom.startswith("k_pairs") else "mfma",
             "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "avg_ms": round(dms, 4), "algorithmic_flop": dflop,
-            "note": ("k_pairs: 74 FLOP per pair within R_cut on the vector ALU (FP32, unpacked: its own "
-                     "issue peak is 78.6 TFLOP/s); k_sk / k_recip_force: half of the 21 FLOP per (k, atom) "
+            "lanes": lanes,
+            "note": ("k_pairs+k_jgather: the pair stage (the half-list pair kernel and the per-atom sums of "
+                     "its j-reaction records), 74 FLOP per pair within R_cut on the vector ALU (FP32, "
+                     "unpacked: its own issue peak is 78.6 TFLOP/s; lanes.lane_occupancy: pairs within "
+                     "R_cut per 64 lanes of the 8x8 cluster-pair entries); k_sk / k_recip_force: half of the 21 FLOP per (k, atom) "
                      "term of FURIR each, computed on the matrix cores as FP32 products split into f16 "
                      "hi/lo pairs (v_mfma_f32_16x16x32_f16, three products), FP64 sums; peak: "
                      "MI355X FP32 157.3 TFLOP/s (dense matrix or packed vector); traffic: HBM bytes per "
@@ -453,8 +538,11 @@ def main() -> int:
         "device": dev,
         "roofline": roof,
     }
+    out["detail_call"] = bench_detail(eng, max(args.steps // 4, 3))
     eng.close()
     print("plain: %.3f ms/step" % ms_step, file=sys.stderr, flush=True)
+    out["md"] = bench_md(s, topo, prm, max(args.steps, 20), args.warmup)
+    print("md: %.3f ms/step" % out["md"]["ms_per_step"], file=sys.stderr, flush=True)
     if not args.no_batched:
         out["batched"] = bench_batched(max(args.steps // 2, 5), args.warmup)
         print("batched: %.3f ms/call" % out["batched"]["ms_per_call"], file=sys.stderr, flush=True)

@@ -220,6 +220,10 @@ mdx_status mdx_set_molecule_com(mdx_engine *h, const double *x, const double *y,
 mdx_status mdx_build_list(mdx_engine *h);
 /* Pair counts of the last build (MBSH, MBLN). */
 mdx_status mdx_list_counts(mdx_engine *h, int64_t *n_short, int64_t *n_long);
+/* Cluster-pair entries of the last build (8 x 8 atom pairs each): all of
+ * them, and those holding a short-class pair.  No reference counterpart
+ * (the reference lists atom pairs); it gives the pair kernel's lane count. */
+mdx_status mdx_list_entries(mdx_engine *h, int64_t *entries, int64_t *front_entries);
 /* Export the lists in the reference layout and order (NBS1/NBS2, NBL1/NBL2:
  * 1-based, the larger index of a normal pair first, 1-4 pairs as
  * (-outer, inner) in the short list).  Capacities in pairs. */

@@ -147,6 +147,9 @@ struct mdx_engine {
     double *d_fs = nullptr, *d_fl = nullptr;     // [3][n] each
     unsigned long long *d_fix = nullptr;          // [9][n]: G, HL, HB
     unsigned long long *d_fixj = nullptr;         // [slot][8]: pair j reactions HL xyz, G xyz, pad
+    // pinned host staging of the host-array copies (positions in, forces
+    // out): 9 n doubles, allocated at the first such copy
+    double *h_stage = nullptr;
     double *d_out = nullptr;                      // [9][n]: F, G, H
     double *d_epart = nullptr;
     int epart_rows = 0;
@@ -666,6 +669,18 @@ mdx_status alloc_list(mdx_engine *h) {
 }
 
 
+// Pinned host staging buffer of the host-array copies (9 n doubles).
+mdx_status ensure_stage(mdx_engine *h) {
+    if (h->h_stage) return MDX_OK;
+    if (hipHostMalloc((void **)&h->h_stage, sizeof(double) * 9 * (size_t)h->n, hipHostMallocDefault) != hipSuccess) {
+        h->h_stage = nullptr;
+        h->err = "pinned host staging allocation failed";
+        return MDX_ERR_NOMEM;
+    }
+    return MDX_OK;
+}
+
+
 double list_radius2(const mdx_engine *h) {
     const double rl = h->p.rcut + (h->p.list_mode == MDX_LIST_VERLET ? h->p.skin : 0.0);
     return rl * rl;
@@ -1177,6 +1192,7 @@ mdx_status mdx_destroy(mdx_engine *h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->self) mdx_batch_destroy(h->self);
     for (auto &kv : h->dev_sizes) hipFree(kv.first);
+    if (h->h_stage) hipHostFree(h->h_stage);
     for (int k = 0; k < 2; ++k)
         if (h->ev[k]) hipEventDestroy(h->ev[k]);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -1186,10 +1202,17 @@ mdx_status mdx_destroy(mdx_engine *h) {
 
 mdx_status mdx_set_positions(mdx_engine *h, const double *x, const double *y, const double *z) {
     if (!h || !x || !y || !z) return MDX_ERR_ARG;
-    const size_t b = sizeof(double) * h->n;
-    HIPCHK(h, hipMemcpyAsync(h->d_x, x, b, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->d_y, y, b, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->d_z, z, b, hipMemcpyHostToDevice, h->stream));
+    const size_t n = h->n, b = sizeof(double) * n;
+    CHK(ensure_stage(h));
+    // through pinned memory: pageable copies are staged by the runtime in
+    // small synchronous pieces
+    HIPCHK(h, hipStreamSynchronize(h->stream));   // the stage may still feed a copy
+    std::memcpy(h->h_stage, x, b);
+    std::memcpy(h->h_stage + n, y, b);
+    std::memcpy(h->h_stage + 2 * n, z, b);
+    HIPCHK(h, hipMemcpyAsync(h->d_x, h->h_stage, b, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_y, h->h_stage + n, b, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_z, h->h_stage + 2 * n, b, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->have_pos = true;
     return MDX_OK;
@@ -1373,6 +1396,19 @@ mdx_status mdx_list_counts(mdx_engine *h, int64_t *n_short, int64_t *n_long) {
     return MDX_OK;
 }
 
+mdx_status mdx_list_entries(mdx_engine *h, int64_t *entries, int64_t *front_entries) {
+    if (!h) return MDX_ERR_ARG;
+    if (!h->have_list) return MDX_ERR_STATE;
+    std::vector<int> nl(h->nclu), ns(h->nclu);
+    HIPCHK(h, hipMemcpy(nl.data(), h->d_nlist, sizeof(int) * h->nclu, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ns.data(), h->d_nshort, sizeof(int) * h->nclu, hipMemcpyDeviceToHost));
+    int64_t e = 0, f = 0;
+    for (int c = 0; c < h->nclu; ++c) { e += nl[c]; f += ns[c]; }
+    if (entries) *entries = e;
+    if (front_entries) *front_entries = f;
+    return MDX_OK;
+}
+
 mdx_status mdx_get_list(mdx_engine *h, int32_t *nbs1, int32_t *nbs2, int64_t cap_short, int32_t *nbl1,
                         int32_t *nbl2, int64_t cap_long) {
     if (!h) return MDX_ERR_ARG;
@@ -1516,7 +1552,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         const int n = h->n;
         b->nmt_max = std::max(b->nmt_max, h->ntypes);
         if (h->d_dpart)
-            b->dred.add(DetailArgs{h->det_rows, h->ndet, h->d_dpart, b->d_det + (size_t)s * kMaxDet}, 1);
+            b->dred.add(DetailArgs{h->det_rows, h->ndet, h->d_dpart, b->d_det + (size_t)s * kMaxDet}, h->ndet);
         b->nlj_max = std::max(b->nlj_max, h->nlj);
         b->soft = b->soft || h->p.ee_enabled;
         const long long nz1 = (long long)h->rows_total * Q_N;
@@ -1766,9 +1802,19 @@ void fill_energies(const mdx_batch *b, int s, int group, bool detail, mdx_energi
 mdx_status copy_forces(mdx_engine *h, double *fx, double *fy, double *fz, double *gx, double *gy, double *gz,
                        double *hx, double *hy, double *hz, hipMemcpyKind kind) {
     double *outs[9] = {fx, fy, fz, gx, gy, gz, hx, hy, hz};
+    const size_t n = h->n, b = sizeof(double) * n;
+    if (kind == hipMemcpyDeviceToHost) {
+        // into pinned memory at full link speed, then to the caller's arrays
+        CHK(ensure_stage(h));
+        for (int k = 0; k < 9; ++k)
+            if (outs[k]) HIPCHK(h, hipMemcpyAsync(h->h_stage + k * n, h->d_out + k * n, b, kind, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int k = 0; k < 9; ++k)
+            if (outs[k]) std::memcpy(outs[k], h->h_stage + k * n, b);
+        return MDX_OK;
+    }
     for (int k = 0; k < 9; ++k)
-        if (outs[k])
-            HIPCHK(h, hipMemcpyAsync(outs[k], h->d_out + (size_t)k * h->n, sizeof(double) * h->n, kind, h->stream));
+        if (outs[k]) HIPCHK(h, hipMemcpyAsync(outs[k], h->d_out + k * n, b, kind, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return MDX_OK;
 }

@@ -188,10 +188,11 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
     }
 
     double Fs[3] = {0, 0, 0}, Fl[3] = {0, 0, 0};
-    double E[6] = {0, 0, 0, 0, 0, 0};   // el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
-    // the FP64 virial sums per lane live in LDS (touched once per block of
-    // entries; registers stay free for the pair loop): short xyz, long xyz
-    for (int q = 0; q < 6; ++q) s_acc[q * 64 + lane] = 0.0;
+    // the FP64 energy and virial sums per lane live in LDS (touched once per
+    // block of entries; registers stay free for the pair loop): virial
+    // short xyz, long xyz, then energies el_s, el_l, lj_s, lj_l, vlj_s, vlj_l
+    double *const sE = s_acc + 6 * 64 + lane;
+    for (int q = 0; q < 12; ++q) s_acc[q * 64 + lane] = 0.0;
     int cnt_s = 0, cnt_l = 0;   // wave totals (ballot counts)
     const int ns = sld(a.nshort)[ci];
     const int n = sld(a.nlist)[ci];
@@ -360,15 +361,15 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
                 eB = eD;
             }
             for (int q = 0; q < 3; ++q) Fl[q] += (double)s.fl[q];
-            E[1] += (double)s.el + (double)s.el_lo;
-            E[3] += ((double)s.lj + (double)s.a6 + (double)(MIXED ? 0.f : SOFT ? s.w6 : s.vr)) * (1.0 / 12.0);
-            E[5] += (double)s.vr;
+            sE[64] += (double)s.el + (double)s.el_lo;
+            sE[192] += ((double)s.lj + (double)s.a6 + (double)(MIXED ? 0.f : SOFT ? s.w6 : s.vr)) * (1.0 / 12.0);
+            sE[320] += (double)s.vr;
             for (int q = 0; q < 3; ++q) s_acc[(3 + q) * 64 + lane] += (double)s.vl[q];
             if (MIXED) {
                 for (int q = 0; q < 3; ++q) Fs[q] += (double)s.fs[q];
-                E[0] += (double)s.es;
-                E[2] += (double)s.ljs * (1.0 / 12.0);
-                E[4] += (double)s.vrs;
+                sE[0] += (double)s.es;
+                sE[128] += (double)s.ljs * (1.0 / 12.0);
+                sE[256] += (double)s.vrs;
                 for (int q = 0; q < 3; ++q) s_acc[q * 64 + lane] += (double)s.vs[q];
             }
         }
@@ -389,6 +390,8 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
     const double qc = ai >= 0 ? gld(a.q)[ai] * a.coulf : 0.0;
     double Vs[3], Vl[3];
     for (int q = 0; q < 3; ++q) { Vs[q] = s_acc[q * 64 + lane]; Vl[q] = s_acc[(3 + q) * 64 + lane]; }
+    double E[6];
+    for (int q = 0; q < 6; ++q) E[q] = sE[q * 64];
     E[0] *= qc;
     E[1] *= qc;
     if (jl == 0 && ai >= 0) {
@@ -443,7 +446,7 @@ template <bool SOFT, bool DETAIL>
 __global__ __launch_bounds__(64) void k_pairs(const PairArgs *__restrict__ args, const int2 *__restrict__ map,
                                               int group) {
     extern __shared__ float2 s_lj[];      // nlj x nlj (6 A6, 12 B12), then detail buckets
-    __shared__ double s_acc[6 * 64];      // per-lane FP64 virial sums
+    __shared__ double s_acc[12 * 64];     // per-lane FP64 virial and energy sums
     const int2 m = map[blockIdx.x];
     pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, s_acc, group);
 }
@@ -663,15 +666,24 @@ __global__ __launch_bounds__(256) void k_special(const SpecialArgs *__restrict__
 template __global__ void k_special<false>(const SpecialArgs *, const int2 *, int);
 template __global__ void k_special<true>(const SpecialArgs *, const int2 *, int);
 
-// Column sums of the detail rows of one system, in row order.
+// Column sums of the detail rows of one system: one block per column;
+// thread t sums rows t, t + 256, ... in order, then a fixed tree over the
+// 256 partial sums.
 __global__ __launch_bounds__(256) void k_detail_reduce(const DetailArgs *__restrict__ args,
                                                        const int2 *__restrict__ map) {
-    const DetailArgs a = args[map[blockIdx.x].x];
-    for (int c = threadIdx.x; c < a.ndet; c += blockDim.x) {
-        double s = 0.0;
-        for (int r = 0; r < a.nrows; ++r) s += a.dpart[(size_t)r * a.ndet + c];
-        a.out[c] = s;
+    __shared__ double red[256];
+    const int2 m = map[blockIdx.x];
+    const DetailArgs a = args[m.x];
+    const int c = m.y;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < a.nrows; r += 256) s += a.dpart[(size_t)r * a.ndet + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) a.out[c] = red[0];
 }
 
 }  // namespace mdx

@@ -56,6 +56,7 @@ def load_library() -> C.CDLL:
     L.mdx_set_molecule_com.argtypes = [pv, pd, pd, pd]
     L.mdx_build_list.argtypes = [pv]
     L.mdx_list_counts.argtypes = [pv, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.mdx_list_entries.argtypes = [pv, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.mdx_get_list.argtypes = [pv, pi32, pi32, C.c_int64, pi32, pi32, C.c_int64]
     L.mdx_compute.argtypes = [pv, C.c_int32, C.c_int32] + [pd] * 9 + [pv]
     L.mdx_compute_device.argtypes = [pv, C.c_int32, C.c_int32, pv, pv, pv, pv]
@@ -84,7 +85,7 @@ def load_library() -> C.CDLL:
     L.mdx_furir_device.argtypes = [pv, C.c_int32] + [pv] * 8 + [pv, C.c_int64, pv]
     L.mdx_furir_device_noalloc.argtypes = [pv, C.c_int32] + [pv] * 8 + [pv]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
-                 "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_get_list", "mdx_compute",
+                 "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_list_entries", "mdx_get_list", "mdx_compute",
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
@@ -167,6 +168,12 @@ class ForceEngine:
         s, l = C.c_int64(), C.c_int64()
         self._check(self.lib.mdx_list_counts(self._h, C.byref(s), C.byref(l)))
         return s.value, l.value
+
+    def list_entries(self) -> Tuple[int, int]:
+        """Cluster-pair entries of the last build: (all, with a short-class pair)."""
+        e, f = C.c_int64(), C.c_int64()
+        self._check(self.lib.mdx_list_entries(self._h, C.byref(e), C.byref(f)))
+        return e.value, f.value
 
     def get_list(self):
         """NBS1, NBS2, NBL1, NBL2 in the reference layout and order."""
