@@ -364,8 +364,8 @@ typedef struct mdx_md_params {
     const int32_t *cons_i;    /*   in the reference bond order (0-based)    */
     const int32_t *cons_j;
     const double *cons_len;   /* RB                                         */
-    int32_t integrator;       /* MDX_MD_LEAPFROG (SULTAN) or MDX_MD_RESPA   */
-    int32_t nfreq;            /* NFREQ: short steps per long step (RESPA)   */
+    int32_t integrator;       /* MDX_MD_LEAPFROG, MDX_MD_RESPA, MDX_MD_PIMD */
+    int32_t nfreq;            /* NFREQ: short steps per long step           */
 } mdx_md_params;
 
 /* Integrators of mdx_md_run (mdx_md_params.integrator):
@@ -381,6 +381,17 @@ typedef struct mdx_md_params {
  *                  step; no constraints (ncons must be 0). */
 #define MDX_MD_LEAPFROG 0
 #define MDX_MD_RESPA 1
+/* MDX_MD_PIMD      PIMD (pimd.f:18-314): the beads of a path-integral ring
+ *                  batch (mdx_batch_set_ring) advanced by the multiple-time-
+ *                  step scheme of MDX_MD_RESPA with the forces of the ring
+ *                  (physical forces / P, PIFORCES springs added to the fast
+ *                  forces) and, in every short step between the fast half
+ *                  kick and the drift, BEADNOSE's per-atom thermostat chains
+ *                  (mdx_batch_pimd_init); every bead is initialised with
+ *                  this integrator, the same nfreq and tau_t = 0 (the
+ *                  reference's PI_thermo switches the Nose thermostat off,
+ *                  input.f:660-664); run with mdx_batch_md_run. */
+#define MDX_MD_PIMD 2
 
 /* Start a segment from the current positions and the given velocities:
  * builds the list and evaluates the forces (SULTAN's CHCNB + ALLFORCE). */
@@ -402,6 +413,24 @@ mdx_status mdx_md_shake_failures(mdx_engine *h, int64_t *count);
  * mdx_md_init), one fused launch per kernel and step; trace: K x nsteps x 5
  * or NULL. */
 mdx_status mdx_batch_md_run(mdx_batch *b, int32_t nsteps, double *trace);
+
+/* Bead thermostat of a path-integral run ("PI_thermo T BTAU JNH",
+ * input.f:660-666): one Nose-Hoover chain of nchain links per atom, shared
+ * by the atom's beads, driven by its kinetic temperature averaged over the
+ * beads (BEADNOSE, pimd.f:402-441); tau_bead 0: no bead thermostat (NVE).
+ * Units as mdx_md_params (tau in its time unit). */
+typedef struct mdx_pimd_params {
+    double temperature;       /* TRTEMP                                     */
+    double tau_bead;          /* BTAU                                       */
+    int32_t nchain;           /* JNH (>= 1)                                 */
+} mdx_pimd_params;
+/* Make a ring batch a path-integral MD run (PIMD, pimd.f:18-314): every
+ * bead initialised by mdx_md_init with MDX_MD_PIMD; the chains start at
+ * zero and the ring forces are evaluated at the current positions
+ * (pimd.f:46-69).  Then mdx_batch_md_run advances the beads. */
+mdx_status mdx_batch_pimd_init(mdx_batch *b, const mdx_pimd_params *p);
+/* The chain variables (1 / time), nchain x n (link-major). */
+mdx_status mdx_batch_pimd_chains(mdx_batch *b, double *xi);
 
 /*
  * Serving: MD-segment jobs admitted by an HBM budget and advanced together

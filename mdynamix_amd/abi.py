@@ -30,6 +30,7 @@ MDX_LIST_VERLET = 1
 MDX_FORCES_ALL = 0
 MDX_MD_LEAPFROG = 0                  # SULTAN: leap-frog with SHEJK constraints
 MDX_MD_RESPA = 1                     # DOUBLE: multiple time step (NFREQ short steps)
+MDX_MD_PIMD = 2                      # PIMD: path-integral beads of a ring batch
 MDX_FORCES_FAST = 1
 MDX_FORCES_SLOW = 2
 MDX_WANT_DETAIL = 1
@@ -82,6 +83,10 @@ class MdxMdParams(C.Structure):
         ("cons_i", C.POINTER(C.c_int32)), ("cons_j", C.POINTER(C.c_int32)), ("cons_len", C.POINTER(C.c_double)),
         ("integrator", C.c_int32), ("nfreq", C.c_int32),
     ]
+
+
+class MdxPimdParams(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("tau_bead", C.c_double), ("nchain", C.c_int32)]
 
 
 def make_md_params(d: Dict):

@@ -194,6 +194,11 @@ struct KPlan {
 
 struct mdx_batch {
     std::vector<mdx_engine *> sys;
+    // path-integral MD of a ring batch (mdx_batch_pimd_init)
+    bool pimd = false;
+    mdx_pimd_params pi{};
+    double *d_xi = nullptr;          // [nchain][n] bead thermostat chains
+    double **d_vptr = nullptr;       // [4][K] the beads' velocity and x, y, z arrays
     std::vector<long long> seen;     // argument versions last uploaded
     std::string err;
     hipStream_t stream = nullptr;
@@ -1834,6 +1839,9 @@ MdArgs md_args(mdx_batch *b, int s, double *trace, int cap) {
     a.gx = h->d_out + 3 * (size_t)n; a.gy = a.gx + n; a.gz = a.gx + 2 * (size_t)n;
     a.hx = h->d_out + 6 * (size_t)n; a.hy = a.hx + n; a.hz = a.hx + 2 * (size_t)n;
     a.minv = h->d_minv;
+    // path-integral beads move with mass m / P (MASSD, setup.f:149)
+    a.mfac = b->pimd ? 1.0 / (double)b->sys.size() : 1.0;
+    a.imfac = b->pimd ? (double)b->sys.size() : 1.0;
     a.mass = h->d_mass;
     a.mstart = h->d_mstart;
     a.comgrp = h->d_comgrp;
@@ -1905,6 +1913,32 @@ mdx_status md_plan(mdx_batch *b, int nsteps, bool want_trace) {
     return MDX_OK;
 }
 
+// Bead-thermostat arguments of a path-integral ring batch.
+PimdArgs pimd_args(mdx_batch *b) {
+    const mdx_engine *h = b->sys[0];
+    const double nb = (double)b->sys.size();
+    PimdArgs a{};
+    a.n = h->n;
+    a.nbeads = (int)b->sys.size();
+    a.nchain = std::max(1, b->pi.nchain);
+    a.v = b->d_vptr;
+    a.mass = h->d_mass;
+    a.xi = b->d_xi;
+    a.dti = h->md.dt / h->md.nfreq;
+    const double tau = b->pi.tau_bead > 0.0 ? b->pi.tau_bead : 1.0;
+    a.inv_tau2 = 1.0 / (tau * tau);
+    a.fnh = a.inv_tau2 / (3.0 * nb);
+    a.t0 = b->pi.temperature;
+    a.conv = h->md.ekin_factor / (3.0 * h->md.boltz * nb * nb);
+    a.nmol = h->nmol;
+    a.mstart = h->d_mstart;
+    a.x = b->d_vptr + a.nbeads;
+    a.y = b->d_vptr + 2 * a.nbeads;
+    a.z = b->d_vptr + 3 * a.nbeads;
+    a.box = h->box;
+    return a;
+}
+
 // nsteps long steps of the multiple-time-step algorithm DOUBLE
 // (mdstep.f:281-530) for every system; a list rebuild falls before the
 // force call of short step nfreq / 2 + 1 of every ICHNB-th long step
@@ -1913,6 +1947,23 @@ mdx_status md_plan(mdx_batch *b, int nsteps, bool want_trace) {
 mdx_status batch_respa_run(mdx_batch *b, int nsteps, int nfreq, double *trace) {
     hipStream_t st = b->stream;
     const int nf2 = nfreq / 2 + 1;
+    // a short step's kick and drift; path-integral beads take their
+    // thermostat chains between the two (pimd.f:104-129)
+    const PimdArgs pa = b->pimd ? pimd_args(b) : PimdArgs{};
+    auto kick_drift = [&]() {
+        if (!b->pimd) {
+            launch(b, b->md_kd, 256, 0, k_md_kick_drift_fast);
+            return;
+        }
+        launch(b, b->md_kd, 256, 0, k_md_kick_fast);
+        if (b->pi.tau_bead > 0.0) k_pimd_beadnose<<<cdiv(pa.n, 256), 256, 0, st>>>(pa);
+        launch(b, b->md_kd, 256, 0, k_md_drift);
+    };
+    // GETCOM; path-integral beads by their centroids' images
+    auto getcom = [&]() {
+        if (b->pimd) k_pimd_getcom<<<cdiv(pa.nmol, 256), 256, 0, st>>>(pa);
+        else launch(b, b->md_gc, 64, 0, k_md_getcom);
+    };
     auto scaling1 = [&]() {
         launch(b, b->md_ke, 256, 0, k_md_ke<false>);
         launch(b, b->md_th, 256, 0, k_md_thermo);
@@ -1933,8 +1984,8 @@ mdx_status batch_respa_run(mdx_batch *b, int nsteps, int nfreq, double *trace) {
                 scaling1();
                 launch(b, b->md_kd, 256, 0, k_md_kick_slow);
                 for (int k = 1; k <= nfreq; ++k) {
-                    launch(b, b->md_kd, 256, 0, k_md_kick_drift_fast);
-                    launch(b, b->md_gc, 64, 0, k_md_getcom);
+                    kick_drift();
+                    getcom();
                     launch_call(b, k < nfreq ? MDX_FORCES_FAST : MDX_FORCES_ALL, false, false);
                     launch(b, b->md_kd, 256, 0, k_md_kick_fast);
                 }
@@ -1951,8 +2002,8 @@ mdx_status batch_respa_run(mdx_batch *b, int nsteps, int nfreq, double *trace) {
         scaling1();
         launch(b, b->md_kd, 256, 0, k_md_kick_slow);
         for (int k = 1; k <= nfreq; ++k) {
-            launch(b, b->md_kd, 256, 0, k_md_kick_drift_fast);
-            launch(b, b->md_gc, 64, 0, k_md_getcom);
+            kick_drift();
+            getcom();
             if (k == nf2) {
                 bool synced = false;
                 for (mdx_engine *h : b->sys)
@@ -2002,7 +2053,7 @@ mdx_status batch_md_run(mdx_batch *b, int nsteps, double *trace) {
             return MDX_ERR_ARG;
         }
     CHK(md_plan(b, std::max(nsteps, 1), trace != nullptr));
-    const int kind = integrator * 65536 + (integrator == MDX_MD_RESPA ? nfreq : 0);
+    const int kind = integrator * 65536 + (integrator != MDX_MD_LEAPFROG ? nfreq : 0);
     if (b->md_exec && b->md_exec_kind != kind) {
         hipGraphExecDestroy(b->md_exec);
         hipGraphDestroy(b->md_graph);
@@ -2013,6 +2064,13 @@ mdx_status batch_md_run(mdx_batch *b, int nsteps, double *trace) {
     for (mdx_engine *h : b->sys)
         HIPCHK(b, hipMemsetAsync(h->d_mdstate + MD_STEP, 0, sizeof(double), b->stream));
     if (integrator == MDX_MD_RESPA) return batch_respa_run(b, nsteps, nfreq, trace);
+    if (integrator == MDX_MD_PIMD) {
+        if (!b->pimd) {
+            b->err = "path-integral MD before mdx_batch_pimd_init";
+            return MDX_ERR_STATE;
+        }
+        return batch_respa_run(b, nsteps, nfreq, trace);
+    }
     hipStream_t st = b->stream;
     auto md_kernels = [&]() {
         launch(b, b->md_kd, 256, 0, k_md_kick_drift);
@@ -2286,8 +2344,9 @@ mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, 
     }
     if (!(p->dt > 0.0) || !(p->ndof > 0.0) || !(p->boltz > 0.0) || p->ncons < 0 ||
         (p->ncons > 0 && (!p->cons_i || !p->cons_j || !p->cons_len)) || p->shake_maxit < 0 ||
-        (p->integrator != MDX_MD_LEAPFROG && p->integrator != MDX_MD_RESPA) ||
-        (p->integrator == MDX_MD_RESPA && (p->nfreq < 1 || p->nfreq > 1000 || p->ncons > 0))) {
+        (p->integrator != MDX_MD_LEAPFROG && p->integrator != MDX_MD_RESPA && p->integrator != MDX_MD_PIMD) ||
+        (p->integrator != MDX_MD_LEAPFROG && (p->nfreq < 1 || p->nfreq > 1000 || p->ncons > 0)) ||
+        (p->integrator == MDX_MD_PIMD && p->tau_t > 0.0)) {
         h->err = "invalid MD parameters";
         return MDX_ERR_ARG;
     }
@@ -2352,6 +2411,71 @@ mdx_status mdx_md_run(mdx_engine *h, int32_t nsteps, double *trace) {
 mdx_status mdx_batch_md_run(mdx_batch *b, int32_t nsteps, double *trace) {
     if (!b) return MDX_ERR_ARG;
     return batch_md_run(b, nsteps, trace);
+}
+
+mdx_status mdx_batch_pimd_init(mdx_batch *b, const mdx_pimd_params *p) {
+    if (!b || !p) return MDX_ERR_ARG;
+    const int K = (int)b->sys.size();
+    if (!b->ring || K < 2) {
+        b->err = "path-integral MD needs a ring batch of at least 2 beads";
+        return MDX_ERR_STATE;
+    }
+    const mdx_engine *h0 = b->sys[0];
+    for (mdx_engine *h : b->sys)
+        if (!h->md_on || h->md.integrator != MDX_MD_PIMD || h->md.nfreq != h0->md.nfreq || h->n != h0->n) {
+            b->err = "every bead needs mdx_md_init with MDX_MD_PIMD and the same nfreq";
+            return MDX_ERR_STATE;
+        }
+    if (p->tau_bead < 0.0 || (p->tau_bead > 0.0 && (p->nchain < 1 || !(p->temperature > 0.0)))) {
+        b->err = "invalid bead thermostat";
+        return MDX_ERR_ARG;
+    }
+    const int n = h0->n, nchain = std::max(1, p->nchain);
+    if (b->d_xi) hipFree(b->d_xi);
+    if (b->d_vptr) hipFree(b->d_vptr);
+    b->d_xi = nullptr;
+    b->d_vptr = nullptr;
+    std::vector<double *> vp(4 * (size_t)K);
+    for (int s = 0; s < K; ++s) {
+        vp[s] = b->sys[s]->d_v;
+        vp[K + s] = b->sys[s]->d_x;
+        vp[2 * K + s] = b->sys[s]->d_y;
+        vp[3 * K + s] = b->sys[s]->d_z;
+    }
+    if (hipMalloc((void **)&b->d_xi, sizeof(double) * (size_t)nchain * n) != hipSuccess ||
+        hipMalloc((void **)&b->d_vptr, sizeof(double *) * 4 * K) != hipSuccess) {
+        b->err = "device allocation failed";
+        return MDX_ERR_NOMEM;
+    }
+    HIPCHK(b, hipMemsetAsync(b->d_xi, 0, sizeof(double) * (size_t)nchain * n, b->stream));
+    HIPCHK(b, hipMemcpyAsync(b->d_vptr, vp.data(), sizeof(double *) * vp.size(), hipMemcpyHostToDevice, b->stream));
+    b->pi = *p;
+    b->pi.nchain = nchain;
+    b->pimd = true;
+    if (b->md_exec) {
+        hipGraphExecDestroy(b->md_exec);
+        hipGraphDestroy(b->md_graph);
+        b->md_exec = nullptr;
+        b->md_graph = nullptr;
+    }
+    // the ring's forces at the start positions (pimd.f:46-69)
+    for (mdx_engine *h : b->sys)
+        if (h->stream != b->stream) HIPCHK(b, hipStreamSynchronize(h->stream));
+    CHK(batch_compute(b, MDX_FORCES_ALL, false));
+    HIPCHK(b, hipStreamSynchronize(b->stream));
+    return MDX_OK;
+}
+
+mdx_status mdx_batch_pimd_chains(mdx_batch *b, double *xi) {
+    if (!b || !xi) return MDX_ERR_ARG;
+    if (!b->pimd) {
+        b->err = "mdx_batch_pimd_chains before mdx_batch_pimd_init";
+        return MDX_ERR_STATE;
+    }
+    const size_t cnt = (size_t)b->pi.nchain * b->sys[0]->n;
+    HIPCHK(b, hipMemcpyAsync(xi, b->d_xi, sizeof(double) * cnt, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(b, hipStreamSynchronize(b->stream));
+    return MDX_OK;
 }
 
 mdx_status mdx_md_get_state(mdx_engine *h, double *x, double *y, double *z, double *vx, double *vy, double *vz) {
@@ -2452,6 +2576,8 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     kplan_free(b->md_kd); kplan_free(b->md_sh); kplan_free(b->md_ke); kplan_free(b->md_th);
     kplan_free(b->md_gc); kplan_free(b->md_tr);
     if (b->d_trace) hipFree(b->d_trace);
+    if (b->d_xi) hipFree(b->d_xi);
+    if (b->d_vptr) hipFree(b->d_vptr);
     if (b->d_energy) hipFree(b->d_energy);
     if (b->h_energy) hipHostFree(b->h_energy);
     if (b->d_det) hipFree(b->d_det);

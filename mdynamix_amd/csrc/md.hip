@@ -191,7 +191,7 @@ __device__ inline void md_shake_body(const MdArgs &a, int blk) {
 __device__ inline void md_kick_slow_body(const MdArgs &a, int blk) {
     const int i = blk * 256 + threadIdx.x;
     if (i >= a.n) return;
-    const double s = 0.5 * a.dt * a.accel * a.minv[i];
+    const double s = 0.5 * a.dt * a.accel * a.minv[i] * a.imfac;
     a.vx[i] = a.vx[i] + s * a.gx[i];
     a.vy[i] = a.vy[i] + s * a.gy[i];
     a.vz[i] = a.vz[i] + s * a.gz[i];
@@ -200,7 +200,7 @@ __device__ inline void md_kick_slow_body(const MdArgs &a, int blk) {
 __device__ inline void md_kick_fast_body(const MdArgs &a, int blk, bool drift) {
     const int i = blk * 256 + threadIdx.x;
     if (i >= a.n) return;
-    const double s = 0.5 * a.dti * a.accel * a.minv[i];
+    const double s = 0.5 * a.dti * a.accel * a.minv[i] * a.imfac;
     const double vx = a.vx[i] + s * a.hx[i], vy = a.vy[i] + s * a.hy[i], vz = a.vz[i] + s * a.hz[i];
     a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
     if (drift) {
@@ -231,7 +231,7 @@ __device__ inline void md_scale_ke_body(const MdArgs &a, int blk, double *red) {
             vx *= scv; vy *= scv; vz *= scv;
             a.vx[i] = vx; a.vy[i] = vy; a.vz[i] = vz;
         }
-        v2 = a.mass[i] * (vx * vx + vy * vy + vz * vz);
+        v2 = a.mass[i] * a.mfac * (vx * vx + vy * vy + vz * vz);
     }
     red[threadIdx.x] = v2;
     __syncthreads();
@@ -335,6 +335,110 @@ __device__ inline void md_trace_body(const MdArgs &a) {
     t[2] = a.state[MD_TEMP];
     t[3] = a.state[MD_SC];
     t[4] = e[Q_NPS] + e[Q_NPL];
+}
+
+// Drift of a short step: S += dti V (PIMD, pimd.f:122-129).
+__device__ inline void md_drift_body(const MdArgs &a, int blk) {
+    const int i = blk * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    a.x[i] = a.vx[i] * a.dti + a.x[i];
+    a.y[i] = a.vy[i] * a.dti + a.y[i];
+    a.z[i] = a.vz[i] * a.dti + a.z[i];
+}
+
+// BEADNOSE (pimd.f:402-441) for atom i: its kinetic energy summed over the
+// beads (bead mass m / nbeads; in bead order, the reference's
+// MPI_ALLREDUCE), the temperature ATEMP = that / (3/2 k_B nbeads), the chain
+//   xi_1 += (ATEMP / T0 - 1) dti / BTAU^2
+//   xi_j += (xi_j-1^2 - FNH) dti,  xi_j-1 *= 1 - xi_j dti   (j = 2..nchain)
+// and every bead's velocity of atom i scaled by 1 - xi_1 dti.
+__device__ inline void pimd_beadnose_body(const PimdArgs &a, int blk) {
+    const int i = blk * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    double s = 0.0;
+    for (int b = 0; b < a.nbeads; ++b) {
+        const double *v = a.v[b];
+        const double vx = v[i], vy = v[a.n + i], vz = v[2 * (size_t)a.n + i];
+        s += vx * vx + vy * vy + vz * vz;
+    }
+    const double atemp = a.conv * a.mass[i] * s;
+    double xi_prev = a.xi[i] + (atemp / a.t0 - 1.0) * a.inv_tau2 * a.dti;
+    double xi1 = xi_prev;
+    for (int j = 1; j < a.nchain; ++j) {
+        double xj = a.xi[(size_t)j * a.n + i];
+        xj = xj + (xi_prev * xi_prev - a.fnh) * a.dti;
+        xi_prev = xi_prev * (1.0 - xj * a.dti);
+        a.xi[(size_t)(j - 1) * a.n + i] = xi_prev;
+        if (j == 1) xi1 = xi_prev;
+        xi_prev = xj;
+    }
+    a.xi[(size_t)(a.nchain - 1) * a.n + i] = xi_prev;
+    const double scv = 1.0 - xi1 * a.dti;
+    for (int b = 0; b < a.nbeads; ++b) {
+        double *v = a.v[b];
+        v[i] *= scv;
+        v[a.n + i] *= scv;
+        v[2 * (size_t)a.n + i] *= scv;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pimd_beadnose(PimdArgs a) { pimd_beadnose_body(a, blockIdx.x); }
+
+// GETCOM of path-integral beads (service.f:463-563 with PI_COM,
+// pimd.f:498-520): every bead's molecular centre of mass (atoms in order),
+// their average over the beads (the centroid's centre), and every bead's
+// atoms of the molecule shifted by the image shift of that centroid (one
+// shift per axis, PBC service.f:953-956), so a molecule's beads move
+// together.  One molecule per thread.
+__device__ inline void pimd_getcom_body(const PimdArgs &a, int blk) {
+    const int m = blk * 256 + threadIdx.x;
+    if (m >= a.nmol) return;
+    const int i0 = a.mstart[m], i1 = a.mstart[m + 1];
+    double sm = 0.0;
+    for (int i = i0; i < i1; ++i) sm += a.mass[i];
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < a.nbeads; ++b) {
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int i = i0; i < i1; ++i) {
+            cx = cx + a.mass[i] * a.x[b][i];
+            cy = cy + a.mass[i] * a.y[b][i];
+            cz = cz + a.mass[i] * a.z[b][i];
+        }
+        c[0] += cx / sm;
+        c[1] += cy / sm;
+        c[2] += cz / sm;
+    }
+    double sh[3];
+    for (int q = 0; q < 3; ++q) {
+        const double cc = c[q] / a.nbeads;
+        double w = cc;
+        if (fabs(w) > a.box.h[q]) w = w - copysign(a.box.l[q], w);
+        sh[q] = w - cc;
+    }
+    // S = (S - R_b) + (R_b + shift), R_b the bead's own centre, as the
+    // reference rebuilds the coordinates (service.f:523-557)
+    for (int b = 0; b < a.nbeads; ++b) {
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int i = i0; i < i1; ++i) {
+            cx = cx + a.mass[i] * a.x[b][i];
+            cy = cy + a.mass[i] * a.y[b][i];
+            cz = cz + a.mass[i] * a.z[b][i];
+        }
+        cx /= sm; cy /= sm; cz /= sm;
+        const double nx = cx + sh[0], ny = cy + sh[1], nz = cz + sh[2];
+        for (int i = i0; i < i1; ++i) {
+            a.x[b][i] = (a.x[b][i] - cx) + nx;
+            a.y[b][i] = (a.y[b][i] - cy) + ny;
+            a.z[b][i] = (a.z[b][i] - cz) + nz;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pimd_getcom(PimdArgs a) { pimd_getcom_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_md_drift(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    md_drift_body(args[m.x], m.y);
 }
 
 __global__ __launch_bounds__(256) void k_md_kick_drift(const MdArgs *__restrict__ args, const int2 *__restrict__ map) {

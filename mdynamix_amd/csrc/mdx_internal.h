@@ -409,6 +409,9 @@ struct MdArgs {
     const double *gx, *gy, *gz;    // slow forces G (multiple time step)
     const double *hx, *hy, *hz;    // fast forces H
     const double *minv, *mass;     // 1/m and m per atom
+    // dynamical mass m * mfac (imfac = 1 / mfac): 1, or 1 / P for the beads
+    // of a path-integral ring (MASSD = MASS / NBEADS, setup.f:149)
+    double mfac, imfac;
     const int *mstart;             // atoms of molecule m: [mstart[m], mstart[m+1])
     const int *atom_mol;           // molecule of every atom
     const int *comgrp;             // molecule groups of k_md_getcom
@@ -428,6 +431,27 @@ struct MdArgs {
     double spei;
     double *trace;                 // [step][kTrace] (step = state[MD_STEP])
     int trace_cap;
+};
+
+// Path-integral bead thermostat (BEADNOSE, pimd.f:402-441): one
+// Nose-Hoover chain per atom, shared by its beads, driven by the atom's
+// kinetic temperature averaged over the beads.
+struct PimdArgs {
+    int n, nbeads, nchain;
+    double *const *v;              // [nbeads] -> the bead's velocities (x, y, z blocks of n)
+    const double *mass;
+    double *xi;                    // [nchain][n] chain variables (1 / time)
+    double dti;                    // short step
+    double inv_tau2;               // DQQT = 1 / BTAU^2
+    double fnh;                    // DQQT / (3 nbeads): target of chain links 2..
+    double t0;                     // TRTEMP
+    double conv;                   // m v^2 -> bead-averaged kinetic temperature (bead mass m / nbeads):
+                                   // ekin_factor / (3 k_B nbeads^2)
+    // centroid wrapping (GETCOM with PI_COM, service.f:523-533)
+    int nmol;
+    const int *mstart;             // atoms of molecule m: [mstart[m], mstart[m+1])
+    double *const *x, *const *y, *const *z;   // [nbeads] -> the bead's coordinates
+    Box box;
 };
 
 }  // namespace mdx

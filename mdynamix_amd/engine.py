@@ -84,6 +84,8 @@ def load_library() -> C.CDLL:
     # p, natoms, x, y, z, q, fx, fy, fz, energy4, [workspace, bytes,] stream
     L.mdx_furir_device.argtypes = [pv, C.c_int32] + [pv] * 8 + [pv, C.c_int64, pv]
     L.mdx_furir_device_noalloc.argtypes = [pv, C.c_int32] + [pv] * 8 + [pv]
+    L.mdx_batch_pimd_init.argtypes = [pv, C.POINTER(abi.MdxPimdParams)]
+    L.mdx_batch_pimd_chains.argtypes = [pv, pd]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
                  "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_list_entries", "mdx_get_list", "mdx_compute",
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
@@ -91,7 +93,7 @@ def load_library() -> C.CDLL:
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
                  "mdx_batch_md_run", "mdx_autotune", "mdx_set_kspace_shape", "mdx_get_kvectors",
                  "mdx_enumerate_kvectors", "mdx_md_shake_failures", "mdx_furir_workspace_bytes",
-                 "mdx_furir_device", "mdx_furir_device_noalloc"):
+                 "mdx_furir_device", "mdx_furir_device_noalloc", "mdx_batch_pimd_init", "mdx_batch_pimd_chains"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -320,6 +322,20 @@ class ForceBatch:
         tr = np.zeros((k, max(nsteps, 1), 5)) if trace else None
         self._check(self.lib.mdx_batch_md_run(self._b, nsteps, as_ptr(tr) if trace else None))
         return tr[:, :nsteps] if trace else None
+
+    def pimd_init(self, temperature: float, tau_bead: float, nchain: int = 1) -> None:
+        """Path-integral MD of a ring batch (every engine md_init'ed with
+        integrator MDX_MD_PIMD): bead thermostat chains (PI_thermo T BTAU
+        JNH; tau_bead 0: none) and the ring forces at the start."""
+        p = abi.MdxPimdParams(float(temperature), float(tau_bead), int(nchain))
+        self._nchain = max(1, int(nchain))
+        self._check(self.lib.mdx_batch_pimd_init(self._b, C.byref(p)))
+
+    def pimd_chains(self) -> np.ndarray:
+        """Bead thermostat chain variables, (nchain, n)."""
+        xi = np.zeros((self._nchain, self.engines[0].n))
+        self._check(self.lib.mdx_batch_pimd_chains(self._b, as_ptr(xi)))
+        return xi
 
     def close(self) -> None:
         if self._b and self._b.value:

@@ -140,6 +140,11 @@ class System:
     ee_pw2: float = 1.0 / 3.0
     ee_bee: float = 3.0
     seed: int = 0
+    # path-integral run of the reference ("PIMD P", input.f:642-659) and its
+    # per-atom bead thermostat chains ("PI_thermo T BTAU JNH",
+    # input.f:660-666; it switches the Nose thermostat off)
+    pimd_beads: int = 0
+    pi_thermo: Optional[Tuple[float, float, int]] = None
 
     # ------------------------------------------------------------------
     @property
@@ -170,9 +175,10 @@ class System:
                  "Molecule_types %d" % len(self.types)]
         for t, c in zip(self.types, self.counts):
             lines.append("%s %d" % (t.name, c))
-        lines += ["Box %.12f %.12f %.12f" % self.box, "PBC rect",
-                  "Nose_thermostat %.3f 30." % self.temperature,
-                  "Time_step %.6f" % self.dt_fs, "Number_steps %d" % nsteps]
+        lines += ["Box %.12f %.12f %.12f" % self.box, "PBC rect"]
+        if self.pi_thermo is None:
+            lines.append("Nose_thermostat %.3f 30." % self.temperature)
+        lines += ["Time_step %.6f" % self.dt_fs, "Number_steps %d" % nsteps]
         if self.shake:
             if self.shake_types is None:
                 lines.append("Constraints 1.d-6")
@@ -191,6 +197,10 @@ class System:
                       "EE_type %d" % (self.ee_type + 1), "EE_start 1",
                       "EE_offset %.12f" % self.ee_bee,
                       "EE_power %.12f %.12f" % (self.ee_pw1, self.ee_pw2)]
+        if self.pimd_beads:
+            lines.append("PIMD %d" % self.pimd_beads)
+            if self.pi_thermo is not None:
+                lines.append("PI_thermo %.6f %.6f %d" % self.pi_thermo)
         lines.append("End")
         with open(os.path.join(directory, "md.in"), "w") as f:
             f.write("\n".join(lines) + "\n")

@@ -315,6 +315,16 @@ BOLTZ_J = 1.38054e-23
 BOLTZ_KJMOL = AVSNO * BOLTZ_J * 1e-3     # kJ/mol/K
 
 
+def pimd_beadfac(temperature: float, nbeads: int) -> float:
+    """Spring constant per unit mass of the path-integral ring in kJ/mol/A^2
+    per g/mol (BEADFAC, setup.f:696-697, converted from the reference's
+    internal units): E = BEADFAC m |r_k - r_k+1|^2 per link with
+    BEADFAC = P (k_B T / hbar)^2 / 2, the reference's k_B = 1.38054e-23 J/K
+    and hbar = 1.0545e-34 J s (prcm.h:22-26)."""
+    kt_hbar = 1.38054e-23 * temperature / 1.0545e-34
+    return 0.5e-26 * nbeads * kt_hbar * kt_hbar
+
+
 def md_params(system, tolerance: float = 1e-6) -> Dict:
     """MD-segment parameters of a generated system in the host units
     (A, fs, kJ/mol, g/mol): the constrained bonds of every molecule when

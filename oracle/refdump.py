@@ -74,6 +74,30 @@ def run_trajectory(system, nsteps: int, workdir: str | None = None, timeout: flo
     return out
 
 
+PIMD_EXE = os.path.join(HERE, "_ref", "mdx_pimd")
+
+
+def run_pimd(system, nsteps: int, workdir: str | None = None, timeout: float = 600.0) -> List[Dict]:
+    """Run the reference path-integral integrator (oracle/ref/mdx_pimd.f
+    around the reference PIMD) as system.pimd_beads MPI ranks, one bead per
+    rank, for nsteps long steps; returns one dump per bead (start state,
+    end state, thermostat chains, last energies)."""
+    if not (os.path.exists(PIMD_EXE) and os.path.exists(MPIEXEC)):
+        raise FileNotFoundError("PIMD harness not built (make -C oracle/ref pimd) or mpiexec missing")
+    d = tempfile.mkdtemp(prefix="mdxpi_") if workdir is None else workdir
+    system.write_reference_inputs(d, nsteps=nsteps)
+    res = subprocess.run([MPIEXEC, "-n", str(int(system.pimd_beads)), PIMD_EXE, "md.in"], cwd=d,
+                         capture_output=True, text=True, timeout=timeout)
+    out = []
+    for r in range(system.pimd_beads):
+        path = os.path.join(d, "mdx_pimd_%d.bin" % r)
+        if not os.path.exists(path):
+            raise RuntimeError("reference PIMD run failed:\n" + res.stdout[-4000:] + res.stderr[-2000:])
+        out.append(read_dump(path))
+    out[0]["_stdout"] = [np.array(res.stdout)]
+    return out
+
+
 TIME_EXE = os.path.join(HERE, "_ref", "mdx_time")
 TIME_MPI_EXE = os.path.join(HERE, "_ref", "mdx_time_mpi")
 MPIEXEC = "/opt/conda/bin/mpiexec"

@@ -11,6 +11,7 @@ virial scalar, per-type breakdowns).  Fixtures are data only.
     python tests/golden/make_golden.py traj   # trajectory fixtures only
     python tests/golden/make_golden.py traj traj_spcfw_respa   # one of them
     python tests/golden/make_golden.py force fspc_tiny         # one force fixture
+    python tests/golden/make_golden.py pimd   # path-integral fixtures (MPI reference)
 """
 from __future__ import annotations
 
@@ -74,6 +75,51 @@ TRAJ_CASES = {
 }
 
 
+# path-integral fixtures: name -> (builder, long-step counts); the
+# reference PIMD (oracle/ref/mdx_pimd.f, one MPI rank per bead) from the
+# reference's own start state (its per-bead position jitter included).
+# PI_thermo chains of one link: the reference's THERMO array has one
+# column (INHM = 1, dimpar.h:61), longer chains overrun it.
+def _pimd_case():
+    s = systems.cfg_flex_small()
+    return dataclasses.replace(s, dt_fs=1.0, nfreq=4, rebuild_every=5, pimd_beads=4,
+                               pi_thermo=(300.0, 5.0, 1))
+
+
+PIMD_CASES = {
+    "pimd_spcfw_4": (_pimd_case, (1, 10)),
+}
+
+
+def main_pimd(names=None) -> None:
+    for name, (builder, counts) in PIMD_CASES.items():
+        if names and name not in names:
+            continue
+        s = builder()
+        out = {}
+        for n in counts:
+            beads = refdump.run_pimd(s, n)
+            if "PSIZES" not in out:
+                out["PSIZES"] = beads[0]["PSIZES"][0]
+                out["PSCAL"] = beads[0]["PSCAL"][0]
+                out["MASSDI"] = beads[0]["MASSDI"][0]
+            for b, d in enumerate(beads):
+                for k in ("SX", "SY", "SZ", "VX", "VY", "VZ"):
+                    key0 = "%s_b%d_0" % (k, b)
+                    if key0 not in out:
+                        out[key0] = d[k][0]
+                    else:
+                        assert np.array_equal(out[key0], d[k][0]), "start state differs between runs"
+                    out["%s_b%d_%d" % (k, b, n)] = d[k][1]
+                out["THERMO_b%d_%d" % (b, n)] = np.stack(d["THERMO"])
+                out["PEND_b%d_%d" % (b, n)] = d["PEND"][0]
+        out["COUNTS"] = np.array(counts)
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **out)
+        print("%-16s atoms=%d beads=%d steps=%s -> %s (%d KB)" % (name, s.natoms, s.pimd_beads, counts, path,
+                                                               os.path.getsize(path) // 1024))
+
+
 def parse_steps(stdout: str) -> np.ndarray:
     """Per-step lines of SULTAN (mdstep.f:187-190) or DOUBLE
     (mdstep.f:489-490): MSTEP, POTEN, PENS, PEL, ETOT, TEMP, TRYCK(M),
@@ -119,6 +165,9 @@ def main_traj(names=None) -> None:
 def main() -> None:
     if sys.argv[1:2] == ["traj"]:
         main_traj(sys.argv[2:])
+        return
+    if sys.argv[1:2] == ["pimd"]:
+        main_pimd(sys.argv[2:])
         return
     only = None
     if sys.argv[1:2] == ["force"]:
