@@ -222,6 +222,38 @@ def bench_md(s, topo, prm, steps: int, warmup: int) -> dict:
             "shake_failures": int(fails)}
 
 
+def bench_fp64(s, topo, prm, calls: int, f_mixed) -> dict:
+    """The same box in the FP64 precision mode (MDX_PRECISION_FP64): time per
+    force call (forces left in HBM, energies read back, as the headline
+    steps) and the mixed path's force RMS relative difference from it."""
+    from mdynamix_amd import abi
+    from mdynamix_amd.engine import ForceEngine
+    eng = ForceEngine(topo, dict(prm, precision=abi.MDX_PRECISION_FP64))
+    eng.set_positions(s.pos)
+    eng.chcnb()
+    f64 = eng.allforce()[0]
+    eng.set_timing(True)
+    for _ in range(2):
+        eng.compute_device(0, 0, 0, energies=True)
+    st = {}
+    pairs = 0
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        e = eng.compute_device(0, 0, 0, energies=True)
+        pairs += e["npairs_short"] + e["npairs_long"]
+        for k, v in eng.stage_times().items():
+            if k != "list":
+                st[k] = st.get(k, 0.0) + v
+    t = time.perf_counter() - t0
+    eng.close()
+    rms = float(np.sqrt(((f_mixed - f64) ** 2).sum() / (f64 ** 2).sum()))
+    return {"calls": calls, "ms_per_call": 1e3 * t / calls, "pair_interactions_per_s": pairs / t,
+            "stages_ms": {"pairs": round(st.get("pairs", 0.0) / calls, 4),
+                          "recip_fp64": round(st.get("recip_force", 0.0) / calls, 4)},
+            "mixed_vs_fp64_force_rms": rms,
+            "what": "FP64 pair kernel (k_pairs64) and FP64 FURIR; the mixed path's forces against these"}
+
+
 def call_roofline(pairs_per_s: float, kterms_per_s: float) -> dict:
     """Whole-call FLOP rate of a batched or served form against the FP32
     peak: 74 FLOP per pair within R_cut and 21 FLOP per (k vector, atom)
@@ -539,7 +571,10 @@ def main() -> int:
         "roofline": roof,
     }
     out["detail_call"] = bench_detail(eng, max(args.steps // 4, 3))
+    f_mixed = eng.allforce()[0]
     eng.close()
+    out["fp64_mode"] = bench_fp64(s, topo, prm, max(args.steps // 10, 3), f_mixed)
+    print("fp64 mode: %.3f ms/call" % out["fp64_mode"]["ms_per_call"], file=sys.stderr, flush=True)
     print("plain: %.3f ms/step" % ms_step, file=sys.stderr, flush=True)
     out["md"] = bench_md(s, topo, prm, max(args.steps, 20), args.warmup)
     print("md: %.3f ms/step" % out["md"]["ms_per_step"], file=sys.stderr, flush=True)

@@ -61,6 +61,21 @@ enum {
     MDX_LIST_VERLET = 1
 };
 
+/* Arithmetic of the force call (mdx_params.precision). */
+enum {
+    /* FP32 pair arithmetic from anchor-relative coordinates with FP64 and
+     * 64-bit fixed-point sums; the reciprocal sum's phase products on the
+     * matrix cores (f16 hi/lo pairs) with FP64 sums; FP64 bonded terms,
+     * exclusion corrections, energies and virials. */
+    MDX_PRECISION_MIXED = 0,
+    /* FP64 throughout: the pair terms from the FP64 coordinates with the
+     * reference's one-shift minimum image (forces.f:1078-1086), FURIR in
+     * FP64 (mdx_furir_device's kernels).  The same list, classes and
+     * outputs as MDX_PRECISION_MIXED; the accuracy reference of the
+     * engine on the GPU, slower. */
+    MDX_PRECISION_FP64 = 1
+};
+
 /* Force groups of mdx_compute. */
 enum {
     MDX_FORCES_ALL = 0,   /* ALLFORCE: fast + slow, F = G + H                    */
@@ -167,6 +182,7 @@ typedef struct mdx_params {
     double ee_raex;               /* RAEX: soft-core shift of EE pairs       */
     double ee_self_scale;         /* EC(ME): charge scale of EE types in the
                                      Ewald self term (forces.f:1434-1435)   */
+    int32_t precision;            /* MDX_PRECISION_MIXED / MDX_PRECISION_FP64 */
 } mdx_params;
 
 /* Energies and virials of one force call (COMMON /ENERGY/, /VIRRR/,

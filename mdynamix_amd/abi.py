@@ -27,6 +27,8 @@ MDX_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 MDX_LIST_REFERENCE = 0
 MDX_LIST_VERLET = 1
+MDX_PRECISION_MIXED = 0
+MDX_PRECISION_FP64 = 1
 MDX_FORCES_ALL = 0
 MDX_MD_LEAPFROG = 0                  # SULTAN: leap-frog with SHEJK constraints
 MDX_MD_RESPA = 1                     # DOUBLE: multiple time step (NFREQ short steps)
@@ -70,7 +72,7 @@ class MdxParams(C.Structure):
         ("box", C.c_double * 3), ("rcut", C.c_double), ("rshort", C.c_double),
         ("skin", C.c_double), ("alpha", C.c_double), ("fexp", C.c_double),
         ("coulomb_factor", C.c_double), ("list_mode", C.c_int32), ("ee_enabled", C.c_int32),
-        ("ee_raex", C.c_double), ("ee_self_scale", C.c_double),
+        ("ee_raex", C.c_double), ("ee_self_scale", C.c_double), ("precision", C.c_int32),
     ]
 
 
@@ -269,7 +271,8 @@ class TopologyArrays:
 
 
 def make_params(box, rcut, rshort, alpha, fexp, coulomb_factor, list_mode=MDX_LIST_REFERENCE,
-                skin=0.0, ee_enabled=False, ee_raex=0.0, ee_self_scale=1.0) -> MdxParams:
+                skin=0.0, ee_enabled=False, ee_raex=0.0, ee_self_scale=1.0,
+                precision=MDX_PRECISION_MIXED) -> MdxParams:
     p = MdxParams()
     for k in range(3):
         p.box[k] = float(box[k])
@@ -283,6 +286,7 @@ def make_params(box, rcut, rshort, alpha, fexp, coulomb_factor, list_mode=MDX_LI
     p.ee_enabled = 1 if ee_enabled else 0
     p.ee_raex = float(ee_raex)
     p.ee_self_scale = float(ee_self_scale)
+    p.precision = int(precision)
     return p
 
 

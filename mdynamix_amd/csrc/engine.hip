@@ -57,6 +57,7 @@ struct mdx_engine {
     double *d_q = nullptr, *d_q0 = nullptr, *d_mass = nullptr, *d_c14el = nullptr, *d_c14lj = nullptr;
     double *d_a6_0 = nullptr, *d_b12_0 = nullptr, *d_a6_14 = nullptr, *d_b12_14 = nullptr;
     float2 *d_lj = nullptr;
+    double2 *d_lj64 = nullptr;  // (A6, B12) for the FP64 pair kernel
     int *d_excl_start = nullptr, *d_excl_atom = nullptr;
     signed char *d_excl_is14 = nullptr;
     int *d_mstart = nullptr;
@@ -143,6 +144,8 @@ struct mdx_engine {
 
     double *d_rpart = nullptr;
     std::vector<int4> h_kidx;   // (nx, ny, nz) in INIFUR order
+    // FP64 precision mode: FURIR workspace (fur_layout of n atoms, nkv k)
+    char *d_furws = nullptr;
     // forces
     double *d_fs = nullptr, *d_fl = nullptr;     // [3][n] each
     unsigned long long *d_fix = nullptr;          // [9][n]: G, HL, HB
@@ -212,6 +215,8 @@ struct mdx_batch {
     KPlan<ZeroSeg> zero;
     KPlan<PackArgs> pack, com;
     KPlan<PairArgs> pairs;
+    KPlan<PairArgs> pairs64;         // systems in the FP64 precision mode
+    std::vector<int> fur64;          // the same systems (FP64 FURIR, one launch set each)
     KPlan<SpecialArgs> special;
     KPlan<RecipArgs> rprep, sk, skred, rforce;
     int mz = 1;
@@ -360,20 +365,26 @@ bool fur_params_ok(const mdx_params *p) {
 }
 
 // Queue the FP64 FURIR kernels on `st` with every buffer in `ws`
-// (fur_layout(n, nkv).total bytes).
-void fur_launch(const mdx_params &p, int n, long long nkv, const double *x, const double *y, const double *z,
-                const double *q, double *fx, double *fy, double *fz, double *e4, char *ws, hipStream_t st) {
+// (fur_layout(n, nkv).total bytes).  kbox: the box the k set is enumerated
+// at (null: p.box); e4 and qrow: the outputs (either may be null).
+void fur_launch(const mdx_params &p, const double *kbox, int n, long long nkv, const double *x, const double *y,
+                const double *z, const double *q, double *fx, double *fy, double *fz, double *e4, double *qrow,
+                char *ws, hipStream_t st) {
     const FurLayout L = fur_layout(n, (int)nkv);
     FurArgs a{};
     a.n = n;
     a.nkv = (int)nkv;
     a.nchunk = cdiv(std::max(n, 1), kFurChunk);
-    inifur_cube(p, a.kmax, a.kmay, a.kmaz, a.cutk2);
+    mdx_params pk = p;
+    if (kbox)
+        for (int k = 0; k < 3; ++k) pk.box[k] = kbox[k];
+    inifur_cube(pk, a.kmax, a.kmay, a.kmaz, a.cutk2);
+    a.klx = pk.box[0]; a.kly = pk.box[1]; a.klz = pk.box[2];
     a.lx = p.box[0]; a.ly = p.box[1]; a.lz = p.box[2];
     a.alp2 = p.alpha * p.alpha;
     a.cfur = 4.0 * kPI * p.coulomb_factor / (p.box[0] * p.box[1] * p.box[2]);
     a.x = x; a.y = y; a.z = z; a.q = q;
-    a.fx = fx; a.fy = fy; a.fz = fz; a.e4 = e4;
+    a.fx = fx; a.fy = fy; a.fz = fz; a.e4 = e4; a.qrow = qrow;
     a.kv = reinterpret_cast<int4 *>(ws + L.kv);
     a.spart = reinterpret_cast<double2 *>(ws + L.spart);
     a.ak = reinterpret_cast<double2 *>(ws + L.ak);
@@ -574,6 +585,7 @@ mdx_status setup_kspace(mdx_engine *h) {
     h->nslice = h->tuned_nslice > 0 ? h->tuned_nslice : std::max(1, std::min(16, cdiv(6000, std::max(1, wav))));
     h->tiles_per_slice = cdiv(std::max(1, h->ntiles), h->nslice);
     CHK(dalloc(h, &h->d_rpart, (size_t)h->nslice * 3 * h->n));
+    if (h->p.precision == MDX_PRECISION_FP64) CHK(dalloc(h, &h->d_furws, fur_layout(h->n, h->nkv).total));
     return MDX_OK;
 }
 
@@ -786,6 +798,7 @@ void fill_args(mdx_engine *h) {
     pa.coulf_f = (float)h->p.coulomb_factor;
     pa.coulf = h->p.coulomb_factor;
     pa.lj = h->d_lj;
+    pa.lj64 = h->d_lj64;
     pa.nlj = h->nlj;
     pa.clu_atom = h->d_clu_atom;
     pa.alpha = (float)h->p.alpha;
@@ -803,6 +816,8 @@ void fill_args(mdx_engine *h) {
         pa.box = h->box;
     }
     pa.raex = (float)h->p.ee_raex;
+    pa.alphad = h->p.alpha;
+    pa.raexd = h->p.ee_raex;
     pa.soft = h->p.ee_enabled ? 1 : 0;
     {
         pa.lx = (float)h->p.box[0]; pa.ly = (float)h->p.box[1]; pa.lz = (float)h->p.box[2];
@@ -945,6 +960,10 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
             h->err = "fixed molecules (LMOVE = .false.) are not supported";
             return fail(MDX_ERR_ARG);
         }
+    if (p->precision != MDX_PRECISION_MIXED && p->precision != MDX_PRECISION_FP64) {
+        h->err = "precision must be MDX_PRECISION_MIXED or MDX_PRECISION_FP64";
+        return fail(MDX_ERR_ARG);
+    }
     {
         const int n = h->n;
         auto bad = [&](const char *what) { h->err = std::string("index out of range: ") + what; return fail(MDX_ERR_ARG); };
@@ -1025,9 +1044,16 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
     std::vector<double> mass(n, 1.0);
     if (t->atom_mass) mass.assign(t->atom_mass, t->atom_mass + n);
     if ((s = upload(h, &h->d_mass, mass.data(), n))) return fail(s);
+    // LJ only where B12 != 0, as LOCALF/FORCES test it (forces.f:1160)
     std::vector<float2> lj((size_t)h->nlj * h->nlj);
-    for (size_t k = 0; k < lj.size(); ++k) lj[k] = make_float2((float)(6.0 * t->a6[k]), (float)(12.0 * t->b12[k]));
+    std::vector<double2> lj64(lj.size());
+    for (size_t k = 0; k < lj.size(); ++k) {
+        const double a6 = t->b12[k] != 0.0 ? t->a6[k] : 0.0;
+        lj[k] = make_float2((float)(6.0 * a6), (float)(12.0 * t->b12[k]));
+        lj64[k] = make_double2(a6, t->b12[k]);
+    }
     if ((s = upload(h, &h->d_lj, lj.data(), lj.size()))) return fail(s);
+    if ((s = upload(h, &h->d_lj64, lj64.data(), lj64.size()))) return fail(s);
     // molecules are contiguous atom ranges
     std::vector<int> mstart(h->nmol + 1, 0);
     for (int i = 0; i < n; ++i) {
@@ -1544,7 +1570,8 @@ void drop_graphs(mdx_batch *b) {
 // maps = true the block maps are rebuilt too (first use, box changes).
 mdx_status plan_fill(mdx_batch *b, bool maps) {
     const int K = (int)b->sys.size();
-    b->zero.clear(); b->pack.clear(); b->com.clear(); b->pairs.clear(); b->special.clear();
+    b->zero.clear(); b->pack.clear(); b->com.clear(); b->pairs.clear(); b->pairs64.clear(); b->special.clear();
+    b->fur64.clear();
     b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
     b->mz = 1;
     b->et.clear(); b->bonds.clear(); b->spc.clear(); b->angles.clear(); b->tors.clear();
@@ -1566,13 +1593,24 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         b->zero.add(ZeroSeg{h->d_fixsp, 3LL * n}, cdiv(3LL * n, 256));
         b->pack.add(h->a_pack, cdiv((long long)h->nclu_cap * kClu, 256));
         b->com.add(h->a_pack, h->ncomgrp);
-        b->pairs.add(h->a_pair, h->nclu_cap);
+        const bool fp64 = h->p.precision == MDX_PRECISION_FP64;
+        if (fp64) {
+            // j reactions by atomics into cleared words; FP64 FURIR
+            const long long nj = (long long)h->nclu_cap * kClu * 8;
+            b->zero.add(ZeroSeg{h->d_fixj, nj}, cdiv(nj, 256));
+            b->pairs64.add(h->a_pair, h->nclu_cap);
+            b->fur64.push_back(s);
+        } else {
+            b->pairs.add(h->a_pair, h->nclu_cap);
+        }
         if (h->nspecial) b->special.add(h->a_special, cdiv(h->nspecial, 256));
-        b->rprep.add(h->a_recip, cdiv(n, 256));
-        b->sk.add(h->a_recip, h->nsgroups * h->nchunk);
-        b->skred.add(h->a_recip, cdiv(h->npk + h->nfrows, 256));
-        b->rforce.add(h->a_recip, h->a_recip.natom_blocks * h->nslice);
-        b->mz = std::max(b->mz, h->mz);
+        if (!fp64) {
+            b->rprep.add(h->a_recip, cdiv(n, 256));
+            b->sk.add(h->a_recip, h->nsgroups * h->nchunk);
+            b->skred.add(h->a_recip, cdiv(h->npk + h->nfrows, 256));
+            b->rforce.add(h->a_recip, h->a_recip.natom_blocks * h->nslice);
+            b->mz = std::max(b->mz, h->mz);
+        }
         if (h->neterm) b->et.add(h->a_et, cdiv(h->neterm, 256));
         if (h->nb) b->bonds.add(h->a_bonds, cdiv(h->nb, 256));
         if (h->nspc) b->spc.add(h->a_spc, cdiv(h->nspc, 256));
@@ -1601,7 +1639,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         ca.fsx = h->d_fs; ca.fsy = h->d_fs + n; ca.fsz = h->d_fs + 2 * n;
         ca.flx = h->d_fl; ca.fly = h->d_fl + n; ca.flz = h->d_fl + 2 * n;
         ca.recip = h->d_rpart;
-        ca.nslice = h->nslice;
+        ca.nslice = fp64 ? (h->nkv > 0 ? 1 : 0) : h->nslice;
         ca.gx = fixG; ca.gy = fixG + n; ca.gz = fixG + 2 * n;
         ca.hlx = fixHL; ca.hly = fixHL + n; ca.hlz = fixHL + 2 * n;
         ca.hbx = fixHB; ca.hby = fixHB + n; ca.hbz = fixHB + 2 * n;
@@ -1626,7 +1664,8 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         b->red.add(ReduceArgs{h->rows_total, h->d_epart, b->d_energy + (size_t)s * Q_N}, Q_N * 8);
     }
     CHK(kplan_upload(b, b->zero, maps)); CHK(kplan_upload(b, b->pack, maps)); CHK(kplan_upload(b, b->com, maps));
-    CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->special, maps));
+    CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->pairs64, maps));
+    CHK(kplan_upload(b, b->special, maps));
     CHK(kplan_upload(b, b->rprep, maps)); CHK(kplan_upload(b, b->sk, maps));
     CHK(kplan_upload(b, b->skred, maps)); CHK(kplan_upload(b, b->rforce, maps));
     CHK(kplan_upload(b, b->et, maps)); CHK(kplan_upload(b, b->bonds, maps)); CHK(kplan_upload(b, b->spc, maps));
@@ -1670,6 +1709,19 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
         }
         k_jgather<<<nb, 64, 0, st>>>(pa, pm, group);
     }
+    if (b->pairs64.nblocks()) {
+        const int nb = b->pairs64.nblocks();
+        const PairArgs *pa = b->pairs64.d_args;
+        const int2 *pm = b->pairs64.d_map;
+        const size_t l64 = sizeof(double) * (b->nmt_max + 1) * 2 * 64 + sizeof(int) * 8;
+        if (detail) {
+            if (b->soft) k_pairs64<true, true><<<nb, 64, l64, st>>>(pa, pm, group);
+            else k_pairs64<false, true><<<nb, 64, l64, st>>>(pa, pm, group);
+        } else {
+            if (b->soft) k_pairs64<true, false><<<nb, 64, 0, st>>>(pa, pm, group);
+            else k_pairs64<false, false><<<nb, 64, 0, st>>>(pa, pm, group);
+        }
+    }
     mark(2);
     if (b->special.nblocks()) {
         if (detail) k_special<true><<<b->special.nblocks(), 256, 0, st>>>(b->special.d_args, b->special.d_map, group);
@@ -1688,6 +1740,15 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
         if (b->mz <= 16) launch(b, b->rforce, 256, 0, k_recip_force<16>);
         else if (b->mz <= 32) launch(b, b->rforce, 256, 0, k_recip_force<32>);
         else launch(b, b->rforce, 256, 0, k_recip_force<48>);
+        // FP64 precision mode: FURIR in FP64 into the force slice and the
+        // first structure-factor energy row of the system
+        for (int s : b->fur64) {
+            mdx_engine *h = b->sys[s];
+            const int n = h->n;
+            if (h->nkv > 0)
+                fur_launch(h->p, h->kbox, n, h->nkv, h->d_x, h->d_y, h->d_z, h->d_q, h->d_rpart, h->d_rpart + n,
+                           h->d_rpart + 2 * (size_t)n, nullptr, erow(h, h->row_sk), h->d_furws, st);
+        }
         mark(7);
         launch(b, b->et, 256, 0, k_eterms);
     } else {
@@ -1737,10 +1798,12 @@ mdx_status batch_prepare(mdx_batch *b, int group, bool detail) {
     if (stale) {
         // launch shapes depend on capacities only; a new box (k set) or a
         // regrown capacity changes the maps and the captured graphs
-        std::vector<int> before = {b->pairs.nblocks(), b->sk.nblocks(), b->rforce.nblocks(), b->skred.nblocks(),
+        std::vector<int> before = {b->pairs.nblocks(), b->pairs64.nblocks(), b->sk.nblocks(), b->rforce.nblocks(),
+                                   b->skred.nblocks(),
                                    b->zero.nblocks(), b->mz, b->nmt_max, b->nlj_max};
         CHK(plan_fill(b, true));
-        std::vector<int> after = {b->pairs.nblocks(), b->sk.nblocks(), b->rforce.nblocks(), b->skred.nblocks(),
+        std::vector<int> after = {b->pairs.nblocks(), b->pairs64.nblocks(), b->sk.nblocks(), b->rforce.nblocks(),
+                                  b->skred.nblocks(),
                                   b->zero.nblocks(), b->mz, b->nmt_max, b->nlj_max};
         if (before != after) drop_graphs(b);
     }
@@ -2173,6 +2236,12 @@ mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) 
         h->err = "mdx_autotune before mdx_build_list";
         return MDX_ERR_STATE;
     }
+    if (h->p.precision == MDX_PRECISION_FP64) {
+        // the FP64 reciprocal sum has no tunable shape
+        if (chunk_out) *chunk_out = h->chunk;
+        if (nslice_out) *nslice_out = h->nslice;
+        return MDX_OK;
+    }
     mdx_batch *b = h->self;
     const bool timing0 = b->timing;
     b->timing = true;
@@ -2262,7 +2331,7 @@ mdx_status mdx_furir_device(const mdx_params *p, int32_t natoms, const double *x
     if (nkv > 4000000) return MDX_ERR_NKV;
     if ((int64_t)fur_layout(natoms, (int)nkv).total > workspace_bytes) return MDX_ERR_ARG;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    fur_launch(*p, natoms, nkv, x, y, z, q, fx, fy, fz, energy4, static_cast<char *>(workspace), st);
+    fur_launch(*p, nullptr, natoms, nkv, x, y, z, q, fx, fy, fz, energy4, nullptr, static_cast<char *>(workspace), st);
     return hipGetLastError() == hipSuccess ? MDX_OK : MDX_ERR_HIP;
 }
 
@@ -2305,7 +2374,7 @@ mdx_status mdx_furir_device_noalloc(const mdx_params *p, int32_t natoms, const d
     if ((long long)fur_layout(natoms, (int)nkv).total > MDX_NOALLOC_SCRATCH_BYTES) return MDX_ERR_LIMIT;
     char *ws = nullptr;
     CHK(scratch_of_current_device(&ws));
-    fur_launch(*p, natoms, nkv, x, y, z, q, fx, fy, fz, energy4, ws, reinterpret_cast<hipStream_t>(stream));
+    fur_launch(*p, nullptr, natoms, nkv, x, y, z, q, fx, fy, fz, energy4, nullptr, ws, reinterpret_cast<hipStream_t>(stream));
     return hipGetLastError() == hipSuccess ? MDX_OK : MDX_ERR_HIP;
 }
 
@@ -2569,7 +2638,8 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     if (!b) return MDX_OK;
     if (b->stream) hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->com); kplan_free(b->pairs); kplan_free(b->special);
+    kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->com); kplan_free(b->pairs); kplan_free(b->pairs64);
+    kplan_free(b->special);
     kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
     kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->spc); kplan_free(b->angles); kplan_free(b->tors);
     kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);

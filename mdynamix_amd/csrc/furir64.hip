@@ -21,7 +21,10 @@
 //                 over the k set staged through LDS (k order)
 //   k_fur_esum    one block: energy and virial totals (block order)
 // Phases are 2 pi (n . f) with fractional coordinates f = r / L, reduced
-// to [-1/2, 1/2] turns before sincospi.
+// to [-1/2, 1/2] turns before sincospi.  Inside FURIR's 5 % window of box
+// changes the k set stays the one enumerated at the old box while the
+// vectors 2 pi n / L take the current one (forces.f:1250-1276): the
+// enumeration box (klx, kly, klz) is given apart from the current box.
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -34,9 +37,11 @@ struct FurArgs {
     int kmax, kmay, kmaz;          // cube of candidate vectors (INIFUR KMAX..)
     double cutk2;                  // 4 alpha^2 FEXP
     double lx, ly, lz, alp2, cfur; // box, alpha^2, 4 pi COULF / V
+    double klx, kly, klz;          // box of the k-set enumeration (INIFUR)
     const double *x, *y, *z, *q;
     double *fx, *fy, *fz;          // forces out
-    double *e4;                    // QPE, VIRX, VIRY, VIRZ out
+    double *e4;                    // QPE, VIRX, VIRY, VIRZ out (or null)
+    double *qrow;                  // or null: an engine energy row (Q_N slots) to fill
     int4 *kv;                      // [nkv]
     double2 *spart;                // [nchunk][nkv]
     double2 *ak;                   // [nkv]
@@ -70,7 +75,7 @@ __global__ __launch_bounds__(1024) void k_fur_kset(FurArgs a) {
             const bool half = ix > 0 || iy > 0 || (iy == 0 && iz > 0);
             if (half) {
                 const double rxv = twopi * ix, ryv = twopi * iy, rzv = twopi * iz;
-                const double bx = rxv / a.lx, by = ryv / a.ly, bz = rzv / a.lz;
+                const double bx = rxv / a.klx, by = ryv / a.kly, bz = rzv / a.klz;
                 double rk2 = bx * bx + by * by;
                 rk2 = rk2 + bz * bz;
                 keep = rk2 <= a.cutk2;
@@ -219,7 +224,14 @@ __global__ __launch_bounds__(kFurT) void k_fur_esum(FurArgs a) {
             for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x < 4) a.e4[threadIdx.x] = red[threadIdx.x][0];
+    if (a.e4 && threadIdx.x < 4) a.e4[threadIdx.x] = red[threadIdx.x][0];
+    if (a.qrow && threadIdx.x == 0) {
+        for (int q = 0; q < Q_N; ++q) a.qrow[q] = 0.0;
+        a.qrow[Q_PELS1] = red[0][0];
+        a.qrow[Q_VIRX] = red[1][0];
+        a.qrow[Q_VIRY] = red[2][0];
+        a.qrow[Q_VIRZ] = red[3][0];
+    }
 }
 
 // Workspace layout of one call (byte offsets, 256-aligned).

@@ -201,6 +201,7 @@ struct PairArgs {
     float coulf_f;               // coulomb factor (FP32, forces)
     double coulf;                // coulomb factor (FP64, energies)
     const float2 *lj;            // nlj x nlj: (6 A6, 12 B12)
+    const double2 *lj64;         // nlj x nlj: (A6, B12) in FP64 (k_pairs64)
     int nlj;
     const int *clu_atom;
     float alpha;
@@ -216,6 +217,7 @@ struct PairArgs {
     const double *x, *y, *z;     // caller coordinates (FP64)
     Box box;
     float raex;
+    double alphad, raexd;        // alpha and RAEX in FP64 (k_pairs64)
     int soft;                    // lambda-coupled types present
     float lx, ly, lz, ilx, ily, ilz;
     double *fs_x, *fs_y, *fs_z;  // short-class pair forces (per atom)

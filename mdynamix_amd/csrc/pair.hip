@@ -152,6 +152,30 @@ struct PairAcc {
 // PSR141), as LOCALF/FORCES do (forces.f:945-974, 1155-1183).  Each lane
 // keeps its own FP64 buckets in LDS (bucket = type of the j molecule, or
 // the intramolecular bucket), so the sums stay in a fixed order.
+// Detail row of one i-cluster from the per-lane buckets: bucket b of lane
+// L holds the Coulomb (2 b) and LJ (2 b + 1) energies of lane L's pairs
+// whose j molecule has type b, or (b = nmt) lies in the i molecule; each
+// output column sums its (lane, bucket) entries in lane order.  The pair
+// type index is symmetric, so each pair counts once in its bucket.
+__device__ inline void detail_out(const PairArgs &a, int ci, const double *s_det, const int *s_ti) {
+    for (int o = threadIdx.x; o < a.ndet; o += 64) {
+        double sum = 0.0;
+        for (int L = 0; L < 64; ++L) {
+            const int ti = s_ti[L >> 3];
+            if (o < 2 * a.ntp) {
+                const int k = o < a.ntp ? 0 : 1, mtr = o < a.ntp ? o : o - a.ntp;
+                for (int b = 0; b < a.nmt; ++b)
+                    if (a.type_pair[ti * a.nmt + b] == mtr) sum += s_det[(b * 2 + k) * 64 + L];
+            } else {
+                const int r = o - 2 * a.ntp;
+                const int k = r < a.nmt ? 0 : 1, t = r < a.nmt ? r : r - a.nmt;
+                if (ti == t) sum += s_det[(a.nmt * 2 + k) * 64 + L];
+            }
+        }
+        a.dpart[(size_t)ci * a.ndet + o] = sum;
+    }
+}
+
 template <bool SOFT, bool DETAIL>
 __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, double *s_acc, int group) {
     const int lane = threadIdx.x;
@@ -418,27 +442,10 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
         o[Q_NPL] = q[13];
     }
     if (DETAIL) {
-        // bare Coulomb buckets per unit q_i C -> energies; then each output
-        // column sums its (lane, bucket) entries in lane order; the pair
-        // type index is symmetric, so each pair counts once in its bucket
+        // bare Coulomb buckets per unit q_i C -> energies
         for (int b = 0; b <= a.nmt; ++b) s_det[(b * 2) * 64 + lane] *= qc;
         __syncthreads();
-        for (int o = lane; o < a.ndet; o += 64) {
-            double sum = 0.0;
-            for (int L = 0; L < 64; ++L) {
-                const int ti = s_ti[L >> 3];
-                if (o < 2 * a.ntp) {
-                    const int k = o < a.ntp ? 0 : 1, mtr = o < a.ntp ? o : o - a.ntp;
-                    for (int b = 0; b < a.nmt; ++b)
-                        if (a.type_pair[ti * a.nmt + b] == mtr) sum += s_det[(b * 2 + k) * 64 + L];
-                } else {
-                    const int r = o - 2 * a.ntp;
-                    const int k = r < a.nmt ? 0 : 1, t = r < a.nmt ? r : r - a.nmt;
-                    if (ti == t) sum += s_det[(a.nmt * 2 + k) * 64 + L];
-                }
-            }
-            a.dpart[(size_t)ci * a.ndet + o] = sum;
-        }
+        detail_out(a, ci, s_det, s_ti);
     }
 }
 
@@ -455,6 +462,178 @@ template __global__ void k_pairs<false, false>(const PairArgs *, const int2 *, i
 template __global__ void k_pairs<true, false>(const PairArgs *, const int2 *, int);
 template __global__ void k_pairs<false, true>(const PairArgs *, const int2 *, int);
 template __global__ void k_pairs<true, true>(const PairArgs *, const int2 *, int);
+
+// FP64 precision mode: the pairs of the same list, each in FP64 from the
+// caller's coordinates as FORCES and LOCALF form it (forces.f:1078-1183,
+// 870-990): the one-shift minimum image of the atom pair, r = sqrt(r^2),
+// the A&S polynomial times exp(-(alpha r)^2), the soft-core shift RAE of
+// lambda-coupled pairs in 1/r but not in the erfc argument, LJ only where
+// B12 != 0.  Lane (i_slot * 8 + j_slot) as in pairs_body, one list entry
+// at a time (front segment, then the long-class entries); the i forces
+// are summed per lane in FP64 in list order; the j reactions are summed
+// over the 8 i slots by a butterfly of FP64 shuffles and added into the j
+// slots' 64-bit fixed-point words (PairArgs::jf, cleared before the call)
+// with integer atomics, whose totals do not depend on the order of
+// arrival.  k_jgather does not run in this mode.
+template <bool SOFT, bool DETAIL>
+__device__ inline void pairs64_body(const PairArgs &a, int ci, double *s_det, int group) {
+    const int lane = threadIdx.x;
+    int *s_ti = reinterpret_cast<int *>(s_det + (a.nmt + 1) * 2 * 64);
+    if (DETAIL) {
+        for (int b = 0; b < (a.nmt + 1) * 2; ++b) s_det[b * 64 + lane] = 0.0;
+        if (ci >= a.nclu) {
+            for (int o = lane; o < a.ndet; o += 64) a.dpart[(size_t)ci * a.ndet + o] = 0.0;
+            return;
+        }
+    }
+    __syncthreads();
+    if (ci >= a.nclu) return;
+    const int il = lane >> 3, jl = lane & 7;
+    const int ai = a.clu_atom[ci * kClu + il];
+    const int infoi = a.aux[ci * kClu + il].x;
+    const double xi = ai >= 0 ? a.x[ai] : 0.0, yi = ai >= 0 ? a.y[ai] : 0.0, zi = ai >= 0 ? a.z[ai] : 0.0;
+    const double qic = ai >= 0 ? a.q[ai] * a.coulf : 0.0;
+    const double2 *ljrow = a.lj64 + (size_t)(infoi & 0xffff) * a.nlj;
+    int moli = -1;
+    if (DETAIL) {
+        moli = ai >= 0 ? a.atom_mol[ai] : -1;
+        if (jl == 0) s_ti[il] = ai >= 0 ? a.atom_moltype[ai] : 0;
+    }
+    // per lane: forces and virial projections (short, long), energies
+    // Coulomb short/long, LJ short/long, LJ virial short/long
+    double Fs[3] = {0, 0, 0}, Fl[3] = {0, 0, 0}, Vs[3] = {0, 0, 0}, Vl[3] = {0, 0, 0};
+    double E[6] = {0, 0, 0, 0, 0, 0};
+    int cnt_s = 0, cnt_l = 0;
+    const int ns = a.nshort[ci], n = a.nlist[ci];
+    const PairEntry *L = a.list + (size_t)ci * a.list_cap;
+    const int nk = group == 1 ? ns : n;
+    for (int k = 0; k < nk; ++k) {
+        const bool front = k < ns;
+        const PairEntry e = L[front ? k : a.list_cap - 1 - (k - ns)];
+        const int cj = (int)((unsigned)e.cj & ~kEntryWrap);
+        unsigned long long m = e.mask;
+        if (group == 1) m &= e.mshort;
+        else if (group == 2) m &= ~e.mshort;
+        const int sj = cj * kClu + jl;
+        const bool isshort = (e.mshort >> lane) & 1ull;
+        double d[3] = {0.0, 0.0, 0.0}, force = 0.0;
+        if ((m >> lane) & 1ull) {
+#pragma clang fp contract(off)
+            const int aj = a.clu_atom[sj];
+            d[0] = xi - a.x[aj];
+            d[1] = yi - a.y[aj];
+            d[2] = zi - a.z[aj];
+            min_image(d[0], d[1], d[2], a.box);
+            double rr = d[0] * d[0] + d[1] * d[1];
+            rr = rr + d[2] * d[2];
+            if (rr <= a.rc2d) {
+                const int infoj = a.aux[sj].x;
+                const double r1 = sqrt(rr), r1i = 1.0 / r1;
+                const double rae = (SOFT && ((infoi >> 16) & (infoj >> 16)) == 0) ? a.raexd : 0.0;
+                const double r1ei = 1.0 / (r1 + rae);
+                const double ar = a.alphad * r1;
+                const double t = 1.0 / (1.0 + kB1 * ar);
+                const double ex = exp(-(ar * ar));
+                const double erfct = ((((kA5 * t + kA4) * t + kA3) * t + kA2) * t + kA1) * t * ex;
+                const double ee1 = qic * a.q[aj] * r1ei;          // bare Coulomb
+                const double ees = ee1 * erfct;
+                force = ee1 * (kTOTPI * a.alphad * ex + erfct * r1ei) * r1i;
+                const double2 lj = ljrow[infoj & 0xffff];        // (A6, B12)
+                double esr = 0.0, vr = 0.0;
+                if (lj.y != 0.0) {
+                    const double r2i = r1ei * r1ei, r6i = r2i * r2i * r2i;
+                    esr = (lj.x + lj.y * r6i) * r6i;
+                    const double fsr = (6.0 * lj.x + 12.0 * lj.y * r6i) * r6i * r1ei;
+                    force = force + fsr * r1i;
+                    vr = fsr * r1;
+                }
+                const int c = isshort ? 0 : 1;
+                E[c] += ees;
+                E[2 + c] += esr;
+                E[4 + c] += vr;
+                double *F = isshort ? Fs : Fl, *V = isshort ? Vs : Vl;
+                for (int q = 0; q < 3; ++q) {
+                    F[q] += force * d[q];
+                    V[q] += force * (d[q] * d[q]);
+                }
+                if (isshort) ++cnt_s;
+                else ++cnt_l;
+                if (DETAIL) {
+                    const int b = a.atom_mol[aj] == moli ? a.nmt : a.atom_moltype[aj];
+                    s_det[(b * 2) * 64 + lane] += ee1;
+                    s_det[(b * 2 + 1) * 64 + lane] += esr;
+                }
+            }
+        }
+        // reactions on the j slot: minus the force on i, summed over i slots
+        double rl[3], rs[3];
+        for (int q = 0; q < 3; ++q) {
+            const double v = -force * d[q];
+            rs[q] = isshort ? v : 0.0;
+            rl[q] = isshort ? 0.0 : v;
+        }
+        for (int off = 8; off < 64; off <<= 1)
+            for (int q = 0; q < 3; ++q) {
+                rl[q] += __shfl_xor(rl[q], off);
+                if (front) rs[q] += __shfl_xor(rs[q], off);
+            }
+        if (il == 0) {
+            unsigned long long *w = a.jf + (size_t)sj * 8;
+            for (int q = 0; q < 3; ++q) {
+                const long long vl = __double2ll_rn(rl[q] * a.fscale);
+                if (vl != 0) atomicAdd(w + 3 + q, (unsigned long long)vl);
+                if (front) {
+                    const long long vs = __double2ll_rn(rs[q] * a.fscale);
+                    if (vs != 0) atomicAdd(w + q, (unsigned long long)vs);
+                }
+            }
+        }
+    }
+    for (int off = 1; off < 8; off <<= 1)
+        for (int q = 0; q < 3; ++q) {
+            Fs[q] += __shfl_xor(Fs[q], off);
+            Fl[q] += __shfl_xor(Fl[q], off);
+        }
+    if (jl == 0 && ai >= 0) {
+        if (a.fs_x) { a.fs_x[ai] = Fs[0]; a.fs_y[ai] = Fs[1]; a.fs_z[ai] = Fs[2]; }
+        if (a.fl_x) { a.fl_x[ai] = Fl[0]; a.fl_y[ai] = Fl[1]; a.fl_z[ai] = Fl[2]; }
+    }
+    double q[14] = {E[0], E[1], E[2], E[3], E[4], E[5], Vs[0], Vs[1], Vs[2], Vl[0], Vl[1], Vl[2],
+                    (double)cnt_s, (double)cnt_l};
+    for (int off = 1; off < 64; off <<= 1)
+        for (int k = 0; k < 14; ++k) q[k] += __shfl_xor(q[k], off);
+    if (lane == 0) {
+        double *o = a.epart + (size_t)ci * Q_N;
+        for (int k = 0; k < Q_N; ++k) o[k] = 0.0;
+        o[Q_PELS2] = q[0];
+        o[Q_PELS1] = q[1];
+        o[Q_PE2] = q[2];
+        o[Q_PE1] = q[3];
+        o[Q_VIR2] = q[4];
+        o[Q_VIR1] = q[5];
+        o[Q_VIRFX] = q[6]; o[Q_VIRFY] = q[7]; o[Q_VIRFZ] = q[8];
+        o[Q_VIRX] = q[9]; o[Q_VIRY] = q[10]; o[Q_VIRZ] = q[11];
+        o[Q_NPS] = q[12];
+        o[Q_NPL] = q[13];
+    }
+    if (DETAIL) {
+        __syncthreads();
+        detail_out(a, ci, s_det, s_ti);
+    }
+}
+
+template <bool SOFT, bool DETAIL>
+__global__ __launch_bounds__(64) void k_pairs64(const PairArgs *__restrict__ args, const int2 *__restrict__ map,
+                                                int group) {
+    extern __shared__ double s_det64[];   // detail buckets
+    const int2 m = map[blockIdx.x];
+    pairs64_body<SOFT, DETAIL>(args[m.x], m.y, s_det64, group);
+}
+
+template __global__ void k_pairs64<false, false>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs64<true, false>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs64<false, true>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs64<true, true>(const PairArgs *, const int2 *, int);
 
 // j reactions per atom: one wavefront per j-cluster, lanes 0-47 (48-63
 // idle), each on one record float: the front-segment records (two units:

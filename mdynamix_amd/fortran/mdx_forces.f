@@ -15,6 +15,8 @@
 *  read (ZEROFS/CM_ADD* semantics, service.f:397-453, scalar.f:28-116).
 *  A nonzero engine status is reported and ends the run through FINAL,
 *  as the reference does on its own errors (forces.f:2188-2194).
+*  MDX_PRECISION=fp64 in the environment selects the engine's FP64 mode
+*  (MDX_PRECISION_FP64) instead of the default mixed-precision path.
 *
 *  Not supported here (the engine returns MDX_ERR_ARG or the shim stops):
 *  reaction field, hexagonal/octahedral cells, fixed molecules, restraints
@@ -55,6 +57,7 @@
         real(c_double) :: coulomb_factor
         integer(c_int32_t) :: list_mode, ee_enabled
         real(c_double) :: ee_raex, ee_self_scale
+        integer(c_int32_t) :: precision
       end type
       type, bind(C) :: mdx_energies_t
         real(c_double) :: pe1, pe2, pels1, pels2, spe, pint
@@ -173,6 +176,7 @@
       include "common_inc.f"
       integer(c_int) IST
       integer NEX, K, L, IA, JA, NTT
+      character*16 PRECV
       N=NSTOT
       allocate(amtyp(N),amol(N),aljt(N),aq(N),aq0(N),am(N))
       do I=1,N
@@ -388,6 +392,10 @@
       prm%ee_enabled=0
       prm%ee_raex=0.d0
       prm%ee_self_scale=1.d0
+C  MDX_PRECISION=fp64 in the environment: the engine's FP64 mode
+      prm%precision=0
+      call get_environment_variable('MDX_PRECISION',PRECV)
+      if(PRECV(1:4).eq.'fp64'.or.PRECV(1:4).eq.'FP64')prm%precision=1
       if(LMDEE)then
         prm%ee_enabled=1
         prm%ee_raex=RAEX

@@ -33,8 +33,9 @@ def test_version_string():
 
 
 def test_struct_layout():
-    # mdx_params: 3 box + rcut + rshort + skin + alpha + fexp + coulf (9 doubles), 2 ints, 2 doubles
-    assert C.sizeof(abi.MdxParams) == 9 * 8 + 2 * 4 + 2 * 8
+    # mdx_params: 3 box + rcut + rshort + skin + alpha + fexp + coulf (9 doubles), 2 ints, 2 doubles,
+    # precision (int, padded to 8 bytes)
+    assert C.sizeof(abi.MdxParams) == 9 * 8 + 2 * 4 + 2 * 8 + 8
     # energies: 17 doubles, 2 int64, 2*MAX_TYPEPAIRS + 3*MAX_MOLTYPES doubles
     assert C.sizeof(abi.MdxEnergies) == 8 * (17 + 2 + 2 * abi.MDX_MAX_TYPEPAIRS + 3 * abi.MDX_MAX_MOLTYPES + 2)
 
