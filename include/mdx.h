@@ -85,8 +85,10 @@ enum {
 
 /* Optional work in mdx_compute. */
 enum {
-    MDX_WANT_DETAIL = 1   /* per type-pair / per type energy breakdown
+    MDX_WANT_DETAIL = 1,  /* per type-pair / per type energy breakdown
                              (POTE1, POTL1, PES141, PSR141, SPEE)             */
+    MDX_WANT_TERMS = 2    /* per-template bonded sums BR/BE, AR/AE, TR/TE
+                             (mdx_get_terms; fast-group calls only)          */
 };
 
 #define MDX_MAX_MOLTYPES 16
@@ -166,6 +168,11 @@ typedef struct mdx_topology {
     int32_t nspc;
     const int32_t *spc_atom;      /* [3 nspc] */
     const double  *spc_par;       /* [12 nspc] */
+    /* [nspc] or NULL: bond template (parameter index, IADB - 1) of each
+       record's R_OH1 bond; R_OH2 and R_HH are the next two templates.
+       STRBND adds its bond lengths and energies there (BR/BE,
+       forces.f:655-663); NULL leaves them out of mdx_get_terms. */
+    const int32_t *spc_bond;
 } mdx_topology;
 
 /* Run parameters (COMMON /SIZES/, /ENERGY/ COULF, /MDEE/ RAEX). */
@@ -257,6 +264,19 @@ mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want,
 /* Same call with device output arrays of natoms (forces stay in HBM). */
 mdx_status mdx_compute_device(mdx_engine *h, int32_t group, int32_t want,
                               double *dfx, double *dfy, double *dfz, mdx_energies *e);
+
+/* Per-template bonded sums of the last force call made with MDX_WANT_TERMS
+ * that evaluated the fast group (MDX_FORCES_ALL or MDX_FORCES_FAST), as
+ * GETBND, STRBND, GETANG and GETTRS leave them for CM_ADDF / CM_ADDLA
+ * (forces.f:119-182, 655-663, 232-273, 336-447; scalar.f:42-51, 95-106):
+ * over the terms of template m, each divided by NSPEC of its molecule type,
+ *   br[m] bond lengths, be[m] bond energies (STRBND records included),
+ *   ar[m] bend angles (radians), ae[m] their energies,
+ *   tr[m] |dihedral angle| (radians), te[m] their energies
+ * (dihedrals with collinear bonds contribute nothing).  br/be hold
+ * nbondparams values, ar/ae nangleparams, tr/te ntorsparams; any may be
+ * NULL.  Zeros before the first such call.  Sums in a fixed order. */
+mdx_status mdx_get_terms(mdx_engine *h, double *br, double *be, double *ar, double *ae, double *tr, double *te);
 
 /* Forces of the last call (F, G, H) to caller-owned host arrays (any may be
  * NULL), e.g. after a batched call. */

@@ -36,6 +36,7 @@ MDX_MD_PIMD = 2                      # PIMD: path-integral beads of a ring batch
 MDX_FORCES_FAST = 1
 MDX_FORCES_SLOW = 2
 MDX_WANT_DETAIL = 1
+MDX_WANT_TERMS = 2
 MDX_MAX_MOLTYPES = 16
 MDX_MAX_TYPEPAIRS = MDX_MAX_MOLTYPES * (MDX_MAX_MOLTYPES + 1) // 2 + 1
 
@@ -63,7 +64,7 @@ class MdxTopology(C.Structure):
         ("tors_param", _pi), ("ntorsparams", C.c_int32), ("tp_kind", _pi), ("tp_mult", _pi),
         ("tp_k", _pd), ("tp_delta", _pd), ("tp_k1", _pd), ("tp_k2", _pd), ("tp_k3", _pd),
         ("tp_k4", _pd), ("tp_k5", _pd),
-        ("nspc", C.c_int32), ("spc_atom", _pi), ("spc_par", _pd),
+        ("nspc", C.c_int32), ("spc_atom", _pi), ("spc_par", _pd), ("spc_bond", _pi),
     ]
 
 
@@ -210,6 +211,8 @@ class TopologyArrays:
     # fSPC water (STRBND): per molecule atoms O, H1, H2 and 12 parameters
     spc_atom: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3), np.int32))
     spc_par: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 12)))
+    # bond template (parameter index) of each record's R_OH1 bond, or None
+    spc_bond: Optional[np.ndarray] = None
     _keep: Optional[list] = None
 
     @property
@@ -266,6 +269,7 @@ class TopologyArrays:
         s.nspc = int(len(self.spc_atom))
         s.spc_atom = ptr(np.asarray(self.spc_atom).ravel(), "i")
         s.spc_par = ptr(np.asarray(self.spc_par).ravel(), "d")
+        s.spc_bond = ptr(self.spc_bond, "i") if self.spc_bond is not None else None
         self._keep = keep
         return s
 

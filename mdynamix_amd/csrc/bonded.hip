@@ -110,6 +110,7 @@ __device__ inline void bonds_body(const BondArgs &t, int blk) {
         double fj[3];
         for (int c = 0; c < 3; ++c) fj[c] = -g * d[c];
         v[0] = energy;
+        t.tv[n] = make_double2(len, energy);
         v[1] = -slope * len;
         for (int c = 0; c < 3; ++c) v[2 + c] = d[c] * fj[c];
         unsigned long long *F[3] = {a.fx, a.fy, a.fz};
@@ -169,7 +170,11 @@ __device__ inline void strbnd_body(const SpcArgs &t, int blk) {
         fa = (fa - eee * sh - ggg * sb) / ass;
         fb = (fb - eee * sh - ggg * sa) / bss;
         const double fc = (-2.0 * ccc * sh - eee * sa - eee * sb) / css;
-        v[0] = wa + wb + (ccc * sh * sh + ggg * sa * sb);
+        const double wc = ccc * sh * sh + ggg * sa * sb;
+        v[0] = wa + wb + wc;
+        t.tv[3 * n] = make_double2(ass, wa);
+        t.tv[3 * n + 1] = make_double2(bss, wb);
+        t.tv[3 * n + 2] = make_double2(css, wc);
         if (lmove) {
             const double fax = fa * ax, fay = fa * ay, faz = fa * az;
             const double fbx = fb * bx, fby = fb * by, fbz = fb * bz;
@@ -222,6 +227,7 @@ __device__ inline void angles_body(const AngleArgs &t, int blk) {
         const double theta = acos(c);
         const double dev = theta - t.theta0[m];
         v[0] = t.k[m] * dev * dev;
+        t.tv[n] = make_double2(theta, v[0]);
         const double h = 2.0 * t.k[m] * dev / sqrt(1.0 - c * c);   // dE/dtheta / sin theta
         double fi[3], fk[3];
         for (int q = 0; q < 3; ++q) {
@@ -283,6 +289,7 @@ __device__ inline void tors_body(const TorsArgs &t, int blk) {
         cross(b1, b2, mv);
         cross(b2, b3, nv);
         const double mm = dot(mv, mv), nn = dot(nv, nv), bb = dot(b2, b2);
+        double2 term = make_double2(0.0, 0.0);
         if (mm * nn > 1.0e-10) {
             const double lb = sqrt(bb);
             const double phi = atan2(lb * dot(b1, nv), dot(mv, nv));
@@ -311,6 +318,7 @@ __device__ inline void tors_body(const TorsArgs &t, int blk) {
                 dedp = 2.0 * t.k[m] * w;
             }
             v[0] = energy;
+            term = make_double2(fabs(phi), energy);
             const double s1 = dot(b1, b2) / bb, s3 = dot(b3, b2) / bb;
             double fi[3], fj[3], fk[3], fl[3];
             for (int q = 0; q < 3; ++q) {
@@ -329,6 +337,7 @@ __device__ inline void tors_body(const TorsArgs &t, int blk) {
                 v[1 + q] = -b1[q] * fi[q] + b2[q] * fk[q] + (b2[q] + b3[q]) * fl[q];
             }
         }
+        t.tv[n] = term;
     }
     double *o = a.epart + (size_t)blk * Q_N;
     zero_slots(o);
@@ -337,6 +346,27 @@ __device__ inline void tors_body(const TorsArgs &t, int blk) {
 }
 
 
+
+// Per-template sums of the bonded terms (BR/BE, AR/AE, TR/TE): one block
+// per template, each thread sums its strided slots in FP64, then the block
+// sums in the fixed tree order of block_reduce_store.
+__device__ inline void term_reduce_body(const TermArgs &a, int m) {
+    double q[2] = {0.0, 0.0};
+    for (int p = a.start[m] + threadIdx.x; p < a.start[m + 1]; p += blockDim.x) {
+        const int s = a.slot[p];
+        const double2 v = a.tv[s];
+        q[0] += v.x * a.w[s];
+        q[1] += v.y * a.w[s];
+    }
+    const int slots[2] = {2 * m, 2 * m + 1};
+    block_reduce_store<2>(q, slots, a.out);
+}
+
+__global__ __launch_bounds__(256) void k_term_reduce(const TermArgs *__restrict__ args,
+                                                     const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    term_reduce_body(args[m.x], m.y);
+}
 
 __device__ inline double fixd(const unsigned long long *p, int i, double inv) {
     return (double)(long long)p[i] * inv;

@@ -75,6 +75,12 @@ struct mdx_engine {
     int2 *d_et_pairs = nullptr;
     // bonded
     int nb = 0, na = 0, nt = 0, nspc = 0;
+    // per-template sums of the bonded terms (mdx_get_terms)
+    int ntm = 0, nbtm = 0, natm = 0, nttm = 0;   // templates: all, bond, angle, dihedral
+    double2 *d_tv = nullptr;                      // per term (value, energy)
+    int *d_tm_start = nullptr, *d_tm_slot = nullptr;
+    double *d_tm_w = nullptr, *d_terms = nullptr;
+    TermArgs a_terms{};
     std::vector<void *> owned;
     // argument structs of the force kernels, refreshed whenever a list
     // rebuild or box change moves a pointer or count (version counts those)
@@ -225,6 +231,7 @@ struct mdx_batch {
     KPlan<SpcArgs> spc;
     KPlan<AngleArgs> angles;
     KPlan<TorsArgs> tors;
+    KPlan<TermArgs> terms;
     KPlan<SpringArgs> springs;
     KPlan<CombineArgs> comb;
     KPlan<ReduceArgs> red;
@@ -1154,6 +1161,57 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
     if ((s = upload(h, &dp, t->tp_k3, t->ntorsparams))) return fail(s); h->a_tors.k3 = dp;
     if ((s = upload(h, &dp, t->tp_k4, t->ntorsparams))) return fail(s); h->a_tors.k4 = dp;
     if ((s = upload(h, &dp, t->tp_k5, t->ntorsparams))) return fail(s); h->a_tors.k5 = dp;
+    // per-template sums (BR/BE, AR/AE, TR/TE): term slots are the bonds,
+    // the STRBND records' three bonds, the angles, the dihedrals; templates
+    // the bond, angle and dihedral parameter sets in that order; each term
+    // counts 1 / NSPEC of its molecule type (forces.f:140, 240, 345, 560)
+    {
+        std::vector<int> nspec(h->ntypes, 0);
+        for (int i = 0; i < n; ++i)
+            if (i == 0 || t->atom_mol[i] != t->atom_mol[i - 1]) ++nspec[t->atom_moltype[i]];
+        h->nbtm = t->nbondparams;
+        h->natm = t->nangleparams;
+        h->nttm = t->ntorsparams;
+        h->ntm = h->nbtm + h->natm + h->nttm;
+        const int nslot = h->nb + 3 * h->nspc + h->na + h->nt;
+        std::vector<int> tmpl(nslot, -1);
+        std::vector<double> w(nslot, 0.0);
+        auto put = [&](int slot, int tm, int atom) {
+            tmpl[slot] = tm;
+            w[slot] = 1.0 / (double)nspec[t->atom_moltype[atom]];
+        };
+        for (int k = 0; k < h->nb; ++k) put(k, t->bond_param[k], t->bond_i[k]);
+        for (int r = 0; r < h->nspc && t->spc_bond; ++r) {
+            if (t->spc_bond[r] < 0 || t->spc_bond[r] + 2 >= h->nbtm) {
+                h->err = "index out of range: spc_bond";
+                return fail(MDX_ERR_ARG);
+            }
+            for (int q = 0; q < 3; ++q) put(h->nb + 3 * r + q, t->spc_bond[r] + q, t->spc_atom[3 * r]);
+        }
+        const int a0 = h->nb + 3 * h->nspc, t0 = a0 + h->na;
+        for (int k = 0; k < h->na; ++k) put(a0 + k, h->nbtm + t->angle_param[k], t->angle_i[k]);
+        for (int k = 0; k < h->nt; ++k) put(t0 + k, h->nbtm + h->natm + t->tors_param[k], t->tors_i[k]);
+        std::vector<int> start(h->ntm + 1, 0), slot;
+        for (int k = 0; k < nslot; ++k)
+            if (tmpl[k] >= 0) ++start[tmpl[k] + 1];
+        for (int m = 0; m < h->ntm; ++m) start[m + 1] += start[m];
+        slot.assign(start[h->ntm], 0);
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int k = 0; k < nslot; ++k)
+            if (tmpl[k] >= 0) slot[fill[tmpl[k]]++] = k;
+        const std::vector<double> zeros(2 * (size_t)std::max(1, h->ntm), 0.0);
+        if ((s = dalloc(h, &h->d_tv, std::max(1, nslot))) ||
+            (s = upload(h, &h->d_tm_start, start.data(), start.size())) ||
+            (s = upload(h, &h->d_tm_slot, slot.data(), slot.size())) ||
+            (s = upload(h, &h->d_tm_w, w.data(), w.size())) ||
+            (s = upload(h, &h->d_terms, zeros.data(), zeros.size())))
+            return fail(s);
+        h->a_terms = TermArgs{h->ntm, h->d_tm_start, h->d_tm_slot, h->d_tm_w, h->d_tv, h->d_terms};
+        h->a_bonds.tv = h->d_tv;
+        h->a_spc.tv = h->d_tv + h->nb;
+        h->a_angles.tv = h->d_tv + a0;
+        h->a_tors.tv = h->d_tv + t0;
+    }
     // coordinates, COMs, energies
     if ((s = dalloc(h, &h->d_x, n)) || (s = dalloc(h, &h->d_y, n)) || (s = dalloc(h, &h->d_z, n)) ||
         (s = dalloc(h, &h->d_cx, h->nmol)) || (s = dalloc(h, &h->d_cy, h->nmol)) ||
@@ -1574,7 +1632,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     b->fur64.clear();
     b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
     b->mz = 1;
-    b->et.clear(); b->bonds.clear(); b->spc.clear(); b->angles.clear(); b->tors.clear();
+    b->et.clear(); b->bonds.clear(); b->spc.clear(); b->angles.clear(); b->tors.clear(); b->terms.clear();
     b->springs.clear(); b->comb.clear(); b->red.clear(); b->dred.clear();
     b->nmt_max = 1;
     b->nlj_max = 1;
@@ -1616,6 +1674,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         if (h->nspc) b->spc.add(h->a_spc, cdiv(h->nspc, 256));
         if (h->na) b->angles.add(h->a_angles, cdiv(h->na, 256));
         if (h->nt) b->tors.add(h->a_tors, cdiv(h->nt, 256));
+        if (h->ntm) b->terms.add(h->a_terms, h->ntm);
         if (b->ring && K > 1) {
             mdx_engine *hp = b->sys[(s + 1) % K], *hm = b->sys[(s + K - 1) % K];
             SpringArgs sp;
@@ -1669,7 +1728,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     CHK(kplan_upload(b, b->rprep, maps)); CHK(kplan_upload(b, b->sk, maps));
     CHK(kplan_upload(b, b->skred, maps)); CHK(kplan_upload(b, b->rforce, maps));
     CHK(kplan_upload(b, b->et, maps)); CHK(kplan_upload(b, b->bonds, maps)); CHK(kplan_upload(b, b->spc, maps));
-    CHK(kplan_upload(b, b->angles, maps)); CHK(kplan_upload(b, b->tors, maps));
+    CHK(kplan_upload(b, b->angles, maps)); CHK(kplan_upload(b, b->tors, maps)); CHK(kplan_upload(b, b->terms, maps));
     CHK(kplan_upload(b, b->springs, maps)); CHK(kplan_upload(b, b->comb, maps));
     CHK(kplan_upload(b, b->red, maps));
     CHK(kplan_upload(b, b->dred, maps));
@@ -1686,7 +1745,7 @@ void launch(mdx_batch *b, KPlan<T> &k, int nthreads, size_t lds, F kernel) {
 }
 
 // The kernel sequence of one force call over all systems of the batch.
-void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
+void launch_call(mdx_batch *b, int group, bool timing, bool detail, bool terms = false) {
     hipStream_t st = b->stream;
     auto mark = [&](int k) { if (timing) hipEventRecord(b->ev[k], st); };
     mark(0);
@@ -1760,6 +1819,7 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail) {
         launch(b, b->angles, 256, 0, k_angles);
         launch(b, b->tors, 256, 0, k_tors);
         launch(b, b->springs, 256, 0, k_springs);
+        if (terms) launch(b, b->terms, 256, 0, k_term_reduce);
     }
     mark(8);
     if (b->comb.nblocks()) k_combine<<<b->comb.nblocks(), 256, 0, st>>>(b->comb.d_args, b->comb.d_map, group);
@@ -1812,9 +1872,9 @@ mdx_status batch_prepare(mdx_batch *b, int group, bool detail) {
 
 // Queue one force call for every system of the batch on the batch stream
 // (no host synchronisation).  events: eager launches with stage events.
-mdx_status batch_launch(mdx_batch *b, int group, bool detail, bool events) {
+mdx_status batch_launch(mdx_batch *b, int group, bool detail, bool events, bool terms = false) {
     CHK(batch_prepare(b, group, detail));
-    if (b->use_graph && !events && !detail) {
+    if (b->use_graph && !events && !detail && !terms) {
         if (!b->exec[group]) {
             HIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             launch_call(b, group, false, false);
@@ -1823,13 +1883,13 @@ mdx_status batch_launch(mdx_batch *b, int group, bool detail, bool events) {
         }
         HIPCHK(b, hipGraphLaunch(b->exec[group], b->stream));
     } else {
-        launch_call(b, group, events, detail);
+        launch_call(b, group, events, detail, terms);
     }
     return MDX_OK;
 }
 
-mdx_status batch_compute(mdx_batch *b, int group, bool detail) {
-    CHK(batch_launch(b, group, detail, b->timing));
+mdx_status batch_compute(mdx_batch *b, int group, bool detail, bool terms = false) {
+    CHK(batch_launch(b, group, detail, b->timing, terms));
     HIPCHK(b, hipStreamSynchronize(b->stream));
     HIPCHK(b, hipGetLastError());
     if (b->timing) {
@@ -2194,10 +2254,10 @@ extern "C" {
 mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want, double *fx, double *fy, double *fz, double *gx,
                        double *gy, double *gz, double *hx, double *hy, double *hz, mdx_energies *e) {
     if (!h) return MDX_ERR_ARG;
-    const bool detail = (want & MDX_WANT_DETAIL) != 0;
+    const bool detail = (want & MDX_WANT_DETAIL) != 0, terms = (want & MDX_WANT_TERMS) != 0;
     mdx_batch *b = h->self;
     b->timing = h->timing;
-    mdx_status s = batch_compute(b, group, detail);
+    mdx_status s = batch_compute(b, group, detail, terms);
     if (s != MDX_OK) {
         h->err = b->err;
         return s;
@@ -2210,10 +2270,10 @@ mdx_status mdx_compute(mdx_engine *h, int32_t group, int32_t want, double *fx, d
 mdx_status mdx_compute_device(mdx_engine *h, int32_t group, int32_t want, double *dfx, double *dfy, double *dfz,
                               mdx_energies *e) {
     if (!h) return MDX_ERR_ARG;
-    const bool detail = (want & MDX_WANT_DETAIL) != 0;
+    const bool detail = (want & MDX_WANT_DETAIL) != 0, terms = (want & MDX_WANT_TERMS) != 0;
     mdx_batch *b = h->self;
     b->timing = h->timing;
-    mdx_status s = batch_compute(b, group, detail);
+    mdx_status s = batch_compute(b, group, detail, terms);
     if (s != MDX_OK) {
         h->err = b->err;
         return s;
@@ -2617,6 +2677,23 @@ mdx_status mdx_batch_set_ring(mdx_batch *b, double beadfac) {
     return MDX_OK;
 }
 
+mdx_status mdx_get_terms(mdx_engine *h, double *br, double *be, double *ar, double *ae, double *tr, double *te) {
+    if (!h) return MDX_ERR_ARG;
+    std::vector<double> v(2 * (size_t)std::max(1, h->ntm));
+    HIPCHK(h, hipStreamSynchronize(h->self->stream));
+    HIPCHK(h, hipMemcpy(v.data(), h->d_terms, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+    auto out = [&](double *val, double *en, int m0, int cnt) {
+        for (int m = 0; m < cnt; ++m) {
+            if (val) val[m] = v[2 * (size_t)(m0 + m)];
+            if (en) en[m] = v[2 * (size_t)(m0 + m) + 1];
+        }
+    };
+    out(br, be, 0, h->nbtm);
+    out(ar, ae, h->nbtm, h->natm);
+    out(tr, te, h->nbtm + h->natm, h->nttm);
+    return MDX_OK;
+}
+
 mdx_status mdx_batch_use_graph(mdx_batch *b, int32_t on) {
     if (!b) return MDX_ERR_ARG;
     b->use_graph = on != 0;
@@ -2625,8 +2702,8 @@ mdx_status mdx_batch_use_graph(mdx_batch *b, int32_t on) {
 
 mdx_status mdx_batch_compute(mdx_batch *b, int32_t group, int32_t want, mdx_energies *e) {
     if (!b) return MDX_ERR_ARG;
-    const bool detail = (want & MDX_WANT_DETAIL) != 0;
-    CHK(batch_compute(b, group, detail));
+    const bool detail = (want & MDX_WANT_DETAIL) != 0, terms = (want & MDX_WANT_TERMS) != 0;
+    CHK(batch_compute(b, group, detail, terms));
     if (e)
         for (size_t s = 0; s < b->sys.size(); ++s) fill_energies(b, (int)s, group, detail, e + s);
     return MDX_OK;
@@ -2641,7 +2718,7 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->com); kplan_free(b->pairs); kplan_free(b->pairs64);
     kplan_free(b->special);
     kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
-    kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->spc); kplan_free(b->angles); kplan_free(b->tors);
+    kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->spc); kplan_free(b->angles); kplan_free(b->tors); kplan_free(b->terms);
     kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);
     kplan_free(b->md_kd); kplan_free(b->md_sh); kplan_free(b->md_ke); kplan_free(b->md_th);
     kplan_free(b->md_gc); kplan_free(b->md_tr);

@@ -315,12 +315,16 @@ struct EtermArgs {
     ScatterArgs s;
 };
 
+// tv (every bonded kernel): per term (value, energy) -- the bond length,
+// bend angle, |dihedral| or, for STRBND, each record's three bonds -- for
+// the per-template sums of k_term_reduce; written on every call.
 struct BondArgs {
     int n;
     const int *bi, *bj, *bpar;
     const int *morse;
     const double *r0, *k, *d, *rho;
     ScatterArgs s;
+    double2 *tv;                   // [n]
 };
 
 // Flexible SPC water (STRBND, forces.f:530-800): one molecule per record,
@@ -331,6 +335,7 @@ struct SpcArgs {
     const int *atom;
     const double *par;
     ScatterArgs s;
+    double2 *tv;                   // [3 n]: O-H1, O-H2, H-H
 };
 
 struct AngleArgs {
@@ -338,6 +343,7 @@ struct AngleArgs {
     const int *ai, *aj, *ak, *apar;
     const double *theta0, *k;
     ScatterArgs s;
+    double2 *tv;                   // [n]
 };
 
 struct TorsArgs {
@@ -346,6 +352,19 @@ struct TorsArgs {
     const int *kind, *mult;
     const double *k, *delta, *k1, *k2, *k3, *k4, *k5;
     ScatterArgs s;
+    double2 *tv;                   // [n]
+};
+
+// Per-template sums of the bonded terms' (value, energy) pairs (BR/BE,
+// AR/AE, TR/TE): template m sums tv[slot[p]] * w[slot[p]] over p in
+// [start[m], start[m + 1]) (slots in ascending order) into out[2 m],
+// out[2 m + 1].
+struct TermArgs {
+    int ntm;
+    const int *start, *slot;
+    const double *w;               // 1 / NSPEC of the term's molecule type
+    const double2 *tv;
+    double *out;                   // [2 ntm]
 };
 
 // Path-integral ring: harmonic springs to the neighbouring beads

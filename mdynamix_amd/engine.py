@@ -62,6 +62,7 @@ def load_library() -> C.CDLL:
     L.mdx_compute_device.argtypes = [pv, C.c_int32, C.c_int32, pv, pv, pv, pv]
     L.mdx_stage_times.argtypes = [pv, pd, C.c_int32]
     L.mdx_get_forces.argtypes = [pv] + [pd] * 9
+    L.mdx_get_terms.argtypes = [pv] + [pd] * 6
     L.mdx_set_timing.argtypes = [pv, C.c_int32]
     L.mdx_memory_bytes.argtypes = [pv, C.POINTER(C.c_int64)]
     L.mdx_batch_create.argtypes = [C.POINTER(pv), C.c_int32, C.POINTER(pv)]
@@ -87,7 +88,7 @@ def load_library() -> C.CDLL:
     L.mdx_batch_pimd_init.argtypes = [pv, C.POINTER(abi.MdxPimdParams)]
     L.mdx_batch_pimd_chains.argtypes = [pv, pd]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
-                 "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_list_entries", "mdx_get_list", "mdx_compute",
+                 "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_list_entries", "mdx_get_list", "mdx_compute", "mdx_get_terms",
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
@@ -242,6 +243,17 @@ class ForceEngine:
 
     def set_kspace_shape(self, chunk: int, nslice: int) -> None:
         self._check(self.lib.mdx_set_kspace_shape(self._h, int(chunk), int(nslice)))
+
+    def terms(self) -> Dict[str, np.ndarray]:
+        """Per-template bonded sums of the last call made with
+        MDX_WANT_TERMS: br/be (bond templates), ar/ae (angle), tr/te
+        (dihedral), as GETBND/STRBND/GETANG/GETTRS leave BR..TE."""
+        t = self.topo
+        nb, na, nt = len(t.bp_r0), len(t.ap_k), len(t.tp_kind)
+        out = {k: np.zeros(max(m, 1)) for k, m in (("br", nb), ("be", nb), ("ar", na), ("ae", na),
+                                                    ("tr", nt), ("te", nt))}
+        self._check(self.lib.mdx_get_terms(self._h, *[as_ptr(out[k]) for k in ("br", "be", "ar", "ae", "tr", "te")]))
+        return {k: v[:m] for (k, v), m in zip(out.items(), (nb, nb, na, na, nt, nt))}
 
     def get_forces(self):
         """(F, G, H) of the last force call, e.g. after a batched call."""

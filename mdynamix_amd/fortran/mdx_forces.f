@@ -18,10 +18,13 @@
 *  MDX_PRECISION=fp64 in the environment selects the engine's FP64 mode
 *  (MDX_PRECISION_FP64) instead of the default mixed-precision path.
 *
+*  With Average_int yes (ISTAV > 0, input.f:372-374) the fast-group calls
+*  also fill the per-template bonded sums BR/BE, AR/AE, TR/TE that GETAVR
+*  averages (aver.f:43-112), from mdx_get_terms.
+*
 *  Not supported here (the engine returns MDX_ERR_ARG or the shim stops):
 *  reaction field, hexagonal/octahedral cells, fixed molecules, restraints
-*  and external fields of PULLBACK (forces.f:1756-1799), per-bond
-*  averages BR/BE/AR/AE/TR/TE.
+*  and external fields of PULLBACK (forces.f:1756-1799).
 *
       module mdxc
       use iso_c_binding
@@ -50,7 +53,7 @@
         type(c_ptr) :: tp_kind, tp_mult, tp_k, tp_delta
         type(c_ptr) :: tp_k1, tp_k2, tp_k3, tp_k4, tp_k5
         integer(c_int32_t) :: nspc
-        type(c_ptr) :: spc_atom, spc_par
+        type(c_ptr) :: spc_atom, spc_par, spc_bond
       end type
       type, bind(C) :: mdx_params_t
         real(c_double) :: box(3), rcut, rshort, skin, alpha, fexp
@@ -126,6 +129,12 @@
           type(c_ptr), value :: gx, gy, gz, hx, hy, hz
           type(mdx_energies_t), intent(out) :: e
         end function
+        integer(c_int) function mdx_get_terms(h, br, be, ar, ae, tr,
+     +    te) bind(C)
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double) :: br(*), be(*), ar(*), ae(*), tr(*), te(*)
+        end function
         type(c_ptr) function mdx_last_error(h) bind(C)
           import :: c_ptr
           type(c_ptr), value :: h
@@ -141,7 +150,7 @@
      +  amol(:), aljt(:), tee(:), tmov(:), tpair(:), exs(:), exa(:),
      +  bi(:), bj(:), bp(:), bmo(:), ai(:), aj(:), ak(:), ap(:),
      +  ti(:), tj(:), tk(:), tl(:), tp(:), tkind(:), tmult(:),
-     +  sat(:)
+     +  sat(:), sbd(:)
       integer(c_int8_t), allocatable, target, save :: ex14(:)
       real(c_double), allocatable, target, save :: aq(:), aq0(:),
      +  am(:), c14e(:), c14l(:), ta6(:), tb12(:), ta60(:), tb120(:),
@@ -286,6 +295,7 @@
         if(IPOT(ITYP).eq.1.or.IPOT(ITYP).eq.2)NSPCM=NSPCM+NSPEC(ITYP)
       end do
       allocate(sat(max(3*NSPCM,1)),spar(max(12*NSPCM,1)))
+      allocate(sbd(max(NSPCM,1)))
       K=0
       do ITYP=1,NTYPES
         if(IPOT(ITYP).eq.1.or.IPOT(ITYP).eq.2)then
@@ -296,6 +306,7 @@
             sat(3*K-2)=IML
             sat(3*K-1)=IML+1
             sat(3*K)=IML+2
+            sbd(K)=I1-1
             spar(12*K-11)=RB(I1)
             spar(12*K-10)=RB(I1+1)
             spar(12*K-9)=RB(I1+2)
@@ -379,6 +390,7 @@
       topo%nspc=NSPCM
       topo%spc_atom=c_loc(sat)
       topo%spc_par=c_loc(spar)
+      topo%spc_bond=c_loc(sbd)
       prm%box(1)=BOXL
       prm%box(2)=BOYL
       prm%box(3)=BOZL
@@ -403,13 +415,6 @@ C  MDX_PRECISION=fp64 in the environment: the engine's FP64 mode
       end if
       if(LHEX.or.LOCT.or.LMNBL)then
         write(*,*)'!!! GPU force engine: rectangular cell, Ewald only'
-        call FINAL
-      end if
-C  per-term bonded averages (Average_int yes, input.f:372-374) need the
-C  BR/BE/AR/AE/TR/TE sums of GETBND/GETANG/GETTRS, which this shim does
-C  not fill: stop rather than let GETAVR average zeros (aver.f:43-112)
-      if(ISTAV.gt.0)then
-        write(*,*)'!!! GPU force engine: Average_int yes not supported'
         call FINAL
       end if
       IST=mdx_create(topo,prm,hdl)
@@ -515,6 +520,26 @@ C  not fill: stop rather than let GETAVR average zeros (aver.f:43-112)
       end if
       end
 *
+*  Options of a fast-group call: the energy breakdown always, the
+*  per-template bonded sums when GETAVR averages them (ISTAV > 0).
+      integer(c_int32_t) function MDXWANT()
+      use mdxc
+      include "common_inc.f"
+      MDXWANT=1
+      if(ISTAV.gt.0)MDXWANT=3
+      end
+*
+*  BR/BE, AR/AE, TR/TE as GETBND, STRBND, GETANG and GETTRS leave them
+*  (forces.f:119-182, 655-663, 232-273, 336-447), when averaged.
+      subroutine MDXTRM
+      use mdxc
+      include "common_inc.f"
+      integer(c_int) IST
+      if(ISTAV.le.0)return
+      IST=mdx_get_terms(hdl,BR,BE,AR,AE,TR,TE)
+      if(IST.ne.0)call MDXERR(IST,'mdx_get_terms')
+      end
+*
 *  Restraints and external fields of PULLBACK are not part of this build.
       subroutine MDXNOPULL
       include "common_inc.f"
@@ -530,12 +555,14 @@ C  not fill: stop rather than let GETAVR average zeros (aver.f:43-112)
       include "common_inc.f"
       type(mdx_energies_t) E
       integer(c_int) IST
+      integer(c_int32_t) MDXWANT
       call ZEROAV
       call MDXPOS
-      IST=mdx_compute(hdl,0,1,FX,FY,FZ,GX,GY,GZ,HX,HY,HZ,E)
+      IST=mdx_compute(hdl,0,MDXWANT(),FX,FY,FZ,GX,GY,GZ,HX,HY,HZ,E)
       if(IST.ne.0)call MDXERR(IST,'mdx_compute(ALL)')
       call MDXPUT(E,0)
-*  CM_ADDLA accumulations (scalar.f:82-100)
+      call MDXTRM
+*  CM_ADDLA accumulations (scalar.f:82-106)
       do I=1,MOLINT
         POTES(I)=POTES(I)+POTE1(I)
         POTLJ(I)=POTLJ(I)+POTL1(I)
@@ -544,6 +571,18 @@ C  not fill: stop rather than let GETAVR average zeros (aver.f:43-112)
         PES14(I)=PES14(I)+PES141(I)
         PSR14(I)=PSR14(I)+PSR141(I)
         SELFPE(I)=SELFPE(I)+SPEE(I)
+      end do
+      do M=1,NRBB
+        BB(M)=BB(M)+BR(M)
+        EB(M)=EB(M)+BE(M)
+      end do
+      do M=1,NRAA
+        AA(M)=AA(M)+AR(M)
+        EA(M)=EA(M)+AE(M)
+      end do
+      do M=1,NRTT
+        TT(M)=TT(M)+TR(M)
+        ET(M)=ET(M)+TE(M)
       end do
       call MDXNOPULL
       call SCLFRC(3)
@@ -556,11 +595,13 @@ C  not fill: stop rather than let GETAVR average zeros (aver.f:43-112)
       logical LAVER
       type(mdx_energies_t) E
       integer(c_int) IST
+      integer(c_int32_t) MDXWANT
       call MDXPOS
-      IST=mdx_compute_f(hdl,1,1,HX,HY,HZ,c_null_ptr,c_null_ptr,
+      IST=mdx_compute_f(hdl,1,MDXWANT(),HX,HY,HZ,c_null_ptr,c_null_ptr,
      +  c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,E)
       if(IST.ne.0)call MDXERR(IST,'mdx_compute(FAST)')
       call MDXPUT(E,1)
+      call MDXTRM
       call CM_ADDF(LAVER)
       call SCLFRC(1)
       end

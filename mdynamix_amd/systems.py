@@ -145,6 +145,9 @@ class System:
     # input.f:660-666; it switches the Nose thermostat off)
     pimd_beads: int = 0
     pi_thermo: Optional[Tuple[float, float, int]] = None
+    # reference input "Average_int yes": per-template bond, angle and
+    # dihedral averages (ISTAV, input.f:372-374; GETAVR, aver.f)
+    average_int: bool = False
 
     # ------------------------------------------------------------------
     @property
@@ -191,6 +194,9 @@ class System:
                   "Neighbour_list %d" % self.rebuild_every,
                   "Electrostatics Ewald %.12f %.12f" % (self.ewald_a, self.ewald_b),
                   "Startup XMOL", "Output 1", "COM_check yes 1000000"]
+        if self.average_int:
+            # one averaging series over the run (GETAVR(2), mdstep.f:243)
+            lines += ["Average_int yes", "Serie_av %d" % nsteps]
         if self.ee_type is not None:
             # two sub-ensembles: the job's lambda (started in) and lambda=1
             lines += ["MDEE 2 list", "%.12f 0.0" % self.ee_lambda, "1.0 0.0",

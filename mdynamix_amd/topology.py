@@ -171,7 +171,7 @@ def from_system(system: System, lam: float | None = None) -> Tuple[TopologyArray
     ai, aj, ak, apar, ath, akk = [], [], [], [], [], []
     ti, tj, tk, tl, tpar = [], [], [], [], []
     tkind, tmult, tkk, tdel, tk1, tk2, tk3, tk4, tk5 = [], [], [], [], [], [], [], [], []
-    spc_at, spc_pr = [], []
+    spc_at, spc_pr, spc_bd = [], [], []
     base = 0
     for t, c in zip(types, counts):
         b0, a0, t0 = len(br0), len(ath), len(tkind)
@@ -196,6 +196,7 @@ def from_system(system: System, lam: float | None = None) -> Tuple[TopologyArray
                 spc_at.append((off, off + 1, off + 2))
                 spc_pr.append((b1[3], b2[3], b3[3], b1[5], b2[5], b3[4], b1[6], b2[6],
                                FSPC_EEP, FSPC_GGP, 2.0, 1.0))
+                spc_bd.append(b0)
                 continue
             for kb, b in enumerate(t.bonds):
                 bi.append(off + b[1] - 1); bj.append(off + b[2] - 1); bpar.append(b0 + kb)
@@ -225,7 +226,8 @@ def from_system(system: System, lam: float | None = None) -> Tuple[TopologyArray
         tp_kind=np.array(tkind, np.int32), tp_mult=np.array(tmult, np.int32), tp_k=np.array(tkk),
         tp_delta=np.array(tdel), tp_k1=np.array(tk1), tp_k2=np.array(tk2), tp_k3=np.array(tk3),
         tp_k4=np.array(tk4), tp_k5=np.array(tk5),
-        spc_atom=np.array(spc_at, np.int32).reshape(-1, 3), spc_par=np.array(spc_pr).reshape(-1, 12))
+        spc_atom=np.array(spc_at, np.int32).reshape(-1, 3), spc_par=np.array(spc_pr).reshape(-1, 12),
+        spc_bond=np.array(spc_bd, np.int32))
     raex = system.ee_bee * (1.0 - lam ** system.ee_pw2) if system.ee_type is not None else 0.0
     params = dict(box=system.box, rcut=system.rcut, rshort=system.rshort, alpha=system.alpha,
                   fexp=system.ewald_b, coulomb_factor=COULF_KJ,
@@ -235,16 +237,17 @@ def from_system(system: System, lam: float | None = None) -> Tuple[TopologyArray
     return topo, params
 
 
-def _spc_records(d: Dict[str, List[np.ndarray]]) -> Tuple[np.ndarray, np.ndarray]:
+def _spc_records(d: Dict[str, List[np.ndarray]]) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
     """STRBND records (forces.f:555-590) of every type with IPOT 1 or 2:
     atoms ISADDR+3(n-1)+1..3 of molecule n, parameters from the type's
     first three bond records (IADB), cross constants EEP, GGP times
-    FACTOR = 1e3/AVSNO/UNITE."""
+    FACTOR = 1e3/AVSNO/UNITE; and each record's first bond template
+    (IADB - 1, where STRBND adds its BR/BE terms, forces.f:655-663)."""
     g = lambda k: d[k][0]
     ipot, iadb, nspec, nsits = g("IPOT"), g("IADB"), g("NSPEC"), g("NSITS")
     rb, fb, db, ro, lmove = g("RB"), g("FB"), g("DB"), g("RO"), g("LMOVE")
     factor = 1.0e3 / 6.02252e23 / g("SCALARS")[11]
-    at, pr = [], []
+    at, pr, bd = [], [], []
     base = 0
     for it in range(len(ipot)):
         if ipot[it] in (1, 2):
@@ -256,8 +259,9 @@ def _spc_records(d: Dict[str, List[np.ndarray]]) -> Tuple[np.ndarray, np.ndarray
                 at.append((off, off + 1, off + 2))
                 pr.append((rb[i1], rb[i2], rb[i3], a_, b_, fb[i3], ro[i1], ro[i2], FSPC_EEP * factor,
                            FSPC_GGP * factor, float(ipot[it]), float(lmove[it] != 0)))
+                bd.append(i1)
         base += nspec[it] * nsits[it]
-    return np.array(at, np.int32).reshape(-1, 3), np.array(pr, float).reshape(-1, 12)
+    return np.array(at, np.int32).reshape(-1, 3), np.array(pr, float).reshape(-1, 12), np.array(bd, np.int32)
 
 
 def from_dump(d: Dict[str, List[np.ndarray]], lam: float = 1.0) -> Tuple[TopologyArrays, Dict]:
@@ -297,7 +301,7 @@ def from_dump(d: Dict[str, List[np.ndarray]], lam: float = 1.0) -> Tuple[Topolog
         tp_k5=g("FT5"))
     assert nbond == len(topo.bond_i)
     if "IPOT" in d:
-        topo.spc_atom, topo.spc_par = _spc_records(d)
+        topo.spc_atom, topo.spc_par, topo.spc_bond = _spc_records(d)
     ee_enabled = bool(sizes[11])
     params = dict(box=(sc[0], sc[1], sc[2]), rcut=sc[3], rshort=sc[5], alpha=sc[7], fexp=sc[8],
                   coulomb_factor=sc[9], list_mode=MDX_LIST_REFERENCE, skin=0.0,

@@ -58,7 +58,7 @@ STATIC = ["SIZES", "SCALARS", "ITYPE", "NNUM", "NSITE", "INBTA", "CHARGEA", "Q",
           "KTK", "LTL", "ITUK", "ITORS", "NMUL", "FT", "RT", "FT1", "FT2", "FT3", "FT4", "FT5",
           "LISTLEN"]
 PERCALL = ["SX", "SY", "SZ", "XCOM", "YCOM", "ZCOM", "FX", "FY", "FZ", "GX", "GY", "GZ", "HX", "HY",
-           "HZ", "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE"]
+           "HZ", "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE", "BR", "BE", "AR", "AE", "TR", "TE"]
 
 
 def list_digest(a: np.ndarray, b: np.ndarray) -> str:

@@ -12,6 +12,12 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 CASES = ("water_tiny", "water_508", "chains_tiny", "ee_tiny", "fspc_tiny")
 
 
+# per-call arrays of the force fixtures (one per ALLFORCE / SLFORCES +
+# FFORCES call of the reference harness)
+PERCALL = ("SX", "SY", "SZ", "XCOM", "YCOM", "ZCOM", "FX", "FY", "FZ", "GX", "GY", "GZ", "HX", "HY", "HZ",
+           "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE", "BR", "BE", "AR", "AE", "TR", "TE")
+
+
 def load_golden(name):
     """Fixture as a dump-like dict: static arrays as [a], per-call arrays as
     [call0, call1, ...]."""
@@ -19,14 +25,12 @@ def load_golden(name):
     d = {}
     ncall = int(z["NCALL"])
     for k in z.files:
-        if k[-2:] in ("_0", "_1") and k[:-2] in ("SX", "SY", "SZ", "XCOM", "YCOM", "ZCOM", "FX", "FY", "FZ",
-                                                 "GX", "GY", "GZ", "HX", "HY", "HZ", "ENERGY", "POTE1",
-                                                 "POTL1", "PES141", "PSR141", "SPEE"):
+        if k[-2:] in ("_0", "_1") and k[:-2] in PERCALL:
             continue
         d[k] = [z[k]]
-    for k in ("SX", "SY", "SZ", "XCOM", "YCOM", "ZCOM", "FX", "FY", "FZ", "GX", "GY", "GZ", "HX", "HY",
-              "HZ", "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE"):
-        d[k] = [z["%s_%d" % (k, c)] for c in range(ncall)]
+    for k in PERCALL:
+        if "%s_0" % k in z.files:
+            d[k] = [z["%s_%d" % (k, c)] for c in range(ncall)]
     d["lam"] = float(z["lam"])
     return d
 
