@@ -278,6 +278,28 @@ mdx_status mdx_compute_device(mdx_engine *h, int32_t group, int32_t want,
  * NULL.  Zeros before the first such call.  Sums in a fixed order. */
 mdx_status mdx_get_terms(mdx_engine *h, double *br, double *be, double *ar, double *ae, double *tr, double *te);
 
+/* Expanded-ensemble energies at trial coupling parameters: the sums DIFEN
+ * and DIFFUR form for a move between subensembles (mdee.f:265-534), as
+ * functions of lambda, at the current coordinates and list, in FP64, for
+ * nlam <= MDX_EE_MAX_LAMBDA values of lambda at once.  The reference's
+ * energies toward subensemble ME1 are EEN = e_pair(EC(ME1)),
+ * EEO = e_pair(EC(ME)), EFN = e_recip(EC(ME1)), EFO = e_recip(EC(ME)).
+ * e_pair: over the listed pairs with an atom of a lambda-coupled type
+ * (type_ee = 0) in different molecules (reference lists: no cutoff test;
+ * Verlet lists: r <= R_cut), s = lambda^PW1, shift = BEE (1 - lambda^PW2):
+ *   s^(coupled atoms) q0_i q0_j COULF erfc(alpha r) / (r + shift)
+ *   + s (A6_0 + B12_0 (r + shift)^-6) (r + shift)^-6.
+ * e_recip: sum_k AKC |S(k)|^2 with the coupled atoms' charges s q0, minus
+ * their self terms COULF (s q0)^2 alpha / sqrt(pi), plus over the pairs
+ * inside coupled molecules s^2 COULF q0_i q0_j (erfc(alpha r) - 1) / r.
+ * Either output may be NULL.  Synchronous; needs a built list. */
+#define MDX_EE_MAX_LAMBDA 32
+typedef struct mdx_ee_params {
+    double pw1, pw2, bee;         /* EE_power PW1 PW2, EE_offset BEE      */
+} mdx_ee_params;
+mdx_status mdx_ee_energies(mdx_engine *h, const mdx_ee_params *p, int32_t nlam, const double *lambda,
+                           double *e_pair, double *e_recip);
+
 /* Forces of the last call (F, G, H) to caller-owned host arrays (any may be
  * NULL), e.g. after a batched call. */
 mdx_status mdx_get_forces(mdx_engine *h, double *fx, double *fy, double *fz, double *gx, double *gy, double *gz,

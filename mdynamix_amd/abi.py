@@ -77,6 +77,10 @@ class MdxParams(C.Structure):
     ]
 
 
+class MdxEeParams(C.Structure):
+    _fields_ = [("pw1", C.c_double), ("pw2", C.c_double), ("bee", C.c_double)]
+
+
 class MdxMdParams(C.Structure):
     _fields_ = [
         ("dt", C.c_double), ("accel_factor", C.c_double), ("ekin_factor", C.c_double), ("boltz", C.c_double),

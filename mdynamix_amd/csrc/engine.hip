@@ -27,6 +27,7 @@
 #include "bonded.hip"
 #include "md.hip"
 #include "furir64.hip"
+#include "mdee.hip"
 
 using namespace mdx;
 
@@ -152,6 +153,14 @@ struct mdx_engine {
     std::vector<int4> h_kidx;   // (nx, ny, nz) in INIFUR order
     // FP64 precision mode: FURIR workspace (fur_layout of n atoms, nkv k)
     char *d_furws = nullptr;
+    // expanded-ensemble energies (mdx_ee_energies), allocated on first use
+    double *d_ee_lam = nullptr, *d_ee_rows = nullptr, *d_ee_out = nullptr, *d_ee_kpart = nullptr;
+    double4 *d_ee_spart = nullptr;
+    int4 *d_ee_kv = nullptr;
+    int2 *d_ee_pairs = nullptr;
+    int ee_rows_cap = 0, ee_nkv = -1, ee_npairs = -1;
+    double ee_kbox[3] = {0, 0, 0};
+    double ee_self0 = 0.0;
     // forces
     double *d_fs = nullptr, *d_fl = nullptr;     // [3][n] each
     unsigned long long *d_fix = nullptr;          // [9][n]: G, HL, HB
@@ -2674,6 +2683,120 @@ mdx_status mdx_batch_set_ring(mdx_batch *b, double beadfac) {
     b->beadfac = beadfac;
     b->planned = false;
     drop_graphs(b);
+    return MDX_OK;
+}
+
+mdx_status mdx_ee_energies(mdx_engine *h, const mdx_ee_params *p, int32_t nlam, const double *lambda,
+                           double *e_pair, double *e_recip) {
+    if (!h || !p || !lambda || nlam <= 0) return MDX_ERR_ARG;
+    if (nlam > kEeMaxLam) {
+        h->err = "mdx_ee_energies: at most MDX_EE_MAX_LAMBDA lambda values per call";
+        return MDX_ERR_LIMIT;
+    }
+    if (!h->have_list) {
+        h->err = "mdx_ee_energies before mdx_build_list";
+        return MDX_ERR_STATE;
+    }
+    for (int L = 0; L < nlam; ++L)
+        if (!(lambda[L] >= 0.0)) return MDX_ERR_ARG;
+    hipStream_t st = h->stream;
+    HIPCHK(h, hipStreamSynchronize(st));
+    const int n = h->n;
+    // one-time set-up: the pairs inside lambda-coupled molecules and their
+    // self-term sum (DIFFUR, mdee.f:320-366)
+    if (h->ee_npairs < 0) {
+        std::vector<int2> pr;
+        std::vector<double> q0(n);
+        HIPCHK(h, hipMemcpy(q0.data(), h->d_q0, sizeof(double) * n, hipMemcpyDeviceToHost));
+        const double abc = h->p.alpha / std::sqrt(kPI);
+        h->ee_self0 = 0.0;
+        for (int i = 0; i < n; ++i) {
+            if (h->h_type_ee[h->h_moltype[i]] != 0) continue;
+            h->ee_self0 += h->p.coulomb_factor * q0[i] * q0[i] * abc;
+            for (int j = i + 1; j < n && h->h_mol[j] == h->h_mol[i]; ++j) pr.push_back(make_int2(i, j));
+        }
+        h->ee_npairs = (int)pr.size();
+        CHK(upload(h, &h->d_ee_pairs, pr.data(), std::max<size_t>(1, pr.size())));
+        CHK(dalloc(h, &h->d_ee_lam, 2 * kEeMaxLam));
+        CHK(dalloc(h, &h->d_ee_out, 3 * kEeMaxLam));
+    }
+    if (h->ee_rows_cap < h->nclu_cap) {
+        CHK(dalloc(h, &h->d_ee_rows, (size_t)h->nclu_cap * kEeMaxLam));
+        h->ee_rows_cap = h->nclu_cap;
+    }
+    // the k set of the enumeration box (FURIR's 5 % window)
+    std::vector<int4> ks = h->h_kidx;
+    if (ks.empty() && h->p.alpha > 0.0) {
+        mdx_params pk = h->p;
+        for (int k = 0; k < 3; ++k) pk.box[k] = h->kbox[k];
+        enumerate_k(pk, ks);
+    }
+    const int nkv = (int)ks.size(), nchunk = cdiv(std::max(n, 1), kFurChunk), nkb = cdiv(std::max(nkv, 1), kFurT);
+    if (nkv != h->ee_nkv || std::memcmp(h->ee_kbox, h->kbox, sizeof(h->ee_kbox)) != 0) {
+        CHK(upload(h, &h->d_ee_kv, ks.data(), std::max<size_t>(1, ks.size())));
+        CHK(dalloc(h, &h->d_ee_spart, (size_t)nchunk * std::max(1, nkv)));
+        CHK(dalloc(h, &h->d_ee_kpart, (size_t)nkb * kEeMaxLam));
+        h->ee_nkv = nkv;
+        std::memcpy(h->ee_kbox, h->kbox, sizeof(h->ee_kbox));
+    }
+    std::vector<double> lam(2 * nlam), sq(nlam);
+    for (int L = 0; L < nlam; ++L) {
+        sq[L] = std::pow(lambda[L], p->pw1);
+        lam[2 * L] = sq[L];
+        lam[2 * L + 1] = p->bee * (1.0 - std::pow(lambda[L], p->pw2));
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_ee_lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice, st));
+    // pairs (DIFEN)
+    EeArgs ea{};
+    ea.nclu = h->nclu;
+    ea.list_cap = h->list_cap;
+    ea.nlam = nlam;
+    ea.list = h->d_list; ea.nlist = h->d_nlist; ea.nshort = h->d_nshort; ea.clu_atom = h->d_clu_atom;
+    ea.x = h->d_x; ea.y = h->d_y; ea.z = h->d_z;
+    ea.box = h->box;
+    ea.rc2d = h->a_pair.rc2d;
+    ea.atom_mol = h->d_mol; ea.atom_moltype = h->d_moltype; ea.atom_ljtype = h->d_ljtype; ea.type_ee = h->d_type_ee;
+    ea.q0 = h->d_q0;
+    ea.a6_0 = h->d_a6_0; ea.b12_0 = h->d_b12_0; ea.nlj = h->nlj;
+    ea.coulf = h->p.coulomb_factor;
+    ea.alpha = h->p.alpha;
+    ea.lam = h->d_ee_lam;
+    ea.rows = h->d_ee_rows;
+    if (h->nclu > 0) {
+        k_ee_pairs<<<h->nclu, 64, 0, st>>>(ea);
+        k_ee_colsum<<<nlam, 256, 0, st>>>(h->nclu, nlam, h->d_ee_rows, h->d_ee_out);
+    } else {
+        HIPCHK(h, hipMemsetAsync(h->d_ee_out, 0, sizeof(double) * nlam, st));
+    }
+    // reciprocal part (DIFFUR) and the intramolecular term at lambda = 1
+    if (nkv > 0 && n > 0) {
+        EeFurArgs fa{};
+        fa.n = n; fa.nkv = nkv; fa.nchunk = nchunk; fa.nlam = nlam;
+        fa.lx = h->p.box[0]; fa.ly = h->p.box[1]; fa.lz = h->p.box[2];
+        fa.alp2 = h->p.alpha * h->p.alpha;
+        fa.cfur = 4.0 * kPI * h->p.coulomb_factor / (h->p.box[0] * h->p.box[1] * h->p.box[2]);
+        fa.x = h->d_x; fa.y = h->d_y; fa.z = h->d_z;
+        fa.q = h->d_q; fa.q0 = h->d_q0;
+        fa.atom_moltype = h->d_moltype; fa.type_ee = h->d_type_ee;
+        fa.kv = h->d_ee_kv; fa.spart = h->d_ee_spart; fa.lam = h->d_ee_lam; fa.kpart = h->d_ee_kpart;
+        k_ee_sk<<<dim3(nkb, nchunk), kFurT, 0, st>>>(fa);
+        k_ee_sred<<<nkb, kFurT, 0, st>>>(fa);
+        k_ee_colsum<<<nlam, 256, 0, st>>>(nkb, nlam, h->d_ee_kpart, h->d_ee_out + kEeMaxLam);
+    } else {
+        HIPCHK(h, hipMemsetAsync(h->d_ee_out + kEeMaxLam, 0, sizeof(double) * nlam, st));
+    }
+    k_ee_intra<<<1, 256, 0, st>>>(h->ee_npairs, h->d_ee_pairs, h->d_x, h->d_y, h->d_z, h->d_q0, h->box,
+                                  h->p.coulomb_factor, h->p.alpha, h->d_ee_out + 2 * kEeMaxLam);
+    std::vector<double> out(3 * kEeMaxLam);
+    HIPCHK(h, hipMemcpyAsync(out.data(), h->d_ee_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    HIPCHK(h, hipGetLastError());
+    const double intra0 = out[2 * kEeMaxLam];
+    for (int L = 0; L < nlam; ++L) {
+        if (e_pair) e_pair[L] = out[L];
+        const double s2 = sq[L] * sq[L];
+        if (e_recip) e_recip[L] = out[kEeMaxLam + L] - s2 * h->ee_self0 + s2 * intra0;
+    }
     return MDX_OK;
 }
 

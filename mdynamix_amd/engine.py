@@ -63,6 +63,7 @@ def load_library() -> C.CDLL:
     L.mdx_stage_times.argtypes = [pv, pd, C.c_int32]
     L.mdx_get_forces.argtypes = [pv] + [pd] * 9
     L.mdx_get_terms.argtypes = [pv] + [pd] * 6
+    L.mdx_ee_energies.argtypes = [pv, C.POINTER(abi.MdxEeParams), C.c_int32, pd, pd, pd]
     L.mdx_set_timing.argtypes = [pv, C.c_int32]
     L.mdx_memory_bytes.argtypes = [pv, C.POINTER(C.c_int64)]
     L.mdx_batch_create.argtypes = [C.POINTER(pv), C.c_int32, C.POINTER(pv)]
@@ -88,7 +89,7 @@ def load_library() -> C.CDLL:
     L.mdx_batch_pimd_init.argtypes = [pv, C.POINTER(abi.MdxPimdParams)]
     L.mdx_batch_pimd_chains.argtypes = [pv, pd]
     for name in ("mdx_create", "mdx_destroy", "mdx_set_positions", "mdx_set_positions_device", "mdx_set_box",
-                 "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_list_entries", "mdx_get_list", "mdx_compute", "mdx_get_terms",
+                 "mdx_set_molecule_com", "mdx_build_list", "mdx_list_counts", "mdx_list_entries", "mdx_get_list", "mdx_compute", "mdx_get_terms", "mdx_ee_energies",
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
@@ -254,6 +255,16 @@ class ForceEngine:
                                                     ("tr", nt), ("te", nt))}
         self._check(self.lib.mdx_get_terms(self._h, *[as_ptr(out[k]) for k in ("br", "be", "ar", "ae", "tr", "te")]))
         return {k: v[:m] for (k, v), m in zip(out.items(), (nb, nb, na, na, nt, nt))}
+
+    def ee_energies(self, lambdas, pw1: float, pw2: float, bee: float):
+        """(e_pair, e_recip) at each lambda: the sums DIFEN and DIFFUR form
+        for an expanded-ensemble move (mdee.f:265-534) as functions of the
+        coupling parameter, at the current coordinates and list."""
+        lam = np.ascontiguousarray(lambdas, dtype=np.float64)
+        ep, er = np.zeros(len(lam)), np.zeros(len(lam))
+        p = abi.MdxEeParams(pw1, pw2, bee)
+        self._check(self.lib.mdx_ee_energies(self._h, C.byref(p), len(lam), as_ptr(lam), as_ptr(ep), as_ptr(er)))
+        return ep, er
 
     def get_forces(self):
         """(F, G, H) of the last force call, e.g. after a batched call."""

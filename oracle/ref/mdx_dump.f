@@ -12,6 +12,9 @@
 *  Record layout (stream access): 8-character name, int32 kind
 *  (1 = int32, 2 = real*8), int32 count, then count values.
 *
+*  Expanded-ensemble runs also record the reference's DIFFUR/DIFEN energy
+*  differences toward every subensemble at each dumped call.
+*
 *  SULTAN additionally displaces the atoms by a deterministic pattern and
 *  calls ALLFORCE again WITHOUT rebuilding the list, so the dump also pins
 *  the stale-list semantics (pairs evaluated without a cutoff test).
@@ -325,4 +328,25 @@
       call MDXWD('AE      ',NRAA,AE)
       call MDXWD('TR      ',NRTT,TR)
       call MDXWD('TE      ',NRTT,TE)
+*  expanded ensemble: the reference's own energy differences toward every
+*  subensemble ME1 (DIFFUR, DIFEN; mdee.f:265-534), with the current lists
+*  and coordinates: EFN, EFO, EEN, EEO per ME1; then ME, PW1, PW2, BEE and
+*  the subensembles' EC
+      if(LMDEE)then
+        do ME1=1,NE
+          call DIFFUR(ME1,EFN,EFO)
+          call DIFEN(ME1,EEN,EEO)
+          DS(4*ME1-3)=EFN
+          DS(4*ME1-2)=EFO
+          DS(4*ME1-1)=EEN
+          DS(4*ME1)=EEO
+        end do
+        call MDXWD('EEDIFF  ',4*NE,DS)
+        DS(1)=ME
+        DS(2)=PW1
+        DS(3)=PW2
+        DS(4)=BEE
+        call MDXWD('EEPAR   ',4,DS)
+        call MDXWD('EC      ',NE,EC)
+      end if
       END

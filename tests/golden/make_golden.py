@@ -58,7 +58,8 @@ STATIC = ["SIZES", "SCALARS", "ITYPE", "NNUM", "NSITE", "INBTA", "CHARGEA", "Q",
           "KTK", "LTL", "ITUK", "ITORS", "NMUL", "FT", "RT", "FT1", "FT2", "FT3", "FT4", "FT5",
           "LISTLEN"]
 PERCALL = ["SX", "SY", "SZ", "XCOM", "YCOM", "ZCOM", "FX", "FY", "FZ", "GX", "GY", "GZ", "HX", "HY",
-           "HZ", "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE", "BR", "BE", "AR", "AE", "TR", "TE"]
+           "HZ", "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE", "BR", "BE", "AR", "AE", "TR", "TE",
+           "EEDIFF", "EEPAR", "EC"]
 
 
 def list_digest(a: np.ndarray, b: np.ndarray) -> str:
@@ -187,7 +188,8 @@ def main() -> None:
         out["NCALL"] = np.array(ncall)
         for c in range(ncall):
             for k in PERCALL:
-                out["%s_%d" % (k, c)] = d[k][c]
+                if k in d:
+                    out["%s_%d" % (k, c)] = d[k][c]
         if full_lists:
             for k in ("NBS1", "NBS2", "NBL1", "NBL2"):
                 out[k] = d[k][0]

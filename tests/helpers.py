@@ -15,7 +15,8 @@ CASES = ("water_tiny", "water_508", "chains_tiny", "ee_tiny", "fspc_tiny")
 # per-call arrays of the force fixtures (one per ALLFORCE / SLFORCES +
 # FFORCES call of the reference harness)
 PERCALL = ("SX", "SY", "SZ", "XCOM", "YCOM", "ZCOM", "FX", "FY", "FZ", "GX", "GY", "GZ", "HX", "HY", "HZ",
-           "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE", "BR", "BE", "AR", "AE", "TR", "TE")
+           "ENERGY", "POTE1", "POTL1", "PES141", "PSR141", "SPEE", "BR", "BE", "AR", "AE", "TR", "TE",
+           "EEDIFF", "EEPAR", "EC")
 
 
 def load_golden(name):
