@@ -148,6 +148,12 @@ class System:
     # reference input "Average_int yes": per-template bond, angle and
     # dihedral averages (ISTAV, input.f:372-374; GETAVR, aver.f)
     average_int: bool = False
+    # restart files (RESTRT, restart.f): "Dump_restart N [ASCII]" and
+    # "Read_restart yes [ASCII]" of the reference input (input.f:191-205)
+    dump_every: int = 1000000
+    dump_ascii: bool = False
+    read_restart: bool = False
+    read_ascii: bool = False
 
     # ------------------------------------------------------------------
     @property
@@ -174,7 +180,9 @@ class System:
             with open(os.path.join(directory, t.name + ".mmol"), "w") as f:
                 f.write(t.to_mmol())
         lines = ["Main_filename %s" % self.name, "Verbose_level %d" % verbose,
-                 "Path_DB .", "Read_restart no", "Dump_restart 1000000", "Check_only no",
+                 "Path_DB .",
+                 "Read_restart %s" % (("yes ASCII" if self.read_ascii else "yes") if self.read_restart else "no"),
+                 "Dump_restart %d%s" % (self.dump_every, " ASCII" if self.dump_ascii else ""), "Check_only no",
                  "Molecule_types %d" % len(self.types)]
         for t, c in zip(self.types, self.counts):
             lines.append("%s %d" % (t.name, c))
@@ -210,16 +218,9 @@ class System:
         lines.append("End")
         with open(os.path.join(directory, "md.in"), "w") as f:
             f.write("\n".join(lines) + "\n")
-        with open(os.path.join(directory, self.name + ".start"), "w") as f:
-            f.write("%d\n" % self.natoms)
-            f.write("generated  BOX %.12f %.12f %.12f\n" % self.box)
-            k = 0
-            for t, c in zip(self.types, self.counts):
-                for _ in range(c):
-                    for s in t.sites:
-                        x, y, z = self.pos[k]
-                        f.write("%s %.17g %.17g %.17g\n" % (s.name[0], x, y, z))
-                        k += 1
+        from .restart import write_start
+        names = [s.name[0] for t, c in zip(self.types, self.counts) for _ in range(c) for s in t.sites]
+        write_start(os.path.join(directory, self.name + ".start"), names, self.pos, self.box)
 
 
 # ----------------------------------------------------------------------
