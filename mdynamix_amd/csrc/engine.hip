@@ -7,7 +7,6 @@
 // mdx_compute is ALLFORCE / FFORCES / SLFORCES.  All device work of a call
 // is queued on the handle's stream; the host waits only where a result
 // must come back (list sizes after a rebuild, energies and forces).
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -111,8 +110,7 @@ struct mdx_engine {
     unsigned long long *d_keys = nullptr, *d_keys2 = nullptr;
     int *d_vals = nullptr, *d_vals2 = nullptr;
     int *d_colcount = nullptr, *d_colstart = nullptr, *d_colclu_n = nullptr, *d_colclu = nullptr;
-    void *d_cub = nullptr;
-    size_t cub_bytes = 0;
+    int *d_nclu = nullptr;                        // cluster count of the last rebuild (device)
     int *d_clu_atom = nullptr, *d_atom_slot = nullptr, *d_clu_col = nullptr;
     double4 *d_spos = nullptr;                    // per slot: FP64 x, y, z, molecule (list build)
     double *d_anchor = nullptr;
@@ -659,12 +657,7 @@ mdx_status alloc_clusters(mdx_engine *h) {
     CHK(dalloc(h, &h->d_overflow, 1));
     CHK(dalloc(h, &h->d_xq, (size_t)h->nclu_cap * kClu));
     CHK(dalloc(h, &h->d_aux, (size_t)h->nclu_cap * kClu));
-    // temporary storage of the sort and scans
-    size_t b1 = 0, b2 = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, b1, h->d_keys, h->d_keys2, h->d_vals, h->d_vals2, h->n, 0, 64);
-    hipcub::DeviceScan::ExclusiveSum(nullptr, b2, h->d_colcount, h->d_colstart, h->ncol);
-    h->cub_bytes = std::max(b1, b2);
-    CHK(dalloc(h, (char **)&h->d_cub, h->cub_bytes));
+    CHK(dalloc(h, &h->d_nclu, 1));
     // list capacities from the density: clusters whose boxes come within
     // r_list + 2 cluster extents
     const double rl = h->p.rcut + (h->p.list_mode == MDX_LIST_VERLET ? h->p.skin : 0.0);
@@ -1380,52 +1373,51 @@ mdx_status mdx_build_list(mdx_engine *h) {
     hipStream_t st = h->stream;
     const int n = h->n;
     if (h->timing) hipEventRecord(h->ev[0], st);
+    // atoms binned into x-y columns and sorted by z inside each (the
+    // column's (z code, atom index) order); clusters of 8 per column.  The
+    // cluster count stays on the device: every later kernel reads it and
+    // the grids are sized by the capacity n / 8 + ncol + 1, which bounds
+    // the sum over columns of ceil(count / 8)
     k_fill_i32<<<cdiv(h->ncol, 256), 256, 0, st>>>(h->ncol, h->d_colcount, 0);
     k_bin<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_x, h->d_y, h->d_z, h->box, h->ncx, h->ncy, h->d_keys, h->d_vals,
                                          h->d_colcount);
-    int colbits = 1;
-    while ((1 << colbits) < h->ncol) ++colbits;
-    size_t tb = h->cub_bytes;
-    HIPCHK(h, hipcub::DeviceRadixSort::SortPairs(h->d_cub, tb, h->d_keys, h->d_keys2, h->d_vals, h->d_vals2, n, 0,
-                                                 32 + colbits, st));
-    tb = h->cub_bytes;
-    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_cub, tb, h->d_colcount, h->d_colstart, h->ncol, st));
+    k_scan_int<<<1, 1024, 0, st>>>(h->ncol, h->d_colcount, h->d_colstart, nullptr);
+    k_scatter<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_keys, h->d_vals, h->d_colstart, h->d_keys2);
+    k_col_sort<<<h->ncol, 256, 0, st>>>(h->d_colcount, h->d_colstart, h->d_keys2, h->d_keys, h->d_vals2);
     k_col_clusters<<<cdiv(h->ncol, 256), 256, 0, st>>>(h->ncol, h->d_colcount, h->d_colclu_n);
-    tb = h->cub_bytes;
-    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->d_cub, tb, h->d_colclu_n, h->d_colclu, h->ncol, st));
-    int last[2];
-    HIPCHK(h, hipMemcpyAsync(&last[0], h->d_colclu + h->ncol - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipMemcpyAsync(&last[1], h->d_colclu_n + h->ncol - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipStreamSynchronize(st));
-    h->nclu = last[0] + last[1];
-    if (h->nclu > h->nclu_cap) {
-        h->err = "internal: cluster count above bound";
-        return MDX_ERR_STATE;
-    }
-    const int nslot = h->nclu * kClu;
-    k_fill_i32<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, -1);
-    k_fill<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_keys2, h->d_vals2, h->d_colstart, h->d_colclu, h->d_clu_atom,
+    k_scan_int<<<1, 1024, 0, st>>>(h->ncol, h->d_colclu_n, h->d_colclu, h->d_nclu);
+    const int cap = h->nclu_cap, nslot_cap = cap * kClu;
+    k_fill_i32<<<cdiv(nslot_cap, 256), 256, 0, st>>>(nslot_cap, h->d_clu_atom, -1);
+    k_fill<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_keys, h->d_vals2, h->d_colstart, h->d_colclu, h->d_clu_atom,
                                           h->d_atom_slot, h->d_clu_col);
-    k_cluster_geom<<<cdiv(h->nclu, 256), 256, 0, st>>>(h->nclu, h->d_clu_atom, h->d_x, h->d_y, h->d_z, h->box,
-                                                        h->d_anchor, h->d_ext);
-    k_slot_pos<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, h->d_x, h->d_y, h->d_z, h->d_mol, h->d_spos);
+    k_cluster_geom<<<cdiv(cap, 256), 256, 0, st>>>(h->d_nclu, h->d_clu_atom, h->d_x, h->d_y, h->d_z, h->box,
+                                                    h->d_anchor, h->d_ext);
+    k_slot_pos<<<cdiv(nslot_cap, 256), 256, 0, st>>>(h->d_nclu, h->d_clu_atom, h->d_x, h->d_y, h->d_z, h->d_mol,
+                                                      h->d_spos);
     const double rl2 = list_radius2(h);
     const double rl = std::sqrt(rl2);
     const double cwx = h->p.box[0] / h->ncx, cwy = h->p.box[1] / h->ncy;
     const int rx = (int)std::ceil((rl + cwx) / cwx), ry = (int)std::ceil((rl + cwy) / cwy);
     for (int attempt = 0; attempt < 4; ++attempt) {
         k_fill_i32<<<1, 64, 0, st>>>(1, h->d_overflow, 0);
-        k_candidates<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_anchor, h->d_ext, h->d_clu_col, h->d_colclu,
+        k_candidates<<<cap, 64, 0, st>>>(h->d_nclu, h->d_anchor, h->d_ext, h->d_clu_col, h->d_colclu,
                                                           h->d_colclu_n, h->ncx, h->ncy, rx, ry, h->box, rl,
                                                           h->cand_cap, h->d_cand, h->d_ncand, h->d_overflow);
-        k_masks<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_cand, h->d_ncand, h->cand_cap, h->d_clu_atom, h->d_spos,
+        k_masks<<<cap, 64, 0, st>>>(h->d_nclu, h->d_cand, h->d_ncand, h->cand_cap, h->d_clu_atom, h->d_spos,
                                         h->d_anchor, h->d_ext, rl, h->d_moltype, h->d_type_ee, h->d_excl_start,
                                         h->d_excl_atom, h->d_excl_is14, h->box, rl2,
                                         h->p.rshort * h->p.rshort, h->d_list, h->d_nlist, h->d_nshort,
                                         h->list_cap, h->d_overflow);
-        int ovf = 0;
-        HIPCHK(h, hipMemcpyAsync(&ovf, h->d_overflow, sizeof(int), hipMemcpyDeviceToHost, st));
+        int got[2] = {0, 0};   // overflow, cluster count: the rebuild's one host synchronisation
+        HIPCHK(h, hipMemcpyAsync(&got[0], h->d_overflow, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipMemcpyAsync(&got[1], h->d_nclu, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(h, hipStreamSynchronize(st));
+        h->nclu = got[1];
+        if (h->nclu > h->nclu_cap) {
+            h->err = "internal: cluster count above bound";
+            return MDX_ERR_STATE;
+        }
+        const int ovf = got[0];
         if (ovf == 0) break;
         // grow and retry (the reference stops instead, forces.f:2188-2194)
         const int cap = (ovf + ovf / 4 + 63) / 64 * 64;
@@ -1440,12 +1432,12 @@ mdx_status mdx_build_list(mdx_engine *h) {
     }
     // transposed list for the j-reaction sums
     k_fill_i32<<<cdiv(2 * h->nclu, 256), 256, 0, st>>>(2 * h->nclu, (int *)h->d_jcnt, 0);
-    k_jcount<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_list, h->d_nlist, h->d_nshort, h->list_cap, h->d_jcnt,
+    k_jcount<<<h->nclu, 64, 0, st>>>(h->d_nclu, h->d_list, h->d_nlist, h->d_nshort, h->list_cap, h->d_jcnt,
                                      h->d_jdest);
-    k_scan_excl<<<1, 1024, 0, st>>>(h->nclu, h->d_jcnt, h->d_jstart);
-    k_jplace<<<h->nclu, 64, 0, st>>>(h->nclu, h->d_list, h->d_nlist, h->d_nshort, h->list_cap, h->d_jstart,
+    k_scan_excl<<<1, 1024, 0, st>>>(h->d_nclu, h->d_jcnt, h->d_jstart);
+    k_jplace<<<h->nclu, 64, 0, st>>>(h->d_nclu, h->d_list, h->d_nlist, h->d_nshort, h->list_cap, h->d_jstart,
                                      h->d_jcnt, h->d_jdest);
-    k_slot_info<<<cdiv(nslot, 256), 256, 0, st>>>(nslot, h->d_clu_atom, h->d_ljtype, h->d_moltype, h->d_type_ee,
+    k_slot_info<<<cdiv(nslot_cap, 256), 256, 0, st>>>(h->d_nclu, h->d_clu_atom, h->d_ljtype, h->d_moltype, h->d_type_ee,
                                                    h->d_q, h->d_aux);
     if (h->nspecial > 0)
         k_special_classify<<<cdiv(h->nspecial, 256), 256, 0, st>>>(h->nspecial, h->d_sp_pairs, h->d_sp_kind, h->d_x,

@@ -19,7 +19,7 @@ namespace mdx {
 // Wrap into [0, L) and bin into x-y columns; key = column << 32 | z rank.
 __global__ void k_bin(int n, const double *__restrict__ x, const double *__restrict__ y,
                       const double *__restrict__ z, Box b, int ncx, int ncy,
-                      unsigned long long *__restrict__ keys, int *__restrict__ vals,
+                      unsigned long long *__restrict__ keys, int *__restrict__ rank,
                       int *__restrict__ colcount) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -35,8 +35,74 @@ __global__ void k_bin(int n, const double *__restrict__ x, const double *__restr
     double fz = w[2] / b.l[2] * 4294967295.0;
     unsigned int zq = (unsigned int)fmin(fmax(fz, 0.0), 4294967295.0);
     keys[i] = ((unsigned long long)col << 32) | zq;
-    vals[i] = i;
-    atomicAdd(&colcount[col], 1);
+    rank[i] = atomicAdd(&colcount[col], 1);   // arrival rank in the column
+}
+
+// Exclusive prefix sum of n ints, one block of 1,024 threads (each sums a
+// contiguous chunk; chunk sums scanned per wavefront, then across the 16
+// wavefronts); *total, when given, receives the sum of all n.
+__global__ __launch_bounds__(1024) void k_scan_int(int n, const int *__restrict__ in, int *__restrict__ out,
+                                                   int *__restrict__ total) {
+    __shared__ int wtot[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (n + 1023) / 1024;
+    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
+    int sum = 0;
+    for (int i = i0; i < i1; ++i) sum += in[i];
+    int inc = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(inc, off);
+        if (lane >= off) inc += v;
+    }
+    if (lane == 63) wtot[w] = inc;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; ++k) base += wtot[k];
+    int run = base + inc - sum;
+    for (int i = i0; i < i1; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (total && t == 1023) *total = base + inc;
+}
+
+// Atoms into their column's range in arrival order, as the column-local
+// sort key (z code << 32 | atom index): unique, and ordered as the stable
+// sort by (column, z code) orders atoms of equal z code (by index).
+__global__ void k_scatter(int n, const unsigned long long *__restrict__ keys, const int *__restrict__ rank,
+                          const int *__restrict__ colstart, unsigned long long *__restrict__ tmp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long k = keys[i];
+    const int col = (int)(k >> 32);
+    tmp[colstart[col] + rank[i]] = ((k & 0xffffffffull) << 32) | (unsigned)i;
+}
+
+// One block per column: each atom's position in its column is the number
+// of the column's keys below its own (keys staged through LDS 1,024 at a
+// time); writes the sorted (column << 32 | z code) keys and atom indices.
+__global__ __launch_bounds__(256) void k_col_sort(const int *__restrict__ colcount, const int *__restrict__ colstart,
+                                                  const unsigned long long *__restrict__ tmp,
+                                                  unsigned long long *__restrict__ skeys, int *__restrict__ svals) {
+    __shared__ unsigned long long sk[1024];
+    const int col = blockIdx.x, t = threadIdx.x;
+    const int m = colcount[col], base = colstart[col];
+    for (int g = 0; g < m; g += 256) {
+        const int e = g + t;
+        const unsigned long long ke = e < m ? tmp[base + e] : ~0ull;
+        int cnt = 0;
+        for (int c0 = 0; c0 < m; c0 += 1024) {
+            const int c1 = min(m - c0, 1024);
+            __syncthreads();
+            for (int f = t; f < c1; f += 256) sk[f] = tmp[base + c0 + f];
+            __syncthreads();
+            for (int f = 0; f < c1; ++f) cnt += sk[f] < ke ? 1 : 0;
+        }
+        if (e < m) {
+            skeys[base + cnt] = ((unsigned long long)col << 32) | (ke >> 32);
+            svals[base + cnt] = (int)(ke & 0xffffffffull);
+        }
+    }
 }
 
 // Clusters per column: ceil(count / 8).
@@ -63,11 +129,11 @@ __global__ void k_fill(int n, const unsigned long long *__restrict__ skeys, cons
 
 // Cluster anchor (bounding-box centre of the wrapped coordinates, FP64)
 // and half extents.
-__global__ void k_cluster_geom(int nclu, const int *__restrict__ clu_atom, const double *__restrict__ x,
+__global__ void k_cluster_geom(const int *__restrict__ pnclu, const int *__restrict__ clu_atom, const double *__restrict__ x,
                                const double *__restrict__ y, const double *__restrict__ z, Box b,
                                double *__restrict__ anchor, float *__restrict__ ext) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nclu) return;
+    if (c >= *pnclu) return;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int s = 0; s < kClu; ++s) {
         const int a = clu_atom[c * kClu + s];
@@ -91,14 +157,14 @@ __global__ void k_cluster_geom(int nclu, const int *__restrict__ clu_atom, const
 // neighbour column (periodic wrap, each column once) are tested 64 at a
 // time and the hits compacted in column order with a ballot, so the
 // candidate order is that of a serial scan.
-__global__ __launch_bounds__(64) void k_candidates(int nclu, const double *__restrict__ anchor,
+__global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu, const double *__restrict__ anchor,
                                                    const float *__restrict__ ext, const int *__restrict__ clu_col,
                                                    const int *__restrict__ colclu, const int *__restrict__ colclu_n,
                                                    int ncx, int ncy, int rx, int ry, Box b, double rlist, int cap,
                                                    int *__restrict__ cand, int *__restrict__ ncand,
                                                    int *__restrict__ overflow) {
     const int ci = blockIdx.x;
-    if (ci >= nclu) return;
+    if (ci >= *pnclu) return;
     const int lane = threadIdx.x;
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const int col = clu_col[ci];
@@ -157,7 +223,7 @@ __device__ inline int bound_kind(int i, int j, const int *__restrict__ excl_star
 // Pair masks for every candidate of an i-cluster; one wavefront per
 // i-cluster, lane = i_slot * 8 + j_slot.
 __global__ __launch_bounds__(64) void k_masks(
-    int nclu, const int *__restrict__ cand, const int *__restrict__ ncand, int cap,
+    const int *__restrict__ pnclu, const int *__restrict__ cand, const int *__restrict__ ncand, int cap,
     const int *__restrict__ clu_atom, const double4 *__restrict__ spos, const double *__restrict__ anchor,
     const float *__restrict__ ext, double rlist, const int *__restrict__ atom_moltype, const int *__restrict__ type_ee, const int *__restrict__ excl_start,
     const int *__restrict__ excl_atom, const signed char *__restrict__ excl_is14, Box b, double rl2,
@@ -165,6 +231,7 @@ __global__ __launch_bounds__(64) void k_masks(
     int list_cap, int *__restrict__ overflow) {
 #pragma clang fp contract(off)
     const int ci = blockIdx.x;
+    if (ci >= *pnclu) return;
     const int lane = threadIdx.x;
     const int il = lane >> 3, jl = lane & 7;
     const int ai = clu_atom[ci * kClu + il];
@@ -253,11 +320,11 @@ __global__ __launch_bounds__(64) void k_masks(
 // k_jplace turns each entry's rank into its record position (jdest, per
 // list slot).  The order inside each part follows the atomics; the sums
 // over it are integer sums, so results do not depend on that order.
-__global__ __launch_bounds__(64) void k_jcount(int nclu, const PairEntry *__restrict__ list,
+__global__ __launch_bounds__(64) void k_jcount(const int *__restrict__ pnclu, const PairEntry *__restrict__ list,
                                                const int *__restrict__ nlist, const int *__restrict__ nshort,
                                                int list_cap, int2 *__restrict__ jcnt, unsigned *__restrict__ jrank) {
     const int ci = blockIdx.x;
-    if (ci >= nclu) return;
+    if (ci >= *pnclu) return;
     const int n = nlist[ci], ns = nshort[ci];
     for (int k = threadIdx.x; k < n; k += 64) {
         const size_t p = (size_t)ci * list_cap + (k < ns ? k : list_cap - 1 - (k - ns));
@@ -266,12 +333,12 @@ __global__ __launch_bounds__(64) void k_jcount(int nclu, const PairEntry *__rest
     }
 }
 
-__global__ __launch_bounds__(64) void k_jplace(int nclu, const PairEntry *__restrict__ list,
+__global__ __launch_bounds__(64) void k_jplace(const int *__restrict__ pnclu, const PairEntry *__restrict__ list,
                                                const int *__restrict__ nlist, const int *__restrict__ nshort,
                                                int list_cap, const unsigned *__restrict__ jstart,
                                                const int2 *__restrict__ jcnt, unsigned *__restrict__ jdest) {
     const int ci = blockIdx.x;
-    if (ci >= nclu) return;
+    if (ci >= *pnclu) return;
     const int n = nlist[ci], ns = nshort[ci];
     for (int k = threadIdx.x; k < n; k += 64) {
         const size_t p = (size_t)ci * list_cap + (k < ns ? k : list_cap - 1 - (k - ns));
@@ -286,8 +353,10 @@ __global__ __launch_bounds__(64) void k_jplace(int nclu, const PairEntry *__rest
 // sums a contiguous chunk, the chunk sums are scanned across the block
 // (wavefront scans, then the 16 wavefront totals), and each thread writes
 // its chunk's prefixes.
-__global__ __launch_bounds__(1024) void k_scan_excl(int n, const int2 *__restrict__ in, unsigned *__restrict__ out) {
+__global__ __launch_bounds__(1024) void k_scan_excl(const int *__restrict__ pn, const int2 *__restrict__ in,
+                                                    unsigned *__restrict__ out) {
     __shared__ unsigned wtot[16];
+    const int n = *pn;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n + 1023) / 1024;
     const int i0 = min(n, t * per), i1 = min(n, i0 + per);
@@ -312,11 +381,11 @@ __global__ __launch_bounds__(1024) void k_scan_excl(int n, const int2 *__restric
 
 // Per-slot FP64 coordinates (the caller's, as the reference differences
 // them) and molecule index (-1: empty slot) for the mask pass.
-__global__ void k_slot_pos(int nslot, const int *__restrict__ clu_atom, const double *__restrict__ x,
+__global__ void k_slot_pos(const int *__restrict__ pnclu, const int *__restrict__ clu_atom, const double *__restrict__ x,
                            const double *__restrict__ y, const double *__restrict__ z,
                            const int *__restrict__ atom_mol, double4 *__restrict__ spos) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslot) return;
+    if (s >= *pnclu * kClu) return;
     const int a = clu_atom[s];
     spos[s] = a >= 0 ? make_double4(x[a], y[a], z[a], (double)atom_mol[a]) : make_double4(0.0, 0.0, 0.0, -1.0);
 }

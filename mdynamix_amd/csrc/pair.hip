@@ -711,11 +711,11 @@ __global__ __launch_bounds__(64) void k_jgather(const PairArgs *__restrict__ arg
 }
 
 // Per-slot LJ type, ee flag and the low part of the charge (at list build).
-__global__ void k_slot_info(int nslot, const int *__restrict__ clu_atom, const int *__restrict__ atom_ljtype,
+__global__ void k_slot_info(const int *__restrict__ pnclu, const int *__restrict__ clu_atom, const int *__restrict__ atom_ljtype,
                             const int *__restrict__ atom_moltype, const int *__restrict__ type_ee,
                             const double *__restrict__ q, int2 *__restrict__ aux) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslot) return;
+    if (s >= *pnclu * kClu) return;
     const int a = clu_atom[s];
     int2 v = make_int2(0, 0);
     if (a >= 0) {
