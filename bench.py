@@ -304,13 +304,42 @@ def bench_batched(steps: int, warmup: int, beads: int = 32, n_water: int = 4000)
     pairs_a = run(warmup, steps, False)
     ta = time.perf_counter() - t0
     ring.close()
+    pimd = bench_pimd_md(bs, engs, topo, max(steps // 2, 5), warmup)
     for e in engs:
         e.close()
     kterms = _count_k(prm) * topo.natoms * beads * steps
     return {"workload": "pimd_%dx%d_spcfw" % (beads, n_water), "beads": beads, "atoms_per_bead": topo.natoms,
             "steps": steps, "ms_per_call": 1e3 * tb / steps, "pair_interactions_per_s": pairs / tb,
             "one_at_a_time": {"ms_per_call": 1e3 * ta / steps, "pair_interactions_per_s": pairs_a / ta},
-            "speedup": ta / tb, "roofline": call_roofline(pairs / tb, kterms / tb)}
+            "speedup": ta / tb, "roofline": call_roofline(pairs / tb, kterms / tb), "pimd_md": pimd}
+
+
+def bench_pimd_md(bs, engs, topo, long_steps: int, warmup: int) -> dict:
+    """The same beads advanced by the device path-integral loop (PIMD,
+    pimd.f:18-314): 1 fs long steps of 4 short steps, bead masses m / P,
+    ring springs of the physical BEADFAC at 300 K, BEADNOSE chains (5 fs),
+    centroid GETCOM, list rebuild every 10 long steps; every short step is
+    a fused force call over all beads."""
+    from mdynamix_amd import abi, systems, topology
+    from mdynamix_amd.engine import ForceBatch
+    P = len(engs)
+    md = topology.md_params(bs[0])
+    md.update(integrator=abi.MDX_MD_PIMD, nfreq=4, dt=1.0, tau_t=0.0, rebuild_every=10)
+    # Maxwell velocities of the bead mass m / P at 300 K (sqrt(P) x those of m)
+    for k, (e, b) in enumerate(zip(engs, bs)):
+        e.set_positions(b.pos)
+        e.md_init(md, systems.maxwell_velocities(b, 100 + k) * np.sqrt(P))
+    ring = ForceBatch(engs, ring_beadfac=topology.pimd_beadfac(300.0, P))
+    ring.pimd_init(temperature=300.0, tau_bead=5.0, nchain=1)
+    ring.md_run(warmup)
+    t0 = time.perf_counter()
+    tr = ring.md_run(long_steps)
+    t = time.perf_counter() - t0
+    ring.close()
+    ms = 1e3 * t / long_steps
+    return {"integrator": "PIMD (beads' multiple time step, BEADNOSE)", "dt_fs": 1.0, "nfreq": 4,
+            "long_steps": long_steps, "ms_per_long_step": ms, "ns_per_day": 86400.0 / (ms * 1e-3) * 1e-6,
+            "final_mean_bead_temperature_K": float(np.mean(tr[:, -1, 2])) if tr is not None else None}
 
 
 def bench_respa(long_steps: int, warmup: int) -> dict:

@@ -112,9 +112,11 @@ def _springs(beads, mass, box, beadfac):
     return out
 
 
-def test_pimd_ring_matches_beads_alone_plus_springs():
-    P = 4
-    beads = systems.pimd_beads(n_beads=P, n_water=400, box=22.9, seed=41)
+@pytest.mark.parametrize("P, n_water, box", [(4, 400, 22.9), (32, 4000, 49.3)])
+def test_pimd_ring_matches_beads_alone_plus_springs(P, n_water, box):
+    """4 beads of 400 waters, and the benchmark's ring: 32 beads of 4,000
+    flexible SPC-FW waters (12,000 atoms each, 49.3 A box)."""
+    beads = systems.pimd_beads(n_beads=P, n_water=n_water, box=box, seed=41)
     engs = []
     for s in beads:
         topo, prm = topology.from_system(s)
@@ -150,3 +152,5 @@ def test_pimd_ring_matches_beads_alone_plus_springs():
                 for c, key in enumerate(("virfx", "virfy", "virfz")):
                     assert eb[k][key] == pytest.approx(ea[key] + virf[c], rel=1e-11, abs=1e-9)
     ring.close()
+    for eng in engs:
+        eng.close()

@@ -121,3 +121,40 @@ def test_pimd_graph_replay_matches_eager():
         for u, v in zip(x, y):
             assert np.array_equal(u, v)
     assert np.array_equal(a[-1], b[-1])
+
+
+def test_pimd_benchmark_ring_graph_matches_eager():
+    """The benchmark's ring (32 beads x 4,000 flexible SPC-FW waters,
+    12,000 atoms each, 49.3 A, 300 K): 3 long steps of the device PIMD loop
+    (1 fs, 4 short steps, BEADNOSE 5 fs, list rebuild every 2 long steps)
+    replayed from the captured hipGraph give the bits of the eager launches,
+    and the bead temperatures stay finite and positive."""
+    from mdynamix_amd import systems as sysm
+    bs = sysm.pimd_beads(n_beads=32, n_water=4000, box=49.3, seed=2027)
+    topo, prm = topology.from_system(bs[0])
+    md = topology.md_params(bs[0])
+    md.update(integrator=abi.MDX_MD_PIMD, nfreq=4, dt=1.0, tau_t=0.0, rebuild_every=2)
+    out = []
+    for graph in (False, True):
+        engs = [ForceEngine(topo, prm) for _ in bs]
+        ring = None
+        try:
+            for k, (e, b) in enumerate(zip(engs, bs)):
+                e.set_positions(b.pos)
+                e.md_init(md, sysm.maxwell_velocities(b, 100 + k))
+            ring = ForceBatch(engs, ring_beadfac=topology.pimd_beadfac(300.0, 32))
+            ring.pimd_init(temperature=300.0, tau_bead=5.0, nchain=1)
+            ring.lib.mdx_batch_use_graph(ring._b, 1 if graph else 0)
+            print("case pimd ring 32 x 12000 graph", graph, flush=True)
+            tr = ring.md_run(3)
+            out.append(([e.md_state() for e in engs], ring.pimd_chains(), tr))
+        finally:
+            if ring is not None:
+                ring.close()
+            for e in engs:
+                e.close()
+    (sa, xa, ta), (sb, xb, tb) = out
+    for (pa, va), (pb, vb) in zip(sa, sb):
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    assert np.array_equal(xa, xb) and np.array_equal(ta, tb)
+    assert np.all(np.isfinite(ta)) and np.all(ta[:, :, 2] > 0.0)
