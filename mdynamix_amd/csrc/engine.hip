@@ -258,6 +258,9 @@ struct mdx_batch {
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev[12] = {};
+    // the two side streams of a forked force call and its fork/join events
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
     bool timing = false;
     double stage_ms[ST_N] = {};
 };
@@ -1740,13 +1743,21 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
 }
 
 template <typename T, typename F>
-void launch(mdx_batch *b, KPlan<T> &k, int nthreads, size_t lds, F kernel) {
+void launch(mdx_batch *b, KPlan<T> &k, int nthreads, size_t lds, F kernel, hipStream_t st = nullptr) {
     if (k.nblocks() == 0) return;
-    kernel<<<k.nblocks(), nthreads, lds, b->stream>>>(k.d_args, k.d_map);
+    kernel<<<k.nblocks(), nthreads, lds, st ? st : b->stream>>>(k.d_args, k.d_map);
 }
 
+void launch_forked(mdx_batch *b, int group, bool detail, bool terms);
+
 // The kernel sequence of one force call over all systems of the batch.
+// With stage timing the kernels run in order on the batch stream, with an
+// event between stages; otherwise the call forks (launch_forked).
 void launch_call(mdx_batch *b, int group, bool timing, bool detail, bool terms = false) {
+    if (!timing) {
+        launch_forked(b, group, detail, terms);
+        return;
+    }
     hipStream_t st = b->stream;
     auto mark = [&](int k) { if (timing) hipEventRecord(b->ev[k], st); };
     mark(0);
@@ -1831,6 +1842,99 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail, bool terms =
         hipMemcpyAsync(b->h_det, b->d_det, sizeof(double) * kMaxDet * b->sys.size(), hipMemcpyDeviceToHost, st);
     }
     mark(9);
+}
+
+// One force call as three chains that share nothing until the force
+// combine: after the fill kernel (which clears the energy rows every
+// kernel then writes, and the fixed-point accumulators of the scattered
+// terms), the batch stream runs the packing, the pair kernel and its
+// gather; side stream 0 the reciprocal sum; side stream 1 the special
+// pairs, exclusion corrections and bonded terms (integer atomics into the
+// cleared accumulators).  The batch stream waits for both side streams
+// before the combine.  Every sum has a fixed order or is an integer sum,
+// so the results do not depend on how the chains interleave; the fork and
+// join are events, so the sequence can be captured into a graph.
+void launch_forked(mdx_batch *b, int group, bool detail, bool terms) {
+    hipStream_t st = b->stream, sk = b->side[0], sb = b->side[1];
+    launch(b, b->zero, 256, 0, k_zero);
+    hipEventRecord(b->fork_ev, st);
+    hipStreamWaitEvent(sk, b->fork_ev, 0);
+    hipStreamWaitEvent(sb, b->fork_ev, 0);
+    // side stream 0: the reciprocal sum
+    if (group != MDX_FORCES_FAST) {
+        launch(b, b->rprep, 256, 0, k_recip_prep, sk);
+        if (b->mz <= 16) launch(b, b->sk, 256, 0, k_sk<3>, sk);
+        else if (b->mz <= 32) launch(b, b->sk, 256, 0, k_sk<5>, sk);
+        else launch(b, b->sk, 256, 0, k_sk<7>, sk);
+        launch(b, b->skred, 256, 0, k_sk_reduce, sk);
+        if (b->mz <= 16) launch(b, b->rforce, 256, 0, k_recip_force<16>, sk);
+        else if (b->mz <= 32) launch(b, b->rforce, 256, 0, k_recip_force<32>, sk);
+        else launch(b, b->rforce, 256, 0, k_recip_force<48>, sk);
+        for (int s : b->fur64) {
+            mdx_engine *h = b->sys[s];
+            const int n = h->n;
+            if (h->nkv > 0)
+                fur_launch(h->p, h->kbox, n, h->nkv, h->d_x, h->d_y, h->d_z, h->d_q, h->d_rpart, h->d_rpart + n,
+                           h->d_rpart + 2 * (size_t)n, nullptr, erow(h, h->row_sk), h->d_furws, sk);
+        }
+    }
+    // side stream 1: special pairs, exclusions, bonded terms, springs
+    if (b->special.nblocks()) {
+        if (detail) k_special<true><<<b->special.nblocks(), 256, 0, sb>>>(b->special.d_args, b->special.d_map, group);
+        else k_special<false><<<b->special.nblocks(), 256, 0, sb>>>(b->special.d_args, b->special.d_map, group);
+    }
+    if (group != MDX_FORCES_FAST) launch(b, b->et, 256, 0, k_eterms, sb);
+    if (group != MDX_FORCES_SLOW) {
+        launch(b, b->spc, 256, 0, k_strbnd, sb);
+        launch(b, b->bonds, 256, 0, k_bonds, sb);
+        launch(b, b->angles, 256, 0, k_angles, sb);
+        launch(b, b->tors, 256, 0, k_tors, sb);
+        launch(b, b->springs, 256, 0, k_springs, sb);
+        if (terms) launch(b, b->terms, 256, 0, k_term_reduce, sb);
+    }
+    // batch stream: packing, molecule centres, the pair stage
+    launch(b, b->pack, 256, 0, k_pack);
+    launch(b, b->com, 64, 0, k_com);
+    const size_t lj = sizeof(float2) * b->nlj_max * b->nlj_max;
+    const size_t ldet = lj + sizeof(double) * (b->nmt_max + 1) * 2 * 64 + sizeof(int) * 8;
+    if (b->pairs.nblocks()) {
+        const int nb = b->pairs.nblocks();
+        const PairArgs *pa = b->pairs.d_args;
+        const int2 *pm = b->pairs.d_map;
+        if (detail) {
+            if (b->soft) k_pairs<true, true><<<nb, 64, ldet, st>>>(pa, pm, group);
+            else k_pairs<false, true><<<nb, 64, ldet, st>>>(pa, pm, group);
+        } else {
+            if (b->soft) k_pairs<true, false><<<nb, 64, lj, st>>>(pa, pm, group);
+            else k_pairs<false, false><<<nb, 64, lj, st>>>(pa, pm, group);
+        }
+        k_jgather<<<nb, 64, 0, st>>>(pa, pm, group);
+    }
+    if (b->pairs64.nblocks()) {
+        const int nb = b->pairs64.nblocks();
+        const PairArgs *pa = b->pairs64.d_args;
+        const int2 *pm = b->pairs64.d_map;
+        const size_t l64 = sizeof(double) * (b->nmt_max + 1) * 2 * 64 + sizeof(int) * 8;
+        if (detail) {
+            if (b->soft) k_pairs64<true, true><<<nb, 64, l64, st>>>(pa, pm, group);
+            else k_pairs64<false, true><<<nb, 64, l64, st>>>(pa, pm, group);
+        } else {
+            if (b->soft) k_pairs64<true, false><<<nb, 64, 0, st>>>(pa, pm, group);
+            else k_pairs64<false, false><<<nb, 64, 0, st>>>(pa, pm, group);
+        }
+    }
+    // join
+    hipEventRecord(b->join_ev[0], sk);
+    hipEventRecord(b->join_ev[1], sb);
+    hipStreamWaitEvent(st, b->join_ev[0], 0);
+    hipStreamWaitEvent(st, b->join_ev[1], 0);
+    if (b->comb.nblocks()) k_combine<<<b->comb.nblocks(), 256, 0, st>>>(b->comb.d_args, b->comb.d_map, group);
+    launch(b, b->red, 1024, 0, k_ereduce);
+    hipMemcpyAsync(b->h_energy, b->d_energy, sizeof(double) * Q_N * b->sys.size(), hipMemcpyDeviceToHost, st);
+    if (detail) {
+        launch(b, b->dred, 256, 0, k_detail_reduce);
+        hipMemcpyAsync(b->h_det, b->d_det, sizeof(double) * kMaxDet * b->sys.size(), hipMemcpyDeviceToHost, st);
+    }
 }
 
 // Checks and plan refresh before a force call (argument structs of systems
@@ -2652,6 +2756,16 @@ mdx_status mdx_batch_create(mdx_engine *const *systems, int32_t k, mdx_batch **o
     }
     b->own_stream = true;
     for (int e = 0; e < 12; ++e) hipEventCreate(&b->ev[e]);
+    for (int k = 0; k < 2; ++k)
+        if (hipStreamCreate(&b->side[k]) != hipSuccess ||
+            hipEventCreateWithFlags(&b->join_ev[k], hipEventDisableTiming) != hipSuccess) {
+            *out = b;
+            return MDX_ERR_HIP;
+        }
+    if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) {
+        *out = b;
+        return MDX_ERR_HIP;
+    }
     if (hipMalloc((void **)&b->d_energy, sizeof(double) * Q_N * k) != hipSuccess ||
         hipHostMalloc((void **)&b->h_energy, sizeof(double) * Q_N * k, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&b->d_det, sizeof(double) * kMaxDet * k) != hipSuccess ||
@@ -2846,6 +2960,11 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     if (b->h_det) hipHostFree(b->h_det);
     for (int e = 0; e < 12; ++e)
         if (b->ev[e]) hipEventDestroy(b->ev[e]);
+    for (int k = 0; k < 2; ++k) {
+        if (b->join_ev[k]) hipEventDestroy(b->join_ev[k]);
+        if (b->side[k]) hipStreamDestroy(b->side[k]);
+    }
+    if (b->fork_ev) hipEventDestroy(b->fork_ev);
     if (b->own_stream && b->stream) hipStreamDestroy(b->stream);
     delete b;
     return MDX_OK;

@@ -112,9 +112,10 @@ __device__ inline float mul_unfused(float a, float b) {
 }
 
 // x + x(lane ^ 8): the two halves of each row of 16 lanes added (DPP
-// row_ror:8 folded into the add).
+// row_ror:8; every lane reads a lane of its own row, so the move has no
+// old value to keep and folds into the add as its DPP operand).
 __device__ inline float add_xor8(float x) {
-    return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x128, 0xf, 0xf, false));
+    return x + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x128, 0xf, 0xf, false));
 }
 
 // j reactions of the half list leave the pair kernel as a record per list
@@ -280,8 +281,7 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
             dy = fmaf(-bl[1], rintf(dy * bi[1]), dy);
             dz = fmaf(-bl[2], rintf(dz * bi[2]), dz);
         }
-        const float dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
-        const float r2 = dx2 + dy2 + dz2;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         // lanes in the list and inside the cutoff, as a wave mask; pairs
         // within the FP32 error band of the cutoff take the reference's
         // FP64 test (no band for reference lists: rlo = rhi)
@@ -317,8 +317,6 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
         const float fsc = (SOFT ? rinv * rei : r2i) * t2;   // |F| / r
         // every other pair takes the same operations in both instantiations
         const float vr = softp ? w6 * (rei * r) : w6;
-        // reactions on j (minus the force on i)
-        const float nx = -fsc * dx, ny = -fsc * dy, nz = -fsc * dz;
         const float qlo = __int_as_float(auxj.y);
         if (DETAIL && in) {
             const int aj = a.clu_atom[jclu(e) * kClu + jl];
@@ -329,13 +327,19 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
         if (MIXED) {
             const bool isshort = __builtin_amdgcn_inverse_ballot_w64(e.mshort);
             const float ws = isshort ? 1.f : 0.f, wl = 1.f - ws;
-            const float sx = ws * nx, sy = ws * ny, sz = ws * nz;
-            const float lx = nx - sx, ly = ny - sy, lz = nz - sz;
-            const float fss = ws * fsc, fsl = fsc - fss;
+            // each class's |F| / r is fsc or exactly 0 (a product with its
+            // 0/1 weight), and its reactions on j (minus the force on i)
+            // are formed from it, so the other class gets exact zeros; a
+            // difference such as fsc - ws fsc could keep the rounding error
+            // of a product that contraction fused into it
+            const float fss = ws * fsc, fsl = wl * fsc;
+            const float sx = -fss * dx, sy = -fss * dy, sz = -fss * dz;
+            const float lx = -fsl * dx, ly = -fsl * dy, lz = -fsl * dz;
             s.fs[0] = fmaf(fss, dx, s.fs[0]); s.fs[1] = fmaf(fss, dy, s.fs[1]); s.fs[2] = fmaf(fss, dz, s.fs[2]);
             s.fl[0] = fmaf(fsl, dx, s.fl[0]); s.fl[1] = fmaf(fsl, dy, s.fl[1]); s.fl[2] = fmaf(fsl, dz, s.fl[2]);
-            s.vs[0] = fmaf(fss, dx2, s.vs[0]); s.vs[1] = fmaf(fss, dy2, s.vs[1]); s.vs[2] = fmaf(fss, dz2, s.vs[2]);
-            s.vl[0] = fmaf(fsl, dx2, s.vl[0]); s.vl[1] = fmaf(fsl, dy2, s.vl[1]); s.vl[2] = fmaf(fsl, dz2, s.vl[2]);
+            // virial f . d per axis from the reactions (-sx dx = fss dx^2)
+            s.vs[0] = fmaf(-sx, dx, s.vs[0]); s.vs[1] = fmaf(-sy, dy, s.vs[1]); s.vs[2] = fmaf(-sz, dz, s.vs[2]);
+            s.vl[0] = fmaf(-lx, dx, s.vl[0]); s.vl[1] = fmaf(-ly, dy, s.vl[1]); s.vl[2] = fmaf(-lz, dz, s.vl[2]);
             const float ees = fmaf(qlo, phi, pj.w * phi);
             s.es = fmaf(ws, ees, s.es); s.ljs = fmaf(ws, e12, s.ljs); s.vrs = fmaf(ws, vr, s.vrs);
             s.el = fmaf(wl, ees, s.el); s.lj = fmaf(wl, e12, s.lj); s.vr = fmaf(wl, vr, s.vr);
@@ -346,8 +350,10 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
             put(d0, kPatMix0, fr0, slot);
             put(d1, kPatMix1, fr1, slot);
         } else {
+            // reactions on j (minus the force on i)
+            const float nx = -fsc * dx, ny = -fsc * dy, nz = -fsc * dz;
             s.fl[0] = fmaf(fsc, dx, s.fl[0]); s.fl[1] = fmaf(fsc, dy, s.fl[1]); s.fl[2] = fmaf(fsc, dz, s.fl[2]);
-            s.vl[0] = fmaf(fsc, dx2, s.vl[0]); s.vl[1] = fmaf(fsc, dy2, s.vl[1]); s.vl[2] = fmaf(fsc, dz2, s.vl[2]);
+            s.vl[0] = fmaf(-nx, dx, s.vl[0]); s.vl[1] = fmaf(-ny, dy, s.vl[1]); s.vl[2] = fmaf(-nz, dz, s.vl[2]);
             s.el = fmaf(pj.w, phi, s.el); s.el_lo = fmaf(qlo, phi, s.el_lo);
             s.a6 = fmaf(lj.x, r6i, s.a6);
             if (SOFT) s.w6 += w6;
