@@ -402,6 +402,14 @@ __device__ inline void combine_body(const CombineArgs &a, int blk, int group) {
             H[1] += fixd(a.spy, i, a.inv_scale);
             H[2] += fixd(a.spz, i, a.inv_scale);
         }
+        // every accumulator word of the atom back to zero, read or not
+        // (a class not evaluated in this call discards what it got)
+        a.gx[i] = 0ull; a.gy[i] = 0ull; a.gz[i] = 0ull;
+        a.hlx[i] = 0ull; a.hly[i] = 0ull; a.hlz[i] = 0ull;
+        a.hbx[i] = 0ull; a.hby[i] = 0ull; a.hbz[i] = 0ull;
+        if (a.spx) { a.spx[i] = 0ull; a.spy[i] = 0ull; a.spz[i] = 0ull; }
+        if (a.clear_jf)
+            for (int k = 0; k < 8; ++k) a.jf[(size_t)a.atom_slot[i] * 8 + k] = 0ull;
         const int m = a.atom_mol[i];
         const double wx = a.x[i] - a.cx[m], wy = a.y[i] - a.cy[m], wz = a.z[i] - a.cz[m];
         v[0] = -G[0] * wx - G[1] * wy - G[2] * wz;
@@ -421,19 +429,23 @@ __device__ inline void combine_body(const CombineArgs &a, int blk, int group) {
     block_reduce_store<2>(v, slots, o);
 }
 
-// Sum rows [0, nrows) of epart; block q (1,024 threads) sums slot q in a
+// Sum rows [0, nrows) of epart, clearing each value once read (the next
+// call then starts from zeros); block q (1,024 threads) sums slot q in a
 // fixed order: thread t keeps eight running sums over rows t + 1024 (8 i +
 // u), u = 0..7 (eight loads in flight per thread), adds them pairwise,
 // then butterflies inside each wavefront and the 16 wavefronts' sums in
 // order.
 __device__ inline void ereduce_body(const ReduceArgs &a, int q, double *wred) {
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const double *e = a.epart + q;
+    double *e = a.epart + q;
     for (int r0 = threadIdx.x; r0 < a.nrows; r0 += 8192)
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int r = r0 + 1024 * u;
-            if (r < a.nrows) s[u] += e[(size_t)r * Q_N];
+            if (r < a.nrows) {
+                s[u] += e[(size_t)r * Q_N];
+                e[(size_t)r * Q_N] = 0.0;   // cleared for the next call
+            }
         }
     double v = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -444,13 +456,6 @@ __device__ inline void ereduce_body(const ReduceArgs &a, int q, double *wred) {
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t += wred[w];
         a.out[q] = t;
     }
-}
-
-// Clear accumulators (a kernel, not a memset node, so the call sequence
-// can be captured in a graph).
-__device__ inline void zero_body(const ZeroSeg &z, int blk) {
-    const long long i = (long long)blk * 256 + threadIdx.x;
-    if (i < z.n) z.p[i] = 0ull;
 }
 
 // Path-integral springs to both neighbouring beads (PIFORCES,
@@ -507,7 +512,6 @@ MDX_BATCHED(k_strbnd, SpcArgs, strbnd_body, 256)
 MDX_BATCHED(k_angles, AngleArgs, angles_body, 256)
 MDX_BATCHED(k_tors, TorsArgs, tors_body, 256)
 MDX_BATCHED(k_springs, SpringArgs, springs_body, 256)
-MDX_BATCHED(k_zero, ZeroSeg, zero_body, 256)
 
 __global__ __launch_bounds__(256) void k_combine(const CombineArgs *__restrict__ args, const int2 *__restrict__ map,
                                                  int group) {

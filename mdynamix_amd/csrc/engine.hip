@@ -95,7 +95,7 @@ struct mdx_engine {
     TorsArgs a_tors{};
     long long version = 1;
     // per-system rows of the energy partials (fixed by capacities)
-    int row_pairs = 0, row_sp = 0, row_sk = 0, row_et = 0, row_b = 0, row_s = 0, row_a = 0, row_t = 0, row_spr = 0,
+    int row_pairs = 0, row_pp = 0, row_sp = 0, row_sk = 0, row_et = 0, row_b = 0, row_s = 0, row_a = 0, row_t = 0, row_spr = 0,
         row_c = 0, rows_total = 0;
     unsigned long long *d_fixsp = nullptr;        // [3][n] bead springs
     std::map<void *, size_t> dev_sizes;           // live device allocations
@@ -225,7 +225,6 @@ struct mdx_batch {
     bool planned = false;
     int nlj_max = 1;
     bool soft = false;
-    KPlan<ZeroSeg> zero;
     KPlan<PackArgs> pack, com;
     KPlan<PairArgs> pairs;
     KPlan<PairArgs> pairs64;         // systems in the FP64 precision mode
@@ -241,7 +240,7 @@ struct mdx_batch {
     KPlan<TermArgs> terms;
     KPlan<SpringArgs> springs;
     KPlan<CombineArgs> comb;
-    KPlan<ReduceArgs> red;
+    KPlan<ReduceArgs> red, redp;    // energy rows; the pair rows first, on the pair stream
     KPlan<DetailArgs> dred;
     int nmt_max = 1;
     double *d_det = nullptr;         // [K][kMaxDet]
@@ -307,6 +306,17 @@ mdx_status dalloc(mdx_engine *h, T **p, size_t count) {
         return MDX_ERR_NOMEM;
     }
     h->dev_sizes[(void *)*p] = sizeof(T) * count;
+    return MDX_OK;
+}
+
+// A device buffer cleared once at allocation.  The accumulators and energy
+// rows of a force call are cleared again by the kernels that read them
+// (force combine, energy reduction), so every call starts from zeros
+// without a fill kernel of its own.
+template <typename T>
+mdx_status dalloc_zero(mdx_engine *h, T **p, size_t count) {
+    CHK(dalloc(h, p, count));
+    if (!h->dry) HIPCHK(h, hipMemset(*p, 0, sizeof(T) * std::max<size_t>(count, 1)));
     return MDX_OK;
 }
 
@@ -676,8 +686,8 @@ mdx_status alloc_clusters(mdx_engine *h) {
     // per-call buffers
     CHK(dalloc(h, &h->d_fs, (size_t)3 * h->n));
     CHK(dalloc(h, &h->d_fl, (size_t)3 * h->n));
-    CHK(dalloc(h, &h->d_fix, (size_t)9 * h->n));
-    CHK(dalloc(h, &h->d_fixj, (size_t)h->nclu_cap * kClu * 8));
+    CHK(dalloc_zero(h, &h->d_fix, (size_t)9 * h->n));
+    CHK(dalloc_zero(h, &h->d_fixj, (size_t)h->nclu_cap * kClu * 8));
     CHK(dalloc(h, &h->d_out, (size_t)9 * h->n));
     return MDX_OK;
 }
@@ -721,6 +731,9 @@ double list_radius2(const mdx_engine *h) {
 void layout_rows(mdx_engine *h) {
     int r = 0;
     h->row_pairs = r; r += h->nclu_cap;
+    // the pair rows' sum (formed on the pair kernel's stream), first of
+    // the rows the final reduction reads
+    h->row_pp = r; r += 1;
     h->row_sp = r; r += h->nspecial ? cdiv(h->nspecial, 256) : 0;
     h->row_sk = r; r += cdiv(h->npk + h->nfrows, 256);
     h->row_et = r; r += h->neterm ? cdiv(h->neterm, 256) : 0;
@@ -1227,9 +1240,9 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
     for (int i = 0; i < n; ++i) h->qabs += std::fabs(t->atom_charge[i]);
     if ((s = setup_kspace(h))) return fail(s);
     if ((s = alloc_clusters(h))) return fail(s);
-    if ((s = dalloc(h, &h->d_fixsp, (size_t)3 * n))) return fail(s);
+    if ((s = dalloc_zero(h, &h->d_fixsp, (size_t)3 * n))) return fail(s);
     layout_rows(h);
-    if ((s = dalloc(h, &h->d_epart, (size_t)h->rows_total * Q_N))) return fail(s);
+    if ((s = dalloc_zero(h, &h->d_epart, (size_t)h->rows_total * Q_N))) return fail(s);
     fill_args(h);
     if (dry) {
         *out = h;
@@ -1360,7 +1373,7 @@ mdx_status mdx_set_box(mdx_engine *h, const double box[3]) {
     if (redo) {
         CHK(setup_kspace(h));
         layout_rows(h);
-        CHK(dalloc(h, &h->d_epart, (size_t)h->rows_total * Q_N));
+        CHK(dalloc_zero(h, &h->d_epart, (size_t)h->rows_total * Q_N));
     }
     h->have_list = false;
     fill_args(h);
@@ -1632,12 +1645,12 @@ void drop_graphs(mdx_batch *b) {
 // maps = true the block maps are rebuilt too (first use, box changes).
 mdx_status plan_fill(mdx_batch *b, bool maps) {
     const int K = (int)b->sys.size();
-    b->zero.clear(); b->pack.clear(); b->com.clear(); b->pairs.clear(); b->pairs64.clear(); b->special.clear();
+    b->pack.clear(); b->com.clear(); b->pairs.clear(); b->pairs64.clear(); b->special.clear();
     b->fur64.clear();
     b->rprep.clear(); b->sk.clear(); b->skred.clear(); b->rforce.clear();
     b->mz = 1;
     b->et.clear(); b->bonds.clear(); b->spc.clear(); b->angles.clear(); b->tors.clear(); b->terms.clear();
-    b->springs.clear(); b->comb.clear(); b->red.clear(); b->dred.clear();
+    b->springs.clear(); b->comb.clear(); b->red.clear(); b->redp.clear(); b->dred.clear();
     b->nmt_max = 1;
     b->nlj_max = 1;
     b->soft = false;
@@ -1649,17 +1662,12 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
             b->dred.add(DetailArgs{h->det_rows, h->ndet, h->d_dpart, b->d_det + (size_t)s * kMaxDet}, h->ndet);
         b->nlj_max = std::max(b->nlj_max, h->nlj);
         b->soft = b->soft || h->p.ee_enabled;
-        const long long nz1 = (long long)h->rows_total * Q_N;
-        b->zero.add(ZeroSeg{(unsigned long long *)h->d_epart, nz1}, cdiv(nz1, 256));
-        b->zero.add(ZeroSeg{h->d_fix, 9LL * n}, cdiv(9LL * n, 256));
-        b->zero.add(ZeroSeg{h->d_fixsp, 3LL * n}, cdiv(3LL * n, 256));
         b->pack.add(h->a_pack, cdiv((long long)h->nclu_cap * kClu, 256));
         b->com.add(h->a_pack, h->ncomgrp);
         const bool fp64 = h->p.precision == MDX_PRECISION_FP64;
         if (fp64) {
-            // j reactions by atomics into cleared words; FP64 FURIR
-            const long long nj = (long long)h->nclu_cap * kClu * 8;
-            b->zero.add(ZeroSeg{h->d_fixj, nj}, cdiv(nj, 256));
+            // j reactions by atomics into cleared words (cleared again by
+            // the combine); FP64 FURIR
             b->pairs64.add(h->a_pair, h->nclu_cap);
             b->fur64.push_back(s);
         } else {
@@ -1707,6 +1715,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         ca.hlx = fixHL; ca.hly = fixHL + n; ca.hlz = fixHL + 2 * n;
         ca.hbx = fixHB; ca.hby = fixHB + n; ca.hbz = fixHB + 2 * n;
         ca.jf = h->d_fixj;
+        ca.clear_jf = fp64 ? 1 : 0;
         ca.atom_slot = h->d_atom_slot;
         const bool spr = b->ring && K > 1;
         ca.spx = spr ? h->d_fixsp : nullptr;
@@ -1724,9 +1733,10 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         // 8 blocks per energy slot, of which only every eighth works: the
         // working blocks of one system then share an XCD (block index mod
         // 8) and its L2, which reads the interleaved rows from HBM once
-        b->red.add(ReduceArgs{h->rows_total, h->d_epart, b->d_energy + (size_t)s * Q_N}, Q_N * 8);
+        b->redp.add(ReduceArgs{h->nclu_cap, erow(h, h->row_pairs), erow(h, h->row_pp)}, Q_N * 8);
+        b->red.add(ReduceArgs{h->rows_total - h->row_pp, erow(h, h->row_pp), b->d_energy + (size_t)s * Q_N}, Q_N * 8);
     }
-    CHK(kplan_upload(b, b->zero, maps)); CHK(kplan_upload(b, b->pack, maps)); CHK(kplan_upload(b, b->com, maps));
+    CHK(kplan_upload(b, b->pack, maps)); CHK(kplan_upload(b, b->com, maps));
     CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->pairs64, maps));
     CHK(kplan_upload(b, b->special, maps));
     CHK(kplan_upload(b, b->rprep, maps)); CHK(kplan_upload(b, b->sk, maps));
@@ -1734,7 +1744,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     CHK(kplan_upload(b, b->et, maps)); CHK(kplan_upload(b, b->bonds, maps)); CHK(kplan_upload(b, b->spc, maps));
     CHK(kplan_upload(b, b->angles, maps)); CHK(kplan_upload(b, b->tors, maps)); CHK(kplan_upload(b, b->terms, maps));
     CHK(kplan_upload(b, b->springs, maps)); CHK(kplan_upload(b, b->comb, maps));
-    CHK(kplan_upload(b, b->red, maps));
+    CHK(kplan_upload(b, b->red, maps)); CHK(kplan_upload(b, b->redp, maps));
     CHK(kplan_upload(b, b->dred, maps));
     b->seen.assign(K, 0);
     for (int s = 0; s < K; ++s) b->seen[s] = b->sys[s]->version;
@@ -1761,7 +1771,6 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail, bool terms =
     hipStream_t st = b->stream;
     auto mark = [&](int k) { if (timing) hipEventRecord(b->ev[k], st); };
     mark(0);
-    launch(b, b->zero, 256, 0, k_zero);
     launch(b, b->pack, 256, 0, k_pack);
     launch(b, b->com, 64, 0, k_com);
     mark(1);
@@ -1834,6 +1843,7 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail, bool terms =
         if (terms) launch(b, b->terms, 256, 0, k_term_reduce);
     }
     mark(8);
+    launch(b, b->redp, 1024, 0, k_ereduce);
     if (b->comb.nblocks()) k_combine<<<b->comb.nblocks(), 256, 0, st>>>(b->comb.d_args, b->comb.d_map, group);
     launch(b, b->red, 1024, 0, k_ereduce);
     hipMemcpyAsync(b->h_energy, b->d_energy, sizeof(double) * Q_N * b->sys.size(), hipMemcpyDeviceToHost, st);
@@ -1845,18 +1855,16 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail, bool terms =
 }
 
 // One force call as three chains that share nothing until the force
-// combine: after the fill kernel (which clears the energy rows every
-// kernel then writes, and the fixed-point accumulators of the scattered
-// terms), the batch stream runs the packing, the pair kernel and its
-// gather; side stream 0 the reciprocal sum; side stream 1 the special
-// pairs, exclusion corrections and bonded terms (integer atomics into the
-// cleared accumulators).  The batch stream waits for both side streams
-// before the combine.  Every sum has a fixed order or is an integer sum,
+// combine: the batch stream runs the packing, the pair kernel, its gather
+// and the sum of its energy rows; side stream 0 the reciprocal sum; side
+// stream 1 the special pairs, exclusion corrections and bonded terms
+// (integer atomics into accumulators that the previous call's combine left
+// cleared).  The batch stream waits for both side streams before the
+// combine.  Every sum has a fixed order or is an integer sum,
 // so the results do not depend on how the chains interleave; the fork and
 // join are events, so the sequence can be captured into a graph.
 void launch_forked(mdx_batch *b, int group, bool detail, bool terms) {
     hipStream_t st = b->stream, sk = b->side[0], sb = b->side[1];
-    launch(b, b->zero, 256, 0, k_zero);
     hipEventRecord(b->fork_ev, st);
     hipStreamWaitEvent(sk, b->fork_ev, 0);
     hipStreamWaitEvent(sb, b->fork_ev, 0);
@@ -1923,6 +1931,7 @@ void launch_forked(mdx_batch *b, int group, bool detail, bool terms) {
             else k_pairs64<false, false><<<nb, 64, 0, st>>>(pa, pm, group);
         }
     }
+    launch(b, b->redp, 1024, 0, k_ereduce);
     // join
     hipEventRecord(b->join_ev[0], sk);
     hipEventRecord(b->join_ev[1], sb);
@@ -1965,11 +1974,11 @@ mdx_status batch_prepare(mdx_batch *b, int group, bool detail) {
         // regrown capacity changes the maps and the captured graphs
         std::vector<int> before = {b->pairs.nblocks(), b->pairs64.nblocks(), b->sk.nblocks(), b->rforce.nblocks(),
                                    b->skred.nblocks(),
-                                   b->zero.nblocks(), b->mz, b->nmt_max, b->nlj_max};
+                                   b->mz, b->nmt_max, b->nlj_max};
         CHK(plan_fill(b, true));
         std::vector<int> after = {b->pairs.nblocks(), b->pairs64.nblocks(), b->sk.nblocks(), b->rforce.nblocks(),
                                   b->skred.nblocks(),
-                                  b->zero.nblocks(), b->mz, b->nmt_max, b->nlj_max};
+                                  b->mz, b->nmt_max, b->nlj_max};
         if (before != after) drop_graphs(b);
     }
     return MDX_OK;
@@ -2944,11 +2953,11 @@ mdx_status mdx_batch_destroy(mdx_batch *b) {
     if (!b) return MDX_OK;
     if (b->stream) hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    kplan_free(b->zero); kplan_free(b->pack); kplan_free(b->com); kplan_free(b->pairs); kplan_free(b->pairs64);
+    kplan_free(b->pack); kplan_free(b->com); kplan_free(b->pairs); kplan_free(b->pairs64);
     kplan_free(b->special);
     kplan_free(b->rprep); kplan_free(b->sk); kplan_free(b->skred); kplan_free(b->rforce);
     kplan_free(b->et); kplan_free(b->bonds); kplan_free(b->spc); kplan_free(b->angles); kplan_free(b->tors); kplan_free(b->terms);
-    kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->dred);
+    kplan_free(b->springs); kplan_free(b->comb); kplan_free(b->red); kplan_free(b->redp); kplan_free(b->dred);
     kplan_free(b->md_kd); kplan_free(b->md_sh); kplan_free(b->md_ke); kplan_free(b->md_th);
     kplan_free(b->md_gc); kplan_free(b->md_tr);
     if (b->d_trace) hipFree(b->d_trace);

@@ -169,11 +169,6 @@ __device__ inline void fx_add_g(MDX_GLOBAL unsigned long long *p, double v, doub
 // ---------------------------------------------------------------------
 // Argument structs of the force-call kernels (one per system).
 
-struct ZeroSeg {                 // memory to clear before a call
-    unsigned long long *p;
-    long long n;                 // 8-byte words
-};
-
 struct PackArgs {
     int nclu;
     const int *clu_atom;
@@ -389,11 +384,14 @@ struct CombineArgs {
     const double *flx, *fly, *flz; // pair long
     const double *recip;           // [nslice][3][n]
     int nslice;
-    const unsigned long long *gx, *gy, *gz;    // fixed: slow scatter (ETERMS, long specials)
-    const unsigned long long *hlx, *hly, *hlz; // fixed: LOCALF specials
-    const unsigned long long *hbx, *hby, *hbz; // fixed: bonded
-    const unsigned long long *spx, *spy, *spz; // fixed: bead springs (unscaled) or null
-    const unsigned long long *jf;              // fixed: pair j reactions [slot][8]
+    // fixed-point accumulators, cleared again once read (so the next call
+    // starts from zeros without a fill kernel)
+    unsigned long long *gx, *gy, *gz;    // slow scatter (ETERMS, long specials)
+    unsigned long long *hlx, *hly, *hlz; // LOCALF specials
+    unsigned long long *hbx, *hby, *hbz; // bonded
+    unsigned long long *spx, *spy, *spz; // bead springs (unscaled) or null
+    unsigned long long *jf;              // pair j reactions [slot][8]
+    int clear_jf;                        // jf summed by atomics (FP64 mode): clear it too
     const int *atom_slot;
     double inv_scale;
     const double *x, *y, *z;
@@ -413,7 +411,7 @@ struct DetailArgs {
 
 struct ReduceArgs {
     int nrows;
-    const double *epart;
+    double *epart;                 // [nrows][Q_N], cleared once read
     double *out;                   // Q_N
 };
 

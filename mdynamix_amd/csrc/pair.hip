@@ -281,7 +281,8 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
             dy = fmaf(-bl[1], rintf(dy * bi[1]), dy);
             dz = fmaf(-bl[2], rintf(dz * bi[2]), dz);
         }
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float dx2 = dx * dx;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx2));
         // lanes in the list and inside the cutoff, as a wave mask; pairs
         // within the FP32 error band of the cutoff take the reference's
         // FP64 test (no band for reference lists: rlo = rhi)
@@ -353,7 +354,10 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
             // reactions on j (minus the force on i)
             const float nx = -fsc * dx, ny = -fsc * dy, nz = -fsc * dz;
             s.fl[0] = fmaf(fsc, dx, s.fl[0]); s.fl[1] = fmaf(fsc, dy, s.fl[1]); s.fl[2] = fmaf(fsc, dz, s.fl[2]);
-            s.vl[0] = fmaf(-nx, dx, s.vl[0]); s.vl[1] = fmaf(-ny, dy, s.vl[1]); s.vl[2] = fmaf(-nz, dz, s.vl[2]);
+            // (the reactions go into destructive lane swaps below; squares
+            // of d here need no copies of them)
+            s.vl[0] = fmaf(fsc, dx2, s.vl[0]); s.vl[1] = fmaf(fsc, dy * dy, s.vl[1]);
+            s.vl[2] = fmaf(fsc, dz * dz, s.vl[2]);
             s.el = fmaf(pj.w, phi, s.el); s.el_lo = fmaf(qlo, phi, s.el_lo);
             s.a6 = fmaf(lj.x, r6i, s.a6);
             if (SOFT) s.w6 += w6;
