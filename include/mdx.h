@@ -462,8 +462,9 @@ mdx_status mdx_md_init(mdx_engine *h, const mdx_md_params *p, const double *vx, 
 mdx_status mdx_md_run(mdx_engine *h, int32_t nsteps, double *trace);
 mdx_status mdx_md_get_state(mdx_engine *h, double *x, double *y, double *z, double *vx, double *vy, double *vz);
 /* Constraint failures since mdx_md_init: molecules whose SHEJK sweeps hit
- * MAXIT without convergence, plus bond updates whose projection on the
- * old bond fell below BDTOL (the reference prints "NO CONVERGENCE" and
+ * MAXIT without convergence, plus molecules in which a bond update's
+ * projection on the old bond fell below BDTOL (one count per molecule and
+ * step; the reference prints "NO CONVERGENCE" and
  * continues, mdstep.f:1264-1268; the device run continues too and counts
  * them here).  0 in a healthy run. */
 mdx_status mdx_md_shake_failures(mdx_engine *h, int64_t *count);

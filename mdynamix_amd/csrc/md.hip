@@ -51,6 +51,38 @@ __device__ inline void md_kick_drift_body(const MdArgs &a, int blk) {
 constexpr int kShakeRegAtoms = 4;
 constexpr int kShakeRegCons = 6;
 
+// Per-constraint values that do not change between sweeps: the old bond
+// vector OX(I) - OX(J) with its minimum image, 2 (1/m_I + 1/m_J), and the
+// two tolerance products (the reference forms them in every sweep with the
+// same operations, mdstep.f:1183-1209).
+struct ShakeCons {
+    double dx, dy, dz, k, tgt2, tol, bd;
+};
+
+// One constraint of a molecule held in registers, atoms I and J (slots of
+// the molecule) fixed at compile time; returns whether it was corrected,
+// and sets bad when the bond turned too far (VDOT < BNDSQ BDTOL).
+template <int NA, int I, int J>
+__device__ inline bool shake_pair(const MdArgs &a, double (&x)[NA], double (&y)[NA], double (&z)[NA],
+                                  const double (&mi)[NA], const ShakeCons &c, bool &bad) {
+#pragma clang fp contract(off)
+    double bx = x[I] - x[J], by = y[I] - y[J], bz = z[I] - z[J];
+    min_image(bx, by, bz, a.box);
+    double cur2 = bx * bx + by * by;
+    cur2 = cur2 + bz * bz;
+    const double gap2 = c.tgt2 - cur2;
+    if (!(fabs(gap2) > c.tol)) return false;
+    double proj = bx * c.dx + by * c.dy;
+    proj = proj + bz * c.dz;
+    bad = bad || proj < c.bd;
+    const double lam = gap2 / (c.k * proj);
+    const double dx = c.dx * lam, dy = c.dy * lam, dz = c.dz * lam;
+    const double wi = mi[I], wj = mi[J];
+    x[I] = x[I] + wi * dx; y[I] = y[I] + wi * dy; z[I] = z[I] + wi * dz;
+    x[J] = x[J] - wj * dx; y[J] = y[J] - wj * dy; z[J] = z[J] - wj * dz;
+    return true;
+}
+
 template <int NA>
 __device__ inline void md_shake_regs(const MdArgs &a, int c0, int c1, int a0) {
 #pragma clang fp contract(off)
@@ -62,49 +94,58 @@ __device__ inline void md_shake_regs(const MdArgs &a, int c0, int c1, int a0) {
         ox[k] = a.ox[i]; oy[k] = a.oy[i]; oz[k] = a.oz[i];
         mi[k] = a.minv[i];
     }
-    // constraints of the molecule in registers (at most kShakeRegCons)
+    // constraints of the molecule in registers (at most kShakeRegCons), as
+    // slot pairs code = 4 i + j: each constraint dispatches on its code to
+    // a body with compile-time slots, so the coordinates stay in registers
+    // without select chains (molecules of one type share their codes, so
+    // a wavefront of them takes one branch)
     const int nc = c1 - c0;
-    int ci[kShakeRegCons], cj[kShakeRegCons];
-    double cl[kShakeRegCons];
+    int code[kShakeRegCons];
+    ShakeCons cs[kShakeRegCons];
     unsigned moved = 0u, moving = 0u;
 #pragma unroll
     for (int c = 0; c < kShakeRegCons; ++c) {
-        ci[c] = 0; cj[c] = 0; cl[c] = 1.0;
+        code[c] = 0;
+        cs[c] = ShakeCons{0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0};
         if (c < nc) {
             const int2 p = a.cons[c0 + c];
-            ci[c] = p.x - a0;
-            cj[c] = p.y - a0;
-            cl[c] = a.cons_len2[c0 + c];
-            moved |= (1u << ci[c]) | (1u << cj[c]);
+            const int i = p.x - a0, j = p.y - a0;
+            code[c] = 4 * i + j;
+            double dx = 0.0, dy = 0.0, dz = 0.0, wi = 0.0, wj = 0.0;
+#pragma unroll
+            for (int k = 0; k < NA; ++k) {
+                if (k == i) { dx += ox[k]; dy += oy[k]; dz += oz[k]; wi = mi[k]; }
+                if (k == j) { dx -= ox[k]; dy -= oy[k]; dz -= oz[k]; wj = mi[k]; }
+            }
+            min_image(dx, dy, dz, a.box);
+            const double t2 = a.cons_len2[c0 + c];
+            cs[c] = ShakeCons{dx, dy, dz, 2.0 * (wi + wj), t2, t2 * a.tol2, t2 * a.bdtol};
+            moved |= (1u << i) | (1u << j);
         }
     }
-    bool done = false;
+    bool done = false, bad = false;
     int iter = 0;
     while (!done && iter <= a.maxit) {
         done = true;
 #pragma unroll
         for (int c = 0; c < kShakeRegCons; ++c) {
             if (c >= nc) break;
-            const int i = ci[c], j = cj[c];
+            const int i = code[c] >> 2, j = code[c] & 3;
             const unsigned bi = 1u << i, bj = 1u << j;
             if (!((moved & bi) || (moved & bj))) continue;
-            const double target2 = cl[c];
-            double bx = x[i] - x[j], by = y[i] - y[j], bz = z[i] - z[j];
-            min_image(bx, by, bz, a.box);
-            double cur2 = bx * bx + by * by;
-            cur2 = cur2 + bz * bz;
-            const double gap2 = target2 - cur2;
-            if (fabs(gap2) > target2 * a.tol2) {
-                double dx = ox[i] - ox[j], dy = oy[i] - oy[j], dz = oz[i] - oz[j];
-                min_image(dx, dy, dz, a.box);
-                double proj = bx * dx + by * dy;
-                proj = proj + bz * dz;
-                if (proj < target2 * a.bdtol) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
-                const double wi = mi[i], wj = mi[j];
-                const double lam = gap2 / (2.0 * (wi + wj) * proj);
-                dx = dx * lam; dy = dy * lam; dz = dz * lam;
-                x[i] = x[i] + wi * dx; y[i] = y[i] + wi * dy; z[i] = z[i] + wi * dz;
-                x[j] = x[j] - wj * dx; y[j] = y[j] - wj * dy; z[j] = z[j] - wj * dz;
+            bool corr = false;
+            switch (code[c]) {
+#define MDX_SHAKE_CASE(I, J)                                                                   \
+    case 4 * I + J:                                                                            \
+        if constexpr (I < NA && J < NA) corr = shake_pair<NA, I, J>(a, x, y, z, mi, cs[c], bad); \
+        break;
+                MDX_SHAKE_CASE(0, 1) MDX_SHAKE_CASE(1, 0) MDX_SHAKE_CASE(0, 2) MDX_SHAKE_CASE(2, 0)
+                MDX_SHAKE_CASE(0, 3) MDX_SHAKE_CASE(3, 0) MDX_SHAKE_CASE(1, 2) MDX_SHAKE_CASE(2, 1)
+                MDX_SHAKE_CASE(1, 3) MDX_SHAKE_CASE(3, 1) MDX_SHAKE_CASE(2, 3) MDX_SHAKE_CASE(3, 2)
+#undef MDX_SHAKE_CASE
+                default: break;
+            }
+            if (corr) {
                 moving |= bi | bj;
                 done = false;
             }
@@ -113,6 +154,9 @@ __device__ inline void md_shake_regs(const MdArgs &a, int c0, int c1, int a0) {
         moving = 0u;
         ++iter;
     }
+    // a bond turned too far in some sweep, or no convergence: one count
+    // each for the molecule
+    if (bad) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
     if (!done) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
 #pragma unroll
     for (int k = 0; k < NA; ++k) {
@@ -144,7 +188,7 @@ __device__ inline void md_shake_body(const MdArgs &a, int blk) {
         moved |= 1ull << (a.cons[c].x - a0);
         moved |= 1ull << (a.cons[c].y - a0);
     }
-    bool done = false;
+    bool done = false, bad = false;
     int iter = 0;
     while (!done && iter <= a.maxit) {
         done = true;
@@ -163,7 +207,7 @@ __device__ inline void md_shake_body(const MdArgs &a, int blk) {
                 min_image(dx, dy, dz, a.box);
                 double proj = bx * dx + by * dy;
                 proj = proj + bz * dz;
-                if (proj < target2 * a.bdtol) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
+                bad = bad || proj < target2 * a.bdtol;
                 const double wi = a.minv[i], wj = a.minv[j];
                 const double lam = gap2 / (2.0 * (wi + wj) * proj);
                 dx = dx * lam; dy = dy * lam; dz = dz * lam;
@@ -178,6 +222,7 @@ __device__ inline void md_shake_body(const MdArgs &a, int blk) {
         moving = 0ull;
         ++iter;
     }
+    if (bad) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
     if (!done) atomicAdd(&a.state[MD_SHAKE_FAIL], 1.0);
     for (int i = a0; i < a1; ++i) {
         a.vx[i] = a.vx[i] + (a.x[i] - a.ux[i]) / a.dt;
