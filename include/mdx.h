@@ -309,12 +309,20 @@ mdx_status mdx_get_forces(mdx_engine *h, double *fx, double *fy, double *fz, dou
 mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
 /* Autotuner: times the reciprocal-space kernels of this system over a set
  * of launch shapes (atoms per structure-factor block; row slices of the
- * force pass) on the current coordinates and keeps the fastest; the choice
- * is remembered for the process by size class (atoms, k vectors) and used
- * by later mdx_create calls of that class.  The shape changes only how the
- * sums are split into partial sums (results agree to rounding; each shape
- * is deterministic).  Needs a built list. */
+ * force pass), then the pair stage over cluster geometries (the width of
+ * the x-y columns whose z-sorted atoms form the 8-atom clusters: 0.65,
+ * 0.75, 0.85, 1 and 1.15 times the cube root of 8 atoms' volume, scored by the pair
+ * stage plus a tenth of a list build), on the current coordinates, and
+ * keeps the fastest; the list is rebuilt at the current coordinates with
+ * the chosen grid.  The choice is remembered for the process by size
+ * class (atoms, k vectors) and used by later mdx_create calls of that
+ * class.  Shape and grid change only how the sums are split into partial
+ * sums (results agree to rounding; each choice is deterministic).  Needs a
+ * built list.  No counterpart in the reference. */
 mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk, int32_t *nslice);
+/* The cluster grid in use: column width (A, along x) and the column counts
+ * along x and y.  No counterpart in the reference. */
+mdx_status mdx_cluster_grid(const mdx_engine *h, double *column_width, int32_t *ncx, int32_t *ncy);
 /* The k vectors of the reciprocal sum in INIFUR order (forces.f:1375-1427,
  * rectangular cell): integer triples (n_x, n_y, n_z) with k = 2 pi n / L,
  * i.e. RKX = 2 pi n_x as INIFUR stores it.  *nkv receives the count; the

@@ -107,6 +107,7 @@ struct mdx_engine {
     bool user_com = false;
     // clusters and lists
     int ncx = 0, ncy = 0, ncol = 0, nclu = 0, nclu_cap = 0;
+    double cw_scale = 1.0;           // column width / (8 atoms' volume)^(1/3) (autotuned)
     unsigned long long *d_keys = nullptr, *d_keys2 = nullptr;
     int *d_vals = nullptr, *d_vals2 = nullptr;
     int *d_colcount = nullptr, *d_colstart = nullptr, *d_colclu_n = nullptr, *d_colclu = nullptr;
@@ -643,7 +644,7 @@ void self_energy(mdx_engine *h, const mdx_topology *t) {
 mdx_status alloc_clusters(mdx_engine *h) {
     const double vol = h->p.box[0] * h->p.box[1] * h->p.box[2];
     const double rho = h->n / vol;
-    const double cw = std::cbrt(kClu / rho);
+    const double cw = std::cbrt(kClu / rho) * h->cw_scale;
     h->ncx = std::max(1, (int)(h->p.box[0] / cw));
     h->ncy = std::max(1, (int)(h->p.box[1] / cw));
     h->ncol = h->ncx * h->ncy;
@@ -683,12 +684,13 @@ mdx_status alloc_clusters(mdx_engine *h) {
     CHK(alloc_list(h));
     CHK(dalloc(h, &h->d_jcnt, h->nclu_cap));
     CHK(dalloc(h, &h->d_jstart, h->nclu_cap));
-    // per-call buffers
-    CHK(dalloc(h, &h->d_fs, (size_t)3 * h->n));
-    CHK(dalloc(h, &h->d_fl, (size_t)3 * h->n));
-    CHK(dalloc_zero(h, &h->d_fix, (size_t)9 * h->n));
+    // per-call buffers; those sized by the atom count only are kept when
+    // the grid changes (the forces F, G, H of the last call stay valid)
+    if (!h->d_fs) CHK(dalloc(h, &h->d_fs, (size_t)3 * h->n));
+    if (!h->d_fl) CHK(dalloc(h, &h->d_fl, (size_t)3 * h->n));
+    if (!h->d_fix) CHK(dalloc_zero(h, &h->d_fix, (size_t)9 * h->n));
     CHK(dalloc_zero(h, &h->d_fixj, (size_t)h->nclu_cap * kClu * 8));
-    CHK(dalloc(h, &h->d_out, (size_t)9 * h->n));
+    if (!h->d_out) CHK(dalloc(h, &h->d_out, (size_t)9 * h->n));
     return MDX_OK;
 }
 
@@ -769,8 +771,14 @@ struct TuneKey {
     int n, k;
     bool operator<(const TuneKey &o) const { return n != o.n ? n < o.n : k < o.k; }
 };
+// A size class's tuned shape: k-space chunk and slices, cluster column
+// width scale (0: not tuned)
+struct TuneShape {
+    int chunk = 0, nslice = 0;
+    double cw = 0.0;
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<int, int>> g_tune;
+std::map<TuneKey, TuneShape> g_tune;
 
 TuneKey tune_key(const mdx_engine *h) {
     return TuneKey{(int)std::lround(4.0 * std::log2(std::max(1, h->n))),
@@ -1239,6 +1247,13 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
     h->qabs = 0.0;
     for (int i = 0; i < n; ++i) h->qabs += std::fabs(t->atom_charge[i]);
     if ((s = setup_kspace(h))) return fail(s);
+    TuneShape shape;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(tune_key(h));
+        if (it != g_tune.end()) shape = it->second;
+    }
+    if (shape.cw > 0.0) h->cw_scale = shape.cw;
     if ((s = alloc_clusters(h))) return fail(s);
     if ((s = dalloc_zero(h, &h->d_fixsp, (size_t)3 * n))) return fail(s);
     layout_rows(h);
@@ -1248,15 +1263,7 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
         *out = h;
         return MDX_OK;
     }
-    {
-        std::pair<int, int> shape(0, 0);
-        {
-            std::lock_guard<std::mutex> lk(g_tune_mu);
-            auto it = g_tune.find(tune_key(h));
-            if (it != g_tune.end()) shape = it->second;
-        }
-        if (shape.first > 0 && (s = set_kspace_shape(h, shape.first, shape.second))) return fail(s);
-    }
+    if (shape.chunk > 0 && (s = set_kspace_shape(h, shape.chunk, shape.nslice))) return fail(s);
     mdx_batch *self = nullptr;
     mdx_engine *one[1] = {h};
     if ((s = mdx_batch_create(one, 1, &self))) return fail(s);
@@ -2404,6 +2411,27 @@ mdx_status mdx_get_forces(mdx_engine *h, double *fx, double *fy, double *fz, dou
     return copy_forces(h, fx, fy, fz, gx, gy, gz, hx, hy, hz, hipMemcpyDeviceToHost);
 }
 
+// The cluster grid with column width scale `scale`: cluster arrays, list
+// storage and energy rows allocated again (capacities follow the grid);
+// the list must be built again before the next force call.
+static mdx_status regrid_clusters(mdx_engine *h, double scale) {
+    h->cw_scale = scale;
+    CHK(alloc_clusters(h));
+    layout_rows(h);
+    CHK(dalloc_zero(h, &h->d_epart, (size_t)h->rows_total * Q_N));
+    h->have_list = false;
+    fill_args(h);
+    return MDX_OK;
+}
+
+mdx_status mdx_cluster_grid(const mdx_engine *h, double *column_width, int32_t *ncx, int32_t *ncy) {
+    if (!h) return MDX_ERR_ARG;
+    if (column_width) *column_width = h->p.box[0] / std::max(1, h->ncx);
+    if (ncx) *ncx = h->ncx;
+    if (ncy) *ncy = h->ncy;
+    return MDX_OK;
+}
+
 mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) {
     if (!h) return MDX_ERR_ARG;
     if (!h->have_list) {
@@ -2449,10 +2477,39 @@ mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) 
         if (fo < best_force) { best_force = fo; best_slice = ns; }
     }
     CHK(set_kspace_shape(h, best_chunk, best_slice));
+    // Cluster geometry: the width of the x-y columns whose z-sorted atoms
+    // form the 8-atom clusters (a scale of the cube of 8 atoms' volume).
+    // Wider columns give flatter clusters, narrower ones taller clusters;
+    // both change how many of a list entry's 64 lanes hold a pair inside
+    // R_cut and how many entries the list has.  Each width is scored by
+    // the pair stage plus a tenth of a list build (a rebuild every ten
+    // steps); the list is rebuilt at the current coordinates.
+    const bool htiming0 = h->timing;
+    h->timing = true;
+    double best_cw = h->cw_scale, best_score = 1e30;
+    for (double cw : {1.0, 0.85, 0.75, 0.65, 1.15}) {
+        CHK(regrid_clusters(h, cw));
+        CHK(mdx_build_list(h));
+        const double list_ms = h->stage_ms[ST_LIST];
+        double pair_ms = 1e30;
+        for (int rep = 0; rep < 6; ++rep) {
+            mdx_status st = batch_compute(b, MDX_FORCES_ALL, false);
+            if (st != MDX_OK) {
+                h->err = b->err;
+                return st;
+            }
+            if (rep > 0) pair_ms = std::min(pair_ms, (double)b->stage_ms[ST_PAIRS]);
+        }
+        const double score = pair_ms + 0.1 * list_ms;
+        if (score < best_score) { best_score = score; best_cw = cw; }
+    }
+    CHK(regrid_clusters(h, best_cw));
+    CHK(mdx_build_list(h));
+    h->timing = htiming0;
     b->timing = timing0;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune[tune_key(h)] = std::make_pair(best_chunk, best_slice);
+        g_tune[tune_key(h)] = TuneShape{best_chunk, best_slice, best_cw};
     }
     if (chunk_out) *chunk_out = best_chunk;
     if (nslice_out) *nslice_out = best_slice;

@@ -74,6 +74,7 @@ def load_library() -> C.CDLL:
     L.mdx_batch_last_error.restype = C.c_char_p
     L.mdx_batch_destroy.argtypes = [pv]
     L.mdx_autotune.argtypes = [pv, pi32, pi32]
+    L.mdx_cluster_grid.argtypes = [pv, C.POINTER(C.c_double), pi32, pi32]
     L.mdx_set_kspace_shape.argtypes = [pv, C.c_int32, C.c_int32]
     L.mdx_get_kvectors.argtypes = [pv, pi32, pi32, pi32, C.c_int64, C.POINTER(C.c_int64)]
     L.mdx_enumerate_kvectors.argtypes = [pv, pi32, pi32, pi32, C.c_int64, C.POINTER(C.c_int64)]
@@ -93,7 +94,7 @@ def load_library() -> C.CDLL:
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
-                 "mdx_batch_md_run", "mdx_autotune", "mdx_set_kspace_shape", "mdx_get_kvectors",
+                 "mdx_batch_md_run", "mdx_autotune", "mdx_cluster_grid", "mdx_set_kspace_shape", "mdx_get_kvectors",
                  "mdx_enumerate_kvectors", "mdx_md_shake_failures", "mdx_furir_workspace_bytes",
                  "mdx_furir_device", "mdx_furir_device_noalloc", "mdx_batch_pimd_init", "mdx_batch_pimd_chains"):
         getattr(L, name).restype = C.c_int
@@ -230,12 +231,19 @@ class ForceEngine:
         the force call from a captured hipGraph."""
         self._check(self.lib.mdx_set_timing(self._h, 1 if on else 0))
 
-    def autotune(self) -> Dict[str, int]:
-        """Pick the k-space launch shape of this size class by timing (see
-        mdx_autotune); returns the chosen shape."""
+    def autotune(self) -> Dict[str, float]:
+        """Pick the k-space launch shape and the cluster grid of this size
+        class by timing (see mdx_autotune; the list is rebuilt at the
+        current coordinates); returns the chosen shape and column width."""
         c, sl = C.c_int32(), C.c_int32()
         self._check(self.lib.mdx_autotune(self._h, C.byref(c), C.byref(sl)))
-        return {"chunk": c.value, "nslice": sl.value}
+        return {"chunk": c.value, "nslice": sl.value, "column_width": self.cluster_grid()["column_width"]}
+
+    def cluster_grid(self) -> Dict[str, float]:
+        """The cluster grid in use (mdx_cluster_grid)."""
+        w, nx, ny = C.c_double(), C.c_int32(), C.c_int32()
+        self._check(self.lib.mdx_cluster_grid(self._h, C.byref(w), C.byref(nx), C.byref(ny)))
+        return {"column_width": w.value, "ncx": nx.value, "ncy": ny.value}
 
     def kvectors(self) -> np.ndarray:
         """The engine's k set as an (nkv, 3) array of integer triples n in

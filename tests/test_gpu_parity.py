@@ -197,14 +197,18 @@ def test_repeat_call_is_bit_identical():
 
 
 def test_autotuned_shape_keeps_parity():
-    """The autotuner only changes launch shapes: forces and energies stay
-    within the tolerances and the list is unchanged."""
+    """The autotuner only changes launch shapes and the cluster grid: the
+    reference-layout list it rebuilds is the same, and forces and energies
+    stay within the tolerances."""
     d, topo, prm, eng = _engine_from_golden("water_508")
     eng.set_positions(golden_positions(d))
     eng.chcnb()
+    lists0 = eng.get_list()
     shape = eng.autotune()
-    print("case water_508 tuned", shape, flush=True)
-    assert shape["chunk"] > 0 and shape["nslice"] > 0
+    print("case water_508 tuned", shape, eng.cluster_grid(), flush=True)
+    assert shape["chunk"] > 0 and shape["nslice"] > 0 and shape["column_width"] > 0.0
+    for a, b in zip(lists0, eng.get_list()):
+        assert np.array_equal(a, b)
     eng.set_molecule_com(golden_com(d, 0))
     f, g, h, e = eng.allforce()
     assert rms_rel(f, golden_forces(d, "F", 0)) <= FORCE_RMS_TOL
