@@ -468,8 +468,9 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=4.0)
-    ap.add_argument("--kspace-shape", default="", metavar="CHUNK,NSLICE",
-                    help="fixed k-space launch shape instead of autotuning (profiling runs)")
+    ap.add_argument("--kspace-shape", default="", metavar="CHUNK,NSLICE[,GRID]",
+                    help="fixed k-space launch shape (and cluster grid scale) instead of autotuning "
+                         "(profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (traffic counter calibration)")
     args = ap.parse_args()
@@ -485,9 +486,13 @@ def main() -> int:
     dev = device_info()
     eng.chcnb()
     if args.kspace_shape:
-        c, n = (int(v) for v in args.kspace_shape.split(","))
+        v = args.kspace_shape.split(",")
+        c, n = int(v[0]), int(v[1])
+        if len(v) > 2:
+            eng.set_cluster_grid(float(v[2]))
         eng.set_kspace_shape(c, n)
-        tuned = {"chunk": c, "nslice": n, "source": "--kspace-shape"}
+        tuned = {"chunk": c, "nslice": n, "column_width": eng.cluster_grid()["column_width"],
+                 "source": "--kspace-shape"}
     else:
         tuned = dict(eng.autotune(), source="autotune")
     print("k-space shape: %s" % tuned, file=sys.stderr, flush=True)

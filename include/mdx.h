@@ -310,9 +310,10 @@ mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
 /* Autotuner: times the reciprocal-space kernels of this system over a set
  * of launch shapes (atoms per structure-factor block; row slices of the
  * force pass), then the pair stage over cluster geometries (the width of
- * the x-y columns whose z-sorted atoms form the 8-atom clusters: 0.65,
- * 0.75, 0.85, 1 and 1.15 times the cube root of 8 atoms' volume, scored by the pair
- * stage plus a tenth of a list build), on the current coordinates, and
+ * the x-y columns whose z-sorted atoms form the 8-atom clusters: every
+ * column count along x between 0.75 and 1.3 times the cube root of 8
+ * atoms' volume, scored by the pair stage plus a tenth of a list build),
+ * on the current coordinates, and
  * keeps the fastest; the list is rebuilt at the current coordinates with
  * the chosen grid.  The choice is remembered for the process by size
  * class (atoms, k vectors) and used by later mdx_create calls of that
@@ -320,6 +321,11 @@ mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
  * sums (results agree to rounding; each choice is deterministic).  Needs a
  * built list.  No counterpart in the reference. */
 mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk, int32_t *nslice);
+/* A fixed cluster grid instead of the tuned one: column width = scale
+ * (0.5 to 2) times the cube root of 8 atoms' volume; the list is rebuilt
+ * at the current coordinates when one was built.  No counterpart in the
+ * reference. */
+mdx_status mdx_set_cluster_grid(mdx_engine *h, double scale);
 /* The cluster grid in use: column width (A, along x) and the column counts
  * along x and y.  No counterpart in the reference. */
 mdx_status mdx_cluster_grid(const mdx_engine *h, double *column_width, int32_t *ncx, int32_t *ncy);

@@ -2424,6 +2424,14 @@ static mdx_status regrid_clusters(mdx_engine *h, double scale) {
     return MDX_OK;
 }
 
+mdx_status mdx_set_cluster_grid(mdx_engine *h, double scale) {
+    if (!h || !(scale >= 0.5 && scale <= 2.0)) return MDX_ERR_ARG;
+    const bool had = h->have_list;
+    CHK(regrid_clusters(h, scale));
+    if (had) CHK(mdx_build_list(h));
+    return MDX_OK;
+}
+
 mdx_status mdx_cluster_grid(const mdx_engine *h, double *column_width, int32_t *ncx, int32_t *ncy) {
     if (!h) return MDX_ERR_ARG;
     if (column_width) *column_width = h->p.box[0] / std::max(1, h->ncx);
@@ -2478,16 +2486,27 @@ mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) 
     }
     CHK(set_kspace_shape(h, best_chunk, best_slice));
     // Cluster geometry: the width of the x-y columns whose z-sorted atoms
-    // form the 8-atom clusters (a scale of the cube of 8 atoms' volume).
-    // Wider columns give flatter clusters, narrower ones taller clusters;
-    // both change how many of a list entry's 64 lanes hold a pair inside
-    // R_cut and how many entries the list has.  Each width is scored by
-    // the pair stage plus a tenth of a list build (a rebuild every ten
-    // steps); the list is rebuilt at the current coordinates.
+    // form the 8-atom clusters.  Wider columns give flatter clusters,
+    // narrower ones taller clusters; the column count also sets which
+    // clusters share a column and how the i-clusters' blocks fall on the
+    // XCDs, and the pair stage's time is not smooth in it (at 96,174
+    // atoms: 22 columns along x 0.27 ms, 21 or 24 columns 0.24 ms).  Every
+    // column count along x from 0.75 to 1.3 times the cube root of 8
+    // atoms' volume is tried, scored by the pair stage plus a tenth of a
+    // list build (a rebuild every ten steps); the list is rebuilt at the
+    // current coordinates.
     const bool htiming0 = h->timing;
     h->timing = true;
     double best_cw = h->cw_scale, best_score = 1e30;
-    for (double cw : {1.0, 0.85, 0.75, 0.65, 1.15}) {
+    const double cw0 = std::cbrt(kClu * h->p.box[0] * h->p.box[1] * h->p.box[2] / h->n);
+    std::vector<double> scales;
+    for (int nc = std::max(1, (int)std::floor(h->p.box[0] / (1.3 * cw0)));
+         nc <= (int)std::ceil(h->p.box[0] / (0.75 * cw0)); ++nc) {
+        // exactly nc columns along x (the grid takes floor(L / width))
+        const double sc = h->p.box[0] / (nc * cw0) * (1.0 - 1e-9);
+        if (sc >= 0.5 && sc <= 2.0) scales.push_back(sc);
+    }
+    for (double cw : scales) {
         CHK(regrid_clusters(h, cw));
         CHK(mdx_build_list(h));
         const double list_ms = h->stage_ms[ST_LIST];
