@@ -51,7 +51,7 @@ def measured_traffic(kernel: str):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) >= 3 and f[0] == "traffic_bytes" and (f[1] == kernel or f[1].startswith(kernel + "<")):
+        if len(f) >= 3 and f[0] == "traffic_bytes" and " ".join(f[1:-1]) == kernel:
             return int(f[-1])
     return None
 
@@ -557,11 +557,12 @@ def main() -> int:
     achieved = dflop / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
     if dom == "k_pairs":
         # the pair stage: k_pairs and the gather of its j-reaction records
-        tp, tg = measured_traffic("k_pairs"), measured_traffic("k_jgather")
+        tp, tg = measured_traffic("k_pairs<false, false>"), measured_traffic("k_jgather")
         traffic = tp + tg if tp is not None and tg is not None else None
         dom = "k_pairs+k_jgather"
     else:
-        traffic = measured_traffic(dom)
+        # the instantiations this box runs (n_z up to 32 per row)
+        traffic = measured_traffic({"k_sk": "k_sk<5>", "k_recip_force": "k_recip_force<32>"}[dom])
     entries, front = eng.list_entries()
     lanes = {"list_entries": entries, "front_entries": front, "pairs_in_cutoff": pairs_call,
              "lane_occupancy": round(pairs_call / (64.0 * entries), 4) if entries else None}
