@@ -45,7 +45,7 @@ constexpr int kRecUnit = 24;       // floats per unit of the j-reaction records
 // [POTE1(ntp) | POTL1(ntp) | PES141(nmt) | PSR141(nmt)]
 constexpr int kMaxDet = 2 * 137 + 2 * 16;
 constexpr int kWave = 64;          // wavefront size on gfx950
-constexpr int kForceTiles = 4;     // 16-atom tiles per wavefront of the reciprocal force pass
+constexpr int kForceTiles = 3;     // 16-atom tiles per wavefront of the reciprocal force pass
 constexpr int kForceBlock = 64 * kForceTiles;   // atoms per block (4 wavefronts) of that pass
 constexpr int kComAtoms = 512;     // LDS stage (atoms) of the centre-of-mass kernels
 constexpr int kSkRows = 64;        // k rows per structure-factor block (16 per wavefront)
