@@ -648,17 +648,19 @@ template __global__ void k_pairs64<true, false>(const PairArgs *, const int2 *, 
 template __global__ void k_pairs64<false, true>(const PairArgs *, const int2 *, int);
 template __global__ void k_pairs64<true, true>(const PairArgs *, const int2 *, int);
 
-// j reactions per atom: one wavefront per j-cluster, lanes 0-47 (48-63
-// idle), each on one record float: the front-segment records (two units:
-// long x, z, y, short x, y, z, 8 j slots each) one per step with lane =
-// float; then the long-class records (one unit: long x, z, y) two per step,
-// lane = 24 (record of the pair) + float.  Every lane sums its floats as
-// 64-bit fixed point (round(v fscale) as the low bits of v fscale +
-// 1.5 2^52, exact for |v fscale| < 2^51, else converted directly), so the
-// integer totals do not depend on the record order; lanes 0-23 then add
-// the long sums of lanes 24-47 and every word is stored once: no atomics,
-// no clearing.  A fast-class call (group 1) sums the short floats of the
-// front-segment records only, a slow-class call (group 2) the long ones.
+// j reactions per atom: one wavefront per j-cluster.  Every lane loads
+// 16 B (four floats: four j slots of one component) of a record per step:
+// the front-segment records (two units: long x, z, y, short x, y, z, 8 j
+// slots each; lane = 12 (record of the step) + quarter, five records per
+// step), then the long-class records (one unit: long x, z, y; lane = 6
+// (record) + quarter, ten per step).  Each lane sums its floats as 64-bit
+// fixed point (round(v fscale) as the low bits of v fscale + 1.5 2^52,
+// exact for |v fscale| < 2^51, else converted directly), so the integer
+// totals do not depend on the record order; the lanes' sums meet in LDS
+// and lanes 0-47 each add up one (component, j slot) word, stored once:
+// no atomics, no clearing.  A fast-class call (group 1) sums the short
+// floats of the front-segment records only, a slow-class call (group 2)
+// the long ones.
 __device__ inline long long to_fixed(float v, double fscale, float vlim) {
     long long qv = __double_as_longlong(fma((double)v, fscale, kMagic)) - __double_as_longlong(kMagic);
     const bool big = fabsf(v) >= vlim;
@@ -670,57 +672,81 @@ __device__ inline long long to_fixed(float v, double fscale, float vlim) {
     return qv;
 }
 
-__device__ inline void jgather_body(const PairArgs &a, int c, int group) {
+__device__ inline void jgather_body(const PairArgs &a, int c, int group, long long *sh) {
     if (c >= a.nclu) return;
     const int lane = threadIdx.x;
-    const bool live = lane < 2 * kRecUnit;
-    const bool is_long = lane < kRecUnit;             // front records: long floats
-    const bool want1 = live && (is_long ? group != 1 : group != 2);
+    // front records: 12 quarters each, 5 per step (lanes 0-59)
+    const int fq = lane % 12, fr = lane / 12;
+    const bool fshort = fq >= 6;
+    const bool fwant = fr < 5 && (fshort ? group != 2 : group != 1);
+    // long records: 6 quarters each, 10 per step (lanes 0-59)
+    const int lq = lane % 6, lr = lane / 6;
+    const bool lwant = lr < 10 && group != 1;
     const unsigned u0 = sld(a.jstart)[c];
     const int2 nc = ldv(sld(a.jcnt) + c);
-    const MDX_GLOBAL float *rec = gld(a.rec) + (size_t)u0 * kRecUnit + lane;
+    using f4 = float __attribute__((ext_vector_type(4)));
+    const MDX_GLOBAL f4 *rec = reinterpret_cast<const MDX_GLOBAL f4 *>(gld(a.rec) + (size_t)u0 * kRecUnit);
     const double fscale = a.fscale;
     const float vlim = (float)(2251799813685248.0 / fscale);   // 2^51 / fscale
-    long long acc1 = 0, acc2 = 0;
-    constexpr int U = 8;       // loads in flight per lane
-    // front-segment records: 2 units each
-    for (int t0 = 0; t0 < nc.x; t0 += U) {
-        float v[U];
+    long long af[4] = {0, 0, 0, 0}, al[4] = {0, 0, 0, 0};
+    auto add4 = [&](long long (&acc)[4], f4 v) {
+        acc[0] += to_fixed(v.x, fscale, vlim);
+        acc[1] += to_fixed(v.y, fscale, vlim);
+        acc[2] += to_fixed(v.z, fscale, vlim);
+        acc[3] += to_fixed(v.w, fscale, vlim);
+    };
+    constexpr int U = 4;       // 16-byte loads in flight per lane
+    const f4 z4 = {0.f, 0.f, 0.f, 0.f};
+    for (int t0 = 0; t0 < nc.x; t0 += 5 * U) {
+        f4 v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            v[u] = (t0 + u < nc.x && want1) ? __builtin_nontemporal_load(rec + (size_t)(t0 + u) * 2 * kRecUnit) : 0.f;
+        for (int u = 0; u < U; ++u) {
+            const int r = t0 + 5 * u + fr;
+            v[u] = (fwant && r < nc.x) ? __builtin_nontemporal_load(rec + (size_t)r * 12 + fq) : z4;
+        }
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc1 += to_fixed(v[u], fscale, vlim);
+        for (int u = 0; u < U; ++u) add4(af, v[u]);
     }
-    // long-class records: 1 unit each, two per step
     if (group != 1) {
-        const MDX_GLOBAL float *rl = rec + (size_t)nc.x * 2 * kRecUnit;
-        const int half = lane < kRecUnit ? 0 : 1;
-        for (int t0 = 0; t0 < nc.y; t0 += 2 * U) {
-            float v[U];
+        const MDX_GLOBAL f4 *rl = rec + (size_t)nc.x * 12;
+        for (int t0 = 0; t0 < nc.y; t0 += 10 * U) {
+            f4 v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                v[u] = (live && t0 + 2 * u + half < nc.y) ? __builtin_nontemporal_load(rl + (size_t)(t0 + 2 * u) * kRecUnit)
-                                                          : 0.f;
+            for (int u = 0; u < U; ++u) {
+                const int r = t0 + 10 * u + lr;
+                v[u] = (lwant && r < nc.y) ? __builtin_nontemporal_load(rl + (size_t)r * 6 + lq) : z4;
+            }
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc2 += to_fixed(v[u], fscale, vlim);
+            for (int u = 0; u < U; ++u) add4(al, v[u]);
         }
     }
-    // lane l < 24 adds the long-record sums of lane l + 24
-    const long long up = (long long)__shfl_down((unsigned long long)acc2, kRecUnit);
-    const int jl = lane & 7;
-    if (is_long && group != 1) {
-        constexpr int word_long[3] = {3, 5, 4};           // long x, z, y
-        gst(a.jf)[((size_t)c * kClu + jl) * 8 + word_long[lane >> 3]] = (unsigned long long)(acc1 + acc2 + up);
-    } else if (live && !is_long && group != 2) {
-        gst(a.jf)[((size_t)c * kClu + jl) * 8 + (lane >> 3) - 3] = (unsigned long long)acc1;   // short x, y, z
+    // sh[0..255]: front sums [lane][4], sh[256..511]: long sums [lane][4]
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        sh[lane * 4 + k] = af[k];
+        sh[256 + lane * 4 + k] = al[k];
+    }
+    __syncthreads();
+    if (lane < 48) {
+        // word: component comp (record rows: long x, z, y, short x, y, z)
+        // and j slot jl; its floats are quarter 2 comp + jl / 4, element jl % 4
+        const int comp = lane >> 3, jl = lane & 7;
+        const int q = 2 * comp + (jl >> 2), e = jl & 3;
+        long long t = 0;
+        for (int r = 0; r < 5; ++r) t += sh[(12 * r + q) * 4 + e];
+        if (comp < 3)
+            for (int r = 0; r < 10; ++r) t += sh[256 + (6 * r + q) * 4 + e];
+        constexpr int word[6] = {3, 5, 4, 0, 1, 2};   // long x, z, y -> 3, 5, 4; short x, y, z -> 0, 1, 2
+        const bool want = comp < 3 ? group != 1 : group != 2;
+        if (want) gst(a.jf)[((size_t)c * kClu + jl) * 8 + word[comp]] = (unsigned long long)t;
     }
 }
 
 __global__ __launch_bounds__(64) void k_jgather(const PairArgs *__restrict__ args, const int2 *__restrict__ map,
                                                 int group) {
+    __shared__ long long sh[512];
     const int2 m = map[blockIdx.x];
-    jgather_body(args[m.x], m.y, group);
+    jgather_body(args[m.x], m.y, group, sh);
 }
 
 // Per-slot LJ type, ee flag and the low part of the charge (at list build).
