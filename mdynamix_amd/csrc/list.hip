@@ -252,8 +252,34 @@ __global__ __launch_bounds__(64) void k_masks(
     double4 p1 = spos[c1 * kClu + jl];
     // entries with a short-class pair fill the list from the front, the
     // others from the back (in discovery order both), so a fast-class call
-    // reads only the front segment
+    // reads only the front segment.  A listed candidate's (j-cluster,
+    // masks, list slot) go into lane nb of a buffer (a select per value); every 64
+    // of them, each lane builds its entry (image shift, per-pair-image
+    // flag) with its own loads of the j-cluster's anchor and extents.
     int cnt = 0, cnt_s = 0;
+    int nb = 0;
+    int bcj = 0, bslot = 0, bm0 = 0, bm1 = 0, bs0 = 0, bs1 = 0;
+    auto emit = [&]() {
+        if (lane < nb) {
+            PairEntry e;
+            double s[3];
+            bool wrap = false;
+            for (int q = 0; q < 3; ++q) {
+                // d * (1/L) and d / L round to the same image unless the
+                // anchors are near half a box apart; such entries take the
+                // minimum image per pair, so either image serves
+                const double d = aci[q] - anchor[bcj * 3 + q];
+                s[q] = d - b.l[q] * rint(d * ilb[q]);
+                wrap = wrap || rlist + eci[q] + (double)ext[bcj * 3 + q] + kImageMargin >= b.h[q];
+            }
+            e.cj = (int)((unsigned)bcj | (wrap ? kEntryWrap : 0u));
+            e.sx = (float)s[0]; e.sy = (float)s[1]; e.sz = (float)s[2];
+            e.mask = ((unsigned long long)(unsigned)bm1 << 32) | (unsigned)bm0;
+            e.mshort = ((unsigned long long)(unsigned)bs1 << 32) | (unsigned)bs0;
+            list[(size_t)ci * list_cap + bslot] = e;
+        }
+        nb = 0;
+    };
     for (int k = 0; k < nc; ++k) {
         const int cj = c1;
         const double4 pj = p1;
@@ -280,28 +306,21 @@ __global__ __launch_bounds__(64) void k_masks(
         const unsigned long long ms = __ballot(listed && shrt);
         if (m != 0ull) {
             const int slot = ms != 0ull ? cnt_s : list_cap - 1 - (cnt - cnt_s);
-            if (lane == 0 && cnt < list_cap) {
-                PairEntry e;
-                double s[3];
-                bool wrap = false;
-                for (int q = 0; q < 3; ++q) {
-                    // d * (1/L) and d / L round to the same image unless the
-                    // anchors are near half a box apart; such entries take the
-                    // minimum image per pair (below), so either image serves
-                    double d = aci[q] - anchor[cj * 3 + q];
-                    s[q] = d - b.l[q] * rint(d * ilb[q]);
-                    wrap = wrap || rlist + eci[q] + (double)ext[cj * 3 + q] + kImageMargin >= b.h[q];
-                }
-                e.cj = (int)((unsigned)cj | (wrap ? kEntryWrap : 0u));
-                e.sx = (float)s[0]; e.sy = (float)s[1]; e.sz = (float)s[2];
-                e.mask = m;
-                e.mshort = ms;
-                list[(size_t)ci * list_cap + slot] = e;
+            if (cnt < list_cap) {
+                const bool mine = lane == nb;
+                bcj = mine ? cj : bcj;
+                bslot = mine ? slot : bslot;
+                bm0 = mine ? (int)(unsigned)m : bm0;
+                bm1 = mine ? (int)(unsigned)(m >> 32) : bm1;
+                bs0 = mine ? (int)(unsigned)ms : bs0;
+                bs1 = mine ? (int)(unsigned)(ms >> 32) : bs1;
+                if (++nb == 64) emit();
             }
             ++cnt;
             if (ms != 0ull) ++cnt_s;
         }
     }
+    emit();
     if (lane == 0) {
         nlist[ci] = min(cnt, list_cap);
         nshort[ci] = min(cnt_s, list_cap);
