@@ -1402,8 +1402,10 @@ mdx_status mdx_build_list(mdx_engine *h) {
     // the grids are sized by the capacity n / 8 + ncol + 1, which bounds
     // the sum over columns of ceil(count / 8)
     k_fill_i32<<<cdiv(h->ncol, 256), 256, 0, st>>>(h->ncol, h->d_colcount, 0);
-    k_bin<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_x, h->d_y, h->d_z, h->box, h->ncx, h->ncy, h->d_keys, h->d_vals,
-                                         h->d_colcount);
+    // per-block column counts in LDS up to 16,384 columns (64 KiB)
+    const int lds_cols = h->ncol <= 16384 ? h->ncol : 0;
+    k_bin<<<cdiv(n, 256), 256, sizeof(int) * lds_cols, st>>>(n, h->d_x, h->d_y, h->d_z, h->box, h->ncx, h->ncy,
+                                                             lds_cols, h->d_keys, h->d_vals, h->d_colcount);
     k_scan_int<<<1, 1024, 0, st>>>(h->ncol, h->d_colcount, h->d_colstart, nullptr);
     k_scatter<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_keys, h->d_vals, h->d_colstart, h->d_keys2);
     k_col_sort<<<h->ncol, 256, 0, st>>>(h->d_colcount, h->d_colstart, h->d_keys2, h->d_keys, h->d_vals2);

@@ -16,26 +16,47 @@
 
 namespace mdx {
 
-// Wrap into [0, L) and bin into x-y columns; key = column << 32 | z rank.
-__global__ void k_bin(int n, const double *__restrict__ x, const double *__restrict__ y,
-                      const double *__restrict__ z, Box b, int ncx, int ncy,
-                      unsigned long long *__restrict__ keys, int *__restrict__ rank,
-                      int *__restrict__ colcount) {
+// Wrap into [0, L) and bin into x-y columns; key = column << 32 | z code.
+// Each atom gets a rank in its column, unique (the column sort orders by
+// (z code, atom index), so the ranks' order does not matter).  With
+// per-block counts in LDS (lds_cols > 0, the column count) a block ranks
+// its atoms with LDS atomics and adds each column's block count to the
+// global count once; otherwise every atom takes a global atomic.
+__global__ __launch_bounds__(256) void k_bin(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                             const double *__restrict__ z, Box b, int ncx, int ncy, int lds_cols,
+                                             unsigned long long *__restrict__ keys, int *__restrict__ rank,
+                                             int *__restrict__ colcount) {
+    extern __shared__ int bcnt[];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double w[3] = {x[i], y[i], z[i]};
-    for (int k = 0; k < 3; ++k) {
-        w[k] -= b.l[k] * floor(w[k] / b.l[k]);
-        if (!(w[k] < b.l[k]) || w[k] < 0.0) w[k] = 0.0;
+    for (int c = threadIdx.x; c < lds_cols; c += blockDim.x) bcnt[c] = 0;
+    int col = -1;
+    if (i < n) {
+        double w[3] = {x[i], y[i], z[i]};
+        for (int k = 0; k < 3; ++k) {
+            w[k] -= b.l[k] * floor(w[k] / b.l[k]);
+            if (!(w[k] < b.l[k]) || w[k] < 0.0) w[k] = 0.0;
+        }
+        int cx = (int)(w[0] / b.l[0] * ncx), cy = (int)(w[1] / b.l[1] * ncy);
+        cx = min(max(cx, 0), ncx - 1);
+        cy = min(max(cy, 0), ncy - 1);
+        col = cy * ncx + cx;
+        double fz = w[2] / b.l[2] * 4294967295.0;
+        unsigned int zq = (unsigned int)fmin(fmax(fz, 0.0), 4294967295.0);
+        keys[i] = ((unsigned long long)col << 32) | zq;
     }
-    int cx = (int)(w[0] / b.l[0] * ncx), cy = (int)(w[1] / b.l[1] * ncy);
-    cx = min(max(cx, 0), ncx - 1);
-    cy = min(max(cy, 0), ncy - 1);
-    const int col = cy * ncx + cx;
-    double fz = w[2] / b.l[2] * 4294967295.0;
-    unsigned int zq = (unsigned int)fmin(fmax(fz, 0.0), 4294967295.0);
-    keys[i] = ((unsigned long long)col << 32) | zq;
-    rank[i] = atomicAdd(&colcount[col], 1);   // arrival rank in the column
+    if (lds_cols == 0) {
+        if (col >= 0) rank[i] = atomicAdd(&colcount[col], 1);   // arrival rank in the column
+        return;
+    }
+    __syncthreads();
+    const int local = col >= 0 ? atomicAdd(&bcnt[col], 1) : 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < lds_cols; c += blockDim.x) {
+        const int k = bcnt[c];
+        if (k > 0) bcnt[c] = atomicAdd(&colcount[c], k);    // the block's base in the column
+    }
+    __syncthreads();
+    if (col >= 0) rank[i] = bcnt[col] + local;
 }
 
 // Exclusive prefix sum of n ints, one block of 1,024 threads (each sums a
@@ -154,8 +175,8 @@ __global__ void k_cluster_geom(const int *__restrict__ pnclu, const int *__restr
 // Candidate j-clusters of each i-cluster: bounding boxes within r_list,
 // j >= i (a half list: every cluster pair once, the i-cluster's own pairs
 // with i slot < j slot).  One wavefront per i-cluster; the clusters of each
-// neighbour column (periodic wrap, each column once) are tested 64 at a
-// time and the hits compacted in column order with a ballot, so the
+// row of neighbour columns (periodic wrap, each column once) are tested 64
+// at a time and the hits compacted in column order with a ballot, so the
 // candidate order is that of a serial scan.
 __global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu, const double *__restrict__ anchor,
                                                    const float *__restrict__ ext, const int *__restrict__ clu_col,
@@ -177,33 +198,47 @@ __global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu
     const int sy = (2 * ry + 1 >= ncy) ? 0 : cy0 - ry + ncy;
     const double r2 = rlist * rlist;
     int cnt = 0;
+    // clusters [lo, hi) tested 64 at a time, hits compacted in index order
+    // (j >= i: clusters below ci, which include every cluster of a column
+    // numbered below the i-cluster's, are left out)
+    auto scan = [&](int lo, int hi) {
+        for (int base = max(lo, ci); base < hi; base += 64) {
+            const int cj = base + lane;
+            bool hit = false;
+            if (cj < hi) {
+                double d2 = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    double d = ai[k] - anchor[cj * 3 + k];
+                    d -= b.l[k] * rint(d * il[k]);
+                    const double g = fabs(d) - (double)(ei[k] + ext[cj * 3 + k]);
+                    if (g > 0.0) d2 += g * g;
+                }
+                hit = d2 <= r2;
+            }
+            const unsigned long long m = __ballot(hit);
+            const int pos = cnt + __popcll(m & below);
+            if (hit && pos < cap) cand[(size_t)ci * cap + pos] = cj;
+            cnt += __popcll(m);
+        }
+    };
+    // clusters are numbered column by column, so the neighbour columns of
+    // one row are one index range, or two where they wrap around x (first
+    // the columns from the left edge of the neighbourhood to the box's
+    // edge, then those from x = 0): the order of a serial scan column by
+    // column
+    auto cols = [&](int c0, int c1) { return make_int2(colclu[c0], colclu[c1] + colclu_n[c1]); };
     for (int oy = 0; oy < ny; ++oy) {
         const int cy = (sy + oy) % ncy;
-        for (int ox = 0; ox < nx; ++ox) {
-            const int cx = (sx + ox) % ncx;
-            const int c2 = cy * ncx + cx;
-            // clusters are numbered column by column, so a column below the
-            // i-cluster's own holds only j < i: none of its pairs is kept
-            if (c2 < col) continue;
-            const int beg = colclu[c2], num = colclu_n[c2];
-            for (int base = 0; base < num; base += 64) {
-                const int cj = beg + base + lane;
-                bool hit = false;
-                if (base + lane < num) {
-                    double d2 = 0.0;
-                    for (int k = 0; k < 3; ++k) {
-                        double d = ai[k] - anchor[cj * 3 + k];
-                        d -= b.l[k] * rint(d * il[k]);
-                        const double g = fabs(d) - (double)(ei[k] + ext[cj * 3 + k]);
-                        if (g > 0.0) d2 += g * g;
-                    }
-                    hit = d2 <= r2 && cj >= ci;
-                }
-                const unsigned long long m = __ballot(hit);
-                const int pos = cnt + __popcll(m & below);
-                if (hit && pos < cap) cand[(size_t)ci * cap + pos] = cj;
-                cnt += __popcll(m);
-            }
+        const int r0 = cy * ncx;
+        if (r0 + ncx - 1 < col) continue;       // every column of the row below the i-cluster's
+        const int x0 = sx % ncx;
+        if (x0 + nx <= ncx) {
+            const int2 r = cols(r0 + x0, r0 + x0 + nx - 1);
+            scan(r.x, r.y);
+        } else {
+            const int2 ra = cols(r0 + x0, r0 + ncx - 1), rb = cols(r0, r0 + x0 + nx - ncx - 1);
+            scan(ra.x, ra.y);
+            scan(rb.x, rb.y);
         }
     }
     if (lane == 0) {
