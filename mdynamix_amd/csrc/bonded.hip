@@ -11,7 +11,7 @@
 // k_combine  F = G + H (CM_ADDLA, scalar.f:82-116) and the molecular virial
 //            sums WIRS/WIRSS (scalar.f:72-76, forces.f:991-995)
 // k_springs  path-integral bead springs (PIFORCES, pimd.f:321-394)
-// k_ereduce  final FP64 sum of all per-block partials
+// k_ereduce, k_ereduce_last  final FP64 sum of all per-block partials
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -430,31 +430,64 @@ __device__ inline void combine_body(const CombineArgs &a, int blk, int group) {
 }
 
 // Sum rows [0, nrows) of epart, clearing each value once read (the next
-// call then starts from zeros); block q (1,024 threads) sums slot q in a
-// fixed order: thread t keeps eight running sums over rows t + 1024 (8 i +
-// u), u = 0..7 (eight loads in flight per thread), adds them pairwise,
-// then butterflies inside each wavefront and the 16 wavefronts' sums in
-// order.
-__device__ inline void ereduce_body(const ReduceArgs &a, int q, double *wred) {
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double *e = a.epart + q;
-    for (int r0 = threadIdx.x; r0 < a.nrows; r0 += 8192)
+// call then starts from zeros), in two launches.  k_ereduce: block blk
+// takes rows kRedRows blk .. kRedRows (blk + 1) - 1; with RP = 256 / Q_N,
+// thread t < RP Q_N holds slot q = t % Q_N of rows p + RP u (p = t / Q_N,
+// u = 0..15: each of the block's loads covers RP consecutive rows, all
+// sixteen in flight) and adds them in a pairwise tree; the block's sum per
+// slot adds the RP threads' sums in order of p and goes to bpart[blk].
+// k_ereduce_last (one block per system): thread (q, p) sums the block sums
+// p, p + RP, ... in order, eight loads in flight, then the RP sums in order
+// of p.  Every addition has a fixed order.
+constexpr int kRedRP = 256 / Q_N;
+static_assert(kRedRows == 16 * kRedRP, "rows per reduction block");
+
+__device__ inline void ereduce_body(const ReduceArgs &a, int blk, double *wred) {
+    const int t = threadIdx.x, q = t % Q_N, p = t / Q_N;
+    const bool act = p < kRedRP;
+    double v[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int r = r0 + 1024 * u;
-            if (r < a.nrows) {
-                s[u] += e[(size_t)r * Q_N];
-                e[(size_t)r * Q_N] = 0.0;   // cleared for the next call
-            }
-        }
-    double v = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = v;
+    for (int u = 0; u < 16; ++u) {
+        const int r = blk * kRedRows + p + kRedRP * u;
+        v[u] = act && r < a.nrows ? a.epart[(size_t)r * Q_N + q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int r = blk * kRedRows + p + kRedRP * u;
+        if (act && r < a.nrows) a.epart[(size_t)r * Q_N + q] = 0.0;
+    }
+#pragma unroll
+    for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < h; ++u) v[u] += v[u + h];
+    wred[t] = v[0];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = wred[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t += wred[w];
-        a.out[q] = t;
+    if (t < Q_N) {
+        double b = wred[t];
+        for (int k = 1; k < kRedRP; ++k) b += wred[k * Q_N + t];
+        a.bpart[(size_t)blk * Q_N + t] = b;
+    }
+}
+
+__device__ inline void ereduce_last_body(const ReduceArgs &a, double *wred) {
+    const int t = threadIdx.x, q = t % Q_N, p = t / Q_N;
+    double s = 0.0;
+    if (p < kRedRP)
+        for (int b0 = p; b0 < a.nblk; b0 += 8 * kRedRP) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int b = b0 + kRedRP * u;
+                v[u] = b < a.nblk ? a.bpart[(size_t)b * Q_N + q] : 0.0;
+            }
+            s += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        }
+    wred[t] = s;
+    __syncthreads();
+    if (t < Q_N) {
+        double r = wred[t];
+        for (int k = 1; k < kRedRP; ++k) r += wred[k * Q_N + t];
+        a.out[t] = r;
     }
 }
 
@@ -519,11 +552,15 @@ __global__ __launch_bounds__(256) void k_combine(const CombineArgs *__restrict__
     combine_body(args[m.x], m.y, group);
 }
 
-__global__ __launch_bounds__(1024) void k_ereduce(const ReduceArgs *__restrict__ args, const int2 *__restrict__ map) {
-    __shared__ double wred[16];
+__global__ __launch_bounds__(256) void k_ereduce(const ReduceArgs *__restrict__ args, const int2 *__restrict__ map) {
+    __shared__ double wred[256];
     const int2 m = map[blockIdx.x];
-    if (m.y & 7) return;   // spacer block (keeps a system's slots on one XCD)
-    ereduce_body(args[m.x], m.y >> 3, wred);
+    ereduce_body(args[m.x], m.y, wred);
+}
+
+__global__ __launch_bounds__(256) void k_ereduce_last(const ReduceArgs *__restrict__ args) {
+    __shared__ double wred[256];
+    ereduce_last_body(args[blockIdx.x], wred);
 }
 
 __global__ void k_fill_i32(int n, int *__restrict__ p, int v) {

@@ -96,7 +96,7 @@ struct mdx_engine {
     long long version = 1;
     // per-system rows of the energy partials (fixed by capacities)
     int row_pairs = 0, row_pp = 0, row_sp = 0, row_sk = 0, row_et = 0, row_b = 0, row_s = 0, row_a = 0, row_t = 0, row_spr = 0,
-        row_c = 0, rows_total = 0;
+        row_c = 0, rows_main = 0, row_bp = 0, row_br = 0, rows_total = 0;
     unsigned long long *d_fixsp = nullptr;        // [3][n] bead springs
     std::map<void *, size_t> dev_sizes;           // live device allocations
     mdx_batch *self = nullptr;                    // batch of one for mdx_compute
@@ -745,6 +745,10 @@ void layout_rows(mdx_engine *h) {
     h->row_t = r; r += h->nt ? cdiv(h->nt, 256) : 0;
     h->row_spr = r; r += cdiv(h->n, 256);
     h->row_c = r; r += cdiv(h->n, 256);
+    h->rows_main = r;
+    // block sums of the two reductions (pair rows; rows_pp .. rows_main)
+    h->row_bp = r; r += cdiv(h->nclu_cap, kRedRows);
+    h->row_br = r; r += cdiv(h->rows_main - h->row_pp, kRedRows);
     h->rows_total = r;
 }
 
@@ -1739,11 +1743,14 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         ca.Hx = h->d_out + 6 * (size_t)n; ca.Hy = ca.Hx + n; ca.Hz = ca.Hx + 2 * n;
         ca.epart = erow(h, h->row_c);
         b->comb.add(ca, cdiv(n, 256));
-        // 8 blocks per energy slot, of which only every eighth works: the
-        // working blocks of one system then share an XCD (block index mod
-        // 8) and its L2, which reads the interleaved rows from HBM once
-        b->redp.add(ReduceArgs{h->nclu_cap, erow(h, h->row_pairs), erow(h, h->row_pp)}, Q_N * 8);
-        b->red.add(ReduceArgs{h->rows_total - h->row_pp, erow(h, h->row_pp), b->d_energy + (size_t)s * Q_N}, Q_N * 8);
+        // energy rows: blocks of kRedRows rows, then one block per system
+        // over their sums (one argument record per system, in order)
+        const int nbp = cdiv(h->nclu_cap, kRedRows), nbr = cdiv(h->rows_main - h->row_pp, kRedRows);
+        b->redp.add(ReduceArgs{h->nclu_cap, nbp, erow(h, h->row_pairs), erow(h, h->row_pp), erow(h, h->row_bp)},
+                    nbp);
+        b->red.add(ReduceArgs{h->rows_main - h->row_pp, nbr, erow(h, h->row_pp), b->d_energy + (size_t)s * Q_N,
+                              erow(h, h->row_br)},
+                   nbr);
     }
     CHK(kplan_upload(b, b->pack, maps)); CHK(kplan_upload(b, b->com, maps));
     CHK(kplan_upload(b, b->pairs, maps)); CHK(kplan_upload(b, b->pairs64, maps));
@@ -1765,6 +1772,13 @@ template <typename T, typename F>
 void launch(mdx_batch *b, KPlan<T> &k, int nthreads, size_t lds, F kernel, hipStream_t st = nullptr) {
     if (k.nblocks() == 0) return;
     kernel<<<k.nblocks(), nthreads, lds, st ? st : b->stream>>>(k.d_args, k.d_map);
+}
+
+// Energy rows of every system: block sums, then one block per system.
+void launch_reduce(mdx_batch *b, KPlan<ReduceArgs> &k, hipStream_t st) {
+    if (k.nblocks() == 0) return;
+    k_ereduce<<<k.nblocks(), 256, 0, st>>>(k.d_args, k.d_map);
+    k_ereduce_last<<<(int)k.h_args.size(), 256, 0, st>>>(k.d_args);
 }
 
 void launch_forked(mdx_batch *b, int group, bool detail, bool terms);
@@ -1852,9 +1866,9 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail, bool terms =
         if (terms) launch(b, b->terms, 256, 0, k_term_reduce);
     }
     mark(8);
-    launch(b, b->redp, 1024, 0, k_ereduce);
+    launch_reduce(b, b->redp, st);
     if (b->comb.nblocks()) k_combine<<<b->comb.nblocks(), 256, 0, st>>>(b->comb.d_args, b->comb.d_map, group);
-    launch(b, b->red, 1024, 0, k_ereduce);
+    launch_reduce(b, b->red, st);
     hipMemcpyAsync(b->h_energy, b->d_energy, sizeof(double) * Q_N * b->sys.size(), hipMemcpyDeviceToHost, st);
     if (detail) {
         launch(b, b->dred, 256, 0, k_detail_reduce);
@@ -1940,14 +1954,14 @@ void launch_forked(mdx_batch *b, int group, bool detail, bool terms) {
             else k_pairs64<false, false><<<nb, 64, 0, st>>>(pa, pm, group);
         }
     }
-    launch(b, b->redp, 1024, 0, k_ereduce);
+    launch_reduce(b, b->redp, st);
     // join
     hipEventRecord(b->join_ev[0], sk);
     hipEventRecord(b->join_ev[1], sb);
     hipStreamWaitEvent(st, b->join_ev[0], 0);
     hipStreamWaitEvent(st, b->join_ev[1], 0);
     if (b->comb.nblocks()) k_combine<<<b->comb.nblocks(), 256, 0, st>>>(b->comb.d_args, b->comb.d_map, group);
-    launch(b, b->red, 1024, 0, k_ereduce);
+    launch_reduce(b, b->red, st);
     hipMemcpyAsync(b->h_energy, b->d_energy, sizeof(double) * Q_N * b->sys.size(), hipMemcpyDeviceToHost, st);
     if (detail) {
         launch(b, b->dred, 256, 0, k_detail_reduce);

@@ -50,6 +50,7 @@ constexpr int kForceBlock = 64 * kForceTiles;   // atoms per block (4 wavefronts
 constexpr int kComAtoms = 512;     // LDS stage (atoms) of the centre-of-mass kernels
 constexpr int kSkRows = 64;        // k rows per structure-factor block (16 per wavefront)
 constexpr int kSkSub = 32;         // atoms per LDS stage of the structure-factor block
+constexpr int kRedRows = 192;      // energy rows per block of the final reduction (16 x 256 / Q_N)
 
 // Energy/virial slots of the per-block partial sums.
 enum EQ {
@@ -410,9 +411,10 @@ struct DetailArgs {
 };
 
 struct ReduceArgs {
-    int nrows;
+    int nrows, nblk;               // nblk = ceil(nrows / kRedRows) blocks
     double *epart;                 // [nrows][Q_N], cleared once read
     double *out;                   // Q_N
+    double *bpart;                 // [nblk][Q_N] block sums
 };
 
 // Device-resident leap-frog step (SULTAN, mdstep.f:29-260) of one system.
