@@ -116,6 +116,7 @@ struct mdx_engine {
     double4 *d_spos = nullptr;                    // per slot: FP64 x, y, z, molecule (list build)
     double *d_anchor = nullptr;
     float *d_ext = nullptr;
+    double *d_rbox = nullptr;          // per cluster: lowest, highest x, y, z as stored (not wrapped)
     int cand_cap = 0, list_cap = 0;
     int *d_cand = nullptr, *d_ncand = nullptr, *d_nlist = nullptr, *d_nshort = nullptr, *d_overflow = nullptr;
     PairEntry *d_list = nullptr;
@@ -649,6 +650,10 @@ mdx_status alloc_clusters(mdx_engine *h) {
     h->ncy = std::max(1, (int)(h->p.box[1] / cw));
     h->ncol = h->ncx * h->ncy;
     h->nclu_cap = h->n / kClu + h->ncol + 1;
+    if (h->nclu_cap > (int)kCandIdx) {
+        h->err = "too many atoms for the cluster index of the candidate lists";
+        return MDX_ERR_ARG;
+    }
     CHK(dalloc(h, &h->d_keys, h->n));
     CHK(dalloc(h, &h->d_keys2, h->n));
     CHK(dalloc(h, &h->d_vals, h->n));
@@ -665,6 +670,7 @@ mdx_status alloc_clusters(mdx_engine *h) {
     CHK(dalloc(h, &h->d_clu_col, h->nclu_cap));
     CHK(dalloc(h, &h->d_anchor, (size_t)h->nclu_cap * 3));
     CHK(dalloc(h, &h->d_ext, (size_t)h->nclu_cap * 3));
+    CHK(dalloc(h, &h->d_rbox, (size_t)h->nclu_cap * 6));
     CHK(dalloc(h, &h->d_ncand, h->nclu_cap));
     CHK(dalloc(h, &h->d_nlist, h->nclu_cap));
     CHK(dalloc(h, &h->d_nshort, h->nclu_cap));
@@ -1420,7 +1426,7 @@ mdx_status mdx_build_list(mdx_engine *h) {
     k_fill<<<cdiv(n, 256), 256, 0, st>>>(n, h->d_keys, h->d_vals2, h->d_colstart, h->d_colclu, h->d_clu_atom,
                                           h->d_atom_slot, h->d_clu_col);
     k_cluster_geom<<<cdiv(cap, 256), 256, 0, st>>>(h->d_nclu, h->d_clu_atom, h->d_x, h->d_y, h->d_z, h->box,
-                                                    h->d_anchor, h->d_ext);
+                                                    h->d_anchor, h->d_ext, h->d_rbox);
     k_slot_pos<<<cdiv(nslot_cap, 256), 256, 0, st>>>(h->d_nclu, h->d_clu_atom, h->d_x, h->d_y, h->d_z, h->d_mol,
                                                       h->d_spos);
     const double rl2 = list_radius2(h);
@@ -1429,7 +1435,7 @@ mdx_status mdx_build_list(mdx_engine *h) {
     const int rx = (int)std::ceil((rl + cwx) / cwx), ry = (int)std::ceil((rl + cwy) / cwy);
     for (int attempt = 0; attempt < 4; ++attempt) {
         k_fill_i32<<<1, 64, 0, st>>>(1, h->d_overflow, 0);
-        k_candidates<<<cap, 64, 0, st>>>(h->d_nclu, h->d_anchor, h->d_ext, h->d_clu_col, h->d_colclu,
+        k_candidates<<<cap, 64, 0, st>>>(h->d_nclu, h->d_anchor, h->d_ext, h->d_rbox, h->d_clu_col, h->d_colclu,
                                                           h->d_colclu_n, h->ncx, h->ncy, rx, ry, h->box, rl,
                                                           h->cand_cap, h->d_cand, h->d_ncand, h->d_overflow);
         k_masks<<<cap, 64, 0, st>>>(h->d_nclu, h->d_cand, h->d_ncand, h->cand_cap, h->d_clu_atom, h->d_spos,

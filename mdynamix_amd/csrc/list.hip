@@ -152,15 +152,18 @@ __global__ void k_fill(int n, const unsigned long long *__restrict__ skeys, cons
 // and half extents.
 __global__ void k_cluster_geom(const int *__restrict__ pnclu, const int *__restrict__ clu_atom, const double *__restrict__ x,
                                const double *__restrict__ y, const double *__restrict__ z, Box b,
-                               double *__restrict__ anchor, float *__restrict__ ext) {
+                               double *__restrict__ anchor, float *__restrict__ ext, double *__restrict__ rbox) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= *pnclu) return;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    double rlo[3] = {1e300, 1e300, 1e300}, rhi[3] = {-1e300, -1e300, -1e300};
     for (int s = 0; s < kClu; ++s) {
         const int a = clu_atom[c * kClu + s];
         if (a < 0) continue;
         double w[3] = {x[a], y[a], z[a]};
         for (int k = 0; k < 3; ++k) {
+            rlo[k] = fmin(rlo[k], w[k]);
+            rhi[k] = fmax(rhi[k], w[k]);
             w[k] -= b.l[k] * floor(w[k] / b.l[k]);
             lo[k] = fmin(lo[k], w[k]);
             hi[k] = fmax(hi[k], w[k]);
@@ -169,7 +172,29 @@ __global__ void k_cluster_geom(const int *__restrict__ pnclu, const int *__restr
     for (int k = 0; k < 3; ++k) {
         anchor[c * 3 + k] = 0.5 * (lo[k] + hi[k]);
         ext[c * 3 + k] = (float)(0.5 * (hi[k] - lo[k])) * 1.0001f + 1e-4f;
+        rbox[c * 6 + k] = rlo[k];
+        rbox[c * 6 + 3 + k] = rhi[k];
     }
+}
+
+// Image code of a cluster pair (bits kCandCode + 2 q, axis q): the stored
+// coordinates' differences x_i - x_j of all its atom pairs lie between the
+// rounded differences of the two clusters' extreme stored coordinates
+// (rounding is monotonic), so when those bounds fall on one side of the
+// minimum-image test |d| > L / 2, the test has the same outcome for every
+// pair: 0 none, 1 every d - L, 2 every d + L, 3 mixed (tested per pair).
+__device__ inline unsigned image_code(const double (&rli)[3], const double (&rhi)[3], const double *rj, const Box &b) {
+#pragma clang fp contract(off)
+    unsigned code = 0;
+    for (int q = 0; q < 3; ++q) {
+        const double dmin = rli[q] - rj[3 + q], dmax = rhi[q] - rj[q];
+        unsigned c = 3u;
+        if (dmin > b.h[q]) c = 1u;
+        else if (dmax < -b.h[q]) c = 2u;
+        else if (dmax <= b.h[q] && dmin >= -b.h[q]) c = 0u;
+        code |= c << (kCandCode + 2 * q);
+    }
+    return code;
 }
 
 // Candidate j-clusters of each i-cluster: bounding boxes within r_list,
@@ -179,7 +204,8 @@ __global__ void k_cluster_geom(const int *__restrict__ pnclu, const int *__restr
 // at a time and the hits compacted in column order with a ballot, so the
 // candidate order is that of a serial scan.
 __global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu, const double *__restrict__ anchor,
-                                                   const float *__restrict__ ext, const int *__restrict__ clu_col,
+                                                   const float *__restrict__ ext, const double *__restrict__ rbox,
+                                                   const int *__restrict__ clu_col,
                                                    const int *__restrict__ colclu, const int *__restrict__ colclu_n,
                                                    int ncx, int ncy, int rx, int ry, Box b, double rlist, int cap,
                                                    int *__restrict__ cand, int *__restrict__ ncand,
@@ -197,6 +223,8 @@ __global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu
     const int sx = (2 * rx + 1 >= ncx) ? 0 : cx0 - rx + ncx;
     const int sy = (2 * ry + 1 >= ncy) ? 0 : cy0 - ry + ncy;
     const double r2 = rlist * rlist;
+    const double rli[3] = {rbox[ci * 6], rbox[ci * 6 + 1], rbox[ci * 6 + 2]};
+    const double rhi[3] = {rbox[ci * 6 + 3], rbox[ci * 6 + 4], rbox[ci * 6 + 5]};
     int cnt = 0;
     // clusters [lo, hi) tested 64 at a time, hits compacted in index order
     // (j >= i: clusters below ci, which include every cluster of a column
@@ -217,7 +245,7 @@ __global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu
             }
             const unsigned long long m = __ballot(hit);
             const int pos = cnt + __popcll(m & below);
-            if (hit && pos < cap) cand[(size_t)ci * cap + pos] = cj;
+            if (hit && pos < cap) cand[(size_t)ci * cap + pos] = (int)((unsigned)cj | image_code(rli, rhi, rbox + cj * 6, b));
             cnt += __popcll(m);
         }
     };
@@ -284,7 +312,7 @@ __global__ __launch_bounds__(64) void k_masks(
     const int nl = max(nc - 1, 0);
     int c1 = nc > 0 ? cl[0] : ci, c2 = cl[min(1, nl)];
     if (nc == 0) c2 = ci;
-    double4 p1 = spos[c1 * kClu + jl];
+    double4 p1 = spos[(int)((unsigned)c1 & kCandIdx) * kClu + jl];
     // entries with a short-class pair fill the list from the front, the
     // others from the back (in discovery order both), so a fast-class call
     // reads only the front segment.  A listed candidate's (j-cluster,
@@ -316,17 +344,29 @@ __global__ __launch_bounds__(64) void k_masks(
         nb = 0;
     };
     for (int k = 0; k < nc; ++k) {
-        const int cj = c1;
+        const int cj = (int)((unsigned)c1 & kCandIdx);
+        const unsigned code = (unsigned)c1 >> kCandCode;   // image code (image_code)
         const double4 pj = p1;
         c1 = c2;
-        p1 = spos[c1 * kClu + jl];
+        p1 = spos[(int)((unsigned)c1 & kCandIdx) * kClu + jl];
         c2 = cl[min(k + 2, nl)];
         bool listed = false, shrt = false;
         // an empty slot has molecule -1; (cj != ci || il < jl) keeps each
         // pair once and leaves out an atom with itself
         if (moli >= 0 && pj.w >= 0.0 && (cj != ci || il < jl)) {
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            min_image(dx, dy, dz, b);
+            // the reference's minimum image: per pair where the cluster
+            // pair's code leaves it open, otherwise its known outcome (the
+            // same subtraction, so the same rounding)
+            if (code != 0u) {
+                if ((code & (code >> 1) & 0x15u) != 0u) min_image(dx, dy, dz, b);
+                else {
+                    const unsigned cx = code & 3u, cy = (code >> 2) & 3u, cz = (code >> 4) & 3u;
+                    if (cx) dx = cx == 1u ? dx - b.l[0] : dx + b.l[0];
+                    if (cy) dy = cy == 1u ? dy - b.l[1] : dy + b.l[1];
+                    if (cz) dz = cz == 1u ? dz - b.l[2] : dz + b.l[2];
+                }
+            }
             double rr2 = dx * dx + dy * dy;
             rr2 = rr2 + dz * dz;
             listed = rr2 <= rl2;

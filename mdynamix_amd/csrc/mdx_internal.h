@@ -74,6 +74,10 @@ struct PairEntry {          // 32 bytes, read with scalar loads
 // to the pair distance and twice to the extents).
 constexpr uint32_t kEntryWrap = 0x80000000u;
 constexpr double kImageMargin = 4.0;
+// Candidate list word: j-cluster index in the low kCandCode bits, the pair's
+// image code (2 bits per axis) above.
+constexpr int kCandCode = 26;
+constexpr unsigned kCandIdx = (1u << kCandCode) - 1u;
 
 // Special pair classes.
 enum { SP_NONE = 0, SP_SHORT14 = 1, SP_SHORT = 2, SP_LONG = 3 };
