@@ -116,7 +116,7 @@ struct mdx_engine {
     double4 *d_spos = nullptr;                    // per slot: FP64 x, y, z, molecule (list build)
     double *d_anchor = nullptr;
     float *d_ext = nullptr;
-    double *d_rbox = nullptr;          // per cluster: lowest, highest x, y, z as stored (not wrapped)
+    float *d_rbox = nullptr;           // per cluster: lowest, highest stored x, y, z (not wrapped; FP32 outwards)
     int cand_cap = 0, list_cap = 0;
     int *d_cand = nullptr, *d_ncand = nullptr, *d_nlist = nullptr, *d_nshort = nullptr, *d_overflow = nullptr;
     PairEntry *d_list = nullptr;

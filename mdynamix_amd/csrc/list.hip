@@ -152,7 +152,7 @@ __global__ void k_fill(int n, const unsigned long long *__restrict__ skeys, cons
 // and half extents.
 __global__ void k_cluster_geom(const int *__restrict__ pnclu, const int *__restrict__ clu_atom, const double *__restrict__ x,
                                const double *__restrict__ y, const double *__restrict__ z, Box b,
-                               double *__restrict__ anchor, float *__restrict__ ext, double *__restrict__ rbox) {
+                               double *__restrict__ anchor, float *__restrict__ ext, float *__restrict__ rbox) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= *pnclu) return;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -172,22 +172,23 @@ __global__ void k_cluster_geom(const int *__restrict__ pnclu, const int *__restr
     for (int k = 0; k < 3; ++k) {
         anchor[c * 3 + k] = 0.5 * (lo[k] + hi[k]);
         ext[c * 3 + k] = (float)(0.5 * (hi[k] - lo[k])) * 1.0001f + 1e-4f;
-        rbox[c * 6 + k] = rlo[k];
-        rbox[c * 6 + 3 + k] = rhi[k];
+        rbox[c * 6 + k] = __double2float_rd(rlo[k]);        // rounded outwards
+        rbox[c * 6 + 3 + k] = __double2float_ru(rhi[k]);
     }
 }
 
-// Image code of a cluster pair (bits kCandCode + 2 q, axis q): the stored
-// coordinates' differences x_i - x_j of all its atom pairs lie between the
-// rounded differences of the two clusters' extreme stored coordinates
-// (rounding is monotonic), so when those bounds fall on one side of the
-// minimum-image test |d| > L / 2, the test has the same outcome for every
-// pair: 0 none, 1 every d - L, 2 every d + L, 3 mixed (tested per pair).
-__device__ inline unsigned image_code(const double (&rli)[3], const double (&rhi)[3], const double *rj, const Box &b) {
+// Image code of a cluster pair (bits kCandCode + 2 q, axis q): the FP64
+// differences x_i - x_j of the stored coordinates of all its atom pairs lie
+// between the FP64 differences of the clusters' coordinate bounds (FP32,
+// rounded outwards; rounding is monotonic), so when those fall on one side
+// of the minimum-image test |d| > L / 2, the test has the same outcome for
+// every pair: 0 none, 1 every d - L, 2 every d + L, 3 mixed (tested per
+// pair).
+__device__ inline unsigned image_code(const float (&rli)[3], const float (&rhi)[3], const float (&rj)[6], const Box &b) {
 #pragma clang fp contract(off)
     unsigned code = 0;
     for (int q = 0; q < 3; ++q) {
-        const double dmin = rli[q] - rj[3 + q], dmax = rhi[q] - rj[q];
+        const double dmin = (double)rli[q] - (double)rj[3 + q], dmax = (double)rhi[q] - (double)rj[q];
         unsigned c = 3u;
         if (dmin > b.h[q]) c = 1u;
         else if (dmax < -b.h[q]) c = 2u;
@@ -204,7 +205,7 @@ __device__ inline unsigned image_code(const double (&rli)[3], const double (&rhi
 // at a time and the hits compacted in column order with a ballot, so the
 // candidate order is that of a serial scan.
 __global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu, const double *__restrict__ anchor,
-                                                   const float *__restrict__ ext, const double *__restrict__ rbox,
+                                                   const float *__restrict__ ext, const float *__restrict__ rbox,
                                                    const int *__restrict__ clu_col,
                                                    const int *__restrict__ colclu, const int *__restrict__ colclu_n,
                                                    int ncx, int ncy, int rx, int ry, Box b, double rlist, int cap,
@@ -223,8 +224,8 @@ __global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu
     const int sx = (2 * rx + 1 >= ncx) ? 0 : cx0 - rx + ncx;
     const int sy = (2 * ry + 1 >= ncy) ? 0 : cy0 - ry + ncy;
     const double r2 = rlist * rlist;
-    const double rli[3] = {rbox[ci * 6], rbox[ci * 6 + 1], rbox[ci * 6 + 2]};
-    const double rhi[3] = {rbox[ci * 6 + 3], rbox[ci * 6 + 4], rbox[ci * 6 + 5]};
+    const float rli[3] = {rbox[ci * 6], rbox[ci * 6 + 1], rbox[ci * 6 + 2]};
+    const float rhi[3] = {rbox[ci * 6 + 3], rbox[ci * 6 + 4], rbox[ci * 6 + 5]};
     int cnt = 0;
     // clusters [lo, hi) tested 64 at a time, hits compacted in index order
     // (j >= i: clusters below ci, which include every cluster of a column
@@ -233,19 +234,27 @@ __global__ __launch_bounds__(64) void k_candidates(const int *__restrict__ pnclu
         for (int base = max(lo, ci); base < hi; base += 64) {
             const int cj = base + lane;
             bool hit = false;
+            float rj[6];                    // loaded with the anchor (one round of loads)
             if (cj < hi) {
+                double aj[3];
+                float ej[3];
+                for (int k = 0; k < 3; ++k) {
+                    aj[k] = anchor[cj * 3 + k];
+                    ej[k] = ext[cj * 3 + k];
+                }
+                for (int k = 0; k < 6; ++k) rj[k] = rbox[cj * 6 + k];
                 double d2 = 0.0;
                 for (int k = 0; k < 3; ++k) {
-                    double d = ai[k] - anchor[cj * 3 + k];
+                    double d = ai[k] - aj[k];
                     d -= b.l[k] * rint(d * il[k]);
-                    const double g = fabs(d) - (double)(ei[k] + ext[cj * 3 + k]);
+                    const double g = fabs(d) - (double)(ei[k] + ej[k]);
                     if (g > 0.0) d2 += g * g;
                 }
                 hit = d2 <= r2;
             }
             const unsigned long long m = __ballot(hit);
             const int pos = cnt + __popcll(m & below);
-            if (hit && pos < cap) cand[(size_t)ci * cap + pos] = (int)((unsigned)cj | image_code(rli, rhi, rbox + cj * 6, b));
+            if (hit && pos < cap) cand[(size_t)ci * cap + pos] = (int)((unsigned)cj | image_code(rli, rhi, rj, b));
             cnt += __popcll(m);
         }
     };
