@@ -312,7 +312,8 @@ mdx_status mdx_set_timing(mdx_engine *h, int32_t on);
  * force pass), then the pair stage over cluster geometries (the width of
  * the x-y columns whose z-sorted atoms form the 8-atom clusters: every
  * column count along x between 0.75 and 1.3 times the cube root of 8
- * atoms' volume, scored by the pair stage plus a tenth of a list build),
+ * atoms' volume, scored by the pair stage plus a tenth of a list build)
+ * and over the pair kernel's occupancy (five or six wavefronts per SIMD),
  * on the current coordinates, and
  * keeps the fastest; the list is rebuilt at the current coordinates with
  * the chosen grid.  The choice is remembered for the process by size
@@ -329,6 +330,11 @@ mdx_status mdx_set_cluster_grid(mdx_engine *h, double scale);
 /* The cluster grid in use: column width (A, along x) and the column counts
  * along x and y.  No counterpart in the reference. */
 mdx_status mdx_cluster_grid(const mdx_engine *h, double *column_width, int32_t *ncx, int32_t *ncy);
+/* Wavefronts per SIMD of the plain pair kernel (5 or 6; the kernel's
+ * arithmetic is the same, only its register budget differs): set (a fixed
+ * choice for profiling runs) and read.  No counterpart in the reference. */
+mdx_status mdx_set_pair_waves(mdx_engine *h, int32_t waves);
+mdx_status mdx_pair_waves(const mdx_engine *h, int32_t *waves);
 /* The k vectors of the reciprocal sum in INIFUR order (forces.f:1375-1427,
  * rectangular cell): integer triples (n_x, n_y, n_z) with k = 2 pi n / L,
  * i.e. RKX = 2 pi n_x as INIFUR stores it.  *nkv receives the count; the

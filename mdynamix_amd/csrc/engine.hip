@@ -94,6 +94,7 @@ struct mdx_engine {
     AngleArgs a_angles{};
     TorsArgs a_tors{};
     long long version = 1;
+    int pair_waves = 6;              // wavefronts per SIMD of the plain pair kernel (5 or 6)
     // per-system rows of the energy partials (fixed by capacities)
     int row_pairs = 0, row_pp = 0, row_sp = 0, row_sk = 0, row_et = 0, row_b = 0, row_s = 0, row_a = 0, row_t = 0, row_spr = 0,
         row_c = 0, rows_main = 0, row_bp = 0, row_br = 0, rows_total = 0;
@@ -227,6 +228,7 @@ struct mdx_batch {
     bool planned = false;
     int nlj_max = 1;
     bool soft = false;
+    int pair_waves = 6;              // the first system's choice
     KPlan<PackArgs> pack, com;
     KPlan<PairArgs> pairs;
     KPlan<PairArgs> pairs64;         // systems in the FP64 precision mode
@@ -786,6 +788,7 @@ struct TuneKey {
 struct TuneShape {
     int chunk = 0, nslice = 0;
     double cw = 0.0;
+    int pw = 0;                      // pair kernel wavefronts per SIMD
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneShape> g_tune;
@@ -1264,6 +1267,7 @@ static mdx_status create_impl(const mdx_topology *t, const mdx_params *p, mdx_en
         if (it != g_tune.end()) shape = it->second;
     }
     if (shape.cw > 0.0) h->cw_scale = shape.cw;
+    if (shape.pw > 0) h->pair_waves = shape.pw;
     if ((s = alloc_clusters(h))) return fail(s);
     if ((s = dalloc_zero(h, &h->d_fixsp, (size_t)3 * n))) return fail(s);
     layout_rows(h);
@@ -1673,6 +1677,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
     b->nmt_max = 1;
     b->nlj_max = 1;
     b->soft = false;
+    b->pair_waves = b->sys.empty() ? 6 : b->sys[0]->pair_waves;
     for (int s = 0; s < K; ++s) {
         mdx_engine *h = b->sys[s];
         const int n = h->n;
@@ -1810,11 +1815,12 @@ void launch_call(mdx_batch *b, int group, bool timing, bool detail, bool terms =
         const PairArgs *pa = b->pairs.d_args;
         const int2 *pm = b->pairs.d_map;
         if (detail) {
-            if (b->soft) k_pairs<true, true><<<nb, 64, ldet, st>>>(pa, pm, group);
-            else k_pairs<false, true><<<nb, 64, ldet, st>>>(pa, pm, group);
+            if (b->soft) k_pairs<true, true, 5><<<nb, 64, ldet, st>>>(pa, pm, group);
+            else k_pairs<false, true, 5><<<nb, 64, ldet, st>>>(pa, pm, group);
         } else {
-            if (b->soft) k_pairs<true, false><<<nb, 64, lj, st>>>(pa, pm, group);
-            else k_pairs<false, false><<<nb, 64, lj, st>>>(pa, pm, group);
+            if (b->soft) k_pairs<true, false, 5><<<nb, 64, lj, st>>>(pa, pm, group);
+            else if (b->pair_waves == 5) k_pairs<false, false, 5><<<nb, 64, lj, st>>>(pa, pm, group);
+            else k_pairs<false, false, 6><<<nb, 64, lj, st>>>(pa, pm, group);
         }
         k_jgather<<<nb, 64, 0, st>>>(pa, pm, group);
     }
@@ -1939,11 +1945,12 @@ void launch_forked(mdx_batch *b, int group, bool detail, bool terms) {
         const PairArgs *pa = b->pairs.d_args;
         const int2 *pm = b->pairs.d_map;
         if (detail) {
-            if (b->soft) k_pairs<true, true><<<nb, 64, ldet, st>>>(pa, pm, group);
-            else k_pairs<false, true><<<nb, 64, ldet, st>>>(pa, pm, group);
+            if (b->soft) k_pairs<true, true, 5><<<nb, 64, ldet, st>>>(pa, pm, group);
+            else k_pairs<false, true, 5><<<nb, 64, ldet, st>>>(pa, pm, group);
         } else {
-            if (b->soft) k_pairs<true, false><<<nb, 64, lj, st>>>(pa, pm, group);
-            else k_pairs<false, false><<<nb, 64, lj, st>>>(pa, pm, group);
+            if (b->soft) k_pairs<true, false, 5><<<nb, 64, lj, st>>>(pa, pm, group);
+            else if (b->pair_waves == 5) k_pairs<false, false, 5><<<nb, 64, lj, st>>>(pa, pm, group);
+            else k_pairs<false, false, 6><<<nb, 64, lj, st>>>(pa, pm, group);
         }
         k_jgather<<<nb, 64, 0, st>>>(pa, pm, group);
     }
@@ -2003,11 +2010,11 @@ mdx_status batch_prepare(mdx_batch *b, int group, bool detail) {
         // regrown capacity changes the maps and the captured graphs
         std::vector<int> before = {b->pairs.nblocks(), b->pairs64.nblocks(), b->sk.nblocks(), b->rforce.nblocks(),
                                    b->skred.nblocks(),
-                                   b->mz, b->nmt_max, b->nlj_max};
+                                   b->mz, b->nmt_max, b->nlj_max, b->pair_waves};
         CHK(plan_fill(b, true));
         std::vector<int> after = {b->pairs.nblocks(), b->pairs64.nblocks(), b->sk.nblocks(), b->rforce.nblocks(),
                                   b->skred.nblocks(),
-                                  b->mz, b->nmt_max, b->nlj_max};
+                                  b->mz, b->nmt_max, b->nlj_max, b->pair_waves};
         if (before != after) drop_graphs(b);
     }
     return MDX_OK;
@@ -2454,6 +2461,19 @@ mdx_status mdx_set_cluster_grid(mdx_engine *h, double scale) {
     return MDX_OK;
 }
 
+mdx_status mdx_set_pair_waves(mdx_engine *h, int32_t waves) {
+    if (!h || (waves != 5 && waves != 6)) return MDX_ERR_ARG;
+    h->pair_waves = waves;
+    h->version++;                    // batches plan and capture again
+    return MDX_OK;
+}
+
+mdx_status mdx_pair_waves(const mdx_engine *h, int32_t *waves) {
+    if (!h || !waves) return MDX_ERR_ARG;
+    *waves = h->pair_waves;
+    return MDX_OK;
+}
+
 mdx_status mdx_cluster_grid(const mdx_engine *h, double *column_width, int32_t *ncx, int32_t *ncy) {
     if (!h) return MDX_ERR_ARG;
     if (column_width) *column_width = h->p.box[0] / std::max(1, h->ncx);
@@ -2546,11 +2566,28 @@ mdx_status mdx_autotune(mdx_engine *h, int32_t *chunk_out, int32_t *nslice_out) 
     }
     CHK(regrid_clusters(h, best_cw));
     CHK(mdx_build_list(h));
+    // the plain pair kernel's wavefronts per SIMD on the chosen grid
+    int best_pw = h->pair_waves;
+    double best_pair = 1e30;
+    for (int pw : {6, 5}) {
+        CHK(mdx_set_pair_waves(h, pw));
+        double pair_ms = 1e30;
+        for (int rep = 0; rep < 6; ++rep) {
+            mdx_status st = batch_compute(b, MDX_FORCES_ALL, false);
+            if (st != MDX_OK) {
+                h->err = b->err;
+                return st;
+            }
+            if (rep > 0) pair_ms = std::min(pair_ms, (double)b->stage_ms[ST_PAIRS]);
+        }
+        if (pair_ms < best_pair) { best_pair = pair_ms; best_pw = pw; }
+    }
+    CHK(mdx_set_pair_waves(h, best_pw));
     h->timing = htiming0;
     b->timing = timing0;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune[tune_key(h)] = TuneShape{best_chunk, best_slice, best_cw};
+        g_tune[tune_key(h)] = TuneShape{best_chunk, best_slice, best_cw, best_pw};
     }
     if (chunk_out) *chunk_out = best_chunk;
     if (nslice_out) *nslice_out = best_slice;

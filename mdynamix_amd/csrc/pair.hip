@@ -177,8 +177,11 @@ __device__ inline void detail_out(const PairArgs &a, int ci, const double *s_det
     }
 }
 
+// (inlined into every kernel instantiation, which then gets its own
+// register budget)
 template <bool SOFT, bool DETAIL>
-__device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, double *s_acc, int group) {
+__device__ __attribute__((always_inline)) inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj,
+                                                                 double *s_acc, int group) {
     const int lane = threadIdx.x;
     const int nlj2 = a.nlj * a.nlj;
     for (int t = lane; t < nlj2; t += 64) s_lj[t] = ldv(gld(a.lj) + t);
@@ -459,11 +462,12 @@ __device__ inline void pairs_body(const PairArgs &a, int ci, float2 *s_lj, doubl
     }
 }
 
-// Six wavefronts per SIMD for the plain kernel (79 VGPRs, no spill:
-// 4-5 % faster than five at 96,174 atoms); the soft-core and detail
-// instantiations hold more live values and stay at five.
-template <bool SOFT, bool DETAIL>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((SOFT || DETAIL) ? 5 : 6)))
+// WPE wavefronts per SIMD: the plain kernel comes at six (79 VGPRs, no
+// spill: 4-5 % faster than five at 96,174 atoms) and at five, and the
+// autotuner picks one per size class; the soft-core and detail
+// instantiations hold more live values and run at five.
+template <bool SOFT, bool DETAIL, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_pairs(const PairArgs *__restrict__ args, const int2 *__restrict__ map, int group) {
     extern __shared__ float2 s_lj[];      // nlj x nlj (6 A6, 12 B12), then detail buckets
     __shared__ double s_acc[12 * 64];     // per-lane FP64 virial and energy sums
@@ -471,10 +475,11 @@ void k_pairs(const PairArgs *__restrict__ args, const int2 *__restrict__ map, in
     pairs_body<SOFT, DETAIL>(args[m.x], m.y, s_lj, s_acc, group);
 }
 
-template __global__ void k_pairs<false, false>(const PairArgs *, const int2 *, int);
-template __global__ void k_pairs<true, false>(const PairArgs *, const int2 *, int);
-template __global__ void k_pairs<false, true>(const PairArgs *, const int2 *, int);
-template __global__ void k_pairs<true, true>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<false, false, 6>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<false, false, 5>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<true, false, 5>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<false, true, 5>(const PairArgs *, const int2 *, int);
+template __global__ void k_pairs<true, true, 5>(const PairArgs *, const int2 *, int);
 
 // FP64 precision mode: the pairs of the same list, each in FP64 from the
 // caller's coordinates as FORCES and LOCALF form it (forces.f:1078-1183,

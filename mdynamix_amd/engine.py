@@ -76,6 +76,8 @@ def load_library() -> C.CDLL:
     L.mdx_autotune.argtypes = [pv, pi32, pi32]
     L.mdx_cluster_grid.argtypes = [pv, C.POINTER(C.c_double), pi32, pi32]
     L.mdx_set_cluster_grid.argtypes = [pv, C.c_double]
+    L.mdx_set_pair_waves.argtypes = [pv, C.c_int32]
+    L.mdx_pair_waves.argtypes = [pv, pi32]
     L.mdx_set_kspace_shape.argtypes = [pv, C.c_int32, C.c_int32]
     L.mdx_get_kvectors.argtypes = [pv, pi32, pi32, pi32, C.c_int64, C.POINTER(C.c_int64)]
     L.mdx_enumerate_kvectors.argtypes = [pv, pi32, pi32, pi32, C.c_int64, C.POINTER(C.c_int64)]
@@ -95,7 +97,8 @@ def load_library() -> C.CDLL:
                  "mdx_compute_device", "mdx_stage_times", "mdx_device_info", "mdx_get_forces", "mdx_set_timing",
                  "mdx_memory_bytes", "mdx_batch_create", "mdx_batch_set_ring", "mdx_batch_use_graph",
                  "mdx_batch_compute", "mdx_batch_destroy", "mdx_md_init", "mdx_md_run", "mdx_md_get_state",
-                 "mdx_batch_md_run", "mdx_autotune", "mdx_cluster_grid", "mdx_set_cluster_grid", "mdx_set_kspace_shape", "mdx_get_kvectors",
+                 "mdx_batch_md_run", "mdx_autotune", "mdx_cluster_grid", "mdx_set_cluster_grid",
+                 "mdx_set_pair_waves", "mdx_pair_waves", "mdx_set_kspace_shape", "mdx_get_kvectors",
                  "mdx_enumerate_kvectors", "mdx_md_shake_failures", "mdx_furir_workspace_bytes",
                  "mdx_furir_device", "mdx_furir_device_noalloc", "mdx_batch_pimd_init", "mdx_batch_pimd_chains"):
         getattr(L, name).restype = C.c_int
@@ -233,12 +236,25 @@ class ForceEngine:
         self._check(self.lib.mdx_set_timing(self._h, 1 if on else 0))
 
     def autotune(self) -> Dict[str, float]:
-        """Pick the k-space launch shape and the cluster grid of this size
-        class by timing (see mdx_autotune; the list is rebuilt at the
-        current coordinates); returns the chosen shape and column width."""
+        """Pick the k-space launch shape, the cluster grid and the pair
+        kernel's wavefronts per SIMD of this size class by timing (see
+        mdx_autotune; the list is rebuilt at the current coordinates);
+        returns the chosen shape, column width and wavefronts."""
         c, sl = C.c_int32(), C.c_int32()
         self._check(self.lib.mdx_autotune(self._h, C.byref(c), C.byref(sl)))
-        return {"chunk": c.value, "nslice": sl.value, "column_width": self.cluster_grid()["column_width"]}
+        return {"chunk": c.value, "nslice": sl.value, "column_width": self.cluster_grid()["column_width"],
+                "pair_waves": self.pair_waves()}
+
+    def set_pair_waves(self, waves: int) -> None:
+        """A fixed occupancy of the plain pair kernel, 5 or 6 wavefronts per
+        SIMD (mdx_set_pair_waves)."""
+        self._check(self.lib.mdx_set_pair_waves(self._h, int(waves)))
+
+    def pair_waves(self) -> int:
+        """Wavefronts per SIMD of the plain pair kernel (mdx_pair_waves)."""
+        w = C.c_int32()
+        self._check(self.lib.mdx_pair_waves(self._h, C.byref(w)))
+        return w.value
 
     def set_cluster_grid(self, scale: float) -> None:
         """A fixed cluster grid (mdx_set_cluster_grid)."""

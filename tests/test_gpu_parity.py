@@ -207,12 +207,41 @@ def test_autotuned_shape_keeps_parity():
     shape = eng.autotune()
     print("case water_508 tuned", shape, eng.cluster_grid(), flush=True)
     assert shape["chunk"] > 0 and shape["nslice"] > 0 and shape["column_width"] > 0.0
+    assert shape["pair_waves"] in (5, 6)
     for a, b in zip(lists0, eng.get_list()):
         assert np.array_equal(a, b)
     eng.set_molecule_com(golden_com(d, 0))
     f, g, h, e = eng.allforce()
     assert rms_rel(f, golden_forces(d, "F", 0)) <= FORCE_RMS_TOL
     check_energies(e, dict(zip(abi.MdxEnergies.SCALARS, d["ENERGY"][0])))
+
+
+def test_pair_occupancy_choices_agree():
+    """The plain pair kernel at five and at six wavefronts per SIMD (the
+    autotuner's two choices) is the same arithmetic: identical forces and
+    energies, through the eager call and the captured graph."""
+    d, topo, prm, eng = _engine_from_golden("water_508")
+    eng.set_positions(golden_positions(d))
+    eng.chcnb()
+    eng.set_molecule_com(golden_com(d, 0))
+    out = {}
+    for w in (6, 5):
+        eng.set_pair_waves(w)
+        assert eng.pair_waves() == w
+        for timing in (True, False):
+            eng.set_timing(timing)
+            f, g, h, e = eng.allforce()
+            print("case water_508 pair_waves", w, "timing", timing, flush=True)
+            out[(w, timing)] = (f.copy(), g.copy(), h.copy(), dict(e))
+    ref = out[(6, True)]
+    for k, v in out.items():
+        for a, b in zip(v[:3], ref[:3]):
+            assert np.array_equal(a, b), k
+        for name in ref[3]:
+            assert np.array_equal(np.asarray(v[3][name]), np.asarray(ref[3][name])), (k, name)
+    eng.set_timing(True)
+    with pytest.raises(Exception):
+        eng.set_pair_waves(4)
 
 
 def test_all_bonded_kinds_vs_oracle():
