@@ -139,16 +139,18 @@ class MdxEnergies(C.Structure):
                "viry", "virz", "virfx", "virfy", "virfz", "wirs", "wirss")
 
     def as_dict(self, nmoltypes: int = MDX_MAX_MOLTYPES, ntypepairs: int = MDX_MAX_TYPEPAIRS) -> Dict:
-        d = {k: float(getattr(self, k)) for k in self.SCALARS}
-        d["qek"] = float(self.qek)
-        d["virpi"] = float(self.virpi)
-        d["npairs_short"] = int(self.npairs_short)
-        d["npairs_long"] = int(self.npairs_long)
-        d["pote1"] = np.array(self.pote1[:ntypepairs])
-        d["potl1"] = np.array(self.potl1[:ntypepairs])
-        d["pes141"] = np.array(self.pes141[:nmoltypes])
-        d["psr141"] = np.array(self.psr141[:nmoltypes])
-        d["spee"] = np.array(self.spee[:nmoltypes])
+        # read through the struct's buffer (one view per field group; the
+        # scalars are the leading doubles, in SCALARS order)
+        c = type(self)
+        sc = np.frombuffer(self, np.float64, len(self.SCALARS), 0).tolist()
+        d = dict(zip(self.SCALARS, sc))
+        d["qek"], d["virpi"] = np.frombuffer(self, np.float64, 2, c.qek.offset).tolist()
+        d["npairs_short"], d["npairs_long"] = np.frombuffer(self, np.int64, 2, c.npairs_short.offset).tolist()
+        d["pote1"] = np.frombuffer(self, np.float64, ntypepairs, c.pote1.offset).copy()
+        d["potl1"] = np.frombuffer(self, np.float64, ntypepairs, c.potl1.offset).copy()
+        d["pes141"] = np.frombuffer(self, np.float64, nmoltypes, c.pes141.offset).copy()
+        d["psr141"] = np.frombuffer(self, np.float64, nmoltypes, c.psr141.offset).copy()
+        d["spee"] = np.frombuffer(self, np.float64, nmoltypes, c.spee.offset).copy()
         return d
 
 
