@@ -745,7 +745,7 @@ void layout_rows(mdx_engine *h) {
     // the rows the final reduction reads
     h->row_pp = r; r += 1;
     h->row_sp = r; r += h->nspecial ? cdiv(h->nspecial, 256) : 0;
-    h->row_sk = r; r += cdiv(h->npk + h->nfrows, 256);
+    h->row_sk = r; r += cdiv(h->npk + h->nfrows, kSkRedItems);
     h->row_et = r; r += h->neterm ? cdiv(h->neterm, 256) : 0;
     h->row_b = r; r += h->nb ? cdiv(h->nb, 256) : 0;
     h->row_s = r; r += h->nspc ? cdiv(h->nspc, 256) : 0;
@@ -1701,7 +1701,7 @@ mdx_status plan_fill(mdx_batch *b, bool maps) {
         if (!fp64) {
             b->rprep.add(h->a_recip, cdiv(n, 256));
             b->sk.add(h->a_recip, h->nsgroups * h->nchunk);
-            b->skred.add(h->a_recip, cdiv(h->npk + h->nfrows, 256));
+            b->skred.add(h->a_recip, cdiv(h->npk + h->nfrows, kSkRedItems));
             b->rforce.add(h->a_recip, h->a_recip.natom_blocks * h->nslice);
             b->mz = std::max(b->mz, h->mz);
         }

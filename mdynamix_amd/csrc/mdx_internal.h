@@ -50,6 +50,7 @@ constexpr int kForceBlock = 64 * kForceTiles;   // atoms per block (4 wavefronts
 constexpr int kComAtoms = 512;     // LDS stage (atoms) of the centre-of-mass kernels
 constexpr int kSkRows = 64;        // k rows per structure-factor block (16 per wavefront)
 constexpr int kSkSub = 32;         // atoms per LDS stage of the structure-factor block
+constexpr int kSkRedItems = 64;    // pair entries / rows per block of k_sk_reduce (four lanes each)
 constexpr int kRedRows = 192;      // energy rows per block of the final reduction (16 x 256 / Q_N)
 
 // Energy/virial slots of the per-block partial sums.

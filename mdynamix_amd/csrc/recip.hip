@@ -396,7 +396,15 @@ __device__ inline float2 k_terms(const RecipArgs &a, int ix, int iy, int iz, dou
 // 16, lane 16 (u / 4) + 4 row + s (s = 0..3 the sum), element 2 (u mod 4)
 // + (0: Re p_j, 1: Im p_j); hi halves first, then lo.
 __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)[256]) {
-    const int e = blk * 256 + threadIdx.x;
+    // four lanes per item: lane q sums the chunks q, q + 4, ... and the
+    // four lanes' sums are added in a fixed order (q 0 + 1, 2 + 3, then
+    // both); for a pair entry lane q then writes sum q of the operand
+    const int q = threadIdx.x & 3;
+    const int e = blk * kSkRedItems + (threadIdx.x >> 2);
+    auto quad_sum = [](double v) {
+        v += __shfl_xor(v, 1);
+        return v + __shfl_xor(v, 2);
+    };
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (e < a.npk) {
         const int4 k = ldv(gld(a.fpk) + e);
@@ -404,13 +412,17 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
         float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
         if (k.x >= 0 || k.y >= 0) {
             double ur = 0.0, ui = 0.0, vr = 0.0, vi = 0.0;
-            for (int c = 0; c < a.nchunk; ++c) {
+            for (int c = q; c < a.nchunk; c += 4) {
                 const double4 p = ldv(gld(a.ppart) + (size_t)c * a.npk + e);
                 ur += p.x;
                 ui += p.y;
                 vr += p.z;
                 vi += p.w;
             }
+            ur = quad_sum(ur);
+            ui = quad_sum(ui);
+            vr = quad_sum(vr);
+            vi = quad_sum(vi);
             float2 ap = make_float2(0.f, 0.f), am = make_float2(0.f, 0.f);
             if (k.x >= 0) ap = k_terms(a, fr.ix, fr.iy, k.z, ur - vi, ui + vr, acc);
             if (k.y >= 0) am = k_terms(a, fr.ix, fr.iy, -k.z, ur + vi, ui - vr, acc);
@@ -424,13 +436,12 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
                                    {d.x * sc, d.w * sc, fj * d.y, fj * d.z}};
             MDX_GLOBAL _Float16 *w = reinterpret_cast<MDX_GLOBAL _Float16 *>(gst(a.wm)) + fr.pad1 +
                                      (size_t)(jm >> 4) * kForceStepH + 8 * (16 * (u >> 2) + 4 * rl) + 2 * (u & 3);
-            for (int s = 0; s < 4; ++s)
-                for (int cm = 0; cm < 2; ++cm) {
-                    _Float16 hi, lo;
-                    split1(c[cm][s], hi, lo);
-                    w[8 * s + cm] = hi;
-                    w[512 + 8 * s + cm] = lo;
-                }
+            for (int cm = 0; cm < 2; ++cm) {
+                _Float16 hi, lo;
+                split1(c[cm][q], hi, lo);
+                w[8 * q + cm] = hi;
+                w[512 + 8 * q + cm] = lo;
+            }
         }
     } else if (e < a.npk + a.nfrows) {
         const int r = e - a.npk;
@@ -438,15 +449,19 @@ __device__ inline void sk_reduce_body(const RecipArgs &a, int blk, double (*red)
         float2 a0 = make_float2(0.f, 0.f);
         if (fr.k0 >= 0) {
             double sr = 0.0, si = 0.0;
-            for (int c = 0; c < a.nchunk; ++c) {
+            for (int c = q; c < a.nchunk; c += 4) {
                 const double2 p = ldv(gld(a.s0part) + (size_t)c * a.nfrows + r);
                 sr += p.x;
                 si += p.y;
             }
+            sr = quad_sum(sr);
+            si = quad_sum(si);
             a0 = k_terms(a, fr.ix, fr.iy, 0, sr, si, acc);
         }
-        stv(gst(a.fa0) + r, a0);
+        if (q == 0) stv(gst(a.fa0) + r, a0);
     }
+    if (q != 0)   // every lane of an item formed the same terms: count them once
+        for (int i = 0; i < 4; ++i) acc[i] = 0.0;
     for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = acc[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
