@@ -38,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3     # MI355X vector FP32 (published)
+F16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense f16 matrix (published)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 PAIR_FLOP = 74               # per pair within R_cut (forces.f:1071-1198 operation count)
 RECIP_FLOP = 21              # per (k, atom) term (forces.f:1288-1325 operation count)
@@ -59,6 +60,24 @@ def measured_traffic(kernel: str):
         for name, v in rows:
             if name.startswith("k_pairs<false, false"):
                 return v
+    return None
+
+
+def measured_mfma(kernel: str):
+    """Matrix instructions per launch of `kernel` (SQ_INSTS_MFMA) from the
+    committed instruction-counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "counters_spce_nacl_32k.md")
+    if not os.path.exists(path):
+        return None
+    names = None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if cells and cells[0] == "counter":
+            names = cells[1:]
+        elif names and cells and cells[0] == "SQ_INSTS_MFMA":
+            for name, v in zip(names, cells[1:]):
+                if name == kernel:
+                    return float(v)
     return None
 
 
@@ -591,6 +610,20 @@ def main() -> int:
                      "MI355X FP32 157.3 TFLOP/s (dense matrix or packed vector); traffic: HBM bytes per "
                      "launch from profiles/traffic_spce_nacl_32k.md (FETCH_SIZE/WRITE_SIZE, "
                      "copy-calibrated)")}
+    # the k-space passes against the matrix cores' own peak, from the work
+    # they execute: matrix instructions per launch (committed counters) x
+    # 16,384 FLOP (v_mfma_f32_16x16x32_f16) over the stage's duration
+    kspace = {}
+    for stage, kern in (("recip_sk", "k_sk<5>"), ("recip_force", "k_recip_force<32>")):
+        n_mfma, ms = measured_mfma(kern), stages.get(stage, 0.0)
+        if n_mfma and ms > 0:
+            tf = n_mfma * 16384.0 / (ms * 1e-3) / 1e12
+            kspace[kern] = {"mfma_per_launch": n_mfma, "avg_ms": round(ms, 4), "achieved": round(tf, 1),
+                            "peak": F16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": round(tf / F16_MFMA_PEAK_TFLOPS, 4),
+                            "algorithmic_tflops": round(RECIP_FLOP * kv_count * topo.natoms / 2.0
+                                                        / (ms * 1e-3) / 1e12, 1)}
+    roof["kspace_matrix"] = kspace
     out = {
         "metric": "pair-interactions/s (32k SPC/E + NaCl, full force call incl. Ewald k-space)",
         "value": value,
