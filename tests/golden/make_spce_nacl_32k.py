@@ -19,7 +19,7 @@ Two calls:
 The positions are regenerated by mdynamix_amd.systems.cfg_large(2025) and
 checked against the stored SHA-256 of their bytes.
 
-    python tests/golden/make_cfg2.py     # writes tests/golden/spce_nacl_32k.npz (~6 min, 2 processes)
+    python tests/golden/make_spce_nacl_32k.py     # writes tests/golden/spce_nacl_32k.npz (~6 min, 2 processes)
 """
 from __future__ import annotations
 

@@ -169,7 +169,7 @@ def test_fp64_full_size_box_vs_oracle_and_mixed_path():
     modes with the reference lists: the FP64 GPU forces and energies
     against the oracle's FP64 fixture; the mixed path's forces against the
     FP64 GPU ones."""
-    from tests.test_gpu_cfg2 import _fixture, _system
+    from tests.test_gpu_spce_nacl_32k import _fixture, _system
     fx = _fixture()
     s = _system(fx)
     topo, prm = topology.from_system(s)

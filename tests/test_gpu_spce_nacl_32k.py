@@ -1,7 +1,7 @@
 """The 32k SPC/E + 0.15 M NaCl box at its full size and parameters (96,174
 atoms, 98.7 A, R_cut 12 A, R_short 5 A, alpha R_cut 3.2, FEXP 11.5,
 48,052 k vectors), the configuration the benchmark measures, against the
-FP64 oracle's fixture (tests/golden/make_cfg2.py).
+FP64 oracle's fixture (tests/golden/make_spce_nacl_32k.py).
 
 Tolerances (the engine's precision contract): force RMS relative error
 <= 1e-5 against FP64 (F, and G and H, whose fixture copies are FP32, i.e.
@@ -46,7 +46,7 @@ def _system(fx):
     return s
 
 
-def test_cfg2_reference_lists_kset_and_forces():
+def test_spce_nacl_32k_reference_lists_kset_and_forces():
     fx = _fixture()
     s = _system(fx)
     topo, prm = topology.from_system(s)
@@ -54,7 +54,7 @@ def test_cfg2_reference_lists_kset_and_forces():
     eng = ForceEngine(topo, prm)
     try:
         eng.set_positions(s.pos)
-        print("case cfg2 reference lists", topo.natoms, flush=True)
+        print("case spce_nacl_32k reference lists", topo.natoms, flush=True)
         eng.chcnb()
         assert eng.list_counts() == tuple(int(v) for v in fx["LISTLEN_0"])
         s1, s2, l1, l2 = eng.get_list()
@@ -69,14 +69,14 @@ def test_cfg2_reference_lists_kset_and_forces():
         for shape in (None, (4096, 6)):
             if shape is not None:
                 eng.set_kspace_shape(*shape)
-            print("case cfg2 k-space shape", shape, flush=True)
+            print("case spce_nacl_32k k-space shape", shape, flush=True)
             f, g, h, e = eng.allforce()
             assert rms_rel(f, fx["F_0"]) <= FORCE_RMS_TOL, shape
             assert rms_rel(g, fx["G_0"].astype(np.float64)) <= FORCE_RMS_TOL, shape
             assert rms_rel(h, fx["H_0"].astype(np.float64)) <= FORCE_RMS_TOL, shape
             assert (e["npairs_short"], e["npairs_long"]) == tuple(int(v) for v in fx["NPAIRS_0"])
             check_energies(e, ref)
-            print("cfg2 shape %s: F rms %.2e, total energy rel %.2e" % (
+            print("spce_nacl_32k shape %s: F rms %.2e, total energy rel %.2e" % (
                 shape, rms_rel(f, fx["F_0"]),
                 abs(sum(e[k] for k in ENERGY_KEYS) - sum(ref[k] for k in ENERGY_KEYS)) /
                 abs(sum(ref[k] for k in ENERGY_KEYS))), flush=True)
@@ -84,7 +84,7 @@ def test_cfg2_reference_lists_kset_and_forces():
         eng.close()
 
 
-def test_cfg2_verlet_list_at_displaced_positions():
+def test_spce_nacl_32k_verlet_list_at_displaced_positions():
     """The benchmark's list (Verlet, 1 A skin) built at the generator's
     positions and evaluated after every atom moved up to 0.25 A per axis:
     the pairs within R_cut are those of a reference list built at the new
@@ -103,7 +103,7 @@ def test_cfg2_verlet_list_at_displaced_positions():
         eng.set_positions(s.pos)
         eng.chcnb()
         eng.set_positions(pos2)
-        print("case cfg2 verlet displaced", flush=True)
+        print("case spce_nacl_32k verlet displaced", flush=True)
         f, g, h, e = eng.allforce()
         ref = dict(zip(abi.MdxEnergies.SCALARS, fx["ENERGY_1"]))
         assert e["npairs_short"] + e["npairs_long"] == int(fx["NPAIRS_1"].sum())
